@@ -1,0 +1,282 @@
+"""glimpsw_amd: MI355X-native meshlet raster -> vis-buffer -> resolve path.
+
+Python binding of the C ABI in include/glimp.h (libglimp.so, HIP, gfx950).
+There is no CPU fallback: every call goes to the HIP library, and loading
+fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libglimp.so")
+
+GLIMP_DRAW_NO_CLIPPING = 0x1
+GLIMP_DRAW_NO_GUARDBAND = 0x2
+GLIMP_DRAW_DIRECT = 0x4
+GLIMP_DRAW_COUNT_FRAGMENTS = 0x8
+GLIMP_PIPELINE_VISBUFFER = 0
+MAX_RENDER_SIZE = 2896
+
+u8p = C.POINTER(C.c_uint8)
+u16p = C.POINTER(C.c_uint16)
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+f32p = C.POINTER(C.c_float)
+
+
+class GlimpError(RuntimeError):
+    pass
+
+
+class MeshletsDesc(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("bounds", f32p), ("num_vertices", u8p), ("num_triangles", u8p),
+                ("material_id", u32p), ("tangent_handedness", u64p), ("positions", f32p),
+                ("texcoords", u32p), ("normal_tangents", u32p), ("indices", u8p)]
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("mip_levels", C.c_uint32),
+                ("num_layers", C.c_uint32), ("row_shift", C.c_uint32), ("layer_stride", C.c_uint32),
+                ("mip_offsets", C.c_uint32 * 16), ("texels", u32p)]
+
+
+class MaterialDesc(C.Structure):
+    _fields_ = [("texture", C.c_int32), ("double_sided", C.c_uint8), ("alpha_cutoff", C.c_uint8),
+                ("pad", C.c_uint8 * 2)]
+
+
+class DrawDesc(C.Structure):
+    _fields_ = [("meshlet_offset", C.c_uint32), ("meshlet_count", C.c_uint32),
+                ("object_to_clip", C.c_float * 16), ("frustum_planes", f32p), ("flags", C.c_uint32),
+                ("pipeline", C.c_uint32)]
+
+
+class CountersStruct(C.Structure):
+    _fields_ = [("triangles_processed", C.c_uint64), ("triangles_rasterized", C.c_uint64),
+                ("triangles_clipped", C.c_uint64), ("bins_rasterized", C.c_uint64),
+                ("fragments_covered", C.c_uint64), ("fragments_passed", C.c_uint64),
+                ("setup_records", C.c_uint64), ("bin_entries", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+# Every symbol include/glimp.h declares, with its ctypes signature.
+_SIGNATURES = {
+    "glimp_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    "glimp_destroy": (None, [C.c_void_p]),
+    "glimp_last_error": (C.c_char_p, []),
+    "glimp_scene_create": (C.c_int, [C.c_void_p, C.POINTER(MeshletsDesc), C.POINTER(C.c_void_p)]),
+    "glimp_scene_destroy": (None, [C.c_void_p]),
+    "glimp_texture_create": (C.c_int, [C.c_void_p, C.POINTER(TextureDesc), C.POINTER(C.c_void_p)]),
+    "glimp_texture_destroy": (None, [C.c_void_p]),
+    "glimp_scene_set_materials": (C.c_int, [C.c_void_p, C.POINTER(MaterialDesc), C.c_uint32,
+                                            C.POINTER(C.c_void_p), C.c_uint32]),
+    "glimp_framebuffer_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "glimp_framebuffer_destroy": (None, [C.c_void_p]),
+    "glimp_framebuffer_clear": (C.c_int, [C.c_void_p, C.c_uint32, C.c_float, C.c_void_p]),
+    "glimp_framebuffer_read": (C.c_int, [C.c_void_p, u32p, f32p, C.c_int, C.c_void_p]),
+    "glimp_framebuffer_keys": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]),
+    "glimp_draw_meshlets": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(DrawDesc), C.c_void_p]),
+    "glimp_cull_meshlets": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, f32p, u16p,
+                                      C.POINTER(C.c_uint32), C.c_void_p]),
+    "glimp_get_counters": (C.c_int, [C.c_void_p, C.POINTER(CountersStruct)]),
+    "glimp_reset_counters": (C.c_int, [C.c_void_p]),
+}
+
+_lib = None
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def load(build_if_missing: bool = False):
+    """Loads libglimp.so. Raises if it is missing (no silent fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        if build_if_missing:
+            from . import build as _b
+            _b.build()
+        else:
+            raise GlimpError(f"libglimp.so not built ({LIB_PATH}); run python -m glimpsw_amd.build")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = load().glimp_last_error().decode(errors="replace")
+        raise GlimpError(f"{what} failed ({rc}): {msg}")
+
+
+def _ptr(a: np.ndarray, t):
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data_as(t)
+
+
+def _stream_ptr(stream) -> C.c_void_p:
+    if stream is None:
+        return C.c_void_p(0)
+    if isinstance(stream, int):
+        return C.c_void_p(stream)
+    return C.c_void_p(int(stream.cuda_stream))
+
+
+class Context:
+    """A glimp context on one HIP device (include/glimp.h glimp_create)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        _check(self.lib.glimp_create(C.byref(h), device), "glimp_create")
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            self.lib.glimp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def counters(self) -> dict:
+        c = CountersStruct()
+        _check(self.lib.glimp_get_counters(self.handle, C.byref(c)), "glimp_get_counters")
+        return c.as_dict()
+
+    def reset_counters(self):
+        _check(self.lib.glimp_reset_counters(self.handle), "glimp_reset_counters")
+
+
+class Texture:
+    def __init__(self, ctx: Context, tex):
+        self.ctx = ctx
+        d = TextureDesc()
+        d.width, d.height, d.mip_levels, d.num_layers = tex.width, tex.height, tex.mip_levels, tex.num_layers
+        d.row_shift, d.layer_stride = tex.row_shift, tex.layer_stride
+        for i in range(16):
+            d.mip_offsets[i] = int(tex.mip_offsets[i])
+        data = np.ascontiguousarray(tex.data, dtype=np.uint32)
+        d.texels = _ptr(data, u32p)
+        h = C.c_void_p()
+        _check(ctx.lib.glimp_texture_create(ctx.handle, C.byref(d), C.byref(h)), "glimp_texture_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.glimp_texture_destroy(self.handle)
+            self.handle = None
+
+
+class Scene:
+    """Device copy of a glimpsw_amd.scene.Scene (meshlets, materials, textures)."""
+
+    def __init__(self, ctx: Context, scene):
+        self.ctx = ctx
+        self.host = scene
+        s = scene
+        d = MeshletsDesc(s.count, _ptr(s.bounds, f32p), _ptr(s.num_vertices, u8p), _ptr(s.num_triangles, u8p),
+                         _ptr(s.material_id, u32p), _ptr(s.tangent_handedness, u64p), _ptr(s.positions, f32p),
+                         _ptr(s.texcoords, u32p), _ptr(s.normal_tangents, u32p), _ptr(s.indices, u8p))
+        h = C.c_void_p()
+        _check(ctx.lib.glimp_scene_create(ctx.handle, C.byref(d), C.byref(h)), "glimp_scene_create")
+        self.handle = h
+        self.textures = [Texture(ctx, t) for t in s.textures]
+        mats = (MaterialDesc * max(1, len(s.materials)))()
+        for i, (tex, ds, ac) in enumerate(s.materials):
+            mats[i] = MaterialDesc(tex, int(ds), ac)
+        texh = (C.c_void_p * max(1, len(self.textures)))(*[t.handle.value for t in self.textures])
+        _check(ctx.lib.glimp_scene_set_materials(h, mats, len(s.materials), texh, len(self.textures)),
+               "glimp_scene_set_materials")
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.glimp_scene_destroy(self.handle)
+            self.handle = None
+        for t in self.textures:
+            t.close()
+        self.textures = []
+
+    def cull(self, planes: np.ndarray, first: int, count: int, stream=None):
+        bitmap = np.zeros(max(1, (count + 15) // 16), dtype=np.uint16)
+        vis = C.c_uint32()
+        planes = np.ascontiguousarray(planes, dtype=np.float32)
+        _check(self.ctx.lib.glimp_cull_meshlets(self.ctx.handle, self.handle, first, count, _ptr(planes, f32p),
+                                                _ptr(bitmap, u16p), C.byref(vis), _stream_ptr(stream)),
+               "glimp_cull_meshlets")
+        return bitmap[: (count + 15) // 16], int(vis.value)
+
+
+class Framebuffer:
+    """Device vis-buffer of 64-bit depth-test keys (include/glimp.h)."""
+
+    def __init__(self, ctx: Context, width: int, height: int):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(ctx.lib.glimp_framebuffer_create(ctx.handle, width, height, C.byref(h)), "glimp_framebuffer_create")
+        self.handle = h
+        self.width, self.height = width, height
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.glimp_framebuffer_destroy(self.handle)
+            self.handle = None
+
+    def clear(self, color: int = 0, depth: float = 0.0, stream=None):
+        _check(self.ctx.lib.glimp_framebuffer_clear(self.handle, color, depth, _stream_ptr(stream)),
+               "glimp_framebuffer_clear")
+
+    def draw(self, scene: Scene, object_to_clip, count: int, offset: int = 0, planes=None, flags: int = 0,
+             stream=None):
+        d = DrawDesc()
+        d.meshlet_offset, d.meshlet_count = offset, count
+        for i in range(16):
+            d.object_to_clip[i] = float(object_to_clip[i])
+        keep = None
+        if planes is not None:
+            keep = np.ascontiguousarray(planes, dtype=np.float32)
+            d.frustum_planes = _ptr(keep, f32p)
+        d.flags = flags
+        d.pipeline = GLIMP_PIPELINE_VISBUFFER
+        _check(self.ctx.lib.glimp_draw_meshlets(self.ctx.handle, self.handle, scene.handle, C.byref(d),
+                                                _stream_ptr(stream)), "glimp_draw_meshlets")
+
+    def read(self, linear: bool = True, stream=None):
+        ids = np.zeros((self.height, self.width), dtype=np.uint32)
+        depth = np.zeros((self.height, self.width), dtype=np.float32)
+        _check(self.ctx.lib.glimp_framebuffer_read(self.handle, _ptr(ids, u32p), _ptr(depth, f32p), int(linear),
+                                                   _stream_ptr(stream)), "glimp_framebuffer_read")
+        return ids, depth
+
+    def keys_ptr(self):
+        p = C.c_void_p()
+        n = C.c_size_t()
+        _check(self.ctx.lib.glimp_framebuffer_keys(self.handle, C.byref(p), C.byref(n)), "glimp_framebuffer_keys")
+        return p.value, n.value
+
+
+def render_frame(ctx: Context, dscene: Scene, view, fb: Framebuffer | None = None, cull: bool = True,
+                 flags: int = 0, stream=None) -> Framebuffer:
+    """Clear, cull and draw one view: the reference frame sequence
+    (Main.cpp:213-240) on the device."""
+    fb = fb or Framebuffer(ctx, view.width, view.height)
+    fb.clear(0, 0.0, stream)
+    planes = view.frustum_planes() if cull else None
+    fb.draw(dscene, view.object_to_clip, view.meshlet_count, view.meshlet_offset, planes, flags, stream)
+    return fb

@@ -1,0 +1,556 @@
+// Meshlet raster kernels for gfx950 (wave64).
+//
+// Pipeline of one draw (Rasterizer::DrawMeshlets, Rasterizer.h:213):
+//   k_mesh_setup  one wave per meshlet: frustum cull bit, vertex transform
+//                 into LDS, outcodes, triangle setup; survivors become
+//                 32-byte TriRecords, guard-band cases go to a clip queue.
+//   k_clip        one lane per queued triangle: Sutherland-Hodgman against
+//                 the set outcode planes, fan triangulation, setup.
+//   k_raster_direct  load-balanced over 4x4 tiles of the records' render
+//                 boxes; covered fragments update the 64-bit key buffer
+//                 with atomicMax.
+//
+// Depth test. The reference keeps the nearest fragment (reverse-Z, strictly
+// greater depth wins, Shading.cpp:313) and, with one thread, resolves ties in
+// draw order. Each fragment here carries key = depth_bits << 32 | ~rank,
+// rank being its position in that draw order; all depths that can pass are
+// positive, so the largest key is the reference's surviving fragment and the
+// result does not depend on the order the GPU produces fragments in.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+
+namespace glimp {
+
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t mbcnt(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Render bounding box of a snapped triangle (GetBoundingBox and
+// GetRenderBoundingBox, Rasterizer.cpp:331-351), including the carry of the
+// 32-bit add across the packed halves. Returns min/max as packed x | y << 16.
+__device__ __forceinline__ void render_bbox(uint32_t p0, uint32_t p1, uint32_t p2, int hw, int hh,
+                                            uint32_t& bmin, uint32_t& bmax) {
+    int32_t mnx = min(min(lo16(p0), lo16(p1)), lo16(p2));
+    int32_t mny = min(min(hi16(p0), hi16(p1)), hi16(p2));
+    int32_t mxx = max(max(lo16(p0), lo16(p1)), lo16(p2));
+    int32_t mxy = max(max(hi16(p0), hi16(p1)), hi16(p2));
+    uint32_t mn = pk16(mnx, mny) + 0x00070007u;
+    uint32_t mx = pk16(mxx, mxy) + 0x00070007u;
+    int32_t ax = lo16(mn) >> 4, ay = hi16(mn) >> 4;
+    int32_t bx = lo16(mx) >> 4, by = hi16(mx) >> 4;
+    int W = hw * 2, H = hh * 2;
+    ax = min(max((int32_t)(int16_t)(ax + hw), 0), W);
+    ay = min(max((int32_t)(int16_t)(ay + hh), 0), H);
+    bx = min(max((int32_t)(int16_t)(bx + hw), 0), W);
+    by = min(max((int32_t)(int16_t)(by + hh), 0), H);
+    bmin = pk16(ax, ay) & ~0x00030003u;
+    bmax = (pk16(bx, by) + 0x00030003u) & ~0x00030003u;
+}
+
+// TrianglePacket::Setup (Rasterizer.cpp:256-289) for one triangle plus the
+// float part of TriangleEdgeVars::Setup (Rasterizer.cpp:296-329). Fills the
+// record's positions and depth plane; returns whether the triangle is kept.
+__device__ bool setup_triangle(const ApproxTables& T, const float v[3][4], int hw, int hh, int cull_mode,
+                               TriRecord& rec) {
+    float rw[3], x[3], y[3], z[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        rw[k] = frcp(T, v[k][3]);
+        x[k] = v[k][0] * rw[k];
+        y[k] = v[k][1] * rw[k];
+        z[k] = v[k][2] * rw[k];
+    }
+    float det = __builtin_fmaf(x[2] - x[0], y[1] - y[0], (x[0] - x[1]) * (y[2] - y[0]));
+    if (cull_mode != 1) {
+        bool flip = (cull_mode == 2) ? true : (det < 0.0f);
+        det = flip ? -det : det;
+    }
+    if (!(det > 0.0f)) return false;
+    float fsx = (float)(hw * 16), fsy = (float)(hh * 16);
+    int32_t X[3], Y[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        X[k] = cvt_rne(x[k] * fsx);
+        Y[k] = cvt_rne(y[k] * fsy);
+        rec.pos[k] = ((uint32_t)X[k] & 0xFFFFu) | ((uint32_t)Y[k] << 16);
+    }
+    uint32_t bmin, bmax;
+    render_bbox(rec.pos[0], rec.pos[1], rec.pos[2], hw, hh, bmin, bmax);
+    if (lo16(bmin) >= lo16(bmax) || hi16(bmin) >= hi16(bmax)) return false;
+    // integer doubled area of the snapped triangle, made positive
+    int32_t x0 = lo16(rec.pos[0]), y0 = hi16(rec.pos[0]);
+    int32_t x1 = lo16(rec.pos[1]), y1 = hi16(rec.pos[1]);
+    int32_t x2 = lo16(rec.pos[2]), y2 = hi16(rec.pos[2]);
+    uint32_t A01 = (uint32_t)(y1 - y0), B01 = (uint32_t)(x0 - x1);
+    uint32_t A20 = (uint32_t)(y0 - y2), B20 = (uint32_t)(x2 - x0);
+    int32_t idet = (int32_t)(B20 * A01 - B01 * A20);
+    if (idet < 0) idet = (int32_t)(0u - (uint32_t)idet);
+    float rcpArea = fdiv(T, 16.0f, (float)idet);
+    rec.z0 = z[0];
+    rec.z10 = (z[1] - z[0]) * rcpArea;
+    rec.z20 = (z[2] - z[0]) * rcpArea;
+    return true;
+}
+
+// Edge equations of a record (TriangleEdgeVars::Setup, Rasterizer.cpp:291-318).
+struct Edges {
+    int32_t e0, e1, e2;            // value at pixel (0,0)
+    int32_t a12, a20, a01;         // x steps
+    int32_t b12, b20, b01;         // y steps
+};
+
+__device__ __forceinline__ int32_t compute_edge(int32_t a, int32_t x, int32_t b, int32_t y) {
+    uint32_t w = (uint32_t)a * (uint32_t)x + (uint32_t)b * (uint32_t)y;
+    w += (a > 0 || (a == 0 && b > 0)) ? 0u : 0xFFFFFFFFu;
+    return (int32_t)w >> 4;
+}
+
+__device__ __forceinline__ Edges edge_setup(uint32_t p0, uint32_t p1, uint32_t p2, int hw, int hh) {
+    int32_t x0 = lo16(p0), y0 = hi16(p0);
+    int32_t x1 = lo16(p1), y1 = hi16(p1);
+    int32_t x2 = lo16(p2), y2 = hi16(p2);
+    int32_t A01 = y1 - y0, B01 = x0 - x1;
+    int32_t A12 = y2 - y1, B12 = x1 - x2;
+    int32_t A20 = y0 - y2, B20 = x2 - x0;
+    int32_t det = (int32_t)((uint32_t)B20 * (uint32_t)A01 - (uint32_t)B01 * (uint32_t)A20);
+    if (det < 0) {
+        A01 = -A01; B01 = -B01; A12 = -A12; B12 = -B12; A20 = -A20; B20 = -B20;
+    }
+    int32_t sx = -(hw << 4) + 8, sy = -(hh << 4) + 8;
+    Edges e;
+    e.e0 = compute_edge(A12, sx - x1, B12, sy - y1);
+    e.e1 = compute_edge(A20, sx - x2, B20, sy - y2);
+    e.e2 = compute_edge(A01, sx - x0, B01, sy - y0);
+    e.a12 = A12; e.a20 = A20; e.a01 = A01;
+    e.b12 = B12; e.b20 = B20; e.b01 = B01;
+    return e;
+}
+
+__device__ __forceinline__ uint32_t pixel_offset(uint32_t W, uint32_t x, uint32_t y) {
+    return ((x & ~3u) << 2) + (y & ~3u) * W + (x & 3u) + ((y & 3u) << 2);
+}
+
+// ---------------------------------------------------------------- cull
+
+// Frustum test of bound spheres (Shading.cpp:797-809): one lane per meshlet,
+// the bitmap word for 16 meshlets assembled from a wave ballot.
+__global__ void k_cull(DevMeshlets m, uint32_t first, uint32_t count, Planes5 pl, uint16_t* bitmap,
+                       uint32_t* visible) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool vis = false;
+    if (i < count) {
+        float4 b = m.bounds[first + i];
+        vis = true;
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            const float* P = pl.p + 4 * k;
+            float d = __builtin_fmaf(b.x, P[0], __builtin_fmaf(b.y, P[1], b.z * P[2])) + P[3];
+            vis = vis && (d > -b.w);
+        }
+    }
+    unsigned long long mask = __ballot(vis);
+    uint32_t lane = lane_id();
+    uint32_t word = i / 16;
+    if ((lane & 15) == 0 && i < count) bitmap[word] = (uint16_t)(mask >> (lane & 48));
+    if (visible && lane == 0) {
+        uint32_t c = __popcll(mask);
+        if (c) atomicAdd(visible, c);
+    }
+}
+
+// ---------------------------------------------------------------- mesh + setup
+
+__global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
+    __shared__ float lds_pos[4][4][64];
+    const uint32_t wid = threadIdx.x / kWave;
+    const uint32_t lane = lane_id();
+    const uint32_t w = blockIdx.x * 4 + wid;   // meshlet index within the draw
+    const bool have = w < p.meshlet_count;
+    const uint32_t mi = p.meshlet_offset + w;
+    const int hw = p.width / 2, hh = p.height / 2;
+
+    bool visible = have;
+    if (have && p.use_cull) visible = ((p.cull_bitmap[w / 16] >> (w % 16)) & 1) != 0;
+    uint32_t nt = visible ? p.m.num_triangles[mi] : 0;
+    int cull_mode = 1, fs_id = 0;
+    if (visible) {
+        uint32_t mat = p.m.material_id[mi];
+        if (mat != 0xFFFFFFFFu && mat < p.num_materials) {
+            cull_mode = p.materials[mat].cull_mode;
+            fs_id = p.materials[mat].fs_id;
+        }
+        // ShadeMeshlet (Shading.cpp:295-298): lane = vertex
+        const float* P = p.m.positions + (size_t)mi * 192;
+        float px = P[lane], py = P[64 + lane], pz = P[128 + lane];
+#pragma unroll
+        for (int c = 0; c < 4; c++) lds_pos[wid][c][lane] = xform(p.clip, c, px, py, pz);
+    }
+    __syncthreads();
+    if (visible && lane == 0) {
+        atomicAdd(&p.counters->triangles_processed, (unsigned long long)nt);
+    }
+    const uint32_t seq = p.seq_base + w;
+    const uint8_t* I = p.m.indices + (size_t)mi * 384;
+
+    for (uint32_t round = 0; round < 2; round++) {
+        const uint32_t t = round * 64 + lane;
+        bool valid = t < nt;
+        bool accept = false, nontrivial = false;
+        uint32_t partial = 0;
+        float v[3][4];
+        if (valid) {
+            uint8_t combined = 255;
+            bool trivial = true;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                uint32_t ix = I[k * 128 + t];
+                uint32_t vi = ix < 32 ? ix : 32 + (ix & 31);   // gather_preload64 (SIMD.h:219-227)
+#pragma unroll
+                for (int c = 0; c < 4; c++) v[k][c] = lds_pos[wid][c][vi];
+                float x = v[k][0], y = v[k][1], z = v[k][2], ww = v[k][3];
+                uint32_t oc = 0;
+                oc |= (x < -ww) ? 1u : 0u;
+                oc |= (x > ww) ? 2u : 0u;
+                oc |= (y < -ww) ? 4u : 0u;
+                oc |= (y > ww) ? 8u : 0u;
+                oc |= (z < -ww) ? 16u : 0u;
+                oc |= (z > ww) ? 32u : 0u;
+                partial |= oc;
+                combined &= (uint8_t)oc;
+                trivial = trivial && (fabsf(x) < ww * p.guard_x) && (fabsf(y) < ww * p.guard_y);
+            }
+            trivial = trivial && ((partial & 0x30u) == 0);
+            accept = (combined == 0) && trivial;
+            nontrivial = (combined == 0) && !trivial && p.clipping;
+        }
+        const uint32_t packet = t >> 4, plane = t & 15;
+        TriRecord rec;
+        bool keep = false;
+        if (accept) {
+            keep = setup_triangle(p.tables, v, hw, hh, cull_mode, rec);
+            rec.rank = make_rank(seq, packet, 0, plane, 0);
+            rec.flags = (uint32_t)fs_id;
+        }
+        unsigned long long km = __ballot(keep);
+        if (km) {
+            uint32_t n = __popcll(km);
+            uint32_t base = 0;
+            if (lane == 0) {
+                base = atomicAdd(&p.scratch->num_records, n);
+                atomicAdd(&p.counters->triangles_rasterized, (unsigned long long)n);
+            }
+            base = __shfl(base, 0);
+            if (keep) {
+                uint32_t slot = base + mbcnt(km);
+                if (slot < p.record_capacity) p.records[slot] = rec;
+                else p.scratch->overflow = 1;
+            }
+        }
+        unsigned long long cm = __ballot(nontrivial);
+        if (cm) {
+            uint32_t n = __popcll(cm);
+            uint32_t base = 0;
+            if (lane == 0) {
+                base = atomicAdd(&p.scratch->num_clip, n);
+                atomicAdd(&p.counters->triangles_clipped, (unsigned long long)n);
+            }
+            base = __shfl(base, 0);
+            if (nontrivial) {
+                uint32_t slot = base + mbcnt(cm);
+                if (slot < p.clip_capacity) {
+                    ClipEntry ce;
+                    ce.rank_base = make_rank(seq, packet, 1, plane, 0);
+                    ce.info = partial | ((uint32_t)cull_mode << 8) | ((uint32_t)fs_id << 10);
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+#pragma unroll
+                        for (int c = 0; c < 4; c++) ce.v[k][c] = v[k][c];
+                    ce.pad[0] = ce.pad[1] = 0;
+                    p.clip_queue[slot] = ce;
+                } else {
+                    p.scratch->overflow = 1;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- clipper
+
+// Clipper::ClipTriangles for one triangle (Rasterizer.cpp:398-478):
+// Sutherland-Hodgman over the planes in the triangle's combined outcode,
+// in plane order L, R, T, B, N, F, distances `+-c + w*scale` with the
+// guard-band scale for x/y; then a fan over the result. Each fan triangle
+// goes through setup with culling (FlushPacket, Rasterizer.cpp:479-491).
+__global__ void __launch_bounds__(64) k_clip(DrawParams p) {
+    const uint32_t n = min(p.scratch->num_clip, p.clip_capacity);
+    const int hw = p.width / 2, hh = p.height / 2;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const ClipEntry ce = p.clip_queue[q];
+        // vertex store: the 3 inputs + up to 16 generated vertices, the
+        // room the reference's Clipper::Vertices leaves per triangle
+        float V[4][19];
+        uint8_t idx[32], out[32];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) V[c][k] = ce.v[k][c];
+            idx[k] = (uint8_t)k;
+        }
+        uint32_t vertCount = 3, next = 3;
+        const uint32_t codes = ce.info & 0x3Fu;
+        for (uint32_t plane = 0; plane < 6; plane++) {
+            if (!((codes >> plane) & 1u)) continue;
+            float scale = plane < 2 ? p.guard_x : (plane < 4 ? p.guard_y : 1.0f);
+            uint32_t comp = plane >> 1;
+            bool neg = plane & 1u;
+            uint32_t outCount = 0;
+            for (uint32_t vi = 0; vi < vertCount; vi++) {
+                uint32_t ia = idx[vi], ib = idx[(vi + 1) % vertCount];
+                float ca = V[comp][ia], cb = V[comp][ib];
+                if (neg) { ca = -ca; cb = -cb; }
+                float da = __builtin_fmaf(V[3][ia], scale, ca);
+                float db = __builtin_fmaf(V[3][ib], scale, cb);
+                if (da >= 0.0f) out[outCount++] = (uint8_t)ia;
+                if ((da >= 0.0f) != (db >= 0.0f) && next < 19) {
+                    float t = da / (da - db);
+#pragma unroll
+                    for (int c = 0; c < 4; c++)
+                        V[c][next] = __builtin_fmaf(V[c][ib], t, __builtin_fmaf(-t, V[c][ia], V[c][ia]));
+                    out[outCount++] = (uint8_t)next++;
+                }
+            }
+            if (vertCount < 3) break;
+            vertCount = outCount;
+            for (uint32_t k = 0; k < outCount; k++) idx[k] = out[k];
+        }
+        if (vertCount < 3) continue;
+        const int cull_mode = (ce.info >> 8) & 3;
+        uint32_t kept = 0;
+        for (uint32_t fan = 0; fan + 2 < vertCount && fan < 8; fan++) {
+            uint32_t ii[3] = { idx[0], idx[fan + 1], idx[fan + 2] };
+            float v[3][4];
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) v[k][c] = V[c][ii[k]];
+            TriRecord rec;
+            if (!setup_triangle(p.tables, v, hw, hh, cull_mode, rec)) continue;
+            rec.rank = ce.rank_base | fan;
+            rec.flags = (ce.info >> 10) & 3u;
+            uint32_t slot = atomicAdd(&p.scratch->num_records, 1u);
+            if (slot < p.record_capacity) p.records[slot] = rec;
+            else p.scratch->overflow = 1;
+            kept++;
+        }
+        if (kept) atomicAdd(&p.counters->triangles_rasterized, (unsigned long long)kept);
+    }
+}
+
+// ---------------------------------------------------------------- direct raster
+
+// Each wave takes 64 records, lays their render boxes out as a list of 4x4
+// tiles (exclusive scan of tile counts) and walks that list four tiles at a
+// time: lane = pixel, 16 lanes = one tile, so a tile's 16 keys (128 bytes)
+// are one contiguous access. Covered pixels evaluate the depth plane exactly
+// as DrawTriangle does (Rasterizer.h:293-296) and update the key with
+// atomicMax when the key can still win.
+__global__ void __launch_bounds__(256) k_raster_direct(DrawParams p) {
+    struct TriLds {
+        int32_t e0, e1, e2, a12, a20, a01, b12, b20, b01;
+        float z0, z10, z20;
+        uint32_t keylo, org, tw, start;
+    };
+    __shared__ TriLds tl[4][64];
+    const uint32_t wid = threadIdx.x / kWave;
+    const uint32_t lane = lane_id();
+    const uint32_t n = min(p.scratch->num_records, p.record_capacity);
+    const int hw = p.width / 2, hh = p.height / 2;
+    const uint32_t W = (uint32_t)p.width;
+    const unsigned long long clearKey = (unsigned long long)fbits(p.clear_depth) << 32;
+    const bool countFrags = (p.flags & 8u) != 0;
+    const uint32_t waves = gridDim.x * 4;
+
+    for (uint32_t chunk = blockIdx.x * 4 + wid; chunk * 64 < n; chunk += waves) {
+        const uint32_t r = chunk * 64 + lane;
+        uint32_t ntiles = 0;
+        TriLds t;
+        if (r < n) {
+            const TriRecord rec = p.records[r];
+            uint32_t bmin, bmax;
+            render_bbox(rec.pos[0], rec.pos[1], rec.pos[2], hw, hh, bmin, bmax);
+            Edges e = edge_setup(rec.pos[0], rec.pos[1], rec.pos[2], hw, hh);
+            t.e0 = e.e0; t.e1 = e.e1; t.e2 = e.e2;
+            t.a12 = e.a12; t.a20 = e.a20; t.a01 = e.a01;
+            t.b12 = e.b12; t.b20 = e.b20; t.b01 = e.b01;
+            t.z0 = rec.z0; t.z10 = rec.z10; t.z20 = rec.z20;
+            t.keylo = ~rec.rank;
+            t.org = bmin;
+            uint32_t tw = (uint32_t)(lo16(bmax) - lo16(bmin)) >> 2;
+            uint32_t th = (uint32_t)(hi16(bmax) - hi16(bmin)) >> 2;
+            t.tw = tw;
+            ntiles = tw * th;
+        }
+        // exclusive scan of ntiles across the wave
+        uint32_t incl = ntiles;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t o = __shfl_up(incl, d);
+            if ((int)lane >= d) incl += o;
+        }
+        const uint32_t total = __shfl(incl, 63);
+        t.start = incl - ntiles;
+        if (r < n) tl[wid][lane] = t;
+        else tl[wid][lane].start = 0xFFFFFFFFu;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+        uint32_t covered = 0, passed = 0;
+        const uint32_t sub = lane >> 4, px = lane & 3, py = (lane >> 2) & 3;
+        for (uint32_t base = 0; base < total; base += 4) {
+            const uint32_t j = base + sub;
+            if (j < total) {
+                // last triangle whose start <= j (starts are non-decreasing)
+                uint32_t k = 0;
+#pragma unroll
+                for (uint32_t step = 32; step > 0; step >>= 1) {
+                    if (tl[wid][k + step].start <= j) k += step;
+                }
+                const TriLds& T = tl[wid][k];
+                const uint32_t pt = j - T.start;
+                const uint32_t ty = pt / T.tw;
+                const uint32_t tx = pt - ty * T.tw;
+                const uint32_t x = (uint32_t)lo16(T.org) + tx * 4 + px;
+                const uint32_t y = (uint32_t)hi16(T.org) + ty * 4 + py;
+                int32_t w0 = (int32_t)((uint32_t)T.e0 + (uint32_t)T.a12 * x + (uint32_t)T.b12 * y);
+                int32_t w1 = (int32_t)((uint32_t)T.e1 + (uint32_t)T.a20 * x + (uint32_t)T.b20 * y);
+                int32_t w2 = (int32_t)((uint32_t)T.e2 + (uint32_t)T.a01 * x + (uint32_t)T.b01 * y);
+                if ((w0 | w1 | w2) >= 0 && x < W && y < (uint32_t)p.height) {
+                    covered++;
+                    float depth = __builtin_fmaf((float)w1, T.z10, __builtin_fmaf((float)w2, T.z20, T.z0));
+                    if (depth > p.clear_depth) {
+                        unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
+                        unsigned long long* dst = p.keys + pixel_offset(W, x, y);
+                        if (key > __atomic_load_n(dst, __ATOMIC_RELAXED)) {
+                            atomicMax(dst, key);
+                        }
+                        passed++;
+                    }
+                }
+            }
+        }
+        if (countFrags) {
+            for (int d = 32; d > 0; d >>= 1) {
+                covered += __shfl_xor(covered, d);
+                passed += __shfl_xor(passed, d);
+            }
+            if (lane == 0) {
+                atomicAdd(&p.counters->fragments_covered, (unsigned long long)covered);
+                atomicAdd(&p.counters->fragments_passed, (unsigned long long)passed);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    (void)clearKey;
+}
+
+__global__ void k_begin_draw(DrawScratch* s) {
+    if (threadIdx.x == 0) {
+        s->num_records = 0;
+        s->num_clip = 0;
+        s->num_visible = 0;
+        s->overflow = 0;
+    }
+}
+
+__global__ void k_clear(unsigned long long* keys, uint32_t n, unsigned long long key) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) keys[i] = key;
+}
+
+// Vis-buffer keys -> reference layers: surface id (MeshletOffset +
+// MeshletId) * 128 + PrimId (Shading.cpp:328) and depth.
+__global__ void k_decode(const unsigned long long* keys, int width, int height, unsigned long long clear_key,
+                         uint32_t clear_color, const DrawEntry* draws, uint32_t num_draws, uint32_t* ids,
+                         float* depth, int linear) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = (uint32_t)width * (uint32_t)height;
+    if (i >= n) return;
+    unsigned long long k = keys[i];
+    uint32_t id = clear_color;
+    uint32_t db = (uint32_t)(clear_key >> 32);
+    if (k != clear_key) {
+        uint32_t rank = ~(uint32_t)k;
+        db = (uint32_t)(k >> 32);
+        uint32_t seq = rank >> 11;
+        uint32_t prim = ((rank >> 8) & 7u) * 16u + ((rank >> 3) & 15u);
+        uint32_t lo = 0, hi = num_draws;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) / 2;
+            if (draws[mid].seq_base <= seq) lo = mid; else hi = mid;
+        }
+        uint32_t meshlet = draws[lo].meshlet_offset + (seq - draws[lo].seq_base);
+        id = meshlet * 128u + prim;
+    }
+    uint32_t o = i;
+    if (linear) {
+        // i is a tiled offset; recover (x, y)
+        uint32_t W = (uint32_t)width;
+        uint32_t tileRow = i / (W * 4u);
+        uint32_t rem = i - tileRow * W * 4u;
+        uint32_t tx = rem >> 4, q = rem & 15u;
+        uint32_t x = tx * 4 + (q & 3u), y = tileRow * 4 + (q >> 2);
+        o = y * W + x;
+    }
+    if (ids) ids[o] = id;
+    if (depth) depth[o] = bitsf(db);
+}
+
+}  // namespace
+
+hipError_t launch_clear(unsigned long long* keys, uint32_t n, unsigned long long key, hipStream_t s) {
+    uint32_t blocks = min((n + 255) / 256, 8192u);
+    hipLaunchKernelGGL(k_clear, dim3(blocks), dim3(256), 0, s, keys, n, key);
+    return hipGetLastError();
+}
+
+hipError_t launch_decode(const unsigned long long* keys, int width, int height, unsigned long long clear_key,
+                         uint32_t clear_color, const DrawEntry* draws, uint32_t num_draws, uint32_t* ids,
+                         float* depth, int linear, hipStream_t s) {
+    uint32_t n = (uint32_t)width * (uint32_t)height;
+    hipLaunchKernelGGL(k_decode, dim3((n + 255) / 256), dim3(256), 0, s, keys, width, height, clear_key, clear_color,
+                       draws, num_draws, ids, depth, linear);
+    return hipGetLastError();
+}
+
+hipError_t launch_cull(const DevMeshlets& m, uint32_t first, uint32_t count, const float* planes, uint16_t* bitmap,
+                       uint32_t* visible, hipStream_t s) {
+    Planes5 pl;
+    for (int i = 0; i < 20; i++) pl.p[i] = planes[i];
+    hipLaunchKernelGGL(k_cull, dim3((count + 255) / 256), dim3(256), 0, s, m, first, count, pl, bitmap, visible);
+    return hipGetLastError();
+}
+
+hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s) {
+    hipLaunchKernelGGL(k_begin_draw, dim3(1), dim3(64), 0, s, p.scratch);
+    if (p.use_cull) {
+        Planes5 pl;
+        for (int i = 0; i < 20; i++) pl.p[i] = p.planes[i];
+        hipLaunchKernelGGL(k_cull, dim3((p.meshlet_count + 255) / 256), dim3(256), 0, s, p.m, p.meshlet_offset,
+                           p.meshlet_count, pl, p.cull_bitmap, &p.scratch->num_visible);
+    }
+    hipLaunchKernelGGL(k_mesh_setup, dim3((p.meshlet_count + 3) / 4), dim3(256), 0, s, p);
+    if (p.clipping) hipLaunchKernelGGL(k_clip, dim3(256), dim3(64), 0, s, p);
+    hipLaunchKernelGGL(k_raster_direct, dim3(2048), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+}  // namespace glimp

@@ -1,0 +1,140 @@
+// Shared device-side structures and kernel launchers of the glimp library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "device_math.h"
+
+namespace glimp {
+
+// Device copy of the meshlet structure-of-arrays (Scene.h:15-30).
+struct DevMeshlets {
+    uint32_t count;
+    const float4* bounds;
+    const uint8_t* num_vertices;
+    const uint8_t* num_triangles;
+    const uint32_t* material_id;
+    const uint64_t* tangent_handedness;
+    const float* positions;        // [M][3][64]
+    const uint32_t* texcoords;     // [M][64]
+    const uint32_t* normal_tangents;
+    const uint8_t* indices;        // [M][3][128]
+};
+
+// Per material: face culling and fragment entry point as ShadeMeshlet
+// derives them (Shading.cpp:302-306), plus the texture slot.
+struct DevMaterial {
+    int32_t texture;
+    uint8_t cull_mode;             // 0 None, 1 FrontCCW
+    uint8_t fs_id;                 // 0 = FS_EncodeSurfaceId<false>, 1 = <true>
+    uint8_t alpha_cutoff;
+    uint8_t pad;
+};
+
+struct DevTexture {                // Texture2D header (Texture.h:323-328)
+    uint32_t width, height, mip_levels, num_layers;
+    uint32_t row_shift, layer_stride;
+    float scale_u, scale_v, scale_lerp_u, scale_lerp_v;
+    int32_t mask_lerp_u, mask_lerp_v;
+    uint32_t mip_offsets[16];
+    const uint32_t* data;
+};
+
+// One triangle that survived setup, 32 bytes. Edge equations and the
+// render box are re-derived from the snapped positions where needed.
+struct TriRecord {
+    uint32_t pos[3];               // snapped 1/16 px, x | y << 16 (s16 each)
+    float z0, z10, z20;            // depth plane (Rasterizer.cpp:321-323)
+    uint32_t rank;                 // draw order, see key layout
+    uint32_t flags;                // bits 0-1 fragment entry point
+};
+static_assert(sizeof(TriRecord) == 32, "record size");
+
+// A triangle that needs guard-band clipping, with its clip-space vertices.
+struct ClipEntry {
+    uint32_t rank_base;            // rank with fan index 0
+    uint32_t info;                 // outcodes | cull_mode << 8 | fs_id << 10
+    float v[3][4];
+    uint32_t pad[2];
+};
+static_assert(sizeof(ClipEntry) == 64, "clip entry size");
+
+// Draw-order rank carried in the low 32 bits of a depth-test key:
+//   seq[31:11] packet[10:8] clipped[7] lane[6:3] fan[2:0]
+// seq = meshlets drawn into the framebuffer before this one. Within a
+// meshlet this is DrawMeshletsST's order (Rasterizer.cpp:166-249): packets
+// ascending, accepted lanes in lane order, then clipped sub-triangles by
+// source lane and fan position.
+__host__ __device__ constexpr uint32_t make_rank(uint32_t seq, uint32_t packet, uint32_t clipped,
+                                                 uint32_t lane, uint32_t fan) {
+    return (seq << 11) | (packet << 8) | (clipped << 7) | (lane << 3) | fan;
+}
+
+struct Planes5 {                   // 5 normalised frustum planes, xyz + w
+    float p[20];
+};
+
+struct DrawEntry {                 // maps a seq range back to meshlet ids
+    uint32_t seq_base;
+    uint32_t count;
+    uint32_t meshlet_offset;
+    uint32_t pad;
+};
+
+struct Counters {                  // device counters, u64 each
+    unsigned long long triangles_processed;
+    unsigned long long triangles_rasterized;
+    unsigned long long triangles_clipped;
+    unsigned long long bins_rasterized;
+    unsigned long long fragments_covered;
+    unsigned long long fragments_passed;
+    unsigned long long setup_records;
+    unsigned long long bin_entries;
+};
+
+// Per-draw scratch counters, cleared by the first kernel of each draw.
+struct DrawScratch {
+    uint32_t num_records;
+    uint32_t num_clip;
+    uint32_t num_visible;
+    uint32_t overflow;
+};
+
+struct DrawParams {
+    DevMeshlets m;
+    const DevMaterial* materials;
+    uint32_t num_materials;
+    uint32_t meshlet_offset;
+    uint32_t meshlet_count;
+    uint32_t seq_base;
+    float clip[16];
+    float planes[20];
+    int use_cull;
+    int width, height;
+    float guard_x, guard_y;
+    int clipping;
+    uint32_t flags;
+    ApproxTables tables;
+    uint16_t* cull_bitmap;         // scratch: (count + 15) / 16 words
+    TriRecord* records;
+    uint32_t record_capacity;
+    ClipEntry* clip_queue;
+    uint32_t clip_capacity;
+    DrawScratch* scratch;
+    Counters* counters;
+    unsigned long long* keys;      // vis-buffer, 4x4 tiles
+    float clear_depth;
+};
+
+// Launchers (glimp_kernels.hip).
+hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s);
+hipError_t launch_clear(unsigned long long* keys, uint32_t n, unsigned long long key, hipStream_t s);
+hipError_t launch_decode(const unsigned long long* keys, int width, int height, unsigned long long clear_key,
+                         uint32_t clear_color, const DrawEntry* draws, uint32_t num_draws, uint32_t* ids,
+                         float* depth, int linear, hipStream_t s);
+hipError_t launch_cull(const DevMeshlets& m, uint32_t first, uint32_t count, const float* planes,
+                       uint16_t* bitmap, uint32_t* visible, hipStream_t s);
+
+}  // namespace glimp

@@ -1,0 +1,137 @@
+"""RGBA8 textures in the reference's storage layout.
+
+``CreateTexture2D`` (Texture.h:599-636): power-of-two sizes, ``RowShift =
+ctz(max(width, 8))``, mip level m kept while both sides are >= 4, each level
+padded to 64 texels (256 bytes), layers ``LayerStride`` apart. Texels are
+stored TiledY8 (Texture.h:482-501): offset ``(y & 7) | (x << 3) | ((y & ~7)
+<< stride)`` with ``stride = RowShift - m``.
+
+Mips are built with the reference's box filter (Texture.h:577-596): unpack
+to float (byte * (1/255)), average the 2x2 block as ``(c00 + c10 + c01 +
+c11) * 0.25``, pack with round-to-nearest-even and saturation
+(Texture.h:55-67). Mips are inputs to the path: they are built once here and
+uploaded, never recomputed on the device.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+LERP_FRAC_BITS = 8
+
+
+@dataclasses.dataclass
+class Texture:
+    width: int
+    height: int
+    mip_levels: int
+    num_layers: int
+    row_shift: int
+    layer_stride: int
+    mip_offsets: np.ndarray   # (16,) u32
+    data: np.ndarray          # u32, layer_stride * num_layers + 64 slack texels
+
+    @property
+    def scale_u(self) -> float:
+        return float(self.width)
+
+    @property
+    def scale_v(self) -> float:
+        return float(self.height)
+
+    @property
+    def mask_lerp_u(self) -> int:
+        return (self.width << LERP_FRAC_BITS) - 1
+
+    @property
+    def mask_lerp_v(self) -> int:
+        return (self.height << LERP_FRAC_BITS) - 1
+
+    def level(self, m: int, layer: int = 0) -> np.ndarray:
+        """Row-major (h, w) view of one mip level (for tests)."""
+        w, h = self.width >> m, self.height >> m
+        ys, xs = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+        off = tiled_offset(xs, ys, self.row_shift - m)
+        base = layer * self.layer_stride + int(self.mip_offsets[m])
+        return self.data[base + off]
+
+
+def tiled_offset(x, y, stride):
+    x = np.asarray(x, dtype=np.int64)
+    y = np.asarray(y, dtype=np.int64)
+    return (y & 7) | (x << 3) | ((y & ~7) << stride)
+
+
+def layout(width: int, height: int, max_levels: int = 16):
+    assert width & (width - 1) == 0 and height & (height - 1) == 0
+    row_shift = (max(width, 8) & -max(width, 8)).bit_length() - 1
+    offsets = np.zeros(16, dtype=np.uint32)
+    stride = 0
+    mip = 0
+    while mip < max_levels:
+        if (width >> mip) < 4 or (height >> mip) < 4:
+            break
+        offsets[mip] = stride
+        stride += ((width >> mip) * (height >> mip) + 63) & ~63
+        mip += 1
+    return row_shift, offsets, stride, mip
+
+
+def unpack_f32(texels: np.ndarray) -> np.ndarray:
+    t = texels.astype(np.uint32)
+    scale = np.float32(1.0) / np.float32(255.0)
+    ch = [((t >> np.uint32(8 * k)) & np.uint32(255)).astype(np.float32) * scale for k in range(4)]
+    return np.stack(ch, axis=-1)
+
+
+def pack_f32(c: np.ndarray) -> np.ndarray:
+    v = np.rint(c.astype(np.float32) * np.float32(255.0))   # RNE
+    v = np.clip(v, 0, 255).astype(np.uint32)
+    return v[..., 0] | (v[..., 1] << 8) | (v[..., 2] << 16) | (v[..., 3] << 24)
+
+
+def next_mip(level: np.ndarray) -> np.ndarray:
+    c = unpack_f32(level)
+    c00, c10 = c[0::2, 0::2], c[0::2, 1::2]
+    c01, c11 = c[1::2, 0::2], c[1::2, 1::2]
+    avg = (((c00 + c10) + c01) + c11) * np.float32(0.25)
+    return pack_f32(avg)
+
+
+def create(levels0: list, max_levels: int = 16) -> Texture:
+    """levels0: one (h, w) u32 RGBA8 array per layer (level 0)."""
+    h, w = levels0[0].shape
+    row_shift, offsets, layer_stride, nmips = layout(w, h, max_levels)
+    data = np.zeros(layer_stride * len(levels0) + 64, dtype=np.uint32)
+    for layer, lvl in enumerate(levels0):
+        cur = lvl.astype(np.uint32)
+        for m in range(nmips):
+            hh, ww = cur.shape
+            ys, xs = np.meshgrid(np.arange(hh), np.arange(ww), indexing="ij")
+            off = tiled_offset(xs, ys, row_shift - m)
+            data[layer * layer_stride + int(offsets[m]) + off] = cur
+            if m + 1 < nmips:
+                cur = next_mip(cur)
+    return Texture(w, h, nmips, len(levels0), row_shift, layer_stride, offsets, data)
+
+
+def checker_texture(size: int = 256) -> Texture:
+    """Procedural checker: 32-texel cells, with red/green ramps inside the
+    cells so bilinear filtering has gradients to show."""
+    y, x = np.meshgrid(np.arange(size, dtype=np.uint32), np.arange(size, dtype=np.uint32), indexing="ij")
+    cell = ((x >> 5) ^ (y >> 5)) & 1
+    r = (x * 255 // (size - 1)) & 255
+    g = (y * 255 // (size - 1)) & 255
+    b = np.where(cell == 1, 230, 40).astype(np.uint32)
+    img = r | (g << 8) | (b << 16) | (np.uint32(255) << 24)
+    return create([img.astype(np.uint32)])
+
+
+def noise_texture(size: int, seed: int) -> Texture:
+    from .scene import SplitMix64
+
+    rng = SplitMix64(seed)
+    bits = rng.next_u64(size * size // 2)
+    img = bits.view(np.uint32).reshape(size, size) | np.uint32(0xFF000000)
+    return create([img])

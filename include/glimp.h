@@ -1,0 +1,176 @@
+/*
+ * glimp: C ABI of the MI355X meshlet raster -> vis-buffer -> resolve path.
+ *
+ * This is the boundary a host renderer binds to. Each entry point replaces
+ * one operation of the reference renderer (dubiousconst282/GLimpSW,
+ * src/SwRast); the reference location is cited on each declaration.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative GLIMP_E_* code on
+ *    failure; glimp_last_error() then describes the failure (thread local).
+ *  - No exceptions cross this boundary. Plain pointers and sizes only.
+ *  - Pointers documented as "host" are read or written synchronously before
+ *    the call returns. Pointers documented as "device" must be device
+ *    memory of the context's GPU.
+ *  - `stream` is a hipStream_t (NULL = the default stream). Work is
+ *    enqueued asynchronously unless a function says otherwise.
+ *  - A context is bound to one GPU; use one context per host thread.
+ */
+#ifndef GLIMP_H
+#define GLIMP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GLIMP_OK 0
+#define GLIMP_E_INVALID (-1)      /* bad argument */
+#define GLIMP_E_HIP (-2)          /* HIP runtime error */
+#define GLIMP_E_NOMEM (-3)        /* allocation failed */
+#define GLIMP_E_LIMIT (-4)        /* a size limit was exceeded */
+#define GLIMP_E_UNSUPPORTED (-5)  /* feature not built */
+
+/* Largest framebuffer side: the 32-bit edge-equation bound
+ * (Rasterizer.h:203, MaxRenderSize). */
+#define GLIMP_MAX_RENDER_SIZE 2896u
+/* Meshlet limits (Rasterizer.h:83, ShadedMeshlet::MaxVertices/MaxPrims). */
+#define GLIMP_MAX_VERTICES 64u
+#define GLIMP_MAX_PRIMS 128u
+/* Meshlets that may be drawn into one framebuffer between two clears:
+ * the depth test key carries a 21-bit draw sequence number. */
+#define GLIMP_MAX_DRAWN_MESHLETS (1u << 21)
+
+typedef struct glimp_context glimp_context;
+typedef struct glimp_scene glimp_scene;
+typedef struct glimp_texture glimp_texture;
+typedef struct glimp_framebuffer glimp_framebuffer;
+
+/* ------------------------------------------------------------ context */
+
+/* Creates a context on HIP device `device`. No reference counterpart: the
+ * reference's Rasterizer(numThreads) (Rasterizer.h:210) owns a CPU thread
+ * pool; here the context owns device scratch and the approximation tables. */
+int glimp_create(glimp_context** out, int device);
+void glimp_destroy(glimp_context* ctx);
+/* Message for the last failure on this thread ("" if none). */
+const char* glimp_last_error(void);
+
+/* ------------------------------------------------------------ resources */
+
+/* Meshlets as structure-of-arrays of the reference Meshlet record
+ * (Scene.h:15-30); all host pointers, `count` entries each. */
+typedef struct glimp_meshlets_desc {
+    uint32_t count;
+    const float* bounds;               /* [count][4] BoundCenter.xyz, BoundRadius */
+    const uint8_t* num_vertices;       /* [count] (<= 64) */
+    const uint8_t* num_triangles;      /* [count] (<= 128) */
+    const uint32_t* material_id;       /* [count], 0xFFFFFFFF = none */
+    const uint64_t* tangent_handedness;/* [count] */
+    const float* positions;            /* [count][3][64] */
+    const uint32_t* texcoords;         /* [count][64] half2 */
+    const uint32_t* normal_tangents;   /* [count][64] octahedral n.xy t.xy, unorm8 */
+    const uint8_t* indices;            /* [count][3][128] */
+} glimp_meshlets_desc;
+
+/* Uploads meshlets (the reference keeps them in Scene::Meshlets,
+ * Scene.h:117-128). */
+int glimp_scene_create(glimp_context* ctx, const glimp_meshlets_desc* desc, glimp_scene** out);
+void glimp_scene_destroy(glimp_scene* scene);
+
+/* RGBA8 texture storage exactly as CreateTexture2D lays it out
+ * (Texture.h:599-636): TiledY8 texels, mip offsets in texels, layers
+ * `layer_stride` texels apart. `texels` (host) holds
+ * layer_stride * num_layers entries. */
+typedef struct glimp_texture_desc {
+    uint32_t width, height, mip_levels, num_layers;
+    uint32_t row_shift, layer_stride;
+    uint32_t mip_offsets[16];
+    const uint32_t* texels;
+} glimp_texture_desc;
+int glimp_texture_create(glimp_context* ctx, const glimp_texture_desc* desc, glimp_texture** out);
+void glimp_texture_destroy(glimp_texture* tex);
+
+/* Material table (Scene.h:7-14). `texture` indexes the `textures` array
+ * given with it (-1 = none). */
+typedef struct glimp_material {
+    int32_t texture;
+    uint8_t double_sided;
+    uint8_t alpha_cutoff;
+    uint8_t pad[2];
+} glimp_material;
+int glimp_scene_set_materials(glimp_scene* scene, const glimp_material* materials, uint32_t count,
+                              glimp_texture* const* textures, uint32_t num_textures);
+
+/* Vis-buffer: per pixel one 64-bit depth-test key in 4x4 tiles, the device
+ * form of the reference's two-layer Framebuffer (Rasterizer.h:10-78).
+ * Width and height must be multiples of 4 and <= GLIMP_MAX_RENDER_SIZE. */
+int glimp_framebuffer_create(glimp_context* ctx, uint32_t width, uint32_t height, glimp_framebuffer** out);
+void glimp_framebuffer_destroy(glimp_framebuffer* fb);
+
+/* Framebuffer::Clear(color, depth) (Rasterizer.h:35-38). `depth` must be
+ * >= 0 (reverse-Z; fragments pass the test only when strictly nearer). */
+int glimp_framebuffer_clear(glimp_framebuffer* fb, uint32_t color, float depth, void* stream);
+
+/* Framebuffer::GetPixels for the id layer and the depth layer
+ * (ImageHelpers.cpp:109-147). Writes row-major (linear != 0) or 4x4-tiled
+ * (linear == 0) host arrays of width*height entries; either may be NULL.
+ * Synchronous. */
+int glimp_framebuffer_read(glimp_framebuffer* fb, uint32_t* ids, float* depth, int linear, void* stream);
+
+/* Device pointer to the raw 64-bit key buffer (4x4 tiles) and its size. */
+int glimp_framebuffer_keys(glimp_framebuffer* fb, void** keys, size_t* bytes);
+
+/* ------------------------------------------------------------ drawing */
+
+#define GLIMP_DRAW_NO_CLIPPING 0x1u   /* Rasterizer::EnableClipping = false (Rasterizer.h:207) */
+#define GLIMP_DRAW_NO_GUARDBAND 0x2u  /* Rasterizer::EnableGuardband = false (Rasterizer.h:208) */
+#define GLIMP_DRAW_DIRECT 0x4u        /* per-fragment global atomics instead of tile bins */
+#define GLIMP_DRAW_COUNT_FRAGMENTS 0x8u /* maintain the fragment counters (slower) */
+
+/* Pre-instantiated shader pipelines: the device counterpart of a dispatch
+ * table built by GetDispatchTable<MeshFn, FragFn...> (Rasterizer.h:342-364). */
+#define GLIMP_PIPELINE_VISBUFFER 0u   /* ShadingContext::VisBufferShader (Shading.cpp:650-654) */
+
+typedef struct glimp_draw_desc {
+    uint32_t meshlet_offset;           /* ShadingContext::MeshletOffset */
+    uint32_t meshlet_count;
+    float object_to_clip[16];          /* ShadingContext::ObjectToClipMat, column-major */
+    const float* frustum_planes;       /* host [5][4] normalised planes, or NULL: no culling */
+    uint32_t flags;                    /* GLIMP_DRAW_* */
+    uint32_t pipeline;                 /* GLIMP_PIPELINE_* */
+} glimp_draw_desc;
+
+/* Rasterizer::DrawMeshlets(fb, count, {VisBufferShader, &ctx})
+ * (Rasterizer.h:213), preceded by ShadingContext::CullMeshlets' frustum
+ * test (Shading.cpp:775-869) when planes are given. Depth-test ties resolve
+ * in the reference's single-thread draw order. Asynchronous. */
+int glimp_draw_meshlets(glimp_context* ctx, glimp_framebuffer* fb, glimp_scene* scene,
+                        const glimp_draw_desc* desc, void* stream);
+
+/* ShadingContext::CullMeshlets frustum branch (Shading.cpp:775-869): writes
+ * one bit per meshlet, 16 per word (host `bitmap`), returns the visible
+ * count in *visible. Synchronous. */
+int glimp_cull_meshlets(glimp_context* ctx, glimp_scene* scene, uint32_t first, uint32_t count,
+                        const float planes[20], uint16_t* bitmap, uint32_t* visible, void* stream);
+
+/* Perf counters (Rasterizer.h:366-379), accumulated since the last reset. */
+typedef struct glimp_counters {
+    uint64_t triangles_processed;      /* PerfCounter::TrianglesProcessed */
+    uint64_t triangles_rasterized;     /* PerfCounter::TrianglesRasterized */
+    uint64_t triangles_clipped;        /* PerfCounter::TrianglesClipped */
+    uint64_t bins_rasterized;          /* PerfCounter::BinQueueFlushes */
+    uint64_t fragments_covered;        /* with GLIMP_DRAW_COUNT_FRAGMENTS */
+    uint64_t fragments_passed;
+    uint64_t setup_records;            /* triangles that left setup */
+    uint64_t bin_entries;              /* (triangle, bin) pairs */
+} glimp_counters;
+int glimp_get_counters(glimp_context* ctx, glimp_counters* out); /* synchronous */
+int glimp_reset_counters(glimp_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

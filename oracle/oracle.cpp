@@ -1,0 +1,879 @@
+// CPU oracle: scalar restatement of the reference's single-threaded,
+// non-binned draw path (Rasterizer::DrawMeshletsST, Rasterizer.cpp:153-254)
+// and of its vis-buffer resolve (Shading.cpp:417-579, 658-773).
+//
+// TEST INFRASTRUCTURE ONLY (see oracle.h). Built with -ffp-contract=off and
+// without fast-math, so every rounding below is the one written: where the
+// reference's -ffast-math build contracts a product into an FMA or turns a
+// vector division into VRCP14PS plus a Newton step, this file spells that
+// out, as established by oracle/probes/fastmath_probe.cpp.
+//
+// The reference evaluates 16 SIMD lanes at a time; lanes are independent
+// except where noted (packet order decides draw order; the texture filter
+// choice is taken over a 4x4 tile). This restatement keeps the 16-lane packet
+// structure so draw order and those tile-wide decisions are reproduced.
+#include "oracle.h"
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------- primitives
+
+uint16_t g_rcp[65536];
+uint16_t g_rsq[2][32768];
+
+inline uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+inline float bitsf(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+// VRCP14PS (SIMD.h:295), table model; exhaustive agreement with the
+// instruction is checked by glimpsw_amd/data/gen_approx14.c --verify.
+float rcp14(float xf) {
+    uint32_t x = fbits(xf);
+    uint32_t s = x & 0x80000000u;
+    int e = (int)((x >> 23) & 255);
+    uint32_t m = x & 0x7FFFFFu;
+    if (e == 255) return bitsf(m ? (x | 0x400000u) : s);
+    if (e == 0) {
+        if (m == 0) return bitsf(s | 0x7F800000u);
+        while (!(m & 0x800000u)) { m <<= 1; e--; }
+        m &= 0x7FFFFFu;
+        e += 1;
+    }
+    int re;
+    uint32_t mant;
+    if (m == 0) { re = 254 - e; mant = 0; }
+    else { re = 253 - e; mant = (uint32_t)g_rcp[m >> 7] << 7; }
+    if (re >= 255) return bitsf(s | 0x7F800000u);
+    if (re <= 0) {
+        int sh = 1 - re;
+        if (sh > 24) return bitsf(s);
+        return bitsf(s | ((mant | 0x800000u) >> sh));
+    }
+    return bitsf(s | ((uint32_t)re << 23) | mant);
+}
+
+// VRSQRT14PS (SIMD.h:294), table model.
+float rsqrt14(float xf) {
+    uint32_t x = fbits(xf);
+    uint32_t s = x & 0x80000000u;
+    int e = (int)((x >> 23) & 255);
+    uint32_t m = x & 0x7FFFFFu;
+    if (e == 255) { if (m) return bitsf(x | 0x400000u); return bitsf(s ? 0xFFC00000u : 0); }
+    if (e == 0) {
+        if (m == 0) return bitsf(s | 0x7F800000u);
+        while (!(m & 0x800000u)) { m <<= 1; e--; }
+        m &= 0x7FFFFFu;
+        e += 1;
+    }
+    if (s) return bitsf(0xFFC00000u);
+    int ue = e - 127;
+    int oc = ue & 1;
+    int half = (ue - oc) / 2;
+    if (m == 0 && oc == 0) return bitsf((uint32_t)(127 - half) << 23);
+    return bitsf(((uint32_t)(126 - half) << 23) | ((uint32_t)g_rsq[oc][m >> 8] << 7));
+}
+
+// Vector a/b under -ffast-math on AVX-512 (probe_div, probe_recip):
+//   r = rcp14(b); q = a*r; e = fma(q, b, -a); result = fma(-r, e, q)
+// For a == 1 the product a*r is exact, giving the 1/w sequence.
+float fdiv(float a, float b) {
+    float r = rcp14(b);
+    float q = a * r;
+    float e = fmaf(q, b, -a);
+    return fmaf(-r, e, q);
+}
+float frcp(float b) { return fdiv(1.0f, b); }
+
+// cvtps2dq: round to nearest even, "integer indefinite" when out of range.
+int32_t cvt_rne(float x) {
+    if (!(x >= -2147483648.0f && x < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)nearbyintf(x);
+}
+
+inline int32_t wrap_add(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+inline int32_t wrap_sub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+inline int32_t wrap_mul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+
+// Packed signed 16-bit pairs (x in the low half, y in the high half).
+inline int16_t lo16(uint32_t v) { return (int16_t)(v & 0xFFFF); }
+inline int16_t hi16(uint32_t v) { return (int16_t)(v >> 16); }
+inline uint32_t pk16(int lo, int hi) { return (uint32_t)(uint16_t)lo | ((uint32_t)(uint16_t)hi << 16); }
+
+// IEEE half -> float (vcvtph2ps is exact).
+float half_to_float(uint16_t h) {
+    uint32_t s = (uint32_t)(h & 0x8000) << 16;
+    uint32_t e = (h >> 10) & 31, m = h & 1023;
+    if (e == 0) {
+        if (m == 0) return bitsf(s);
+        float f = (float)m * (1.0f / 16777216.0f);  // m * 2^-24, exact
+        return s ? -f : f;
+    }
+    if (e == 31) return bitsf(s | 0x7F800000u | (m << 13));
+    return bitsf(s | ((e + 112) << 23) | (m << 13));
+}
+
+// simd::mul(mat4, v) with v.w = 1 (SIMD.h:457-464).
+void transform(const float* m, float x, float y, float z, float out[4]) {
+    for (int c = 0; c < 4; c++) {
+        out[c] = fmaf(x, m[0 * 4 + c], fmaf(y, m[1 * 4 + c], fmaf(z, m[2 * 4 + c], 1.0f * m[3 * 4 + c])));
+    }
+}
+
+constexpr uint32_t kMaxRenderSize = 2896;  // Rasterizer.h:203
+
+// ---------------------------------------------------------------- mesh shader
+
+struct ShadedMeshlet {  // Rasterizer.h:82-99
+    uint32_t PrimCount = 0;
+    int CullMode = 1;  // 0 None, 1 FrontCCW, 2 FrontCW
+    int FragmentShaderId = 0;
+    uint8_t Indices[3][128];
+    float Position[4][64];
+};
+
+// ShadeMeshlet (Shading.cpp:281-307).
+void shade_meshlet(const orc_shading_ctx& ctx, uint32_t index, ShadedMeshlet& out) {
+    if (ctx.cull_bitmap) {
+        uint16_t mask = ctx.cull_bitmap[index / 16];
+        if (((mask >> (index % 16)) & 1) == 0) { out.PrimCount = 0; return; }
+    }
+    const orc_meshlets& ms = *ctx.meshlets;
+    uint32_t mi = ctx.meshlet_offset + index;
+    const float* pos = ms.positions + (size_t)mi * 192;
+    uint32_t nv = ms.num_vertices[mi];
+    for (uint32_t i = 0; i < nv; i += 16) {
+        for (uint32_t l = 0; l < 16; l++) {
+            float c[4];
+            transform(ctx.object_to_clip, pos[0 * 64 + i + l], pos[1 * 64 + i + l], pos[2 * 64 + i + l], c);
+            for (int k = 0; k < 4; k++) out.Position[k][i + l] = c[k];
+        }
+    }
+    out.PrimCount = ms.num_triangles[mi];
+    memcpy(out.Indices, ms.indices + (size_t)mi * 384, 384);
+    uint32_t mat = ms.material_id[mi];
+    if (mat != UINT32_MAX && ctx.materials) {
+        const orc_material& mt = ctx.materials[mat];
+        out.CullMode = mt.double_sided ? 0 : 1;
+        out.FragmentShaderId = mt.alpha_cutoff < 255 ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------- packets
+
+struct V4 { float x, y, z, w; };
+
+struct ClipCodes {  // Rasterizer.cpp:51-55
+    uint16_t AcceptMask = 0, NonTrivialMask = 0;
+    uint8_t OutCodes[16] = {};
+};
+
+// Clipper::ComputeClipCodes (Rasterizer.cpp:353-397). The all-inside fast
+// path returns the same per-lane masks as the general path, so only the
+// general path is stated.
+ClipCodes compute_clip_codes(const V4 v[3][16], float bx, float by) {
+    ClipCodes cc;
+    for (int l = 0; l < 16; l++) {
+        uint8_t partial = 0, combined = 255;
+        bool trivial = true;
+        for (int k = 0; k < 3; k++) {
+            const V4& p = v[k][l];
+            uint8_t oc = 0;
+            if (p.x < -p.w) oc |= 1 << 0;
+            if (p.x > +p.w) oc |= 1 << 1;
+            if (p.y < -p.w) oc |= 1 << 2;
+            if (p.y > +p.w) oc |= 1 << 3;
+            if (p.z < -p.w) oc |= 1 << 4;
+            if (p.z > +p.w) oc |= 1 << 5;
+            partial |= oc;
+            combined &= oc;
+            trivial &= (fabsf(p.x) < p.w * bx) && (fabsf(p.y) < p.w * by);
+        }
+        trivial &= (partial & 0x30) == 0;
+        bool accept = combined == 0;
+        if (accept && trivial) cc.AcceptMask |= (uint16_t)(1u << l);
+        if (accept && !trivial) cc.NonTrivialMask |= (uint16_t)(1u << l);
+        cc.OutCodes[l] = partial;
+    }
+    return cc;
+}
+
+struct TrianglePacket {  // Rasterizer.h:145-161
+    uint32_t Pos0[16], Pos1[16], Pos2[16];
+    float Z0[16], Z1[16], Z2[16];
+    float W0[16], W1[16], W2[16];
+};
+
+// GetBoundingBox + GetRenderBoundingBox (Rasterizer.cpp:331-351).
+void render_bbox(const TrianglePacket& t, int l, int hw, int hh, uint32_t& bmin, uint32_t& bmax) {
+    uint32_t p0 = t.Pos0[l], p1 = t.Pos1[l], p2 = t.Pos2[l];
+    int mnx = std::min(std::min(lo16(p0), lo16(p1)), lo16(p2));
+    int mny = std::min(std::min(hi16(p0), hi16(p1)), hi16(p2));
+    int mxx = std::max(std::max(lo16(p0), lo16(p1)), lo16(p2));
+    int mxy = std::max(std::max(hi16(p0), hi16(p1)), hi16(p2));
+    // 32-bit add of 0x0007'0007 to the packed pair: the low half may carry
+    // into the high half; then an arithmetic >> 4 per 16-bit half.
+    uint32_t mn = pk16(mnx, mny) + 0x00070007u;
+    uint32_t mx = pk16(mxx, mxy) + 0x00070007u;
+    mn = pk16(lo16(mn) >> 4, hi16(mn) >> 4);
+    mx = pk16(lo16(mx) >> 4, hi16(mx) >> 4);
+    uint32_t vp = (uint32_t)hw | ((uint32_t)hh << 16);
+    uint32_t vp2 = vp * 2;
+    auto clampv = [&](uint32_t b) {
+        int x = (int16_t)(lo16(b) + lo16(vp));
+        int y = (int16_t)(hi16(b) + hi16(vp));
+        x = std::min(std::max(x, 0), (int)lo16(vp2));
+        y = std::min(std::max(y, 0), (int)hi16(vp2));
+        return pk16(x, y);
+    };
+    mn = clampv(mn);
+    mx = clampv(mx);
+    bmin = mn & ~0x00030003u;
+    bmax = (mx + 0x00030003u) & ~0x00030003u;
+}
+
+// TrianglePacket::Setup (Rasterizer.cpp:256-289). Returns the lane mask of
+// front-facing triangles whose tile-aligned render box is not empty.
+uint16_t setup_packet(TrianglePacket& t, const V4 v[3][16], int hw, int hh, int cullMode) {
+    uint16_t mask = 0;
+    float fsx = (float)(hw * 16), fsy = (float)(hh * 16);
+    for (int l = 0; l < 16; l++) {
+        float x[3], y[3], z[3], rw[3];
+        for (int k = 0; k < 3; k++) {
+            rw[k] = frcp(v[k][l].w);
+            x[k] = v[k][l].x * rw[k];
+            y[k] = v[k][l].y * rw[k];
+            z[k] = v[k][l].z * rw[k];
+        }
+        // (x2-x0)*(y1-y0) - (x0-x1)*(y0-y2), contracted as in probe_setup
+        float det = fmaf(x[2] - x[0], y[1] - y[0], (x[0] - x[1]) * (y[2] - y[0]));
+        if (cullMode != 1) {
+            bool flip = (cullMode == 2) ? true : (det < 0);
+            det = flip ? -det : det;
+        }
+        bool keep = det > 0;
+        int32_t X[3], Y[3];
+        for (int k = 0; k < 3; k++) {
+            X[k] = cvt_rne(x[k] * fsx);
+            Y[k] = cvt_rne(y[k] * fsy);
+        }
+        t.Pos0[l] = ((uint32_t)X[0] & 0xFFFF) | ((uint32_t)Y[0] << 16);
+        t.Pos1[l] = ((uint32_t)X[1] & 0xFFFF) | ((uint32_t)Y[1] << 16);
+        t.Pos2[l] = ((uint32_t)X[2] & 0xFFFF) | ((uint32_t)Y[2] << 16);
+        t.Z0[l] = z[0]; t.Z1[l] = z[1]; t.Z2[l] = z[2];
+        t.W0[l] = rw[0]; t.W1[l] = rw[1]; t.W2[l] = rw[2];
+        uint32_t bmin, bmax;
+        render_bbox(t, l, hw, hh, bmin, bmax);
+        if (lo16(bmin) >= lo16(bmax) || hi16(bmin) >= hi16(bmax)) keep = false;
+        if (keep) mask |= (uint16_t)(1u << l);
+    }
+    return mask;
+}
+
+struct EdgeVars {  // Rasterizer.h:162-176
+    int32_t Edge0[16], Edge1[16], Edge2[16];
+    int32_t A01[16], A12[16], A20[16];
+    int32_t B01[16], B12[16], B20[16];
+    float Z0[16], Z10[16], Z20[16];
+};
+
+int32_t compute_edge(int32_t a, int32_t x, int32_t b, int32_t y) {  // Rasterizer.cpp:291-295
+    int32_t w = wrap_add(wrap_mul(a, x), wrap_mul(b, y));
+    w = wrap_add(w, (a > 0 || (a == 0 && b > 0)) ? 0 : -1);
+    return w >> 4;
+}
+
+// TriangleEdgeVars::Setup (Rasterizer.cpp:296-329).
+void setup_edges(EdgeVars& e, const TrianglePacket& t, int hw, int hh) {
+    for (int l = 0; l < 16; l++) {
+        int32_t x0 = lo16(t.Pos0[l]), y0 = (int32_t)t.Pos0[l] >> 16;
+        int32_t x1 = lo16(t.Pos1[l]), y1 = (int32_t)t.Pos1[l] >> 16;
+        int32_t x2 = lo16(t.Pos2[l]), y2 = (int32_t)t.Pos2[l] >> 16;
+        int32_t A01 = y1 - y0, B01 = x0 - x1;
+        int32_t A12 = y2 - y1, B12 = x1 - x2;
+        int32_t A20 = y0 - y2, B20 = x2 - x0;
+        int32_t det = wrap_sub(wrap_mul(B20, A01), wrap_mul(B01, A20));
+        if (det < 0) {
+            A01 = -A01; B01 = -B01; A12 = -A12; B12 = -B12; A20 = -A20; B20 = -B20;
+            det = wrap_sub(0, det);
+        }
+        int32_t sx = -(hw << 4) + 8, sy = -(hh << 4) + 8;
+        e.Edge0[l] = compute_edge(A12, sx - x1, B12, sy - y1);
+        e.Edge1[l] = compute_edge(A20, sx - x2, B20, sy - y2);
+        e.Edge2[l] = compute_edge(A01, sx - x0, B01, sy - y0);
+        e.A01[l] = A01; e.A12[l] = A12; e.A20[l] = A20;
+        e.B01[l] = B01; e.B12[l] = B12; e.B20[l] = B20;
+        float rcpArea = fdiv(16.0f, (float)det);
+        e.Z0[l] = t.Z0[l];
+        e.Z10[l] = (t.Z1[l] - t.Z0[l]) * rcpArea;
+        e.Z20[l] = (t.Z2[l] - t.Z0[l]) * rcpArea;
+    }
+}
+
+inline uint32_t pixel_offset(uint32_t W, uint32_t x, uint32_t y) {  // Rasterizer.h:50-56
+    return ((x & ~3u) << 2) + (y & ~3u) * W + (x & 3) + (y & 3) * 4;
+}
+
+struct DrawState {
+    orc_framebuffer* fb;
+    const orc_raster_opts* opts;
+    orc_counters* ctr;
+};
+
+// Rasterizer::DrawTriangle (Rasterizer.h:250-328) with FS_EncodeSurfaceId
+// without alpha test (Shading.cpp:309-331) inlined.
+void draw_triangle(DrawState& st, const EdgeVars& e, int l, uint32_t bmin, uint32_t bmax, uint32_t surfaceId) {
+    orc_framebuffer* fb = st.fb;
+    uint32_t W = fb->width;
+    uint32_t* ids = fb->data;
+    float* depthBuf = (float*)(fb->data + fb->layer_stride);
+    int minX = lo16(bmin), minY = hi16(bmin), maxX = lo16(bmax), maxY = hi16(bmax);
+    for (int ty = minY; ty < maxY; ty += 4) {
+        for (int tx = minX; tx < maxX; tx += 4) {
+            for (int p = 0; p < 16; p++) {
+                int x = tx + (p & 3), y = ty + (p >> 2);
+                int32_t w0 = wrap_add(wrap_add(e.Edge0[l], wrap_mul(e.A12[l], x)), wrap_mul(e.B12[l], y));
+                int32_t w1 = wrap_add(wrap_add(e.Edge1[l], wrap_mul(e.A20[l], x)), wrap_mul(e.B20[l], y));
+                int32_t w2 = wrap_add(wrap_add(e.Edge2[l], wrap_mul(e.A01[l], x)), wrap_mul(e.B01[l], y));
+                if ((w0 | w1 | w2) < 0) continue;
+                st.ctr->fragments_covered++;
+                if (st.opts && st.opts->coverage_count) st.opts->coverage_count[(size_t)y * W + x]++;
+                float u = (float)w1, v = (float)w2;
+                float depth = fmaf(u, e.Z10[l], fmaf(v, e.Z20[l], e.Z0[l]));
+                uint32_t off = pixel_offset(W, (uint32_t)x, (uint32_t)y);
+                if (!(depth > depthBuf[off])) continue;
+                depthBuf[off] = depth;
+                ids[off] = surfaceId;
+                st.ctr->fragments_passed++;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- clipper
+
+// Clipper (Rasterizer.cpp:56-82, 398-491).
+struct Clipper {
+    float bx, by;
+    float Vertices[6][16 * 3 + 16];
+    V4 Out[3][8][16];
+    uint8_t PrimId[8][16];
+    uint16_t Masks[8];
+    TrianglePacket Tris[8];
+    uint32_t OutIdx = 0;
+
+    float dist(uint8_t i, uint32_t plane, float scale) const {
+        float a = Vertices[plane / 2][i];
+        if (plane % 2) a = -a;
+        return fmaf(Vertices[3][i], scale, a);  // a + w*scale, contracted
+    }
+
+    void flush(int hw, int hh, int cullMode, bool isLast) {
+        uint32_t pi = OutIdx / 16;
+        V4 v[3][16];
+        for (int k = 0; k < 3; k++)
+            for (int l = 0; l < 16; l++) v[k][l] = Out[k][pi][l];
+        uint16_t m = setup_packet(Tris[pi], v, hw, hh, cullMode);
+        m &= (uint16_t)((1u << (OutIdx % 16 + (isLast ? 0 : 1))) - 1);
+        Masks[pi] = m;
+    }
+
+    void clip(const V4 vin[3][16], const ClipCodes& cc, int hw, int hh, int cullMode) {
+        for (int k = 0; k < 3; k++) {
+            for (int l = 0; l < 16; l++) {
+                int j = k * 16 + l;
+                Vertices[0][j] = vin[k][l].x;
+                Vertices[1][j] = vin[k][l].y;
+                Vertices[2][j] = vin[k][l].z;
+                Vertices[3][j] = vin[k][l].w;
+                Vertices[4][j] = k == 1 ? 1.0f : 0.0f;
+                Vertices[5][j] = k == 2 ? 1.0f : 0.0f;
+            }
+        }
+        for (int prim = 0; prim < 16; prim++) {
+            if (!((cc.NonTrivialMask >> prim) & 1)) continue;
+            uint8_t idx[32] = { (uint8_t)prim, (uint8_t)(prim + 16), (uint8_t)(prim + 32) };
+            uint32_t vertCount = 3, nextIdx = 48;
+            for (uint32_t plane = 0; plane < 8; plane++) {
+                if (!((cc.OutCodes[prim] >> plane) & 1)) continue;
+                uint8_t out[32];
+                uint32_t outCount = 0;
+                float scale = plane < 4 ? (plane / 2 == 0 ? bx : by) : 1.0f;
+                for (uint32_t vi = 0; vi < vertCount; vi++) {
+                    uint8_t ia = idx[vi], ib = idx[(vi + 1) % vertCount];
+                    float da = dist(ia, plane, scale), db = dist(ib, plane, scale);
+                    if (da >= 0) out[outCount++] = ia;
+                    if ((da >= 0) != (db >= 0)) {
+                        float t = da / (da - db);
+                        for (int k = 0; k < 6; k++)
+                            Vertices[k][nextIdx] = fmaf(Vertices[k][ib], t, fmaf(-t, Vertices[k][ia], Vertices[k][ia]));
+                        out[outCount++] = (uint8_t)nextIdx++;
+                    }
+                }
+                if (vertCount < 3) break;
+                vertCount = outCount;
+                memcpy(idx, out, sizeof(idx));
+            }
+            if (vertCount < 2) continue;
+            for (uint32_t vi = 0; vi + 2 < vertCount; vi++) {
+                uint32_t i0 = idx[0], i1 = idx[vi + 1], i2 = idx[vi + 2];
+                uint32_t lane = OutIdx % 16, pi = OutIdx / 16;
+                uint32_t ii[3] = { i0, i1, i2 };
+                for (int k = 0; k < 3; k++)
+                    Out[k][pi][lane] = { Vertices[0][ii[k]], Vertices[1][ii[k]], Vertices[2][ii[k]], Vertices[3][ii[k]] };
+                PrimId[pi][lane] = (uint8_t)prim;
+                if (lane == 15) flush(hw, hh, cullMode, false);
+                if (++OutIdx >= 128) return;
+            }
+        }
+        if (OutIdx % 16 != 0) flush(hw, hh, cullMode, true);
+    }
+};
+
+// ---------------------------------------------------------------- textures
+
+struct Tex { const orc_texture* t; };
+
+inline uint32_t texel_offset(int32_t x, int32_t y, int32_t stride) {  // TiledY8, Texture.h:482-491
+    return (uint32_t)((y & 7) | (x << 3) | ((y & ~7) << stride));
+}
+
+inline uint32_t lerp16(uint32_t a, uint32_t b, uint32_t t) {  // SIMD.h:450 per 16-bit lane
+    uint32_t r = 0;
+    for (int h = 0; h < 2; h++) {
+        int16_t av = (int16_t)(a >> (16 * h)), bv = (int16_t)(b >> (16 * h)), tv = (int16_t)(t >> (16 * h));
+        int16_t d = (int16_t)(bv - av);
+        int32_t prod = (int32_t)d * (int32_t)tv;
+        int16_t mh = (int16_t)(((prod >> 14) + 1) >> 1);  // mulhrs
+        r |= (uint32_t)(uint16_t)(int16_t)(av + mh) << (16 * h);
+    }
+    return r;
+}
+
+// Texture2D::SampleLevel for RGBA8u (Texture.h:412-459, 506-556). The filter
+// is chosen once for the 16 lanes of a 4x4 tile (Texture.h:432).
+void sample_level16(const orc_texture* tex, const float* u, const float* v, int32_t layer,
+                    const int32_t* mipIn, int wrap, bool magLinear, bool minLinear,
+                    const bool* active, uint32_t* out) {
+    int32_t ix[16], iy[16], mip[16];
+    bool anyPos = false;
+    for (int l = 0; l < 16; l++) {
+        float su = u[l] * tex->scale_lerp_u, sv = v[l] * tex->scale_lerp_v;
+        ix[l] = cvt_rne(su);
+        iy[l] = cvt_rne(sv);
+        if (wrap == 0) {  // Repeat
+            ix[l] &= tex->mask_lerp_u;
+            iy[l] &= tex->mask_lerp_v;
+        } else if (wrap == 1) {  // ClampToEdge
+            ix[l] = std::min(std::max(ix[l], 0), tex->mask_lerp_u);
+            iy[l] = std::min(std::max(iy[l], 0), tex->mask_lerp_v);
+        } else {  // MirroredRepeat
+            ix[l] = (ix[l] & tex->mask_lerp_u) ^ ((ix[l] & (tex->mask_lerp_u + 1)) != 0 ? tex->mask_lerp_u : 0);
+            iy[l] = (iy[l] & tex->mask_lerp_v) ^ ((iy[l] & (tex->mask_lerp_v + 1)) != 0 ? tex->mask_lerp_v : 0);
+        }
+        if ((!active || active[l]) && mipIn[l] > 0) anyPos = true;
+    }
+    bool linear = anyPos ? minLinear : magLinear;
+    int maxLevel = (int)tex->mip_levels - 1;
+    for (int l = 0; l < 16; l++) {
+        int m = std::min(std::max(mipIn[l], 0), maxLevel);
+        mip[l] = m;
+        int32_t offset = layer * (int32_t)tex->layer_stride + (int32_t)tex->mip_offsets[m];
+        int32_t stride = (int32_t)tex->row_shift - m;
+        int32_t x = ix[l] >> m, y = iy[l] >> m;
+        const uint32_t* D = tex->data;
+        if (!linear) {
+            out[l] = D[offset + (int32_t)texel_offset(x >> 8, y >> 8, stride)];
+            continue;
+        }
+        int32_t xf = std::max(x - 127, 0), yf = std::max(y - 127, 0);
+        int32_t tx = xf >> 8, ty = yf >> 8;
+        bool inX = ((tx + 1) << m) < (int32_t)tex->width;
+        bool inY = ((ty + 1) << m) < (int32_t)tex->height;
+        int32_t i00 = offset + (int32_t)texel_offset(tx, ty, stride);
+        int32_t i01 = offset + (int32_t)texel_offset(tx, ty + (inY ? 1 : 0), stride);
+        uint32_t d00 = D[i00], d10 = D[i00 + 8], d01 = D[i01], d11 = D[i01 + 8];
+        uint32_t fx = (uint32_t)(xf & 255) << 7, fy = (uint32_t)(yf & 255) << 7;
+        fx = (fx << 16) | fx;
+        fy = (fy << 16) | fy;
+        if (!inX) fx = 0;
+        uint32_t rb1 = lerp16(d00 & 0x00FF00FF, d10 & 0x00FF00FF, fx);
+        uint32_t ga1 = lerp16((d00 >> 8) & 0x00FF00FF, (d10 >> 8) & 0x00FF00FF, fx);
+        uint32_t rb2 = lerp16(d01 & 0x00FF00FF, d11 & 0x00FF00FF, fx);
+        uint32_t ga2 = lerp16((d01 >> 8) & 0x00FF00FF, (d11 >> 8) & 0x00FF00FF, fx);
+        uint32_t rb = lerp16(rb1, rb2, fy), ga = lerp16(ga1, ga2, fy);
+        out[l] = rb | (ga << 8);
+    }
+}
+
+// ---------------------------------------------------------------- resolve
+
+// IntersectTriangle (Shading.cpp:417-464) in the operation order clang's
+// fast-math build uses (probe_intersect).
+void intersect_triangle(const float c[3][4], float su, float sv, float sw, float sh,
+                        float bary[3], float ddx[3], float ddy[3]) {
+    float iw0 = frcp(c[0][3]), iw1 = frcp(c[1][3]), iw2 = frcp(c[2][3]);
+    float p0x = c[0][0] * iw0, p0y = c[0][1] * iw0;
+    float p1x = c[1][0] * iw1, p1y = c[1][1] * iw1;
+    float p2x = c[2][0] * iw2, p2y = c[2][1] * iw2;
+    float m0x = p2x - p1x, m1x = p0x - p1x, m1y = p0y - p1y;
+    float d12y = p1y - p2y;
+    float det = fmaf(m0x, m1y, d12y * m1x);
+    float invDet = frcp(det);
+    float a0 = invDet * iw0, a1 = invDet * iw1, a2 = invDet * iw2;
+    float dx0 = a0 * d12y, dx1 = a1 * (p2y - p0y), dx2 = a2 * m1y;
+    float dy0 = a0 * m0x, dy1 = a1 * (p0x - p2x), dy2 = a2 * (p1x - p0x);
+    float dxs = (dx0 + dx1) + dx2, dys = (dy0 + dy1) + dy2;
+    float r0x = su - p0x, r0y = sv - p0y;
+    float iiw = fmaf(dys, r0y, fmaf(dxs, r0x, iw0));
+    float iW = frcp(iiw);
+    float by = iW * fmaf(dx1, r0x, dy1 * r0y);
+    float bz = iW * fmaf(dx2, r0x, dy2 * r0y);
+    float bx = 1.0f - (by + bz);
+    float kx = 2.0f / sw, ky = -2.0f / sh;
+    float wdx = frcp(fmaf(kx, dxs, iiw)), wdy = frcp(fmaf(ky, dys, iiw));
+    float b[3] = { bx, by, bz }, dxk[3] = { dx0, dx1, dx2 }, dyk[3] = { dy0, dy1, dy2 };
+    for (int k = 0; k < 3; k++) {
+        float bi = b[k] * iiw;
+        ddx[k] = fmaf(wdx, fmaf(kx, dxk[k], bi), -b[k]);
+        ddy[k] = fmaf(wdy, fmaf(ky, dyk[k], bi), -b[k]);
+        bary[k] = b[k];
+    }
+}
+
+struct SurfaceLane {
+    bool active;
+    float u, v;
+    int32_t mip;       // per material texture scale
+    uint32_t material;
+    float bary[3];
+    float nrm[3];
+    uint32_t albedo;
+};
+
+// ResolveSurface (Shading.cpp:471-579) for one 4x4 tile: vertex fetch,
+// re-transform, analytic barycentrics, UV gradients and albedo sampling.
+// The tile-wide filter choice is taken over the tile's covered lanes; the
+// reference also lets uncovered (sky) lanes vote with register contents left
+// over from the waterfall loop, which this restatement leaves out.
+void resolve_surface_tile(const orc_shading_ctx& ctx, const uint32_t* ids, const bool* active,
+                          const float* suv_x, const float* suv_y, float sw, float sh,
+                          SurfaceLane* lanes) {
+    const orc_meshlets& ms = *ctx.meshlets;
+    float gradU[16][2], gradV[16][2];  // (dx, dy) of u and v
+    for (int l = 0; l < 16; l++) {
+        SurfaceLane& L = lanes[l];
+        L.active = active[l];
+        L.albedo = 0;
+        L.material = UINT32_MAX;
+        L.u = L.v = 0;
+        gradU[l][0] = gradU[l][1] = gradV[l][0] = gradV[l][1] = 0;
+        if (!active[l]) continue;
+        uint32_t sid = ids[l];
+        uint32_t mi = sid / 128, tri = sid % 128;
+        float c[3][4];
+        float uv[3][2];
+        for (int k = 0; k < 3; k++) {
+            uint32_t vi = ms.indices[(size_t)mi * 384 + k * 128 + tri];
+            const float* P = ms.positions + (size_t)mi * 192;
+            transform(ctx.object_to_clip, P[vi], P[64 + vi], P[128 + vi], c[k]);
+            uint32_t tc = ms.texcoords[(size_t)mi * 64 + vi];
+            uv[k][0] = half_to_float((uint16_t)(tc & 0xFFFF));
+            uv[k][1] = half_to_float((uint16_t)(tc >> 16));
+        }
+        L.material = ms.material_id[mi];
+        float bary[3], ddx[3], ddy[3];
+        intersect_triangle(c, suv_x[l], suv_y[l], sw, sh, bary, ddx, ddy);
+        for (int k = 0; k < 3; k++) L.bary[k] = bary[k];
+        float t10x = uv[1][0] - uv[0][0], t10y = uv[1][1] - uv[0][1];
+        float t20x = uv[2][0] - uv[0][0], t20y = uv[2][1] - uv[0][1];
+        L.u = fmaf(bary[2], t20x, fmaf(bary[1], t10x, uv[0][0]));
+        L.v = fmaf(bary[2], t20y, fmaf(bary[1], t10y, uv[0][1]));
+        gradU[l][0] = fmaf(t20x, ddx[2], t10x * ddx[1]);
+        gradV[l][0] = fmaf(t20y, ddx[2], t10y * ddx[1]);
+        gradU[l][1] = fmaf(t20x, ddy[2], t10x * ddy[1]);
+        gradV[l][1] = fmaf(t20y, ddy[2], t10y * ddy[1]);
+    }
+    // Material waterfall (Shading.cpp:532-545): for each distinct material in
+    // the tile, the mip level of every lane is taken with that material's
+    // texture scale, and the filter choice is made over all covered lanes.
+    bool done[16] = {};
+    for (int l0 = 0; l0 < 16; l0++) {
+        if (!active[l0] || done[l0] || lanes[l0].material == UINT32_MAX) continue;
+        uint32_t mat = lanes[l0].material;
+        const orc_material& mt = ctx.materials[mat];
+        if (mt.texture < 0) {
+            for (int l = 0; l < 16; l++) if (active[l] && lanes[l].material == mat) done[l] = true;
+            continue;
+        }
+        const orc_texture* tex = &ctx.textures[mt.texture];
+        float ssu = tex->scale_u * tex->scale_u, ssv = tex->scale_v * tex->scale_v;
+        int32_t mip[16];
+        float U[16], V[16];
+        for (int l = 0; l < 16; l++) {
+            float dx = fmaf(gradU[l][0], gradU[l][0], gradV[l][0] * gradV[l][0]) * ssu;
+            float dy = fmaf(gradU[l][1], gradU[l][1], gradV[l][1] * gradV[l][1]) * ssv;
+            float mx = (dx > dy) ? dx : dy;  // vmaxps: second operand on NaN/equal
+            mip[l] = wrap_sub((int32_t)fbits(mx), 127 << 23) >> 24;
+            U[l] = lanes[l].u;
+            V[l] = lanes[l].v;
+        }
+        uint32_t res[16];
+        // SurfaceSampler: Repeat, Mag Linear, Min Nearest (Shading.cpp:6-10)
+        sample_level16(tex, U, V, 0, mip, 0, true, false, active, res);
+        for (int l = 0; l < 16; l++) {
+            if (active[l] && lanes[l].material == mat) {
+                lanes[l].albedo = res[l];
+                done[l] = true;
+            }
+        }
+    }
+}
+
+// RGBA8u::Pack of (r,g,b,a) (Texture.h:55-67): RNE, saturating packs.
+uint32_t pack_rgba8(float r, float g, float b, float a) {
+    float c[4] = { r, g, b, a };
+    uint32_t out = 0;
+    for (int k = 0; k < 4; k++) {
+        int32_t i = cvt_rne(c[k] * 255.0f);
+        i = std::min(std::max(i, -32768), 32767);  // packs_epi32
+        i = std::min(std::max(i, 0), 255);          // packus_epi16
+        out |= (uint32_t)i << (8 * k);
+    }
+    return out;
+}
+
+}  // namespace
+
+// ====================================================================== API
+
+extern "C" {
+
+void orc_load_tables(const uint16_t* tables) {
+    memcpy(g_rcp, tables, sizeof(g_rcp));
+    memcpy(g_rsq, tables + 65536, sizeof(g_rsq));
+}
+float orc_rcp14(float x) { return rcp14(x); }
+float orc_rsqrt14(float x) { return rsqrt14(x); }
+float orc_div(float a, float b) { return fdiv(a, b); }
+int32_t orc_cvt_rne(float x) { return cvt_rne(x); }
+
+void orc_fb_clear(orc_framebuffer* fb, uint32_t color, float depth) {  // Rasterizer.h:35-48
+    uint32_t n = fb->width * fb->height;
+    for (uint32_t i = 0; i < n; i++) fb->data[i] = color;
+    uint32_t db = fbits(depth);
+    for (uint32_t i = 0; i < n; i++) fb->data[fb->layer_stride + i] = db;
+}
+
+void orc_get_pixels(const orc_framebuffer* fb, uint32_t layer, uint32_t* dest, uint32_t stride) {
+    const uint32_t* src = fb->data + (size_t)layer * fb->layer_stride;
+    for (uint32_t y = 0; y < fb->height; y++)
+        for (uint32_t x = 0; x < fb->width; x++)
+            dest[(size_t)y * stride + x] = src[pixel_offset(fb->width, x, y)];
+}
+
+void orc_draw_meshlets(orc_framebuffer* fb, uint32_t count, const orc_shading_ctx* ctx,
+                       const orc_raster_opts* opts, orc_counters* counters) {
+    orc_counters localCtr = {};
+    orc_counters* ctr = counters ? counters : &localCtr;
+    DrawState st = { fb, opts, ctr };
+    int hw = (int)fb->width / 2, hh = (int)fb->height / 2;
+    bool guard = !opts || opts->enable_guardband;
+    bool clipping = !opts || opts->enable_clipping;
+    float bx = guard ? (float)kMaxRenderSize / (float)fb->width : 1.0f;
+    float by = guard ? (float)kMaxRenderSize / (float)fb->height : 1.0f;
+
+    ShadedMeshlet mesh;
+    for (uint32_t meshIdx = 0; meshIdx < count; meshIdx++) {
+        mesh = ShadedMeshlet();
+        memset(mesh.Position, 0, sizeof(mesh.Position));
+        shade_meshlet(*ctx, meshIdx, mesh);
+        if (mesh.PrimCount == 0) continue;
+        ctr->triangles_processed += mesh.PrimCount;
+        uint32_t meshletId = ctx->meshlet_offset + meshIdx;
+
+        for (uint32_t primOffset = 0; primOffset < mesh.PrimCount; primOffset += 16) {
+            V4 v[3][16];
+            for (int k = 0; k < 3; k++) {
+                for (int l = 0; l < 16; l++) {
+                    // gather_preload64 (SIMD.h:219-227): two 32-entry
+                    // permutes chosen by idx < 32
+                    uint32_t ix = mesh.Indices[k][primOffset + l];
+                    uint32_t vi = ix < 32 ? ix : 32 + (ix & 31);
+                    v[k][l] = { mesh.Position[0][vi], mesh.Position[1][vi], mesh.Position[2][vi], mesh.Position[3][vi] };
+                }
+            }
+            ClipCodes cc = compute_clip_codes(v, bx, by);
+            if (primOffset + 16 > mesh.PrimCount) {
+                uint16_t trail = (uint16_t)((1u << (mesh.PrimCount % 16)) - 1);
+                cc.AcceptMask &= trail;
+                cc.NonTrivialMask &= trail;
+            }
+            if (cc.AcceptMask != 0) {
+                TrianglePacket tris;
+                uint16_t accept = setup_packet(tris, v, hw, hh, mesh.CullMode) & cc.AcceptMask;
+                if (accept) {
+                    EdgeVars e;
+                    setup_edges(e, tris, hw, hh);
+                    for (int l = 0; l < 16; l++) {
+                        if (!((accept >> l) & 1)) continue;
+                        uint32_t bmin, bmax;
+                        render_bbox(tris, l, hw, hh, bmin, bmax);
+                        draw_triangle(st, e, l, bmin, bmax, meshletId * 128 + primOffset + (uint32_t)l);
+                        ctr->triangles_rasterized++;
+                    }
+                }
+            }
+            if (cc.NonTrivialMask != 0 && clipping) {
+                for (int l = 0; l < 16; l++) ctr->triangles_clipped += (cc.NonTrivialMask >> l) & 1;
+                static thread_local Clipper clipper;
+                clipper.bx = bx;
+                clipper.by = by;
+                clipper.OutIdx = 0;
+                clipper.clip(v, cc, hw, hh, mesh.CullMode);
+                for (uint32_t i = 0; i < clipper.OutIdx; i += 16) {
+                    const TrianglePacket& tris = clipper.Tris[i / 16];
+                    EdgeVars e;
+                    setup_edges(e, tris, hw, hh);
+                    uint16_t drawMask = clipper.Masks[i / 16];
+                    for (int l = 0; l < 16; l++) {
+                        if (!((drawMask >> l) & 1)) continue;
+                        uint32_t bmin, bmax;
+                        render_bbox(tris, l, hw, hh, bmin, bmax);
+                        uint32_t prim = clipper.PrimId[i / 16][l];
+                        draw_triangle(st, e, l, bmin, bmax, meshletId * 128 + primOffset + prim);
+                        ctr->triangles_rasterized++;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// CullMeshlets frustum branch (Shading.cpp:797-809, 865-866); the planes are
+// computed by the caller (Gribb-Hartmann, Shading.cpp:783-791).
+uint32_t orc_cull_meshlets(uint16_t* bitmap, const orc_meshlets* m, uint32_t first, uint32_t count,
+                           const float planes[20]) {
+    uint32_t visible = 0;
+    for (uint32_t off = 0; off < count; off += 16) {
+        uint16_t mask = 0;
+        for (uint32_t l = 0; l < 16 && off + l < count; l++) {
+            const float* b = m->bounds + (size_t)(first + off + l) * 4;
+            bool vis = true;
+            for (int i = 0; i < 5; i++) {
+                const float* P = planes + i * 4;
+                float d = fmaf(b[0], P[0], fmaf(b[1], P[1], b[2] * P[2])) + P[3];
+                vis = vis && (d > -b[3]);
+            }
+            if (vis) { mask |= (uint16_t)(1u << l); visible++; }
+        }
+        bitmap[off / 16] = mask;
+    }
+    return visible;
+}
+
+void orc_sample_level16(const orc_texture* tex, const float* u, const float* v, int32_t layer,
+                        const int32_t* mip, int32_t wrap, int32_t mag_linear, int32_t min_linear,
+                        uint32_t* out) {
+    sample_level16(tex, u, v, layer, mip, wrap, mag_linear != 0, min_linear != 0, nullptr, out);
+}
+
+void orc_intersect_triangle16(const float* clip, const float* suv, float sw, float sh,
+                              float* bary, float* ddx, float* ddy) {
+    for (int l = 0; l < 16; l++) {
+        float c[3][4];
+        for (int k = 0; k < 3; k++)
+            for (int j = 0; j < 4; j++) c[k][j] = clip[(k * 4 + j) * 16 + l];
+        float b[3], dx[3], dy[3];
+        intersect_triangle(c, suv[l], suv[16 + l], sw, sh, b, dx, dy);
+        for (int k = 0; k < 3; k++) {
+            bary[k * 16 + l] = b[k];
+            ddx[k * 16 + l] = dx[k];
+            ddy[k * 16 + l] = dy[k];
+        }
+    }
+}
+
+void orc_setup16(const float* clip, int32_t w, int32_t h, int32_t cull_mode,
+                 int32_t* X, int32_t* Y, float* Z, float* W, float* det, uint16_t* mask) {
+    V4 v[3][16];
+    for (int k = 0; k < 3; k++)
+        for (int l = 0; l < 16; l++)
+            v[k][l] = { clip[(k * 4 + 0) * 16 + l], clip[(k * 4 + 1) * 16 + l], clip[(k * 4 + 2) * 16 + l], clip[(k * 4 + 3) * 16 + l] };
+    TrianglePacket t;
+    *mask = setup_packet(t, v, w / 2, h / 2, cull_mode);
+    uint32_t* P[3] = { t.Pos0, t.Pos1, t.Pos2 };
+    float* Zs[3] = { t.Z0, t.Z1, t.Z2 };
+    float* Ws[3] = { t.W0, t.W1, t.W2 };
+    for (int k = 0; k < 3; k++) {
+        for (int l = 0; l < 16; l++) {
+            X[k * 16 + l] = lo16(P[k][l]);
+            Y[k * 16 + l] = (int32_t)P[k][l] >> 16;
+            Z[k * 16 + l] = Zs[k][l];
+            W[k * 16 + l] = Ws[k][l];
+        }
+    }
+    // det is not kept by the packet; recompute it for the probe comparison
+    for (int l = 0; l < 16; l++) {
+        float x[3], y[3];
+        for (int k = 0; k < 3; k++) {
+            float rw = frcp(v[k][l].w);
+            x[k] = v[k][l].x * rw;
+            y[k] = v[k][l].y * rw;
+        }
+        float d = fmaf(x[2] - x[0], y[1] - y[0], (x[0] - x[1]) * (y[2] - y[0]));
+        if (cull_mode != 1) {
+            bool flip = (cull_mode == 2) ? true : (d < 0);
+            d = flip ? -d : d;
+        }
+        det[l] = d;
+    }
+}
+
+// ResolveDebug(DebugLayer::BaseColor) (Shading.cpp:734-773).
+void orc_resolve_albedo(orc_framebuffer* fb, const orc_shading_ctx* ctx) {
+    uint32_t W = fb->width, H = fb->height;
+    float sw = (float)W, sh = (float)H;
+    float scaleX = 2.0f / sw, scaleY = 2.0f / sh;
+    float centerX = fmaf(0.5f, scaleX, -1.0f), centerY = fmaf(0.5f, scaleY, -1.0f);
+    const float* depth = (const float*)(fb->data + fb->layer_stride);
+    std::vector<uint32_t> out(fb->layer_stride);
+    memcpy(out.data(), fb->data, (size_t)W * H * 4);
+    for (uint32_t y0 = 0; y0 < H; y0 += 4) {
+        for (uint32_t x0 = 0; x0 < W; x0 += 4) {
+            uint32_t off = pixel_offset(W, x0, y0);
+            bool active[16];
+            float sx[16], sy[16];
+            uint32_t ids[16];
+            for (int l = 0; l < 16; l++) {
+                active[l] = !(depth[off + l] <= 0.0f);
+                ids[l] = fb->data[off + l];
+                sx[l] = fmaf((float)(int32_t)(x0 + (l & 3)), scaleX, centerX);
+                sy[l] = fmaf((float)(int32_t)(y0 + (l >> 2)), scaleY, centerY);
+            }
+            SurfaceLane lanes[16];
+            bool anyActive = false;
+            for (int l = 0; l < 16; l++) anyActive |= active[l];
+            if (anyActive) resolve_surface_tile(*ctx, ids, active, sx, sy, sw, sh, lanes);
+            uint32_t bg = ((x0 ^ y0) & 4) ? 0xFFA0A0A0u : 0xFFFFFFFFu;
+            for (int l = 0; l < 16; l++) {
+                if (!active[l]) { out[off + l] = bg; continue; }
+                uint32_t a = lanes[l].albedo;
+                // Unpack * (1/255) then Pack: the round trip returns the bytes
+                float r = (float)(a & 255) * (1.0f / 255), g = (float)((a >> 8) & 255) * (1.0f / 255);
+                float b = (float)((a >> 16) & 255) * (1.0f / 255);
+                out[off + l] = pack_rgba8(r, g, b, 1.0f);
+            }
+        }
+    }
+    memcpy(fb->data, out.data(), (size_t)W * H * 4);
+}
+
+void orc_resolve_lit(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_resolve_params* p) {
+    (void)fb; (void)ctx; (void)p;  // filled in with the lit resolve
+}
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+/*
+ * CPU oracle for the meshlet raster -> vis-buffer -> resolve path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a scalar restatement of the reference
+ * renderer's algorithm (dubiousconst282/GLimpSW, src/SwRast), written from
+ * its source text and from probes of how clang lowers its floating-point
+ * expressions under the reference's -ffast-math flags. Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, as the
+ * checker the HIP path is compared against. The product never calls it.
+ *
+ * Layouts follow the reference:
+ *  - meshlets as structure-of-arrays of the Meshlet record (Scene.h:15-30);
+ *  - framebuffer layers in 4x4 tiles (Rasterizer.h:10-78);
+ *  - textures in TiledY8 layout with 64-texel aligned mip offsets
+ *    (Texture.h:482-501, 599-636).
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_meshlets {
+    uint32_t count;
+    const float* bounds;              /* [count][4]  BoundCenter.xyz, BoundRadius */
+    const uint8_t* num_vertices;      /* [count] */
+    const uint8_t* num_triangles;     /* [count] */
+    const uint32_t* material_id;      /* [count], UINT32_MAX = none */
+    const uint64_t* tangent_handedness; /* [count] */
+    const float* positions;           /* [count][3][64] */
+    const uint32_t* texcoords;        /* [count][64] half2 */
+    const uint32_t* normal_tangents;  /* [count][64] oct8 x4 */
+    const uint8_t* indices;           /* [count][3][128] */
+} orc_meshlets;
+
+typedef struct orc_texture {
+    uint32_t width, height, mip_levels, num_layers;
+    uint32_t row_shift, layer_stride;
+    float scale_u, scale_v, scale_lerp_u, scale_lerp_v;
+    int32_t mask_u, mask_v, mask_lerp_u, mask_lerp_v;
+    uint32_t mip_offsets[16];
+    const uint32_t* data;             /* layer_stride * num_layers texels (+ 64 slack) */
+} orc_texture;
+
+typedef struct orc_material {
+    int32_t texture;                  /* index into textures, -1 = none */
+    uint8_t double_sided;
+    uint8_t alpha_cutoff;
+    uint8_t pad[2];
+} orc_material;
+
+typedef struct orc_shading_ctx {
+    const orc_meshlets* meshlets;
+    const orc_material* materials;
+    const orc_texture* textures;
+    uint32_t meshlet_offset;
+    const uint16_t* cull_bitmap;      /* 1 bit per meshlet, 16 per word; NULL = all visible */
+    float object_to_clip[16];         /* column-major, m[c*4+r] */
+} orc_shading_ctx;
+
+typedef struct orc_framebuffer {
+    uint32_t width, height, layer_stride, num_layers;
+    uint32_t* data;                   /* layer_stride * num_layers */
+} orc_framebuffer;
+
+typedef struct orc_raster_opts {
+    uint8_t enable_clipping;
+    uint8_t enable_guardband;
+    uint8_t pad[2];
+    uint32_t* coverage_count;         /* optional [height][width]: covered samples per pixel, before depth test */
+} orc_raster_opts;
+
+typedef struct orc_counters {
+    uint64_t triangles_processed;
+    uint64_t triangles_rasterized;
+    uint64_t triangles_clipped;
+    uint64_t fragments_covered;       /* covered samples before the depth test */
+    uint64_t fragments_passed;        /* samples that passed the depth test */
+} orc_counters;
+
+typedef struct orc_light {
+    uint32_t type;                    /* 0 directional, 1 point, 2 spot */
+    float position[3];
+    float direction[3];
+    float color[3];
+    float intensity;
+    float inv_radius_sq, spot_scale, spot_offset;
+} orc_light;
+
+typedef struct orc_resolve_params {
+    float world_to_clip[16];          /* for the inverse screen projection */
+    float object_to_clip[16];
+    float object_to_world[9];         /* mat3 column-major */
+    float view_pos[3];
+    float exposure;
+    const orc_light* lights;
+    uint32_t num_lights;
+} orc_resolve_params;
+
+/* Tables reproducing VRCP14PS / VRSQRT14PS; must be loaded before any call. */
+void orc_load_tables(const uint16_t* tables /* 131072 entries */);
+float orc_rcp14(float x);
+float orc_rsqrt14(float x);
+float orc_div(float a, float b);      /* vector a/b under -ffast-math */
+int32_t orc_cvt_rne(float x);         /* cvtps2dq */
+
+void orc_fb_clear(orc_framebuffer* fb, uint32_t color, float depth);
+void orc_get_pixels(const orc_framebuffer* fb, uint32_t layer, uint32_t* dest, uint32_t stride);
+void orc_draw_meshlets(orc_framebuffer* fb, uint32_t count, const orc_shading_ctx* ctx,
+                       const orc_raster_opts* opts, orc_counters* counters);
+
+/* Frustum cull of bound spheres against 5 normalised planes (xyz,w each). */
+uint32_t orc_cull_meshlets(uint16_t* bitmap, const orc_meshlets* m, uint32_t first, uint32_t count,
+                           const float planes[20]);
+
+/* ResolveDebug(BaseColor): layer 0 becomes RGBA8 albedo, sky = checker bg. */
+void orc_resolve_albedo(orc_framebuffer* fb, const orc_shading_ctx* ctx);
+/* Resolve(): lit, tonemapped RGBA8 into layer 0. */
+void orc_resolve_lit(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_resolve_params* p);
+
+/* Sampler entry for tests: 16 lanes = one 4x4 tile (Texture.h:412-459). */
+void orc_sample_level16(const orc_texture* tex, const float* u, const float* v, int32_t layer,
+                        const int32_t* mip, int32_t wrap, int32_t mag_linear, int32_t min_linear,
+                        uint32_t* out);
+void orc_intersect_triangle16(const float* clip /*[3][4][16]*/, const float* suv /*[2][16]*/,
+                              float sw, float sh, float* bary, float* ddx, float* ddy);
+void orc_setup16(const float* clip /*[3][4][16]*/, int32_t w, int32_t h, int32_t cull_mode,
+                 int32_t* X, int32_t* Y, float* Z, float* W, float* det, uint16_t* mask);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,189 @@
+// Toolchain probe: how clang lowers the reference's floating-point expression
+// shapes under the reference's flags (-O2 -march=native
+// -mprefer-vector-width=512 -ffast-math, src/SwRast/CMakeLists.txt:6).
+//
+// Each function below evaluates one expression shape of the reference on
+// 16-lane clang ext-vectors, the way the reference writes it (its own 16-lane
+// vector types, SIMD.h:16-19). The code is written here from the formulas; it
+// is not the reference's source. Compiled with those flags, the functions
+// show which divisions become VRCP14PS + Newton step and which products are
+// contracted into FMAs. The oracle states the resulting operation order
+// explicitly; tests/test_oracle_fastmath.py runs these probes on random inputs
+// and requires the oracle to match them bit for bit.
+//
+// Only tests load this library; it is test infrastructure, never product code.
+#include <cstdint>
+#include <cstring>
+#include <immintrin.h>
+
+typedef float vf __attribute__((ext_vector_type(16)));
+typedef int32_t vi __attribute__((ext_vector_type(16)));
+
+static inline vf ld(const float* p) { vf v; memcpy(&v, p, 64); return v; }
+static inline vi ldi(const int32_t* p) { vi v; memcpy(&v, p, 64); return v; }
+static inline void st(float* p, vf v) { memcpy(p, &v, 64); }
+static inline void sti(int32_t* p, vi v) { memcpy(p, &v, 64); }
+static inline vf vfma(vf a, vf b, vf c) { return __builtin_elementwise_fma(a, b, c); }
+static inline vi rne(vf x) { return (vi)_mm512_cvtps_epi32((__m512)x); }
+
+extern "C" {
+
+// 1/w (SIMD.h:474)
+void probe_recip(const float* w, float* out, int n) {
+    for (int i = 0; i < n; i += 16) st(out + i, 1.0f / ld(w + i));
+}
+// generic a/b on vectors
+void probe_div(const float* a, const float* b, float* out, int n) {
+    for (int i = 0; i < n; i += 16) st(out + i, ld(a + i) / ld(b + i));
+}
+// VRCP14PS / VRSQRT14PS (SIMD.h:294-295)
+void probe_rcp14(const float* x, float* out, int n) {
+    for (int i = 0; i < n; i += 16) st(out + i, (vf)_mm512_rcp14_ps((__m512)ld(x + i)));
+}
+void probe_rsqrt14(const float* x, float* out, int n) {
+    for (int i = 0; i < n; i += 16) st(out + i, (vf)_mm512_rsqrt14_ps((__m512)ld(x + i)));
+}
+
+// Triangle setup shape (Rasterizer.cpp:256-289 with SIMD.h:473-476):
+// perspective divide of three clip-space vertices, signed area with the
+// cull-mode flip, and RNE snapping to 1/16 px.
+// in: clip[3][4][n] (vertex, xyzw, lane); cullMode 0=None 1=FrontCCW 2=FrontCW
+// out: X[3][n], Y[3][n] snapped ints, Z[3][n], W[3][n] floats, det[n]
+void probe_setup(const float* clip, int n, float fsx, float fsy, int cullMode,
+                 int32_t* X, int32_t* Y, float* Z, float* W, float* det) {
+    for (int i = 0; i < n; i += 16) {
+        vf x[3], y[3], z[3], rw[3];
+        for (int k = 0; k < 3; k++) {
+            vf cx = ld(clip + (k * 4 + 0) * n + i), cy = ld(clip + (k * 4 + 1) * n + i);
+            vf cz = ld(clip + (k * 4 + 2) * n + i), cw = ld(clip + (k * 4 + 3) * n + i);
+            rw[k] = 1.0f / cw;
+            x[k] = cx * rw[k]; y[k] = cy * rw[k]; z[k] = cz * rw[k];
+        }
+        vf d = (x[2] - x[0]) * (y[1] - y[0]) - (x[0] - x[1]) * (y[0] - y[2]);
+        if (cullMode != 1) {
+            vi flip = (cullMode == 2) ? vi(-1) : (d < 0);
+            d = flip ? -d : d;
+        }
+        st(det + i, d);
+        for (int k = 0; k < 3; k++) {
+            sti(X + k * n + i, rne(x[k] * fsx));
+            sti(Y + k * n + i, rne(y[k] * fsy));
+            st(Z + k * n + i, z[k]);
+            st(W + k * n + i, rw[k]);
+        }
+    }
+}
+
+// Edge setup float part (Rasterizer.cpp:320-328): rcpArea from the integer
+// doubled area, depth plane and scaled 1/w terms.
+void probe_edge(const int32_t* det, const float* Z, const float* W, int n,
+                float* Z10, float* Z20, float* W0S, float* W1S, float* W2S) {
+    for (int i = 0; i < n; i += 16) {
+        vf rcpArea = 16.0f / __builtin_convertvector(ldi(det + i), vf);
+        vf z0 = ld(Z + i), z1 = ld(Z + n + i), z2 = ld(Z + 2 * n + i);
+        st(Z10 + i, (z1 - z0) * rcpArea);
+        st(Z20 + i, (z2 - z0) * rcpArea);
+        st(W0S + i, ld(W + i) * rcpArea);
+        st(W1S + i, ld(W + n + i) * rcpArea);
+        st(W2S + i, ld(W + 2 * n + i) * rcpArea);
+    }
+}
+
+// Per-fragment depth and perspective-correct barycentrics
+// (Rasterizer.h:293-319, unclipped case).
+void probe_fragment(const int32_t* e1, const int32_t* e2, const float* coef, int n,
+                    float* depth, float* bu, float* bv) {
+    // coef: Z0, Z10, Z20, W0, W0S, W1S, W2S (scalars broadcast)
+    vf Z0 = coef[0], Z10 = coef[1], Z20 = coef[2], W0 = coef[3], W0S = coef[4], W1S = coef[5], W2S = coef[6];
+    for (int i = 0; i < n; i += 16) {
+        vf u = __builtin_convertvector(ldi(e1 + i), vf);
+        vf v = __builtin_convertvector(ldi(e2 + i), vf);
+        st(depth + i, vfma(u, Z10, vfma(v, Z20, Z0)));
+        vf pw0 = vfma(u + v, -W0S, W0);
+        vf w = vfma(u, W1S, vfma(v, W2S, pw0));
+        vf r = (vf)_mm512_rcp14_ps((__m512)w);
+        r *= vfma(-w, r, 2.0f);
+        u *= W1S * r;
+        v *= W2S * r;
+        st(bu + i, u);
+        st(bv + i, v);
+    }
+}
+
+// Screen UV of a 4x4 tile in a full-screen pass (Rasterizer.h:226-237).
+void probe_tile_uv(const int32_t* px, int n, float size, float* out) {
+    float scale = 2.0f / size;
+    float center = 0.5f * scale - 1.0f;
+    for (int i = 0; i < n; i += 16) {
+        vf p = __builtin_convertvector(ldi(px + i), vf);
+        st(out + i, p * scale + center);
+    }
+}
+
+// Analytic barycentrics and screen derivatives of a triangle given clip
+// positions and a screen UV (filmic-worlds method, Shading.cpp:417-464).
+// in: v[3][4][n], suv[2][n]; out: bary[3][n], ddx[3][n], ddy[3][n]
+void probe_intersect(const float* v, const float* suv, int n, float sw, float sh,
+                     float* bary, float* ddxo, float* ddyo) {
+    for (int i = 0; i < n; i += 16) {
+        vf x0 = ld(v + 0 * n + i), y0 = ld(v + 1 * n + i), w0 = ld(v + 3 * n + i);
+        vf x1 = ld(v + 4 * n + i), y1 = ld(v + 5 * n + i), w1 = ld(v + 7 * n + i);
+        vf x2 = ld(v + 8 * n + i), y2 = ld(v + 9 * n + i), w2 = ld(v + 11 * n + i);
+        vf su = ld(suv + i), sv = ld(suv + n + i);
+        vf iw0 = 1.0f / w0, iw1 = 1.0f / w1, iw2 = 1.0f / w2;
+        vf p0x = x0 * iw0, p0y = y0 * iw0;
+        vf p1x = x1 * iw1, p1y = y1 * iw1;
+        vf p2x = x2 * iw2, p2y = y2 * iw2;
+        vf m0x = p2x - p1x, m0y = p2y - p1y, m1x = p0x - p1x, m1y = p0y - p1y;
+        vf invDet = 1.0f / (m0x * m1y - m1x * m0y);
+        vf dx0 = (p1y - p2y) * (invDet * iw0), dx1 = (p2y - p0y) * (invDet * iw1), dx2 = (p0y - p1y) * (invDet * iw2);
+        vf dy0 = (p2x - p1x) * (invDet * iw0), dy1 = (p0x - p2x) * (invDet * iw1), dy2 = (p1x - p0x) * (invDet * iw2);
+        vf dxs = dx0 + dx1 + dx2;
+        vf dys = dy0 + dy1 + dy2;
+        vf r0x = su - p0x, r0y = sv - p0y;
+        vf interpInvW = iw0 + r0x * dxs + r0y * dys;
+        vf interpW = 1.0f / interpInvW;
+        vf by = interpW * (r0x * dx1 + r0y * dy1);
+        vf bz = interpW * (r0x * dx2 + r0y * dy2);
+        vf bx = 1 - by - bz;
+        float kx = 2.0f / sw, ky = -(2.0f / sh);
+        dx0 *= kx; dx1 *= kx; dx2 *= kx;
+        dy0 *= ky; dy1 *= ky; dy2 *= ky;
+        dxs *= kx; dys *= ky;
+        vf wdx = 1.0f / (interpInvW + dxs);
+        vf wdy = 1.0f / (interpInvW + dys);
+        vf ox0 = wdx * (bx * interpInvW + dx0) - bx, ox1 = wdx * (by * interpInvW + dx1) - by, ox2 = wdx * (bz * interpInvW + dx2) - bz;
+        vf oy0 = wdy * (bx * interpInvW + dy0) - bx, oy1 = wdy * (by * interpInvW + dy1) - by, oy2 = wdy * (bz * interpInvW + dy2) - bz;
+        st(bary + i, bx); st(bary + n + i, by); st(bary + 2 * n + i, bz);
+        st(ddxo + i, ox0); st(ddxo + n + i, ox1); st(ddxo + 2 * n + i, ox2);
+        st(ddyo + i, oy0); st(ddyo + n + i, oy1); st(ddyo + 2 * n + i, oy2);
+    }
+}
+
+// Texture coordinate interpolation, UV gradients and mip level
+// (Shading.cpp:516-539, Texture.h:276-280 and SIMD.h:426).
+// in: t[3][2][n] (vertex uv), bary[3][n], ddx[3][n], ddy[3][n]
+// out: uv[2][n], mip[n]
+void probe_uvgrad(const float* t, const float* bary, const float* ddx, const float* ddy, int n,
+                  float su, float sv, float* uv, int32_t* mip) {
+    for (int i = 0; i < n; i += 16) {
+        vf t0x = ld(t + 0 * n + i), t0y = ld(t + 1 * n + i);
+        vf t10x = ld(t + 2 * n + i) - t0x, t10y = ld(t + 3 * n + i) - t0y;
+        vf t20x = ld(t + 4 * n + i) - t0x, t20y = ld(t + 5 * n + i) - t0y;
+        vf by = ld(bary + n + i), bz = ld(bary + 2 * n + i);
+        vf dxy = ld(ddx + n + i), dxz = ld(ddx + 2 * n + i);
+        vf dyy = ld(ddy + n + i), dyz = ld(ddy + 2 * n + i);
+        vf u = t0x + t10x * by + t20x * bz;
+        vf v = t0y + t10y * by + t20y * bz;
+        vf gx = t10x * dxy + t20x * dxz, gy = t10y * dxy + t20y * dxz;
+        vf gz = t10x * dyy + t20x * dyz, gw = t10y * dyy + t20y * dyz;
+        vf dx = vfma(gx, gx, gy * gy) * (su * su);
+        vf dy = vfma(gz, gz, gw * gw) * (sv * sv);
+        vf mx = __builtin_elementwise_maxnum(dx, dy);
+        vi l = (((vi)mx) - (127 << 23)) >> 23;
+        st(uv + i, u); st(uv + n + i, v);
+        sti(mip + i, l >> 1);
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+"""Scene variants for parity tests (built from the seeded generator)."""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from glimpsw_amd import scene as S
+from glimpsw_amd import textures
+
+
+def small_scene(n: int = 512, seed_offset: int = 0, spread: float = 10.0, near_frac: float = 0.0,
+                double_sided: bool = False):
+    """Config-0 style patches; `near_frac` of them straddle the camera plane
+    beside the viewer so the guard-band clipper gets work."""
+    rng = S.SplitMix64(S.SEED_BASE + 100 + seed_offset)
+    centers = rng.uniform(n * 3, -spread, spread).reshape(n, 3)
+    R = S._random_rotations(rng, n)
+    scales = rng.uniform(n, 0.5, 2.0)
+    k = int(n * near_frac)
+    if k:
+        s = scales[:k]
+        side = np.where(rng.uniform(k) < 0.5, -1.0, 1.0)
+        centers[:k, 0] = side * rng.uniform(k, 0.2, 0.9) * s
+        centers[:k, 1] = rng.uniform(k, -0.5, 0.5) * s
+        centers[:k, 2] = rng.uniform(k, -0.4, 0.4) * s - S.NEAR_Z
+    tex = textures.checker_texture(256)
+    return S.build_patches(centers, R, scales, np.zeros(n, np.uint32), [(0, double_sided, 255)], [tex])
+
+
+def ragged(scene, seed: int = 7):
+    """Random triangle counts (1..98) and vertex counts (>= highest index)."""
+    rng = S.SplitMix64(S.SEED_BASE + 500 + seed)
+    n = scene.count
+    nt = (1 + (rng.next_u64(n) % np.uint64(98))).astype(np.uint8)
+    idx = scene.indices.copy()
+    nv = np.zeros(n, dtype=np.uint8)
+    for i in range(n):
+        idx[i, :, nt[i]:] = 0
+        nv[i] = int(idx[i, :, : nt[i]].max()) + 1
+    return dataclasses.replace(scene, num_triangles=nt, num_vertices=nv, indices=np.ascontiguousarray(idx))
+
+
+def fullscreen_grid(width: int, height: int, cells: int = 7, jitter_seed: int = 3):
+    """A single plane facing the camera, tessellated into meshlet patches that
+    share vertices along their borders, covering the whole viewport: every
+    pixel must be covered exactly once (watertightness)."""
+    rng = S.SplitMix64(S.SEED_BASE + 900 + jitter_seed)
+    aspect = width / height
+    f = 1.0 / np.tan(np.radians(30.0))
+    depth = 3.0
+    half_w = depth * aspect / f * 1.3
+    half_h = depth / f * 1.3
+    nx = ny = cells
+    # shared grid of (7*nx+1) x (7*ny+1) vertices with jittered interior
+    gx = np.linspace(-half_w, half_w, 7 * nx + 1)
+    gy = np.linspace(-half_h, half_h, 7 * ny + 1)
+    X, Y = np.meshgrid(gx, gy)
+    jit = rng.uniform(X.size * 2, -0.3, 0.3).reshape(2, *X.shape)
+    dx, dy = gx[1] - gx[0], gy[1] - gy[0]
+    interior = np.zeros_like(X, dtype=bool)
+    interior[1:-1, 1:-1] = True
+    X = X + np.where(interior, jit[0] * dx, 0)
+    Y = Y + np.where(interior, jit[1] * dy, 0)
+    Z = -depth + 0.2 * np.sin(X) * np.cos(Y)
+    m = nx * ny
+    positions = np.zeros((m, 3, 64), dtype=np.float32)
+    for j in range(ny):
+        for i in range(nx):
+            k = j * nx + i
+            sl = (slice(7 * j, 7 * j + 8), slice(7 * i, 7 * i + 8))
+            positions[k, 0] = X[sl].reshape(64)
+            positions[k, 1] = Y[sl].reshape(64)
+            positions[k, 2] = Z[sl].reshape(64)
+    sc = S.build_patches(np.zeros((m, 3)), np.tile(np.eye(3), (m, 1, 1)), np.ones(m),
+                         np.zeros(m, np.uint32), [(0, True, 255)], [textures.checker_texture(64)])
+    bounds = np.zeros((m, 4), dtype=np.float32)
+    bounds[:, :3] = positions.mean(axis=2)
+    bounds[:, 3] = 1e6
+    return dataclasses.replace(sc, positions=positions, bounds=bounds)

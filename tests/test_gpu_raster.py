@@ -1,0 +1,190 @@
+"""Parity of the HIP vis-buffer path with the CPU oracle (bit-exact ids and
+depth), on seeded scenes and the edge cases the reference's draw loop has:
+ragged meshlets, guard-band clipping, back faces, several draws into one
+framebuffer, tiny and maximum-size framebuffers, empty draws."""
+import numpy as np
+import pytest
+
+import oracle
+from glimpsw_amd import scene as S
+from tests import scenes as T
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_frame(ctx, sc, view, cull=True, flags=0, draws=None):
+    import glimpsw_amd as g
+
+    ds = g.Scene(ctx, sc)
+    fb = g.Framebuffer(ctx, view.width, view.height)
+    fb.clear(0, 0.0)
+    planes = view.frustum_planes() if cull else None
+    for off, cnt in (draws or [(view.meshlet_offset, view.meshlet_count)]):
+        fb.draw(ds, view.object_to_clip, cnt, off, planes if cull else None, flags)
+    ids, depth = fb.read()
+    fb.close()
+    ds.close()
+    return ids, depth
+
+
+def _oracle_frame(sc, view, cull=True, clipping=True, guardband=True, draws=None):
+    b = oracle.SceneBinding(sc)
+    fb = oracle.OracleFramebuffer(view.width, view.height)
+    fb.clear(0, 0.0)
+    ctr = None
+    for off, cnt in (draws or [(view.meshlet_offset, view.meshlet_count)]):
+        bitmap = None
+        if cull:
+            bitmap, _ = oracle.cull(b, view.frustum_planes(), off, cnt)
+        ctr, _ = oracle.draw(fb, b, view.object_to_clip, cnt, off, bitmap, clipping=clipping, guardband=guardband)
+    return fb.ids(), fb.depth(), ctr
+
+
+def _assert_same(g_ids, g_depth, o_ids, o_depth, label):
+    print(f"[parity] {label}: covered={(o_depth > 0).sum()}", flush=True)
+    bad = (g_ids != o_ids) | (g_depth.view(np.uint32) != o_depth.view(np.uint32))
+    if bad.any():
+        ys, xs = np.nonzero(bad)
+        msg = [f"{label}: {bad.sum()} pixels differ, first at (x={xs[0]}, y={ys[0]})"]
+        for y, x in list(zip(ys, xs))[:5]:
+            msg.append(f"  ({x},{y}) gpu id={g_ids[y, x]} d={g_depth[y, x]!r} oracle id={o_ids[y, x]} d={o_depth[y, x]!r}")
+        pytest.fail("\n".join(msg))
+
+
+def test_config0_frame(gpu_ctx):
+    sc, view = S.config0()
+    g = _gpu_frame(gpu_ctx, sc, view)
+    o = _oracle_frame(sc, view)
+    assert (o[1] > 0).sum() > 10000
+    _assert_same(*g, o[0], o[1], "config0")
+
+
+def test_config0_no_cull(gpu_ctx):
+    sc, view = S.config0()
+    g = _gpu_frame(gpu_ctx, sc, view, cull=False)
+    o = _oracle_frame(sc, view, cull=False)
+    _assert_same(*g, o[0], o[1], "config0 no cull")
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_clipping_scene(gpu_ctx, seed):
+    sc = T.small_scene(384, seed_offset=seed, spread=6.0, near_frac=0.25)
+    view = S.make_view(320, 240, count=sc.count)
+    o = _oracle_frame(sc, view)
+    assert o[2]["triangles_clipped"] > 0
+    g = _gpu_frame(gpu_ctx, sc, view)
+    _assert_same(*g, o[0], o[1], f"clip seed {seed}")
+
+
+def test_clipping_disabled_and_no_guardband(gpu_ctx):
+    import glimpsw_amd as g
+
+    sc = T.small_scene(256, seed_offset=5, spread=5.0, near_frac=0.3)
+    view = S.make_view(256, 192, count=sc.count)
+    o = _oracle_frame(sc, view, clipping=False)
+    gg = _gpu_frame(gpu_ctx, sc, view, flags=g.GLIMP_DRAW_NO_CLIPPING)
+    _assert_same(*gg, o[0], o[1], "no clipping")
+    o = _oracle_frame(sc, view, guardband=False)
+    gg = _gpu_frame(gpu_ctx, sc, view, flags=g.GLIMP_DRAW_NO_GUARDBAND)
+    _assert_same(*gg, o[0], o[1], "no guardband")
+
+
+def test_ragged_meshlets(gpu_ctx):
+    sc = T.ragged(T.small_scene(512, seed_offset=11, spread=8.0, near_frac=0.1))
+    view = S.make_view(400, 300, count=sc.count)
+    o = _oracle_frame(sc, view)
+    g = _gpu_frame(gpu_ctx, sc, view)
+    _assert_same(*g, o[0], o[1], "ragged")
+
+
+def test_double_sided(gpu_ctx):
+    sc = T.small_scene(300, seed_offset=13, spread=7.0, near_frac=0.1, double_sided=True)
+    view = S.make_view(320, 240, count=sc.count)
+    o = _oracle_frame(sc, view)
+    g = _gpu_frame(gpu_ctx, sc, view)
+    _assert_same(*g, o[0], o[1], "double sided")
+
+
+def test_multiple_draws_order(gpu_ctx):
+    """Later draws come later in draw order: depth ties go to the earlier
+    draw, ids carry each draw's meshlet offset."""
+    sc = T.small_scene(400, seed_offset=17, spread=6.0)
+    # duplicate the meshlets so the second draw is coplanar with the first
+    import dataclasses
+    dup = dataclasses.replace(
+        sc, bounds=np.concatenate([sc.bounds] * 2), num_vertices=np.concatenate([sc.num_vertices] * 2),
+        num_triangles=np.concatenate([sc.num_triangles] * 2), material_id=np.concatenate([sc.material_id] * 2),
+        tangent_handedness=np.concatenate([sc.tangent_handedness] * 2),
+        positions=np.concatenate([sc.positions] * 2), texcoords=np.concatenate([sc.texcoords] * 2),
+        normal_tangents=np.concatenate([sc.normal_tangents] * 2), indices=np.concatenate([sc.indices] * 2))
+    view = S.make_view(320, 240, count=dup.count)
+    draws = [(400, 150), (0, 400), (150, 250)]
+    o = _oracle_frame(dup, view, draws=draws)
+    g = _gpu_frame(gpu_ctx, dup, view, draws=draws)
+    _assert_same(*g, o[0], o[1], "multi draw")
+    ids = g[0][g[1] > 0]
+    assert (ids // 128 >= 400).any() and (ids // 128 < 400).any()
+
+
+def test_tiny_and_max_framebuffer(gpu_ctx):
+    sc, _ = S.config0(256)
+    for w, h in [(4, 4), (8, 12), (2896, 1628)]:
+        view = S.make_view(w, h, count=sc.count)
+        o = _oracle_frame(sc, view)
+        g = _gpu_frame(gpu_ctx, sc, view)
+        _assert_same(*g, o[0], o[1], f"{w}x{h}")
+
+
+def test_watertight_grid(gpu_ctx):
+    import glimpsw_amd as g
+
+    sc = T.fullscreen_grid(256, 200)
+    view = S.make_view(256, 200, count=sc.count)
+    b = oracle.SceneBinding(sc)
+    fb = oracle.OracleFramebuffer(256, 200)
+    fb.clear(0, 0.0)
+    _, cov = oracle.draw(fb, b, view.object_to_clip, sc.count, coverage=True)
+    assert (cov == 1).all(), "oracle: grid must cover every pixel exactly once"
+    gg = _gpu_frame(gpu_ctx, sc, view, cull=False)
+    _assert_same(*gg, fb.ids(), fb.depth(), "grid")
+
+
+def test_empty_draw_and_clear_values(gpu_ctx):
+    import glimpsw_amd as g
+
+    sc, view = S.config0(64)
+    ds = g.Scene(gpu_ctx, sc)
+    fb = g.Framebuffer(gpu_ctx, 64, 48)
+    fb.clear(0xFF000000, 0.0)
+    fb.draw(ds, view.object_to_clip, 0)
+    ids, depth = fb.read()
+    assert (ids == 0xFF000000).all() and (depth == 0).all()
+    with pytest.raises(g.GlimpError):
+        fb.clear(0, -1.0)
+    with pytest.raises(g.GlimpError):
+        g.Framebuffer(gpu_ctx, 30, 32)
+    with pytest.raises(g.GlimpError):
+        fb.draw(ds, view.object_to_clip, 65)
+    fb.close()
+    ds.close()
+
+
+def test_counters_match_oracle(gpu_ctx):
+    import glimpsw_amd as g
+
+    sc = T.small_scene(384, seed_offset=2, spread=6.0, near_frac=0.25)
+    view = S.make_view(320, 240, count=sc.count)
+    o = _oracle_frame(sc, view)
+    gpu_ctx.reset_counters()
+    _gpu_frame(gpu_ctx, sc, view, flags=g.GLIMP_DRAW_COUNT_FRAGMENTS)
+    c = gpu_ctx.counters()
+    for k in ("triangles_processed", "triangles_rasterized", "triangles_clipped", "fragments_covered"):
+        assert c[k] == o[2][k], (k, c[k], o[2][k])
+
+
+@pytest.mark.slow
+def test_config1_frame(gpu_ctx):
+    sc, view = S.config1()
+    o = _oracle_frame(sc, view)
+    g = _gpu_frame(gpu_ctx, sc, view)
+    _assert_same(*g, o[0], o[1], "config1")
