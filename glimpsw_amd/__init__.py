@@ -52,7 +52,7 @@ class MaterialDesc(C.Structure):
 class DrawDesc(C.Structure):
     _fields_ = [("meshlet_offset", C.c_uint32), ("meshlet_count", C.c_uint32),
                 ("object_to_clip", C.c_float * 16), ("frustum_planes", f32p), ("flags", C.c_uint32),
-                ("pipeline", C.c_uint32)]
+                ("pipeline", C.c_uint32), ("bin_size", C.c_uint32), ("bin_capacity", C.c_uint32)]
 
 
 class CountersStruct(C.Structure):
@@ -63,6 +63,10 @@ class CountersStruct(C.Structure):
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 40), ("launches", C.c_uint64), ("total_ms", C.c_double)]
 
 
 # Every symbol include/glimp.h declares, with its ctypes signature.
@@ -86,6 +90,8 @@ _SIGNATURES = {
                                       C.POINTER(C.c_uint32), C.c_void_p]),
     "glimp_get_counters": (C.c_int, [C.c_void_p, C.POINTER(CountersStruct)]),
     "glimp_reset_counters": (C.c_int, [C.c_void_p]),
+    "glimp_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "glimp_profile_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_uint32, C.POINTER(C.c_uint32), C.c_int]),
 }
 
 _lib = None
@@ -162,6 +168,16 @@ class Context:
 
     def reset_counters(self):
         _check(self.lib.glimp_reset_counters(self.handle), "glimp_reset_counters")
+
+    def profile(self, enable: bool):
+        _check(self.lib.glimp_profile_enable(self.handle, int(enable)), "glimp_profile_enable")
+
+    def profile_read(self, reset: bool = True) -> dict:
+        """{kernel name: (launches, total ms)} since the last reset."""
+        buf = (KernelTime * 64)()
+        n = C.c_uint32()
+        _check(self.lib.glimp_profile_read(self.handle, buf, 64, C.byref(n), int(reset)), "glimp_profile_read")
+        return {buf[i].name.decode(): (int(buf[i].launches), float(buf[i].total_ms)) for i in range(min(n.value, 64))}
 
 
 class Texture:
@@ -243,7 +259,7 @@ class Framebuffer:
                "glimp_framebuffer_clear")
 
     def draw(self, scene: Scene, object_to_clip, count: int, offset: int = 0, planes=None, flags: int = 0,
-             stream=None):
+             stream=None, bin_size: int = 0, bin_capacity: int = 0):
         d = DrawDesc()
         d.meshlet_offset, d.meshlet_count = offset, count
         for i in range(16):
@@ -254,6 +270,7 @@ class Framebuffer:
             d.frustum_planes = _ptr(keep, f32p)
         d.flags = flags
         d.pipeline = GLIMP_PIPELINE_VISBUFFER
+        d.bin_size, d.bin_capacity = bin_size, bin_capacity
         _check(self.ctx.lib.glimp_draw_meshlets(self.ctx.handle, self.handle, scene.handle, C.byref(d),
                                                 _stream_ptr(stream)), "glimp_draw_meshlets")
 

@@ -64,6 +64,13 @@ struct glimp_context {
     size_t clip_cap = 0;
     uint16_t* bitmap = nullptr;
     size_t bitmap_cap = 0;
+    uint32_t* bin_counts = nullptr;
+    size_t bin_counts_cap = 0;
+    uint32_t* bin_lists = nullptr;
+    size_t bin_lists_cap = 0;
+    uint32_t* spill = nullptr;
+    size_t spill_cap = 0;
+    Profiler prof;
 
     ApproxTables approx() const { return ApproxTables{ tables, tables + 65536 }; }
 };
@@ -90,6 +97,7 @@ struct glimp_framebuffer {
     unsigned long long* keys = nullptr;
     uint32_t clear_color = 0;
     float clear_depth = 0.0f;
+    bool pending_clear = false;        // cleared, keys not yet written
     uint32_t next_seq = 0;
     std::vector<DrawEntry> draws;
     DrawEntry* d_draws = nullptr;
@@ -97,6 +105,26 @@ struct glimp_framebuffer {
 };
 
 namespace {
+
+hipError_t flush_clear(glimp_framebuffer* f, hipStream_t s) {
+    if (!f->pending_clear) return hipSuccess;
+    uint32_t db;
+    memcpy(&db, &f->clear_depth, 4);
+    hipError_t e = launch_clear(f->keys, f->width * f->height, (unsigned long long)db << 32, s);
+    if (e == hipSuccess) f->pending_clear = false;
+    return e;
+}
+
+template <typename T>
+int grow_one(T*& buf, size_t& cap, size_t need) {
+    if (need <= cap) return GLIMP_OK;
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+    cap = 0;
+    HIP_TRY(dev_alloc(&buf, need));
+    cap = need;
+    return GLIMP_OK;
+}
 
 int grow(glimp_context* c, size_t records, size_t clip, size_t bitmap) {
     if (records > c->records_cap) {
@@ -159,7 +187,8 @@ void glimp_destroy(glimp_context* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
-    void* ptrs[] = { c->tables, c->counters, c->scratch, c->visible, c->records, c->clip, c->bitmap };
+    void* ptrs[] = { c->tables, c->counters, c->scratch, c->visible, c->records, c->clip, c->bitmap,
+                     c->bin_counts, c->bin_lists, c->spill };
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete c;
@@ -343,16 +372,22 @@ int glimp_framebuffer_clear(glimp_framebuffer* f, uint32_t color, float depth, v
     uint32_t db;
     memcpy(&db, &depth, 4);
     key = (unsigned long long)db << 32;
-    HIP_TRY(launch_clear(f->keys, f->width * f->height, key, (hipStream_t)stream));
+    (void)key;
+    (void)stream;
+    // The keys are written by the first draw (each bin starts from the clear
+    // key) or, if no binned draw follows, by flush_clear before use.
     f->clear_color = color;
     f->clear_depth = depth;
     f->next_seq = 0;
     f->draws.clear();
+    f->pending_clear = true;
     return GLIMP_OK;
 }
 
 int glimp_framebuffer_keys(glimp_framebuffer* f, void** keys, size_t* bytes) {
     if (!f || !keys) return fail(GLIMP_E_INVALID, "glimp_framebuffer_keys: NULL argument");
+    HIP_TRY(hipSetDevice(f->ctx->device));
+    HIP_TRY(flush_clear(f, nullptr));
     *keys = f->keys;
     if (bytes) *bytes = (size_t)f->width * f->height * 8;
     return GLIMP_OK;
@@ -372,6 +407,7 @@ int glimp_framebuffer_read(glimp_framebuffer* f, uint32_t* ids, float* depth, in
         HIP_TRY(dev_alloc(&f->d_draws, draws.size()));
         f->d_draws_cap = draws.size();
     }
+    HIP_TRY(flush_clear(f, s));
     HIP_TRY(hipMemcpyAsync(f->d_draws, draws.data(), draws.size() * sizeof(DrawEntry), hipMemcpyHostToDevice, s));
     uint32_t* d_ids = nullptr;
     float* d_depth = nullptr;
@@ -435,7 +471,33 @@ int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, 
     p.counters = c->counters;
     p.keys = f->keys;
     p.clear_depth = f->clear_depth;
-    HIP_TRY(launch_draw_direct(p, (hipStream_t)stream));
+    hipStream_t st = (hipStream_t)stream;
+    if (d->flags & GLIMP_DRAW_DIRECT) {
+        HIP_TRY(flush_clear(f, st));
+        HIP_TRY(launch_draw_direct(p, st, &c->prof));
+    } else {
+        uint32_t bs = d->bin_size ? d->bin_size : 32;
+        if (bs != 8 && bs != 16 && bs != 32 && bs != 64)
+            return fail(GLIMP_E_INVALID, "glimp_draw_meshlets: bin_size must be 8, 16, 32 or 64");
+        uint32_t cap = d->bin_capacity ? d->bin_capacity : 1024;
+        BinParams& B = p.bins;
+        B.bin_size = bs;
+        B.bin_shift = (uint32_t)__builtin_ctz(bs);
+        B.bins_x = (f->width + bs - 1) / bs;
+        B.bins_y = (f->height + bs - 1) / bs;
+        B.bin_capacity = cap;
+        size_t nb = (size_t)B.bins_x * B.bins_y;
+        rc = grow_one(c->bin_counts, c->bin_counts_cap, nb);
+        if (rc == GLIMP_OK) rc = grow_one(c->bin_lists, c->bin_lists_cap, nb * cap);
+        if (rc == GLIMP_OK) rc = grow_one(c->spill, c->spill_cap, c->records_cap);
+        if (rc != GLIMP_OK) return rc;
+        B.bin_counts = c->bin_counts;
+        B.bin_lists = c->bin_lists;
+        B.spill = c->spill;
+        B.fresh = f->pending_clear ? 1 : 0;
+        HIP_TRY(launch_draw_binned(p, st, &c->prof));
+        f->pending_clear = false;
+    }
     f->draws.push_back(DrawEntry{ f->next_seq, d->meshlet_count, d->meshlet_offset, 0 });
     f->next_seq += d->meshlet_count;
     return GLIMP_OK;
@@ -474,6 +536,31 @@ int glimp_get_counters(glimp_context* c, glimp_counters* out) {
     out->fragments_passed = h.fragments_passed;
     out->setup_records = h.setup_records;
     out->bin_entries = h.bin_entries;
+    return GLIMP_OK;
+}
+
+int glimp_profile_enable(glimp_context* c, int enable) {
+    if (!c) return fail(GLIMP_E_INVALID, "glimp_profile_enable: NULL context");
+    c->prof.enabled = enable != 0;
+    return GLIMP_OK;
+}
+
+int glimp_profile_read(glimp_context* c, glimp_kernel_time* out, uint32_t capacity, uint32_t* count, int reset) {
+    if (!c || !count) return fail(GLIMP_E_INVALID, "glimp_profile_read: NULL argument");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(c->prof.collect());
+    uint32_t n = 0;
+    for (const auto& kv : c->prof.stats()) {
+        if (out && n < capacity) {
+            memset(&out[n], 0, sizeof(out[n]));
+            strncpy(out[n].name, kv.first.c_str(), sizeof(out[n].name) - 1);
+            out[n].launches = kv.second.launches;
+            out[n].total_ms = kv.second.total_ms;
+        }
+        n++;
+    }
+    *count = n;
+    if (reset) c->prof.reset();
     return GLIMP_OK;
 }
 

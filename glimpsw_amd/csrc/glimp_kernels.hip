@@ -357,49 +357,67 @@ __global__ void __launch_bounds__(64) k_clip(DrawParams p) {
 
 // ---------------------------------------------------------------- direct raster
 
-// Each wave takes 64 records, lays their render boxes out as a list of 4x4
-// tiles (exclusive scan of tile counts) and walks that list four tiles at a
-// time: lane = pixel, 16 lanes = one tile, so a tile's 16 keys (128 bytes)
-// are one contiguous access. Covered pixels evaluate the depth plane exactly
-// as DrawTriangle does (Rasterizer.h:293-296) and update the key with
-// atomicMax when the key can still win.
-__global__ void __launch_bounds__(256) k_raster_direct(DrawParams p) {
-    struct TriLds {
-        int32_t e0, e1, e2, a12, a20, a01, b12, b20, b01;
-        float z0, z10, z20;
-        uint32_t keylo, org, tw, start;
-    };
+// Per-triangle raster state kept in LDS while a wave or block walks the
+// triangles' 4x4 tiles.
+struct TriLds {
+    int32_t e0, e1, e2, a12, a20, a01, b12, b20, b01;
+    float z0, z10, z20;
+    uint32_t keylo, org, tw, start;
+};
+
+__device__ __forceinline__ void load_tri(const DrawParams& p, uint32_t r, int hw, int hh, TriLds& t,
+                                         uint32_t& bmin, uint32_t& bmax) {
+    const TriRecord rec = p.records[r];
+    render_bbox(rec.pos[0], rec.pos[1], rec.pos[2], hw, hh, bmin, bmax);
+    Edges e = edge_setup(rec.pos[0], rec.pos[1], rec.pos[2], hw, hh);
+    t.e0 = e.e0; t.e1 = e.e1; t.e2 = e.e2;
+    t.a12 = e.a12; t.a20 = e.a20; t.a01 = e.a01;
+    t.b12 = e.b12; t.b20 = e.b20; t.b01 = e.b01;
+    t.z0 = rec.z0; t.z10 = rec.z10; t.z20 = rec.z20;
+    t.keylo = ~rec.rank;
+}
+
+// Coverage and depth of pixel (x, y) for triangle T (DrawTriangle,
+// Rasterizer.h:281-296). Returns false when the pixel is not covered.
+__device__ __forceinline__ bool shade_pixel(const TriLds& T, uint32_t x, uint32_t y, float& depth) {
+    int32_t w0 = (int32_t)((uint32_t)T.e0 + (uint32_t)T.a12 * x + (uint32_t)T.b12 * y);
+    int32_t w1 = (int32_t)((uint32_t)T.e1 + (uint32_t)T.a20 * x + (uint32_t)T.b20 * y);
+    int32_t w2 = (int32_t)((uint32_t)T.e2 + (uint32_t)T.a01 * x + (uint32_t)T.b01 * y);
+    if ((w0 | w1 | w2) < 0) return false;
+    depth = __builtin_fmaf((float)w1, T.z10, __builtin_fmaf((float)w2, T.z20, T.z0));
+    return true;
+}
+
+// Each wave takes 64 records (all records, or the bin spill list), lays
+// their render boxes out as one list of 4x4 tiles (exclusive scan of tile
+// counts) and walks it four tiles at a time: lane = pixel, 16 lanes = one
+// tile, so a tile's 16 keys (128 bytes) are one contiguous access. Covered
+// pixels update the key with atomicMax when the key can still win.
+__global__ void __launch_bounds__(256) k_raster_direct(DrawParams p, int use_spill) {
     __shared__ TriLds tl[4][64];
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
-    const uint32_t n = min(p.scratch->num_records, p.record_capacity);
+    const uint32_t n = use_spill ? min(p.scratch->num_spill, p.record_capacity)
+                                 : min(p.scratch->num_records, p.record_capacity);
     const int hw = p.width / 2, hh = p.height / 2;
     const uint32_t W = (uint32_t)p.width;
-    const unsigned long long clearKey = (unsigned long long)fbits(p.clear_depth) << 32;
-    const bool countFrags = (p.flags & 8u) != 0;
+    const bool countFrags = (p.flags & 8u) != 0 && !use_spill;
     const uint32_t waves = gridDim.x * 4;
 
     for (uint32_t chunk = blockIdx.x * 4 + wid; chunk * 64 < n; chunk += waves) {
-        const uint32_t r = chunk * 64 + lane;
+        const uint32_t e = chunk * 64 + lane;
         uint32_t ntiles = 0;
         TriLds t;
-        if (r < n) {
-            const TriRecord rec = p.records[r];
+        if (e < n) {
+            const uint32_t r = use_spill ? p.bins.spill[e] : e;
             uint32_t bmin, bmax;
-            render_bbox(rec.pos[0], rec.pos[1], rec.pos[2], hw, hh, bmin, bmax);
-            Edges e = edge_setup(rec.pos[0], rec.pos[1], rec.pos[2], hw, hh);
-            t.e0 = e.e0; t.e1 = e.e1; t.e2 = e.e2;
-            t.a12 = e.a12; t.a20 = e.a20; t.a01 = e.a01;
-            t.b12 = e.b12; t.b20 = e.b20; t.b01 = e.b01;
-            t.z0 = rec.z0; t.z10 = rec.z10; t.z20 = rec.z20;
-            t.keylo = ~rec.rank;
+            load_tri(p, r, hw, hh, t, bmin, bmax);
             t.org = bmin;
             uint32_t tw = (uint32_t)(lo16(bmax) - lo16(bmin)) >> 2;
             uint32_t th = (uint32_t)(hi16(bmax) - hi16(bmin)) >> 2;
             t.tw = tw;
             ntiles = tw * th;
         }
-        // exclusive scan of ntiles across the wave
         uint32_t incl = ntiles;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -408,18 +426,15 @@ __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p) {
         }
         const uint32_t total = __shfl(incl, 63);
         t.start = incl - ntiles;
-        if (r < n) tl[wid][lane] = t;
+        if (e < n) tl[wid][lane] = t;
         else tl[wid][lane].start = 0xFFFFFFFFu;
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
         uint32_t covered = 0, passed = 0;
         const uint32_t sub = lane >> 4, px = lane & 3, py = (lane >> 2) & 3;
         for (uint32_t base = 0; base < total; base += 4) {
             const uint32_t j = base + sub;
             if (j < total) {
-                // last triangle whose start <= j (starts are non-decreasing)
                 uint32_t k = 0;
 #pragma unroll
                 for (uint32_t step = 32; step > 0; step >>= 1) {
@@ -431,18 +446,13 @@ __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p) {
                 const uint32_t tx = pt - ty * T.tw;
                 const uint32_t x = (uint32_t)lo16(T.org) + tx * 4 + px;
                 const uint32_t y = (uint32_t)hi16(T.org) + ty * 4 + py;
-                int32_t w0 = (int32_t)((uint32_t)T.e0 + (uint32_t)T.a12 * x + (uint32_t)T.b12 * y);
-                int32_t w1 = (int32_t)((uint32_t)T.e1 + (uint32_t)T.a20 * x + (uint32_t)T.b20 * y);
-                int32_t w2 = (int32_t)((uint32_t)T.e2 + (uint32_t)T.a01 * x + (uint32_t)T.b01 * y);
-                if ((w0 | w1 | w2) >= 0 && x < W && y < (uint32_t)p.height) {
+                float depth;
+                if (x < W && y < (uint32_t)p.height && shade_pixel(T, x, y, depth)) {
                     covered++;
-                    float depth = __builtin_fmaf((float)w1, T.z10, __builtin_fmaf((float)w2, T.z20, T.z0));
                     if (depth > p.clear_depth) {
                         unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
                         unsigned long long* dst = p.keys + pixel_offset(W, x, y);
-                        if (key > __atomic_load_n(dst, __ATOMIC_RELAXED)) {
-                            atomicMax(dst, key);
-                        }
+                        if (key > __atomic_load_n(dst, __ATOMIC_RELAXED)) atomicMax(dst, key);
                         passed++;
                     }
                 }
@@ -460,16 +470,210 @@ __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p) {
         }
         __builtin_amdgcn_wave_barrier();
     }
-    (void)clearKey;
 }
 
-__global__ void k_begin_draw(DrawScratch* s) {
-    if (threadIdx.x == 0) {
+// ---------------------------------------------------------------- binning
+
+// Screen binning of the setup records (the role of DistributeToBins and
+// BinQueue::InsertBin, Rasterizer.cpp:663-776, with fixed-capacity bins):
+// each wave expands 64 records into their (record, bin) pairs with an
+// exclusive scan and inserts the pairs in parallel, so a triangle spanning
+// many bins costs one pass of the wave, not a serial loop.
+__global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
+    __shared__ uint32_t s_start[4][64], s_org[4][64], s_bw[4][64], s_spill[4][64];
+    const uint32_t wid = threadIdx.x / kWave;
+    const uint32_t lane = lane_id();
+    const uint32_t n = min(p.scratch->num_records, p.record_capacity);
+    const int hw = p.width / 2, hh = p.height / 2;
+    const BinParams& B = p.bins;
+    const uint32_t waves = gridDim.x * 4;
+    for (uint32_t chunk = blockIdx.x * 4 + wid; chunk * 64 < n; chunk += waves) {
+        const uint32_t r = chunk * 64 + lane;
+        uint32_t nb = 0, org = 0, bw = 0;
+        if (r < n) {
+            const TriRecord rec = p.records[r];
+            uint32_t bmin, bmax;
+            render_bbox(rec.pos[0], rec.pos[1], rec.pos[2], hw, hh, bmin, bmax);
+            uint32_t bx0 = (uint32_t)lo16(bmin) >> B.bin_shift, by0 = (uint32_t)hi16(bmin) >> B.bin_shift;
+            uint32_t bx1 = (uint32_t)(lo16(bmax) - 1) >> B.bin_shift, by1 = (uint32_t)(hi16(bmax) - 1) >> B.bin_shift;
+            bw = bx1 - bx0 + 1;
+            nb = bw * (by1 - by0 + 1);
+            org = bx0 | (by0 << 16);
+        }
+        uint32_t incl = nb;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t o = __shfl_up(incl, d);
+            if ((int)lane >= d) incl += o;
+        }
+        const uint32_t total = __shfl(incl, 63);
+        s_start[wid][lane] = (r < n) ? incl - nb : 0xFFFFFFFFu;
+        s_org[wid][lane] = org;
+        s_bw[wid][lane] = bw;
+        s_spill[wid][lane] = 0;
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t base = 0; base < total; base += 64) {
+            const uint32_t j = base + lane;
+            if (j < total) {
+                uint32_t k = 0;
+#pragma unroll
+                for (uint32_t step = 32; step > 0; step >>= 1) {
+                    if (s_start[wid][k + step] <= j) k += step;
+                }
+                const uint32_t q = j - s_start[wid][k];
+                const uint32_t w = s_bw[wid][k];
+                const uint32_t dy = q / w, dx = q - dy * w;
+                const uint32_t bin = ((s_org[wid][k] >> 16) + dy) * B.bins_x + (s_org[wid][k] & 0xFFFFu) + dx;
+                const uint32_t slot = atomicAdd(&B.bin_counts[bin], 1u);
+                if (slot < B.bin_capacity) B.bin_lists[(size_t)bin * B.bin_capacity + slot] = chunk * 64 + k;
+                else s_spill[wid][k] = 1;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        const bool spilled = (r < n) && s_spill[wid][lane] != 0;
+        unsigned long long sm = __ballot(spilled);
+        if (sm) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&p.scratch->num_spill, (uint32_t)__popcll(sm));
+            base = __shfl(base, 0);
+            if (spilled) p.bins.spill[base + mbcnt(sm)] = r;
+        }
+        if (lane == 0 && total) atomicAdd(&p.counters->bin_entries, (unsigned long long)total);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// One 256-thread block per bin. The bin's keys live in LDS in the same 4x4
+// tile order as the vis-buffer; the block expands its list of records into
+// 4x4 tiles clipped to the bin (block-wide exclusive scan), 16 lanes per
+// tile, and keeps the nearest fragment with an LDS 64-bit max. The finished
+// bin is written to HBM once, one contiguous 4-pixel-high row of tiles at a
+// time. After a clear the bin starts from the clear key, otherwise from the
+// vis-buffer's current contents.
+template <int BS>
+__global__ void __launch_bounds__(256) k_bin_raster(DrawParams p) {
+    constexpr int kThreads = 256;
+    __shared__ unsigned long long keys[BS * BS];
+    __shared__ TriLds tl[kThreads];
+    __shared__ uint32_t s_wave_total[4];
+    const BinParams& B = p.bins;
+    const uint32_t bin = blockIdx.x;
+    const uint32_t bx = bin % B.bins_x, by = bin / B.bins_x;
+    const uint32_t W = (uint32_t)p.width, H = (uint32_t)p.height;
+    const uint32_t x0 = bx * BS, y0 = by * BS;
+    const uint32_t x1 = min(x0 + BS, W), y1 = min(y0 + BS, H);
+    const uint32_t n = min(B.bin_counts[bin], B.bin_capacity);
+    if (n == 0 && !B.fresh) return;  // uniform across the block
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+    const int hw = p.width / 2, hh = p.height / 2;
+    const unsigned long long clearKey = (unsigned long long)fbits(p.clear_depth) << 32;
+    const uint32_t rowElems = (x1 - x0) * 4;   // elements of one 4-row strip of tiles
+
+    for (uint32_t i = tid; i < BS * BS; i += kThreads) {
+        uint32_t strip = i / (BS * 4), rem = i % (BS * 4);
+        unsigned long long v = clearKey;
+        if (!B.fresh && rem < rowElems && y0 + strip * 4 < y1)
+            v = p.keys[(size_t)(y0 + strip * 4) * W + x0 * 4 + rem];
+        keys[i] = v;
+    }
+    __syncthreads();
+
+    const bool countFrags = (p.flags & 8u) != 0;
+    uint32_t covered = 0, passed = 0;
+    for (uint32_t base = 0; base < n; base += kThreads) {
+        const uint32_t e = base + tid;
+        uint32_t ntiles = 0;
+        TriLds t;
+        if (e < n) {
+            const uint32_t r = B.bin_lists[(size_t)bin * B.bin_capacity + e];
+            uint32_t bmin, bmax;
+            load_tri(p, r, hw, hh, t, bmin, bmax);
+            uint32_t cx0 = max((uint32_t)lo16(bmin), x0), cy0 = max((uint32_t)hi16(bmin), y0);
+            uint32_t cx1 = min((uint32_t)lo16(bmax), x1), cy1 = min((uint32_t)hi16(bmax), y1);
+            if (cx1 > cx0 && cy1 > cy0) {
+                t.org = (cx0 - x0) | ((cy0 - y0) << 16);
+                t.tw = (cx1 - cx0) >> 2;
+                ntiles = t.tw * ((cy1 - cy0) >> 2);
+            }
+        }
+        // block-wide exclusive scan of ntiles
+        uint32_t incl = ntiles;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t o = __shfl_up(incl, d);
+            if ((int)lane >= d) incl += o;
+        }
+        if (lane == 63) s_wave_total[wid] = incl;
+        __syncthreads();
+        uint32_t prev = 0, total = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < 4; w++) {
+            uint32_t v = s_wave_total[w];
+            if (w < wid) prev += v;
+            total += v;
+        }
+        t.start = prev + incl - ntiles;
+        tl[tid] = t;
+        if (e >= n) tl[tid].start = 0xFFFFFFFFu;
+        __syncthreads();
+
+        const uint32_t g = tid >> 4, px = tid & 3, py = (tid >> 2) & 3;
+        for (uint32_t it = 0; it < total; it += 16) {
+            const uint32_t j = it + g;
+            if (j < total) {
+                uint32_t k = 0;
+#pragma unroll
+                for (uint32_t step = kThreads / 2; step > 0; step >>= 1) {
+                    if (tl[k + step].start <= j) k += step;
+                }
+                const TriLds& T = tl[k];
+                const uint32_t pt = j - T.start;
+                const uint32_t ty = pt / T.tw;
+                const uint32_t tx = pt - ty * T.tw;
+                const uint32_t lx = (T.org & 0xFFFFu) + tx * 4 + px;
+                const uint32_t ly = (T.org >> 16) + ty * 4 + py;
+                float depth;
+                if (shade_pixel(T, x0 + lx, y0 + ly, depth)) {
+                    covered++;
+                    if (depth > p.clear_depth) {
+                        passed++;
+                        unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
+                        const uint32_t li = ((lx & ~3u) << 2) + (ly & ~3u) * BS + (lx & 3u) + ((ly & 3u) << 2);
+                        if (key > keys[li]) atomicMax(&keys[li], key);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    for (uint32_t i = tid; i < BS * BS; i += kThreads) {
+        uint32_t strip = i / (BS * 4), rem = i % (BS * 4);
+        if (rem < rowElems && y0 + strip * 4 < y1) p.keys[(size_t)(y0 + strip * 4) * W + x0 * 4 + rem] = keys[i];
+    }
+    if (countFrags) {
+        for (int d = 32; d > 0; d >>= 1) {
+            covered += __shfl_xor(covered, d);
+            passed += __shfl_xor(passed, d);
+        }
+        if (lane == 0) {
+            atomicAdd(&p.counters->fragments_covered, (unsigned long long)covered);
+            atomicAdd(&p.counters->fragments_passed, (unsigned long long)passed);
+        }
+    }
+    if (tid == 0 && n) atomicAdd(&p.counters->bins_rasterized, 1ull);
+}
+
+__global__ void k_begin_draw(DrawScratch* s, uint32_t* bin_counts, uint32_t num_bins) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         s->num_records = 0;
         s->num_clip = 0;
         s->num_visible = 0;
         s->overflow = 0;
+        s->num_spill = 0;
     }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < num_bins; i += gridDim.x * blockDim.x)
+        bin_counts[i] = 0;
 }
 
 __global__ void k_clear(unsigned long long* keys, uint32_t n, unsigned long long key) {
@@ -539,17 +743,60 @@ hipError_t launch_cull(const DevMeshlets& m, uint32_t first, uint32_t count, con
     return hipGetLastError();
 }
 
-hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s) {
-    hipLaunchKernelGGL(k_begin_draw, dim3(1), dim3(64), 0, s, p.scratch);
+namespace {
+
+void launch_front(const DrawParams& p, hipStream_t s, Profiler* prof, uint32_t num_bins) {
+    uint32_t zb = min((num_bins + 255) / 256, 64u);
+    hipLaunchKernelGGL(k_begin_draw, dim3(max(zb, 1u)), dim3(256), 0, s, p.scratch, p.bins.bin_counts, num_bins);
     if (p.use_cull) {
+        Profiler::Scope sc(prof, "cull", s);
         Planes5 pl;
         for (int i = 0; i < 20; i++) pl.p[i] = p.planes[i];
         hipLaunchKernelGGL(k_cull, dim3((p.meshlet_count + 255) / 256), dim3(256), 0, s, p.m, p.meshlet_offset,
                            p.meshlet_count, pl, p.cull_bitmap, &p.scratch->num_visible);
     }
-    hipLaunchKernelGGL(k_mesh_setup, dim3((p.meshlet_count + 3) / 4), dim3(256), 0, s, p);
-    if (p.clipping) hipLaunchKernelGGL(k_clip, dim3(256), dim3(64), 0, s, p);
-    hipLaunchKernelGGL(k_raster_direct, dim3(2048), dim3(256), 0, s, p);
+    {
+        Profiler::Scope sc(prof, "mesh_setup", s);
+        hipLaunchKernelGGL(k_mesh_setup, dim3((p.meshlet_count + 3) / 4), dim3(256), 0, s, p);
+    }
+    if (p.clipping) {
+        Profiler::Scope sc(prof, "clip", s);
+        hipLaunchKernelGGL(k_clip, dim3(256), dim3(64), 0, s, p);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s, Profiler* prof) {
+    launch_front(p, s, prof, 0);
+    {
+        Profiler::Scope sc(prof, "raster_direct", s);
+        hipLaunchKernelGGL(k_raster_direct, dim3(2048), dim3(256), 0, s, p, 0);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_draw_binned(const DrawParams& p, hipStream_t s, Profiler* prof) {
+    const uint32_t num_bins = p.bins.bins_x * p.bins.bins_y;
+    launch_front(p, s, prof, num_bins);
+    {
+        Profiler::Scope sc(prof, "bin_scatter", s);
+        hipLaunchKernelGGL(k_bin_scatter, dim3(1024), dim3(256), 0, s, p);
+    }
+    {
+        Profiler::Scope sc(prof, "bin_raster", s);
+        switch (p.bins.bin_size) {
+        case 8: hipLaunchKernelGGL(k_bin_raster<8>, dim3(num_bins), dim3(256), 0, s, p); break;
+        case 16: hipLaunchKernelGGL(k_bin_raster<16>, dim3(num_bins), dim3(256), 0, s, p); break;
+        case 32: hipLaunchKernelGGL(k_bin_raster<32>, dim3(num_bins), dim3(256), 0, s, p); break;
+        case 64: hipLaunchKernelGGL(k_bin_raster<64>, dim3(num_bins), dim3(256), 0, s, p); break;
+        default: return hipErrorInvalidValue;
+        }
+    }
+    {
+        Profiler::Scope sc(prof, "raster_spill", s);
+        hipLaunchKernelGGL(k_raster_direct, dim3(256), dim3(256), 0, s, p, 1);
+    }
     return hipGetLastError();
 }
 

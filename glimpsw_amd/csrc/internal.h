@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "device_math.h"
+#include "profiler.h"
 
 namespace glimp {
 
@@ -99,7 +100,25 @@ struct DrawScratch {
     uint32_t num_records;
     uint32_t num_clip;
     uint32_t num_visible;
-    uint32_t overflow;
+    uint32_t overflow;             // a record or clip queue was full
+    uint32_t num_spill;            // records that did not fit a bin
+    uint32_t pad[3];
+};
+
+// Screen binning parameters of a draw. Bins are bin_size x bin_size pixel
+// squares (a multiple of the 4x4 tile); each bin holds up to bin_capacity
+// record indices. A record that finds a bin full is appended to the spill
+// list and rasterized with global atomics after the bins; the depth-test max
+// is idempotent, so fragments rasterized twice leave the same result.
+struct BinParams {
+    uint32_t bin_size;             // 8, 16, 32 or 64
+    uint32_t bin_shift;            // log2(bin_size)
+    uint32_t bins_x, bins_y;
+    uint32_t bin_capacity;
+    uint32_t* bin_counts;          // [bins_x * bins_y]
+    uint32_t* bin_lists;           // [bins_x * bins_y][bin_capacity]
+    uint32_t* spill;               // [record_capacity]
+    int fresh;                     // first draw after a clear: start bins from the clear key
 };
 
 struct DrawParams {
@@ -126,10 +145,12 @@ struct DrawParams {
     Counters* counters;
     unsigned long long* keys;      // vis-buffer, 4x4 tiles
     float clear_depth;
+    BinParams bins;
 };
 
 // Launchers (glimp_kernels.hip).
-hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s);
+hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s, Profiler* prof);
+hipError_t launch_draw_binned(const DrawParams& p, hipStream_t s, Profiler* prof);
 hipError_t launch_clear(unsigned long long* keys, uint32_t n, unsigned long long key, hipStream_t s);
 hipError_t launch_decode(const unsigned long long* keys, int width, int height, unsigned long long clear_key,
                          uint32_t clear_color, const DrawEntry* draws, uint32_t num_draws, uint32_t* ids,

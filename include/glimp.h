@@ -127,7 +127,7 @@ int glimp_framebuffer_keys(glimp_framebuffer* fb, void** keys, size_t* bytes);
 
 #define GLIMP_DRAW_NO_CLIPPING 0x1u   /* Rasterizer::EnableClipping = false (Rasterizer.h:207) */
 #define GLIMP_DRAW_NO_GUARDBAND 0x2u  /* Rasterizer::EnableGuardband = false (Rasterizer.h:208) */
-#define GLIMP_DRAW_DIRECT 0x4u        /* per-fragment global atomics instead of tile bins */
+#define GLIMP_DRAW_DIRECT 0x4u        /* per-fragment global atomics instead of screen bins */
 #define GLIMP_DRAW_COUNT_FRAGMENTS 0x8u /* maintain the fragment counters (slower) */
 
 /* Pre-instantiated shader pipelines: the device counterpart of a dispatch
@@ -141,6 +141,8 @@ typedef struct glimp_draw_desc {
     const float* frustum_planes;       /* host [5][4] normalised planes, or NULL: no culling */
     uint32_t flags;                    /* GLIMP_DRAW_* */
     uint32_t pipeline;                 /* GLIMP_PIPELINE_* */
+    uint32_t bin_size;                 /* screen bin side in pixels: 8, 16, 32 or 64; 0 = default */
+    uint32_t bin_capacity;             /* records per bin before spilling, 0 = default */
 } glimp_draw_desc;
 
 /* Rasterizer::DrawMeshlets(fb, count, {VisBufferShader, &ctx})
@@ -169,6 +171,19 @@ typedef struct glimp_counters {
 } glimp_counters;
 int glimp_get_counters(glimp_context* ctx, glimp_counters* out); /* synchronous */
 int glimp_reset_counters(glimp_context* ctx);
+
+/* Kernel timing: with profiling enabled every kernel of a draw or resolve is
+ * bracketed by HIP events on its stream; glimp_profile_read waits for them
+ * and returns per-kernel launch counts and summed milliseconds (sorted by
+ * name). The device side of the reference's DrawTime/ResolveTime counters
+ * (Rasterizer.h:372-373, SWR_PERF_BEGIN/END Rasterizer.h:398-399). */
+typedef struct glimp_kernel_time {
+    char name[40];
+    uint64_t launches;
+    double total_ms;
+} glimp_kernel_time;
+int glimp_profile_enable(glimp_context* ctx, int enable);
+int glimp_profile_read(glimp_context* ctx, glimp_kernel_time* out, uint32_t capacity, uint32_t* count, int reset);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,7 @@ from tests import scenes as T
 pytestmark = pytest.mark.gpu
 
 
-def _gpu_frame(ctx, sc, view, cull=True, flags=0, draws=None):
+def _gpu_frame(ctx, sc, view, cull=True, flags=0, draws=None, bin_size=0, bin_capacity=0):
     import glimpsw_amd as g
 
     ds = g.Scene(ctx, sc)
@@ -20,7 +20,8 @@ def _gpu_frame(ctx, sc, view, cull=True, flags=0, draws=None):
     fb.clear(0, 0.0)
     planes = view.frustum_planes() if cull else None
     for off, cnt in (draws or [(view.meshlet_offset, view.meshlet_count)]):
-        fb.draw(ds, view.object_to_clip, cnt, off, planes if cull else None, flags)
+        fb.draw(ds, view.object_to_clip, cnt, off, planes if cull else None, flags, bin_size=bin_size,
+                bin_capacity=bin_capacity)
     ids, depth = fb.read()
     fb.close()
     ds.close()
@@ -188,3 +189,52 @@ def test_config1_frame(gpu_ctx):
     o = _oracle_frame(sc, view)
     g = _gpu_frame(gpu_ctx, sc, view)
     _assert_same(*g, o[0], o[1], "config1")
+
+
+@pytest.mark.parametrize("bin_size", [8, 16, 32, 64])
+def test_bin_sizes(gpu_ctx, bin_size):
+    sc = T.small_scene(384, seed_offset=1, spread=6.0, near_frac=0.25)
+    view = S.make_view(324, 244, count=sc.count)   # partial bins at the right/bottom edges
+    o = _oracle_frame(sc, view)
+    g = _gpu_frame(gpu_ctx, sc, view, bin_size=bin_size)
+    _assert_same(*g, o[0], o[1], f"bin {bin_size}")
+
+
+@pytest.mark.parametrize("cap", [1, 3, 16])
+def test_bin_spill_bit_identical(gpu_ctx, cap):
+    """Bins far too small: most records spill to the global-atomic pass and
+    the vis-buffer must not change."""
+    sc, view = S.config0()
+    o = _oracle_frame(sc, view)
+    g = _gpu_frame(gpu_ctx, sc, view, bin_size=16, bin_capacity=cap)
+    _assert_same(*g, o[0], o[1], f"capacity {cap}")
+
+
+def test_direct_path(gpu_ctx):
+    import glimpsw_amd as g
+
+    sc = T.small_scene(384, seed_offset=0, spread=6.0, near_frac=0.25)
+    view = S.make_view(320, 240, count=sc.count)
+    o = _oracle_frame(sc, view)
+    gg = _gpu_frame(gpu_ctx, sc, view, flags=g.GLIMP_DRAW_DIRECT)
+    _assert_same(*gg, o[0], o[1], "direct")
+
+
+def test_mixed_direct_and_binned_draws(gpu_ctx):
+    """A binned draw after a direct one starts its bins from the keys
+    already in the vis-buffer."""
+    import glimpsw_amd as g
+
+    sc = T.small_scene(400, seed_offset=21, spread=6.0)
+    view = S.make_view(320, 240, count=sc.count)
+    draws = [(0, 200), (200, 200)]
+    o = _oracle_frame(sc, view, draws=draws)
+    ds = g.Scene(gpu_ctx, sc)
+    fb = g.Framebuffer(gpu_ctx, 320, 240)
+    fb.clear(0, 0.0)
+    fb.draw(ds, view.object_to_clip, 200, 0, view.frustum_planes(), g.GLIMP_DRAW_DIRECT)
+    fb.draw(ds, view.object_to_clip, 200, 200, view.frustum_planes(), 0)
+    ids, depth = fb.read()
+    _assert_same(ids, depth, o[0], o[1], "mixed")
+    fb.close()
+    ds.close()
