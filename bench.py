@@ -86,7 +86,22 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--direct", action="store_true", help="global-atomic raster instead of tile bins")
+    ap.add_argument("--bin-size", type=int, default=0, help="screen bin side (8/16/32/64), 0 = library default")
+    ap.add_argument("--bin-capacity", type=int, default=0, help="records per bin, 0 = library default")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy of BYTES bytes 10 times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.empty(args.calibrate_copy, dtype=torch.uint8, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": args.calibrate_copy, "launches": 10}))
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; the path has no multi-GPU mode "
                                    "(views or meshlets would shard without exchange except a min-reduce "
@@ -110,7 +125,8 @@ def main() -> int:
 
     def frame():
         fb.clear(0, 0.0, stream)
-        fb.draw(ds, view.object_to_clip, view.meshlet_count, 0, planes, flags, stream)
+        fb.draw(ds, view.object_to_clip, view.meshlet_count, 0, planes, flags, stream, args.bin_size,
+                args.bin_capacity)
 
     for _ in range(args.warmup):
         frame()
@@ -131,7 +147,8 @@ def main() -> int:
     # counted quantities of one frame (separate, untimed run)
     ctx.reset_counters()
     fb.clear(0, 0.0, stream)
-    fb.draw(ds, view.object_to_clip, view.meshlet_count, 0, planes, flags | g.GLIMP_DRAW_COUNT_FRAGMENTS, stream)
+    fb.draw(ds, view.object_to_clip, view.meshlet_count, 0, planes, flags | g.GLIMP_DRAW_COUNTERS, stream,
+            args.bin_size, args.bin_capacity)
     c = ctx.counters()
     _, visible = ds.cull(planes, 0, view.meshlet_count)
     stats = {

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "_lib", "libglimp.so")
 GLIMP_DRAW_NO_CLIPPING = 0x1
 GLIMP_DRAW_NO_GUARDBAND = 0x2
 GLIMP_DRAW_DIRECT = 0x4
-GLIMP_DRAW_COUNT_FRAGMENTS = 0x8
+GLIMP_DRAW_COUNTERS = 0x8
 GLIMP_PIPELINE_VISBUFFER = 0
 MAX_RENDER_SIZE = 2896
 

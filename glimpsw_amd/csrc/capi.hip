@@ -70,6 +70,8 @@ struct glimp_context {
     size_t bin_lists_cap = 0;
     uint32_t* spill = nullptr;
     size_t spill_cap = 0;
+    uint32_t* meshlet_counts = nullptr;
+    size_t meshlet_counts_cap = 0;
     Profiler prof;
 
     ApproxTables approx() const { return ApproxTables{ tables, tables + 65536 }; }
@@ -188,7 +190,7 @@ void glimp_destroy(glimp_context* c) {
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
     void* ptrs[] = { c->tables, c->counters, c->scratch, c->visible, c->records, c->clip, c->bitmap,
-                     c->bin_counts, c->bin_lists, c->spill };
+                     c->bin_counts, c->bin_lists, c->spill, c->meshlet_counts };
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete c;
@@ -441,6 +443,7 @@ int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, 
     size_t cnt = d->meshlet_count;
     // worst case: every triangle clipped, each into 7 fan triangles
     int rc = grow(c, cnt * 128 * 8, cnt * 128, (cnt + 15) / 16);
+    if (rc == GLIMP_OK) rc = grow_one(c->meshlet_counts, c->meshlet_counts_cap, cnt);
     if (rc != GLIMP_OK) return rc;
 
     DrawParams p;
@@ -464,7 +467,9 @@ int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, 
     p.tables = c->approx();
     p.cull_bitmap = c->bitmap;
     p.records = c->records;
-    p.record_capacity = (uint32_t)std::min<size_t>(c->records_cap, 0xFFFFFFFFu);
+    p.meshlet_counts = c->meshlet_counts;
+    p.clip_record_base = d->meshlet_count * 128;
+    p.clip_record_capacity = (uint32_t)std::min<size_t>(c->records_cap - (size_t)d->meshlet_count * 128, 0x7FFFFFFFu);
     p.clip_queue = c->clip;
     p.clip_capacity = (uint32_t)std::min<size_t>(c->clip_cap, 0xFFFFFFFFu);
     p.scratch = c->scratch;
@@ -476,7 +481,7 @@ int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, 
         HIP_TRY(flush_clear(f, st));
         HIP_TRY(launch_draw_direct(p, st, &c->prof));
     } else {
-        uint32_t bs = d->bin_size ? d->bin_size : 32;
+        uint32_t bs = d->bin_size ? d->bin_size : 16;
         if (bs != 8 && bs != 16 && bs != 32 && bs != 64)
             return fail(GLIMP_E_INVALID, "glimp_draw_meshlets: bin_size must be 8, 16, 32 or 64");
         uint32_t cap = d->bin_capacity ? d->bin_capacity : 1024;

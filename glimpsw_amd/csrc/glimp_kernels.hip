@@ -168,6 +168,13 @@ __global__ void k_cull(DevMeshlets m, uint32_t first, uint32_t count, Planes5 pl
 
 // ---------------------------------------------------------------- mesh + setup
 
+// Record layout of a draw: meshlet w (within the draw) owns record slots
+// [w*128, w*128 + meshlet_counts[w]); clipped sub-triangles are appended
+// after all meshlet slots. Setup therefore needs no global counter: a
+// wave compacts its survivors with a ballot and writes them into its own
+// slots.
+__device__ __forceinline__ bool counting(const DrawParams& p) { return (p.flags & 8u) != 0; }
+
 __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
     __shared__ float lds_pos[4][4][64];
     const uint32_t wid = threadIdx.x / kWave;
@@ -194,13 +201,14 @@ __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
         for (int c = 0; c < 4; c++) lds_pos[wid][c][lane] = xform(p.clip, c, px, py, pz);
     }
     __syncthreads();
-    if (visible && lane == 0) {
-        atomicAdd(&p.counters->triangles_processed, (unsigned long long)nt);
-    }
+    if (!have) return;
     const uint32_t seq = p.seq_base + w;
     const uint8_t* I = p.m.indices + (size_t)mi * 384;
+    TriRecord* out = p.records + (size_t)w * 128;
+    uint32_t kept = 0, clipped = 0;
 
     for (uint32_t round = 0; round < 2; round++) {
+        if (round * 64 >= nt) break;
         const uint32_t t = round * 64 + lane;
         bool valid = t < nt;
         bool accept = false, nontrivial = false;
@@ -240,28 +248,14 @@ __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
             rec.flags = (uint32_t)fs_id;
         }
         unsigned long long km = __ballot(keep);
-        if (km) {
-            uint32_t n = __popcll(km);
-            uint32_t base = 0;
-            if (lane == 0) {
-                base = atomicAdd(&p.scratch->num_records, n);
-                atomicAdd(&p.counters->triangles_rasterized, (unsigned long long)n);
-            }
-            base = __shfl(base, 0);
-            if (keep) {
-                uint32_t slot = base + mbcnt(km);
-                if (slot < p.record_capacity) p.records[slot] = rec;
-                else p.scratch->overflow = 1;
-            }
-        }
+        if (keep) out[kept + mbcnt(km)] = rec;
+        kept += (uint32_t)__popcll(km);
         unsigned long long cm = __ballot(nontrivial);
         if (cm) {
             uint32_t n = __popcll(cm);
+            clipped += n;
             uint32_t base = 0;
-            if (lane == 0) {
-                base = atomicAdd(&p.scratch->num_clip, n);
-                atomicAdd(&p.counters->triangles_clipped, (unsigned long long)n);
-            }
+            if (lane == 0) base = atomicAdd(&p.scratch->num_clip, n);
             base = __shfl(base, 0);
             if (nontrivial) {
                 uint32_t slot = base + mbcnt(cm);
@@ -281,78 +275,171 @@ __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
             }
         }
     }
+    if (lane == 0) {
+        p.meshlet_counts[w] = kept;
+        if (counting(p) && visible) {
+            atomicAdd(&p.counters->triangles_processed, (unsigned long long)nt);
+            atomicAdd(&p.counters->triangles_rasterized, (unsigned long long)kept);
+            if (clipped) atomicAdd(&p.counters->triangles_clipped, (unsigned long long)clipped);
+        }
+    }
 }
 
 // ---------------------------------------------------------------- clipper
 
-// Clipper::ClipTriangles for one triangle (Rasterizer.cpp:398-478):
-// Sutherland-Hodgman over the planes in the triangle's combined outcode,
-// in plane order L, R, T, B, N, F, distances `+-c + w*scale` with the
-// guard-band scale for x/y; then a fan over the result. Each fan triangle
-// goes through setup with culling (FlushPacket, Rasterizer.cpp:479-491).
-__global__ void __launch_bounds__(64) k_clip(DrawParams p) {
+// Clipper::ClipTriangles for one triangle (Rasterizer.cpp:398-478), one wave
+// per queued triangle with the polygon spread over the lanes (lane i holds
+// vertex i, in polygon order). For each plane set in the triangle's combined
+// outcode, in plane order L, R, T, B, N, F: every lane computes its vertex's
+// distance `+-c + w*scale` (guard-band scale for x/y), sees the next
+// vertex's through a shuffle, emits itself when inside and the edge
+// intersection `lerp(a, b, da/(da-db))` when the edge crosses; an exclusive
+// scan of the emit counts gives each output's position and a permute moves
+// the outputs to their lanes. The values and their order are those of the
+// reference's sequential loop. Generated vertices are limited to 16 per
+// triangle, the room the reference's vertex store has. The fan triangles
+// (v0, v[f+1], v[f+2]) then go through setup with culling (FlushPacket,
+// Rasterizer.cpp:479-491) and are appended after the meshlet record slots.
+__device__ __forceinline__ float lane_get(float v, uint32_t src) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute((int)(src << 2), __float_as_int(v)));
+}
+
+__global__ void __launch_bounds__(256) k_clip(DrawParams p) {
     const uint32_t n = min(p.scratch->num_clip, p.clip_capacity);
     const int hw = p.width / 2, hh = p.height / 2;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        const ClipEntry ce = p.clip_queue[q];
-        // vertex store: the 3 inputs + up to 16 generated vertices, the
-        // room the reference's Clipper::Vertices leaves per triangle
-        float V[4][19];
-        uint8_t idx[32], out[32];
+    const uint32_t lane = lane_id();
+    const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+    for (uint32_t q = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; q < n; q += waves) {
+        const ClipEntry& ce = p.clip_queue[q];
+        float V[4];
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-#pragma unroll
-            for (int c = 0; c < 4; c++) V[c][k] = ce.v[k][c];
-            idx[k] = (uint8_t)k;
-        }
-        uint32_t vertCount = 3, next = 3;
+        for (int c = 0; c < 4; c++) V[c] = lane < 3 ? ce.v[lane][c] : 0.0f;
+        uint32_t vc = 3, generated = 0;
         const uint32_t codes = ce.info & 0x3Fu;
         for (uint32_t plane = 0; plane < 6; plane++) {
             if (!((codes >> plane) & 1u)) continue;
-            float scale = plane < 2 ? p.guard_x : (plane < 4 ? p.guard_y : 1.0f);
-            uint32_t comp = plane >> 1;
-            bool neg = plane & 1u;
-            uint32_t outCount = 0;
-            for (uint32_t vi = 0; vi < vertCount; vi++) {
-                uint32_t ia = idx[vi], ib = idx[(vi + 1) % vertCount];
-                float ca = V[comp][ia], cb = V[comp][ib];
-                if (neg) { ca = -ca; cb = -cb; }
-                float da = __builtin_fmaf(V[3][ia], scale, ca);
-                float db = __builtin_fmaf(V[3][ib], scale, cb);
-                if (da >= 0.0f) out[outCount++] = (uint8_t)ia;
-                if ((da >= 0.0f) != (db >= 0.0f) && next < 19) {
-                    float t = da / (da - db);
+            const float scale = plane < 2 ? p.guard_x : (plane < 4 ? p.guard_y : 1.0f);
+            const uint32_t comp = plane >> 1;
+            const bool neg = plane & 1u;
+            const bool active = lane < vc;
+            const uint32_t nxt = (lane + 1 < vc) ? lane + 1 : 0;
+            float Vn[4];
 #pragma unroll
-                    for (int c = 0; c < 4; c++)
-                        V[c][next] = __builtin_fmaf(V[c][ib], t, __builtin_fmaf(-t, V[c][ia], V[c][ia]));
-                    out[outCount++] = (uint8_t)next++;
-                }
+            for (int c = 0; c < 4; c++) Vn[c] = lane_get(V[c], nxt);
+            float ca = comp == 0 ? V[0] : (comp == 1 ? V[1] : V[2]);
+            float cb = comp == 0 ? Vn[0] : (comp == 1 ? Vn[1] : Vn[2]);
+            if (neg) { ca = -ca; cb = -cb; }
+            const float da = __builtin_fmaf(V[3], scale, ca);
+            const float db = __builtin_fmaf(Vn[3], scale, cb);
+            const bool in = active && da >= 0.0f;
+            const bool cross = active && ((da >= 0.0f) != (db >= 0.0f));
+            // crossings are created in polygon order while room is left
+            const unsigned long long xm = __ballot(cross);
+            const uint32_t xrank = mbcnt(xm);
+            const bool make = cross && (generated + xrank < 16);
+            const uint32_t made = min((uint32_t)__popcll(xm), 16u - generated);
+            const uint32_t cnt = (in ? 1u : 0u) + (make ? 1u : 0u);
+            uint32_t incl = cnt;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                uint32_t o = __shfl_up(incl, d);
+                if ((int)lane >= d) incl += o;
             }
-            if (vertCount < 3) break;
-            vertCount = outCount;
-            for (uint32_t k = 0; k < outCount; k++) idx[k] = out[k];
+            const uint32_t outCount = __shfl(incl, 63);
+            const uint32_t pos = incl - cnt;
+            float I[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+            if (make) {
+                const float t = da / (da - db);
+#pragma unroll
+                for (int c = 0; c < 4; c++) I[c] = __builtin_fmaf(Vn[c], t, __builtin_fmaf(-t, V[c], V[c]));
+            }
+            if (vc < 3) break;
+            // output lane j pulls from source lane src(j) = #{i : incl_i <= j}
+            uint32_t src = 0;
+            for (uint32_t k = 0; k < vc; k++) src += (__shfl(incl, (int)k) <= lane) ? 1u : 0u;
+            src = min(src, 63u);
+            const uint32_t srcPos = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)pos);
+            const int srcIn = __builtin_amdgcn_ds_bpermute((int)(src << 2), in ? 1 : 0);
+            const bool isVert = srcIn && lane == srcPos;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const float a = lane_get(V[c], src), b = lane_get(I[c], src);
+                V[c] = isVert ? a : b;
+            }
+            generated += made;
+            vc = outCount;
         }
-        if (vertCount < 3) continue;
+        if (vc < 3) continue;
         const int cull_mode = (ce.info >> 8) & 3;
-        uint32_t kept = 0;
-        for (uint32_t fan = 0; fan + 2 < vertCount && fan < 8; fan++) {
-            uint32_t ii[3] = { idx[0], idx[fan + 1], idx[fan + 2] };
-            float v[3][4];
+        // fan triangle f = (v0, v[f+1], v[f+2]) on lane f
+        const uint32_t f = lane;
+        float tri[3][4];
 #pragma unroll
-            for (int k = 0; k < 3; k++)
-#pragma unroll
-                for (int c = 0; c < 4; c++) v[k][c] = V[c][ii[k]];
-            TriRecord rec;
-            if (!setup_triangle(p.tables, v, hw, hh, cull_mode, rec)) continue;
-            rec.rank = ce.rank_base | fan;
-            rec.flags = (ce.info >> 10) & 3u;
-            uint32_t slot = atomicAdd(&p.scratch->num_records, 1u);
-            if (slot < p.record_capacity) p.records[slot] = rec;
-            else p.scratch->overflow = 1;
-            kept++;
+        for (int c = 0; c < 4; c++) {
+            tri[0][c] = lane_get(V[c], 0);
+            tri[1][c] = lane_get(V[c], min(f + 1, 63u));
+            tri[2][c] = lane_get(V[c], min(f + 2, 63u));
         }
-        if (kept) atomicAdd(&p.counters->triangles_rasterized, (unsigned long long)kept);
+        TriRecord rec;
+        bool keep = false;
+        if (f + 2 < vc && f < 8) {
+            keep = setup_triangle(p.tables, tri, hw, hh, cull_mode, rec);
+            rec.rank = ce.rank_base | f;
+            rec.flags = (ce.info >> 10) & 3u;
+        }
+        const unsigned long long km = __ballot(keep);
+        if (km) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&p.scratch->num_records, (uint32_t)__popcll(km));
+            base = __shfl(base, 0);
+            if (keep) {
+                const uint32_t slot = base + mbcnt(km);
+                if (slot < p.clip_record_capacity) p.records[p.clip_record_base + slot] = rec;
+                else p.scratch->overflow = 1;
+            }
+            if (lane == 0 && counting(p)) atomicAdd(&p.counters->triangles_rasterized, (unsigned long long)__popcll(km));
+        }
     }
+}
+
+// ---------------------------------------------------------------- record chunks
+
+// Chunks of up to 64 records for a wave: chunk c < 2*meshlet_count is half
+// c&1 of meshlet c>>1's slots; later chunks cover the clipped records; in
+// spill mode chunk c covers spill-list entries [64c, 64c+64).
+struct Chunk {
+    uint32_t base, count;
+    const uint32_t* list;      // non-null: record index = list[base + i]
+};
+
+__device__ __forceinline__ uint32_t num_chunks(const DrawParams& p, int spill) {
+    if (spill) return (min(p.scratch->num_spill, p.clip_record_base + p.clip_record_capacity) + 63) / 64;
+    uint32_t nclip = min(p.scratch->num_records, p.clip_record_capacity);
+    return 2 * p.meshlet_count + (nclip + 63) / 64;
+}
+
+__device__ __forceinline__ Chunk get_chunk(const DrawParams& p, int spill, uint32_t c) {
+    Chunk k;
+    k.list = nullptr;
+    if (spill) {
+        uint32_t n = min(p.scratch->num_spill, p.clip_record_base + p.clip_record_capacity);
+        k.base = c * 64;
+        k.count = min(64u, n - k.base);
+        k.list = p.bins.spill;
+        return k;
+    }
+    if (c < 2 * p.meshlet_count) {
+        uint32_t w = c >> 1, half = c & 1;
+        uint32_t cnt = p.meshlet_counts[w];
+        k.base = w * 128 + half * 64;
+        k.count = cnt > half * 64 ? min(64u, cnt - half * 64) : 0;
+        return k;
+    }
+    uint32_t nclip = min(p.scratch->num_records, p.clip_record_capacity);
+    uint32_t off = (c - 2 * p.meshlet_count) * 64;
+    k.base = p.clip_record_base + off;
+    k.count = min(64u, nclip - off);
+    return k;
 }
 
 // ---------------------------------------------------------------- direct raster
@@ -388,28 +475,29 @@ __device__ __forceinline__ bool shade_pixel(const TriLds& T, uint32_t x, uint32_
     return true;
 }
 
-// Each wave takes 64 records (all records, or the bin spill list), lays
-// their render boxes out as one list of 4x4 tiles (exclusive scan of tile
-// counts) and walks it four tiles at a time: lane = pixel, 16 lanes = one
-// tile, so a tile's 16 keys (128 bytes) are one contiguous access. Covered
-// pixels update the key with atomicMax when the key can still win.
+// Each wave takes a chunk of up to 64 records (meshlet slots, clipped
+// records, or bin spill entries), lays their render boxes out as one list of
+// 4x4 tiles (exclusive scan of tile counts) and walks it four tiles at a
+// time: lane = pixel, 16 lanes = one tile, so a tile's 16 keys (128 bytes)
+// are one contiguous access. Covered pixels update the key with atomicMax
+// when the key can still win.
 __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p, int use_spill) {
     __shared__ TriLds tl[4][64];
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
-    const uint32_t n = use_spill ? min(p.scratch->num_spill, p.record_capacity)
-                                 : min(p.scratch->num_records, p.record_capacity);
+    const uint32_t nchunks = num_chunks(p, use_spill);
     const int hw = p.width / 2, hh = p.height / 2;
     const uint32_t W = (uint32_t)p.width;
-    const bool countFrags = (p.flags & 8u) != 0 && !use_spill;
+    const bool countFrags = counting(p) && !use_spill;
     const uint32_t waves = gridDim.x * 4;
 
-    for (uint32_t chunk = blockIdx.x * 4 + wid; chunk * 64 < n; chunk += waves) {
-        const uint32_t e = chunk * 64 + lane;
+    for (uint32_t c = blockIdx.x * 4 + wid; c < nchunks; c += waves) {
+        const Chunk ck = get_chunk(p, use_spill, c);
+        if (ck.count == 0) continue;
         uint32_t ntiles = 0;
         TriLds t;
-        if (e < n) {
-            const uint32_t r = use_spill ? p.bins.spill[e] : e;
+        if (lane < ck.count) {
+            const uint32_t r = ck.list ? ck.list[ck.base + lane] : ck.base + lane;
             uint32_t bmin, bmax;
             load_tri(p, r, hw, hh, t, bmin, bmax);
             t.org = bmin;
@@ -426,7 +514,7 @@ __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p, int use_spi
         }
         const uint32_t total = __shfl(incl, 63);
         t.start = incl - ntiles;
-        if (e < n) tl[wid][lane] = t;
+        if (lane < ck.count) tl[wid][lane] = t;
         else tl[wid][lane].start = 0xFFFFFFFFu;
         __builtin_amdgcn_wave_barrier();
 
@@ -476,21 +564,25 @@ __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p, int use_spi
 
 // Screen binning of the setup records (the role of DistributeToBins and
 // BinQueue::InsertBin, Rasterizer.cpp:663-776, with fixed-capacity bins):
-// each wave expands 64 records into their (record, bin) pairs with an
-// exclusive scan and inserts the pairs in parallel, so a triangle spanning
-// many bins costs one pass of the wave, not a serial loop.
+// each wave expands a chunk of records into their (record, bin) pairs with
+// an exclusive scan and inserts the pairs in parallel, so a triangle
+// spanning many bins costs one pass of the wave, not a serial loop.
 __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
     __shared__ uint32_t s_start[4][64], s_org[4][64], s_bw[4][64], s_spill[4][64];
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
-    const uint32_t n = min(p.scratch->num_records, p.record_capacity);
+    const uint32_t nchunks = num_chunks(p, 0);
     const int hw = p.width / 2, hh = p.height / 2;
     const BinParams& B = p.bins;
     const uint32_t waves = gridDim.x * 4;
-    for (uint32_t chunk = blockIdx.x * 4 + wid; chunk * 64 < n; chunk += waves) {
-        const uint32_t r = chunk * 64 + lane;
+    unsigned long long entries = 0;
+    for (uint32_t c = blockIdx.x * 4 + wid; c < nchunks; c += waves) {
+        const Chunk ck = get_chunk(p, 0, c);
+        if (ck.count == 0) continue;
+        const bool valid = lane < ck.count;
+        const uint32_t r = ck.base + lane;
         uint32_t nb = 0, org = 0, bw = 0;
-        if (r < n) {
+        if (valid) {
             const TriRecord rec = p.records[r];
             uint32_t bmin, bmax;
             render_bbox(rec.pos[0], rec.pos[1], rec.pos[2], hw, hh, bmin, bmax);
@@ -507,15 +599,17 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
             if ((int)lane >= d) incl += o;
         }
         const uint32_t total = __shfl(incl, 63);
-        s_start[wid][lane] = (r < n) ? incl - nb : 0xFFFFFFFFu;
+        entries += total;
+        s_start[wid][lane] = valid ? incl - nb : 0xFFFFFFFFu;
         s_org[wid][lane] = org;
         s_bw[wid][lane] = bw;
         s_spill[wid][lane] = 0;
         __builtin_amdgcn_wave_barrier();
         for (uint32_t base = 0; base < total; base += 64) {
             const uint32_t j = base + lane;
-            if (j < total) {
-                uint32_t k = 0;
+            const bool pv = j < total;
+            uint32_t k = 0, bin = 0xFFFFFFFFu;
+            if (pv) {
 #pragma unroll
                 for (uint32_t step = 32; step > 0; step >>= 1) {
                     if (s_start[wid][k + step] <= j) k += step;
@@ -523,14 +617,17 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
                 const uint32_t q = j - s_start[wid][k];
                 const uint32_t w = s_bw[wid][k];
                 const uint32_t dy = q / w, dx = q - dy * w;
-                const uint32_t bin = ((s_org[wid][k] >> 16) + dy) * B.bins_x + (s_org[wid][k] & 0xFFFFu) + dx;
-                const uint32_t slot = atomicAdd(&B.bin_counts[bin], 1u);
-                if (slot < B.bin_capacity) B.bin_lists[(size_t)bin * B.bin_capacity + slot] = chunk * 64 + k;
+                bin = ((s_org[wid][k] >> 16) + dy) * B.bins_x + (s_org[wid][k] & 0xFFFFu) + dx;
+            }
+            uint32_t slot = 0;
+            if (pv) slot = atomicAdd(&B.bin_counts[bin], 1u);
+            if (pv) {
+                if (slot < B.bin_capacity) B.bin_lists[(size_t)bin * B.bin_capacity + slot] = ck.base + k;
                 else s_spill[wid][k] = 1;
             }
         }
         __builtin_amdgcn_wave_barrier();
-        const bool spilled = (r < n) && s_spill[wid][lane] != 0;
+        const bool spilled = valid && s_spill[wid][lane] != 0;
         unsigned long long sm = __ballot(spilled);
         if (sm) {
             uint32_t base = 0;
@@ -538,24 +635,28 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
             base = __shfl(base, 0);
             if (spilled) p.bins.spill[base + mbcnt(sm)] = r;
         }
-        if (lane == 0 && total) atomicAdd(&p.counters->bin_entries, (unsigned long long)total);
         __builtin_amdgcn_wave_barrier();
     }
+    if (counting(p) && lane == 0 && entries) atomicAdd(&p.counters->bin_entries, entries);
 }
 
-// One 256-thread block per bin. The bin's keys live in LDS in the same 4x4
-// tile order as the vis-buffer; the block expands its list of records into
-// 4x4 tiles clipped to the bin (block-wide exclusive scan), 16 lanes per
-// tile, and keeps the nearest fragment with an LDS 64-bit max. The finished
-// bin is written to HBM once, one contiguous 4-pixel-high row of tiles at a
+// One block per bin. The bin's keys live in LDS in the same 4x4 tile order
+// as the vis-buffer. The block loads up to kThreads records at a time,
+// clips their render boxes to the bin and lays the clipped boxes out as one
+// list of 4x4 tiles (block-wide exclusive scan). The list is cut into
+// equal ranges, one per 16-lane group; a group walks its range tile by tile
+// (lane = pixel), stepping to the next triangle as the range crosses it, and
+// keeps the nearest fragment with an LDS 64-bit max. The finished bin is
+// written to HBM once, one contiguous 4-pixel-high strip of tiles at a
 // time. After a clear the bin starts from the clear key, otherwise from the
 // vis-buffer's current contents.
-template <int BS>
-__global__ void __launch_bounds__(256) k_bin_raster(DrawParams p) {
-    constexpr int kThreads = 256;
+template <int BS, int kThreads>
+__global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
+    constexpr int kWaves = kThreads / kWave;
+    constexpr int kGroups = kThreads / 16;
     __shared__ unsigned long long keys[BS * BS];
     __shared__ TriLds tl[kThreads];
-    __shared__ uint32_t s_wave_total[4];
+    __shared__ uint32_t s_wave_total[kWaves];
     const BinParams& B = p.bins;
     const uint32_t bin = blockIdx.x;
     const uint32_t bx = bin % B.bins_x, by = bin / B.bins_x;
@@ -576,10 +677,10 @@ __global__ void __launch_bounds__(256) k_bin_raster(DrawParams p) {
             v = p.keys[(size_t)(y0 + strip * 4) * W + x0 * 4 + rem];
         keys[i] = v;
     }
-    __syncthreads();
 
-    const bool countFrags = (p.flags & 8u) != 0;
+    const bool countFrags = counting(p);
     uint32_t covered = 0, passed = 0;
+    const uint32_t g = tid >> 4, px = tid & 3, py = (tid >> 2) & 3;
     for (uint32_t base = 0; base < n; base += kThreads) {
         const uint32_t e = base + tid;
         uint32_t ntiles = 0;
@@ -590,13 +691,14 @@ __global__ void __launch_bounds__(256) k_bin_raster(DrawParams p) {
             load_tri(p, r, hw, hh, t, bmin, bmax);
             uint32_t cx0 = max((uint32_t)lo16(bmin), x0), cy0 = max((uint32_t)hi16(bmin), y0);
             uint32_t cx1 = min((uint32_t)lo16(bmax), x1), cy1 = min((uint32_t)hi16(bmax), y1);
+            t.org = 0;
+            t.tw = 1;
             if (cx1 > cx0 && cy1 > cy0) {
                 t.org = (cx0 - x0) | ((cy0 - y0) << 16);
                 t.tw = (cx1 - cx0) >> 2;
                 ntiles = t.tw * ((cy1 - cy0) >> 2);
             }
         }
-        // block-wide exclusive scan of ntiles
         uint32_t incl = ntiles;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -607,7 +709,7 @@ __global__ void __launch_bounds__(256) k_bin_raster(DrawParams p) {
         __syncthreads();
         uint32_t prev = 0, total = 0;
 #pragma unroll
-        for (uint32_t w = 0; w < 4; w++) {
+        for (uint32_t w = 0; w < (uint32_t)kWaves; w++) {
             uint32_t v = s_wave_total[w];
             if (w < wid) prev += v;
             total += v;
@@ -617,19 +719,27 @@ __global__ void __launch_bounds__(256) k_bin_raster(DrawParams p) {
         if (e >= n) tl[tid].start = 0xFFFFFFFFu;
         __syncthreads();
 
-        const uint32_t g = tid >> 4, px = tid & 3, py = (tid >> 2) & 3;
-        for (uint32_t it = 0; it < total; it += 16) {
-            const uint32_t j = it + g;
-            if (j < total) {
-                uint32_t k = 0;
+        // group g walks tiles [j0, j1) of the block's tile list
+        const uint32_t per = (total + kGroups - 1) / kGroups;
+        uint32_t j = min(g * per, total);
+        const uint32_t j1 = min(j + per, total);
+        if (j < j1) {
+            uint32_t k = 0;
 #pragma unroll
-                for (uint32_t step = kThreads / 2; step > 0; step >>= 1) {
-                    if (tl[k + step].start <= j) k += step;
+            for (uint32_t step = kThreads / 2; step > 0; step >>= 1) {
+                if (tl[k + step].start <= j) k += step;
+            }
+            TriLds T = tl[k];
+            uint32_t next = (k + 1 < (uint32_t)kThreads) ? tl[k + 1].start : 0xFFFFFFFFu;
+            uint32_t pt = j - T.start;
+            uint32_t ty = pt / T.tw, tx = pt - ty * T.tw;
+            for (; j < j1; j++) {
+                while (j >= next) {   // step to the triangle owning tile j
+                    k++;
+                    T = tl[k];
+                    next = (k + 1 < (uint32_t)kThreads) ? tl[k + 1].start : 0xFFFFFFFFu;
+                    tx = ty = 0;
                 }
-                const TriLds& T = tl[k];
-                const uint32_t pt = j - T.start;
-                const uint32_t ty = pt / T.tw;
-                const uint32_t tx = pt - ty * T.tw;
                 const uint32_t lx = (T.org & 0xFFFFu) + tx * 4 + px;
                 const uint32_t ly = (T.org >> 16) + ty * 4 + py;
                 float depth;
@@ -642,6 +752,7 @@ __global__ void __launch_bounds__(256) k_bin_raster(DrawParams p) {
                         if (key > keys[li]) atomicMax(&keys[li], key);
                     }
                 }
+                if (++tx == T.tw) { tx = 0; ty++; }
             }
         }
         __syncthreads();
@@ -660,8 +771,8 @@ __global__ void __launch_bounds__(256) k_bin_raster(DrawParams p) {
             atomicAdd(&p.counters->fragments_covered, (unsigned long long)covered);
             atomicAdd(&p.counters->fragments_passed, (unsigned long long)passed);
         }
+        if (tid == 0 && n) atomicAdd(&p.counters->bins_rasterized, 1ull);
     }
-    if (tid == 0 && n) atomicAdd(&p.counters->bins_rasterized, 1ull);
 }
 
 __global__ void k_begin_draw(DrawScratch* s, uint32_t* bin_counts, uint32_t num_bins) {
@@ -761,7 +872,7 @@ void launch_front(const DrawParams& p, hipStream_t s, Profiler* prof, uint32_t n
     }
     if (p.clipping) {
         Profiler::Scope sc(prof, "clip", s);
-        hipLaunchKernelGGL(k_clip, dim3(256), dim3(64), 0, s, p);
+        hipLaunchKernelGGL(k_clip, dim3(256), dim3(256), 0, s, p);
     }
 }
 
@@ -771,7 +882,8 @@ hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s, Profiler* prof
     launch_front(p, s, prof, 0);
     {
         Profiler::Scope sc(prof, "raster_direct", s);
-        hipLaunchKernelGGL(k_raster_direct, dim3(2048), dim3(256), 0, s, p, 0);
+        uint32_t blocks = max((2 * p.meshlet_count + 3) / 4 + 16, 64u);
+        hipLaunchKernelGGL(k_raster_direct, dim3(blocks), dim3(256), 0, s, p, 0);
     }
     return hipGetLastError();
 }
@@ -781,15 +893,17 @@ hipError_t launch_draw_binned(const DrawParams& p, hipStream_t s, Profiler* prof
     launch_front(p, s, prof, num_bins);
     {
         Profiler::Scope sc(prof, "bin_scatter", s);
-        hipLaunchKernelGGL(k_bin_scatter, dim3(1024), dim3(256), 0, s, p);
+        // about one chunk per wave: two per meshlet plus room for clipped records
+        uint32_t blocks = max((2 * p.meshlet_count + 3) / 4 + 16, 64u);
+        hipLaunchKernelGGL(k_bin_scatter, dim3(blocks), dim3(256), 0, s, p);
     }
     {
         Profiler::Scope sc(prof, "bin_raster", s);
         switch (p.bins.bin_size) {
-        case 8: hipLaunchKernelGGL(k_bin_raster<8>, dim3(num_bins), dim3(256), 0, s, p); break;
-        case 16: hipLaunchKernelGGL(k_bin_raster<16>, dim3(num_bins), dim3(256), 0, s, p); break;
-        case 32: hipLaunchKernelGGL(k_bin_raster<32>, dim3(num_bins), dim3(256), 0, s, p); break;
-        case 64: hipLaunchKernelGGL(k_bin_raster<64>, dim3(num_bins), dim3(256), 0, s, p); break;
+        case 8: hipLaunchKernelGGL((k_bin_raster<8, 256>), dim3(num_bins), dim3(256), 0, s, p); break;
+        case 16: hipLaunchKernelGGL((k_bin_raster<16, 256>), dim3(num_bins), dim3(256), 0, s, p); break;
+        case 32: hipLaunchKernelGGL((k_bin_raster<32, 256>), dim3(num_bins), dim3(256), 0, s, p); break;
+        case 64: hipLaunchKernelGGL((k_bin_raster<64, 256>), dim3(num_bins), dim3(256), 0, s, p); break;
         default: return hipErrorInvalidValue;
         }
     }

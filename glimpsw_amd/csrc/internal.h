@@ -97,7 +97,7 @@ struct Counters {                  // device counters, u64 each
 
 // Per-draw scratch counters, cleared by the first kernel of each draw.
 struct DrawScratch {
-    uint32_t num_records;
+    uint32_t num_records;          // records appended by the clipper
     uint32_t num_clip;
     uint32_t num_visible;
     uint32_t overflow;             // a record or clip queue was full
@@ -117,7 +117,7 @@ struct BinParams {
     uint32_t bin_capacity;
     uint32_t* bin_counts;          // [bins_x * bins_y]
     uint32_t* bin_lists;           // [bins_x * bins_y][bin_capacity]
-    uint32_t* spill;               // [record_capacity]
+    uint32_t* spill;               // [clip_record_base + clip_record_capacity]
     int fresh;                     // first draw after a clear: start bins from the clear key
 };
 
@@ -137,8 +137,10 @@ struct DrawParams {
     uint32_t flags;
     ApproxTables tables;
     uint16_t* cull_bitmap;         // scratch: (count + 15) / 16 words
-    TriRecord* records;
-    uint32_t record_capacity;
+    TriRecord* records;            // [meshlet_count * 128 + clip_record_capacity]
+    uint32_t* meshlet_counts;      // [meshlet_count] records kept per meshlet
+    uint32_t clip_record_base;     // = meshlet_count * 128
+    uint32_t clip_record_capacity;
     ClipEntry* clip_queue;
     uint32_t clip_capacity;
     DrawScratch* scratch;

@@ -128,7 +128,7 @@ int glimp_framebuffer_keys(glimp_framebuffer* fb, void** keys, size_t* bytes);
 #define GLIMP_DRAW_NO_CLIPPING 0x1u   /* Rasterizer::EnableClipping = false (Rasterizer.h:207) */
 #define GLIMP_DRAW_NO_GUARDBAND 0x2u  /* Rasterizer::EnableGuardband = false (Rasterizer.h:208) */
 #define GLIMP_DRAW_DIRECT 0x4u        /* per-fragment global atomics instead of screen bins */
-#define GLIMP_DRAW_COUNT_FRAGMENTS 0x8u /* maintain the fragment counters (slower) */
+#define GLIMP_DRAW_COUNTERS 0x8u       /* maintain the perf counters (adds atomics) */
 
 /* Pre-instantiated shader pipelines: the device counterpart of a dispatch
  * table built by GetDispatchTable<MeshFn, FragFn...> (Rasterizer.h:342-364). */
@@ -164,7 +164,7 @@ typedef struct glimp_counters {
     uint64_t triangles_rasterized;     /* PerfCounter::TrianglesRasterized */
     uint64_t triangles_clipped;        /* PerfCounter::TrianglesClipped */
     uint64_t bins_rasterized;          /* PerfCounter::BinQueueFlushes */
-    uint64_t fragments_covered;        /* with GLIMP_DRAW_COUNT_FRAGMENTS */
+    uint64_t fragments_covered;        /* all counters need GLIMP_DRAW_COUNTERS */
     uint64_t fragments_passed;
     uint64_t setup_records;            /* triangles that left setup */
     uint64_t bin_entries;              /* (triangle, bin) pairs */

@@ -406,7 +406,9 @@ struct Clipper {
                     uint8_t ia = idx[vi], ib = idx[(vi + 1) % vertCount];
                     float da = dist(ia, plane, scale), db = dist(ib, plane, scale);
                     if (da >= 0) out[outCount++] = ia;
-                    if ((da >= 0) != (db >= 0)) {
+                    // the vertex store has room for 16 generated vertices
+                    // per triangle (Rasterizer.cpp:64, 441)
+                    if ((da >= 0) != (db >= 0) && nextIdx < 64) {
                         float t = da / (da - db);
                         for (int k = 0; k < 6; k++)
                             Vertices[k][nextIdx] = fmaf(Vertices[k][ib], t, fmaf(-t, Vertices[k][ia], Vertices[k][ia]));
@@ -418,7 +420,9 @@ struct Clipper {
                 memcpy(idx, out, sizeof(idx));
             }
             if (vertCount < 2) continue;
-            for (uint32_t vi = 0; vi + 2 < vertCount; vi++) {
+            // at most 8 fan triangles per source triangle (a clipped
+            // triangle has at most 9 vertices, i.e. 7 fan triangles)
+            for (uint32_t vi = 0; vi + 2 < vertCount && vi < 8; vi++) {
                 uint32_t i0 = idx[0], i1 = idx[vi + 1], i2 = idx[vi + 2];
                 uint32_t lane = OutIdx % 16, pi = OutIdx / 16;
                 uint32_t ii[3] = { i0, i1, i2 };
@@ -458,7 +462,7 @@ inline uint32_t lerp16(uint32_t a, uint32_t b, uint32_t t) {  // SIMD.h:450 per 
 void sample_level16(const orc_texture* tex, const float* u, const float* v, int32_t layer,
                     const int32_t* mipIn, int wrap, bool magLinear, bool minLinear,
                     const bool* active, uint32_t* out) {
-    int32_t ix[16], iy[16], mip[16];
+    int32_t ix[16], iy[16];
     bool anyPos = false;
     for (int l = 0; l < 16; l++) {
         float su = u[l] * tex->scale_lerp_u, sv = v[l] * tex->scale_lerp_v;
@@ -480,7 +484,6 @@ void sample_level16(const orc_texture* tex, const float* u, const float* v, int3
     int maxLevel = (int)tex->mip_levels - 1;
     for (int l = 0; l < 16; l++) {
         int m = std::min(std::max(mipIn[l], 0), maxLevel);
-        mip[l] = m;
         int32_t offset = layer * (int32_t)tex->layer_stride + (int32_t)tex->mip_offsets[m];
         int32_t stride = (int32_t)tex->row_shift - m;
         int32_t x = ix[l] >> m, y = iy[l] >> m;
@@ -831,6 +834,63 @@ void orc_setup16(const float* clip, int32_t w, int32_t h, int32_t cull_mode,
         }
         det[l] = d;
     }
+}
+
+// Float part of TriangleEdgeVars::Setup for 16 lanes (Rasterizer.cpp:320-323).
+void orc_edge16(const int32_t* det, const float* Z, float* Z10, float* Z20) {
+    for (int l = 0; l < 16; l++) {
+        float rcpArea = fdiv(16.0f, (float)det[l]);
+        Z10[l] = (Z[16 + l] - Z[l]) * rcpArea;
+        Z20[l] = (Z[32 + l] - Z[l]) * rcpArea;
+    }
+}
+
+// Texture coordinates, UV gradients and mip level for 16 lanes, the
+// arithmetic of resolve_surface_tile (Shading.cpp:516-539).
+void orc_uvgrad16(const float* t, const float* bary, const float* ddx, const float* ddy, float su, float sv,
+                  float* uv, int32_t* mip) {
+    for (int l = 0; l < 16; l++) {
+        float u0 = t[l], v0 = t[16 + l];
+        float t10x = t[32 + l] - u0, t10y = t[48 + l] - v0;
+        float t20x = t[64 + l] - u0, t20y = t[80 + l] - v0;
+        float by = bary[16 + l], bz = bary[32 + l];
+        uv[l] = fmaf(bz, t20x, fmaf(by, t10x, u0));
+        uv[16 + l] = fmaf(bz, t20y, fmaf(by, t10y, v0));
+        float gx = fmaf(t20x, ddx[32 + l], t10x * ddx[16 + l]);
+        float gy = fmaf(t20y, ddx[32 + l], t10y * ddx[16 + l]);
+        float gz = fmaf(t20x, ddy[32 + l], t10x * ddy[16 + l]);
+        float gw = fmaf(t20y, ddy[32 + l], t10y * ddy[16 + l]);
+        float dx = fmaf(gx, gx, gy * gy) * (su * su);
+        float dy = fmaf(gz, gz, gw * gw) * (sv * sv);
+        float mx = (dx > dy) ? dx : dy;
+        mip[l] = wrap_sub((int32_t)fbits(mx), 127 << 23) >> 24;
+    }
+}
+
+// Clipper::ClipTriangles for a single triangle (lane 0 of a packet): returns
+// the fan triangles' clip-space vertices [n][3][4] and their count.
+int32_t orc_clip_triangle(const float* v /*[3][4]*/, uint32_t outcodes, float gbx, float gby, float* out) {
+    V4 vin[3][16];
+    memset(vin, 0, sizeof(vin));
+    for (int k = 0; k < 3; k++) vin[k][0] = { v[k * 4 + 0], v[k * 4 + 1], v[k * 4 + 2], v[k * 4 + 3] };
+    ClipCodes cc;
+    cc.NonTrivialMask = 1;
+    cc.OutCodes[0] = (uint8_t)outcodes;
+    Clipper c;
+    c.bx = gbx;
+    c.by = gby;
+    c.OutIdx = 0;
+    c.clip(vin, cc, 1 << 14, 1 << 14, 0);  // large viewport: only the geometry is used
+    for (uint32_t i = 0; i < c.OutIdx; i++) {
+        for (int k = 0; k < 3; k++) {
+            const V4& q = c.Out[k][i / 16][i % 16];
+            out[(i * 3 + k) * 4 + 0] = q.x;
+            out[(i * 3 + k) * 4 + 1] = q.y;
+            out[(i * 3 + k) * 4 + 2] = q.z;
+            out[(i * 3 + k) * 4 + 3] = q.w;
+        }
+    }
+    return (int32_t)c.OutIdx;
 }
 
 // ResolveDebug(DebugLayer::BaseColor) (Shading.cpp:734-773).

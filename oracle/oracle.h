@@ -125,6 +125,11 @@ void orc_sample_level16(const orc_texture* tex, const float* u, const float* v, 
                         uint32_t* out);
 void orc_intersect_triangle16(const float* clip /*[3][4][16]*/, const float* suv /*[2][16]*/,
                               float sw, float sh, float* bary, float* ddx, float* ddy);
+void orc_edge16(const int32_t* det, const float* Z /*[3][16]*/, float* Z10, float* Z20);
+void orc_uvgrad16(const float* t /*[3][2][16]*/, const float* bary, const float* ddx, const float* ddy,
+                  float su, float sv, float* uv /*[2][16]*/, int32_t* mip);
+int32_t orc_clip_triangle(const float* v /*[3][4]*/, uint32_t outcodes, float gbx, float gby,
+                          float* out /*[128][3][4]*/);
 void orc_setup16(const float* clip /*[3][4][16]*/, int32_t w, int32_t h, int32_t cull_mode,
                  int32_t* X, int32_t* Y, float* Z, float* W, float* det, uint16_t* mask);
 

@@ -177,7 +177,7 @@ def test_counters_match_oracle(gpu_ctx):
     view = S.make_view(320, 240, count=sc.count)
     o = _oracle_frame(sc, view)
     gpu_ctx.reset_counters()
-    _gpu_frame(gpu_ctx, sc, view, flags=g.GLIMP_DRAW_COUNT_FRAGMENTS)
+    _gpu_frame(gpu_ctx, sc, view, flags=g.GLIMP_DRAW_COUNTERS)
     c = gpu_ctx.counters()
     for k in ("triangles_processed", "triangles_rasterized", "triangles_clipped", "fragments_covered"):
         assert c[k] == o[2][k], (k, c[k], o[2][k])

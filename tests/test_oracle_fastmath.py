@@ -1,0 +1,183 @@
+"""Pins the oracle's floating-point operation order to what clang emits for
+the reference's expression shapes under the reference's flags
+(-O2 -march=native -mprefer-vector-width=512 -ffast-math,
+src/SwRast/CMakeLists.txt:6).
+
+oracle/probes/fastmath_probe.cpp holds those expression shapes compiled with
+those flags; each test feeds both sides the same random inputs and requires
+bit-identical outputs. A dropped Newton step, a different FMA contraction or
+a swapped operand changes low bits and fails here.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+
+f32p = C.POINTER(C.c_float)
+i32p = C.POINTER(C.c_int32)
+u16p = C.POINTER(C.c_uint16)
+
+
+def _has_avx512():
+    try:
+        flags = open("/proc/cpuinfo").read()
+    except OSError:
+        return False
+    return "avx512f" in flags
+
+
+pytestmark = pytest.mark.skipif(not _has_avx512() or not os.path.exists("/opt/rocm/llvm/bin/clang++"),
+                                reason="probe needs clang and an AVX-512 host")
+
+
+@pytest.fixture(scope="module")
+def probe():
+    return oracle.probe_lib()
+
+
+def P(a):
+    return np.ascontiguousarray(a).ctypes.data_as(f32p if a.dtype == np.float32 else i32p)
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def test_rcp14_rsqrt14_random(probe):
+    rng = np.random.default_rng(1)
+    x = rng.integers(0, 2 ** 32, 1 << 16, dtype=np.uint64).astype(np.uint32).view(np.float32).copy()
+    out = np.zeros_like(x)
+    probe.probe_rcp14(P(x), P(out), len(x))
+    L = oracle.lib()
+    mine = np.array([L.orc_rcp14(float(v)) for v in x[:4096]], dtype=np.float32)
+    assert (_bits(mine) == _bits(out[:4096])).all()
+    probe.probe_rsqrt14(P(x), P(out), len(x))
+    mine = np.array([L.orc_rsqrt14(float(v)) for v in x[:4096]], dtype=np.float32)
+    nan = np.isnan(out[:4096])
+    assert (_bits(mine)[~nan] == _bits(out[:4096])[~nan]).all() and np.isnan(mine[nan]).all()
+
+
+def test_vector_division(probe):
+    rng = np.random.default_rng(2)
+    n = 4096
+    a = (rng.standard_normal(n) * 10 ** rng.uniform(-3, 3, n)).astype(np.float32)
+    b = (rng.standard_normal(n) * 10 ** rng.uniform(-3, 3, n)).astype(np.float32)
+    out = np.zeros(n, np.float32)
+    probe.probe_div(P(a), P(b), P(out), n)
+    L = oracle.lib()
+    mine = np.array([L.orc_div(float(x), float(y)) for x, y in zip(a, b)], dtype=np.float32)
+    assert (_bits(mine) == _bits(out)).all()
+    probe.probe_recip(P(b), P(out), n)
+    mine = np.array([L.orc_div(1.0, float(y)) for y in b], dtype=np.float32)
+    assert (_bits(mine) == _bits(out)).all()
+    # the approximation is not IEEE division: it must differ somewhere
+    assert (_bits(out) != _bits((np.float32(1) / b).astype(np.float32))).any()
+
+
+def _random_clip(rng, n):
+    """Triangles in front of the camera with random clip coordinates."""
+    v = np.empty((3, 4, n), np.float32)
+    w = rng.uniform(0.5, 50.0, (3, n))
+    v[:, 3] = w
+    v[:, 0] = rng.uniform(-1.2, 1.2, (3, n)) * w
+    v[:, 1] = rng.uniform(-1.2, 1.2, (3, n)) * w
+    v[:, 2] = 0.01
+    return v
+
+
+@pytest.mark.parametrize("cull_mode", [0, 1, 2])
+def test_triangle_setup(probe, cull_mode):
+    rng = np.random.default_rng(3 + cull_mode)
+    n = 16 * 64
+    v = _random_clip(rng, n)
+    W, H = 1920, 1080
+    X = np.zeros((3, n), np.int32); Y = np.zeros_like(X)
+    Z = np.zeros((3, n), np.float32); Wr = np.zeros_like(Z); det = np.zeros(n, np.float32)
+    probe.probe_setup(P(v), n, C.c_float(W // 2 * 16), C.c_float(H // 2 * 16), cull_mode, P(X), P(Y), P(Z), P(Wr),
+                      P(det))
+    L = oracle.lib()
+    for p0 in range(0, n, 16):
+        clip = np.ascontiguousarray(v[:, :, p0:p0 + 16])
+        x = np.zeros((3, 16), np.int32); y = np.zeros_like(x)
+        z = np.zeros((3, 16), np.float32); w = np.zeros_like(z); d = np.zeros(16, np.float32)
+        mask = np.zeros(1, np.uint16)
+        L.orc_setup16(P(clip), W, H, cull_mode, P(x), P(y), P(z), P(w), P(d), mask.ctypes.data_as(u16p))
+        assert (x == X[:, p0:p0 + 16]).all() and (y == Y[:, p0:p0 + 16]).all()
+        assert (_bits(z) == _bits(Z[:, p0:p0 + 16])).all()
+        assert (_bits(w) == _bits(Wr[:, p0:p0 + 16])).all()
+        assert (_bits(d) == _bits(det[p0:p0 + 16])).all()
+
+
+def test_edge_setup_float_part(probe):
+    rng = np.random.default_rng(5)
+    n = 16 * 32
+    det = rng.integers(1, 1 << 30, n).astype(np.int32)
+    Z = rng.uniform(0.0, 1.0, (3, n)).astype(np.float32)
+    W = rng.uniform(0.0, 1.0, (3, n)).astype(np.float32)
+    z10 = np.zeros(n, np.float32); z20 = np.zeros_like(z10)
+    w0 = np.zeros_like(z10); w1 = np.zeros_like(z10); w2 = np.zeros_like(z10)
+    probe.probe_edge(P(det), P(Z), P(W), n, P(z10), P(z20), P(w0), P(w1), P(w2))
+    L = oracle.lib()
+    for p0 in range(0, n, 16):
+        a = np.zeros(16, np.float32); b = np.zeros(16, np.float32)
+        L.orc_edge16(P(det[p0:p0 + 16].copy()), P(np.ascontiguousarray(Z[:, p0:p0 + 16])), P(a), P(b))
+        assert (_bits(a) == _bits(z10[p0:p0 + 16])).all() and (_bits(b) == _bits(z20[p0:p0 + 16])).all()
+
+
+def test_intersect_triangle(probe):
+    rng = np.random.default_rng(6)
+    n = 16 * 32
+    v = _random_clip(rng, n)
+    suv = rng.uniform(-1, 1, (2, n)).astype(np.float32)
+    bary = np.zeros((3, n), np.float32); ddx = np.zeros_like(bary); ddy = np.zeros_like(bary)
+    probe.probe_intersect(P(v), P(suv), n, C.c_float(1920), C.c_float(1080), P(bary), P(ddx), P(ddy))
+    L = oracle.lib()
+    for p0 in range(0, n, 16):
+        b = np.zeros((3, 16), np.float32); dx = np.zeros_like(b); dy = np.zeros_like(b)
+        L.orc_intersect_triangle16(P(np.ascontiguousarray(v[:, :, p0:p0 + 16])), P(np.ascontiguousarray(suv[:, p0:p0 + 16])),
+                                   C.c_float(1920), C.c_float(1080), P(b), P(dx), P(dy))
+        for mine, ref in ((b, bary), (dx, ddx), (dy, ddy)):
+            r = ref[:, p0:p0 + 16]
+            ok = (_bits(mine) == _bits(r)) | (np.isnan(mine) & np.isnan(r))
+            assert ok.all()
+
+
+def test_uv_gradients_and_mip(probe):
+    rng = np.random.default_rng(7)
+    n = 16 * 32
+    t = rng.uniform(0, 8, (3, 2, n)).astype(np.float32)
+    bary = rng.uniform(0, 1, (3, n)).astype(np.float32)
+    ddx = (rng.standard_normal((3, n)) * 1e-3).astype(np.float32)
+    ddy = (rng.standard_normal((3, n)) * 1e-3).astype(np.float32)
+    uv = np.zeros((2, n), np.float32); mip = np.zeros(n, np.int32)
+    probe.probe_uvgrad(P(t), P(bary), P(ddx), P(ddy), n, C.c_float(256), C.c_float(256), P(uv), P(mip))
+    L = oracle.lib()
+    for p0 in range(0, n, 16):
+        tt = np.ascontiguousarray(t[:, :, p0:p0 + 16])
+        u = np.zeros((2, 16), np.float32); m = np.zeros(16, np.int32)
+        L.orc_uvgrad16(P(tt), P(np.ascontiguousarray(bary[:, p0:p0 + 16])), P(np.ascontiguousarray(ddx[:, p0:p0 + 16])),
+                       P(np.ascontiguousarray(ddy[:, p0:p0 + 16])), C.c_float(256), C.c_float(256), P(u), P(m))
+        assert (_bits(u) == _bits(uv[:, p0:p0 + 16])).all()
+        assert (m == mip[p0:p0 + 16]).all()
+
+
+def test_exhaustive_approx_tables():
+    """The committed VRCP14/VRSQRT14 tables reproduce this host's
+    instructions over all 2^32 inputs (the tables describe Intel's
+    implementation; another vendor's AVX-512 may round differently)."""
+    cpu = open("/proc/cpuinfo").read()
+    if "GenuineIntel" not in cpu:
+        pytest.skip("tables describe Intel's VRCP14PS/VRSQRT14PS")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = os.path.join(root, "glimpsw_amd", "data", "gen_approx14.c")
+    exe = os.path.join(root, "oracle", "_build", "gen_approx14")
+    os.makedirs(os.path.dirname(exe), exist_ok=True)
+    subprocess.run(["gcc", "-O2", "-mavx512f", src, "-o", exe], check=True)
+    r = subprocess.run([exe, "--verify", os.path.join(root, "glimpsw_amd", "data", "approx14.bin")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "rcp14 mismatches 0, rsqrt14 mismatches 0" in r.stdout
