@@ -50,18 +50,24 @@ def algorithmic_bytes(stats: dict) -> dict:
         # written once
         "bin_raster": 36 * P + 8 * N,
         "raster_spill": 0,
-        "resolve_albedo": stats.get("resolve_bytes", 0),
+        # key per pixel in, RGBA8 out; per covered pixel three vertices
+        # (index, position, UV, normal: 1 + 12 + 4 + 4 bytes each) and four
+        # RGBA8 texels
+        "resolve_lit": 8 * N + 4 * N + stats.get("covered_pixels", 0) * (3 * 21 + 16),
     }
 
 
 def cpu_baseline(scene, view, seconds: float) -> dict:
     import oracle
+    from glimpsw_amd import scene as S
 
     b = oracle.SceneBinding(scene)
     frames = 0
     t0 = time.perf_counter()
+    lighting = S.default_lighting(view)
     while True:
         fb, ctr, _ = oracle.render_frame(scene, view, b)
+        oracle.resolve_lit(fb, b, view.object_to_clip, lighting)
         frames += 1
         el = time.perf_counter() - t0
         if el >= seconds:
@@ -72,7 +78,7 @@ def cpu_baseline(scene, view, seconds: float) -> dict:
         "unit": "Mtri/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{frames} frames of the same scene (clear+cull+draw, {tris} submitted triangles each) "
+        "sample": f"{frames} frames of the same scene (clear+cull+draw+lit resolve, {tris} submitted triangles each) "
                   f"in {el:.1f} s on one host core, scalar C++ oracle",
         "fps": round(frames / el, 3),
     }
@@ -123,10 +129,13 @@ def main() -> int:
     planes = view.frustum_planes()
     flags = g.GLIMP_DRAW_DIRECT if args.direct else 0
 
+    lighting = S.default_lighting(view)
+
     def frame():
         fb.clear(0, 0.0, stream)
         fb.draw(ds, view.object_to_clip, view.meshlet_count, 0, planes, flags, stream, args.bin_size,
                 args.bin_capacity)
+        fb.resolve(ds, view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting, stream)
 
     for _ in range(args.warmup):
         frame()
@@ -157,6 +166,8 @@ def main() -> int:
         "fragments": int(c["fragments_covered"]), "pixels": view.width * view.height,
         "bin_entries": int(c["bin_entries"]), "bins_rasterized": int(c["bins_rasterized"]),
     }
+    _, depth = fb.read()
+    stats["covered_pixels"] = int((depth > 0).sum())
     alg = algorithmic_bytes(stats)
     per_kernel = {}
     for name, (launches, total) in kern.items():
@@ -168,7 +179,7 @@ def main() -> int:
     dk = per_kernel[dom]
     tris = scene.triangles()
     result = {
-        "metric": "Mtri/s at 1080p/1M tris (frame: clear+cull+mesh/setup+clip+raster into vis-buffer)",
+        "metric": "Mtri/s at 1080p/1M tris (frame: clear+cull+mesh/setup+clip+bin+raster into vis-buffer+lit resolve)",
         "value": round(tris / (ms * 1e-3) / 1e6, 2),
         "unit": "Mtri/s",
         "n_gpus": 1,

@@ -69,6 +69,38 @@ class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 40), ("launches", C.c_uint64), ("total_ms", C.c_double)]
 
 
+class LightDesc(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("position", C.c_float * 3), ("direction", C.c_float * 3),
+                ("color", C.c_float * 3), ("intensity", C.c_float), ("inv_radius_sq", C.c_float),
+                ("spot_scale", C.c_float), ("spot_offset", C.c_float)]
+
+
+class ResolveDesc(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("object_to_clip", C.c_float * 16), ("object_to_world", C.c_float * 9),
+                ("inv_screen_proj", C.c_float * 16), ("view_pos", C.c_float * 3), ("exposure", C.c_float),
+                ("lights", C.POINTER(LightDesc)), ("num_lights", C.c_uint32)]
+
+
+GLIMP_RESOLVE_ALBEDO = 0
+GLIMP_RESOLVE_LIT = 1
+
+
+def light_structs(lights, cls=None):
+    cls = cls or LightDesc
+    arr = (cls * max(1, len(lights)))()
+    for i, L in enumerate(lights):
+        arr[i].type = L.type
+        for k in range(3):
+            arr[i].position[k] = L.position[k]
+            arr[i].direction[k] = L.direction[k]
+            arr[i].color[k] = L.color[k]
+        arr[i].intensity = L.intensity
+        arr[i].inv_radius_sq = L.inv_radius_sq
+        arr[i].spot_scale = 0.0
+        arr[i].spot_offset = 0.0
+    return arr
+
+
 # Every symbol include/glimp.h declares, with its ctypes signature.
 _SIGNATURES = {
     "glimp_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
@@ -90,6 +122,9 @@ _SIGNATURES = {
                                       C.POINTER(C.c_uint32), C.c_void_p]),
     "glimp_get_counters": (C.c_int, [C.c_void_p, C.POINTER(CountersStruct)]),
     "glimp_reset_counters": (C.c_int, [C.c_void_p]),
+    "glimp_resolve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(ResolveDesc), C.c_void_p]),
+    "glimp_framebuffer_color": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "glimp_framebuffer_read_color": (C.c_int, [C.c_void_p, u32p, C.c_void_p]),
     "glimp_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "glimp_profile_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_uint32, C.POINTER(C.c_uint32), C.c_int]),
 }
@@ -280,6 +315,41 @@ class Framebuffer:
         _check(self.ctx.lib.glimp_framebuffer_read(self.handle, _ptr(ids, u32p), _ptr(depth, f32p), int(linear),
                                                    _stream_ptr(stream)), "glimp_framebuffer_read")
         return ids, depth
+
+    def resolve(self, scene: Scene, object_to_clip, mode: int = GLIMP_RESOLVE_ALBEDO, lighting=None, stream=None):
+        """ShadingContext::Resolve (lit) or ResolveDebug(BaseColor) (albedo)."""
+        d = ResolveDesc()
+        d.mode = mode
+        for i in range(16):
+            d.object_to_clip[i] = float(object_to_clip[i])
+        keep = None
+        if lighting is not None:
+            for i in range(9):
+                d.object_to_world[i] = float(lighting.object_to_world[i])
+            for i in range(16):
+                d.inv_screen_proj[i] = float(lighting.inv_screen_proj[i])
+            for i in range(3):
+                d.view_pos[i] = float(lighting.view_pos[i])
+            d.exposure = lighting.exposure
+            keep = light_structs(lighting.lights)
+            d.lights = C.cast(keep, C.POINTER(LightDesc))
+            d.num_lights = len(lighting.lights)
+        else:
+            d.exposure = 1.0
+            d.object_to_world[0] = d.object_to_world[4] = d.object_to_world[8] = 1.0
+        _check(self.ctx.lib.glimp_resolve(self.ctx.handle, self.handle, scene.handle, C.byref(d), _stream_ptr(stream)),
+               "glimp_resolve")
+
+    def read_color(self, stream=None) -> np.ndarray:
+        out = np.zeros((self.height, self.width), dtype=np.uint32)
+        _check(self.ctx.lib.glimp_framebuffer_read_color(self.handle, _ptr(out, u32p), _stream_ptr(stream)),
+               "glimp_framebuffer_read_color")
+        return out
+
+    def color_ptr(self) -> int:
+        p = C.c_void_p()
+        _check(self.ctx.lib.glimp_framebuffer_color(self.handle, C.byref(p)), "glimp_framebuffer_color")
+        return p.value
 
     def keys_ptr(self):
         p = C.c_void_p()

@@ -2,6 +2,7 @@
 // draw/read entry points over the kernels in glimp_kernels.hip.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -10,6 +11,7 @@
 
 #include "glimp.h"
 #include "internal.h"
+#include "resolve.h"
 
 // 131072 u16: VRCP14 then VRSQRT14 mantissa tables, generated at build time
 // from glimpsw_amd/data/approx14.bin.
@@ -104,6 +106,7 @@ struct glimp_framebuffer {
     std::vector<DrawEntry> draws;
     DrawEntry* d_draws = nullptr;
     size_t d_draws_cap = 0;
+    uint32_t* color = nullptr;         // resolved RGBA8, row-major
 };
 
 namespace {
@@ -362,6 +365,7 @@ void glimp_framebuffer_destroy(glimp_framebuffer* f) {
     (void)hipSetDevice(f->ctx->device);
     if (f->keys) (void)hipFree(f->keys);
     if (f->d_draws) (void)hipFree(f->d_draws);
+    if (f->color) (void)hipFree(f->color);
     delete f;
 }
 
@@ -395,11 +399,10 @@ int glimp_framebuffer_keys(glimp_framebuffer* f, void** keys, size_t* bytes) {
     return GLIMP_OK;
 }
 
-int glimp_framebuffer_read(glimp_framebuffer* f, uint32_t* ids, float* depth, int linear, void* stream) {
-    if (!f) return fail(GLIMP_E_INVALID, "glimp_framebuffer_read: NULL framebuffer");
-    HIP_TRY(hipSetDevice(f->ctx->device));
-    hipStream_t s = (hipStream_t)stream;
-    size_t n = (size_t)f->width * f->height;
+namespace {
+
+// Uploads the framebuffer's draw table (seq ranges -> meshlet offsets).
+int upload_draws(glimp_framebuffer* f, hipStream_t s, uint32_t* count) {
     std::vector<DrawEntry> draws = f->draws;
     if (draws.empty()) draws.push_back(DrawEntry{ 0, 0, 0, 0 });
     if (draws.size() > f->d_draws_cap) {
@@ -409,8 +412,24 @@ int glimp_framebuffer_read(glimp_framebuffer* f, uint32_t* ids, float* depth, in
         HIP_TRY(dev_alloc(&f->d_draws, draws.size()));
         f->d_draws_cap = draws.size();
     }
-    HIP_TRY(flush_clear(f, s));
     HIP_TRY(hipMemcpyAsync(f->d_draws, draws.data(), draws.size() * sizeof(DrawEntry), hipMemcpyHostToDevice, s));
+    // the host vector must outlive the async copy
+    HIP_TRY(hipStreamSynchronize(s));
+    *count = (uint32_t)draws.size();
+    return GLIMP_OK;
+}
+
+}  // namespace
+
+int glimp_framebuffer_read(glimp_framebuffer* f, uint32_t* ids, float* depth, int linear, void* stream) {
+    if (!f) return fail(GLIMP_E_INVALID, "glimp_framebuffer_read: NULL framebuffer");
+    HIP_TRY(hipSetDevice(f->ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    size_t n = (size_t)f->width * f->height;
+    HIP_TRY(flush_clear(f, s));
+    uint32_t ndraws = 0;
+    int rc = upload_draws(f, s, &ndraws);
+    if (rc != GLIMP_OK) return rc;
     uint32_t* d_ids = nullptr;
     float* d_depth = nullptr;
     HIP_TRY(dev_alloc(&d_ids, n));
@@ -419,7 +438,7 @@ int glimp_framebuffer_read(glimp_framebuffer* f, uint32_t* ids, float* depth, in
     memcpy(&db, &f->clear_depth, 4);
     if (e == hipSuccess)
         e = launch_decode(f->keys, (int)f->width, (int)f->height, (unsigned long long)db << 32, f->clear_color,
-                          f->d_draws, (uint32_t)draws.size(), d_ids, d_depth, linear, s);
+                          f->d_draws, ndraws, d_ids, d_depth, linear, s);
     if (e == hipSuccess && ids) e = hipMemcpyAsync(ids, d_ids, n * 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess && depth) e = hipMemcpyAsync(depth, d_depth, n * 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -524,6 +543,72 @@ int glimp_cull_meshlets(glimp_context* c, glimp_scene* s, uint32_t first, uint32
     HIP_TRY(hipMemcpyAsync(&v, c->visible, 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (visible) *visible = v;
+    return GLIMP_OK;
+}
+
+int glimp_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const glimp_resolve_desc* d, void* stream) {
+    if (!c || !f || !sc || !d) return fail(GLIMP_E_INVALID, "glimp_resolve: NULL argument");
+    if (d->mode > GLIMP_RESOLVE_LIT) return fail(GLIMP_E_INVALID, "glimp_resolve: unknown mode");
+    if (d->num_lights > GLIMP_MAX_LIGHTS || (d->num_lights && !d->lights))
+        return fail(GLIMP_E_LIMIT, "glimp_resolve: at most GLIMP_MAX_LIGHTS lights");
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(flush_clear(f, s));
+    if (!f->color) HIP_TRY(dev_alloc(&f->color, (size_t)f->width * f->height));
+    ResolveParams p;
+    memset(&p, 0, sizeof(p));
+    p.m = sc->m;
+    p.materials = sc->materials;
+    p.num_materials = sc->num_materials;
+    p.textures = sc->textures;
+    p.tables = c->approx();
+    p.keys = f->keys;
+    p.width = (int)f->width;
+    p.height = (int)f->height;
+    uint32_t db;
+    memcpy(&db, &f->clear_depth, 4);
+    p.clear_key = (unsigned long long)db << 32;
+    int rc = upload_draws(f, s, &p.num_draws);
+    if (rc != GLIMP_OK) return rc;
+    p.draws = f->d_draws;
+    memcpy(p.object_to_clip, d->object_to_clip, sizeof(p.object_to_clip));
+    memcpy(p.object_to_world, d->object_to_world, sizeof(p.object_to_world));
+    memcpy(p.inv_screen_proj, d->inv_screen_proj, sizeof(p.inv_screen_proj));
+    memcpy(p.view_pos, d->view_pos, sizeof(p.view_pos));
+    p.exposure = d->exposure;
+    p.light_exposure = d->exposure * 0.001f;
+    // screen UV of pixel centres (Rasterizer.h:226-237) and the derivative
+    // steps of IntersectTriangle (Shading.cpp:454-455), IEEE scalar ops
+    float sw = (float)f->width, sh = (float)f->height;
+    p.scale_x = 2.0f / sw;
+    p.scale_y = 2.0f / sh;
+    p.center_x = std::fma(0.5f, p.scale_x, -1.0f);
+    p.center_y = std::fma(0.5f, p.scale_y, -1.0f);
+    p.kx = 2.0f / sw;
+    p.ky = -2.0f / sh;
+    p.num_lights = d->num_lights;
+    for (uint32_t i = 0; i < d->num_lights; i++) memcpy(&p.lights[i], &d->lights[i], sizeof(ResolveLight));
+    p.mode = (int)d->mode;
+    p.out = f->color;
+    p.linear = 1;
+    HIP_TRY(launch_resolve(p, s, &c->prof));
+    return GLIMP_OK;
+}
+
+int glimp_framebuffer_color(glimp_framebuffer* f, uint32_t** color) {
+    if (!f || !color) return fail(GLIMP_E_INVALID, "glimp_framebuffer_color: NULL argument");
+    *color = f->color;
+    if (!f->color) return fail(GLIMP_E_INVALID, "glimp_framebuffer_color: nothing resolved yet");
+    return GLIMP_OK;
+}
+
+int glimp_framebuffer_read_color(glimp_framebuffer* f, uint32_t* rgba, void* stream) {
+    if (!f || !rgba) return fail(GLIMP_E_INVALID, "glimp_framebuffer_read_color: NULL argument");
+    if (!f->color) return fail(GLIMP_E_INVALID, "glimp_framebuffer_read_color: nothing resolved yet");
+    HIP_TRY(hipSetDevice(f->ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipMemcpyAsync(rgba, f->color, (size_t)f->width * f->height * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return GLIMP_OK;
 }
 

@@ -1,0 +1,375 @@
+// Vis-buffer resolve for gfx950: one lane per pixel, 16 lanes per 4x4 tile.
+//
+// ShadingContext::Resolve / ResolveDebug (Shading.cpp:658-773): decode the
+// surface of each covered pixel from its 64-bit key, re-transform the
+// triangle's three vertices, recover perspective-correct barycentrics and
+// their screen derivatives analytically (IntersectTriangle,
+// Shading.cpp:417-464), interpolate UVs, pick the mip level from the UV
+// gradients and sample the material texture (Texture.h:412-575); in lit mode
+// also interpolate the vertex normal and evaluate the lights
+// (Shading.cpp:602-645), tonemap and pack.
+//
+// The reference shades a 4x4 tile per SIMD step; two decisions are taken
+// per tile: the texture filter (minification anywhere in the tile selects the
+// min filter, Texture.h:432) and the per-light early-outs. A 64-lane wave
+// holds four tiles, so those decisions are ballots over 16-lane groups.
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+#include "resolve.h"
+
+namespace glimp {
+namespace {
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ float dot3(const float a[3], const float b[3]) {
+    return __builtin_fmaf(a[0], b[0], __builtin_fmaf(a[1], b[1], a[2] * b[2]));
+}
+
+__device__ __forceinline__ void normalize3(const ApproxTables& T, float v[3]) {
+    float r = rsqrt14(T, dot3(v, v));
+    v[0] = v[0] * r;
+    v[1] = v[1] * r;
+    v[2] = v[2] * r;
+}
+
+__device__ __forceinline__ float mulsign(float x, float y) {
+    return bitsf(fbits(x) ^ (fbits(y) & 0x80000000u));
+}
+
+__device__ __forceinline__ float lerpf(float a, float b, float t) {
+    return __builtin_fmaf(t, b, __builtin_fmaf(-t, a, a));
+}
+
+// IntersectTriangle in the operation order of the reference's fast-math
+// build (see the oracle for the derivation from the compiled shape).
+__device__ void intersect_triangle(const ApproxTables& T, const float c[3][4], float su, float sv, float kx,
+                                   float ky, float bary[3], float ddx[3], float ddy[3]) {
+    float iw0 = frcp(T, c[0][3]), iw1 = frcp(T, c[1][3]), iw2 = frcp(T, c[2][3]);
+    float p0x = c[0][0] * iw0, p0y = c[0][1] * iw0;
+    float p1x = c[1][0] * iw1, p1y = c[1][1] * iw1;
+    float p2x = c[2][0] * iw2, p2y = c[2][1] * iw2;
+    float m0x = p2x - p1x, m1x = p0x - p1x, m1y = p0y - p1y;
+    float d12y = p1y - p2y;
+    float det = __builtin_fmaf(m0x, m1y, d12y * m1x);
+    float invDet = frcp(T, det);
+    float a0 = invDet * iw0, a1 = invDet * iw1, a2 = invDet * iw2;
+    float dx0 = a0 * d12y, dx1 = a1 * (p2y - p0y), dx2 = a2 * m1y;
+    float dy0 = a0 * m0x, dy1 = a1 * (p0x - p2x), dy2 = a2 * (p1x - p0x);
+    float dxs = (dx0 + dx1) + dx2, dys = (dy0 + dy1) + dy2;
+    float r0x = su - p0x, r0y = sv - p0y;
+    float iiw = __builtin_fmaf(dys, r0y, __builtin_fmaf(dxs, r0x, iw0));
+    float iW = frcp(T, iiw);
+    float by = iW * __builtin_fmaf(dx1, r0x, dy1 * r0y);
+    float bz = iW * __builtin_fmaf(dx2, r0x, dy2 * r0y);
+    float bx = 1.0f - (by + bz);
+    float wdx = frcp(T, __builtin_fmaf(kx, dxs, iiw)), wdy = frcp(T, __builtin_fmaf(ky, dys, iiw));
+    float b[3] = { bx, by, bz }, dxk[3] = { dx0, dx1, dx2 }, dyk[3] = { dy0, dy1, dy2 };
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float bi = b[k] * iiw;
+        ddx[k] = __builtin_fmaf(wdx, __builtin_fmaf(kx, dxk[k], bi), -b[k]);
+        ddy[k] = __builtin_fmaf(wdy, __builtin_fmaf(ky, dyk[k], bi), -b[k]);
+        bary[k] = b[k];
+    }
+}
+
+__device__ __forceinline__ uint32_t texel_offset(int32_t x, int32_t y, int32_t stride) {  // TiledY8
+    return (uint32_t)((y & 7) | (x << 3) | ((y & ~7) << stride));
+}
+
+// simd::lerp16 on the two 16-bit halves: a + mulhrs(b - a, t).
+__device__ __forceinline__ uint32_t lerp16(uint32_t a, uint32_t b, uint32_t t) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        int32_t av = (int16_t)(a >> (16 * h)), bv = (int16_t)(b >> (16 * h)), tv = (int16_t)(t >> (16 * h));
+        int32_t d = (int16_t)(bv - av);
+        int32_t prod = d * tv;
+        int32_t mh = (int16_t)(((prod >> 14) + 1) >> 1);
+        r |= (uint32_t)(uint16_t)(int16_t)(av + mh) << (16 * h);
+    }
+    return r;
+}
+
+// Texture2D<RGBA8u>::SampleLevel with Repeat wrap (Texture.h:412-459,
+// 506-556); `linear` is the tile's filter choice.
+__device__ uint32_t sample_level(const DevTexture& tex, float u, float v, int32_t mipIn, bool linear) {
+    float su = u * tex.scale_lerp_u, sv = v * tex.scale_lerp_v;
+    int32_t ix = cvt_rne(su) & tex.mask_lerp_u;
+    int32_t iy = cvt_rne(sv) & tex.mask_lerp_v;
+    int32_t m = min(max(mipIn, 0), (int32_t)tex.mip_levels - 1);
+    int32_t offset = (int32_t)tex.mip_offsets[m];
+    int32_t stride = (int32_t)tex.row_shift - m;
+    int32_t x = ix >> m, y = iy >> m;
+    const uint32_t* D = tex.data;
+    if (!linear) return D[offset + (int32_t)texel_offset(x >> 8, y >> 8, stride)];
+    int32_t xf = max(x - 127, 0), yf = max(y - 127, 0);
+    int32_t tx = xf >> 8, ty = yf >> 8;
+    bool inX = ((tx + 1) << m) < (int32_t)tex.width;
+    bool inY = ((ty + 1) << m) < (int32_t)tex.height;
+    int32_t i00 = offset + (int32_t)texel_offset(tx, ty, stride);
+    int32_t i01 = offset + (int32_t)texel_offset(tx, ty + (inY ? 1 : 0), stride);
+    uint32_t d00 = D[i00], d10 = D[i00 + 8], d01 = D[i01], d11 = D[i01 + 8];
+    uint32_t fx = (uint32_t)(xf & 255) << 7, fy = (uint32_t)(yf & 255) << 7;
+    fx = (fx << 16) | fx;
+    fy = (fy << 16) | fy;
+    if (!inX) fx = 0;
+    uint32_t rb1 = lerp16(d00 & 0x00FF00FFu, d10 & 0x00FF00FFu, fx);
+    uint32_t ga1 = lerp16((d00 >> 8) & 0x00FF00FFu, (d10 >> 8) & 0x00FF00FFu, fx);
+    uint32_t rb2 = lerp16(d01 & 0x00FF00FFu, d11 & 0x00FF00FFu, fx);
+    uint32_t ga2 = lerp16((d01 >> 8) & 0x00FF00FFu, (d11 >> 8) & 0x00FF00FFu, fx);
+    return lerp16(rb1, rb2, fy) | (lerp16(ga1, ga2, fy) << 8);
+}
+
+__device__ __forceinline__ uint32_t pack_rgba8(float r, float g, float b, float a) {  // Texture.h:55-67
+    float c[4] = { r, g, b, a };
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int32_t i = cvt_rne(c[k] * 255.0f);
+        i = min(max(i, -32768), 32767);
+        i = min(max(i, 0), 255);
+        out |= (uint32_t)i << (8 * k);
+    }
+    return out;
+}
+
+__device__ __forceinline__ void unpack_normal(const ApproxTables& T, uint32_t packed, float n[3]) {
+    const float scale = 1.0f / 255;
+    float u = (float)(packed & 255) * scale, v = (float)((packed >> 8) & 255) * scale;
+    u = u * 2.0f - 1.0f;
+    v = v * 2.0f - 1.0f;
+    n[0] = u;
+    n[1] = v;
+    n[2] = (1.0f - fabsf(u)) - fabsf(v);
+    float t = fmaxf(-n[2], 0.0f);
+    n[0] = n[0] - mulsign(t, n[0]);
+    n[1] = n[1] - mulsign(t, n[1]);
+    normalize3(T, n);
+}
+
+__device__ __forceinline__ void unpack_srgb(uint32_t packed, float c[3]) {  // Texture.h:37-54
+    uint32_t rb1 = ((packed << 8) & 0xFF00FF00u) + 0x00FF00FFu;
+    uint32_t ag1 = (packed & 0xFF00FF00u) + 0x00FF00FFu;
+    uint32_t rlo = rb1 & 0xFFFFu, rhi = rb1 >> 16, glo = ag1 & 0xFFFFu;
+    const float scale = 1.0f / 65535;
+    c[0] = (float)((rlo * rlo) >> 16) * scale;
+    c[1] = (float)((glo * glo) >> 16) * scale;
+    c[2] = (float)((rhi * rhi) >> 16) * scale;
+}
+
+__device__ float light_attenuation(const ApproxTables& T, const ResolveLight& L, const float pos[3]) {
+    if (L.type == 0) return 1.0f;
+    float d[3] = { L.position[0] - pos[0], L.position[1] - pos[1], L.position[2] - pos[2] };
+    float d2 = dot3(d, d);
+    float factor = d2 * L.inv_radius_sq;
+    float smooth = fmaxf(1.0f - factor * factor, 0.0f);
+    float att = (smooth * smooth) * rcp14(T, fmaxf(d2, 1e-4f));
+    if (L.type == 2) {
+        float nd[3] = { d[0], d[1], d[2] };
+        normalize3(T, nd);
+        float negDir[3] = { -L.direction[0], -L.direction[1], -L.direction[2] };
+        float cd = dot3(negDir, nd);
+        float sa = fminf(fmaxf(cd * L.spot_scale + L.spot_offset, 0.0f), 1.0f);
+        att *= sa * sa;
+    }
+    return att;
+}
+
+__global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t W = (uint32_t)p.width, H = (uint32_t)p.height;
+    const uint32_t tilesX = W / 4;
+    const uint32_t tile = t >> 4, q = t & 15u;
+    const uint32_t tx = tile % tilesX, ty = tile / tilesX;
+    const bool inside = ty * 4 < H;
+    const uint32_t x = tx * 4 + (q & 3u), y = ty * 4 + (q >> 2);
+    const uint32_t lane = lane_id();
+    const unsigned long long groupMask = 0xFFFFull << (lane & 48u);
+    const ApproxTables& T = p.tables;
+
+    unsigned long long key = inside ? p.keys[t] : p.clear_key;
+    const float depth = bitsf((uint32_t)(key >> 32));
+    const bool active = inside && !(depth <= 0.0f);
+
+    // surface of the pixel
+    float uv0[2] = { 0, 0 }, gradU[2] = { 0, 0 }, gradV[2] = { 0, 0 }, U = 0, V = 0, bary[3] = { 0, 0, 0 };
+    uint32_t material = 0xFFFFFFFFu, mi = 0, tri = 0;
+    if (active) {
+        uint32_t rank = ~(uint32_t)key;
+        uint32_t seq = rank >> 11;
+        tri = ((rank >> 8) & 7u) * 16u + ((rank >> 3) & 15u);
+        uint32_t lo = 0, hi = p.num_draws;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (p.draws[mid].seq_base <= seq) lo = mid; else hi = mid;
+        }
+        mi = p.draws[lo].meshlet_offset + (seq - p.draws[lo].seq_base);
+        const uint8_t* I = p.m.indices + (size_t)mi * 384;
+        const float* P = p.m.positions + (size_t)mi * 192;
+        float c[3][4], uv[3][2];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            uint32_t vi = I[k * 128 + tri];
+            float px = P[vi], py = P[64 + vi], pz = P[128 + vi];
+#pragma unroll
+            for (int j = 0; j < 4; j++) c[k][j] = xform(p.object_to_clip, j, px, py, pz);
+            uint32_t tc = p.m.texcoords[(size_t)mi * 64 + vi];
+            uv[k][0] = half_to_float(tc & 0xFFFFu);
+            uv[k][1] = half_to_float(tc >> 16);
+        }
+        material = p.m.material_id[mi];
+        const float su = __builtin_fmaf((float)(int32_t)x, p.scale_x, p.center_x);
+        const float sv = __builtin_fmaf((float)(int32_t)y, p.scale_y, p.center_y);
+        float ddx[3], ddy[3];
+        intersect_triangle(T, c, su, sv, p.kx, p.ky, bary, ddx, ddy);
+        float t10x = uv[1][0] - uv[0][0], t10y = uv[1][1] - uv[0][1];
+        float t20x = uv[2][0] - uv[0][0], t20y = uv[2][1] - uv[0][1];
+        U = __builtin_fmaf(bary[2], t20x, __builtin_fmaf(bary[1], t10x, uv[0][0]));
+        V = __builtin_fmaf(bary[2], t20y, __builtin_fmaf(bary[1], t10y, uv[0][1]));
+        gradU[0] = __builtin_fmaf(t20x, ddx[2], t10x * ddx[1]);
+        gradV[0] = __builtin_fmaf(t20y, ddx[2], t10y * ddx[1]);
+        gradU[1] = __builtin_fmaf(t20x, ddy[2], t10x * ddy[1]);
+        gradV[1] = __builtin_fmaf(t20y, ddy[2], t10y * ddy[1]);
+        (void)uv0;
+    }
+
+    // material waterfall per 4x4 tile (Shading.cpp:532-545)
+    uint32_t albedo = 0;
+    bool done = !active || material == 0xFFFFFFFFu || material >= p.num_materials;
+    for (;;) {
+        const unsigned long long todo = __ballot(!done);
+        if (!todo) break;
+        const unsigned long long mine = todo & groupMask;
+        const uint32_t leader = mine ? (uint32_t)__ffsll((long long)mine) - 1 : lane;
+        const uint32_t lmat = __shfl(material, (int)leader);
+        int32_t texIdx = -1;
+        if (mine) texIdx = p.materials[lmat].texture;
+        int32_t mip = -64;
+        DevTexture tex;
+        if (texIdx >= 0) {
+            tex = p.textures[texIdx];
+            float ssu = tex.scale_u * tex.scale_u, ssv = tex.scale_v * tex.scale_v;
+            float dx = __builtin_fmaf(gradU[0], gradU[0], gradV[0] * gradV[0]) * ssu;
+            float dy = __builtin_fmaf(gradU[1], gradU[1], gradV[1] * gradV[1]) * ssv;
+            float mx = (dx > dy) ? dx : dy;
+            mip = (int32_t)(fbits(mx) - (127u << 23)) >> 24;
+        }
+        const unsigned long long vote = __ballot(active && mip > 0);
+        const bool minify = (vote & groupMask) != 0;
+        if (!done && material == lmat) {
+            // SurfaceSampler: Repeat, Mag Linear, Min Nearest (Shading.cpp:6-10)
+            if (texIdx >= 0) albedo = sample_level(tex, U, V, mip, !minify);
+            done = true;
+        }
+    }
+
+    uint32_t outc;
+    if (p.mode == 0) {
+        // ResolveDebug(BaseColor) (Shading.cpp:744-771)
+        const uint32_t x0 = tx * 4, y0 = ty * 4;
+        uint32_t bg = ((x0 ^ y0) & 4u) ? 0xFFA0A0A0u : 0xFFFFFFFFu;
+        const float s = 1.0f / 255;
+        outc = active ? pack_rgba8((float)(albedo & 255) * s, (float)((albedo >> 8) & 255) * s,
+                                   (float)((albedo >> 16) & 255) * s, 1.0f)
+                      : bg;
+    } else {
+        float color[3] = { 0, 0, 0 };
+        float N[3] = { 0, 0, 0 }, Vd[3] = { 0, 0, 0 }, wpos[3] = { 0, 0, 0 }, base[3] = { 0, 0, 0 };
+        float f0[3] = { 0, 0, 0 }, diffuse[3] = { 0, 0, 0 }, NoV = 0, aR = 1e-4f;
+        if (active) {
+            const uint8_t* I = p.m.indices + (size_t)mi * 384;
+            float nv[3][3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                uint32_t vi = I[k * 128 + tri];
+                unpack_normal(T, p.m.normal_tangents[(size_t)mi * 64 + vi], nv[k]);
+            }
+            float ni[3];
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                ni[j] = __builtin_fmaf(nv[0][j], bary[0], __builtin_fmaf(nv[1][j], bary[1], nv[2][j] * bary[2]));
+            const float* M = p.object_to_world;
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                N[j] = __builtin_fmaf(ni[0], M[j], __builtin_fmaf(ni[1], M[3 + j], ni[2] * M[6 + j]));
+            normalize3(T, N);
+            float c4[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) c4[j] = xform(p.inv_screen_proj, j, (float)(int32_t)x, (float)(int32_t)y, depth);
+            float rw = frcp(T, c4[3]);
+            wpos[0] = c4[0] * rw; wpos[1] = c4[1] * rw; wpos[2] = c4[2] * rw;
+            unpack_srgb(albedo, base);
+            const float metal = 0.0f, rough = 0.0f;
+            aR = fmaxf(rough * rough, 1e-4f);
+            const float refl = 0.16f * 0.5f * 0.5f;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                f0[k] = lerpf(refl, base[k], metal);
+                diffuse[k] = base[k] * (1.0f - metal);
+                Vd[k] = p.view_pos[k] - wpos[k];
+            }
+            normalize3(T, Vd);
+            NoV = fabsf(dot3(N, Vd)) + 1e-5f;
+        }
+        for (uint32_t li = 0; li < p.num_lights; li++) {
+            const ResolveLight& L = p.lights[li];
+            float Ld[3] = { 0, 0, 0 }, NoL = 0, att = 0;
+            if (active) {
+                if (L.type == 0) {
+                    Ld[0] = -L.direction[0]; Ld[1] = -L.direction[1]; Ld[2] = -L.direction[2];
+                } else {
+                    Ld[0] = L.position[0] - wpos[0]; Ld[1] = L.position[1] - wpos[1]; Ld[2] = L.position[2] - wpos[2];
+                    normalize3(T, Ld);
+                }
+                NoL = dot3(N, Ld);
+            }
+            if (!(__ballot(active && !(NoL < 1e-4f)) & groupMask)) continue;
+            if (active) att = light_attenuation(T, L, wpos) * L.intensity * p.light_exposure;
+            if (!(__ballot(active && !(NoL * att < 1e-4f)) & groupMask)) continue;
+            if (active) {
+                float Hv[3] = { Vd[0] + Ld[0], Vd[1] + Ld[1], Vd[2] + Ld[2] };
+                normalize3(T, Hv);
+                float NoH = fminf(fmaxf(dot3(N, Hv), 0.0f), 1.0f);
+                float LoH = fminf(fmaxf(dot3(Ld, Hv), 0.0f), 1.0f);
+                float a = NoH * aR;
+                float kk = aR * rcp14(T, (1.0f - NoH * NoH) + a * a);
+                float D = kk * kk * 0.3183098861837907f;
+                float va = 2.0f * NoL * NoV;
+                float vb = NoL + NoV;
+                float Vis = fdiv(T, 0.5f, lerpf(va, vb, aR));
+                float u1 = 1.0f - LoH;
+                float f = (u1 * u1) * (u1 * u1) * u1;
+                float w = fmaxf(NoL * att, 0.0f);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    float F = f + f0[c] * (1.0f - f);
+                    float Fr = (D * Vis) * F;
+                    float Fd = diffuse[c] * 0.3183098861837907f;
+                    color[c] = color[c] + (Fd + Fr) * L.color[c] * w;
+                }
+            }
+        }
+        float cc[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            float v = active ? (color[k] + base[k] * 0.05f) * p.exposure : 0.0f;
+            cc[k] = fdiv(T, v, v + 0.155f) * 1.019f;
+        }
+        outc = pack_rgba8(cc[0], cc[1], cc[2], 1.0f);
+    }
+    if (inside) p.out[p.linear ? (size_t)y * W + x : (size_t)t] = outc;
+}
+
+}  // namespace
+
+hipError_t launch_resolve(const ResolveParams& p, hipStream_t s, Profiler* prof) {
+    uint32_t n = (uint32_t)p.width * (uint32_t)p.height;
+    Profiler::Scope sc(prof, p.mode == 0 ? "resolve_albedo" : "resolve_lit", s);
+    hipLaunchKernelGGL(k_resolve, dim3((n + 255) / 256), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+}  // namespace glimp

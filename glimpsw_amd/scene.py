@@ -287,3 +287,52 @@ def config1(num_meshlets: int = 10240, width: int = 1920, height: int = 1080, te
     mid = (np.arange(n) % num_materials).astype(np.uint32)
     scene = build_patches(centers, R, scales, mid, mats, texs)
     return scene, make_view(width, height, count=n)
+
+
+def inverse_screen_proj(world_to_clip: np.ndarray, width: int, height: int) -> np.ndarray:
+    """GetInverseScreenProjMatrix (Camera.h:140-146): inverse(M) then
+    translate(-1,-1,0), scale(2/size, 1), translate(0.5, 0.5, 0), so pixel
+    coordinates (x, y, depth, 1) map to world space. float64, rounded once."""
+    M = world_to_clip.astype(np.float64).reshape(4, 4).T
+    inv = np.linalg.inv(M)
+    T1 = np.eye(4); T1[:2, 3] = -1.0
+    Sc = np.eye(4); Sc[0, 0] = 2.0 / width; Sc[1, 1] = 2.0 / height
+    T2 = np.eye(4); T2[:2, 3] = 0.5
+    R = inv @ T1 @ Sc @ T2
+    return R.T.reshape(16).astype(np.float32)
+
+
+@dataclasses.dataclass
+class Light:
+    type: int                      # 0 directional, 1 point, 2 spot
+    position: tuple = (0.0, 0.0, 0.0)
+    direction: tuple = (0.0, -1.0, 0.0)
+    color: tuple = (1.0, 1.0, 1.0)
+    intensity: float = 1000.0
+    radius: float = 0.0
+
+    @property
+    def inv_radius_sq(self) -> float:
+        r = self.radius if self.radius > 0 else 1e6   # Light::SetRadius (Scene.h:65-68)
+        return float(np.float32(1.0) / (np.float32(r) * np.float32(r)))
+
+
+@dataclasses.dataclass
+class Lighting:
+    lights: list
+    exposure: float
+    view_pos: tuple
+    object_to_world: np.ndarray    # (9,) mat3 column-major
+    inv_screen_proj: np.ndarray    # (16,)
+
+
+def default_lighting(view: View) -> Lighting:
+    """A sun and a point light near the camera; exposure 1."""
+    d = np.array([-0.3, -1.0, -0.5])
+    d = d / np.linalg.norm(d)
+    lights = [Light(0, direction=tuple(d.astype(np.float32).tolist()), color=(1.0, 0.95, 0.9), intensity=3000.0),
+              Light(1, position=(1.5, 1.0, -4.0), color=(0.6, 0.7, 1.0), intensity=4000.0, radius=30.0)]
+    Vinv = np.linalg.inv(view.view.astype(np.float64).reshape(4, 4).T)
+    eye = tuple(Vinv[:3, 3].astype(np.float32).tolist())
+    return Lighting(lights, 1.0, eye, np.eye(3, dtype=np.float32).reshape(9),
+                    inverse_screen_proj(view.object_to_clip, view.width, view.height))

@@ -158,6 +158,44 @@ int glimp_draw_meshlets(glimp_context* ctx, glimp_framebuffer* fb, glimp_scene* 
 int glimp_cull_meshlets(glimp_context* ctx, glimp_scene* scene, uint32_t first, uint32_t count,
                         const float planes[20], uint16_t* bitmap, uint32_t* visible, void* stream);
 
+/* ------------------------------------------------------------ resolve */
+
+#define GLIMP_RESOLVE_ALBEDO 0u   /* ResolveDebug(DebugLayer::BaseColor) (Shading.cpp:734-773) */
+#define GLIMP_RESOLVE_LIT 1u      /* Resolve (Shading.cpp:658-689), without skybox and light sprites */
+#define GLIMP_MAX_LIGHTS 8u
+
+/* Light (Scene.h:52-75) with its precomputed terms. */
+typedef struct glimp_light {
+    uint32_t type;                     /* 0 directional, 1 point, 2 spot */
+    float position[3];
+    float direction[3];
+    float color[3];
+    float intensity;
+    float inv_radius_sq, spot_scale, spot_offset;
+} glimp_light;
+
+typedef struct glimp_resolve_desc {
+    uint32_t mode;                     /* GLIMP_RESOLVE_* */
+    float object_to_clip[16];          /* ShadingContext::ObjectToClipMat */
+    float object_to_world[9];          /* ShadingContext::ObjectToWorldMat (mat3, column-major) */
+    float inv_screen_proj[16];         /* GetInverseScreenProjMatrix(WorldToClipMat, size) (Camera.h:140-146) */
+    float view_pos[3];                 /* ShadingContext::ViewPos */
+    float exposure;                    /* ShadingContext::Exposure */
+    const glimp_light* lights;         /* host array (ShadingContext::Lights) */
+    uint32_t num_lights;               /* <= GLIMP_MAX_LIGHTS */
+} glimp_resolve_desc;
+
+/* ShadingContext::Resolve / ResolveDebug(BaseColor) (Shading.cpp:658-773):
+ * writes RGBA8 per pixel into the framebuffer's color buffer (row-major),
+ * which glimp_framebuffer_color / glimp_framebuffer_read_color expose.
+ * Asynchronous. */
+int glimp_resolve(glimp_context* ctx, glimp_framebuffer* fb, glimp_scene* scene, const glimp_resolve_desc* desc,
+                  void* stream);
+/* Device pointer to the resolved RGBA8 buffer (width*height u32, row-major). */
+int glimp_framebuffer_color(glimp_framebuffer* fb, uint32_t** color);
+/* Copies the resolved RGBA8 buffer to host memory (row-major). Synchronous. */
+int glimp_framebuffer_read_color(glimp_framebuffer* fb, uint32_t* rgba, void* stream);
+
 /* Perf counters (Rasterizer.h:366-379), accumulated since the last reset. */
 typedef struct glimp_counters {
     uint64_t triangles_processed;      /* PerfCounter::TrianglesProcessed */

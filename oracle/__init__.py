@@ -77,7 +77,7 @@ class Light(C.Structure):
 
 
 class ResolveParams(C.Structure):
-    _fields_ = [("world_to_clip", C.c_float * 16), ("object_to_clip", C.c_float * 16),
+    _fields_ = [("inv_screen_proj", C.c_float * 16), ("object_to_clip", C.c_float * 16),
                 ("object_to_world", C.c_float * 9), ("view_pos", C.c_float * 3), ("exposure", C.c_float),
                 ("lights", C.POINTER(Light)), ("num_lights", C.c_uint32)]
 
@@ -230,3 +230,24 @@ def render_frame(scene, view, binding=None, cull_first: bool = True, clipping: b
     ctr, _ = draw(fb, binding, view.object_to_clip, view.meshlet_count, view.meshlet_offset, bitmap,
                   clipping=clipping)
     return fb, ctr, binding
+
+
+def resolve_lit(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip, lighting, meshlet_offset: int = 0):
+    """Resolve (lit, tonemapped) into layer 0; returns row-major RGBA8."""
+    from glimpsw_amd import light_structs
+
+    ctx = binding.ctx(object_to_clip, meshlet_offset)
+    rp = ResolveParams()
+    for i in range(16):
+        rp.inv_screen_proj[i] = float(lighting.inv_screen_proj[i])
+        rp.object_to_clip[i] = float(object_to_clip[i])
+    for i in range(9):
+        rp.object_to_world[i] = float(lighting.object_to_world[i])
+    for i in range(3):
+        rp.view_pos[i] = float(lighting.view_pos[i])
+    rp.exposure = lighting.exposure
+    lights = light_structs(lighting.lights, Light)
+    rp.lights = C.cast(lights, C.POINTER(Light))
+    rp.num_lights = len(lighting.lights)
+    lib().orc_resolve_lit(C.byref(fb.fb), C.byref(ctx), C.byref(rp))
+    return fb.pixels(0)

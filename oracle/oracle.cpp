@@ -562,9 +562,48 @@ struct SurfaceLane {
 // The tile-wide filter choice is taken over the tile's covered lanes; the
 // reference also lets uncovered (sky) lanes vote with register contents left
 // over from the waterfall loop, which this restatement leaves out.
+// RGBA8u::Unpack (Texture.h:28-36).
+void unpack_rgba8(uint32_t p, float c[4]) {
+    const float scale = 1.0f / 255;
+    for (int k = 0; k < 4; k++) c[k] = (float)((p >> (8 * k)) & 255) * scale;
+}
+
+inline float dot3(const float a[3], const float b[3]) {  // SIMD.h:429
+    return fmaf(a[0], b[0], fmaf(a[1], b[1], a[2] * b[2]));
+}
+
+// normalize via VRSQRT14 (SIMD.h:439)
+void normalize3(float v[3]) {
+    float r = rsqrt14(dot3(v, v));
+    for (int k = 0; k < 3; k++) v[k] = v[k] * r;
+}
+
+// y < 0 ? -x : x by sign bit (SIMD.h:341-344)
+inline float mulsign(float x, float y) { return bitsf(fbits(x) ^ (fbits(y) & 0x80000000u)); }
+
+// UnmapOctahedron (Texture.h:289-296).
+void unmap_octahedron(float u, float v, float n[3]) {
+    u = u * 2.0f - 1.0f;
+    v = v * 2.0f - 1.0f;
+    n[0] = u;
+    n[1] = v;
+    n[2] = (1.0f - fabsf(u)) - fabsf(v);
+    float t = std::max(-n[2], 0.0f);
+    n[0] = n[0] - mulsign(t, n[0]);
+    n[1] = n[1] - mulsign(t, n[1]);
+    normalize3(n);
+}
+
+// UnpackNormalTangent (Shading.cpp:232-236), normal part.
+void unpack_normal(uint32_t packed, float n[3]) {
+    float c[4];
+    unpack_rgba8(packed, c);
+    unmap_octahedron(c[0], c[1], n);
+}
+
 void resolve_surface_tile(const orc_shading_ctx& ctx, const uint32_t* ids, const bool* active,
                           const float* suv_x, const float* suv_y, float sw, float sh,
-                          SurfaceLane* lanes) {
+                          SurfaceLane* lanes, const float* object_to_world = nullptr) {
     const orc_meshlets& ms = *ctx.meshlets;
     float gradU[16][2], gradV[16][2];  // (dx, dy) of u and v
     for (int l = 0; l < 16; l++) {
@@ -591,6 +630,20 @@ void resolve_surface_tile(const orc_shading_ctx& ctx, const uint32_t* ids, const
         float bary[3], ddx[3], ddy[3];
         intersect_triangle(c, suv_x[l], suv_y[l], sw, sh, bary, ddx, ddy);
         for (int k = 0; k < 3; k++) L.bary[k] = bary[k];
+        if (object_to_world) {
+            // normalWS = normalize(mul(ObjectToWorldMat, BaryLerp(bary, n0, n1, n2)))
+            // (Shading.cpp:546-552, BaryLerp Rasterizer.h:101-104)
+            float nv[3][3];
+            for (int k = 0; k < 3; k++) {
+                uint32_t vi = ms.indices[(size_t)mi * 384 + k * 128 + tri];
+                unpack_normal(ms.normal_tangents[(size_t)mi * 64 + vi], nv[k]);
+            }
+            float ni[3];
+            for (int j = 0; j < 3; j++) ni[j] = fmaf(nv[0][j], bary[0], fmaf(nv[1][j], bary[1], nv[2][j] * bary[2]));
+            const float* M = object_to_world;
+            for (int j = 0; j < 3; j++) L.nrm[j] = fmaf(ni[0], M[0 * 3 + j], fmaf(ni[1], M[1 * 3 + j], ni[2] * M[2 * 3 + j]));
+            normalize3(L.nrm);
+        }
         float t10x = uv[1][0] - uv[0][0], t10y = uv[1][1] - uv[0][1];
         float t20x = uv[2][0] - uv[0][0], t20y = uv[2][1] - uv[0][1];
         L.u = fmaf(bary[2], t20x, fmaf(bary[1], t10x, uv[0][0]));
@@ -633,6 +686,128 @@ void resolve_surface_tile(const orc_shading_ctx& ctx, const uint32_t* ids, const
                 done[l] = true;
             }
         }
+    }
+}
+
+// RGBA8u::UnpackSrgb (Texture.h:37-54): squares the byte value with a
+// 16-bit high multiply; alpha keeps the linear value.
+void unpack_srgb(uint32_t packed, float c[4]) {
+    uint32_t rb1 = ((packed << 8) & 0xFF00FF00u) + 0x00FF00FFu;
+    uint32_t ag1 = (packed & 0xFF00FF00u) + 0x00FF00FFu;
+    auto mulhi16 = [](uint32_t a) {
+        uint32_t lo = a & 0xFFFFu, hi = a >> 16;
+        return ((lo * lo) >> 16) | (((hi * hi) >> 16) << 16);
+    };
+    uint32_t rb2 = mulhi16(rb1), ag2 = mulhi16(ag1);
+    const float scale = 1.0f / 65535;
+    c[0] = (float)(rb2 & 65535) * scale;
+    c[1] = (float)(ag2 & 65535) * scale;
+    c[2] = (float)(rb2 >> 16) * scale;
+    c[3] = (float)(ag1 >> 16) * scale;
+}
+
+inline float lerpf(float a, float b, float t) { return fmaf(t, b, fmaf(-t, a, a)); }  // SIMD.h:446
+
+float light_attenuation(const orc_light& L, const float pos[3]) {  // Shading.cpp:581-600
+    if (L.type == 0) return 1.0f;
+    float d[3] = { L.position[0] - pos[0], L.position[1] - pos[1], L.position[2] - pos[2] };
+    float d2 = dot3(d, d);
+    float factor = d2 * L.inv_radius_sq;
+    float smooth = std::max(1.0f - factor * factor, 0.0f);
+    float att = (smooth * smooth) * rcp14(std::max(d2, 1e-4f));
+    if (L.type == 2) {
+        float nd[3] = { d[0], d[1], d[2] };
+        normalize3(nd);
+        float negDir[3] = { -L.direction[0], -L.direction[1], -L.direction[2] };
+        float cd = dot3(negDir, nd);
+        float sa = std::min(std::max(cd * L.spot_scale + L.spot_offset, 0.0f), 1.0f);
+        att *= sa * sa;
+    }
+    return att;
+}
+
+// EvalLighting for one 4x4 tile (Shading.cpp:602-645): Lambert diffuse,
+// GGX/Smith/Schlick specular, 5% ambient. The per-light skips are taken over
+// the tile's covered lanes, as the reference takes them over its 16 lanes.
+void lit_tile(const SurfaceLane* lanes, const bool* active, uint32_t x0, uint32_t y0, const float* tileDepth,
+              const orc_resolve_params* p, float exposure, float color[16][3]) {
+    float base[16][3], f0[16][3], diffuse[16][3], N[16][3], V[16][3], NoV[16], aR[16], metal[16];
+    float wpos[16][3];
+    for (int l = 0; l < 16; l++) {
+        if (!active[l]) continue;
+        // world position from the inverse screen projection (Shading.cpp:659-667)
+        float px = (float)(int32_t)(x0 + (l & 3)), py = (float)(int32_t)(y0 + (l >> 2));
+        float c[4];
+        transform(p->inv_screen_proj, px, py, tileDepth[l], c);
+        float rw = frcp(c[3]);
+        wpos[l][0] = c[0] * rw; wpos[l][1] = c[1] * rw; wpos[l][2] = c[2] * rw;
+        float srgb[4];
+        unpack_srgb(lanes[l].albedo, srgb);
+        metal[l] = 0.0f;                       // no metallic/roughness layer
+        float rough = 0.0f;
+        aR[l] = std::max(rough * rough, 1e-4f);
+        const float refl = 0.16f * 0.5f * 0.5f;
+        for (int k = 0; k < 3; k++) {
+            base[l][k] = srgb[k];
+            f0[l][k] = lerpf(refl, base[l][k], metal[l]);
+            diffuse[l][k] = base[l][k] * (1.0f - metal[l]);
+            N[l][k] = lanes[l].nrm[k];
+            V[l][k] = p->view_pos[k] - wpos[l][k];
+        }
+        normalize3(V[l]);
+        NoV[l] = fabsf(dot3(N[l], V[l])) + 1e-5f;
+        for (int k = 0; k < 3; k++) color[l][k] = 0.0f;
+    }
+    for (uint32_t li = 0; li < p->num_lights; li++) {
+        const orc_light& L = p->lights[li];
+        float Ld[16][3], NoL[16], att[16];
+        bool allLow = true;
+        for (int l = 0; l < 16; l++) {
+            if (!active[l]) continue;
+            if (L.type == 0) {
+                for (int k = 0; k < 3; k++) Ld[l][k] = -L.direction[k];
+            } else {
+                for (int k = 0; k < 3; k++) Ld[l][k] = L.position[k] - wpos[l][k];
+                normalize3(Ld[l]);
+            }
+            NoL[l] = dot3(N[l], Ld[l]);
+            allLow &= NoL[l] < 1e-4f;
+        }
+        if (allLow) continue;
+        bool allDark = true;
+        for (int l = 0; l < 16; l++) {
+            if (!active[l]) continue;
+            att[l] = light_attenuation(L, wpos[l]) * L.intensity * exposure;
+            allDark &= NoL[l] * att[l] < 1e-4f;
+        }
+        if (allDark) continue;
+        for (int l = 0; l < 16; l++) {
+            if (!active[l]) continue;
+            float Hv[3] = { V[l][0] + Ld[l][0], V[l][1] + Ld[l][1], V[l][2] + Ld[l][2] };
+            normalize3(Hv);
+            float NoH = std::min(std::max(dot3(N[l], Hv), 0.0f), 1.0f);
+            float LoH = std::min(std::max(dot3(Ld[l], Hv), 0.0f), 1.0f);
+            // D_GGX, V_SmithGGXCorrelatedFast, F_Schlick (Shading.cpp:19-32)
+            float a = NoH * aR[l];
+            float k = aR[l] * rcp14((1.0f - NoH * NoH) + a * a);
+            float D = k * k * 0.3183098861837907f;
+            float va = 2.0f * NoL[l] * NoV[l];
+            float vb = NoL[l] + NoV[l];
+            float Vis = fdiv(0.5f, lerpf(va, vb, aR[l]));
+            float u1 = 1.0f - LoH;
+            float f = (u1 * u1) * (u1 * u1) * u1;
+            float w = std::max(NoL[l] * att[l], 0.0f);
+            for (int c = 0; c < 3; c++) {
+                float F = f + f0[l][c] * (1.0f - f);
+                float Fr = (D * Vis) * F;
+                float Fd = diffuse[l][c] * 0.3183098861837907f;
+                color[l][c] = color[l][c] + (Fd + Fr) * L.color[c] * w;
+            }
+        }
+    }
+    for (int l = 0; l < 16; l++) {
+        if (!active[l]) continue;
+        for (int k = 0; k < 3; k++) color[l][k] = color[l][k] + base[l][k] * 0.05f;
     }
 }
 
@@ -932,8 +1107,49 @@ void orc_resolve_albedo(orc_framebuffer* fb, const orc_shading_ctx* ctx) {
     memcpy(fb->data, out.data(), (size_t)W * H * 4);
 }
 
+// Resolve (Shading.cpp:658-689) without skybox and light sprites: covered
+// pixels get EvalLighting of their surface, uncovered pixels are black.
 void orc_resolve_lit(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_resolve_params* p) {
-    (void)fb; (void)ctx; (void)p;  // filled in with the lit resolve
+    uint32_t W = fb->width, H = fb->height;
+    float sw = (float)W, sh = (float)H;
+    float scaleX = 2.0f / sw, scaleY = 2.0f / sh;
+    float centerX = fmaf(0.5f, scaleX, -1.0f), centerY = fmaf(0.5f, scaleY, -1.0f);
+    const float* depth = (const float*)(fb->data + fb->layer_stride);
+    std::vector<uint32_t> out((size_t)W * H);
+    const float exposure = p->exposure;
+    const float lightExposure = p->exposure * 0.001f;
+    for (uint32_t y0 = 0; y0 < H; y0 += 4) {
+        for (uint32_t x0 = 0; x0 < W; x0 += 4) {
+            uint32_t off = pixel_offset(W, x0, y0);
+            bool active[16];
+            float sx[16], sy[16];
+            uint32_t ids[16];
+            bool anyActive = false;
+            for (int l = 0; l < 16; l++) {
+                active[l] = !(depth[off + l] <= 0.0f);
+                anyActive |= active[l];
+                ids[l] = fb->data[off + l];
+                sx[l] = fmaf((float)(int32_t)(x0 + (l & 3)), scaleX, centerX);
+                sy[l] = fmaf((float)(int32_t)(y0 + (l >> 2)), scaleY, centerY);
+            }
+            float color[16][3] = {};
+            if (anyActive) {
+                SurfaceLane lanes[16];
+                resolve_surface_tile(*ctx, ids, active, sx, sy, sw, sh, lanes, p->object_to_world);
+                lit_tile(lanes, active, x0, y0, depth + off, p, lightExposure, color);
+            }
+            for (int l = 0; l < 16; l++) {
+                float c[3];
+                for (int k = 0; k < 3; k++) {
+                    // Tonemap_Unreal(x * Exposure) (Shading.cpp:221-226, 680)
+                    float x = active[l] ? color[l][k] * exposure : 0.0f;
+                    c[k] = fdiv(x, x + 0.155f) * 1.019f;
+                }
+                out[(size_t)off + l] = pack_rgba8(c[0], c[1], c[2], 1.0f);
+            }
+        }
+    }
+    memcpy(fb->data, out.data(), (size_t)W * H * 4);
 }
 
 }  // extern "C"

@@ -89,7 +89,7 @@ typedef struct orc_light {
 } orc_light;
 
 typedef struct orc_resolve_params {
-    float world_to_clip[16];          /* for the inverse screen projection */
+    float inv_screen_proj[16];        /* GetInverseScreenProjMatrix(WorldToClipMat, size) (Camera.h:140-146) */
     float object_to_clip[16];
     float object_to_world[9];         /* mat3 column-major */
     float view_pos[3];
