@@ -55,6 +55,19 @@ class DrawDesc(C.Structure):
                 ("pipeline", C.c_uint32), ("bin_size", C.c_uint32), ("bin_capacity", C.c_uint32)]
 
 
+class ViewDesc(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("framebuffer", C.c_void_p), ("meshlet_offset", C.c_uint32),
+                ("meshlet_count", C.c_uint32), ("object_to_clip", C.c_float * 16), ("frustum_planes", f32p),
+                ("flags", C.c_uint32), ("clear_color", C.c_uint32), ("clear_depth", C.c_float)]
+
+
+class BatchShape(C.Structure):
+    _fields_ = [("bin_size", C.c_uint32), ("bin_capacity", C.c_uint32), ("raster_threads", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class CountersStruct(C.Structure):
     _fields_ = [("triangles_processed", C.c_uint64), ("triangles_rasterized", C.c_uint64),
                 ("triangles_clipped", C.c_uint64), ("bins_rasterized", C.c_uint64),
@@ -125,6 +138,13 @@ _SIGNATURES = {
     "glimp_resolve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(ResolveDesc), C.c_void_p]),
     "glimp_framebuffer_color": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "glimp_framebuffer_read_color": (C.c_int, [C.c_void_p, u32p, C.c_void_p]),
+    "glimp_batch_create": (C.c_int, [C.c_void_p, C.POINTER(ViewDesc), C.c_uint32, C.POINTER(BatchShape),
+                                     C.POINTER(C.c_void_p)]),
+    "glimp_batch_destroy": (None, [C.c_void_p]),
+    "glimp_batch_set_shape": (C.c_int, [C.c_void_p, C.POINTER(BatchShape)]),
+    "glimp_batch_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "glimp_batch_launch_per_view": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "glimp_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BatchShape), C.POINTER(C.c_float)]),
     "glimp_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "glimp_profile_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_uint32, C.POINTER(C.c_uint32), C.c_int]),
 }
@@ -367,3 +387,70 @@ def render_frame(ctx: Context, dscene: Scene, view, fb: Framebuffer | None = Non
     planes = view.frustum_planes() if cull else None
     fb.draw(dscene, view.object_to_clip, view.meshlet_count, view.meshlet_offset, planes, flags, stream)
     return fb
+
+
+class Batch:
+    """Many views rendered by one launch per pipeline stage, replayable as a
+    single hipGraph (glimp_batch_* in include/glimp.h). ``views`` is a list
+    of ``(scene, framebuffer, view)`` or ``(scene, framebuffer, view, planes,
+    flags)``; every view needs its own framebuffer, and scenes and
+    framebuffers must outlive the batch."""
+
+    def __init__(self, ctx: Context, views, shape: dict | None = None, clear_color: int = 0,
+                 clear_depth: float = 0.0):
+        self.ctx = ctx
+        n = len(views)
+        arr = (ViewDesc * max(1, n))()
+        self._keep = []
+        for i, v in enumerate(views):
+            sc, fb, vw = v[0], v[1], v[2]
+            planes = v[3] if len(v) > 3 else vw.frustum_planes()
+            flags = v[4] if len(v) > 4 else 0
+            d = arr[i]
+            d.scene, d.framebuffer = sc.handle, fb.handle
+            d.meshlet_offset, d.meshlet_count = vw.meshlet_offset, vw.meshlet_count
+            for k in range(16):
+                d.object_to_clip[k] = float(vw.object_to_clip[k])
+            if planes is not None:
+                pl = np.ascontiguousarray(planes, dtype=np.float32)
+                self._keep.append(pl)
+                d.frustum_planes = _ptr(pl, f32p)
+            d.flags = flags
+            d.clear_color, d.clear_depth = clear_color, clear_depth
+        sh = None
+        if shape is not None:
+            sh = BatchShape(shape["bin_size"], shape["bin_capacity"], shape.get("raster_threads", 256))
+        h = C.c_void_p()
+        _check(ctx.lib.glimp_batch_create(ctx.handle, arr, n, C.byref(sh) if sh else None, C.byref(h)),
+               "glimp_batch_create")
+        self.handle = h
+        self.framebuffers = [v[1] for v in views]
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.glimp_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_shape(self, bin_size: int, bin_capacity: int, raster_threads: int = 256):
+        sh = BatchShape(bin_size, bin_capacity, raster_threads)
+        _check(self.ctx.lib.glimp_batch_set_shape(self.handle, C.byref(sh)), "glimp_batch_set_shape")
+
+    def launch(self, graph: bool = True, stream=None):
+        _check(self.ctx.lib.glimp_batch_launch(self.handle, int(graph), _stream_ptr(stream)), "glimp_batch_launch")
+
+    def launch_per_view(self, stream=None):
+        _check(self.ctx.lib.glimp_batch_launch_per_view(self.handle, _stream_ptr(stream)),
+               "glimp_batch_launch_per_view")
+
+    def autotune(self, stream=None) -> tuple[dict, float]:
+        sh = BatchShape()
+        ms = C.c_float()
+        _check(self.ctx.lib.glimp_batch_autotune(self.handle, _stream_ptr(stream), C.byref(sh), C.byref(ms)),
+               "glimp_batch_autotune")
+        return sh.as_dict(), float(ms.value)

@@ -52,6 +52,44 @@ hipError_t upload(T** dst, const T* src, size_t count) {
     return hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice);
 }
 
+template <typename T>
+int grow_one(T*& buf, size_t& cap, size_t need) {
+    if (need <= cap) return GLIMP_OK;
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+    cap = 0;
+    HIP_TRY(dev_alloc(&buf, need));
+    cap = need;
+    return GLIMP_OK;
+}
+
+// Device scratch of one launch (owned by the context for single draws and
+// by the batch for batches, so a captured graph keeps valid pointers).
+struct Scratch {
+    TriRecord* records = nullptr;
+    size_t records_cap = 0;
+    ClipEntry* clip = nullptr;
+    size_t clip_cap = 0;
+    uint16_t* bitmap = nullptr;
+    size_t bitmap_cap = 0;
+    uint32_t* meshlet_counts = nullptr;
+    size_t meshlet_counts_cap = 0;
+    uint32_t* bin_counts = nullptr;
+    size_t bin_counts_cap = 0;
+    uint32_t* bin_lists = nullptr;
+    size_t bin_lists_cap = 0;
+    uint32_t* spill = nullptr;
+    size_t spill_cap = 0;
+    DrawScratch* scratch = nullptr;
+
+    void release() {
+        void* ptrs[] = { records, clip, bitmap, meshlet_counts, bin_counts, bin_lists, spill, scratch };
+        for (void* q : ptrs)
+            if (q) (void)hipFree(q);
+        *this = Scratch();
+    }
+};
+
 }  // namespace
 
 struct glimp_context {
@@ -60,20 +98,9 @@ struct glimp_context {
     Counters* counters = nullptr;
     DrawScratch* scratch = nullptr;
     uint32_t* visible = nullptr;
-    TriRecord* records = nullptr;
-    size_t records_cap = 0;
-    ClipEntry* clip = nullptr;
-    size_t clip_cap = 0;
-    uint16_t* bitmap = nullptr;
+    uint16_t* bitmap = nullptr;        // glimp_cull_meshlets output
     size_t bitmap_cap = 0;
-    uint32_t* bin_counts = nullptr;
-    size_t bin_counts_cap = 0;
-    uint32_t* bin_lists = nullptr;
-    size_t bin_lists_cap = 0;
-    uint32_t* spill = nullptr;
-    size_t spill_cap = 0;
-    uint32_t* meshlet_counts = nullptr;
-    size_t meshlet_counts_cap = 0;
+    Scratch draw_scratch;              // scratch of single draws
     Profiler prof;
 
     ApproxTables approx() const { return ApproxTables{ tables, tables + 65536 }; }
@@ -120,42 +147,6 @@ hipError_t flush_clear(glimp_framebuffer* f, hipStream_t s) {
     return e;
 }
 
-template <typename T>
-int grow_one(T*& buf, size_t& cap, size_t need) {
-    if (need <= cap) return GLIMP_OK;
-    if (buf) (void)hipFree(buf);
-    buf = nullptr;
-    cap = 0;
-    HIP_TRY(dev_alloc(&buf, need));
-    cap = need;
-    return GLIMP_OK;
-}
-
-int grow(glimp_context* c, size_t records, size_t clip, size_t bitmap) {
-    if (records > c->records_cap) {
-        if (c->records) (void)hipFree(c->records);
-        c->records = nullptr;
-        c->records_cap = 0;
-        HIP_TRY(dev_alloc(&c->records, records));
-        c->records_cap = records;
-    }
-    if (clip > c->clip_cap) {
-        if (c->clip) (void)hipFree(c->clip);
-        c->clip = nullptr;
-        c->clip_cap = 0;
-        HIP_TRY(dev_alloc(&c->clip, clip));
-        c->clip_cap = clip;
-    }
-    if (bitmap > c->bitmap_cap) {
-        if (c->bitmap) (void)hipFree(c->bitmap);
-        c->bitmap = nullptr;
-        c->bitmap_cap = 0;
-        HIP_TRY(dev_alloc(&c->bitmap, bitmap));
-        c->bitmap_cap = bitmap;
-    }
-    return GLIMP_OK;
-}
-
 }  // namespace
 
 extern "C" {
@@ -192,10 +183,10 @@ void glimp_destroy(glimp_context* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
-    void* ptrs[] = { c->tables, c->counters, c->scratch, c->visible, c->records, c->clip, c->bitmap,
-                     c->bin_counts, c->bin_lists, c->spill, c->meshlet_counts };
+    void* ptrs[] = { c->tables, c->counters, c->scratch, c->visible, c->bitmap };
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    c->draw_scratch.release();
     delete c;
 }
 
@@ -448,6 +439,105 @@ int glimp_framebuffer_read(glimp_framebuffer* f, uint32_t* ids, float* depth, in
     return GLIMP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+struct LaunchShape {
+    uint32_t bin_size = 16;
+    uint32_t bin_capacity = 1024;
+    uint32_t raster_threads = 256;
+};
+
+// Clipped triangles are rare (the reference measures < 0.1%, README.md:93);
+// the clip queue holds up to two per meshlet and every queued triangle may
+// produce 8 fan triangles. A fuller queue sets the sticky overflow flag,
+// which glimp_framebuffer_read reports.
+size_t clip_capacity_for(size_t meshlets) { return std::max<size_t>(4096, meshlets * 2); }
+
+// One view of a draw (ShadingContext + DrawMeshlets arguments).
+void make_view(glimp_framebuffer* f, glimp_scene* s, uint32_t meshlet_offset, uint32_t meshlet_count,
+               const float* clip, const float* planes, uint32_t flags, ViewParams& V) {
+    memset(&V, 0, sizeof(V));
+    V.m = s->m;
+    V.materials = s->materials;
+    V.num_materials = s->num_materials;
+    V.meshlet_offset = meshlet_offset;
+    V.meshlet_count = meshlet_count;
+    V.seq_base = f->next_seq;
+    V.width = (int)f->width;
+    V.height = (int)f->height;
+    bool guard = !(flags & GLIMP_DRAW_NO_GUARDBAND);
+    V.guard_x = guard ? (float)GLIMP_MAX_RENDER_SIZE / (float)f->width : 1.0f;
+    V.guard_y = guard ? (float)GLIMP_MAX_RENDER_SIZE / (float)f->height : 1.0f;
+    V.clipping = !(flags & GLIMP_DRAW_NO_CLIPPING);
+    V.use_cull = planes != nullptr;
+    if (planes) memcpy(V.planes, planes, sizeof(V.planes));
+    memcpy(V.clip, clip, sizeof(V.clip));
+    V.clear_depth = f->clear_depth;
+    V.fresh = f->pending_clear ? 1 : 0;
+    V.keys = f->keys;
+}
+
+// Lays out bins for `views` (bin_base, bins_x/y) and returns the total.
+uint32_t layout_bins(std::vector<ViewParams>& views, uint32_t bin_size) {
+    uint32_t base = 0;
+    for (auto& V : views) {
+        V.bin_base = base;
+        V.bins_x = ((uint32_t)V.width + bin_size - 1) / bin_size;
+        V.bins_y = ((uint32_t)V.height + bin_size - 1) / bin_size;
+        base += V.bins_x * V.bins_y;
+    }
+    return base;
+}
+
+int prepare_scratch(Scratch& S, size_t total_meshlets, size_t total_bins, const LaunchShape& sh, bool binned) {
+    const size_t clipq = clip_capacity_for(total_meshlets);
+    const size_t recs = total_meshlets * 128 + clipq * 8;
+    int rc = grow_one(S.records, S.records_cap, recs);
+    if (rc == GLIMP_OK) rc = grow_one(S.clip, S.clip_cap, clipq);
+    if (rc == GLIMP_OK) rc = grow_one(S.bitmap, S.bitmap_cap, (total_meshlets + 15) / 16 + 1);
+    if (rc == GLIMP_OK) rc = grow_one(S.meshlet_counts, S.meshlet_counts_cap, total_meshlets);
+    if (rc == GLIMP_OK) rc = grow_one(S.bin_counts, S.bin_counts_cap, std::max<size_t>(total_bins, 1));
+    if (rc == GLIMP_OK && binned) rc = grow_one(S.bin_lists, S.bin_lists_cap, total_bins * sh.bin_capacity);
+    if (rc == GLIMP_OK && binned) rc = grow_one(S.spill, S.spill_cap, recs);
+    if (rc == GLIMP_OK && !S.scratch) {
+        HIP_TRY(dev_alloc(&S.scratch, 1));
+        HIP_TRY(hipMemset(S.scratch, 0, sizeof(DrawScratch)));
+    }
+    return rc;
+}
+
+void fill_params(glimp_context* c, const Scratch& S, size_t total_meshlets, uint32_t total_bins,
+                 const LaunchShape& sh, uint32_t flags, DrawParams& p) {
+    p.total_meshlets = (uint32_t)total_meshlets;
+    p.total_bins = total_bins;
+    p.flags = flags;
+    p.tables = c->approx();
+    p.cull_bitmap = S.bitmap;
+    p.records = S.records;
+    p.meshlet_counts = S.meshlet_counts;
+    p.clip_record_base = (uint32_t)(total_meshlets * 128);
+    p.clip_record_capacity = (uint32_t)std::min<size_t>(S.records_cap - total_meshlets * 128, 0x7FFFFFFFu);
+    p.clip_queue = S.clip;
+    p.clip_capacity = (uint32_t)std::min<size_t>(S.clip_cap, 0x7FFFFFFFu);
+    p.scratch = S.scratch;
+    p.counters = c->counters;
+    p.bin_size = sh.bin_size;
+    p.bin_shift = (uint32_t)__builtin_ctz(sh.bin_size);
+    p.bin_capacity = sh.bin_capacity;
+    p.raster_threads = sh.raster_threads;
+    p.bin_counts = S.bin_counts;
+    p.bin_lists = S.bin_lists;
+    p.spill = S.spill;
+}
+
+bool valid_bin_size(uint32_t b) { return b == 8 || b == 16 || b == 32 || b == 64; }
+
+}  // namespace
+
+extern "C" {
+
 int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, const glimp_draw_desc* d,
                         void* stream) {
     if (!c || !f || !s || !d) return fail(GLIMP_E_INVALID, "glimp_draw_meshlets: NULL argument");
@@ -457,73 +547,295 @@ int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, 
         return fail(GLIMP_E_INVALID, "glimp_draw_meshlets: meshlet range outside the scene");
     if ((uint64_t)f->next_seq + d->meshlet_count > GLIMP_MAX_DRAWN_MESHLETS)
         return fail(GLIMP_E_LIMIT, "glimp_draw_meshlets: more than GLIMP_MAX_DRAWN_MESHLETS meshlets since the last clear");
+    LaunchShape sh;
+    if (d->bin_size) sh.bin_size = d->bin_size;
+    if (d->bin_capacity) sh.bin_capacity = d->bin_capacity;
+    if (!valid_bin_size(sh.bin_size))
+        return fail(GLIMP_E_INVALID, "glimp_draw_meshlets: bin_size must be 8, 16, 32 or 64");
     if (d->meshlet_count == 0) return GLIMP_OK;
     HIP_TRY(hipSetDevice(c->device));
-    size_t cnt = d->meshlet_count;
-    // worst case: every triangle clipped, each into 7 fan triangles
-    int rc = grow(c, cnt * 128 * 8, cnt * 128, (cnt + 15) / 16);
-    if (rc == GLIMP_OK) rc = grow_one(c->meshlet_counts, c->meshlet_counts_cap, cnt);
-    if (rc != GLIMP_OK) return rc;
+    const bool direct = (d->flags & GLIMP_DRAW_DIRECT) != 0;
+    hipStream_t st = (hipStream_t)stream;
+    if (direct) HIP_TRY(flush_clear(f, st));
 
+    std::vector<ViewParams> views(1);
+    make_view(f, s, d->meshlet_offset, d->meshlet_count, d->object_to_clip, d->frustum_planes, d->flags, views[0]);
+    const uint32_t total_bins = direct ? 0 : layout_bins(views, sh.bin_size);
+    int rc = prepare_scratch(c->draw_scratch, d->meshlet_count, total_bins, sh, !direct);
+    if (rc != GLIMP_OK) return rc;
     DrawParams p;
     memset(&p, 0, sizeof(p));
-    p.m = s->m;
-    p.materials = s->materials;
-    p.num_materials = s->num_materials;
-    p.meshlet_offset = d->meshlet_offset;
-    p.meshlet_count = d->meshlet_count;
-    p.seq_base = f->next_seq;
-    memcpy(p.clip, d->object_to_clip, sizeof(p.clip));
-    p.use_cull = d->frustum_planes != nullptr;
-    if (p.use_cull) memcpy(p.planes, d->frustum_planes, sizeof(p.planes));
-    p.width = (int)f->width;
-    p.height = (int)f->height;
-    bool guard = !(d->flags & GLIMP_DRAW_NO_GUARDBAND);
-    p.guard_x = guard ? (float)GLIMP_MAX_RENDER_SIZE / (float)f->width : 1.0f;
-    p.guard_y = guard ? (float)GLIMP_MAX_RENDER_SIZE / (float)f->height : 1.0f;
-    p.clipping = !(d->flags & GLIMP_DRAW_NO_CLIPPING);
-    p.flags = d->flags;
-    p.tables = c->approx();
-    p.cull_bitmap = c->bitmap;
-    p.records = c->records;
-    p.meshlet_counts = c->meshlet_counts;
-    p.clip_record_base = d->meshlet_count * 128;
-    p.clip_record_capacity = (uint32_t)std::min<size_t>(c->records_cap - (size_t)d->meshlet_count * 128, 0x7FFFFFFFu);
-    p.clip_queue = c->clip;
-    p.clip_capacity = (uint32_t)std::min<size_t>(c->clip_cap, 0xFFFFFFFFu);
-    p.scratch = c->scratch;
-    p.counters = c->counters;
-    p.keys = f->keys;
-    p.clear_depth = f->clear_depth;
-    hipStream_t st = (hipStream_t)stream;
-    if (d->flags & GLIMP_DRAW_DIRECT) {
-        HIP_TRY(flush_clear(f, st));
-        HIP_TRY(launch_draw_direct(p, st, &c->prof));
-    } else {
-        uint32_t bs = d->bin_size ? d->bin_size : 16;
-        if (bs != 8 && bs != 16 && bs != 32 && bs != 64)
-            return fail(GLIMP_E_INVALID, "glimp_draw_meshlets: bin_size must be 8, 16, 32 or 64");
-        uint32_t cap = d->bin_capacity ? d->bin_capacity : 1024;
-        BinParams& B = p.bins;
-        B.bin_size = bs;
-        B.bin_shift = (uint32_t)__builtin_ctz(bs);
-        B.bins_x = (f->width + bs - 1) / bs;
-        B.bins_y = (f->height + bs - 1) / bs;
-        B.bin_capacity = cap;
-        size_t nb = (size_t)B.bins_x * B.bins_y;
-        rc = grow_one(c->bin_counts, c->bin_counts_cap, nb);
-        if (rc == GLIMP_OK) rc = grow_one(c->bin_lists, c->bin_lists_cap, nb * cap);
-        if (rc == GLIMP_OK) rc = grow_one(c->spill, c->spill_cap, c->records_cap);
-        if (rc != GLIMP_OK) return rc;
-        B.bin_counts = c->bin_counts;
-        B.bin_lists = c->bin_lists;
-        B.spill = c->spill;
-        B.fresh = f->pending_clear ? 1 : 0;
-        HIP_TRY(launch_draw_binned(p, st, &c->prof));
-        f->pending_clear = false;
-    }
+    fill_params(c, c->draw_scratch, d->meshlet_count, total_bins, sh, d->flags, p);
+    p.num_views = 1;
+    p.views = nullptr;
+    p.view = views[0];
+    if (direct) HIP_TRY(launch_draw_direct(p, st, &c->prof));
+    else HIP_TRY(launch_draw_binned(p, st, &c->prof));
+    f->pending_clear = false;
     f->draws.push_back(DrawEntry{ f->next_seq, d->meshlet_count, d->meshlet_offset, 0 });
     f->next_seq += d->meshlet_count;
+    return GLIMP_OK;
+}
+
+}  // extern "C"
+
+struct glimp_batch {
+    glimp_context* ctx = nullptr;
+    std::vector<glimp_view_desc> descs;
+    std::vector<ViewParams> views;
+    ViewParams* d_views = nullptr;
+    size_t total_meshlets = 0;
+    uint32_t total_bins = 0;
+    LaunchShape shape;
+    Scratch scratch;
+    hipStream_t stream = nullptr;      // graph replays run here, chained to the caller's stream by events
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void drop_graph(glimp_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Rebuilds the view table for the batch's shape and uploads it.
+int layout_batch(glimp_batch* b) {
+    b->total_bins = layout_bins(b->views, b->shape.bin_size);
+    int rc = prepare_scratch(b->scratch, b->total_meshlets, b->total_bins, b->shape, true);
+    if (rc != GLIMP_OK) return rc;
+    if (!b->d_views) HIP_TRY(dev_alloc(&b->d_views, b->views.size()));
+    HIP_TRY(hipMemcpy(b->d_views, b->views.data(), b->views.size() * sizeof(ViewParams), hipMemcpyHostToDevice));
+    drop_graph(b);
+    return GLIMP_OK;
+}
+
+void batch_params(glimp_batch* b, DrawParams& p) {
+    memset(&p, 0, sizeof(p));
+    fill_params(b->ctx, b->scratch, b->total_meshlets, b->total_bins, b->shape, 0, p);
+    p.num_views = (uint32_t)b->views.size();
+    p.views = b->d_views;
+}
+
+bool valid_shape(const glimp_batch_shape& s) {
+    return valid_bin_size(s.bin_size) && s.bin_capacity >= 1 && (s.raster_threads == 128 || s.raster_threads == 256);
+}
+
+// Framebuffer state after a batch launch: cleared, then one draw.
+void mark_drawn(glimp_batch* b) {
+    for (const auto& d : b->descs) {
+        glimp_framebuffer* f = d.framebuffer;
+        f->clear_color = d.clear_color;
+        f->clear_depth = d.clear_depth == 0.0f ? 0.0f : d.clear_depth;
+        f->pending_clear = false;
+        f->draws.assign(1, DrawEntry{ 0, d.meshlet_count, d.meshlet_offset, 0 });
+        f->next_seq = d.meshlet_count;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int glimp_batch_create(glimp_context* c, const glimp_view_desc* views, uint32_t count, const glimp_batch_shape* shape,
+                       glimp_batch** out) {
+    if (!c || !out || (count && !views)) return fail(GLIMP_E_INVALID, "glimp_batch_create: NULL argument");
+    *out = nullptr;
+    if (count == 0) return fail(GLIMP_E_INVALID, "glimp_batch_create: no views");
+    if (count > 65535) return fail(GLIMP_E_LIMIT, "glimp_batch_create: at most 65535 views");
+    std::vector<glimp_framebuffer*> seen;
+    for (uint32_t i = 0; i < count; i++) {
+        const glimp_view_desc& d = views[i];
+        if (!d.scene || !d.framebuffer) return fail(GLIMP_E_INVALID, "glimp_batch_create: view without scene or framebuffer");
+        if ((uint64_t)d.meshlet_offset + d.meshlet_count > d.scene->m.count)
+            return fail(GLIMP_E_INVALID, "glimp_batch_create: meshlet range outside the scene");
+        if (d.meshlet_count > GLIMP_MAX_DRAWN_MESHLETS) return fail(GLIMP_E_LIMIT, "glimp_batch_create: view too large");
+        if (!(d.clear_depth >= 0.0f)) return fail(GLIMP_E_INVALID, "glimp_batch_create: clear depth must be >= 0");
+        for (auto* f : seen)
+            if (f == d.framebuffer) return fail(GLIMP_E_INVALID, "glimp_batch_create: views must not share a framebuffer");
+        seen.push_back(d.framebuffer);
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    glimp_batch* b = new (std::nothrow) glimp_batch();
+    if (!b) return fail(GLIMP_E_NOMEM, "glimp_batch_create: out of host memory");
+    b->ctx = c;
+    if (shape) {
+        if (!valid_shape(*shape)) {
+            delete b;
+            return fail(GLIMP_E_INVALID, "glimp_batch_create: bad shape");
+        }
+        b->shape.bin_size = shape->bin_size;
+        b->shape.bin_capacity = shape->bin_capacity;
+        b->shape.raster_threads = shape->raster_threads;
+    }
+    b->descs.assign(views, views + count);
+    b->views.resize(count);
+    size_t base = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        const glimp_view_desc& d = views[i];
+        glimp_framebuffer tmp;     // the view starts from a cleared framebuffer
+        tmp.width = d.framebuffer->width;
+        tmp.height = d.framebuffer->height;
+        tmp.keys = d.framebuffer->keys;
+        tmp.clear_depth = d.clear_depth == 0.0f ? 0.0f : d.clear_depth;
+        tmp.pending_clear = true;
+        tmp.next_seq = 0;
+        make_view(&tmp, d.scene, d.meshlet_offset, d.meshlet_count, d.object_to_clip, d.frustum_planes, d.flags,
+                  b->views[i]);
+        b->views[i].meshlet_base = (uint32_t)base;
+        base += d.meshlet_count;
+    }
+    b->total_meshlets = base;
+    int rc = layout_batch(b);
+    hipError_t e = hipSuccess;
+    if (rc == GLIMP_OK) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming);
+    if (rc == GLIMP_OK && e != hipSuccess) rc = fail(GLIMP_E_HIP, std::string("glimp_batch_create: ") + hipGetErrorString(e));
+    if (rc != GLIMP_OK) {
+        glimp_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return GLIMP_OK;
+}
+
+void glimp_batch_destroy(glimp_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    if (b->d_views) (void)hipFree(b->d_views);
+    b->scratch.release();
+    if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+    if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int glimp_batch_set_shape(glimp_batch* b, const glimp_batch_shape* shape) {
+    if (!b || !shape) return fail(GLIMP_E_INVALID, "glimp_batch_set_shape: NULL argument");
+    if (!valid_shape(*shape)) return fail(GLIMP_E_INVALID, "glimp_batch_set_shape: bad shape");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    HIP_TRY(hipDeviceSynchronize());   // scratch may be reallocated under launches on any stream
+    b->shape.bin_size = shape->bin_size;
+    b->shape.bin_capacity = shape->bin_capacity;
+    b->shape.raster_threads = shape->raster_threads;
+    return layout_batch(b);
+}
+
+int glimp_batch_launch(glimp_batch* b, int use_graph, void* stream) {
+    if (!b) return fail(GLIMP_E_INVALID, "glimp_batch_launch: NULL batch");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    hipStream_t user = (hipStream_t)stream;
+    DrawParams p;
+    batch_params(b, p);
+    if (!use_graph) {
+        HIP_TRY(launch_draw_binned(p, user, &b->ctx->prof));
+        mark_drawn(b);
+        return GLIMP_OK;
+    }
+    if (!b->exec) {
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t le = launch_draw_binned(p, b->stream, nullptr);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(b->stream, &g);
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(GLIMP_E_HIP, std::string("glimp_batch_launch: capture failed: ") +
+                                         hipGetErrorString(le != hipSuccess ? le : ce));
+        }
+        b->graph = g;
+        HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    Profiler::Scope sc(&b->ctx->prof, "batch_graph", b->stream);
+    HIP_TRY(hipEventRecord(b->ev_in, user));
+    HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_in, 0));
+    HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    HIP_TRY(hipEventRecord(b->ev_out, b->stream));
+    HIP_TRY(hipStreamWaitEvent(user, b->ev_out, 0));
+    mark_drawn(b);
+    return GLIMP_OK;
+}
+
+int glimp_batch_launch_per_view(glimp_batch* b, void* stream) {
+    if (!b) return fail(GLIMP_E_INVALID, "glimp_batch_launch_per_view: NULL batch");
+    for (const auto& d : b->descs) {
+        int rc = glimp_framebuffer_clear(d.framebuffer, d.clear_color, d.clear_depth, stream);
+        if (rc != GLIMP_OK) return rc;
+        glimp_draw_desc dd;
+        memset(&dd, 0, sizeof(dd));
+        dd.meshlet_offset = d.meshlet_offset;
+        dd.meshlet_count = d.meshlet_count;
+        memcpy(dd.object_to_clip, d.object_to_clip, sizeof(dd.object_to_clip));
+        dd.frustum_planes = d.frustum_planes;
+        dd.flags = d.flags & (GLIMP_DRAW_NO_CLIPPING | GLIMP_DRAW_NO_GUARDBAND);
+        dd.pipeline = GLIMP_PIPELINE_VISBUFFER;
+        dd.bin_size = b->shape.bin_size;
+        dd.bin_capacity = b->shape.bin_capacity;
+        rc = glimp_draw_meshlets(b->ctx, d.framebuffer, d.scene, &dd, stream);
+        if (rc != GLIMP_OK) return rc;
+    }
+    return GLIMP_OK;
+}
+
+int glimp_batch_autotune(glimp_batch* b, void* stream, glimp_batch_shape* chosen, float* best_ms) {
+    if (!b) return fail(GLIMP_E_INVALID, "glimp_batch_autotune: NULL batch");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t sizes[] = { 8, 16, 32, 64 };
+    const uint32_t caps[] = { 256, 512, 1024, 2048, 4096 };
+    const uint32_t threads[] = { 128, 256 };
+    const size_t kListBudget = (size_t)2 << 30;   // bytes of bin lists a candidate may use
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    glimp_batch_shape best = { b->shape.bin_size, b->shape.bin_capacity, b->shape.raster_threads };
+    float bestMs = 1e30f;
+    const bool prof = b->ctx->prof.enabled;
+    b->ctx->prof.enabled = false;
+    int rc = GLIMP_OK;
+    for (uint32_t bs : sizes) {
+        for (uint32_t cap : caps) {
+            for (uint32_t th : threads) {
+                std::vector<ViewParams> tmp = b->views;
+                size_t bins = layout_bins(tmp, bs);
+                if (bins * cap * 4 > kListBudget) continue;
+                glimp_batch_shape sh = { bs, cap, th };
+                rc = glimp_batch_set_shape(b, &sh);
+                if (rc != GLIMP_OK) break;
+                for (int i = 0; i < 2 && rc == GLIMP_OK; i++) rc = glimp_batch_launch(b, 0, stream);
+                if (rc != GLIMP_OK) break;
+                const int reps = 5;
+                (void)hipEventRecord(e0, st);
+                for (int i = 0; i < reps && rc == GLIMP_OK; i++) rc = glimp_batch_launch(b, 0, stream);
+                (void)hipEventRecord(e1, st);
+                if (rc != GLIMP_OK) break;
+                float ms = 0.0f;
+                (void)hipEventSynchronize(e1);
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                ms /= reps;
+                if (ms < bestMs) {
+                    bestMs = ms;
+                    best = sh;
+                }
+            }
+            if (rc != GLIMP_OK) break;
+        }
+        if (rc != GLIMP_OK) break;
+    }
+    b->ctx->prof.enabled = prof;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != GLIMP_OK) return rc;
+    rc = glimp_batch_set_shape(b, &best);
+    if (rc != GLIMP_OK) return rc;
+    if (chosen) *chosen = best;
+    if (best_ms) *best_ms = bestMs;
     return GLIMP_OK;
 }
 
@@ -534,7 +846,7 @@ int glimp_cull_meshlets(glimp_context* c, glimp_scene* s, uint32_t first, uint32
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t st = (hipStream_t)stream;
     size_t words = (count + 15) / 16;
-    int rc = grow(c, 0, 0, words);
+    int rc = grow_one(c->bitmap, c->bitmap_cap, words);
     if (rc != GLIMP_OK) return rc;
     HIP_TRY(hipMemsetAsync(c->visible, 0, 4, st));
     if (count) HIP_TRY(launch_cull(s->m, first, count, planes, c->bitmap, c->visible, st));

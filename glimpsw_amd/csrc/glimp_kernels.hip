@@ -1,14 +1,19 @@
 // Meshlet raster kernels for gfx950 (wave64).
 //
-// Pipeline of one draw (Rasterizer::DrawMeshlets, Rasterizer.h:213):
-//   k_mesh_setup  one wave per meshlet: frustum cull bit, vertex transform
-//                 into LDS, outcodes, triangle setup; survivors become
-//                 32-byte TriRecords, guard-band cases go to a clip queue.
-//   k_clip        one lane per queued triangle: Sutherland-Hodgman against
-//                 the set outcode planes, fan triangulation, setup.
-//   k_raster_direct  load-balanced over 4x4 tiles of the records' render
-//                 boxes; covered fragments update the 64-bit key buffer
-//                 with atomicMax.
+// One launch per stage covers every view of a draw or batch (a single
+// DrawMeshlets call is a batch of one view, Rasterizer::DrawMeshlets,
+// Rasterizer.h:213):
+//   k_begin       zero the per-launch counters and bin counts
+//   k_cull        frustum test of bound spheres, one lane per meshlet
+//   k_mesh_setup  one wave per meshlet: vertex transform into LDS, outcodes,
+//                 triangle setup; survivors become 32-byte TriRecords in the
+//                 meshlet's own slots, guard-band cases go to a clip queue
+//   k_clip        one wave per queued triangle: wave-parallel
+//                 Sutherland-Hodgman, fan triangulation, setup
+//   k_bin_scatter (record, screen bin) pairs into fixed-capacity bins
+//   k_bin_raster  one workgroup per bin: LDS-resident 64-bit keys
+//   k_raster_direct  records that found a bin full (or, without binning,
+//                 all records): global 64-bit atomicMax
 //
 // Depth test. The reference keeps the nearest fragment (reverse-Z, strictly
 // greater depth wins, Shading.cpp:313) and, with one thread, resolves ties in
@@ -16,7 +21,6 @@
 // rank being its position in that draw order; all depths that can pass are
 // positive, so the largest key is the reference's surviving fragment and the
 // result does not depend on the order the GPU produces fragments in.
-#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include "internal.h"
@@ -138,12 +142,66 @@ __device__ __forceinline__ uint32_t pixel_offset(uint32_t W, uint32_t x, uint32_
     return ((x & ~3u) << 2) + (y & ~3u) * W + (x & 3u) + ((y & 3u) << 2);
 }
 
+// ---------------------------------------------------------------- views
+
+__device__ __forceinline__ const ViewParams& view_of(const DrawParams& p, uint32_t v) {
+    return p.views ? p.views[v] : p.view;
+}
+
+// View owning global meshlet slot g (views are ordered by meshlet_base).
+__device__ __forceinline__ uint32_t view_of_meshlet(const DrawParams& p, uint32_t g) {
+    uint32_t lo = 0, hi = p.num_views;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (view_of(p, mid).meshlet_base <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// View owning global bin b.
+__device__ __forceinline__ uint32_t view_of_bin(const DrawParams& p, uint32_t b) {
+    uint32_t lo = 0, hi = p.num_views;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (view_of(p, mid).bin_base <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool counting(const DrawParams& p) { return (p.flags & 8u) != 0; }
+
 // ---------------------------------------------------------------- cull
 
-// Frustum test of bound spheres (Shading.cpp:797-809): one lane per meshlet,
-// the bitmap word for 16 meshlets assembled from a wave ballot.
-__global__ void k_cull(DevMeshlets m, uint32_t first, uint32_t count, Planes5 pl, uint16_t* bitmap,
-                       uint32_t* visible) {
+// Frustum test of bound spheres (Shading.cpp:797-809) over the global
+// meshlet numbering; the bitmap word for 16 slots comes from a wave ballot.
+__global__ void k_cull(DrawParams p) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    bool vis = false;
+    if (g < p.total_meshlets) {
+        const ViewParams& V = view_of(p, view_of_meshlet(p, g));
+        vis = true;
+        if (V.use_cull) {
+            float4 b = V.m.bounds[V.meshlet_offset + (g - V.meshlet_base)];
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                const float* P = V.planes + 4 * k;
+                float d = __builtin_fmaf(b.x, P[0], __builtin_fmaf(b.y, P[1], b.z * P[2])) + P[3];
+                vis = vis && (d > -b.w);
+            }
+        }
+    }
+    unsigned long long mask = __ballot(vis);
+    const uint32_t lane = lane_id();
+    if ((lane & 15) == 0 && g < p.total_meshlets) p.cull_bitmap[g / 16] = (uint16_t)(mask >> (lane & 48));
+    if (counting(p) && lane == 0) {
+        uint32_t c = __popcll(mask);
+        if (c) atomicAdd(&p.scratch->num_visible, c);
+    }
+}
+
+// Standalone cull of one meshlet range (ShadingContext::CullMeshlets).
+__global__ void k_cull_range(DevMeshlets m, uint32_t first, uint32_t count, Planes5 pl, uint16_t* bitmap,
+                             uint32_t* visible) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool vis = false;
     if (i < count) {
@@ -158,8 +216,7 @@ __global__ void k_cull(DevMeshlets m, uint32_t first, uint32_t count, Planes5 pl
     }
     unsigned long long mask = __ballot(vis);
     uint32_t lane = lane_id();
-    uint32_t word = i / 16;
-    if ((lane & 15) == 0 && i < count) bitmap[word] = (uint16_t)(mask >> (lane & 48));
+    if ((lane & 15) == 0 && i < count) bitmap[i / 16] = (uint16_t)(mask >> (lane & 48));
     if (visible && lane == 0) {
         uint32_t c = __popcll(mask);
         if (c) atomicAdd(visible, c);
@@ -168,43 +225,43 @@ __global__ void k_cull(DevMeshlets m, uint32_t first, uint32_t count, Planes5 pl
 
 // ---------------------------------------------------------------- mesh + setup
 
-// Record layout of a draw: meshlet w (within the draw) owns record slots
-// [w*128, w*128 + meshlet_counts[w]); clipped sub-triangles are appended
-// after all meshlet slots. Setup therefore needs no global counter: a
-// wave compacts its survivors with a ballot and writes them into its own
-// slots.
-__device__ __forceinline__ bool counting(const DrawParams& p) { return (p.flags & 8u) != 0; }
-
+// Record layout: global meshlet slot g owns records [g*128, g*128 +
+// meshlet_counts[g]); clipped sub-triangles are appended after all meshlet
+// slots. Setup needs no global counter: a wave compacts its survivors with a
+// ballot and writes them into its own slots.
 __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
     __shared__ float lds_pos[4][4][64];
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
-    const uint32_t w = blockIdx.x * 4 + wid;   // meshlet index within the draw
-    const bool have = w < p.meshlet_count;
-    const uint32_t mi = p.meshlet_offset + w;
-    const int hw = p.width / 2, hh = p.height / 2;
+    const uint32_t g = blockIdx.x * 4 + wid;   // global meshlet slot
+    const bool have = g < p.total_meshlets;
+    const uint32_t v = have ? view_of_meshlet(p, g) : 0;
+    const ViewParams& V = view_of(p, v);
+    const uint32_t w = g - V.meshlet_base;     // meshlet index within the view
+    const uint32_t mi = V.meshlet_offset + w;
+    const int hw = V.width / 2, hh = V.height / 2;
 
-    bool visible = have;
-    if (have && p.use_cull) visible = ((p.cull_bitmap[w / 16] >> (w % 16)) & 1) != 0;
-    uint32_t nt = visible ? p.m.num_triangles[mi] : 0;
+    bool visible = have && ((p.cull_bitmap[g / 16] >> (g % 16)) & 1) != 0;
+    uint32_t nt = visible ? V.m.num_triangles[mi] : 0;
     int cull_mode = 1, fs_id = 0;
     if (visible) {
-        uint32_t mat = p.m.material_id[mi];
-        if (mat != 0xFFFFFFFFu && mat < p.num_materials) {
-            cull_mode = p.materials[mat].cull_mode;
-            fs_id = p.materials[mat].fs_id;
+        uint32_t mat = V.m.material_id[mi];
+        if (mat != 0xFFFFFFFFu && mat < V.num_materials) {
+            cull_mode = V.materials[mat].cull_mode;
+            fs_id = V.materials[mat].fs_id;
         }
         // ShadeMeshlet (Shading.cpp:295-298): lane = vertex
-        const float* P = p.m.positions + (size_t)mi * 192;
+        const float* P = V.m.positions + (size_t)mi * 192;
         float px = P[lane], py = P[64 + lane], pz = P[128 + lane];
 #pragma unroll
-        for (int c = 0; c < 4; c++) lds_pos[wid][c][lane] = xform(p.clip, c, px, py, pz);
+        for (int c = 0; c < 4; c++) lds_pos[wid][c][lane] = xform(V.clip, c, px, py, pz);
     }
     __syncthreads();
     if (!have) return;
-    const uint32_t seq = p.seq_base + w;
-    const uint8_t* I = p.m.indices + (size_t)mi * 384;
-    TriRecord* out = p.records + (size_t)w * 128;
+    const uint32_t seq = V.seq_base + w;
+    const uint8_t* I = V.m.indices + (size_t)mi * 384;
+    TriRecord* out = p.records + (size_t)g * 128;
+    const uint32_t flags_hi = v << 16;
     uint32_t kept = 0, clipped = 0;
 
     for (uint32_t round = 0; round < 2; round++) {
@@ -213,7 +270,7 @@ __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
         bool valid = t < nt;
         bool accept = false, nontrivial = false;
         uint32_t partial = 0;
-        float v[3][4];
+        float vv[3][4];
         if (valid) {
             uint8_t combined = 255;
             bool trivial = true;
@@ -222,8 +279,8 @@ __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
                 uint32_t ix = I[k * 128 + t];
                 uint32_t vi = ix < 32 ? ix : 32 + (ix & 31);   // gather_preload64 (SIMD.h:219-227)
 #pragma unroll
-                for (int c = 0; c < 4; c++) v[k][c] = lds_pos[wid][c][vi];
-                float x = v[k][0], y = v[k][1], z = v[k][2], ww = v[k][3];
+                for (int c = 0; c < 4; c++) vv[k][c] = lds_pos[wid][c][vi];
+                float x = vv[k][0], y = vv[k][1], z = vv[k][2], ww = vv[k][3];
                 uint32_t oc = 0;
                 oc |= (x < -ww) ? 1u : 0u;
                 oc |= (x > ww) ? 2u : 0u;
@@ -233,19 +290,19 @@ __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
                 oc |= (z > ww) ? 32u : 0u;
                 partial |= oc;
                 combined &= (uint8_t)oc;
-                trivial = trivial && (fabsf(x) < ww * p.guard_x) && (fabsf(y) < ww * p.guard_y);
+                trivial = trivial && (fabsf(x) < ww * V.guard_x) && (fabsf(y) < ww * V.guard_y);
             }
             trivial = trivial && ((partial & 0x30u) == 0);
             accept = (combined == 0) && trivial;
-            nontrivial = (combined == 0) && !trivial && p.clipping;
+            nontrivial = (combined == 0) && !trivial && V.clipping;
         }
         const uint32_t packet = t >> 4, plane = t & 15;
         TriRecord rec;
         bool keep = false;
         if (accept) {
-            keep = setup_triangle(p.tables, v, hw, hh, cull_mode, rec);
+            keep = setup_triangle(p.tables, vv, hw, hh, cull_mode, rec);
             rec.rank = make_rank(seq, packet, 0, plane, 0);
-            rec.flags = (uint32_t)fs_id;
+            rec.flags = (uint32_t)fs_id | flags_hi;
         }
         unsigned long long km = __ballot(keep);
         if (keep) out[kept + mbcnt(km)] = rec;
@@ -266,8 +323,9 @@ __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
 #pragma unroll
                     for (int k = 0; k < 3; k++)
 #pragma unroll
-                        for (int c = 0; c < 4; c++) ce.v[k][c] = v[k][c];
-                    ce.pad[0] = ce.pad[1] = 0;
+                        for (int c = 0; c < 4; c++) ce.v[k][c] = vv[k][c];
+                    ce.view = v;
+                    ce.pad = 0;
                     p.clip_queue[slot] = ce;
                 } else {
                     p.scratch->overflow = 1;
@@ -276,7 +334,7 @@ __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
         }
     }
     if (lane == 0) {
-        p.meshlet_counts[w] = kept;
+        p.meshlet_counts[g] = kept;
         if (counting(p) && visible) {
             atomicAdd(&p.counters->triangles_processed, (unsigned long long)nt);
             atomicAdd(&p.counters->triangles_rasterized, (unsigned long long)kept);
@@ -292,10 +350,10 @@ __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
 // vertex i, in polygon order). For each plane set in the triangle's combined
 // outcode, in plane order L, R, T, B, N, F: every lane computes its vertex's
 // distance `+-c + w*scale` (guard-band scale for x/y), sees the next
-// vertex's through a shuffle, emits itself when inside and the edge
+// vertex's through a lane shuffle, emits itself when inside and the edge
 // intersection `lerp(a, b, da/(da-db))` when the edge crosses; an exclusive
-// scan of the emit counts gives each output's position and a permute moves
-// the outputs to their lanes. The values and their order are those of the
+// scan of the emit counts gives each output's position and every output lane
+// pulls its value from its source lane. Values and order are those of the
 // reference's sequential loop. Generated vertices are limited to 16 per
 // triangle, the room the reference's vertex store has. The fan triangles
 // (v0, v[f+1], v[f+2]) then go through setup with culling (FlushPacket,
@@ -306,11 +364,12 @@ __device__ __forceinline__ float lane_get(float v, uint32_t src) {
 
 __global__ void __launch_bounds__(256) k_clip(DrawParams p) {
     const uint32_t n = min(p.scratch->num_clip, p.clip_capacity);
-    const int hw = p.width / 2, hh = p.height / 2;
     const uint32_t lane = lane_id();
     const uint32_t waves = gridDim.x * (blockDim.x / kWave);
     for (uint32_t q = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; q < n; q += waves) {
         const ClipEntry& ce = p.clip_queue[q];
+        const ViewParams& VP = view_of(p, ce.view);
+        const int hw = VP.width / 2, hh = VP.height / 2;
         float V[4];
 #pragma unroll
         for (int c = 0; c < 4; c++) V[c] = lane < 3 ? ce.v[lane][c] : 0.0f;
@@ -318,7 +377,7 @@ __global__ void __launch_bounds__(256) k_clip(DrawParams p) {
         const uint32_t codes = ce.info & 0x3Fu;
         for (uint32_t plane = 0; plane < 6; plane++) {
             if (!((codes >> plane) & 1u)) continue;
-            const float scale = plane < 2 ? p.guard_x : (plane < 4 ? p.guard_y : 1.0f);
+            const float scale = plane < 2 ? VP.guard_x : (plane < 4 ? VP.guard_y : 1.0f);
             const uint32_t comp = plane >> 1;
             const bool neg = plane & 1u;
             const bool active = lane < vc;
@@ -385,7 +444,7 @@ __global__ void __launch_bounds__(256) k_clip(DrawParams p) {
         if (f + 2 < vc && f < 8) {
             keep = setup_triangle(p.tables, tri, hw, hh, cull_mode, rec);
             rec.rank = ce.rank_base | f;
-            rec.flags = (ce.info >> 10) & 3u;
+            rec.flags = ((ce.info >> 10) & 3u) | (ce.view << 16);
         }
         const unsigned long long km = __ballot(keep);
         if (km) {
@@ -404,9 +463,9 @@ __global__ void __launch_bounds__(256) k_clip(DrawParams p) {
 
 // ---------------------------------------------------------------- record chunks
 
-// Chunks of up to 64 records for a wave: chunk c < 2*meshlet_count is half
-// c&1 of meshlet c>>1's slots; later chunks cover the clipped records; in
-// spill mode chunk c covers spill-list entries [64c, 64c+64).
+// Chunks of up to 64 records for a wave: chunk c < 2*total_meshlets is half
+// c&1 of meshlet slot c>>1; later chunks cover the clipped records; in spill
+// mode chunk c covers spill-list entries [64c, 64c+64).
 struct Chunk {
     uint32_t base, count;
     const uint32_t* list;      // non-null: record index = list[base + i]
@@ -415,7 +474,7 @@ struct Chunk {
 __device__ __forceinline__ uint32_t num_chunks(const DrawParams& p, int spill) {
     if (spill) return (min(p.scratch->num_spill, p.clip_record_base + p.clip_record_capacity) + 63) / 64;
     uint32_t nclip = min(p.scratch->num_records, p.clip_record_capacity);
-    return 2 * p.meshlet_count + (nclip + 63) / 64;
+    return 2 * p.total_meshlets + (nclip + 63) / 64;
 }
 
 __device__ __forceinline__ Chunk get_chunk(const DrawParams& p, int spill, uint32_t c) {
@@ -425,18 +484,18 @@ __device__ __forceinline__ Chunk get_chunk(const DrawParams& p, int spill, uint3
         uint32_t n = min(p.scratch->num_spill, p.clip_record_base + p.clip_record_capacity);
         k.base = c * 64;
         k.count = min(64u, n - k.base);
-        k.list = p.bins.spill;
+        k.list = p.spill;
         return k;
     }
-    if (c < 2 * p.meshlet_count) {
-        uint32_t w = c >> 1, half = c & 1;
-        uint32_t cnt = p.meshlet_counts[w];
-        k.base = w * 128 + half * 64;
+    if (c < 2 * p.total_meshlets) {
+        uint32_t g = c >> 1, half = c & 1;
+        uint32_t cnt = p.meshlet_counts[g];
+        k.base = g * 128 + half * 64;
         k.count = cnt > half * 64 ? min(64u, cnt - half * 64) : 0;
         return k;
     }
     uint32_t nclip = min(p.scratch->num_records, p.clip_record_capacity);
-    uint32_t off = (c - 2 * p.meshlet_count) * 64;
+    uint32_t off = (c - 2 * p.total_meshlets) * 64;
     k.base = p.clip_record_base + off;
     k.count = min(64u, nclip - off);
     return k;
@@ -452,9 +511,8 @@ struct TriLds {
     uint32_t keylo, org, tw, start;
 };
 
-__device__ __forceinline__ void load_tri(const DrawParams& p, uint32_t r, int hw, int hh, TriLds& t,
-                                         uint32_t& bmin, uint32_t& bmax) {
-    const TriRecord rec = p.records[r];
+__device__ __forceinline__ void load_tri(const TriRecord& rec, int hw, int hh, TriLds& t, uint32_t& bmin,
+                                         uint32_t& bmax) {
     render_bbox(rec.pos[0], rec.pos[1], rec.pos[2], hw, hh, bmin, bmax);
     Edges e = edge_setup(rec.pos[0], rec.pos[1], rec.pos[2], hw, hh);
     t.e0 = e.e0; t.e1 = e.e1; t.e2 = e.e2;
@@ -479,15 +537,14 @@ __device__ __forceinline__ bool shade_pixel(const TriLds& T, uint32_t x, uint32_
 // records, or bin spill entries), lays their render boxes out as one list of
 // 4x4 tiles (exclusive scan of tile counts) and walks it four tiles at a
 // time: lane = pixel, 16 lanes = one tile, so a tile's 16 keys (128 bytes)
-// are one contiguous access. Covered pixels update the key with atomicMax
-// when the key can still win.
+// are one contiguous access. Covered pixels update the key of their view's
+// vis-buffer with atomicMax when the key can still win.
 __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p, int use_spill) {
     __shared__ TriLds tl[4][64];
+    __shared__ uint32_t tview[4][64];
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
     const uint32_t nchunks = num_chunks(p, use_spill);
-    const int hw = p.width / 2, hh = p.height / 2;
-    const uint32_t W = (uint32_t)p.width;
     const bool countFrags = counting(p) && !use_spill;
     const uint32_t waves = gridDim.x * 4;
 
@@ -498,13 +555,17 @@ __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p, int use_spi
         TriLds t;
         if (lane < ck.count) {
             const uint32_t r = ck.list ? ck.list[ck.base + lane] : ck.base + lane;
+            const TriRecord rec = p.records[r];
+            const uint32_t v = rec.flags >> 16;
+            const ViewParams& V = view_of(p, v);
             uint32_t bmin, bmax;
-            load_tri(p, r, hw, hh, t, bmin, bmax);
+            load_tri(rec, V.width / 2, V.height / 2, t, bmin, bmax);
             t.org = bmin;
             uint32_t tw = (uint32_t)(lo16(bmax) - lo16(bmin)) >> 2;
             uint32_t th = (uint32_t)(hi16(bmax) - hi16(bmin)) >> 2;
             t.tw = tw;
             ntiles = tw * th;
+            tview[wid][lane] = v;
         }
         uint32_t incl = ntiles;
 #pragma unroll
@@ -529,17 +590,19 @@ __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p, int use_spi
                     if (tl[wid][k + step].start <= j) k += step;
                 }
                 const TriLds& T = tl[wid][k];
+                const ViewParams& V = view_of(p, tview[wid][k]);
+                const uint32_t W = (uint32_t)V.width;
                 const uint32_t pt = j - T.start;
                 const uint32_t ty = pt / T.tw;
                 const uint32_t tx = pt - ty * T.tw;
                 const uint32_t x = (uint32_t)lo16(T.org) + tx * 4 + px;
                 const uint32_t y = (uint32_t)hi16(T.org) + ty * 4 + py;
                 float depth;
-                if (x < W && y < (uint32_t)p.height && shade_pixel(T, x, y, depth)) {
+                if (x < W && y < (uint32_t)V.height && shade_pixel(T, x, y, depth)) {
                     covered++;
-                    if (depth > p.clear_depth) {
+                    if (depth > V.clear_depth) {
                         unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
-                        unsigned long long* dst = p.keys + pixel_offset(W, x, y);
+                        unsigned long long* dst = V.keys + pixel_offset(W, x, y);
                         if (key > __atomic_load_n(dst, __ATOMIC_RELAXED)) atomicMax(dst, key);
                         passed++;
                     }
@@ -572,8 +635,6 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
     const uint32_t nchunks = num_chunks(p, 0);
-    const int hw = p.width / 2, hh = p.height / 2;
-    const BinParams& B = p.bins;
     const uint32_t waves = gridDim.x * 4;
     unsigned long long entries = 0;
     for (uint32_t c = blockIdx.x * 4 + wid; c < nchunks; c += waves) {
@@ -581,16 +642,19 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
         if (ck.count == 0) continue;
         const bool valid = lane < ck.count;
         const uint32_t r = ck.base + lane;
-        uint32_t nb = 0, org = 0, bw = 0;
+        uint32_t nb = 0, org = 0, bw = 0, pitch = 0;
         if (valid) {
             const TriRecord rec = p.records[r];
+            const ViewParams& V = view_of(p, rec.flags >> 16);
             uint32_t bmin, bmax;
-            render_bbox(rec.pos[0], rec.pos[1], rec.pos[2], hw, hh, bmin, bmax);
-            uint32_t bx0 = (uint32_t)lo16(bmin) >> B.bin_shift, by0 = (uint32_t)hi16(bmin) >> B.bin_shift;
-            uint32_t bx1 = (uint32_t)(lo16(bmax) - 1) >> B.bin_shift, by1 = (uint32_t)(hi16(bmax) - 1) >> B.bin_shift;
+            render_bbox(rec.pos[0], rec.pos[1], rec.pos[2], V.width / 2, V.height / 2, bmin, bmax);
+            uint32_t bx0 = (uint32_t)lo16(bmin) >> p.bin_shift, by0 = (uint32_t)hi16(bmin) >> p.bin_shift;
+            uint32_t bx1 = (uint32_t)(lo16(bmax) - 1) >> p.bin_shift, by1 = (uint32_t)(hi16(bmax) - 1) >> p.bin_shift;
             bw = bx1 - bx0 + 1;
             nb = bw * (by1 - by0 + 1);
-            org = bx0 | (by0 << 16);
+            // first bin of the record's box, in the global bin numbering
+            org = V.bin_base + by0 * V.bins_x + bx0;
+            pitch = V.bins_x;
         }
         uint32_t incl = nb;
 #pragma unroll
@@ -602,27 +666,23 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
         entries += total;
         s_start[wid][lane] = valid ? incl - nb : 0xFFFFFFFFu;
         s_org[wid][lane] = org;
-        s_bw[wid][lane] = bw;
+        s_bw[wid][lane] = bw | (pitch << 16);
         s_spill[wid][lane] = 0;
         __builtin_amdgcn_wave_barrier();
         for (uint32_t base = 0; base < total; base += 64) {
             const uint32_t j = base + lane;
-            const bool pv = j < total;
-            uint32_t k = 0, bin = 0xFFFFFFFFu;
-            if (pv) {
+            if (j < total) {
+                uint32_t k = 0;
 #pragma unroll
                 for (uint32_t step = 32; step > 0; step >>= 1) {
                     if (s_start[wid][k + step] <= j) k += step;
                 }
                 const uint32_t q = j - s_start[wid][k];
-                const uint32_t w = s_bw[wid][k];
+                const uint32_t w = s_bw[wid][k] & 0xFFFFu, pitch_k = s_bw[wid][k] >> 16;
                 const uint32_t dy = q / w, dx = q - dy * w;
-                bin = ((s_org[wid][k] >> 16) + dy) * B.bins_x + (s_org[wid][k] & 0xFFFFu) + dx;
-            }
-            uint32_t slot = 0;
-            if (pv) slot = atomicAdd(&B.bin_counts[bin], 1u);
-            if (pv) {
-                if (slot < B.bin_capacity) B.bin_lists[(size_t)bin * B.bin_capacity + slot] = ck.base + k;
+                const uint32_t bin = s_org[wid][k] + dy * pitch_k + dx;
+                const uint32_t slot = atomicAdd(&p.bin_counts[bin], 1u);
+                if (slot < p.bin_capacity) p.bin_lists[(size_t)bin * p.bin_capacity + slot] = ck.base + k;
                 else s_spill[wid][k] = 1;
             }
         }
@@ -633,17 +693,17 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(&p.scratch->num_spill, (uint32_t)__popcll(sm));
             base = __shfl(base, 0);
-            if (spilled) p.bins.spill[base + mbcnt(sm)] = r;
+            if (spilled) p.spill[base + mbcnt(sm)] = r;
         }
         __builtin_amdgcn_wave_barrier();
     }
     if (counting(p) && lane == 0 && entries) atomicAdd(&p.counters->bin_entries, entries);
 }
 
-// One block per bin. The bin's keys live in LDS in the same 4x4 tile order
-// as the vis-buffer. The block loads up to kThreads records at a time,
-// clips their render boxes to the bin and lays the clipped boxes out as one
-// list of 4x4 tiles (block-wide exclusive scan). The list is cut into
+// One block per bin of any view. The bin's keys live in LDS in the same 4x4
+// tile order as the vis-buffer. The block loads up to kThreads records at a
+// time, clips their render boxes to the bin and lays the clipped boxes out
+// as one list of 4x4 tiles (block-wide exclusive scan). The list is cut into
 // equal ranges, one per 16-lane group; a group walks its range tile by tile
 // (lane = pixel), stepping to the next triangle as the range crosses it, and
 // keeps the nearest fragment with an LDS 64-bit max. The finished bin is
@@ -657,25 +717,27 @@ __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
     __shared__ unsigned long long keys[BS * BS];
     __shared__ TriLds tl[kThreads];
     __shared__ uint32_t s_wave_total[kWaves];
-    const BinParams& B = p.bins;
     const uint32_t bin = blockIdx.x;
-    const uint32_t bx = bin % B.bins_x, by = bin / B.bins_x;
-    const uint32_t W = (uint32_t)p.width, H = (uint32_t)p.height;
+    const ViewParams& V = view_of(p, view_of_bin(p, bin));
+    const uint32_t lbin = bin - V.bin_base;
+    const uint32_t bx = lbin % V.bins_x, by = lbin / V.bins_x;
+    const uint32_t W = (uint32_t)V.width, H = (uint32_t)V.height;
     const uint32_t x0 = bx * BS, y0 = by * BS;
     const uint32_t x1 = min(x0 + BS, W), y1 = min(y0 + BS, H);
-    const uint32_t n = min(B.bin_counts[bin], B.bin_capacity);
-    if (n == 0 && !B.fresh) return;  // uniform across the block
+    const uint32_t n = min(p.bin_counts[bin], p.bin_capacity);
+    if (n == 0 && !V.fresh) return;  // uniform across the block
     const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
-    const int hw = p.width / 2, hh = p.height / 2;
-    const unsigned long long clearKey = (unsigned long long)fbits(p.clear_depth) << 32;
+    const int hw = V.width / 2, hh = V.height / 2;
+    const unsigned long long clearKey = (unsigned long long)fbits(V.clear_depth) << 32;
     const uint32_t rowElems = (x1 - x0) * 4;   // elements of one 4-row strip of tiles
+    unsigned long long* gkeys = V.keys;
 
     for (uint32_t i = tid; i < BS * BS; i += kThreads) {
         uint32_t strip = i / (BS * 4), rem = i % (BS * 4);
-        unsigned long long v = clearKey;
-        if (!B.fresh && rem < rowElems && y0 + strip * 4 < y1)
-            v = p.keys[(size_t)(y0 + strip * 4) * W + x0 * 4 + rem];
-        keys[i] = v;
+        unsigned long long val = clearKey;
+        if (!V.fresh && rem < rowElems && y0 + strip * 4 < y1)
+            val = gkeys[(size_t)(y0 + strip * 4) * W + x0 * 4 + rem];
+        keys[i] = val;
     }
 
     const bool countFrags = counting(p);
@@ -686,9 +748,9 @@ __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
         uint32_t ntiles = 0;
         TriLds t;
         if (e < n) {
-            const uint32_t r = B.bin_lists[(size_t)bin * B.bin_capacity + e];
+            const uint32_t r = p.bin_lists[(size_t)bin * p.bin_capacity + e];
             uint32_t bmin, bmax;
-            load_tri(p, r, hw, hh, t, bmin, bmax);
+            load_tri(p.records[r], hw, hh, t, bmin, bmax);
             uint32_t cx0 = max((uint32_t)lo16(bmin), x0), cy0 = max((uint32_t)hi16(bmin), y0);
             uint32_t cx1 = min((uint32_t)lo16(bmax), x1), cy1 = min((uint32_t)hi16(bmax), y1);
             t.org = 0;
@@ -710,16 +772,16 @@ __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
         uint32_t prev = 0, total = 0;
 #pragma unroll
         for (uint32_t w = 0; w < (uint32_t)kWaves; w++) {
-            uint32_t v = s_wave_total[w];
-            if (w < wid) prev += v;
-            total += v;
+            uint32_t val = s_wave_total[w];
+            if (w < wid) prev += val;
+            total += val;
         }
         t.start = prev + incl - ntiles;
         tl[tid] = t;
         if (e >= n) tl[tid].start = 0xFFFFFFFFu;
         __syncthreads();
 
-        // group g walks tiles [j0, j1) of the block's tile list
+        // group g walks tiles [j, j1) of the block's tile list
         const uint32_t per = (total + kGroups - 1) / kGroups;
         uint32_t j = min(g * per, total);
         const uint32_t j1 = min(j + per, total);
@@ -745,7 +807,7 @@ __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
                 float depth;
                 if (shade_pixel(T, x0 + lx, y0 + ly, depth)) {
                     covered++;
-                    if (depth > p.clear_depth) {
+                    if (depth > V.clear_depth) {
                         passed++;
                         unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
                         const uint32_t li = ((lx & ~3u) << 2) + (ly & ~3u) * BS + (lx & 3u) + ((ly & 3u) << 2);
@@ -760,7 +822,7 @@ __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
 
     for (uint32_t i = tid; i < BS * BS; i += kThreads) {
         uint32_t strip = i / (BS * 4), rem = i % (BS * 4);
-        if (rem < rowElems && y0 + strip * 4 < y1) p.keys[(size_t)(y0 + strip * 4) * W + x0 * 4 + rem] = keys[i];
+        if (rem < rowElems && y0 + strip * 4 < y1) gkeys[(size_t)(y0 + strip * 4) * W + x0 * 4 + rem] = keys[i];
     }
     if (countFrags) {
         for (int d = 32; d > 0; d >>= 1) {
@@ -829,6 +891,23 @@ __global__ void k_decode(const unsigned long long* keys, int width, int height, 
     if (depth) depth[o] = bitsf(db);
 }
 
+void launch_front(const DrawParams& p, hipStream_t s, Profiler* prof) {
+    uint32_t zb = min((p.total_bins + 255) / 256, 64u);
+    hipLaunchKernelGGL(k_begin_draw, dim3(max(zb, 1u)), dim3(256), 0, s, p.scratch, p.bin_counts, p.total_bins);
+    if (p.total_meshlets) {     // a batch of empty views still clears through the bin raster
+        Profiler::Scope sc(prof, "cull", s);
+        hipLaunchKernelGGL(k_cull, dim3((p.total_meshlets + 255) / 256), dim3(256), 0, s, p);
+    }
+    if (p.total_meshlets) {
+        Profiler::Scope sc(prof, "mesh_setup", s);
+        hipLaunchKernelGGL(k_mesh_setup, dim3((p.total_meshlets + 3) / 4), dim3(256), 0, s, p);
+    }
+    {
+        Profiler::Scope sc(prof, "clip", s);
+        hipLaunchKernelGGL(k_clip, dim3(256), dim3(256), 0, s, p);
+    }
+}
+
 }  // namespace
 
 hipError_t launch_clear(unsigned long long* keys, uint32_t n, unsigned long long key, hipStream_t s) {
@@ -850,62 +929,45 @@ hipError_t launch_cull(const DevMeshlets& m, uint32_t first, uint32_t count, con
                        uint32_t* visible, hipStream_t s) {
     Planes5 pl;
     for (int i = 0; i < 20; i++) pl.p[i] = planes[i];
-    hipLaunchKernelGGL(k_cull, dim3((count + 255) / 256), dim3(256), 0, s, m, first, count, pl, bitmap, visible);
+    hipLaunchKernelGGL(k_cull_range, dim3((count + 255) / 256), dim3(256), 0, s, m, first, count, pl, bitmap, visible);
     return hipGetLastError();
 }
 
-namespace {
-
-void launch_front(const DrawParams& p, hipStream_t s, Profiler* prof, uint32_t num_bins) {
-    uint32_t zb = min((num_bins + 255) / 256, 64u);
-    hipLaunchKernelGGL(k_begin_draw, dim3(max(zb, 1u)), dim3(256), 0, s, p.scratch, p.bins.bin_counts, num_bins);
-    if (p.use_cull) {
-        Profiler::Scope sc(prof, "cull", s);
-        Planes5 pl;
-        for (int i = 0; i < 20; i++) pl.p[i] = p.planes[i];
-        hipLaunchKernelGGL(k_cull, dim3((p.meshlet_count + 255) / 256), dim3(256), 0, s, p.m, p.meshlet_offset,
-                           p.meshlet_count, pl, p.cull_bitmap, &p.scratch->num_visible);
-    }
-    {
-        Profiler::Scope sc(prof, "mesh_setup", s);
-        hipLaunchKernelGGL(k_mesh_setup, dim3((p.meshlet_count + 3) / 4), dim3(256), 0, s, p);
-    }
-    if (p.clipping) {
-        Profiler::Scope sc(prof, "clip", s);
-        hipLaunchKernelGGL(k_clip, dim3(256), dim3(256), 0, s, p);
-    }
-}
-
-}  // namespace
-
 hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s, Profiler* prof) {
-    launch_front(p, s, prof, 0);
+    launch_front(p, s, prof);
     {
         Profiler::Scope sc(prof, "raster_direct", s);
-        uint32_t blocks = max((2 * p.meshlet_count + 3) / 4 + 16, 64u);
+        uint32_t blocks = max((2 * p.total_meshlets + 3) / 4 + 16, 64u);
         hipLaunchKernelGGL(k_raster_direct, dim3(blocks), dim3(256), 0, s, p, 0);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_draw_binned(const DrawParams& p, hipStream_t s, Profiler* prof) {
-    const uint32_t num_bins = p.bins.bins_x * p.bins.bins_y;
-    launch_front(p, s, prof, num_bins);
+    launch_front(p, s, prof);
     {
         Profiler::Scope sc(prof, "bin_scatter", s);
         // about one chunk per wave: two per meshlet plus room for clipped records
-        uint32_t blocks = max((2 * p.meshlet_count + 3) / 4 + 16, 64u);
+        uint32_t blocks = max((2 * p.total_meshlets + 3) / 4 + 16, 64u);
         hipLaunchKernelGGL(k_bin_scatter, dim3(blocks), dim3(256), 0, s, p);
     }
     {
         Profiler::Scope sc(prof, "bin_raster", s);
-        switch (p.bins.bin_size) {
-        case 8: hipLaunchKernelGGL((k_bin_raster<8, 256>), dim3(num_bins), dim3(256), 0, s, p); break;
-        case 16: hipLaunchKernelGGL((k_bin_raster<16, 256>), dim3(num_bins), dim3(256), 0, s, p); break;
-        case 32: hipLaunchKernelGGL((k_bin_raster<32, 256>), dim3(num_bins), dim3(256), 0, s, p); break;
-        case 64: hipLaunchKernelGGL((k_bin_raster<64, 256>), dim3(num_bins), dim3(256), 0, s, p); break;
+        const dim3 grid(p.total_bins);
+        const bool small = p.raster_threads == 128;
+#define GLIMP_BIN_CASE(BS)                                                                        \
+    case BS:                                                                                      \
+        if (small) hipLaunchKernelGGL((k_bin_raster<BS, 128>), grid, dim3(128), 0, s, p);         \
+        else hipLaunchKernelGGL((k_bin_raster<BS, 256>), grid, dim3(256), 0, s, p);               \
+        break;
+        switch (p.bin_size) {
+            GLIMP_BIN_CASE(8)
+            GLIMP_BIN_CASE(16)
+            GLIMP_BIN_CASE(32)
+            GLIMP_BIN_CASE(64)
         default: return hipErrorInvalidValue;
         }
+#undef GLIMP_BIN_CASE
     }
     {
         Profiler::Scope sc(prof, "raster_spill", s);

@@ -49,7 +49,7 @@ struct TriRecord {
     uint32_t pos[3];               // snapped 1/16 px, x | y << 16 (s16 each)
     float z0, z10, z20;            // depth plane (Rasterizer.cpp:321-323)
     uint32_t rank;                 // draw order, see key layout
-    uint32_t flags;                // bits 0-1 fragment entry point
+    uint32_t flags;                // bits 0-1 fragment entry point, bits 16-31 view index
 };
 static_assert(sizeof(TriRecord) == 32, "record size");
 
@@ -58,7 +58,8 @@ struct ClipEntry {
     uint32_t rank_base;            // rank with fan index 0
     uint32_t info;                 // outcodes | cull_mode << 8 | fs_id << 10
     float v[3][4];
-    uint32_t pad[2];
+    uint32_t view;                 // view index within the launch
+    uint32_t pad;
 };
 static_assert(sizeof(ClipEntry) == 64, "clip entry size");
 
@@ -105,52 +106,62 @@ struct DrawScratch {
     uint32_t pad[3];
 };
 
-// Screen binning parameters of a draw. Bins are bin_size x bin_size pixel
-// squares (a multiple of the 4x4 tile); each bin holds up to bin_capacity
-// record indices. A record that finds a bin full is appended to the spill
-// list and rasterized with global atomics after the bins; the depth-test max
-// is idempotent, so fragments rasterized twice leave the same result.
-struct BinParams {
-    uint32_t bin_size;             // 8, 16, 32 or 64
-    uint32_t bin_shift;            // log2(bin_size)
-    uint32_t bins_x, bins_y;
-    uint32_t bin_capacity;
-    uint32_t* bin_counts;          // [bins_x * bins_y]
-    uint32_t* bin_lists;           // [bins_x * bins_y][bin_capacity]
-    uint32_t* spill;               // [clip_record_base + clip_record_capacity]
-    int fresh;                     // first draw after a clear: start bins from the clear key
-};
-
-struct DrawParams {
+// One view of a draw or batch: which meshlets of which scene are drawn into
+// which framebuffer with which matrix. A single DrawMeshlets call is a batch
+// of one view; a batch renders many views with one launch per stage.
+struct ViewParams {
     DevMeshlets m;
     const DevMaterial* materials;
     uint32_t num_materials;
-    uint32_t meshlet_offset;
+    uint32_t meshlet_offset;       // ShadingContext::MeshletOffset
     uint32_t meshlet_count;
-    uint32_t seq_base;
-    float clip[16];
-    float planes[20];
-    int use_cull;
+    uint32_t meshlet_base;         // first global meshlet slot of the view
+    uint32_t bin_base;             // first global bin of the view
+    uint32_t bins_x, bins_y;
+    uint32_t seq_base;             // meshlets drawn into the framebuffer before
     int width, height;
     float guard_x, guard_y;
-    int clipping;
+    int use_cull, clipping, fresh;
+    float clear_depth;
+    unsigned long long* keys;      // the view's vis-buffer, 4x4 tiles
+    float clip[16];
+    float planes[20];
+};
+
+// Launch-wide parameters. Meshlets of all views are numbered globally
+// (view v owns slots [meshlet_base, meshlet_base + meshlet_count)), and so
+// are the screen bins (view v owns [bin_base, bin_base + bins_x*bins_y)).
+// Bins are bin_size x bin_size pixel squares (a multiple of the 4x4 tile);
+// each bin holds up to bin_capacity record indices. A record that finds a
+// bin full is appended to the spill list and rasterized with global atomics
+// after the bins; the depth-test max is idempotent, so fragments rasterized
+// twice leave the same result.
+struct DrawParams {
+    uint32_t num_views;
+    const ViewParams* views;       // device array, or null: the inline view
+    ViewParams view;
+    uint32_t total_meshlets;
+    uint32_t total_bins;
     uint32_t flags;
     ApproxTables tables;
-    uint16_t* cull_bitmap;         // scratch: (count + 15) / 16 words
-    TriRecord* records;            // [meshlet_count * 128 + clip_record_capacity]
-    uint32_t* meshlet_counts;      // [meshlet_count] records kept per meshlet
-    uint32_t clip_record_base;     // = meshlet_count * 128
+    uint16_t* cull_bitmap;         // bit g of the global meshlet numbering
+    TriRecord* records;            // [total_meshlets * 128 + clip_record_capacity]
+    uint32_t* meshlet_counts;      // [total_meshlets] records kept per meshlet
+    uint32_t clip_record_base;     // = total_meshlets * 128
     uint32_t clip_record_capacity;
     ClipEntry* clip_queue;
     uint32_t clip_capacity;
     DrawScratch* scratch;
     Counters* counters;
-    unsigned long long* keys;      // vis-buffer, 4x4 tiles
-    float clear_depth;
-    BinParams bins;
+    uint32_t bin_size, bin_shift, bin_capacity;
+    uint32_t raster_threads;       // bin raster workgroup size: 128 or 256
+    uint32_t* bin_counts;          // [total_bins]
+    uint32_t* bin_lists;           // [total_bins][bin_capacity]
+    uint32_t* spill;               // [clip_record_base + clip_record_capacity]
 };
 
-// Launchers (glimp_kernels.hip).
+// Launchers (glimp_kernels.hip). `max_meshlets` bounds the meshlets of
+// the largest view (grid sizing only).
 hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s, Profiler* prof);
 hipError_t launch_draw_binned(const DrawParams& p, hipStream_t s, Profiler* prof);
 hipError_t launch_clear(unsigned long long* keys, uint32_t n, unsigned long long key, hipStream_t s);

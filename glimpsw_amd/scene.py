@@ -289,6 +289,43 @@ def config1(num_meshlets: int = 10240, width: int = 1920, height: int = 1080, te
     return scene, make_view(width, height, count=n)
 
 
+def config2(num_scenes: int = 8, num_views: int = 64, min_meshlets: int = 1000, max_meshlets: int = 50000,
+            min_res: tuple = (256, 256), max_res: tuple = (1920, 1080)) -> tuple[list, list]:
+    """Batching workload: ``num_scenes`` seeded scenes with log-uniform
+    meshlet counts in [min_meshlets, max_meshlets], ``num_views`` views with
+    log-uniform resolution between ``min_res`` and ``max_res`` (multiples of
+    4; one draw per
+    view of its whole scene) and random cameras inside the scene. Patches
+    are as in config0 (scale 0.5-2, random rotation) at config0's density
+    (512 per 20^3), in a cube sized for the count; every scene has one
+    256x256 checker texture. Returns (scenes, [(scene index, View)])."""
+    rng = SplitMix64(SEED_BASE + 2)
+    counts = np.floor(rng.log_uniform(num_scenes, min_meshlets, max_meshlets)).astype(np.int64)
+    tex = textures.checker_texture(256)
+    scenes = []
+    sides = []
+    for n in counts:
+        n = int(n)
+        side = 20.0 * (n / 512.0) ** (1.0 / 3.0)
+        centers = rng.uniform(n * 3, -side / 2, side / 2).reshape(n, 3)
+        R = _random_rotations(rng, n)
+        scales = rng.uniform(n, 0.5, 2.0)
+        scenes.append(build_patches(centers, R, scales, np.zeros(n, np.uint32), [(0, False, 255)], [tex]))
+        sides.append(side)
+    views = []
+    for _ in range(num_views):
+        si = int(rng.uniform(1)[0] * num_scenes) % num_scenes
+        t = float(rng.uniform(1)[0])
+        # framebuffers are 4x4-tiled: sizes are rounded to multiples of 4
+        w = 4 * int(round(min_res[0] * (max_res[0] / min_res[0]) ** t / 4.0))
+        h = 4 * int(round(min_res[1] * (max_res[1] / min_res[1]) ** t / 4.0))
+        eye = rng.uniform(3, -0.4 * sides[si], 0.4 * sides[si])
+        yaw = float(rng.uniform(1, 0.0, 2.0 * math.pi)[0])
+        pitch = float(rng.uniform(1, -0.6, 0.6)[0])
+        views.append((si, make_view(w, h, 60.0, view_matrix(eye, yaw, pitch), 0, scenes[si].count)))
+    return scenes, views
+
+
 def inverse_screen_proj(world_to_clip: np.ndarray, width: int, height: int) -> np.ndarray:
     """GetInverseScreenProjMatrix (Camera.h:140-146): inverse(M) then
     translate(-1,-1,0), scale(2/size, 1), translate(0.5, 0.5, 0), so pixel

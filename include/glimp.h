@@ -158,6 +158,50 @@ int glimp_draw_meshlets(glimp_context* ctx, glimp_framebuffer* fb, glimp_scene* 
 int glimp_cull_meshlets(glimp_context* ctx, glimp_scene* scene, uint32_t first, uint32_t count,
                         const float planes[20], uint16_t* bitmap, uint32_t* visible, void* stream);
 
+/* ------------------------------------------------------------ batches */
+
+/* Many views (scene, meshlet range, matrix, framebuffer) rendered together:
+ * every stage of the pipeline is one launch over all views, and the whole
+ * batch can be replayed as a single hipGraph launch. No reference
+ * counterpart: the reference renders one view per DrawMeshlets call
+ * (Main.cpp:216-245); a batch gives each view exactly the vis-buffer that a
+ * clear plus one DrawMeshlets would. */
+typedef struct glimp_batch glimp_batch;
+
+typedef struct glimp_view_desc {
+    glimp_scene* scene;
+    glimp_framebuffer* framebuffer;    /* one framebuffer per view, not shared */
+    uint32_t meshlet_offset, meshlet_count;
+    float object_to_clip[16];
+    const float* frustum_planes;       /* host [5][4] or NULL */
+    uint32_t flags;                    /* GLIMP_DRAW_NO_CLIPPING / GLIMP_DRAW_NO_GUARDBAND */
+    uint32_t clear_color;
+    float clear_depth;                 /* >= 0 */
+} glimp_view_desc;
+
+/* Launch shape: screen bin side, records per bin, bin-raster workgroup size. */
+typedef struct glimp_batch_shape {
+    uint32_t bin_size;                 /* 8, 16, 32 or 64 */
+    uint32_t bin_capacity;             /* records per bin before spilling */
+    uint32_t raster_threads;           /* 128 or 256 */
+} glimp_batch_shape;
+
+int glimp_batch_create(glimp_context* ctx, const glimp_view_desc* views, uint32_t count,
+                       const glimp_batch_shape* shape /* NULL = default */, glimp_batch** out);
+void glimp_batch_destroy(glimp_batch* batch);
+int glimp_batch_set_shape(glimp_batch* batch, const glimp_batch_shape* shape);
+/* Clears each view's framebuffer and draws its meshlets, one launch per
+ * stage for all views; use_graph != 0 replays the batch as one hipGraph
+ * (captured on first use, re-captured after a shape change). Asynchronous. */
+int glimp_batch_launch(glimp_batch* batch, int use_graph, void* stream);
+/* The same work as glimp_framebuffer_clear + glimp_draw_meshlets per view,
+ * view after view (for comparison with the batched launch). */
+int glimp_batch_launch_per_view(glimp_batch* batch, void* stream);
+/* Autotuner: times the candidate shapes (bin 8-64, capacity 256-4096,
+ * 128/256-thread bin raster) on this batch, keeps the fastest and returns
+ * it with its time per launch. Synchronous. */
+int glimp_batch_autotune(glimp_batch* batch, void* stream, glimp_batch_shape* chosen, float* best_ms);
+
 /* ------------------------------------------------------------ resolve */
 
 #define GLIMP_RESOLVE_ALBEDO 0u   /* ResolveDebug(DebugLayer::BaseColor) (Shading.cpp:734-773) */

@@ -1,0 +1,175 @@
+"""Batched rendering: one launch per stage (and one hipGraph) over many views
+of several scenes gives every view exactly the vis-buffer of a clear plus one
+DrawMeshlets of that view, for every launch shape the autotuner may pick; a
+small batch is also checked against the CPU oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from glimpsw_amd import scene as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_config2(num_views=10):
+    return S.config2(num_scenes=4, num_views=num_views, min_meshlets=100, max_meshlets=1500,
+                     min_res=(64, 64), max_res=(480, 270))
+
+
+class _Setup:
+    def __init__(self, ctx, scenes, views):
+        import glimpsw_amd as g
+
+        self.g = g
+        self.dscenes = [g.Scene(ctx, sc) for sc in scenes]
+        self.fbs = [g.Framebuffer(ctx, v.width, v.height) for _, v in views]
+        self.items = [(self.dscenes[si], fb, v) for (si, v), fb in zip(views, self.fbs)]
+
+    def read_all(self):
+        return [fb.read() for fb in self.fbs]
+
+    def close(self):
+        for fb in self.fbs:
+            fb.close()
+        for d in self.dscenes:
+            d.close()
+
+
+def _same(a, b, label):
+    for i, ((ia, da), (ib, db)) in enumerate(zip(a, b)):
+        bad = (ia != ib) | (da.view(np.uint32) != db.view(np.uint32))
+        assert not bad.any(), f"{label}: view {i} differs at {int(bad.sum())} pixels"
+
+
+@pytest.fixture(scope="module")
+def small(gpu_ctx):
+    scenes, views = _small_config2()
+    st = _Setup(gpu_ctx, scenes, views)
+    yield scenes, views, st
+    st.close()
+
+
+def test_batch_matches_per_view(gpu_ctx, small):
+    scenes, views, st = small
+    g = st.g
+    b = g.Batch(gpu_ctx, st.items)
+    print(f"[batch] per-view vs batched, {len(views)} views", flush=True)
+    b.launch_per_view()
+    want = st.read_all()
+    assert any((ids != 0).any() for ids, _ in want), "nothing drawn"
+    b.launch(graph=False)
+    _same(want, st.read_all(), "eager batch")
+    b.launch(graph=True)
+    _same(want, st.read_all(), "graph batch")
+    b.close()
+
+
+def test_batch_vs_oracle(gpu_ctx, small):
+    scenes, views, st = small
+    g = st.g
+    b = g.Batch(gpu_ctx, st.items)
+    b.launch(graph=True)
+    got = st.read_all()
+    b.close()
+    bindings = {}
+    for i, (si, v) in enumerate(views[:6]):
+        print(f"[batch] oracle view {i} {v.width}x{v.height} scene {si}", flush=True)
+        if si not in bindings:
+            bindings[si] = oracle.SceneBinding(scenes[si])
+        ofb, _, _ = oracle.render_frame(scenes[si], v, bindings[si])
+        ids, depth = ofb.ids(), ofb.depth()
+        _same([(ids, depth)], [got[i]], f"oracle view {i}")
+
+
+def test_graph_replays_with_work_between(gpu_ctx, small):
+    """Every replay of the captured graph (with eager draws clobbering the
+    framebuffers between replays) reproduces the per-view result."""
+    scenes, views, st = small
+    g = st.g
+    b = g.Batch(gpu_ctx, st.items)
+    b.launch_per_view()
+    want = st.read_all()
+    for r in range(3):
+        b.launch(graph=True)
+        _same(want, st.read_all(), f"replay {r}")
+        for (ds, fb, v) in st.items[:3]:
+            fb.clear(0x123, 0.0)
+            fb.draw(ds, v.object_to_clip, v.meshlet_count // 2, 0)
+    b.close()
+
+
+@pytest.mark.parametrize("bin_size,capacity,threads", [
+    (8, 256, 128), (16, 64, 256), (32, 4096, 256), (64, 2048, 128), (64, 3, 256),
+])
+def test_batch_shapes(gpu_ctx, small, bin_size, capacity, threads):
+    scenes, views, st = small
+    g = st.g
+    b = g.Batch(gpu_ctx, st.items, shape={"bin_size": bin_size, "bin_capacity": capacity,
+                                         "raster_threads": threads})
+    print(f"[batch] shape {bin_size}/{capacity}/{threads}", flush=True)
+    b.launch_per_view()
+    want = st.read_all()
+    b.launch(graph=True)
+    _same(want, st.read_all(), f"shape {bin_size}/{capacity}/{threads}")
+    b.close()
+
+
+def test_autotune_keeps_results(gpu_ctx, small):
+    scenes, views, st = small
+    g = st.g
+    b = g.Batch(gpu_ctx, st.items)
+    b.launch_per_view()
+    want = st.read_all()
+    shape, ms = b.autotune()
+    print(f"[batch] autotune -> {shape} {ms:.3f} ms", flush=True)
+    assert shape["bin_size"] in (8, 16, 32, 64)
+    assert 256 <= shape["bin_capacity"] <= 4096
+    assert shape["raster_threads"] in (128, 256)
+    assert ms > 0.0
+    b.launch(graph=True)
+    _same(want, st.read_all(), "autotuned")
+    b.close()
+
+
+def test_batch_then_resolve(gpu_ctx, small):
+    """A batched view resolves like the same view drawn on its own."""
+    scenes, views, st = small
+    g = st.g
+    b = g.Batch(gpu_ctx, st.items)
+    b.launch(graph=True)
+    ds, fb, v = st.items[1]
+    fb.resolve(ds, v.object_to_clip, g.GLIMP_RESOLVE_ALBEDO)
+    got = fb.read_color()
+    b.close()
+    ofb, _, bind = oracle.render_frame(scenes[views[1][0]], v)
+    want = oracle.resolve_albedo(ofb, bind, v.object_to_clip)
+    assert (got == want).all()
+
+
+def test_empty_views_are_cleared(gpu_ctx):
+    import glimpsw_amd as g
+
+    scenes, views = S.config2(num_scenes=2, num_views=3, min_meshlets=50, max_meshlets=200,
+                              min_res=(40, 24), max_res=(100, 60))
+    st = _Setup(gpu_ctx, scenes, views)
+    for _, fb, _ in st.items:
+        fb.clear(0, 0.0)
+        fb.draw(st.items[0][0], views[0][1].object_to_clip, views[0][1].meshlet_count, 0)
+    items = [(ds, fb, S.View(v.width, v.height, v.proj, v.view, v.object_to_clip, 0, 0))
+             for ds, fb, v in st.items]
+    b = g.Batch(gpu_ctx, items, clear_depth=0.25)
+    b.launch(graph=True)
+    for ids, depth in st.read_all():
+        assert (ids == 0).all() and (depth == np.float32(0.25)).all()
+    b.close()
+    st.close()
+
+
+def test_batch_rejects_shared_framebuffer(gpu_ctx, small):
+    scenes, views, st = small
+    g = st.g
+    items = [st.items[0], (st.items[1][0], st.items[0][1], st.items[1][2])]
+    with pytest.raises(g.GlimpError):
+        g.Batch(gpu_ctx, items)
+    with pytest.raises(g.GlimpError):
+        g.Batch(gpu_ctx, st.items[:2], shape={"bin_size": 12, "bin_capacity": 64})
