@@ -4,7 +4,9 @@
 One step = one frame of the seeded 1080p scene with 10,240 grid-patch
 meshlets (1,003,520 triangles, 16 materials): clear, frustum cull, mesh +
 setup, guard-band clip and raster into the 64-bit vis-buffer, then the
-albedo resolve. Inputs are resident in HBM before the timed region.
+lit resolve. Inputs are resident in HBM before the timed region. A second
+measurement renders the 64-view batching workload as one hipGraph replay and
+as per-view launches (views/s, same vis-buffers).
 
 Prints one JSON line: Mtri/s (submitted triangles per second) and fps, the
 roofline of the dominant kernel from HIP-event kernel times taken inside the
@@ -84,6 +86,59 @@ def cpu_baseline(scene, view, seconds: float) -> dict:
     }
 
 
+def bench_batch(ctx, stream, steps: int, warmup: int) -> dict:
+    """Batching workload: 64 views over 8 scenes of 1k-50k meshlets, one
+    hipGraph replay per step (shape picked by the autotuner) against the
+    same views drawn one clear+draw call at a time."""
+    import torch
+    import glimpsw_amd as g
+    from glimpsw_amd import scene as S
+
+    scenes, views = S.batch_views()
+    dscenes = [g.Scene(ctx, sc) for sc in scenes]
+    fbs = [g.Framebuffer(ctx, v.width, v.height) for _, v in views]
+    b = g.Batch(ctx, [(dscenes[si], fb, v) for (si, v), fb in zip(views, fbs)])
+    shape, tune_ms = b.autotune(stream)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    ms_graph = timed(lambda: b.launch(True, stream))
+    want = [fb.read() for fb in fbs]
+    ms_per_view = timed(lambda: b.launch_per_view(stream))
+    got = [fb.read() for fb in fbs]
+    identical = all((wi == gi).all() and (wd.view(np.uint32) == gd.view(np.uint32)).all()
+                    for (wi, wd), (gi, gd) in zip(want, got))
+    b.close()
+    for fb in fbs:
+        fb.close()
+    for d in dscenes:
+        d.close()
+    n = len(views)
+    tris = sum(scenes[si].triangles() for si, _ in views)
+    return {
+        "workload": f"{n} views over {len(scenes)} scenes of {min(s.count for s in scenes)}-"
+                    f"{max(s.count for s in scenes)} meshlets, {sum(v.width * v.height for _, v in views)} pixels",
+        "views_per_s": round(n / (ms_graph * 1e-3), 1),
+        "views_per_s_per_view_launch": round(n / (ms_per_view * 1e-3), 1),
+        "speedup": round(ms_per_view / ms_graph, 2),
+        "ms_per_batch": round(ms_graph, 4),
+        "ms_per_batch_per_view_launch": round(ms_per_view, 4),
+        "submitted_tris": tris,
+        "Mtri_per_s": round(tris / (ms_graph * 1e-3) / 1e6, 1),
+        "autotuned_shape": shape,
+        "autotune_ms": round(tune_ms, 4),
+        "identical_vis_buffers": bool(identical),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +149,7 @@ def main() -> int:
     ap.add_argument("--direct", action="store_true", help="global-atomic raster instead of tile bins")
     ap.add_argument("--bin-size", type=int, default=0, help="screen bin side (8/16/32/64), 0 = library default")
     ap.add_argument("--bin-capacity", type=int, default=0, help="records per bin, 0 = library default")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batching measurement")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES bytes 10 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -119,7 +175,7 @@ def main() -> int:
     from glimpsw_amd import scene as S
 
     t_gen = time.perf_counter()
-    scene, view = S.config1()
+    scene, view = S.frame_1080p()
     t_gen = time.perf_counter() - t_gen
     torch.cuda.init()
     stream = torch.cuda.current_stream()
@@ -192,8 +248,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "int32+fp32",
         "data": "synthetic",
-        "config": {"workload": "C1: 10,240 grid-patch meshlets (1,003,520 tris), 1920x1080, depth complexity ~3, "
-                               "16 materials 2048^2 RGBA8", "raster": "direct" if args.direct else "binned 32x32",
+        "config": {"workload": "10,240 grid-patch meshlets (1,003,520 tris), 1920x1080, depth complexity ~3, "
+                               "16 materials 2048^2 RGBA8", "raster": "direct" if args.direct else f"binned {args.bin_size or 16}x{args.bin_size or 16}",
                    "submitted_tris": tris},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dk["GBps"] / HBM_PEAK_GBS, 4) if dk["GBps"] else None,
@@ -202,6 +258,8 @@ def main() -> int:
         "frame_counts": stats,
         "scene_gen_s": round(t_gen, 2),
     }
+    if not args.no_batch:
+        result["batch"] = bench_batch(ctx, stream, max(5, args.steps // 5), 3)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(scene, view, args.cpu_seconds)
     else:

@@ -68,6 +68,17 @@ class BatchShape(C.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class DiffTextureDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("mip_levels", C.c_uint32), ("pad", C.c_uint32),
+                ("level_offset", C.c_uint64 * 16)]
+
+
+class DiffDesc(C.Structure):
+    _fields_ = [("object_to_clip", C.c_float * 16), ("textures", C.POINTER(DiffTextureDesc)),
+                ("num_textures", C.c_uint32), ("num_texels", C.c_uint64), ("texels", C.c_void_p),
+                ("uv", C.c_void_p), ("color", C.c_void_p)]
+
+
 class CountersStruct(C.Structure):
     _fields_ = [("triangles_processed", C.c_uint64), ("triangles_rasterized", C.c_uint64),
                 ("triangles_clipped", C.c_uint64), ("bins_rasterized", C.c_uint64),
@@ -145,6 +156,10 @@ _SIGNATURES = {
     "glimp_batch_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "glimp_batch_launch_per_view": (C.c_int, [C.c_void_p, C.c_void_p]),
     "glimp_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BatchShape), C.POINTER(C.c_float)]),
+    "glimp_diff_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(DiffDesc), C.c_void_p,
+                                     C.c_void_p]),
+    "glimp_diff_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(DiffDesc), C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "glimp_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "glimp_profile_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_uint32, C.POINTER(C.c_uint32), C.c_int]),
 }

@@ -12,6 +12,7 @@
 #include "glimp.h"
 #include "internal.h"
 #include "resolve.h"
+#include "diff.h"
 
 // 131072 u16: VRCP14 then VRSQRT14 mantissa tables, generated at build time
 // from glimpsw_amd/data/approx14.bin.
@@ -70,8 +71,8 @@ struct Scratch {
     size_t records_cap = 0;
     ClipEntry* clip = nullptr;
     size_t clip_cap = 0;
-    uint16_t* bitmap = nullptr;
-    size_t bitmap_cap = 0;
+    uint32_t* visible_list = nullptr;
+    size_t visible_list_cap = 0;
     uint32_t* meshlet_counts = nullptr;
     size_t meshlet_counts_cap = 0;
     uint32_t* bin_counts = nullptr;
@@ -83,7 +84,7 @@ struct Scratch {
     DrawScratch* scratch = nullptr;
 
     void release() {
-        void* ptrs[] = { records, clip, bitmap, meshlet_counts, bin_counts, bin_lists, spill, scratch };
+        void* ptrs[] = { records, clip, visible_list, meshlet_counts, bin_counts, bin_lists, spill, scratch };
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         *this = Scratch();
@@ -101,6 +102,10 @@ struct glimp_context {
     uint16_t* bitmap = nullptr;        // glimp_cull_meshlets output
     size_t bitmap_cap = 0;
     Scratch draw_scratch;              // scratch of single draws
+    DiffWorkspace diff_ws;             // differentiable resolve backward
+    std::vector<DiffTex> diff_tex;     // last texture table uploaded to d_diff_tex
+    DiffTex* d_diff_tex = nullptr;
+    size_t d_diff_tex_cap = 0;
     Profiler prof;
 
     ApproxTables approx() const { return ApproxTables{ tables, tables + 65536 }; }
@@ -183,7 +188,8 @@ void glimp_destroy(glimp_context* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
-    void* ptrs[] = { c->tables, c->counters, c->scratch, c->visible, c->bitmap };
+    diff_workspace_release(c->diff_ws);
+    void* ptrs[] = { c->tables, c->counters, c->scratch, c->visible, c->bitmap, c->d_diff_tex };
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c->draw_scratch.release();
@@ -496,7 +502,7 @@ int prepare_scratch(Scratch& S, size_t total_meshlets, size_t total_bins, const 
     const size_t recs = total_meshlets * 128 + clipq * 8;
     int rc = grow_one(S.records, S.records_cap, recs);
     if (rc == GLIMP_OK) rc = grow_one(S.clip, S.clip_cap, clipq);
-    if (rc == GLIMP_OK) rc = grow_one(S.bitmap, S.bitmap_cap, (total_meshlets + 15) / 16 + 1);
+    if (rc == GLIMP_OK) rc = grow_one(S.visible_list, S.visible_list_cap, std::max<size_t>(total_meshlets, 1));
     if (rc == GLIMP_OK) rc = grow_one(S.meshlet_counts, S.meshlet_counts_cap, total_meshlets);
     if (rc == GLIMP_OK) rc = grow_one(S.bin_counts, S.bin_counts_cap, std::max<size_t>(total_bins, 1));
     if (rc == GLIMP_OK && binned) rc = grow_one(S.bin_lists, S.bin_lists_cap, total_bins * sh.bin_capacity);
@@ -514,7 +520,7 @@ void fill_params(glimp_context* c, const Scratch& S, size_t total_meshlets, uint
     p.total_bins = total_bins;
     p.flags = flags;
     p.tables = c->approx();
-    p.cull_bitmap = S.bitmap;
+    p.visible_list = S.visible_list;
     p.records = S.records;
     p.meshlet_counts = S.meshlet_counts;
     p.clip_record_base = (uint32_t)(total_meshlets * 128);
@@ -904,6 +910,115 @@ int glimp_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const
     p.out = f->color;
     p.linear = 1;
     HIP_TRY(launch_resolve(p, s, &c->prof));
+    return GLIMP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Shared set-up of the differentiable resolve calls.
+int diff_params(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const glimp_diff_desc* d, hipStream_t s,
+                const char* who, DiffParams& p, uint64_t& num_vertices) {
+    if (!c || !f || !sc || !d) return fail(GLIMP_E_INVALID, std::string(who) + ": NULL argument");
+    if (!d->texels || !d->uv || !d->color || (d->num_textures && !d->textures))
+        return fail(GLIMP_E_INVALID, std::string(who) + ": NULL parameter array");
+    if (d->num_textures > GLIMP_MAX_DIFF_TEXTURES)
+        return fail(GLIMP_E_LIMIT, std::string(who) + ": too many textures");
+    if (d->num_texels >= 0xFFFFFFFFull)
+        return fail(GLIMP_E_LIMIT, std::string(who) + ": texel indices must fit 32 bits");
+    num_vertices = (uint64_t)sc->m.count * 64;
+    if (num_vertices >= 0xFFFFFFFFull) return fail(GLIMP_E_LIMIT, std::string(who) + ": too many vertices");
+    std::vector<DiffTex> tex(d->num_textures);
+    for (uint32_t i = 0; i < d->num_textures; i++) {
+        const glimp_diff_texture& t = d->textures[i];
+        if (t.width == 0 || t.height == 0 || t.mip_levels == 0 || t.mip_levels > 16)
+            return fail(GLIMP_E_INVALID, std::string(who) + ": bad texture size or level count");
+        for (uint32_t m = 0; m < t.mip_levels; m++) {
+            const uint64_t w = std::max<uint64_t>(t.width >> m, 1), h = std::max<uint64_t>(t.height >> m, 1);
+            if ((t.width >> m) == 0 || (t.height >> m) == 0 || t.level_offset[m] + w * h > d->num_texels)
+                return fail(GLIMP_E_INVALID, std::string(who) + ": texture level outside the texel array");
+        }
+        DiffTex& D = tex[i];
+        memset(&D, 0, sizeof(D));
+        D.width = t.width;
+        D.height = t.height;
+        D.levels = t.mip_levels;
+        for (uint32_t m = 0; m < 16; m++) D.level_offset[m] = t.level_offset[m];
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(flush_clear(f, s));
+    const bool same = tex.size() == c->diff_tex.size() &&
+                      (tex.empty() || memcmp(tex.data(), c->diff_tex.data(), tex.size() * sizeof(DiffTex)) == 0);
+    if (!same) {
+        int rc = grow_one(c->d_diff_tex, c->d_diff_tex_cap, std::max<size_t>(tex.size(), 1));
+        if (rc != GLIMP_OK) return rc;
+        if (!tex.empty()) {
+            HIP_TRY(hipStreamSynchronize(s));
+            HIP_TRY(hipMemcpy(c->d_diff_tex, tex.data(), tex.size() * sizeof(DiffTex), hipMemcpyHostToDevice));
+        }
+        c->diff_tex = tex;
+    }
+    memset(&p, 0, sizeof(p));
+    p.m = sc->m;
+    p.materials = sc->materials;
+    p.num_materials = sc->num_materials;
+    p.textures = c->d_diff_tex;
+    p.num_textures = d->num_textures;
+    p.tables = c->approx();
+    p.keys = f->keys;
+    p.width = (int)f->width;
+    p.height = (int)f->height;
+    uint32_t db;
+    memcpy(&db, &f->clear_depth, 4);
+    p.clear_key = (unsigned long long)db << 32;
+    int rc = upload_draws(f, s, &p.num_draws);
+    if (rc != GLIMP_OK) return rc;
+    p.draws = f->d_draws;
+    memcpy(p.object_to_clip, d->object_to_clip, sizeof(p.object_to_clip));
+    const float sw = (float)f->width, sh = (float)f->height;
+    p.scale_x = 2.0f / sw;
+    p.scale_y = 2.0f / sh;
+    p.center_x = std::fma(0.5f, p.scale_x, -1.0f);
+    p.center_y = std::fma(0.5f, p.scale_y, -1.0f);
+    p.kx = 2.0f / sw;
+    p.ky = -2.0f / sh;
+    p.texels = (const float4*)d->texels;
+    p.uv = (const float2*)d->uv;
+    p.color = (const float4*)d->color;
+    return GLIMP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int glimp_diff_forward(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const glimp_diff_desc* d, float* out,
+                       void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    DiffParams p;
+    uint64_t nv = 0;
+    int rc = diff_params(c, f, sc, d, s, "glimp_diff_forward", p, nv);
+    if (rc != GLIMP_OK) return rc;
+    if (!out) return fail(GLIMP_E_INVALID, "glimp_diff_forward: NULL output");
+    p.out = (float4*)out;
+    HIP_TRY(launch_diff_forward(p, s, &c->prof));
+    return GLIMP_OK;
+}
+
+int glimp_diff_backward(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const glimp_diff_desc* d,
+                        const float* grad_out, float* grad_texels, float* grad_uv, float* grad_color, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    DiffParams p;
+    uint64_t nv = 0;
+    int rc = diff_params(c, f, sc, d, s, "glimp_diff_backward", p, nv);
+    if (rc != GLIMP_OK) return rc;
+    if (!grad_out || !grad_texels || !grad_uv || !grad_color)
+        return fail(GLIMP_E_INVALID, "glimp_diff_backward: NULL gradient array");
+    p.grad_out = (const float4*)grad_out;
+    HIP_TRY(diff_workspace_reserve(c->diff_ws, (size_t)f->width * f->height));
+    HIP_TRY(launch_diff_backward(p, c->diff_ws, d->num_texels, (uint32_t)nv, (float4*)grad_texels, (float2*)grad_uv,
+                                 (float4*)grad_color, s, &c->prof));
     return GLIMP_OK;
 }
 

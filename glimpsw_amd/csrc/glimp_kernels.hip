@@ -30,6 +30,8 @@ namespace glimp {
 namespace {
 
 constexpr int kWave = 64;
+constexpr uint32_t kXcds = 8;                // MI355X: 8 XCDs of 32 CUs
+constexpr uint32_t kPersistentBlocks = 2048; // looping grids: 8 blocks of 256 threads per CU
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -173,30 +175,56 @@ __device__ __forceinline__ bool counting(const DrawParams& p) { return (p.flags 
 // ---------------------------------------------------------------- cull
 
 // Frustum test of bound spheres (Shading.cpp:797-809) over the global
-// meshlet numbering; the bitmap word for 16 slots comes from a wave ballot.
-__global__ void k_cull(DrawParams p) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    bool vis = false;
-    if (g < p.total_meshlets) {
-        const ViewParams& V = view_of(p, view_of_meshlet(p, g));
-        vis = true;
-        if (V.use_cull) {
-            float4 b = V.m.bounds[V.meshlet_offset + (g - V.meshlet_base)];
+// meshlet numbering. Survivors are compacted into visible_list so later
+// stages launch work only for them: a block covers kCullPerBlock consecutive
+// meshlets and claims its slice of the list with one atomic.
+constexpr uint32_t kCullPerThread = 4;
+constexpr uint32_t kCullPerBlock = 256 * kCullPerThread;
+
+__global__ void __launch_bounds__(256) k_cull(DrawParams p) {
+    __shared__ uint32_t s_wave[4];
+    __shared__ uint32_t s_base;
+    const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+    bool vis[kCullPerThread];
+    uint32_t mine = 0;
 #pragma unroll
-            for (int k = 0; k < 5; k++) {
-                const float* P = V.planes + 4 * k;
-                float d = __builtin_fmaf(b.x, P[0], __builtin_fmaf(b.y, P[1], b.z * P[2])) + P[3];
-                vis = vis && (d > -b.w);
+    for (uint32_t k = 0; k < kCullPerThread; k++) {
+        const uint32_t g = blockIdx.x * kCullPerBlock + k * 256 + threadIdx.x;
+        bool v = false;
+        if (g < p.total_meshlets) {
+            const ViewParams& V = view_of(p, view_of_meshlet(p, g));
+            v = true;
+            if (V.use_cull) {
+                float4 b = V.m.bounds[V.meshlet_offset + (g - V.meshlet_base)];
+#pragma unroll
+                for (int j = 0; j < 5; j++) {
+                    const float* P = V.planes + 4 * j;
+                    float d = __builtin_fmaf(b.x, P[0], __builtin_fmaf(b.y, P[1], b.z * P[2])) + P[3];
+                    v = v && (d > -b.w);
+                }
             }
         }
+        vis[k] = v;
+        mine += v ? 1u : 0u;
     }
-    unsigned long long mask = __ballot(vis);
-    const uint32_t lane = lane_id();
-    if ((lane & 15) == 0 && g < p.total_meshlets) p.cull_bitmap[g / 16] = (uint16_t)(mask >> (lane & 48));
-    if (counting(p) && lane == 0) {
-        uint32_t c = __popcll(mask);
-        if (c) atomicAdd(&p.scratch->num_visible, c);
+    uint32_t incl = mine;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t o = __shfl_up(incl, d);
+        if ((int)lane >= d) incl += o;
     }
+    if (lane == 63) s_wave[wid] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        s_base = total ? atomicAdd(&p.scratch->num_visible, total) : 0;
+    }
+    __syncthreads();
+    uint32_t pos = s_base + incl - mine;
+    for (uint32_t w = 0; w < wid; w++) pos += s_wave[w];
+#pragma unroll
+    for (uint32_t k = 0; k < kCullPerThread; k++)
+        if (vis[k]) p.visible_list[pos++] = blockIdx.x * kCullPerBlock + k * 256 + threadIdx.x;
 }
 
 // Standalone cull of one meshlet range (ShadingContext::CullMeshlets).
@@ -225,26 +253,22 @@ __global__ void k_cull_range(DevMeshlets m, uint32_t first, uint32_t count, Plan
 
 // ---------------------------------------------------------------- mesh + setup
 
-// Record layout: global meshlet slot g owns records [g*128, g*128 +
-// meshlet_counts[g]); clipped sub-triangles are appended after all meshlet
+// Record layout: the i-th visible meshlet owns records [i*128, i*128 +
+// meshlet_counts[i]); clipped sub-triangles are appended after all meshlet
 // slots. Setup needs no global counter: a wave compacts its survivors with a
-// ballot and writes them into its own slots.
-__global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
-    __shared__ float lds_pos[4][4][64];
-    const uint32_t wid = threadIdx.x / kWave;
-    const uint32_t lane = lane_id();
-    const uint32_t g = blockIdx.x * 4 + wid;   // global meshlet slot
-    const bool have = g < p.total_meshlets;
-    const uint32_t v = have ? view_of_meshlet(p, g) : 0;
+// ballot and writes them into its own slots. Waves loop over the visible
+// list, so the grid is sized to the machine, not to the scene.
+__device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, float (&lds)[4][64]) {
+    const uint32_t g = p.visible_list[i];      // global meshlet number
+    const uint32_t v = view_of_meshlet(p, g);
     const ViewParams& V = view_of(p, v);
     const uint32_t w = g - V.meshlet_base;     // meshlet index within the view
     const uint32_t mi = V.meshlet_offset + w;
     const int hw = V.width / 2, hh = V.height / 2;
 
-    bool visible = have && ((p.cull_bitmap[g / 16] >> (g % 16)) & 1) != 0;
-    uint32_t nt = visible ? V.m.num_triangles[mi] : 0;
+    const uint32_t nt = V.m.num_triangles[mi];
     int cull_mode = 1, fs_id = 0;
-    if (visible) {
+    {
         uint32_t mat = V.m.material_id[mi];
         if (mat != 0xFFFFFFFFu && mat < V.num_materials) {
             cull_mode = V.materials[mat].cull_mode;
@@ -253,14 +277,14 @@ __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
         // ShadeMeshlet (Shading.cpp:295-298): lane = vertex
         const float* P = V.m.positions + (size_t)mi * 192;
         float px = P[lane], py = P[64 + lane], pz = P[128 + lane];
+        __builtin_amdgcn_wave_barrier();   // the previous meshlet's reads are done
 #pragma unroll
-        for (int c = 0; c < 4; c++) lds_pos[wid][c][lane] = xform(V.clip, c, px, py, pz);
+        for (int c = 0; c < 4; c++) lds[c][lane] = xform(V.clip, c, px, py, pz);
+        __builtin_amdgcn_wave_barrier();
     }
-    __syncthreads();
-    if (!have) return;
     const uint32_t seq = V.seq_base + w;
     const uint8_t* I = V.m.indices + (size_t)mi * 384;
-    TriRecord* out = p.records + (size_t)g * 128;
+    TriRecord* out = p.records + (size_t)i * 128;
     const uint32_t flags_hi = v << 16;
     uint32_t kept = 0, clipped = 0;
 
@@ -279,7 +303,7 @@ __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
                 uint32_t ix = I[k * 128 + t];
                 uint32_t vi = ix < 32 ? ix : 32 + (ix & 31);   // gather_preload64 (SIMD.h:219-227)
 #pragma unroll
-                for (int c = 0; c < 4; c++) vv[k][c] = lds_pos[wid][c][vi];
+                for (int c = 0; c < 4; c++) vv[k][c] = lds[c][vi];
                 float x = vv[k][0], y = vv[k][1], z = vv[k][2], ww = vv[k][3];
                 uint32_t oc = 0;
                 oc |= (x < -ww) ? 1u : 0u;
@@ -334,13 +358,22 @@ __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
         }
     }
     if (lane == 0) {
-        p.meshlet_counts[g] = kept;
-        if (counting(p) && visible) {
+        p.meshlet_counts[i] = kept;
+        if (counting(p)) {
             atomicAdd(&p.counters->triangles_processed, (unsigned long long)nt);
             atomicAdd(&p.counters->triangles_rasterized, (unsigned long long)kept);
             if (clipped) atomicAdd(&p.counters->triangles_clipped, (unsigned long long)clipped);
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
+    __shared__ float lds_pos[4][4][64];
+    const uint32_t wid = threadIdx.x / kWave;
+    const uint32_t lane = lane_id();
+    const uint32_t nvis = p.scratch->num_visible;
+    const uint32_t waves = gridDim.x * 4;
+    for (uint32_t i = blockIdx.x * 4 + wid; i < nvis; i += waves) setup_meshlet(p, i, lane, lds_pos[wid]);
 }
 
 // ---------------------------------------------------------------- clipper
@@ -463,9 +496,9 @@ __global__ void __launch_bounds__(256) k_clip(DrawParams p) {
 
 // ---------------------------------------------------------------- record chunks
 
-// Chunks of up to 64 records for a wave: chunk c < 2*total_meshlets is half
-// c&1 of meshlet slot c>>1; later chunks cover the clipped records; in spill
-// mode chunk c covers spill-list entries [64c, 64c+64).
+// Chunks of up to 64 records for a wave: chunk c < 2*num_visible is half
+// c&1 of visible meshlet slot c>>1; later chunks cover the clipped records;
+// in spill mode chunk c covers spill-list entries [64c, 64c+64).
 struct Chunk {
     uint32_t base, count;
     const uint32_t* list;      // non-null: record index = list[base + i]
@@ -474,7 +507,7 @@ struct Chunk {
 __device__ __forceinline__ uint32_t num_chunks(const DrawParams& p, int spill) {
     if (spill) return (min(p.scratch->num_spill, p.clip_record_base + p.clip_record_capacity) + 63) / 64;
     uint32_t nclip = min(p.scratch->num_records, p.clip_record_capacity);
-    return 2 * p.total_meshlets + (nclip + 63) / 64;
+    return 2 * p.scratch->num_visible + (nclip + 63) / 64;
 }
 
 __device__ __forceinline__ Chunk get_chunk(const DrawParams& p, int spill, uint32_t c) {
@@ -487,15 +520,16 @@ __device__ __forceinline__ Chunk get_chunk(const DrawParams& p, int spill, uint3
         k.list = p.spill;
         return k;
     }
-    if (c < 2 * p.total_meshlets) {
-        uint32_t g = c >> 1, half = c & 1;
-        uint32_t cnt = p.meshlet_counts[g];
-        k.base = g * 128 + half * 64;
+    const uint32_t nvis = p.scratch->num_visible;
+    if (c < 2 * nvis) {
+        uint32_t i = c >> 1, half = c & 1;
+        uint32_t cnt = p.meshlet_counts[i];
+        k.base = i * 128 + half * 64;
         k.count = cnt > half * 64 ? min(64u, cnt - half * 64) : 0;
         return k;
     }
     uint32_t nclip = min(p.scratch->num_records, p.clip_record_capacity);
-    uint32_t off = (c - 2 * p.total_meshlets) * 64;
+    uint32_t off = (c - 2 * nvis) * 64;
     k.base = p.clip_record_base + off;
     k.count = min(64u, nclip - off);
     return k;
@@ -710,14 +744,16 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
 // written to HBM once, one contiguous 4-pixel-high strip of tiles at a
 // time. After a clear the bin starts from the clear key, otherwise from the
 // vis-buffer's current contents.
+//
+// One block per bin. Workgroups go to the 8 XCDs round-robin (block b runs
+// on XCD b % 8), so block b takes the (b/8)-th bin of the (b%8)-th eighth of
+// the bin range: neighbouring bins, which share records, are rasterized
+// under the same L2.
 template <int BS, int kThreads>
-__global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
+__device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long (&keys)[BS * BS],
+                           TriLds (&tl)[kThreads], uint32_t (&s_wave_total)[kThreads / kWave]) {
     constexpr int kWaves = kThreads / kWave;
     constexpr int kGroups = kThreads / 16;
-    __shared__ unsigned long long keys[BS * BS];
-    __shared__ TriLds tl[kThreads];
-    __shared__ uint32_t s_wave_total[kWaves];
-    const uint32_t bin = blockIdx.x;
     const ViewParams& V = view_of(p, view_of_bin(p, bin));
     const uint32_t lbin = bin - V.bin_base;
     const uint32_t bx = lbin % V.bins_x, by = lbin / V.bins_x;
@@ -820,6 +856,8 @@ __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
         __syncthreads();
     }
 
+    // each thread writes back the keys it initialised, so the next bin's
+    // initialisation needs no extra barrier
     for (uint32_t i = tid; i < BS * BS; i += kThreads) {
         uint32_t strip = i / (BS * 4), rem = i % (BS * 4);
         if (rem < rowElems && y0 + strip * 4 < y1) gkeys[(size_t)(y0 + strip * 4) * W + x0 * 4 + rem] = keys[i];
@@ -835,6 +873,17 @@ __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
         }
         if (tid == 0 && n) atomicAdd(&p.counters->bins_rasterized, 1ull);
     }
+}
+
+template <int BS, int kThreads>
+__global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
+    __shared__ unsigned long long keys[BS * BS];
+    __shared__ TriLds tl[kThreads];
+    __shared__ uint32_t s_wave_total[kThreads / kWave];
+    const uint32_t q = p.total_bins / kXcds, r = p.total_bins % kXcds;
+    const uint32_t xcd = blockIdx.x % kXcds;
+    const uint32_t bin = xcd * q + min(xcd, r) + blockIdx.x / kXcds;
+    raster_bin<BS, kThreads>(p, bin, keys, tl, s_wave_total);
 }
 
 __global__ void k_begin_draw(DrawScratch* s, uint32_t* bin_counts, uint32_t num_bins) {
@@ -896,11 +945,12 @@ void launch_front(const DrawParams& p, hipStream_t s, Profiler* prof) {
     hipLaunchKernelGGL(k_begin_draw, dim3(max(zb, 1u)), dim3(256), 0, s, p.scratch, p.bin_counts, p.total_bins);
     if (p.total_meshlets) {     // a batch of empty views still clears through the bin raster
         Profiler::Scope sc(prof, "cull", s);
-        hipLaunchKernelGGL(k_cull, dim3((p.total_meshlets + 255) / 256), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(k_cull, dim3((p.total_meshlets + kCullPerBlock - 1) / kCullPerBlock), dim3(256), 0, s, p);
     }
     if (p.total_meshlets) {
         Profiler::Scope sc(prof, "mesh_setup", s);
-        hipLaunchKernelGGL(k_mesh_setup, dim3((p.total_meshlets + 3) / 4), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(k_mesh_setup, dim3(min((p.total_meshlets + 3) / 4, kPersistentBlocks)), dim3(256), 0, s,
+                           p);
     }
     {
         Profiler::Scope sc(prof, "clip", s);
@@ -937,7 +987,7 @@ hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s, Profiler* prof
     launch_front(p, s, prof);
     {
         Profiler::Scope sc(prof, "raster_direct", s);
-        uint32_t blocks = max((2 * p.total_meshlets + 3) / 4 + 16, 64u);
+        uint32_t blocks = min(max((2 * p.total_meshlets + 3) / 4 + 16, 64u), kPersistentBlocks);
         hipLaunchKernelGGL(k_raster_direct, dim3(blocks), dim3(256), 0, s, p, 0);
     }
     return hipGetLastError();
@@ -948,7 +998,7 @@ hipError_t launch_draw_binned(const DrawParams& p, hipStream_t s, Profiler* prof
     {
         Profiler::Scope sc(prof, "bin_scatter", s);
         // about one chunk per wave: two per meshlet plus room for clipped records
-        uint32_t blocks = max((2 * p.total_meshlets + 3) / 4 + 16, 64u);
+        uint32_t blocks = min(max((2 * p.total_meshlets + 3) / 4 + 16, 64u), kPersistentBlocks);
         hipLaunchKernelGGL(k_bin_scatter, dim3(blocks), dim3(256), 0, s, p);
     }
     {

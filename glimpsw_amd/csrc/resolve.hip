@@ -17,6 +17,7 @@
 
 #include "internal.h"
 #include "resolve.h"
+#include "surface.h"
 
 namespace glimp {
 namespace {
@@ -40,39 +41,6 @@ __device__ __forceinline__ float mulsign(float x, float y) {
 
 __device__ __forceinline__ float lerpf(float a, float b, float t) {
     return __builtin_fmaf(t, b, __builtin_fmaf(-t, a, a));
-}
-
-// IntersectTriangle in the operation order of the reference's fast-math
-// build (see the oracle for the derivation from the compiled shape).
-__device__ void intersect_triangle(const ApproxTables& T, const float c[3][4], float su, float sv, float kx,
-                                   float ky, float bary[3], float ddx[3], float ddy[3]) {
-    float iw0 = frcp(T, c[0][3]), iw1 = frcp(T, c[1][3]), iw2 = frcp(T, c[2][3]);
-    float p0x = c[0][0] * iw0, p0y = c[0][1] * iw0;
-    float p1x = c[1][0] * iw1, p1y = c[1][1] * iw1;
-    float p2x = c[2][0] * iw2, p2y = c[2][1] * iw2;
-    float m0x = p2x - p1x, m1x = p0x - p1x, m1y = p0y - p1y;
-    float d12y = p1y - p2y;
-    float det = __builtin_fmaf(m0x, m1y, d12y * m1x);
-    float invDet = frcp(T, det);
-    float a0 = invDet * iw0, a1 = invDet * iw1, a2 = invDet * iw2;
-    float dx0 = a0 * d12y, dx1 = a1 * (p2y - p0y), dx2 = a2 * m1y;
-    float dy0 = a0 * m0x, dy1 = a1 * (p0x - p2x), dy2 = a2 * (p1x - p0x);
-    float dxs = (dx0 + dx1) + dx2, dys = (dy0 + dy1) + dy2;
-    float r0x = su - p0x, r0y = sv - p0y;
-    float iiw = __builtin_fmaf(dys, r0y, __builtin_fmaf(dxs, r0x, iw0));
-    float iW = frcp(T, iiw);
-    float by = iW * __builtin_fmaf(dx1, r0x, dy1 * r0y);
-    float bz = iW * __builtin_fmaf(dx2, r0x, dy2 * r0y);
-    float bx = 1.0f - (by + bz);
-    float wdx = frcp(T, __builtin_fmaf(kx, dxs, iiw)), wdy = frcp(T, __builtin_fmaf(ky, dys, iiw));
-    float b[3] = { bx, by, bz }, dxk[3] = { dx0, dx1, dx2 }, dyk[3] = { dy0, dy1, dy2 };
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        float bi = b[k] * iiw;
-        ddx[k] = __builtin_fmaf(wdx, __builtin_fmaf(kx, dxk[k], bi), -b[k]);
-        ddy[k] = __builtin_fmaf(wdy, __builtin_fmaf(ky, dyk[k], bi), -b[k]);
-        bary[k] = b[k];
-    }
 }
 
 __device__ __forceinline__ uint32_t texel_offset(int32_t x, int32_t y, int32_t stride) {  // TiledY8
@@ -198,15 +166,7 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
     float uv0[2] = { 0, 0 }, gradU[2] = { 0, 0 }, gradV[2] = { 0, 0 }, U = 0, V = 0, bary[3] = { 0, 0, 0 };
     uint32_t material = 0xFFFFFFFFu, mi = 0, tri = 0;
     if (active) {
-        uint32_t rank = ~(uint32_t)key;
-        uint32_t seq = rank >> 11;
-        tri = ((rank >> 8) & 7u) * 16u + ((rank >> 3) & 15u);
-        uint32_t lo = 0, hi = p.num_draws;
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (p.draws[mid].seq_base <= seq) lo = mid; else hi = mid;
-        }
-        mi = p.draws[lo].meshlet_offset + (seq - p.draws[lo].seq_base);
+        decode_surface(key, p.draws, p.num_draws, mi, tri);
         const uint8_t* I = p.m.indices + (size_t)mi * 384;
         const float* P = p.m.positions + (size_t)mi * 192;
         float c[3][4], uv[3][2];

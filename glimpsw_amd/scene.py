@@ -241,7 +241,7 @@ def make_view(width: int, height: int, fov_deg: float = 60.0, view: np.ndarray |
     return View(width, height, proj, v, mat_mul_cm(proj, v), offset, count)
 
 
-def config0(num_meshlets: int = 512, width: int = 640, height: int = 480) -> tuple[Scene, View]:
+def checker_scene(num_meshlets: int = 512, width: int = 640, height: int = 480) -> tuple[Scene, View]:
     """Plain pipeline, CPU-runnable: patches uniform in [-10,10]^3, random
     rotation, scale 0.5-2; camera at the origin, 60 deg FOV; one 256x256
     checker texture with its full mip chain."""
@@ -254,7 +254,7 @@ def config0(num_meshlets: int = 512, width: int = 640, height: int = 480) -> tup
     return scene, make_view(width, height, count=num_meshlets)
 
 
-def config1(num_meshlets: int = 10240, width: int = 1920, height: int = 1080, tex_size: int = 2048,
+def frame_1080p(num_meshlets: int = 10240, width: int = 1920, height: int = 1080, tex_size: int = 2048,
             num_materials: int = 16) -> tuple[Scene, View]:
     """Single 1080p view of ~1M triangles: per-patch triangle area
     log-uniform in [0.5, 500] px, patches spread over a widened frustum (so
@@ -289,14 +289,14 @@ def config1(num_meshlets: int = 10240, width: int = 1920, height: int = 1080, te
     return scene, make_view(width, height, count=n)
 
 
-def config2(num_scenes: int = 8, num_views: int = 64, min_meshlets: int = 1000, max_meshlets: int = 50000,
+def batch_views(num_scenes: int = 8, num_views: int = 64, min_meshlets: int = 1000, max_meshlets: int = 50000,
             min_res: tuple = (256, 256), max_res: tuple = (1920, 1080)) -> tuple[list, list]:
     """Batching workload: ``num_scenes`` seeded scenes with log-uniform
-    meshlet counts in [min_meshlets, max_meshlets], ``num_views`` views with
-    log-uniform resolution between ``min_res`` and ``max_res`` (multiples of
-    4; one draw per
-    view of its whole scene) and random cameras inside the scene. Patches
-    are as in config0 (scale 0.5-2, random rotation) at config0's density
+    meshlet counts in [min_meshlets, max_meshlets], ``num_views`` views (view
+    v shows scene v % num_scenes) with per-axis log-uniform resolution
+    between ``min_res`` and ``max_res`` (rounded down to multiples of 4; one
+    draw per view of its whole scene) and random cameras inside the scene. Patches
+    are as in checker_scene (scale 0.5-2, random rotation) at its density
     (512 per 20^3), in a cube sized for the count; every scene has one
     256x256 checker texture. Returns (scenes, [(scene index, View)])."""
     rng = SplitMix64(SEED_BASE + 2)
@@ -313,12 +313,12 @@ def config2(num_scenes: int = 8, num_views: int = 64, min_meshlets: int = 1000, 
         scenes.append(build_patches(centers, R, scales, np.zeros(n, np.uint32), [(0, False, 255)], [tex]))
         sides.append(side)
     views = []
-    for _ in range(num_views):
-        si = int(rng.uniform(1)[0] * num_scenes) % num_scenes
-        t = float(rng.uniform(1)[0])
-        # framebuffers are 4x4-tiled: sizes are rounded to multiples of 4
-        w = 4 * int(round(min_res[0] * (max_res[0] / min_res[0]) ** t / 4.0))
-        h = 4 * int(round(min_res[1] * (max_res[1] / min_res[1]) ** t / 4.0))
+    for v in range(num_views):
+        si = v % num_scenes
+        tw, th = rng.uniform(2)
+        # per-axis log-uniform size, rounded down to the 4x4 tile
+        w = max(4, 4 * int(min_res[0] * (max_res[0] / min_res[0]) ** float(tw) / 4.0))
+        h = max(4, 4 * int(min_res[1] * (max_res[1] / min_res[1]) ** float(th) / 4.0))
         eye = rng.uniform(3, -0.4 * sides[si], 0.4 * sides[si])
         yaw = float(rng.uniform(1, 0.0, 2.0 * math.pi)[0])
         pitch = float(rng.uniform(1, -0.6, 0.6)[0])

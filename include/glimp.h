@@ -202,6 +202,48 @@ int glimp_batch_launch_per_view(glimp_batch* batch, void* stream);
  * it with its time per launch. Synchronous. */
 int glimp_batch_autotune(glimp_batch* batch, void* stream, glimp_batch_shape* chosen, float* best_ms);
 
+/* ------------------------------------------------------------ differentiable resolve */
+
+/* The texture-sampling resolve as a differentiable function of fp32 texel
+ * mip chains and per-vertex UVs and colours, with coverage and barycentrics
+ * held fixed by the vis-buffer. No reference counterpart: it extends
+ * ResolveDebug(BaseColor) (Shading.cpp:734-773) and SampleLevel
+ * (Texture.h:412-459). For a covered pixel:
+ *   U,V = uv0 + b1 (uv1 - uv0) + b2 (uv2 - uv0)
+ *   col = color0 + b1 (color1 - color0) + b2 (color2 - color0)
+ *   out = Sample(texture of the material, U, V) * col   (RGBA, fp32)
+ * Sample uses the resolve's level and per-4x4-tile filter choice (nearest
+ * texel of the chosen level when minifying, else bilinear at level 0 with
+ * texel centres at +0.5 and clamped edges) and Repeat wrap. Uncovered pixels
+ * give 0, untextured materials sample 1. Gradients are reduced
+ * deterministically (bitwise identical run to run). */
+#define GLIMP_MAX_DIFF_TEXTURES 256
+
+typedef struct glimp_diff_texture {
+    uint32_t width, height, mip_levels, pad;
+    uint64_t level_offset[16];         /* in texels; level m is (width>>m) x (height>>m), row-major */
+} glimp_diff_texture;
+
+typedef struct glimp_diff_desc {
+    float object_to_clip[16];
+    const glimp_diff_texture* textures; /* host; material texture i -> textures[i] */
+    uint32_t num_textures;
+    uint64_t num_texels;               /* texels holds num_texels RGBA fp32 texels */
+    const float* texels;               /* device [num_texels][4] */
+    const float* uv;                   /* device [scene meshlets * 64][2] */
+    const float* color;                /* device [scene meshlets * 64][4] */
+} glimp_diff_desc;
+
+/* out: device [height][width][4] fp32. Asynchronous on `stream`. */
+int glimp_diff_forward(glimp_context* ctx, glimp_framebuffer* fb, glimp_scene* scene, const glimp_diff_desc* desc,
+                       float* out, void* stream);
+/* Gradients of sum(grad_out * out): grad_texels [num_texels][4], grad_uv
+ * [meshlets * 64][2], grad_color [meshlets * 64][4], all device, fully
+ * overwritten. Asynchronous on `stream`; calls on one context share its
+ * workspace and must not overlap. */
+int glimp_diff_backward(glimp_context* ctx, glimp_framebuffer* fb, glimp_scene* scene, const glimp_diff_desc* desc,
+                        const float* grad_out, float* grad_texels, float* grad_uv, float* grad_color, void* stream);
+
 /* ------------------------------------------------------------ resolve */
 
 #define GLIMP_RESOLVE_ALBEDO 0u   /* ResolveDebug(DebugLayer::BaseColor) (Shading.cpp:734-773) */

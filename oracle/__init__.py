@@ -251,3 +251,82 @@ def resolve_lit(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip, li
     rp.num_lights = len(lighting.lights)
     lib().orc_resolve_lit(C.byref(fb.fb), C.byref(ctx), C.byref(rp))
     return fb.pixels(0)
+
+
+# ---------------------------------------------------------------- differentiable resolve
+
+f64p = C.POINTER(C.c_double)
+
+
+class DiffTexture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("mip_levels", C.c_uint32), ("pad", C.c_uint32),
+                ("level_offset", C.c_uint64 * 16)]
+
+
+class DiffParams(C.Structure):
+    _fields_ = [("object_to_clip", C.c_float * 16), ("textures", C.POINTER(DiffTexture)),
+                ("num_textures", C.c_uint32), ("texels", f32p), ("uv", f32p), ("color", f32p),
+                ("texels_d", f64p), ("uv_d", f64p), ("color_d", f64p)]
+
+
+def _diff_params(chains, object_to_clip, texels, uv, color, texels_d=None, uv_d=None, color_d=None):
+    keep = []
+    tex = (DiffTexture * max(1, len(chains.sizes)))()
+    for i, ((w, h, L), offs) in enumerate(zip(chains.sizes, chains.offsets)):
+        tex[i].width, tex[i].height, tex[i].mip_levels = w, h, L
+        for m, o in enumerate(offs):
+            tex[i].level_offset[m] = o
+    p = DiffParams()
+    for i in range(16):
+        p.object_to_clip[i] = float(object_to_clip[i])
+    p.textures = C.cast(tex, C.POINTER(DiffTexture))
+    p.num_textures = len(chains.sizes)
+
+    def f32(a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        keep.append(a)
+        return _ptr(a, f32p)
+
+    def f64(a):
+        if a is None:
+            return f64p()
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return _ptr(a, f64p)
+
+    p.texels, p.uv, p.color = f32(texels), f32(uv), f32(color)
+    p.texels_d, p.uv_d, p.color_d = f64(texels_d), f64(uv_d), f64(color_d)
+    return p, (tex, keep)
+
+
+def diff_forward(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip, chains, texels, uv, color,
+                 texels_d=None, uv_d=None, color_d=None) -> np.ndarray:
+    """FP64 forward of the differentiable resolve: (H, W, 4) float64.
+    Discrete choices come from the fp32 ``texels/uv/color``; the optional
+    ``*_d`` float64 arrays replace their values in the continuous part."""
+    ctx = binding.ctx(object_to_clip)
+    p, keep = _diff_params(chains, object_to_clip, texels, uv, color, texels_d, uv_d, color_d)
+    out = np.zeros((fb.height, fb.width, 4), dtype=np.float64)
+    lib().orc_diff_forward(C.byref(fb.fb), C.byref(ctx), C.byref(p), _ptr(out, f64p))
+    return out
+
+
+def diff_backward(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip, chains, texels, uv, color,
+                  grad_out) -> tuple:
+    """FP64 gradients of sum(grad_out * out): (texels, uv, color)."""
+    ctx = binding.ctx(object_to_clip)
+    p, keep = _diff_params(chains, object_to_clip, texels, uv, color)
+    g = np.ascontiguousarray(grad_out, dtype=np.float64)
+    gt = np.zeros((np.asarray(texels).shape[0], 4), dtype=np.float64)
+    gu = np.zeros((np.asarray(uv).shape[0], 2), dtype=np.float64)
+    gc = np.zeros((np.asarray(color).shape[0], 4), dtype=np.float64)
+    lib().orc_diff_backward(C.byref(fb.fb), C.byref(ctx), C.byref(p), _ptr(g, f64p), _ptr(gt, f64p),
+                            _ptr(gu, f64p), _ptr(gc, f64p))
+    return gt, gu, gc
+
+
+def bary_f64(clip, su: float, sv: float) -> np.ndarray:
+    c = np.ascontiguousarray(clip, dtype=np.float32).reshape(12)
+    out = np.zeros(3, dtype=np.float64)
+    lib().orc_bary_f64(_ptr(c, f32p), C.c_double(su), C.c_double(sv), _ptr(out, f64p))
+    return out

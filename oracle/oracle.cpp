@@ -1152,4 +1152,277 @@ void orc_resolve_lit(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_
     memcpy(fb->data, out.data(), (size_t)W * H * 4);
 }
 
+// ---------------------------------------------------------------- differentiable resolve
+
+void orc_bary_f64(const float* clip, double su, double sv, double* bary) {
+    double px[3], py[3], iw[3];
+    for (int k = 0; k < 3; k++) {
+        iw[k] = 1.0 / (double)clip[k * 4 + 3];
+        px[k] = (double)clip[k * 4 + 0] * iw[k];
+        py[k] = (double)clip[k * 4 + 1] * iw[k];
+    }
+    // screen-space barycentrics, then weight by 1/w and renormalise
+    double area = (px[1] - px[0]) * (py[2] - py[0]) - (py[1] - py[0]) * (px[2] - px[0]);
+    double l1 = ((su - px[0]) * (py[2] - py[0]) - (sv - py[0]) * (px[2] - px[0])) / area;
+    double l2 = ((px[1] - px[0]) * (sv - py[0]) - (py[1] - py[0]) * (su - px[0])) / area;
+    double l0 = 1.0 - l1 - l2;
+    double q0 = l0 * iw[0], q1 = l1 * iw[1], q2 = l2 * iw[2];
+    double s = q0 + q1 + q2;
+    bary[0] = q0 / s;
+    bary[1] = q1 / s;
+    bary[2] = q2 / s;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Discrete choices and fp32 values of one pixel of the differentiable
+// resolve, in the GPU's operation order.
+struct DiffLane {
+    bool active;
+    uint32_t mi, vi[3];
+    int32_t tex;             // -1: untextured
+    int32_t level;
+    bool linear;
+    float floorU, floorV;    // wrap period
+    int32_t tx, ty;          // nearest texel, or the bilinear cell's corner
+    bool cx, cy;             // bilinear: inside the left/top clamp
+    bool ex, ey;             // bilinear: right/bottom neighbour exists
+};
+
+void diff_tile(const orc_shading_ctx& ctx, const orc_diff_params& p, const uint32_t* ids, const bool* active,
+               const float* su, const float* sv, float sw, float sh, DiffLane* L) {
+    const orc_meshlets& ms = *ctx.meshlets;
+    float gradU[16][2], gradV[16][2], U[16], V[16];
+    uint32_t material[16];
+    for (int l = 0; l < 16; l++) {
+        L[l] = DiffLane{};
+        L[l].active = active[l];
+        L[l].tex = -1;
+        material[l] = UINT32_MAX;
+        U[l] = V[l] = 0.0f;
+        gradU[l][0] = gradU[l][1] = gradV[l][0] = gradV[l][1] = 0.0f;
+        if (!active[l]) continue;
+        uint32_t mi = ids[l] / 128, tri = ids[l] % 128;
+        L[l].mi = mi;
+        float c[3][4], uv[3][2];
+        for (int k = 0; k < 3; k++) {
+            uint32_t vi = ms.indices[(size_t)mi * 384 + k * 128 + tri];
+            L[l].vi[k] = vi;
+            const float* P = ms.positions + (size_t)mi * 192;
+            transform(p.object_to_clip, P[vi], P[64 + vi], P[128 + vi], c[k]);
+            uv[k][0] = p.uv[((size_t)mi * 64 + vi) * 2 + 0];
+            uv[k][1] = p.uv[((size_t)mi * 64 + vi) * 2 + 1];
+        }
+        material[l] = ms.material_id[mi];
+        float bary[3], ddx[3], ddy[3];
+        intersect_triangle(c, su[l], sv[l], sw, sh, bary, ddx, ddy);
+        float t10x = uv[1][0] - uv[0][0], t10y = uv[1][1] - uv[0][1];
+        float t20x = uv[2][0] - uv[0][0], t20y = uv[2][1] - uv[0][1];
+        U[l] = fmaf(bary[2], t20x, fmaf(bary[1], t10x, uv[0][0]));
+        V[l] = fmaf(bary[2], t20y, fmaf(bary[1], t10y, uv[0][1]));
+        gradU[l][0] = fmaf(t20x, ddx[2], t10x * ddx[1]);
+        gradV[l][0] = fmaf(t20y, ddx[2], t10y * ddx[1]);
+        gradU[l][1] = fmaf(t20x, ddy[2], t10x * ddy[1]);
+        gradV[l][1] = fmaf(t20y, ddy[2], t10y * ddy[1]);
+    }
+    // material waterfall with the tile-wide filter vote, as in the resolve
+    bool done[16] = {};
+    for (int l0 = 0; l0 < 16; l0++) {
+        if (!active[l0] || done[l0] || material[l0] == UINT32_MAX) continue;
+        const uint32_t mat = material[l0];
+        const int32_t ti = ctx.materials[mat].texture;
+        if (ti < 0 || (uint32_t)ti >= p.num_textures) {
+            for (int l = 0; l < 16; l++) if (active[l] && material[l] == mat) done[l] = true;
+            continue;
+        }
+        const orc_diff_texture& T = p.textures[ti];
+        const float su2 = (float)T.width * (float)T.width, sv2 = (float)T.height * (float)T.height;
+        int32_t mip[16];
+        bool minify = false;
+        for (int l = 0; l < 16; l++) {
+            float dx = fmaf(gradU[l][0], gradU[l][0], gradV[l][0] * gradV[l][0]) * su2;
+            float dy = fmaf(gradU[l][1], gradU[l][1], gradV[l][1] * gradV[l][1]) * sv2;
+            float mx = (dx > dy) ? dx : dy;
+            mip[l] = wrap_sub((int32_t)fbits(mx), 127 << 23) >> 24;
+            if (active[l] && mip[l] > 0) minify = true;
+        }
+        for (int l = 0; l < 16; l++) {
+            if (!active[l] || material[l] != mat) continue;
+            done[l] = true;
+            DiffLane& D = L[l];
+            D.tex = ti;
+            D.linear = !minify;
+            D.floorU = floorf(U[l]);
+            D.floorV = floorf(V[l]);
+            const float fu = U[l] - D.floorU, fv = V[l] - D.floorV;
+            if (D.linear) {
+                D.level = 0;
+                const int32_t W = (int32_t)T.width, H = (int32_t)T.height;
+                const float xs = fu * (float)W - 0.5f, ys = fv * (float)H - 0.5f;
+                D.cx = xs > 0.0f;
+                D.cy = ys > 0.0f;
+                const float xf = D.cx ? xs : 0.0f, yf = D.cy ? ys : 0.0f;
+                D.tx = (int32_t)xf;
+                D.ty = (int32_t)yf;
+                D.ex = D.tx < W - 1;
+                D.ey = D.ty < H - 1;
+                if (!D.ex) D.tx = W - 1;
+                if (!D.ey) D.ty = H - 1;
+            } else {
+                int32_t m = mip[l] < 0 ? 0 : mip[l];
+                if (m > (int32_t)T.mip_levels - 1) m = (int32_t)T.mip_levels - 1;
+                D.level = m;
+                const int32_t Wm = (int32_t)(T.width >> m), Hm = (int32_t)(T.height >> m);
+                int32_t x = (int32_t)(fu * (float)Wm), y = (int32_t)(fv * (float)Hm);
+                D.tx = x < Wm - 1 ? x : Wm - 1;
+                D.ty = y < Hm - 1 ? y : Hm - 1;
+            }
+        }
+    }
+}
+
+struct DiffEval {
+    double w[3];             // d(U,V,col)/d(vertex k)
+    double U, V, col[4], val[4];
+    uint64_t texel[4];       // bilinear corners 00, 10, 01, 11 (or texel[0] for nearest)
+    double tw[4];            // their weights
+    int ntexels;
+    double dval_du[4], dval_dv[4];
+};
+
+void diff_eval(const orc_shading_ctx& ctx, const orc_diff_params& p, const DiffLane& D, uint32_t x, uint32_t y,
+               uint32_t W, uint32_t H, DiffEval& E) {
+    const orc_meshlets& ms = *ctx.meshlets;
+    (void)ms;
+    float c[3][4];
+    const float* P = ctx.meshlets->positions + (size_t)D.mi * 192;
+    for (int k = 0; k < 3; k++) {
+        uint32_t vi = D.vi[k];
+        transform(p.object_to_clip, P[vi], P[64 + vi], P[128 + vi], c[k]);
+    }
+    const double su = (2.0 * x + 1.0) / W - 1.0, sv = (2.0 * y + 1.0) / H - 1.0;
+    double b[3];
+    orc_bary_f64(&c[0][0], su, sv, b);
+    E.w[0] = 1.0 - b[1] - b[2];
+    E.w[1] = b[1];
+    E.w[2] = b[2];
+    auto uvv = [&](int k, int j) {
+        size_t i = ((size_t)D.mi * 64 + D.vi[k]) * 2 + j;
+        return p.uv_d ? p.uv_d[i] : (double)p.uv[i];
+    };
+    auto colv = [&](int k, int j) {
+        size_t i = ((size_t)D.mi * 64 + D.vi[k]) * 4 + j;
+        return p.color_d ? p.color_d[i] : (double)p.color[i];
+    };
+    auto tex = [&](uint64_t t, int j) { return p.texels_d ? p.texels_d[t * 4 + j] : (double)p.texels[t * 4 + j]; };
+    E.U = uvv(0, 0) + b[1] * (uvv(1, 0) - uvv(0, 0)) + b[2] * (uvv(2, 0) - uvv(0, 0));
+    E.V = uvv(0, 1) + b[1] * (uvv(1, 1) - uvv(0, 1)) + b[2] * (uvv(2, 1) - uvv(0, 1));
+    for (int j = 0; j < 4; j++) E.col[j] = colv(0, j) + b[1] * (colv(1, j) - colv(0, j)) + b[2] * (colv(2, j) - colv(0, j));
+    E.ntexels = 0;
+    for (int j = 0; j < 4; j++) E.dval_du[j] = E.dval_dv[j] = 0.0;
+    if (D.tex < 0) {
+        for (int j = 0; j < 4; j++) E.val[j] = 1.0;
+        return;
+    }
+    const orc_diff_texture& T = p.textures[D.tex];
+    const uint64_t base = T.level_offset[D.level];
+    if (!D.linear) {
+        const uint64_t Wm = T.width >> D.level;
+        E.ntexels = 1;
+        E.texel[0] = base + (uint64_t)D.ty * Wm + (uint64_t)D.tx;
+        E.tw[0] = 1.0;
+        for (int j = 0; j < 4; j++) E.val[j] = tex(E.texel[0], j);
+        return;
+    }
+    const double fu = E.U - (double)D.floorU, fv = E.V - (double)D.floorV;
+    const double xf = D.cx ? fu * T.width - 0.5 : 0.0, yf = D.cy ? fv * T.height - 0.5 : 0.0;
+    const double fx = D.ex ? xf - D.tx : 0.0, fy = D.ey ? yf - D.ty : 0.0;
+    const uint64_t x0 = (uint64_t)D.tx, y0 = (uint64_t)D.ty;
+    const uint64_t x1 = D.ex ? x0 + 1 : x0, y1 = D.ey ? y0 + 1 : y0;
+    E.ntexels = 4;
+    E.texel[0] = base + y0 * T.width + x0;
+    E.texel[1] = base + y0 * T.width + x1;
+    E.texel[2] = base + y1 * T.width + x0;
+    E.texel[3] = base + y1 * T.width + x1;
+    E.tw[0] = (1.0 - fx) * (1.0 - fy);
+    E.tw[1] = fx * (1.0 - fy);
+    E.tw[2] = (1.0 - fx) * fy;
+    E.tw[3] = fx * fy;
+    const double sx = (D.cx && D.ex) ? (double)T.width : 0.0, sy = (D.cy && D.ey) ? (double)T.height : 0.0;
+    for (int j = 0; j < 4; j++) {
+        double t00 = tex(E.texel[0], j), t10 = tex(E.texel[1], j), t01 = tex(E.texel[2], j), t11 = tex(E.texel[3], j);
+        double l0 = t00 + fx * (t10 - t00), l1 = t01 + fx * (t11 - t01);
+        E.val[j] = l0 + fy * (l1 - l0);
+        E.dval_du[j] = sx * ((1.0 - fy) * (t10 - t00) + fy * (t11 - t01));
+        E.dval_dv[j] = sy * (l1 - l0);
+    }
+}
+
+template <typename F>
+void diff_for_each(const orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_diff_params* p, F&& fn) {
+    const uint32_t W = fb->width, H = fb->height;
+    const float sw = (float)W, sh = (float)H;
+    const float scaleX = 2.0f / sw, scaleY = 2.0f / sh;
+    const float centerX = fmaf(0.5f, scaleX, -1.0f), centerY = fmaf(0.5f, scaleY, -1.0f);
+    const float* depth = (const float*)(fb->data + fb->layer_stride);
+    for (uint32_t y0 = 0; y0 < H; y0 += 4) {
+        for (uint32_t x0 = 0; x0 < W; x0 += 4) {
+            const uint32_t off = pixel_offset(W, x0, y0);
+            bool active[16], any = false;
+            float sx[16], sy[16];
+            uint32_t ids[16];
+            for (int l = 0; l < 16; l++) {
+                active[l] = !(depth[off + l] <= 0.0f);
+                any |= active[l];
+                ids[l] = fb->data[off + l];
+                sx[l] = fmaf((float)(int32_t)(x0 + (l & 3)), scaleX, centerX);
+                sy[l] = fmaf((float)(int32_t)(y0 + (l >> 2)), scaleY, centerY);
+            }
+            if (!any) continue;
+            DiffLane L[16];
+            diff_tile(*ctx, *p, ids, active, sx, sy, sw, sh, L);
+            for (int l = 0; l < 16; l++) {
+                if (!L[l].active) continue;
+                const uint32_t x = x0 + (l & 3), y = y0 + (l >> 2);
+                DiffEval E;
+                diff_eval(*ctx, *p, L[l], x, y, W, H, E);
+                fn(L[l], E, (size_t)y * W + x);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_diff_forward(const orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_diff_params* p, double* out) {
+    memset(out, 0, sizeof(double) * 4 * fb->width * fb->height);
+    diff_for_each(fb, ctx, p, [&](const DiffLane&, const DiffEval& E, size_t pix) {
+        for (int j = 0; j < 4; j++) out[pix * 4 + j] = E.val[j] * E.col[j];
+    });
+}
+
+void orc_diff_backward(const orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_diff_params* p,
+                       const double* grad_out, double* grad_texels, double* grad_uv, double* grad_color) {
+    diff_for_each(fb, ctx, p, [&](const DiffLane& D, const DiffEval& E, size_t pix) {
+        const double* g = grad_out + pix * 4;
+        double gU = 0.0, gV = 0.0;
+        for (int j = 0; j < 4; j++) {
+            const double gval = g[j] * E.col[j];
+            for (int t = 0; t < E.ntexels; t++) grad_texels[E.texel[t] * 4 + j] += E.tw[t] * gval;
+            gU += gval * E.dval_du[j];
+            gV += gval * E.dval_dv[j];
+        }
+        for (int k = 0; k < 3; k++) {
+            const size_t v = (size_t)D.mi * 64 + D.vi[k];
+            grad_uv[v * 2 + 0] += E.w[k] * gU;
+            grad_uv[v * 2 + 1] += E.w[k] * gV;
+            for (int j = 0; j < 4; j++) grad_color[v * 4 + j] += E.w[k] * g[j] * E.val[j];
+        }
+    });
+}
+
 }  // extern "C"

@@ -133,6 +133,54 @@ int32_t orc_clip_triangle(const float* v /*[3][4]*/, uint32_t outcodes, float gb
 void orc_setup16(const float* clip /*[3][4][16]*/, int32_t w, int32_t h, int32_t cull_mode,
                  int32_t* X, int32_t* Y, float* Z, float* W, float* det, uint16_t* mask);
 
+/* ------------------------------------------------------------ differentiable resolve
+ *
+ * Not a reference operation: the texture-sampling resolve made
+ * differentiable. For a covered pixel of surface (meshlet, triangle), with
+ * barycentrics b and vertex parameters uv_k (fp32 [meshlets][64][2]) and
+ * color_k (fp32 [meshlets][64][4]):
+ *   U,V   = uv_0 + b1 (uv_1 - uv_0) + b2 (uv_2 - uv_0)
+ *   col   = color_0 + b1 (color_1 - color_0) + b2 (color_2 - color_0)
+ *   out   = Sample(texture of the material, U, V) * col      (RGBA)
+ * Sample follows SampleLevel's addressing (Texture.h:412-459) with float
+ * weights: Repeat wrap, level and filter chosen as the resolve does (mip from
+ * the UV gradients, magnification anywhere in the 4x4 tile -> bilinear at
+ * level 0 with texel centres at +0.5 and clamped edges, else nearest texel
+ * of level clamp(mip)). Texels are fp32 RGBA, level m of a texture stored
+ * row-major at level_offset[m]. Uncovered pixels give 0; untextured
+ * materials sample 1.
+ *
+ * The oracle takes every discrete choice (surface, level, filter, wrap
+ * period, texel cell, edge clamps) from the fp32 evaluation the GPU performs
+ * and evaluates everything continuous in double, so its gradient is the FP64
+ * gradient of the same piecewise function. */
+typedef struct orc_diff_texture {
+    uint32_t width, height, mip_levels, pad;
+    uint64_t level_offset[16];        /* in texels (4 floats each) */
+} orc_diff_texture;
+
+typedef struct orc_diff_params {
+    float object_to_clip[16];
+    const orc_diff_texture* textures; /* indexed by material texture */
+    uint32_t num_textures;
+    const float* texels;              /* fp32 params: decisions are taken from these */
+    const float* uv;
+    const float* color;
+    const double* texels_d;           /* optional double values of the same params */
+    const double* uv_d;               /* (finite differences); NULL = use the fp32 ones */
+    const double* color_d;
+} orc_diff_params;
+
+/* out: [height][width][4] double, row-major. */
+void orc_diff_forward(const orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_diff_params* p, double* out);
+/* Gradients of sum(grad_out * out) w.r.t. texels, uv and color (dense,
+ * accumulated into the given zeroed arrays). */
+void orc_diff_backward(const orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_diff_params* p,
+                       const double* grad_out, double* grad_texels, double* grad_uv, double* grad_color);
+/* Perspective-correct barycentrics of one pixel centre in double from
+ * textbook formulas (for checking the fp32 IntersectTriangle). */
+void orc_bary_f64(const float* clip /*[3][4]*/, double su, double sv, double* bary);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,7 +11,7 @@ from glimpsw_amd import textures
 
 def small_scene(n: int = 512, seed_offset: int = 0, spread: float = 10.0, near_frac: float = 0.0,
                 double_sided: bool = False):
-    """Config-0 style patches; `near_frac` of them straddle the camera plane
+    """Checker-scene patches; `near_frac` of them straddle the camera plane
     beside the viewer so the guard-band clipper gets work."""
     rng = S.SplitMix64(S.SEED_BASE + 100 + seed_offset)
     centers = rng.uniform(n * 3, -spread, spread).reshape(n, 3)

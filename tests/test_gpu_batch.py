@@ -11,8 +11,8 @@ from glimpsw_amd import scene as S
 pytestmark = pytest.mark.gpu
 
 
-def _small_config2(num_views=10):
-    return S.config2(num_scenes=4, num_views=num_views, min_meshlets=100, max_meshlets=1500,
+def _small_batch_views(num_views=10):
+    return S.batch_views(num_scenes=4, num_views=num_views, min_meshlets=100, max_meshlets=1500,
                      min_res=(64, 64), max_res=(480, 270))
 
 
@@ -43,7 +43,7 @@ def _same(a, b, label):
 
 @pytest.fixture(scope="module")
 def small(gpu_ctx):
-    scenes, views = _small_config2()
+    scenes, views = _small_batch_views()
     st = _Setup(gpu_ctx, scenes, views)
     yield scenes, views, st
     st.close()
@@ -149,7 +149,7 @@ def test_batch_then_resolve(gpu_ctx, small):
 def test_empty_views_are_cleared(gpu_ctx):
     import glimpsw_amd as g
 
-    scenes, views = S.config2(num_scenes=2, num_views=3, min_meshlets=50, max_meshlets=200,
+    scenes, views = S.batch_views(num_scenes=2, num_views=3, min_meshlets=50, max_meshlets=200,
                               min_res=(40, 24), max_res=(100, 60))
     st = _Setup(gpu_ctx, scenes, views)
     for _, fb, _ in st.items:

@@ -52,19 +52,19 @@ def _assert_same(g_ids, g_depth, o_ids, o_depth, label):
         pytest.fail("\n".join(msg))
 
 
-def test_config0_frame(gpu_ctx):
-    sc, view = S.config0()
+def test_checker_scene_frame(gpu_ctx):
+    sc, view = S.checker_scene()
     g = _gpu_frame(gpu_ctx, sc, view)
     o = _oracle_frame(sc, view)
     assert (o[1] > 0).sum() > 10000
-    _assert_same(*g, o[0], o[1], "config0")
+    _assert_same(*g, o[0], o[1], "checker_scene")
 
 
-def test_config0_no_cull(gpu_ctx):
-    sc, view = S.config0()
+def test_checker_scene_no_cull(gpu_ctx):
+    sc, view = S.checker_scene()
     g = _gpu_frame(gpu_ctx, sc, view, cull=False)
     o = _oracle_frame(sc, view, cull=False)
-    _assert_same(*g, o[0], o[1], "config0 no cull")
+    _assert_same(*g, o[0], o[1], "checker_scene no cull")
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])
@@ -128,7 +128,7 @@ def test_multiple_draws_order(gpu_ctx):
 
 
 def test_tiny_and_max_framebuffer(gpu_ctx):
-    sc, _ = S.config0(256)
+    sc, _ = S.checker_scene(256)
     for w, h in [(4, 4), (8, 12), (2896, 1628)]:
         view = S.make_view(w, h, count=sc.count)
         o = _oracle_frame(sc, view)
@@ -153,7 +153,7 @@ def test_watertight_grid(gpu_ctx):
 def test_empty_draw_and_clear_values(gpu_ctx):
     import glimpsw_amd as g
 
-    sc, view = S.config0(64)
+    sc, view = S.checker_scene(64)
     ds = g.Scene(gpu_ctx, sc)
     fb = g.Framebuffer(gpu_ctx, 64, 48)
     fb.clear(0xFF000000, 0.0)
@@ -184,11 +184,11 @@ def test_counters_match_oracle(gpu_ctx):
 
 
 @pytest.mark.slow
-def test_config1_frame(gpu_ctx):
-    sc, view = S.config1()
+def test_frame_1080p(gpu_ctx):
+    sc, view = S.frame_1080p()
     o = _oracle_frame(sc, view)
     g = _gpu_frame(gpu_ctx, sc, view)
-    _assert_same(*g, o[0], o[1], "config1")
+    _assert_same(*g, o[0], o[1], "frame_1080p")
 
 
 @pytest.mark.parametrize("bin_size", [8, 16, 32, 64])
@@ -204,7 +204,7 @@ def test_bin_sizes(gpu_ctx, bin_size):
 def test_bin_spill_bit_identical(gpu_ctx, cap):
     """Bins far too small: most records spill to the global-atomic pass and
     the vis-buffer must not change."""
-    sc, view = S.config0()
+    sc, view = S.checker_scene()
     o = _oracle_frame(sc, view)
     g = _gpu_frame(gpu_ctx, sc, view, bin_size=16, bin_capacity=cap)
     _assert_same(*g, o[0], o[1], f"capacity {cap}")

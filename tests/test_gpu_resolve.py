@@ -41,9 +41,9 @@ def _check(g_alb, o_alb, g_lit, o_lit, label):
     assert diff <= 1, f"{label}: lit differs by {diff}/255"
 
 
-def test_config0(gpu_ctx):
-    sc, view = S.config0()
-    _check(*_frames(gpu_ctx, sc, view, True), "config0")
+def test_checker_scene(gpu_ctx):
+    sc, view = S.checker_scene()
+    _check(*_frames(gpu_ctx, sc, view, True), "checker_scene")
 
 
 def test_clip_scene(gpu_ctx):
@@ -52,9 +52,9 @@ def test_clip_scene(gpu_ctx):
     _check(*_frames(gpu_ctx, sc, view, True), "clip scene")
 
 
-def test_config1(gpu_ctx):
-    sc, view = S.config1()
-    _check(*_frames(gpu_ctx, sc, view, True), "config1")
+def test_frame_1080p(gpu_ctx):
+    sc, view = S.frame_1080p()
+    _check(*_frames(gpu_ctx, sc, view, True), "frame_1080p")
 
 
 def test_resolve_after_multiple_draws(gpu_ctx):
@@ -84,7 +84,7 @@ def test_resolve_after_multiple_draws(gpu_ctx):
 def test_resolve_without_draw_is_background(gpu_ctx):
     import glimpsw_amd as g
 
-    sc, view = S.config0(16)
+    sc, view = S.checker_scene(16)
     ds = g.Scene(gpu_ctx, sc)
     fb = g.Framebuffer(gpu_ctx, 16, 8)
     fb.clear(0, 0.0)
