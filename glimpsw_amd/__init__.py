@@ -79,6 +79,10 @@ class DiffDesc(C.Structure):
                 ("uv", C.c_void_p), ("color", C.c_void_p)]
 
 
+class DiffViewDesc(C.Structure):
+    _fields_ = [("framebuffer", C.c_void_p), ("object_to_clip", C.c_float * 16)]
+
+
 class CountersStruct(C.Structure):
     _fields_ = [("triangles_processed", C.c_uint64), ("triangles_rasterized", C.c_uint64),
                 ("triangles_clipped", C.c_uint64), ("bins_rasterized", C.c_uint64),
@@ -160,6 +164,11 @@ _SIGNATURES = {
                                      C.c_void_p]),
     "glimp_diff_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(DiffDesc), C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "glimp_diff_forward_views": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DiffDesc), C.POINTER(DiffViewDesc),
+                                           C.c_uint32, C.c_void_p, C.c_void_p]),
+    "glimp_diff_backward_views": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DiffDesc), C.POINTER(DiffViewDesc),
+                                            C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p]),
     "glimp_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "glimp_profile_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_uint32, C.POINTER(C.c_uint32), C.c_int]),
 }
@@ -182,6 +191,15 @@ def load(build_if_missing: bool = False):
             _b.build()
         else:
             raise GlimpError(f"libglimp.so not built ({LIB_PATH}); run python -m glimpsw_amd.build")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7, and a
+    # second copy loaded beside it cannot share the GPU (whichever initialises
+    # second finds no devices). Importing torch first makes libglimp bind to
+    # the copy torch loaded (same soname), so device memory and streams are
+    # shared with torch tensors.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(L, name)

@@ -71,7 +71,7 @@ struct Scratch {
     size_t records_cap = 0;
     ClipEntry* clip = nullptr;
     size_t clip_cap = 0;
-    uint32_t* visible_list = nullptr;
+    uint2* visible_list = nullptr;
     size_t visible_list_cap = 0;
     uint32_t* meshlet_counts = nullptr;
     size_t meshlet_counts_cap = 0;
@@ -106,6 +106,8 @@ struct glimp_context {
     std::vector<DiffTex> diff_tex;     // last texture table uploaded to d_diff_tex
     DiffTex* d_diff_tex = nullptr;
     size_t d_diff_tex_cap = 0;
+    DiffView* d_diff_views = nullptr;
+    size_t d_diff_views_cap = 0;
     Profiler prof;
 
     ApproxTables approx() const { return ApproxTables{ tables, tables + 65536 }; }
@@ -189,7 +191,7 @@ void glimp_destroy(glimp_context* c) {
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
     diff_workspace_release(c->diff_ws);
-    void* ptrs[] = { c->tables, c->counters, c->scratch, c->visible, c->bitmap, c->d_diff_tex };
+    void* ptrs[] = { c->tables, c->counters, c->scratch, c->visible, c->bitmap, c->d_diff_tex, c->d_diff_views };
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c->draw_scratch.release();
@@ -497,6 +499,16 @@ uint32_t layout_bins(std::vector<ViewParams>& views, uint32_t bin_size) {
     return base;
 }
 
+// Bin indices are packed with a 6-bit lane number into 32 bits when binning.
+constexpr uint64_t kMaxBins = 1ull << 26;
+
+uint64_t count_bins(const std::vector<ViewParams>& views, uint32_t bin_size) {
+    uint64_t n = 0;
+    for (const auto& V : views)
+        n += (uint64_t)(((uint32_t)V.width + bin_size - 1) / bin_size) * (((uint32_t)V.height + bin_size - 1) / bin_size);
+    return n;
+}
+
 int prepare_scratch(Scratch& S, size_t total_meshlets, size_t total_bins, const LaunchShape& sh, bool binned) {
     const size_t clipq = clip_capacity_for(total_meshlets);
     const size_t recs = total_meshlets * 128 + clipq * 8;
@@ -611,6 +623,9 @@ void drop_graph(glimp_batch* b) {
 
 // Rebuilds the view table for the batch's shape and uploads it.
 int layout_batch(glimp_batch* b) {
+    std::vector<ViewParams> probe = b->views;
+    if (count_bins(probe, b->shape.bin_size) >= kMaxBins)
+        return fail(GLIMP_E_LIMIT, "glimp_batch: more than 2^26 screen bins; use a larger bin size or fewer views");
     b->total_bins = layout_bins(b->views, b->shape.bin_size);
     int rc = prepare_scratch(b->scratch, b->total_meshlets, b->total_bins, b->shape, true);
     if (rc != GLIMP_OK) return rc;
@@ -808,9 +823,8 @@ int glimp_batch_autotune(glimp_batch* b, void* stream, glimp_batch_shape* chosen
     for (uint32_t bs : sizes) {
         for (uint32_t cap : caps) {
             for (uint32_t th : threads) {
-                std::vector<ViewParams> tmp = b->views;
-                size_t bins = layout_bins(tmp, bs);
-                if (bins * cap * 4 > kListBudget) continue;
+                const uint64_t bins = count_bins(b->views, bs);
+                if (bins >= kMaxBins || bins * cap * 4 > kListBudget) continue;
                 glimp_batch_shape sh = { bs, cap, th };
                 rc = glimp_batch_set_shape(b, &sh);
                 if (rc != GLIMP_OK) break;
@@ -917,27 +931,26 @@ int glimp_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const
 
 namespace {
 
-// Shared set-up of the differentiable resolve calls.
-int diff_params(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const glimp_diff_desc* d, hipStream_t s,
-                const char* who, DiffParams& p, uint64_t& num_vertices) {
-    if (!c || !f || !sc || !d) return fail(GLIMP_E_INVALID, std::string(who) + ": NULL argument");
+// Shared set-up of the differentiable resolve calls: parameter arrays and
+// the texture table.
+int diff_common(glimp_context* c, glimp_scene* sc, const glimp_diff_desc* d, hipStream_t s, const std::string& who,
+                DiffParams& p, uint64_t& num_vertices) {
+    if (!c || !sc || !d) return fail(GLIMP_E_INVALID, who + ": NULL argument");
     if (!d->texels || !d->uv || !d->color || (d->num_textures && !d->textures))
-        return fail(GLIMP_E_INVALID, std::string(who) + ": NULL parameter array");
-    if (d->num_textures > GLIMP_MAX_DIFF_TEXTURES)
-        return fail(GLIMP_E_LIMIT, std::string(who) + ": too many textures");
-    if (d->num_texels >= 0xFFFFFFFFull)
-        return fail(GLIMP_E_LIMIT, std::string(who) + ": texel indices must fit 32 bits");
+        return fail(GLIMP_E_INVALID, who + ": NULL parameter array");
+    if (d->num_textures > GLIMP_MAX_DIFF_TEXTURES) return fail(GLIMP_E_LIMIT, who + ": too many textures");
+    if (d->num_texels >= 0xFFFFFFFFull) return fail(GLIMP_E_LIMIT, who + ": texel indices must fit 32 bits");
     num_vertices = (uint64_t)sc->m.count * 64;
-    if (num_vertices >= 0xFFFFFFFFull) return fail(GLIMP_E_LIMIT, std::string(who) + ": too many vertices");
+    if (num_vertices >= 0xFFFFFFFFull) return fail(GLIMP_E_LIMIT, who + ": too many vertices");
     std::vector<DiffTex> tex(d->num_textures);
     for (uint32_t i = 0; i < d->num_textures; i++) {
         const glimp_diff_texture& t = d->textures[i];
         if (t.width == 0 || t.height == 0 || t.mip_levels == 0 || t.mip_levels > 16)
-            return fail(GLIMP_E_INVALID, std::string(who) + ": bad texture size or level count");
+            return fail(GLIMP_E_INVALID, who + ": bad texture size or level count");
         for (uint32_t m = 0; m < t.mip_levels; m++) {
-            const uint64_t w = std::max<uint64_t>(t.width >> m, 1), h = std::max<uint64_t>(t.height >> m, 1);
-            if ((t.width >> m) == 0 || (t.height >> m) == 0 || t.level_offset[m] + w * h > d->num_texels)
-                return fail(GLIMP_E_INVALID, std::string(who) + ": texture level outside the texel array");
+            const uint64_t w = t.width >> m, h = t.height >> m;
+            if (w == 0 || h == 0 || t.level_offset[m] + w * h > d->num_texels)
+                return fail(GLIMP_E_INVALID, who + ": texture level outside the texel array");
         }
         DiffTex& D = tex[i];
         memset(&D, 0, sizeof(D));
@@ -947,7 +960,6 @@ int diff_params(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const g
         for (uint32_t m = 0; m < 16; m++) D.level_offset[m] = t.level_offset[m];
     }
     HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(flush_clear(f, s));
     const bool same = tex.size() == c->diff_tex.size() &&
                       (tex.empty() || memcmp(tex.data(), c->diff_tex.data(), tex.size() * sizeof(DiffTex)) == 0);
     if (!same) {
@@ -966,26 +978,95 @@ int diff_params(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const g
     p.textures = c->d_diff_tex;
     p.num_textures = d->num_textures;
     p.tables = c->approx();
-    p.keys = f->keys;
-    p.width = (int)f->width;
-    p.height = (int)f->height;
-    uint32_t db;
-    memcpy(&db, &f->clear_depth, 4);
-    p.clear_key = (unsigned long long)db << 32;
-    int rc = upload_draws(f, s, &p.num_draws);
-    if (rc != GLIMP_OK) return rc;
-    p.draws = f->d_draws;
-    memcpy(p.object_to_clip, d->object_to_clip, sizeof(p.object_to_clip));
-    const float sw = (float)f->width, sh = (float)f->height;
-    p.scale_x = 2.0f / sw;
-    p.scale_y = 2.0f / sh;
-    p.center_x = std::fma(0.5f, p.scale_x, -1.0f);
-    p.center_y = std::fma(0.5f, p.scale_y, -1.0f);
-    p.kx = 2.0f / sw;
-    p.ky = -2.0f / sh;
     p.texels = (const float4*)d->texels;
     p.uv = (const float2*)d->uv;
     p.color = (const float4*)d->color;
+    return GLIMP_OK;
+}
+
+// One framebuffer's entry of the view table.
+int diff_view(glimp_framebuffer* f, const float* object_to_clip, uint32_t pixel_base, hipStream_t s,
+              const std::string& who, DiffView& V) {
+    if (!f) return fail(GLIMP_E_INVALID, who + ": NULL framebuffer");
+    HIP_TRY(flush_clear(f, s));
+    memset(&V, 0, sizeof(V));
+    V.keys = f->keys;
+    V.width = (int)f->width;
+    V.height = (int)f->height;
+    uint32_t db;
+    memcpy(&db, &f->clear_depth, 4);
+    V.clear_key = (unsigned long long)db << 32;
+    int rc = upload_draws(f, s, &V.num_draws);
+    if (rc != GLIMP_OK) return rc;
+    V.draws = f->d_draws;
+    V.pixel_base = pixel_base;
+    memcpy(V.object_to_clip, object_to_clip, sizeof(V.object_to_clip));
+    const float sw = (float)f->width, sh = (float)f->height;
+    V.scale_x = 2.0f / sw;
+    V.scale_y = 2.0f / sh;
+    V.center_x = std::fma(0.5f, V.scale_x, -1.0f);
+    V.center_y = std::fma(0.5f, V.scale_y, -1.0f);
+    V.kx = 2.0f / sw;
+    V.ky = -2.0f / sh;
+    return GLIMP_OK;
+}
+
+// Parameters of a call over `count` views (one view is passed inline).
+int diff_setup(glimp_context* c, glimp_scene* sc, const glimp_diff_desc* d, const glimp_diff_view* views,
+               uint32_t count, hipStream_t s, const std::string& who, DiffParams& p, uint64_t& nv) {
+    int rc = diff_common(c, sc, d, s, who, p, nv);
+    if (rc != GLIMP_OK) return rc;
+    if (count == 0 || !views) return fail(GLIMP_E_INVALID, who + ": no views");
+    std::vector<DiffView> table(count);
+    uint64_t base = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        rc = diff_view(views[i].framebuffer, views[i].object_to_clip, (uint32_t)base, s, who, table[i]);
+        if (rc != GLIMP_OK) return rc;
+        base += (uint64_t)table[i].width * (uint64_t)table[i].height;
+        if (base >= (1ull << 31)) return fail(GLIMP_E_LIMIT, who + ": more than 2^31 pixels in one call");
+    }
+    p.num_views = count;
+    p.total_pixels = (uint32_t)base;
+    if (count == 1) {
+        p.view = table[0];
+        p.views = nullptr;
+    } else {
+        rc = grow_one(c->d_diff_views, c->d_diff_views_cap, count);
+        if (rc != GLIMP_OK) return rc;
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipMemcpy(c->d_diff_views, table.data(), count * sizeof(DiffView), hipMemcpyHostToDevice));
+        p.views = c->d_diff_views;
+    }
+    return GLIMP_OK;
+}
+
+int diff_forward(glimp_context* c, glimp_scene* sc, const glimp_diff_desc* d, const glimp_diff_view* views,
+                 uint32_t count, float* out, void* stream, const std::string& who) {
+    hipStream_t s = (hipStream_t)stream;
+    DiffParams p;
+    uint64_t nv = 0;
+    int rc = diff_setup(c, sc, d, views, count, s, who, p, nv);
+    if (rc != GLIMP_OK) return rc;
+    if (!out) return fail(GLIMP_E_INVALID, who + ": NULL output");
+    p.out = (float4*)out;
+    HIP_TRY(launch_diff_forward(p, s, &c->prof));
+    return GLIMP_OK;
+}
+
+int diff_backward(glimp_context* c, glimp_scene* sc, const glimp_diff_desc* d, const glimp_diff_view* views,
+                  uint32_t count, const float* grad_out, float* grad_texels, float* grad_uv, float* grad_color,
+                  void* stream, const std::string& who) {
+    hipStream_t s = (hipStream_t)stream;
+    DiffParams p;
+    uint64_t nv = 0;
+    int rc = diff_setup(c, sc, d, views, count, s, who, p, nv);
+    if (rc != GLIMP_OK) return rc;
+    if (!grad_out || !grad_texels || !grad_uv || !grad_color)
+        return fail(GLIMP_E_INVALID, who + ": NULL gradient array");
+    p.grad_out = (const float4*)grad_out;
+    HIP_TRY(diff_workspace_reserve(c->diff_ws, p.total_pixels));
+    HIP_TRY(launch_diff_backward(p, c->diff_ws, d->num_texels, (uint32_t)nv, (float4*)grad_texels, (float2*)grad_uv,
+                                 (float4*)grad_color, s, &c->prof));
     return GLIMP_OK;
 }
 
@@ -995,31 +1076,32 @@ extern "C" {
 
 int glimp_diff_forward(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const glimp_diff_desc* d, float* out,
                        void* stream) {
-    hipStream_t s = (hipStream_t)stream;
-    DiffParams p;
-    uint64_t nv = 0;
-    int rc = diff_params(c, f, sc, d, s, "glimp_diff_forward", p, nv);
-    if (rc != GLIMP_OK) return rc;
-    if (!out) return fail(GLIMP_E_INVALID, "glimp_diff_forward: NULL output");
-    p.out = (float4*)out;
-    HIP_TRY(launch_diff_forward(p, s, &c->prof));
-    return GLIMP_OK;
+    if (!d) return fail(GLIMP_E_INVALID, "glimp_diff_forward: NULL argument");
+    glimp_diff_view v;
+    v.framebuffer = f;
+    memcpy(v.object_to_clip, d->object_to_clip, sizeof(v.object_to_clip));
+    return diff_forward(c, sc, d, &v, 1, out, stream, "glimp_diff_forward");
 }
 
 int glimp_diff_backward(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const glimp_diff_desc* d,
                         const float* grad_out, float* grad_texels, float* grad_uv, float* grad_color, void* stream) {
-    hipStream_t s = (hipStream_t)stream;
-    DiffParams p;
-    uint64_t nv = 0;
-    int rc = diff_params(c, f, sc, d, s, "glimp_diff_backward", p, nv);
-    if (rc != GLIMP_OK) return rc;
-    if (!grad_out || !grad_texels || !grad_uv || !grad_color)
-        return fail(GLIMP_E_INVALID, "glimp_diff_backward: NULL gradient array");
-    p.grad_out = (const float4*)grad_out;
-    HIP_TRY(diff_workspace_reserve(c->diff_ws, (size_t)f->width * f->height));
-    HIP_TRY(launch_diff_backward(p, c->diff_ws, d->num_texels, (uint32_t)nv, (float4*)grad_texels, (float2*)grad_uv,
-                                 (float4*)grad_color, s, &c->prof));
-    return GLIMP_OK;
+    if (!d) return fail(GLIMP_E_INVALID, "glimp_diff_backward: NULL argument");
+    glimp_diff_view v;
+    v.framebuffer = f;
+    memcpy(v.object_to_clip, d->object_to_clip, sizeof(v.object_to_clip));
+    return diff_backward(c, sc, d, &v, 1, grad_out, grad_texels, grad_uv, grad_color, stream, "glimp_diff_backward");
+}
+
+int glimp_diff_forward_views(glimp_context* c, glimp_scene* sc, const glimp_diff_desc* d,
+                             const glimp_diff_view* views, uint32_t count, float* out, void* stream) {
+    return diff_forward(c, sc, d, views, count, out, stream, "glimp_diff_forward_views");
+}
+
+int glimp_diff_backward_views(glimp_context* c, glimp_scene* sc, const glimp_diff_desc* d,
+                              const glimp_diff_view* views, uint32_t count, const float* grad_out,
+                              float* grad_texels, float* grad_uv, float* grad_color, void* stream) {
+    return diff_backward(c, sc, d, views, count, grad_out, grad_texels, grad_uv, grad_color, stream,
+                         "glimp_diff_backward_views");
 }
 
 int glimp_framebuffer_color(glimp_framebuffer* f, uint32_t** color) {

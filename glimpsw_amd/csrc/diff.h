@@ -14,6 +14,20 @@ struct DiffTex {                   // fp32 RGBA mip chain, level m row-major at 
     uint64_t level_offset[16];     // in texels
 };
 
+// One framebuffer of a differentiable resolve. Pixels of all views are
+// numbered consecutively (view v owns [pixel_base, pixel_base + W*H)); a
+// view's outputs and gradient inputs are row-major at its pixel base.
+struct DiffView {
+    const unsigned long long* keys;
+    int width, height;
+    unsigned long long clear_key;
+    const DrawEntry* draws;
+    uint32_t num_draws;
+    uint32_t pixel_base;
+    float object_to_clip[16];
+    float scale_x, scale_y, center_x, center_y, kx, ky;
+};
+
 struct DiffParams {
     DevMeshlets m;
     const DevMaterial* materials;
@@ -21,18 +35,15 @@ struct DiffParams {
     const DiffTex* textures;
     uint32_t num_textures;
     ApproxTables tables;
-    const unsigned long long* keys;
-    int width, height;
-    unsigned long long clear_key;
-    const DrawEntry* draws;
-    uint32_t num_draws;
-    float object_to_clip[16];
-    float scale_x, scale_y, center_x, center_y, kx, ky;
+    uint32_t num_views;
+    const DiffView* views;         // device table, or null: `view` holds the only view
+    DiffView view;
+    uint32_t total_pixels;
     const float4* texels;
     const float2* uv;              // [meshlets * 64]
     const float4* color;           // [meshlets * 64]
-    float4* out;                   // forward: [height][width]
-    const float4* grad_out;        // backward: [height][width]
+    float4* out;                   // forward: [total_pixels]
+    const float4* grad_out;        // backward: [total_pixels]
     // backward records, 4 texel and 3 vertex records per pixel slot
     uint32_t* tex_keys;
     float4* tex_vals;

@@ -1,8 +1,9 @@
 // Differentiable texture-sampling resolve for gfx950.
 //
 // Forward: per covered pixel, the surface from the vis-buffer key, the
-// triangle's barycentrics (IntersectTriangle, as the resolve computes them;
-// coverage and barycentrics are constants of the pass), then
+// triangle's barycentrics (IntersectTriangle's formula with correctly
+// rounded divisions; coverage and barycentrics are constants of the pass),
+// then
 //   U,V = uv_0 + b1 (uv_1 - uv_0) + b2 (uv_2 - uv_0)
 //   col = color_0 + b1 (color_1 - color_0) + b2 (color_2 - color_0)
 //   out = Sample(U, V) * col
@@ -45,20 +46,51 @@ struct PixelState {
     float xs, ys;
 };
 
+__device__ __forceinline__ const DiffView& view_at(const DiffParams& p, uint32_t v) {
+    return p.views ? p.views[v] : p.view;
+}
+
+// View owning global pixel slot t.
+__device__ __forceinline__ uint32_t view_of_pixel(const DiffParams& p, uint32_t t) {
+    uint32_t lo = 0, hi = p.num_views;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (view_at(p, mid).pixel_base <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Pixel slot t of the global numbering: its view, and the pixel's row-major
+// index and 4x4-tiled index inside that view (slots run over the tiles).
+struct PixelSlot {
+    uint32_t view, x, y, tiled;
+    bool inside;
+};
+
+__device__ __forceinline__ PixelSlot pixel_slot(const DiffParams& p, uint32_t t) {
+    PixelSlot s;
+    s.inside = t < p.total_pixels;
+    s.view = s.inside ? view_of_pixel(p, t) : 0;
+    const DiffView& V = view_at(p, s.view);
+    s.tiled = t - (s.inside ? V.pixel_base : t);
+    const uint32_t W = (uint32_t)V.width;
+    const uint32_t tile = s.tiled >> 4, q = s.tiled & 15u, tilesX = W / 4;
+    s.x = (tile % tilesX) * 4 + (q & 3u);
+    s.y = (tile / tilesX) * 4 + (q >> 2);
+    return s;
+}
+
 // Surface, barycentrics and sampler choices of pixel slot t (tiled order:
-// 16 lanes = one 4x4 tile). Every lane of the wave must call it.
-__device__ void diff_pixel(const DiffParams& p, uint32_t t, PixelState& S) {
-    const uint32_t W = (uint32_t)p.width, H = (uint32_t)p.height;
-    const uint32_t tilesX = W / 4;
-    const uint32_t tile = t >> 4, q = t & 15u;
-    const uint32_t tx = tile % tilesX, ty = tile / tilesX;
-    const bool inside = ty * 4 < H;
-    const uint32_t x = tx * 4 + (q & 3u), y = ty * 4 + (q >> 2);
+// 16 lanes = one 4x4 tile of one view). Every lane of the wave must call it.
+__device__ void diff_pixel(const DiffParams& p, const PixelSlot& ps, PixelState& S) {
+    const DiffView& VW = view_at(p, ps.view);
+    const bool inside = ps.inside;
+    const uint32_t x = ps.x, y = ps.y;
     const uint32_t lane = __lane_id();
     const unsigned long long groupMask = 0xFFFFull << (lane & 48u);
     const ApproxTables& T = p.tables;
 
-    const unsigned long long key = inside ? p.keys[t] : p.clear_key;
+    const unsigned long long key = inside ? VW.keys[ps.tiled] : VW.clear_key;
     const bool active = inside && !(bitsf((uint32_t)(key >> 32)) <= 0.0f);
     S.active = active;
     S.tex = -1;
@@ -69,7 +101,7 @@ __device__ void diff_pixel(const DiffParams& p, uint32_t t, PixelState& S) {
     uint32_t material = 0xFFFFFFFFu;
     if (active) {
         uint32_t tri;
-        decode_surface(key, p.draws, p.num_draws, S.mi, tri);
+        decode_surface(key, VW.draws, VW.num_draws, S.mi, tri);
         const uint8_t* I = p.m.indices + (size_t)S.mi * 384;
         const float* P = p.m.positions + (size_t)S.mi * 192;
         float c[3][4], uv[3][2];
@@ -79,16 +111,16 @@ __device__ void diff_pixel(const DiffParams& p, uint32_t t, PixelState& S) {
             S.vi[k] = vi;
             const float px = P[vi], py = P[64 + vi], pz = P[128 + vi];
 #pragma unroll
-            for (int j = 0; j < 4; j++) c[k][j] = xform(p.object_to_clip, j, px, py, pz);
+            for (int j = 0; j < 4; j++) c[k][j] = xform(VW.object_to_clip, j, px, py, pz);
             const float2 t2 = p.uv[(size_t)S.mi * 64 + vi];
             uv[k][0] = t2.x;
             uv[k][1] = t2.y;
         }
         material = p.m.material_id[S.mi];
-        const float su = __builtin_fmaf((float)(int32_t)x, p.scale_x, p.center_x);
-        const float sv = __builtin_fmaf((float)(int32_t)y, p.scale_y, p.center_y);
+        const float su = __builtin_fmaf((float)(int32_t)x, VW.scale_x, VW.center_x);
+        const float sv = __builtin_fmaf((float)(int32_t)y, VW.scale_y, VW.center_y);
         float bary[3], ddx[3], ddy[3];
-        intersect_triangle(T, c, su, sv, p.kx, p.ky, bary, ddx, ddy);
+        intersect_triangle<true>(T, c, su, sv, VW.kx, VW.ky, bary, ddx, ddy);
         S.b1 = bary[1];
         S.b2 = bary[2];
         const float t10x = uv[1][0] - uv[0][0], t10y = uv[1][1] - uv[0][1];
@@ -228,32 +260,33 @@ __device__ __forceinline__ float4 vertex_color(const DiffParams& p, const PixelS
     return fma4(S.b2, c2 - c0, fma4(S.b1, c1 - c0, c0));
 }
 
+__device__ __forceinline__ size_t out_index(const DiffParams& p, const PixelSlot& ps) {
+    const DiffView& V = view_at(p, ps.view);
+    return (size_t)V.pixel_base + (size_t)ps.y * (uint32_t)V.width + ps.x;
+}
+
 __global__ void __launch_bounds__(256) k_diff_forward(DiffParams p) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const PixelSlot ps = pixel_slot(p, t);
     PixelState S;
-    diff_pixel(p, t, S);
-    const uint32_t W = (uint32_t)p.width;
-    const uint32_t tile = t >> 4, q = t & 15u, tilesX = W / 4;
-    const uint32_t x = (tile % tilesX) * 4 + (q & 3u), y = (tile / tilesX) * 4 + (q >> 2);
-    if (y >= (uint32_t)p.height) return;
+    diff_pixel(p, ps, S);
+    if (!ps.inside) return;
     float4 o = f4(0.0f);
     if (S.active) {
         SampleEval E;
         sample_eval(p, S, E);
         o = E.val * vertex_color(p, S);
     }
-    p.out[(size_t)y * W + x] = o;
+    p.out[out_index(p, ps)] = o;
 }
 
 // Per-pixel gradient records (fixed slots 4t.. and 3t..).
 __global__ void __launch_bounds__(256) k_diff_records(DiffParams p) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const PixelSlot ps = pixel_slot(p, t);
     PixelState S;
-    diff_pixel(p, t, S);
-    const uint32_t W = (uint32_t)p.width;
-    const uint32_t tile = t >> 4, q = t & 15u, tilesX = W / 4;
-    const uint32_t x = (tile % tilesX) * 4 + (q & 3u), y = (tile / tilesX) * 4 + (q >> 2);
-    if (y >= (uint32_t)p.height) return;
+    diff_pixel(p, ps, S);
+    if (!ps.inside) return;
     uint32_t tkeys[4] = { p.tex_sentinel, p.tex_sentinel, p.tex_sentinel, p.tex_sentinel };
     float4 tvals[4] = { f4(0.0f), f4(0.0f), f4(0.0f), f4(0.0f) };
     uint32_t vkeys[3] = { p.vtx_sentinel, p.vtx_sentinel, p.vtx_sentinel };
@@ -261,7 +294,7 @@ __global__ void __launch_bounds__(256) k_diff_records(DiffParams p) {
     if (S.active) {
         SampleEval E;
         sample_eval(p, S, E);
-        const float4 g = p.grad_out[(size_t)y * W + x];
+        const float4 g = p.grad_out[out_index(p, ps)];
         const float4 col = vertex_color(p, S);
         const float4 gval = g * col;
         for (int k = 0; k < E.n; k++) {
@@ -454,7 +487,7 @@ void diff_workspace_release(DiffWorkspace& w) {
 }
 
 hipError_t launch_diff_forward(const DiffParams& p, hipStream_t s, Profiler* prof) {
-    const uint32_t n = (uint32_t)p.width * (uint32_t)p.height;
+    const uint32_t n = p.total_pixels;
     Profiler::Scope sc(prof, "diff_forward", s);
     hipLaunchKernelGGL(k_diff_forward, dim3((n + 255) / 256), dim3(256), 0, s, p);
     return hipGetLastError();
@@ -463,7 +496,7 @@ hipError_t launch_diff_forward(const DiffParams& p, hipStream_t s, Profiler* pro
 hipError_t launch_diff_backward(DiffParams p, DiffWorkspace& w, uint64_t num_texels, uint32_t num_vertices,
                                 float4* grad_texels, float2* grad_uv, float4* grad_color, hipStream_t s,
                                 Profiler* prof) {
-    const uint32_t n = (uint32_t)p.width * (uint32_t)p.height;
+    const uint32_t n = p.total_pixels;
     p.tex_keys = w.tex_keys;
     p.tex_vals = w.tex_vals;
     p.vtx_keys = w.vtx_keys;

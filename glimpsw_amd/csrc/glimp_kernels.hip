@@ -183,29 +183,38 @@ constexpr uint32_t kCullPerBlock = 256 * kCullPerThread;
 
 __global__ void __launch_bounds__(256) k_cull(DrawParams p) {
     __shared__ uint32_t s_wave[4];
-    __shared__ uint32_t s_base;
+    __shared__ uint32_t s_base, s_view0;
     const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+    const uint32_t first = blockIdx.x * kCullPerBlock;
+    // the block's meshlets are consecutive: one search for the first view,
+    // then each thread walks forward from it
+    if (threadIdx.x == 0) s_view0 = view_of_meshlet(p, first);
+    __syncthreads();
+    uint32_t v = s_view0;
     bool vis[kCullPerThread];
+    uint32_t views[kCullPerThread];
     uint32_t mine = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kCullPerThread; k++) {
-        const uint32_t g = blockIdx.x * kCullPerBlock + k * 256 + threadIdx.x;
-        bool v = false;
+        const uint32_t g = first + k * 256 + threadIdx.x;
+        bool ok = false;
         if (g < p.total_meshlets) {
-            const ViewParams& V = view_of(p, view_of_meshlet(p, g));
-            v = true;
+            while (v + 1 < p.num_views && view_of(p, v + 1).meshlet_base <= g) v++;
+            const ViewParams& V = view_of(p, v);
+            ok = true;
             if (V.use_cull) {
                 float4 b = V.m.bounds[V.meshlet_offset + (g - V.meshlet_base)];
 #pragma unroll
                 for (int j = 0; j < 5; j++) {
                     const float* P = V.planes + 4 * j;
                     float d = __builtin_fmaf(b.x, P[0], __builtin_fmaf(b.y, P[1], b.z * P[2])) + P[3];
-                    v = v && (d > -b.w);
+                    ok = ok && (d > -b.w);
                 }
             }
         }
-        vis[k] = v;
-        mine += v ? 1u : 0u;
+        vis[k] = ok;
+        views[k] = v;
+        mine += ok ? 1u : 0u;
     }
     uint32_t incl = mine;
 #pragma unroll
@@ -224,7 +233,7 @@ __global__ void __launch_bounds__(256) k_cull(DrawParams p) {
     for (uint32_t w = 0; w < wid; w++) pos += s_wave[w];
 #pragma unroll
     for (uint32_t k = 0; k < kCullPerThread; k++)
-        if (vis[k]) p.visible_list[pos++] = blockIdx.x * kCullPerBlock + k * 256 + threadIdx.x;
+        if (vis[k]) p.visible_list[pos++] = make_uint2(first + k * 256 + threadIdx.x, views[k]);
 }
 
 // Standalone cull of one meshlet range (ShadingContext::CullMeshlets).
@@ -259,8 +268,8 @@ __global__ void k_cull_range(DevMeshlets m, uint32_t first, uint32_t count, Plan
 // ballot and writes them into its own slots. Waves loop over the visible
 // list, so the grid is sized to the machine, not to the scene.
 __device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, float (&lds)[4][64]) {
-    const uint32_t g = p.visible_list[i];      // global meshlet number
-    const uint32_t v = view_of_meshlet(p, g);
+    const uint2 gv = p.visible_list[i];
+    const uint32_t g = gv.x, v = gv.y;         // global meshlet number, view
     const ViewParams& V = view_of(p, v);
     const uint32_t w = g - V.meshlet_base;     // meshlet index within the view
     const uint32_t mi = V.meshlet_offset + w;
@@ -659,6 +668,21 @@ __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p, int use_spi
 
 // ---------------------------------------------------------------- binning
 
+// Ascending sort of one u32 per lane across the wave (bitonic network).
+__device__ __forceinline__ uint32_t wave_sort_u32(uint32_t v, uint32_t lane) {
+#pragma unroll
+    for (uint32_t k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t o = (uint32_t)__shfl_xor((int)v, (int)j);
+            const bool up = (lane & k) == 0;       // this lane's block sorts ascending
+            const bool low = (lane & j) == 0;      // lower element of the pair
+            v = (low == up) ? min(v, o) : max(v, o);
+        }
+    }
+    return v;
+}
+
 // Screen binning of the setup records (the role of DistributeToBins and
 // BinQueue::InsertBin, Rasterizer.cpp:663-776, with fixed-capacity bins):
 // each wave expands a chunk of records into their (record, bin) pairs with
@@ -705,6 +729,8 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
         __builtin_amdgcn_wave_barrier();
         for (uint32_t base = 0; base < total; base += 64) {
             const uint32_t j = base + lane;
+            // (bin << 6 | record lane) of this lane's pair; the sentinel sorts last
+            uint32_t item = 0xFFFFFFFFu;
             if (j < total) {
                 uint32_t k = 0;
 #pragma unroll
@@ -714,8 +740,27 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
                 const uint32_t q = j - s_start[wid][k];
                 const uint32_t w = s_bw[wid][k] & 0xFFFFu, pitch_k = s_bw[wid][k] >> 16;
                 const uint32_t dy = q / w, dx = q - dy * w;
-                const uint32_t bin = s_org[wid][k] + dy * pitch_k + dx;
-                const uint32_t slot = atomicAdd(&p.bin_counts[bin], 1u);
+                item = ((s_org[wid][k] + dy * pitch_k + dx) << 6) | k;
+            }
+            // A meshlet's triangles share few bins: sort the wave's pairs by
+            // bin so each run of equal bins takes its slots with one atomic.
+            item = wave_sort_u32(item, lane);
+            const uint32_t bin = item >> 6;
+            const bool valid = item != 0xFFFFFFFFu;
+            const uint32_t prev = __shfl_up(item, 1);
+            const bool head = valid && (lane == 0 || (prev >> 6) != bin);
+            const unsigned long long heads = __ballot(head);
+            const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
+            const unsigned long long later = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+            const uint32_t runEnd = later ? (uint32_t)__ffsll((long long)later) - 1 : nvalid;
+            uint32_t got = 0;
+            if (head) got = atomicAdd(&p.bin_counts[bin], runEnd - lane);
+            // first lane of this lane's run: highest head at or below it
+            const unsigned long long upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+            const uint32_t runStart = upto ? 63u - (uint32_t)__clzll((long long)upto) : 0u;
+            const uint32_t slot = __shfl(got, (int)runStart) + (lane - runStart);
+            if (valid) {
+                const uint32_t k = item & 63u;
                 if (slot < p.bin_capacity) p.bin_lists[(size_t)bin * p.bin_capacity + slot] = ck.base + k;
                 else s_spill[wid][k] = 1;
             }

@@ -144,7 +144,7 @@ struct DrawParams {
     uint32_t total_bins;
     uint32_t flags;
     ApproxTables tables;
-    uint32_t* visible_list;        // global meshlet numbers that passed the cull, compacted
+    uint2* visible_list;           // (global meshlet number, view) of each meshlet that passed the cull
     TriRecord* records;            // [total_meshlets * 128 + clip_record_capacity]
     uint32_t* meshlet_counts;      // [total_meshlets] records kept per meshlet
     uint32_t clip_record_base;     // = total_meshlets * 128

@@ -13,7 +13,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import DiffDesc, DiffTextureDesc, _check, _stream_ptr
+from . import DiffDesc, DiffTextureDesc, DiffViewDesc, _check, _stream_ptr
 from . import textures as tx
 
 
@@ -70,16 +70,24 @@ def vertex_uv(scene) -> np.ndarray:
 
 
 class _Binding:
-    def __init__(self, ctx, fb, dscene, object_to_clip, chains: TexelChains):
-        self.ctx, self.fb, self.dscene = ctx, fb, dscene
-        self.clip = [float(v) for v in object_to_clip]
+    """Scene, texel layout and the views (framebuffer, object-to-clip matrix)
+    of a differentiable resolve."""
+
+    def __init__(self, ctx, dscene, views, chains: TexelChains):
+        self.ctx, self.dscene = ctx, dscene
+        self.views = list(views)
         self.chains = chains
         self.tex_descs = chains.descs()
+        self.view_descs = (DiffViewDesc * len(self.views))()
+        for i, (fb, clip) in enumerate(self.views):
+            self.view_descs[i].framebuffer = fb.handle.value if hasattr(fb.handle, "value") else fb.handle
+            for k in range(16):
+                self.view_descs[i].object_to_clip[k] = float(clip[k])
+        self.sizes = [(fb.height, fb.width) for fb, _ in self.views]
+        self.total_pixels = sum(h * w for h, w in self.sizes)
 
     def desc(self, texels, uv, color) -> DiffDesc:
         d = DiffDesc()
-        for i in range(16):
-            d.object_to_clip[i] = self.clip[i]
         d.textures = C.cast(self.tex_descs, C.POINTER(DiffTextureDesc))
         d.num_textures = len(self.chains.sizes)
         d.num_texels = self.chains.num_texels
@@ -90,11 +98,12 @@ class _Binding:
 def _forward(b: _Binding, texels, uv, color):
     import torch
 
-    out = torch.empty((b.fb.height, b.fb.width, 4), dtype=torch.float32, device=texels.device)
+    out = torch.empty((b.total_pixels, 4), dtype=torch.float32, device=texels.device)
     d = b.desc(texels, uv, color)
     stream = torch.cuda.current_stream(texels.device)
-    _check(b.ctx.lib.glimp_diff_forward(b.ctx.handle, b.fb.handle, b.dscene.handle, C.byref(d),
-                                        C.c_void_p(out.data_ptr()), _stream_ptr(stream)), "glimp_diff_forward")
+    _check(b.ctx.lib.glimp_diff_forward_views(b.ctx.handle, b.dscene.handle, C.byref(d), b.view_descs,
+                                              len(b.views), C.c_void_p(out.data_ptr()), _stream_ptr(stream)),
+           "glimp_diff_forward_views")
     return out
 
 
@@ -107,10 +116,11 @@ def _backward(b: _Binding, texels, uv, color, grad_out):
     gc = torch.empty_like(color)
     d = b.desc(texels, uv, color)
     stream = torch.cuda.current_stream(texels.device)
-    _check(b.ctx.lib.glimp_diff_backward(b.ctx.handle, b.fb.handle, b.dscene.handle, C.byref(d),
-                                         C.c_void_p(grad_out.data_ptr()), C.c_void_p(gt.data_ptr()),
-                                         C.c_void_p(gu.data_ptr()), C.c_void_p(gc.data_ptr()),
-                                         _stream_ptr(stream)), "glimp_diff_backward")
+    _check(b.ctx.lib.glimp_diff_backward_views(b.ctx.handle, b.dscene.handle, C.byref(d), b.view_descs,
+                                               len(b.views), C.c_void_p(grad_out.data_ptr()),
+                                               C.c_void_p(gt.data_ptr()), C.c_void_p(gu.data_ptr()),
+                                               C.c_void_p(gc.data_ptr()), _stream_ptr(stream)),
+           "glimp_diff_backward_views")
     return gt, gu, gc
 
 
@@ -140,23 +150,54 @@ def _torch_function():
 _FN = None
 
 
-class DiffResolve:
-    """Resolve of one framebuffer as a differentiable function:
-    ``out = DiffResolve(ctx, fb, scene, object_to_clip, chains)(texels, uv, color)``
-    with ``texels`` (num_texels, 4), ``uv`` (meshlets*64, 2) and ``color``
-    (meshlets*64, 4) float32 CUDA tensors; ``out`` is (H, W, 4)."""
+def _apply(binding, texels, uv, color):
+    global _FN
+    if _FN is None:
+        _FN = _torch_function()
+    return _FN.apply(texels, uv, color, binding)
 
-    def __init__(self, ctx, fb, dscene, object_to_clip, chains: TexelChains):
-        self.binding = _Binding(ctx, fb, dscene, object_to_clip, chains)
+
+class DiffResolveViews:
+    """Differentiable resolve of several framebuffers of one scene in one
+    launch per stage: ``out = op(texels, uv, color)`` is (sum of H*W, 4), the
+    views' row-major images one after another; gradients sum over views."""
+
+    def __init__(self, ctx, dscene, views, chains: TexelChains):
+        self.binding = _Binding(ctx, dscene, views, chains)
 
     def __call__(self, texels, uv, color):
-        global _FN
-        if _FN is None:
-            _FN = _torch_function()
-        return _FN.apply(texels, uv, color, self.binding)
+        return _apply(self.binding, texels, uv, color)
 
     def forward(self, texels, uv, color):
         return _forward(self.binding, texels, uv, color)
 
     def backward(self, texels, uv, color, grad_out):
         return _backward(self.binding, texels, uv, color, grad_out)
+
+    def split(self, out):
+        """Per-view (H, W, 4) views of a (sum of H*W, 4) tensor."""
+        parts, o = [], 0
+        for h, w in self.binding.sizes:
+            parts.append(out[o:o + h * w].view(h, w, 4))
+            o += h * w
+        return parts
+
+
+class DiffResolve(DiffResolveViews):
+    """Resolve of one framebuffer as a differentiable function:
+    ``out = DiffResolve(ctx, fb, scene, object_to_clip, chains)(texels, uv, color)``
+    with ``texels`` (num_texels, 4), ``uv`` (meshlets*64, 2) and ``color``
+    (meshlets*64, 4) float32 CUDA tensors; ``out`` is (H, W, 4)."""
+
+    def __init__(self, ctx, fb, dscene, object_to_clip, chains: TexelChains):
+        super().__init__(ctx, dscene, [(fb, object_to_clip)], chains)
+        self.shape = (fb.height, fb.width, 4)
+
+    def __call__(self, texels, uv, color):
+        return _apply(self.binding, texels, uv, color).view(self.shape)
+
+    def forward(self, texels, uv, color):
+        return _forward(self.binding, texels, uv, color).view(self.shape)
+
+    def backward(self, texels, uv, color, grad_out):
+        return _backward(self.binding, texels, uv, color, grad_out.reshape(-1, 4))

@@ -326,6 +326,41 @@ def batch_views(num_scenes: int = 8, num_views: int = 64, min_meshlets: int = 10
     return scenes, views
 
 
+def stress_views(num_views: int = 16, width: int = 2880, height: int = 1620, patches_per_layer: int = 5120,
+                 layers: int = 8, subpixel_frac: float = 0.30, tex_size: int = 512,
+                 num_materials: int = 16) -> tuple[Scene, list]:
+    """Overdraw stress: ``layers`` depth-stacked layers of grid patches
+    (depths 2 * 1.35^l), each layer spread over the frustum densely enough
+    that its front faces cover the screen about once (depth complexity about
+    ``layers``); a ``subpixel_frac`` share of patches is shrunk 12x so their
+    triangles are sub-pixel. ``num_views`` views near the layout's camera
+    (yaw and pitch within 3 degrees) at ``width`` x ``height``, close to the
+    2896-pixel fixed-point limit. Returns (scene, [View])."""
+    rng = SplitMix64(SEED_BASE + 4)
+    n = patches_per_layer * layers
+    aspect = width / height
+    f = 1.0 / math.tan(math.radians(60.0) / 2.0)
+    layer = np.repeat(np.arange(layers), patches_per_layer)
+    depth = 2.0 * 1.35 ** layer.astype(np.float64)
+    xy = rng.uniform(2 * n, -1.05, 1.05).reshape(n, 2)
+    jitter = rng.uniform(n, -0.05, 0.05)
+    centers = np.stack([xy[:, 0] * depth * aspect / f, xy[:, 1] * depth / f, -depth * (1.0 + jitter)], axis=1)
+    R = _random_rotations(rng, n)
+    scales = depth * 0.049 * rng.uniform(n, 0.8, 1.25)
+    tiny = rng.uniform(n) < subpixel_frac
+    scales[tiny] /= 12.0
+    mats = [(i, False, 255) for i in range(num_materials)]
+    texs = [textures.noise_texture(tex_size, SEED_BASE + 4000 + i) for i in range(num_materials)]
+    mid = (np.arange(n) % num_materials).astype(np.uint32)
+    scene = build_patches(centers, R, scales, mid, mats, texs)
+    views = []
+    for _ in range(num_views):
+        yaw = float(rng.uniform(1, -0.05, 0.05)[0])
+        pitch = float(rng.uniform(1, -0.05, 0.05)[0])
+        views.append(make_view(width, height, 60.0, view_matrix((0.0, 0.0, 0.0), yaw, pitch), 0, n))
+    return scene, views
+
+
 def inverse_screen_proj(world_to_clip: np.ndarray, width: int, height: int) -> np.ndarray:
     """GetInverseScreenProjMatrix (Camera.h:140-146): inverse(M) then
     translate(-1,-1,0), scale(2/size, 1), translate(0.5, 0.5, 0), so pixel

@@ -244,6 +244,21 @@ int glimp_diff_forward(glimp_context* ctx, glimp_framebuffer* fb, glimp_scene* s
 int glimp_diff_backward(glimp_context* ctx, glimp_framebuffer* fb, glimp_scene* scene, const glimp_diff_desc* desc,
                         const float* grad_out, float* grad_texels, float* grad_uv, float* grad_color, void* stream);
 
+/* Several framebuffers of one scene in one launch per stage: outputs and
+ * gradient inputs are the views' row-major images one after another
+ * ([sum of width*height][4]); gradients are summed over all views with the
+ * same deterministic reduction. desc->object_to_clip is not used. */
+typedef struct glimp_diff_view {
+    glimp_framebuffer* framebuffer;
+    float object_to_clip[16];
+} glimp_diff_view;
+
+int glimp_diff_forward_views(glimp_context* ctx, glimp_scene* scene, const glimp_diff_desc* desc,
+                             const glimp_diff_view* views, uint32_t count, float* out, void* stream);
+int glimp_diff_backward_views(glimp_context* ctx, glimp_scene* scene, const glimp_diff_desc* desc,
+                              const glimp_diff_view* views, uint32_t count, const float* grad_out,
+                              float* grad_texels, float* grad_uv, float* grad_color, void* stream);
+
 /* ------------------------------------------------------------ resolve */
 
 #define GLIMP_RESOLVE_ALBEDO 0u   /* ResolveDebug(DebugLayer::BaseColor) (Shading.cpp:734-773) */

@@ -547,6 +547,39 @@ void intersect_triangle(const float c[3][4], float su, float sv, float sw, float
     }
 }
 
+// IntersectTriangle's operation order with correctly rounded divisions (the
+// differentiable resolve's barycentrics).
+void intersect_triangle_exact(const float c[3][4], float su, float sv, float sw, float sh,
+                              float bary[3], float ddx[3], float ddy[3]) {
+    float iw0 = 1.0f / c[0][3], iw1 = 1.0f / c[1][3], iw2 = 1.0f / c[2][3];
+    float p0x = c[0][0] * iw0, p0y = c[0][1] * iw0;
+    float p1x = c[1][0] * iw1, p1y = c[1][1] * iw1;
+    float p2x = c[2][0] * iw2, p2y = c[2][1] * iw2;
+    float m0x = p2x - p1x, m1x = p0x - p1x, m1y = p0y - p1y;
+    float d12y = p1y - p2y;
+    float det = fmaf(m0x, m1y, d12y * m1x);
+    float invDet = 1.0f / det;
+    float a0 = invDet * iw0, a1 = invDet * iw1, a2 = invDet * iw2;
+    float dx0 = a0 * d12y, dx1 = a1 * (p2y - p0y), dx2 = a2 * m1y;
+    float dy0 = a0 * m0x, dy1 = a1 * (p0x - p2x), dy2 = a2 * (p1x - p0x);
+    float dxs = (dx0 + dx1) + dx2, dys = (dy0 + dy1) + dy2;
+    float r0x = su - p0x, r0y = sv - p0y;
+    float iiw = fmaf(dys, r0y, fmaf(dxs, r0x, iw0));
+    float iW = 1.0f / iiw;
+    float by = iW * fmaf(dx1, r0x, dy1 * r0y);
+    float bz = iW * fmaf(dx2, r0x, dy2 * r0y);
+    float bx = 1.0f - (by + bz);
+    float kx = 2.0f / sw, ky = -2.0f / sh;
+    float wdx = 1.0f / fmaf(kx, dxs, iiw), wdy = 1.0f / fmaf(ky, dys, iiw);
+    float b[3] = { bx, by, bz }, dxk[3] = { dx0, dx1, dx2 }, dyk[3] = { dy0, dy1, dy2 };
+    for (int k = 0; k < 3; k++) {
+        float bi = b[k] * iiw;
+        ddx[k] = fmaf(wdx, fmaf(kx, dxk[k], bi), -b[k]);
+        ddy[k] = fmaf(wdy, fmaf(ky, dyk[k], bi), -b[k]);
+        bary[k] = b[k];
+    }
+}
+
 struct SurfaceLane {
     bool active;
     float u, v;
@@ -1182,6 +1215,7 @@ namespace {
 struct DiffLane {
     bool active;
     uint32_t mi, vi[3];
+    float b1, b2;            // barycentrics (constants of the function)
     int32_t tex;             // -1: untextured
     int32_t level;
     bool linear;
@@ -1217,7 +1251,9 @@ void diff_tile(const orc_shading_ctx& ctx, const orc_diff_params& p, const uint3
         }
         material[l] = ms.material_id[mi];
         float bary[3], ddx[3], ddy[3];
-        intersect_triangle(c, su[l], sv[l], sw, sh, bary, ddx, ddy);
+        intersect_triangle_exact(c, su[l], sv[l], sw, sh, bary, ddx, ddy);
+        L[l].b1 = bary[1];
+        L[l].b2 = bary[2];
         float t10x = uv[1][0] - uv[0][0], t10y = uv[1][1] - uv[0][1];
         float t20x = uv[2][0] - uv[0][0], t20y = uv[2][1] - uv[0][1];
         U[l] = fmaf(bary[2], t20x, fmaf(bary[1], t10x, uv[0][0]));
@@ -1294,17 +1330,12 @@ struct DiffEval {
 
 void diff_eval(const orc_shading_ctx& ctx, const orc_diff_params& p, const DiffLane& D, uint32_t x, uint32_t y,
                uint32_t W, uint32_t H, DiffEval& E) {
-    const orc_meshlets& ms = *ctx.meshlets;
-    (void)ms;
-    float c[3][4];
-    const float* P = ctx.meshlets->positions + (size_t)D.mi * 192;
-    for (int k = 0; k < 3; k++) {
-        uint32_t vi = D.vi[k];
-        transform(p.object_to_clip, P[vi], P[64 + vi], P[128 + vi], c[k]);
-    }
-    const double su = (2.0 * x + 1.0) / W - 1.0, sv = (2.0 * y + 1.0) / H - 1.0;
-    double b[3];
-    orc_bary_f64(&c[0][0], su, sv, b);
+    (void)ctx;
+    (void)x;
+    (void)y;
+    (void)W;
+    (void)H;
+    const double b[3] = { 1.0 - (double)D.b1 - (double)D.b2, (double)D.b1, (double)D.b2 };
     E.w[0] = 1.0 - b[1] - b[2];
     E.w[1] = b[1];
     E.w[2] = b[2];

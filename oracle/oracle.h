@@ -150,6 +150,11 @@ void orc_setup16(const float* clip /*[3][4][16]*/, int32_t w, int32_t h, int32_t
  * row-major at level_offset[m]. Uncovered pixels give 0; untextured
  * materials sample 1.
  *
+ * Barycentrics come from IntersectTriangle's formula with correctly rounded
+ * divisions, in fp32 as the GPU evaluates them, and are constants of the
+ * function, like coverage (orc_bary_f64 checks that formula against the
+ * textbook one in double).
+ *
  * The oracle takes every discrete choice (surface, level, filter, wrap
  * period, texel cell, edge clamps) from the fp32 evaluation the GPU performs
  * and evaluates everything continuous in double, so its gradient is the FP64

@@ -78,17 +78,18 @@ def test_forward_constant_texture():
 
 def test_forward_vertex_colors_interpolate():
     """Unit texture: out is the perspective-correct interpolation of the
-    vertex colours at the pixel centre."""
+    vertex colours at the pixel centre. The fp32 barycentrics (IntersectTriangle's
+    formula) agree with the textbook double formula to 1e-5 on 99% of the
+    pixels; the rest are near-degenerate triangles (5e-4 bound)."""
     sc, view, fb, b, chains, uv, color = _setup()
     texels = np.ones_like(chains.data)
     out = oracle.diff_forward(fb, b, view.object_to_clip, chains, texels, uv, color)
     ids = fb.ids()
     depth = fb.depth()
     ys, xs = np.nonzero(depth > 0)
-    rng = np.random.default_rng(3)
     M = view.object_to_clip.astype(np.float64).reshape(4, 4).T
-    for i in rng.choice(len(ys), 200, replace=False):
-        y, x = ys[i], xs[i]
+    errs = []
+    for y, x in zip(ys, xs):
         mi, tri = divmod(int(ids[y, x]), 128)
         vis = [int(sc.indices[mi, k, tri]) for k in range(3)]
         clip = np.zeros((3, 4), np.float32)
@@ -99,7 +100,11 @@ def test_forward_vertex_colors_interpolate():
         sv = (2 * y + 1) / view.height - 1
         bw = oracle.bary_f64(clip, su, sv)
         want = sum(bw[k] * color[mi * 64 + vis[k]].astype(np.float64) for k in range(3))
-        assert np.allclose(out[y, x], want, atol=2e-4), (x, y, out[y, x], want)
+        errs.append(np.abs(out[y, x] - want).max())
+    errs = np.array(errs)
+    assert errs.size > 5000
+    assert np.percentile(errs, 99) < 1e-5, np.percentile(errs, [50, 99, 100])
+    assert errs.max() < 5e-4
 
 
 def test_forward_bilinear_ramp():
