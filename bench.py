@@ -31,32 +31,108 @@ HBM_PEAK_GBS = 8000.0
 
 
 def algorithmic_bytes(stats: dict) -> dict:
-    """Bytes each kernel must move per launch, from counted quantities of
-    the frame (meshlets visible, records, clip entries, fragments, pixels)."""
+    """Bytes each kernel must move per launch at least: every input it needs
+    read once and every output written once, from counted quantities of the
+    frame (meshlets visible, records, clip entries, bin entries, pixels).
+    Data that several work items share (a record in several bins, a vertex
+    of several pixels) counts once."""
     m_all, m_vis, t_vis = stats["meshlets"], stats["visible"], stats["tris_visible"]
     S, Cq, F, N = stats["records"], stats["clipped"], stats["fragments"], stats["pixels"]
     P = stats.get("bin_entries", 0)
+    ncov = stats.get("covered_pixels", 0)
+    # per visible meshlet: positions 768 B, indices 384 B, count + material 5 B
+    meshlet_in = m_vis * (768 + 384 + 5)
     return {
         "clear": 8 * N,
-        "cull": 16 * m_all + 2 * ((m_all + 15) // 16),
-        # positions 768 B + triangle count and material id per visible meshlet,
-        # 3 index bytes per triangle; 32-byte records and 64-byte clip entries out
-        "mesh_setup": m_vis * (768 + 1 + 4) + 3 * t_vis + 32 * S + 64 * Cq + 2 * ((m_all + 15) // 16),
+        # bounds in, survivor list out
+        "cull": 16 * m_all + 8 * m_vis,
+        # meshlets in, 32-byte records and 64-byte clip entries out
+        "mesh_setup": 8 * m_vis + meshlet_in + 32 * S + 64 * Cq,
         "clip": 64 * Cq,
-        # every record read once; every covered fragment reads/updates one 8-byte key
+        # records in, covered fragments update one 8-byte key each
         "raster_direct": 32 * S + 8 * F,
-        # records in, one 4-byte bin entry per (record, bin) pair plus the
-        # pair's bin-counter update
-        "bin_scatter": 32 * S + 8 * P,
-        # per pair the entry and its record are read; each pixel's key is
-        # written once
-        "bin_raster": 36 * P + 8 * N,
+        # records in, one 4-byte entry per (record, bin) pair out
+        "bin_scatter": 32 * S + 4 * P,
+        # entries and records in, every pixel's key written once
+        "bin_raster": 4 * P + 32 * S + 8 * N,
         "raster_spill": 0,
-        # key per pixel in, RGBA8 out; per covered pixel three vertices
-        # (index, position, UV, normal: 1 + 12 + 4 + 4 bytes each) and four
-        # RGBA8 texels
-        "resolve_lit": 8 * N + 4 * N + stats.get("covered_pixels", 0) * (3 * 21 + 16),
+        # keys in, RGBA8 out; vertex data (positions, indices, UVs, normals)
+        # of the visible meshlets once; one RGBA8 texel per covered pixel
+        # (the mip level is chosen for about one texel per pixel)
+        "resolve_lit": 8 * N + 4 * N + m_vis * (768 + 384 + 256 + 256) + 4 * ncov,
     }
+
+
+# profiler scope name -> kernel symbol substring in rocprofv3 output
+KERNEL_SYMBOLS = {
+    "clear": "k_clear(", "cull": "k_cull(", "mesh_setup": "k_mesh_setup(", "clip": "k_clip(",
+    "bin_scatter": "k_bin_scatter(", "bin_raster": "k_bin_raster<", "raster_spill": "k_raster_direct(",
+    "resolve_lit": "k_resolve(",
+}
+
+
+def _rocprof_counter(counter: str, child_args: list, tag: str) -> dict | None:
+    """Runs this script as a child under rocprofv3 --pmc <counter> and returns
+    the per-dispatch average of the counter for every kernel, or None."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None
+    out = tempfile.mkdtemp(prefix="glimp_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
+    cmd = ["timeout", "-s", "KILL", "150", exe, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", tag,
+           "--", sys.executable, os.path.abspath(__file__), *child_args]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=170)
+    except subprocess.TimeoutExpired:
+        return None
+    sums, counts = {}, {}
+    for root, _, files in os.walk(out):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                for row in csv.DictReader(open(os.path.join(root, f))):
+                    k = row["Kernel_Name"]
+                    sums[k] = sums.get(k, 0.0) + float(row["Counter_Value"])
+                    counts[k] = counts.get(k, 0) + 1
+    shutil.rmtree(out, ignore_errors=True)
+    if r.returncode != 0 and not sums:
+        return None
+    return {k: sums[k] / counts[k] for k in sums}
+
+
+def measure_traffic() -> dict | None:
+    """HBM bytes per launch of each kernel of the frame: FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) from two rocprofv3 runs of a short frame
+    loop, each scaled by what the counter reads for a device copy of known
+    size."""
+    copy_bytes = 1 << 28
+    frame = ["--steps", "3", "--warmup", "1", "--no-cpu-baseline", "--no-batch", "--no-traffic", "--no-counts"]
+    copy = ["--calibrate-copy", str(copy_bytes)]
+    fetch = _rocprof_counter("FETCH_SIZE", frame, "frame")
+    write = _rocprof_counter("WRITE_SIZE", frame, "frame")
+    cf = _rocprof_counter("FETCH_SIZE", copy, "copy")
+    cw = _rocprof_counter("WRITE_SIZE", copy, "copy")
+    if not (fetch and write and cf and cw):
+        return None
+
+    def pick(d, sub):
+        vals = [v for k, v in d.items() if sub in k]
+        return vals[0] if vals else None
+
+    kf, kw = pick(cf, "copyBuffer"), pick(cw, "copyBuffer")
+    if not kf or not kw:
+        return None
+    f_scale = copy_bytes / (kf * 1024.0)      # bytes per counted KiB
+    w_scale = copy_bytes / (kw * 1024.0)
+    out = {"fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4)}
+    for name, sub in KERNEL_SYMBOLS.items():
+        fv, wv = pick(fetch, sub), pick(write, sub)
+        if fv is not None and wv is not None:
+            out[name] = int(fv * 1024.0 * f_scale + wv * 1024.0 * w_scale)
+    return out
 
 
 def cpu_baseline(scene, view, seconds: float) -> dict:
@@ -150,6 +226,10 @@ def main() -> int:
     ap.add_argument("--bin-size", type=int, default=0, help="screen bin side (8/16/32/64), 0 = library default")
     ap.add_argument("--bin-capacity", type=int, default=0, help="records per bin, 0 = library default")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batching measurement")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM traffic passes")
+    ap.add_argument("--no-counts", action="store_true", help="skip the counting pass (frame kernels only)")
+    ap.add_argument("--frame-calls", action="store_true",
+                    help="time the frame as separate clear/draw/resolve calls instead of a graph replay")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES bytes 10 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -170,6 +250,10 @@ def main() -> int:
                                    "of vis-buffer keys)", "n_gpus": args.gpus}))
         return 2
 
+    # HBM traffic from counter passes in child processes, before this
+    # process touches the GPU
+    traffic = None if args.no_traffic else measure_traffic()
+
     import torch
     import glimpsw_amd as g
     from glimpsw_amd import scene as S
@@ -187,27 +271,55 @@ def main() -> int:
 
     lighting = S.default_lighting(view)
 
-    def frame():
+    def frame_calls():
+        # the reference's call sequence: Clear, DrawMeshlets (with the cull), Resolve
         fb.clear(0, 0.0, stream)
         fb.draw(ds, view.object_to_clip, view.meshlet_count, 0, planes, flags, stream, args.bin_size,
                 args.bin_capacity)
         fb.resolve(ds, view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting, stream)
 
-    for _ in range(args.warmup):
-        frame()
-    torch.cuda.synchronize()
+    # the same frame with clear + cull + draw replayed as one captured graph
+    # (a batch of one view); the vis-buffer is identical (tests/test_gpu_batch.py)
+    graph = None
+    if not args.direct and not args.frame_calls:
+        shape = {"bin_size": args.bin_size or 16, "bin_capacity": args.bin_capacity or 1024}
+        graph = g.Batch(ctx, [(ds, fb, view, planes, flags)], shape=shape)
 
+    def frame_graph():
+        graph.launch(True, stream)
+        fb.resolve(ds, view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting, stream)
+
+    def run(fn, steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    # per-kernel times: HIP events around every launch of the separate-call
+    # frame, on the launch stream, over the timed steps
+    for _ in range(args.warmup):
+        frame_calls()
     ctx.profile(True)
     ctx.profile_read(reset=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        frame()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    ms_calls = run(frame_calls, args.steps)
     kern = ctx.profile_read(reset=True)
     ctx.profile(False)
-    ms = (t1 - t0) * 1e3 / args.steps
+    # the headline frame time, without per-kernel events
+    frame = frame_graph if graph is not None else frame_calls
+    for _ in range(args.warmup):
+        frame()
+    ms = run(frame, args.steps)
+
+    if args.no_counts:
+        print(json.dumps({"ms_per_step": round(ms, 4), "kernels": {k: v for k, v in kern.items()}}))
+        if graph is not None:
+            graph.close()
+        fb.close()
+        ds.close()
+        ctx.close()
+        return 0
 
     # counted quantities of one frame (separate, untimed run)
     ctx.reset_counters()
@@ -230,7 +342,8 @@ def main() -> int:
         avg_ms = total / max(1, launches)
         b = alg.get(name, 0)
         per_kernel[name] = {"avg_us": round(avg_ms * 1e3, 2), "launches": launches, "alg_bytes": b,
-                            "GBps": round(b / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 else None}
+                            "GBps": round(b / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 else None,
+                            "hbm_bytes": traffic.get(name) if traffic else None}
     dom = max(kern.items(), key=lambda kv: kv[1][1])[0]
     dk = per_kernel[dom]
     tris = scene.triangles()
@@ -243,6 +356,8 @@ def main() -> int:
         "warmup": args.warmup,
         "ms_per_step": round(ms, 4),
         "fps": round(1e3 / ms, 1),
+        "frame": "graph replay of clear+cull+draw, then resolve" if graph is not None else "separate calls",
+        "ms_per_step_separate_calls": round(ms_calls, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -253,7 +368,10 @@ def main() -> int:
                    "submitted_tris": tris},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dk["GBps"] / HBM_PEAK_GBS, 4) if dk["GBps"] else None,
-                     "traffic": None, "alg_bytes_per_launch": dk["alg_bytes"], "avg_us": dk["avg_us"]},
+                     "traffic": traffic.get(dom) if traffic else None,
+                     "alg_bytes_per_launch": dk["alg_bytes"], "avg_us": dk["avg_us"]},
+        "traffic_calibration": ({"fetch_scale": traffic["fetch_scale"], "write_scale": traffic["write_scale"]}
+                                if traffic else None),
         "kernels": per_kernel,
         "frame_counts": stats,
         "scene_gen_s": round(t_gen, 2),
@@ -265,6 +383,8 @@ def main() -> int:
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result))
+    if graph is not None:
+        graph.close()
     fb.close()
     ds.close()
     ctx.close()

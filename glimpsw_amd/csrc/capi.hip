@@ -14,12 +14,13 @@
 #include "resolve.h"
 #include "diff.h"
 
-// 131072 u16: VRCP14 then VRSQRT14 mantissa tables, generated at build time
-// from glimpsw_amd/data/approx14.bin.
-static const uint16_t kApproxTables[] = {
-#include "approx14_tables.inc"
+// 128 (A, B) pairs of the piecewise-linear VRCP14 / VRSQRT14 mantissa
+// tables (device_math.h), generated at build time from
+// glimpsw_amd/data/approx14_pwl.txt and checked there against approx14.bin.
+static const uint32_t kApproxPwl[] = {
+#include "approx14_pwl.inc"
 };
-static_assert(sizeof(kApproxTables) == 131072 * 2, "approximation tables");
+static_assert(sizeof(kApproxPwl) == 256 * 4, "approximation pairs");
 
 using namespace glimp;
 
@@ -95,7 +96,7 @@ struct Scratch {
 
 struct glimp_context {
     int device = 0;
-    uint16_t* tables = nullptr;
+    uint32_t* tables = nullptr;
     Counters* counters = nullptr;
     DrawScratch* scratch = nullptr;
     uint32_t* visible = nullptr;
@@ -110,7 +111,7 @@ struct glimp_context {
     size_t d_diff_views_cap = 0;
     Profiler prof;
 
-    ApproxTables approx() const { return ApproxTables{ tables, tables + 65536 }; }
+    ApproxTables approx() const { return ApproxTables{ reinterpret_cast<const uint2*>(tables) }; }
 };
 
 struct glimp_texture {
@@ -140,6 +141,7 @@ struct glimp_framebuffer {
     std::vector<DrawEntry> draws;
     DrawEntry* d_draws = nullptr;
     size_t d_draws_cap = 0;
+    std::vector<DrawEntry> uploaded;   // contents of d_draws
     uint32_t* color = nullptr;         // resolved RGBA8, row-major
 };
 
@@ -171,7 +173,7 @@ int glimp_create(glimp_context** out, int device) {
     if (!c) return fail(GLIMP_E_NOMEM, "glimp_create: out of host memory");
     c->device = device;
     int rc = GLIMP_OK;
-    hipError_t e = upload(&c->tables, kApproxTables, 131072);
+    hipError_t e = upload(&c->tables, kApproxPwl, 256);
     if (e == hipSuccess) e = dev_alloc(&c->counters, 1);
     if (e == hipSuccess) e = hipMemset(c->counters, 0, sizeof(Counters));
     if (e == hipSuccess) e = dev_alloc(&c->scratch, 1);
@@ -401,9 +403,16 @@ int glimp_framebuffer_keys(glimp_framebuffer* f, void** keys, size_t* bytes) {
 namespace {
 
 // Uploads the framebuffer's draw table (seq ranges -> meshlet offsets).
+// Unchanged tables are not uploaded again, so a steady clear-draw-resolve
+// loop never waits for the device here.
 int upload_draws(glimp_framebuffer* f, hipStream_t s, uint32_t* count) {
     std::vector<DrawEntry> draws = f->draws;
     if (draws.empty()) draws.push_back(DrawEntry{ 0, 0, 0, 0 });
+    *count = (uint32_t)draws.size();
+    if (f->d_draws && draws.size() == f->uploaded.size() &&
+        memcmp(draws.data(), f->uploaded.data(), draws.size() * sizeof(DrawEntry)) == 0)
+        return GLIMP_OK;
+    f->uploaded.clear();
     if (draws.size() > f->d_draws_cap) {
         if (f->d_draws) (void)hipFree(f->d_draws);
         f->d_draws = nullptr;
@@ -414,7 +423,7 @@ int upload_draws(glimp_framebuffer* f, hipStream_t s, uint32_t* count) {
     HIP_TRY(hipMemcpyAsync(f->d_draws, draws.data(), draws.size() * sizeof(DrawEntry), hipMemcpyHostToDevice, s));
     // the host vector must outlive the async copy
     HIP_TRY(hipStreamSynchronize(s));
-    *count = (uint32_t)draws.size();
+    f->uploaded = draws;
     return GLIMP_OK;
 }
 

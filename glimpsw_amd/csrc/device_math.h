@@ -9,8 +9,10 @@
 //  - FMA contraction choices of the compiler.
 // The two approximation instructions are table driven: a 16-bit mantissa
 // looked up by the top 16 (rcp) or 15 (rsqrt) bits of the input mantissa
-// reproduces them over all 2^32 inputs. The tables live in device memory
-// (128 KiB each, hot entries stay in L2/L1).
+// reproduces them over all 2^32 inputs. Those tables are exactly piecewise
+// linear over runs of 1024 entries, so the device evaluates them from 64 + 64
+// integer pairs (A, B): entry j of run i is (A_i - B_i * j) >> K. The 1 KiB
+// of pairs stays in the L1 cache.
 //
 // Device code is compiled with -ffp-contract=off: every FMA here is written
 // out with __builtin_fmaf, every other operation rounds on its own.
@@ -22,14 +24,30 @@
 namespace glimp {
 
 struct ApproxTables {
-    const uint16_t* rcp;      // [65536]
-    const uint16_t* rsqrt;    // [2][32768]
+    const uint2* pwl;         // [128]: 64 rcp runs, then 32 + 32 rsqrt runs (exponent parity 0, 1)
 };
+
+// 16-bit result mantissa of VRCP14PS for the top 16 input mantissa bits.
+__device__ __forceinline__ uint32_t rcp14_mant(const ApproxTables& T, uint32_t idx16) {
+    const uint2 ab = T.pwl[idx16 >> 10];
+    // B < 2^14 and j < 2^10: a full-rate 24-bit multiply is exact
+    return (uint32_t)((int32_t)ab.x - (int32_t)__umul24(ab.y, idx16 & 1023u)) >> 12;
+}
+
+// 16-bit result mantissa of VRSQRT14PS for exponent parity `odd` and the
+// top 15 input mantissa bits.
+__device__ __forceinline__ uint32_t rsqrt14_mant(const ApproxTables& T, uint32_t odd, uint32_t idx15) {
+    const uint2 ab = T.pwl[64 + odd * 32 + (idx15 >> 10)];
+    // B < 2^17 and j < 2^10: a full-rate 24-bit multiply is exact
+    return (ab.x - __umul24(ab.y, idx15 & 1023u)) >> 16;
+}
 
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float bitsf(uint32_t u) { return __uint_as_float(u); }
 
-__device__ __forceinline__ float rcp14(const ApproxTables& T, float xf) {
+// Full VRCP14PS semantics (zeros, denormals, infinities, NaN, results that
+// overflow or underflow); rcp14 only calls it off the fast path.
+__device__ __forceinline__ float rcp14_special(const ApproxTables& T, float xf) {
     uint32_t x = fbits(xf);
     uint32_t s = x & 0x80000000u;
     int e = (int)((x >> 23) & 255);
@@ -49,7 +67,7 @@ __device__ __forceinline__ float rcp14(const ApproxTables& T, float xf) {
         mant = 0;
     } else {
         re = 253 - e;
-        mant = (uint32_t)T.rcp[m >> 7] << 7;
+        mant = rcp14_mant(T, m >> 7) << 7;
     }
     if (re >= 255) return bitsf(s | 0x7F800000u);
     if (re <= 0) {
@@ -60,7 +78,21 @@ __device__ __forceinline__ float rcp14(const ApproxTables& T, float xf) {
     return bitsf(s | ((uint32_t)re << 23) | mant);
 }
 
-__device__ __forceinline__ float rsqrt14(const ApproxTables& T, float xf) {
+// VRCP14PS. Inputs with a biased exponent in [1, 252] have a normal result
+// (exponent 253 - e, or 254 - e for an exact power of two); everything else
+// takes the general path.
+__device__ __forceinline__ float rcp14(const ApproxTables& T, float xf) {
+    const uint32_t x = fbits(xf);
+    const uint32_t e = (x >> 23) & 255u;
+    if (__builtin_expect(e - 1u >= 252u, 0)) return rcp14_special(T, xf);
+    const uint32_t m = x & 0x7FFFFFu;
+    const uint32_t mant = m ? rcp14_mant(T, m >> 7) << 7 : 0u;
+    const uint32_t re = (m ? 253u : 254u) - e;
+    return bitsf((x & 0x80000000u) | (re << 23) | mant);
+}
+
+// Full VRSQRT14PS semantics; rsqrt14 only calls it off the fast path.
+__device__ __forceinline__ float rsqrt14_special(const ApproxTables& T, float xf) {
     uint32_t x = fbits(xf);
     uint32_t s = x & 0x80000000u;
     int e = (int)((x >> 23) & 255);
@@ -81,7 +113,20 @@ __device__ __forceinline__ float rsqrt14(const ApproxTables& T, float xf) {
     int oc = ue & 1;
     int half = (ue - oc) / 2;
     if (m == 0 && oc == 0) return bitsf((uint32_t)(127 - half) << 23);
-    return bitsf(((uint32_t)(126 - half) << 23) | ((uint32_t)T.rsqrt[oc * 32768 + (m >> 8)] << 7));
+    return bitsf(((uint32_t)(126 - half) << 23) | (rsqrt14_mant(T, (uint32_t)oc, m >> 8) << 7));
+}
+
+// VRSQRT14PS. Positive normal inputs take the fast path.
+__device__ __forceinline__ float rsqrt14(const ApproxTables& T, float xf) {
+    const uint32_t x = fbits(xf);
+    const uint32_t e = x >> 23;                     // sign bit set -> >= 256
+    if (__builtin_expect(e - 1u >= 254u, 0)) return rsqrt14_special(T, xf);
+    const uint32_t m = x & 0x7FFFFFu;
+    const int ue = (int)e - 127;
+    const uint32_t oc = (uint32_t)ue & 1u;
+    const int half = (ue - (int)oc) >> 1;           // exact: ue - oc is even
+    if (m == 0 && oc == 0) return bitsf((uint32_t)(127 - half) << 23);
+    return bitsf(((uint32_t)(126 - half) << 23) | (rsqrt14_mant(T, oc, m >> 8) << 7));
 }
 
 // Vector a/b as the reference's fast-math build computes it:
@@ -113,15 +158,20 @@ __device__ __forceinline__ float xform(const float* m, int c, float x, float y, 
 }
 
 // IEEE binary16 -> binary32, exact (what VCVTPH2PS computes).
+// n / d and n % d for n < 2^24, 1 <= d < 2^24: a float estimate within one
+// of the quotient, then one correction step (cheaper than an integer divide).
+__device__ __forceinline__ uint32_t div_small(uint32_t n, uint32_t d, uint32_t& rem) {
+    uint32_t q = (uint32_t)((float)n * __builtin_amdgcn_rcpf((float)d));
+    int32_t r = (int32_t)n - (int32_t)__umul24(q, d);
+    if (r < 0) { q--; r += (int32_t)d; }
+    else if (r >= (int32_t)d) { q++; r -= (int32_t)d; }
+    rem = (uint32_t)r;
+    return q;
+}
+
+// Half to float (exact, like VCVTPH2PS).
 __device__ __forceinline__ float half_to_float(uint32_t h) {
-    uint32_t s = (h & 0x8000u) << 16;
-    uint32_t e = (h >> 10) & 31u, m = h & 1023u;
-    if (e == 0) {
-        float f = (float)m * (1.0f / 16777216.0f);
-        return s ? -f : f;
-    }
-    if (e == 31) return bitsf(s | 0x7F800000u | (m << 13));
-    return bitsf(s | ((e + 112u) << 23) | (m << 13));
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
 }
 
 }  // namespace glimp

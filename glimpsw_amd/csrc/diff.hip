@@ -75,8 +75,10 @@ __device__ __forceinline__ PixelSlot pixel_slot(const DiffParams& p, uint32_t t)
     s.tiled = t - (s.inside ? V.pixel_base : t);
     const uint32_t W = (uint32_t)V.width;
     const uint32_t tile = s.tiled >> 4, q = s.tiled & 15u, tilesX = W / 4;
-    s.x = (tile % tilesX) * 4 + (q & 3u);
-    s.y = (tile / tilesX) * 4 + (q >> 2);
+    uint32_t tx;
+    const uint32_t ty = div_small(tile, tilesX, tx);
+    s.x = tx * 4 + (q & 3u);
+    s.y = ty * 4 + (q >> 2);
     return s;
 }
 
@@ -297,9 +299,12 @@ __global__ void __launch_bounds__(256) k_diff_records(DiffParams p) {
         const float4 g = p.grad_out[out_index(p, ps)];
         const float4 col = vertex_color(p, S);
         const float4 gval = g * col;
-        for (int k = 0; k < E.n; k++) {
-            tkeys[k] = (uint32_t)E.texel[k];
-            tvals[k] = E.w[k] * gval;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (k < E.n) {
+                tkeys[k] = (uint32_t)E.texel[k];
+                tvals[k] = E.w[k] * gval;
+            }
         }
         const float gU = dot4(gval, E.du), gV = dot4(gval, E.dv);
         const float4 gcol = g * E.val;

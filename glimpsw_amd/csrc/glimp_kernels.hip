@@ -30,7 +30,6 @@ namespace glimp {
 namespace {
 
 constexpr int kWave = 64;
-constexpr uint32_t kXcds = 8;                // MI355X: 8 XCDs of 32 CUs
 constexpr uint32_t kPersistentBlocks = 2048; // looping grids: 8 blocks of 256 threads per CU
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -567,10 +566,17 @@ __device__ __forceinline__ void load_tri(const TriRecord& rec, int hw, int hh, T
 
 // Coverage and depth of pixel (x, y) for triangle T (DrawTriangle,
 // Rasterizer.h:281-296). Returns false when the pixel is not covered.
+// Edge steps are differences of s16 coordinates (|A|, |B| < 2^17) and pixel
+// coordinates are below 2^12, so full-rate 24-bit multiplies give the exact
+// products; the sums wrap in 32 bits like the reference's.
+__device__ __forceinline__ int32_t edge_at(int32_t e, int32_t a, int32_t b, uint32_t x, uint32_t y) {
+    return (int32_t)((uint32_t)e + (uint32_t)__mul24(a, (int32_t)x) + (uint32_t)__mul24(b, (int32_t)y));
+}
+
 __device__ __forceinline__ bool shade_pixel(const TriLds& T, uint32_t x, uint32_t y, float& depth) {
-    int32_t w0 = (int32_t)((uint32_t)T.e0 + (uint32_t)T.a12 * x + (uint32_t)T.b12 * y);
-    int32_t w1 = (int32_t)((uint32_t)T.e1 + (uint32_t)T.a20 * x + (uint32_t)T.b20 * y);
-    int32_t w2 = (int32_t)((uint32_t)T.e2 + (uint32_t)T.a01 * x + (uint32_t)T.b01 * y);
+    int32_t w0 = edge_at(T.e0, T.a12, T.b12, x, y);
+    int32_t w1 = edge_at(T.e1, T.a20, T.b20, x, y);
+    int32_t w2 = edge_at(T.e2, T.a01, T.b01, x, y);
     if ((w0 | w1 | w2) < 0) return false;
     depth = __builtin_fmaf((float)w1, T.z10, __builtin_fmaf((float)w2, T.z20, T.z0));
     return true;
@@ -668,6 +674,8 @@ __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p, int use_spi
 
 // ---------------------------------------------------------------- binning
 
+constexpr uint32_t kSortedBinningPairs = 128;
+
 // Ascending sort of one u32 per lane across the wave (bitonic network).
 __device__ __forceinline__ uint32_t wave_sort_u32(uint32_t v, uint32_t lane) {
 #pragma unroll
@@ -742,9 +750,11 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
                 const uint32_t dy = q / w, dx = q - dy * w;
                 item = ((s_org[wid][k] + dy * pitch_k + dx) << 6) | k;
             }
-            // A meshlet's triangles share few bins: sort the wave's pairs by
-            // bin so each run of equal bins takes its slots with one atomic.
-            item = wave_sort_u32(item, lane);
+            // Small triangles of one meshlet share few bins: sort the wave's
+            // pairs by bin so each run of equal bins takes its slots with one
+            // atomic. Chunks of large triangles (many pairs) have mostly
+            // distinct bins and skip the sort. `total` is wave-uniform.
+            if (total <= kSortedBinningPairs) item = wave_sort_u32(item, lane);
             const uint32_t bin = item >> 6;
             const bool valid = item != 0xFFFFFFFFu;
             const uint32_t prev = __shfl_up(item, 1);
@@ -790,10 +800,10 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
 // time. After a clear the bin starts from the clear key, otherwise from the
 // vis-buffer's current contents.
 //
-// One block per bin. Workgroups go to the 8 XCDs round-robin (block b runs
-// on XCD b % 8), so block b takes the (b/8)-th bin of the (b%8)-th eighth of
-// the bin range: neighbouring bins, which share records, are rasterized
-// under the same L2.
+// One block per bin, in bin order. (Keeping neighbouring bins on one XCD's
+// L2 by remapping block b to the (b/8)-th bin of the (b%8)-th eighth of the
+// range measured 4% slower: consecutive blocks then run far-apart bins, and
+// the records they share were already in the MALL.)
 template <int BS, int kThreads>
 __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long (&keys)[BS * BS],
                            TriLds (&tl)[kThreads], uint32_t (&s_wave_total)[kThreads / kWave]) {
@@ -925,10 +935,7 @@ __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
     __shared__ unsigned long long keys[BS * BS];
     __shared__ TriLds tl[kThreads];
     __shared__ uint32_t s_wave_total[kThreads / kWave];
-    const uint32_t q = p.total_bins / kXcds, r = p.total_bins % kXcds;
-    const uint32_t xcd = blockIdx.x % kXcds;
-    const uint32_t bin = xcd * q + min(xcd, r) + blockIdx.x / kXcds;
-    raster_bin<BS, kThreads>(p, bin, keys, tl, s_wave_total);
+    raster_bin<BS, kThreads>(p, blockIdx.x, keys, tl, s_wave_total);
 }
 
 __global__ void k_begin_draw(DrawScratch* s, uint32_t* bin_counts, uint32_t num_bins) {

@@ -54,7 +54,7 @@ __device__ __forceinline__ uint32_t lerp16(uint32_t a, uint32_t b, uint32_t t) {
     for (int h = 0; h < 2; h++) {
         int32_t av = (int16_t)(a >> (16 * h)), bv = (int16_t)(b >> (16 * h)), tv = (int16_t)(t >> (16 * h));
         int32_t d = (int16_t)(bv - av);
-        int32_t prod = d * tv;
+        int32_t prod = __mul24(d, tv);   // 16-bit operands: exact at full rate
         int32_t mh = (int16_t)(((prod >> 14) + 1) >> 1);
         r |= (uint32_t)(uint16_t)(int16_t)(av + mh) << (16 * h);
     }
@@ -151,7 +151,8 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
     const uint32_t W = (uint32_t)p.width, H = (uint32_t)p.height;
     const uint32_t tilesX = W / 4;
     const uint32_t tile = t >> 4, q = t & 15u;
-    const uint32_t tx = tile % tilesX, ty = tile / tilesX;
+    uint32_t tx;
+    const uint32_t ty = div_small(tile, tilesX, tx);
     const bool inside = ty * 4 < H;
     const uint32_t x = tx * 4 + (q & 3u), y = ty * 4 + (q >> 2);
     const uint32_t lane = lane_id();
@@ -196,8 +197,13 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
         (void)uv0;
     }
 
-    // material waterfall per 4x4 tile (Shading.cpp:532-545)
+    // material waterfall per 4x4 tile (Shading.cpp:532-545): each pass
+    // settles one material of every tile (its texture, the pixel's mip level
+    // and the tile's filter vote); the texture is sampled once afterwards
     uint32_t albedo = 0;
+    const DevTexture* myTex = nullptr;
+    int32_t myMip = 0;
+    bool myLinear = false;
     bool done = !active || material == 0xFFFFFFFFu || material >= p.num_materials;
     for (;;) {
         const unsigned long long todo = __ballot(!done);
@@ -208,10 +214,11 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
         int32_t texIdx = -1;
         if (mine) texIdx = p.materials[lmat].texture;
         int32_t mip = -64;
-        DevTexture tex;
-        if (texIdx >= 0) {
-            tex = p.textures[texIdx];
-            float ssu = tex.scale_u * tex.scale_u, ssv = tex.scale_v * tex.scale_v;
+        // a reference into the table, not a local copy: a copy indexed by the
+        // mip level would live in scratch memory
+        const DevTexture* tex = texIdx >= 0 ? &p.textures[texIdx] : nullptr;
+        if (tex) {
+            float ssu = tex->scale_u * tex->scale_u, ssv = tex->scale_v * tex->scale_v;
             float dx = __builtin_fmaf(gradU[0], gradU[0], gradV[0] * gradV[0]) * ssu;
             float dy = __builtin_fmaf(gradU[1], gradU[1], gradV[1] * gradV[1]) * ssv;
             float mx = (dx > dy) ? dx : dy;
@@ -220,11 +227,14 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
         const unsigned long long vote = __ballot(active && mip > 0);
         const bool minify = (vote & groupMask) != 0;
         if (!done && material == lmat) {
-            // SurfaceSampler: Repeat, Mag Linear, Min Nearest (Shading.cpp:6-10)
-            if (texIdx >= 0) albedo = sample_level(tex, U, V, mip, !minify);
+            myTex = tex;
+            myMip = mip;
+            myLinear = !minify;
             done = true;
         }
     }
+    // SurfaceSampler: Repeat, Mag Linear, Min Nearest (Shading.cpp:6-10)
+    if (myTex) albedo = sample_level(*myTex, U, V, myMip, myLinear);
 
     uint32_t outc;
     if (p.mode == 0) {
