@@ -228,6 +228,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batching measurement")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM traffic passes")
     ap.add_argument("--no-counts", action="store_true", help="skip the counting pass (frame kernels only)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the 64-view batching measurement (its JSON object)")
     ap.add_argument("--frame-calls", action="store_true",
                     help="time the frame as separate clear/draw/resolve calls instead of a graph replay")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -249,6 +251,16 @@ def main() -> int:
                                    "(views or meshlets would shard without exchange except a min-reduce "
                                    "of vis-buffer keys)", "n_gpus": args.gpus}))
         return 2
+
+    if args.batch_only:
+        import torch
+        import glimpsw_amd as g
+
+        torch.cuda.init()
+        ctx = g.Context(0)
+        print(json.dumps(bench_batch(ctx, torch.cuda.current_stream(), max(5, args.steps // 5), 3)))
+        ctx.close()
+        return 0
 
     # HBM traffic from counter passes in child processes, before this
     # process touches the GPU

@@ -180,6 +180,7 @@ int glimp_create(glimp_context** out, int device) {
     if (e == hipSuccess) e = hipMemset(c->scratch, 0, sizeof(DrawScratch));
     if (e == hipSuccess) e = dev_alloc(&c->visible, 1);
     if (e != hipSuccess) rc = fail(GLIMP_E_HIP, std::string("glimp_create: ") + hipGetErrorString(e));
+    if (rc == GLIMP_OK) warm_launch_shapes();
     if (rc != GLIMP_OK) {
         glimp_destroy(c);
         return rc;

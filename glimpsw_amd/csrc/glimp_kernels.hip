@@ -23,6 +23,8 @@
 // result does not depend on the order the GPU produces fragments in.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "internal.h"
 
 namespace glimp {
@@ -30,7 +32,25 @@ namespace glimp {
 namespace {
 
 constexpr int kWave = 64;
-constexpr uint32_t kPersistentBlocks = 2048; // looping grids: 8 blocks of 256 threads per CU
+constexpr uint32_t kPersistentBlocks = 2048; // looping grids: at most 8 blocks of 256 threads per CU
+
+// Blocks of a looping kernel that are resident at once on this device (the
+// grid of a kernel whose waves loop over a work list; more blocks would only
+// queue behind the first ones and finish last).
+template <auto Kernel, uint32_t Threads>
+uint32_t resident_blocks() {
+    static uint32_t cached = 0;     // one per kernel (devices of one kind assumed)
+    if (!cached) {
+        int per_cu = 0, dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, Kernel, (int)Threads, 0) != hipSuccess ||
+            per_cu <= 0 || cus <= 0)
+            return kPersistentBlocks;
+        cached = std::min<uint32_t>((uint32_t)(per_cu * cus), kPersistentBlocks);
+    }
+    return cached;
+}
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -266,32 +286,57 @@ __global__ void k_cull_range(DevMeshlets m, uint32_t first, uint32_t count, Plan
 // slots. Setup needs no global counter: a wave compacts its survivors with a
 // ballot and writes them into its own slots. Waves loop over the visible
 // list, so the grid is sized to the machine, not to the scene.
-__device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, float (&lds)[4][64]) {
+// Inputs of one meshlet for a wave (lane = vertex), loaded one meshlet
+// ahead of its setup so the loads overlap the previous meshlet's work.
+struct MeshletFetch {
+    uint32_t g, v, mi, nt, mat;
+    float px, py, pz;
+    uint32_t idx[2];               // this lane's triangle of each round: v0 | v1 << 8 | v2 << 16
+};
+
+__device__ __forceinline__ MeshletFetch fetch_meshlet(const DrawParams& p, uint32_t i, uint32_t lane) {
+    MeshletFetch f;
     const uint2 gv = p.visible_list[i];
-    const uint32_t g = gv.x, v = gv.y;         // global meshlet number, view
+    f.g = gv.x;
+    f.v = gv.y;
+    const ViewParams& V = view_of(p, f.v);
+    f.mi = V.meshlet_offset + (f.g - V.meshlet_base);
+    f.nt = V.m.num_triangles[f.mi];
+    f.mat = V.m.material_id[f.mi];
+    const float* P = V.m.positions + (size_t)f.mi * 192;
+    f.px = P[lane];
+    f.py = P[64 + lane];
+    f.pz = P[128 + lane];
+    const uint8_t* I = V.m.indices + (size_t)f.mi * 384;
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+        f.idx[r] = (uint32_t)I[r * 64 + lane] | ((uint32_t)I[128 + r * 64 + lane] << 8) |
+                   ((uint32_t)I[256 + r * 64 + lane] << 16);
+    return f;
+}
+
+__device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, float (&lds)[4][64],
+                              const MeshletFetch& f) {
+    const uint32_t g = f.g, v = f.v;           // global meshlet number, view
     const ViewParams& V = view_of(p, v);
     const uint32_t w = g - V.meshlet_base;     // meshlet index within the view
-    const uint32_t mi = V.meshlet_offset + w;
     const int hw = V.width / 2, hh = V.height / 2;
 
-    const uint32_t nt = V.m.num_triangles[mi];
+    const uint32_t nt = f.nt;
     int cull_mode = 1, fs_id = 0;
     {
-        uint32_t mat = V.m.material_id[mi];
+        const uint32_t mat = f.mat;
         if (mat != 0xFFFFFFFFu && mat < V.num_materials) {
             cull_mode = V.materials[mat].cull_mode;
             fs_id = V.materials[mat].fs_id;
         }
         // ShadeMeshlet (Shading.cpp:295-298): lane = vertex
-        const float* P = V.m.positions + (size_t)mi * 192;
-        float px = P[lane], py = P[64 + lane], pz = P[128 + lane];
         __builtin_amdgcn_wave_barrier();   // the previous meshlet's reads are done
 #pragma unroll
-        for (int c = 0; c < 4; c++) lds[c][lane] = xform(V.clip, c, px, py, pz);
+        for (int c = 0; c < 4; c++) lds[c][lane] = xform(V.clip, c, f.px, f.py, f.pz);
         __builtin_amdgcn_wave_barrier();
     }
     const uint32_t seq = V.seq_base + w;
-    const uint8_t* I = V.m.indices + (size_t)mi * 384;
     TriRecord* out = p.records + (size_t)i * 128;
     const uint32_t flags_hi = v << 16;
     uint32_t kept = 0, clipped = 0;
@@ -308,7 +353,7 @@ __device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, fl
             bool trivial = true;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                uint32_t ix = I[k * 128 + t];
+                uint32_t ix = (f.idx[round] >> (8 * k)) & 255u;
                 uint32_t vi = ix < 32 ? ix : 32 + (ix & 31);   // gather_preload64 (SIMD.h:219-227)
 #pragma unroll
                 for (int c = 0; c < 4; c++) vv[k][c] = lds[c][vi];
@@ -381,7 +426,18 @@ __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
     const uint32_t lane = lane_id();
     const uint32_t nvis = p.scratch->num_visible;
     const uint32_t waves = gridDim.x * 4;
-    for (uint32_t i = blockIdx.x * 4 + wid; i < nvis; i += waves) setup_meshlet(p, i, lane, lds_pos[wid]);
+    uint32_t i = blockIdx.x * 4 + wid;
+    if (i >= nvis) return;
+    MeshletFetch cur = fetch_meshlet(p, i, lane);
+    for (;;) {
+        const uint32_t next = i + waves;
+        MeshletFetch nxt;
+        if (next < nvis) nxt = fetch_meshlet(p, next, lane);
+        setup_meshlet(p, i, lane, lds_pos[wid], cur);
+        if (next >= nvis) break;
+        cur = nxt;
+        i = next;
+    }
 }
 
 // ---------------------------------------------------------------- clipper
@@ -1001,7 +1057,8 @@ void launch_front(const DrawParams& p, hipStream_t s, Profiler* prof) {
     }
     if (p.total_meshlets) {
         Profiler::Scope sc(prof, "mesh_setup", s);
-        hipLaunchKernelGGL(k_mesh_setup, dim3(min((p.total_meshlets + 3) / 4, kPersistentBlocks)), dim3(256), 0, s,
+        hipLaunchKernelGGL(k_mesh_setup, dim3(min((p.total_meshlets + 3) / 4, resident_blocks<k_mesh_setup, 256>())),
+                           dim3(256), 0, s,
                            p);
     }
     {
@@ -1011,6 +1068,12 @@ void launch_front(const DrawParams& p, hipStream_t s, Profiler* prof) {
 }
 
 }  // namespace
+
+void warm_launch_shapes() {
+    (void)resident_blocks<k_mesh_setup, 256>();
+    (void)resident_blocks<k_raster_direct, 256>();
+    (void)resident_blocks<k_bin_scatter, 256>();
+}
 
 hipError_t launch_clear(unsigned long long* keys, uint32_t n, unsigned long long key, hipStream_t s) {
     uint32_t blocks = min((n + 255) / 256, 8192u);
@@ -1039,7 +1102,7 @@ hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s, Profiler* prof
     launch_front(p, s, prof);
     {
         Profiler::Scope sc(prof, "raster_direct", s);
-        uint32_t blocks = min(max((2 * p.total_meshlets + 3) / 4 + 16, 64u), kPersistentBlocks);
+        uint32_t blocks = min(max((2 * p.total_meshlets + 3) / 4 + 16, 64u), resident_blocks<k_raster_direct, 256>());
         hipLaunchKernelGGL(k_raster_direct, dim3(blocks), dim3(256), 0, s, p, 0);
     }
     return hipGetLastError();
@@ -1050,7 +1113,7 @@ hipError_t launch_draw_binned(const DrawParams& p, hipStream_t s, Profiler* prof
     {
         Profiler::Scope sc(prof, "bin_scatter", s);
         // about one chunk per wave: two per meshlet plus room for clipped records
-        uint32_t blocks = min(max((2 * p.total_meshlets + 3) / 4 + 16, 64u), kPersistentBlocks);
+        uint32_t blocks = min(max((2 * p.total_meshlets + 3) / 4 + 16, 64u), resident_blocks<k_bin_scatter, 256>());
         hipLaunchKernelGGL(k_bin_scatter, dim3(blocks), dim3(256), 0, s, p);
     }
     {

@@ -162,6 +162,9 @@ struct DrawParams {
 
 // Launchers (glimp_kernels.hip). `max_meshlets` bounds the meshlets of
 // the largest view (grid sizing only).
+// Queries the resident block counts of the looping kernels (call once per
+// process on the device, outside any stream capture).
+void warm_launch_shapes();
 hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s, Profiler* prof);
 hipError_t launch_draw_binned(const DrawParams& p, hipStream_t s, Profiler* prof);
 hipError_t launch_clear(unsigned long long* keys, uint32_t n, unsigned long long key, hipStream_t s);
