@@ -942,26 +942,65 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
             uint32_t next = (k + 1 < (uint32_t)kThreads) ? tl[k + 1].start : 0xFFFFFFFFu;
             uint32_t pt = j - T.start;
             uint32_t ty = pt / T.tw, tx = pt - ty * T.tw;
+            // Edge values and the LDS key index of this lane's pixel, stepped
+            // from tile to tile (DrawTriangle's stepping, Rasterizer.h:268-
+            // 296): +4A per tile in x; at the end of a row back to the row
+            // start plus 4B. Sums wrap in 32 bits, as a direct evaluation does.
+            uint32_t w0, w1, w2, r0, r1, r2, li, lrow;
+            auto enter = [&]() {
+                const uint32_t lx = (T.org & 0xFFFFu) + px, ly = (T.org >> 16) + ty * 4 + py;
+                r0 = (uint32_t)edge_at(T.e0, T.a12, T.b12, x0 + lx, y0 + ly);
+                r1 = (uint32_t)edge_at(T.e1, T.a20, T.b20, x0 + lx, y0 + ly);
+                r2 = (uint32_t)edge_at(T.e2, T.a01, T.b01, x0 + lx, y0 + ly);
+                w0 = r0 + (uint32_t)__mul24(T.a12, (int32_t)(4 * tx));
+                w1 = r1 + (uint32_t)__mul24(T.a20, (int32_t)(4 * tx));
+                w2 = r2 + (uint32_t)__mul24(T.a01, (int32_t)(4 * tx));
+                // (lx, ly) of the row start in the bin's 4x4-tiled LDS layout;
+                // the triangle's render box starts on a tile corner
+                lrow = ((lx & ~3u) << 2) + (ly & ~3u) * BS + px + (py << 2);
+                li = lrow + tx * 16;
+            };
+            enter();
             for (; j < j1; j++) {
-                while (j >= next) {   // step to the triangle owning tile j
-                    k++;
+                if (j >= next) {      // step to the triangle owning tile j
+                    do {
+                        k++;
+                        next = (k + 1 < (uint32_t)kThreads) ? tl[k + 1].start : 0xFFFFFFFFu;
+                    } while (j >= next);
                     T = tl[k];
-                    next = (k + 1 < (uint32_t)kThreads) ? tl[k + 1].start : 0xFFFFFFFFu;
                     tx = ty = 0;
+                    enter();
                 }
-                const uint32_t lx = (T.org & 0xFFFFu) + tx * 4 + px;
-                const uint32_t ly = (T.org >> 16) + ty * 4 + py;
-                float depth;
-                if (shade_pixel(T, x0 + lx, y0 + ly, depth)) {
+                // DrawTriangle's coverage test and depth (Rasterizer.h:281-296)
+                if ((int32_t)(w0 | w1 | w2) >= 0) {
+                    const float depth = __builtin_fmaf((float)(int32_t)w1, T.z10,
+                                                       __builtin_fmaf((float)(int32_t)w2, T.z20, T.z0));
                     covered++;
                     if (depth > V.clear_depth) {
                         passed++;
-                        unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
-                        const uint32_t li = ((lx & ~3u) << 2) + (ly & ~3u) * BS + (lx & 3u) + ((ly & 3u) << 2);
-                        if (key > keys[li]) atomicMax(&keys[li], key);
+                        // an LDS max without a returned value: no wait on the
+                        // old key (a read-then-max would stall every step)
+                        const unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
+                        atomicMax(&keys[li], key);
                     }
                 }
-                if (++tx == T.tw) { tx = 0; ty++; }
+                if (++tx == T.tw) {
+                    tx = 0;
+                    ty++;
+                    r0 += (uint32_t)T.b12 << 2;
+                    r1 += (uint32_t)T.b20 << 2;
+                    r2 += (uint32_t)T.b01 << 2;
+                    w0 = r0;
+                    w1 = r1;
+                    w2 = r2;
+                    lrow += 4 * BS;
+                    li = lrow;
+                } else {
+                    w0 += (uint32_t)T.a12 << 2;
+                    w1 += (uint32_t)T.a20 << 2;
+                    w2 += (uint32_t)T.a01 << 2;
+                    li += 16;
+                }
             }
         }
         __syncthreads();
