@@ -51,8 +51,9 @@ def algorithmic_bytes(stats: dict) -> dict:
         "clip": 64 * Cq,
         # records in, covered fragments update one 8-byte key each
         "raster_direct": 32 * S + 8 * F,
-        # records in, one 4-byte entry per (record, bin) pair out
-        "bin_scatter": 32 * S + 4 * P,
+        # records in, one 4-byte entry per (record, bin) pair out; the
+        # queued clip triangles in (the binned pipeline clips here)
+        "bin_scatter": 32 * S + 4 * P + 64 * Cq,
         # entries and records in, every pixel's key written once
         "bin_raster": 4 * P + 32 * S + 8 * N,
         "raster_spill": 0,

@@ -459,11 +459,11 @@ __device__ __forceinline__ float lane_get(float v, uint32_t src) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute((int)(src << 2), __float_as_int(v)));
 }
 
-__global__ void __launch_bounds__(256) k_clip(DrawParams p) {
-    const uint32_t n = min(p.scratch->num_clip, p.clip_capacity);
-    const uint32_t lane = lane_id();
-    const uint32_t waves = gridDim.x * (blockDim.x / kWave);
-    for (uint32_t q = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; q < n; q += waves) {
+// Clips queued triangle q with the whole wave. Lanes whose fan triangle
+// survives setup return true with the record and its index; the records
+// are also stored after the meshlet slots.
+__device__ bool clip_one(const DrawParams& p, uint32_t q, uint32_t lane, TriRecord& out, uint32_t& out_index) {
+    {
         const ClipEntry& ce = p.clip_queue[q];
         const ViewParams& VP = view_of(p, ce.view);
         const int hw = VP.width / 2, hh = VP.height / 2;
@@ -525,7 +525,7 @@ __global__ void __launch_bounds__(256) k_clip(DrawParams p) {
             generated += made;
             vc = outCount;
         }
-        if (vc < 3) continue;
+        if (vc < 3) return false;
         const int cull_mode = (ce.info >> 8) & 3;
         // fan triangle f = (v0, v[f+1], v[f+2]) on lane f
         const uint32_t f = lane;
@@ -544,19 +544,35 @@ __global__ void __launch_bounds__(256) k_clip(DrawParams p) {
             rec.flags = ((ce.info >> 10) & 3u) | (ce.view << 16);
         }
         const unsigned long long km = __ballot(keep);
-        if (km) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(&p.scratch->num_records, (uint32_t)__popcll(km));
-            base = __shfl(base, 0);
-            if (keep) {
-                const uint32_t slot = base + mbcnt(km);
-                if (slot < p.clip_record_capacity) p.records[p.clip_record_base + slot] = rec;
-                else p.scratch->overflow = 1;
-            }
-            if (lane == 0 && counting(p)) atomicAdd(&p.counters->triangles_rasterized, (unsigned long long)__popcll(km));
+        if (!km) return false;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&p.scratch->num_records, (uint32_t)__popcll(km));
+        base = __shfl(base, 0);
+        if (lane == 0 && counting(p)) atomicAdd(&p.counters->triangles_rasterized, (unsigned long long)__popcll(km));
+        if (!keep) return false;
+        const uint32_t slot = base + mbcnt(km);
+        if (slot >= p.clip_record_capacity) {
+            p.scratch->overflow = 1;
+            return false;
         }
+        p.records[p.clip_record_base + slot] = rec;
+        out = rec;
+        out_index = p.clip_record_base + slot;
+        return true;
     }
 }
+
+__global__ void __launch_bounds__(256) k_clip(DrawParams p) {
+    const uint32_t n = min(p.scratch->num_clip, p.clip_capacity);
+    const uint32_t lane = lane_id();
+    const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+    for (uint32_t q = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; q < n; q += waves) {
+        TriRecord rec;
+        uint32_t idx;
+        (void)clip_one(p, q, lane, rec, idx);
+    }
+}
+
 
 // ---------------------------------------------------------------- record chunks
 
@@ -568,8 +584,9 @@ struct Chunk {
     const uint32_t* list;      // non-null: record index = list[base + i]
 };
 
-__device__ __forceinline__ uint32_t num_chunks(const DrawParams& p, int spill) {
+__device__ __forceinline__ uint32_t num_chunks(const DrawParams& p, int spill, bool with_clipped = true) {
     if (spill) return (min(p.scratch->num_spill, p.clip_record_base + p.clip_record_capacity) + 63) / 64;
+    if (!with_clipped) return 2 * p.scratch->num_visible;
     uint32_t nclip = min(p.scratch->num_records, p.clip_record_capacity);
     return 2 * p.scratch->num_visible + (nclip + 63) / 64;
 }
@@ -752,95 +769,118 @@ __device__ __forceinline__ uint32_t wave_sort_u32(uint32_t v, uint32_t lane) {
 // each wave expands a chunk of records into their (record, bin) pairs with
 // an exclusive scan and inserts the pairs in parallel, so a triangle
 // spanning many bins costs one pass of the wave, not a serial loop.
+struct BinScratch {                     // one wave's slice of the block's LDS
+    uint32_t start[64], org[64], bw[64], spill[64], ridx[64];
+};
+
+// Inserts the wave's records (one per lane where `valid`) into the screen
+// bins of their views.
+__device__ void bin_records(const DrawParams& p, bool valid, const TriRecord& rec, uint32_t ridx, uint32_t lane,
+                            BinScratch& sm, unsigned long long& entries) {
+    uint32_t nb = 0, org = 0, bw = 0, pitch = 0;
+    if (valid) {
+        const ViewParams& V = view_of(p, rec.flags >> 16);
+        uint32_t bmin, bmax;
+        render_bbox(rec.pos[0], rec.pos[1], rec.pos[2], V.width / 2, V.height / 2, bmin, bmax);
+        uint32_t bx0 = (uint32_t)lo16(bmin) >> p.bin_shift, by0 = (uint32_t)hi16(bmin) >> p.bin_shift;
+        uint32_t bx1 = (uint32_t)(lo16(bmax) - 1) >> p.bin_shift, by1 = (uint32_t)(hi16(bmax) - 1) >> p.bin_shift;
+        bw = bx1 - bx0 + 1;
+        nb = bw * (by1 - by0 + 1);
+        // first bin of the record's box, in the global bin numbering
+        org = V.bin_base + by0 * V.bins_x + bx0;
+        pitch = V.bins_x;
+    }
+    uint32_t incl = nb;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t o = __shfl_up(incl, d);
+        if ((int)lane >= d) incl += o;
+    }
+    const uint32_t total = __shfl(incl, 63);
+    entries += total;
+    __builtin_amdgcn_wave_barrier();   // the previous call's reads of `sm` are done
+    sm.start[lane] = valid ? incl - nb : 0xFFFFFFFFu;
+    sm.org[lane] = org;
+    sm.bw[lane] = bw | (pitch << 16);
+    sm.spill[lane] = 0;
+    sm.ridx[lane] = ridx;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t base = 0; base < total; base += 64) {
+        const uint32_t j = base + lane;
+        // (bin << 6 | record lane) of this lane's pair; the sentinel sorts last
+        uint32_t item = 0xFFFFFFFFu;
+        if (j < total) {
+            uint32_t k = 0;
+#pragma unroll
+            for (uint32_t step = 32; step > 0; step >>= 1) {
+                if (sm.start[k + step] <= j) k += step;
+            }
+            const uint32_t q = j - sm.start[k];
+            const uint32_t w = sm.bw[k] & 0xFFFFu, pitch_k = sm.bw[k] >> 16;
+            const uint32_t dy = q / w, dx = q - dy * w;
+            item = ((sm.org[k] + dy * pitch_k + dx) << 6) | k;
+        }
+        // Small triangles of one meshlet share few bins: sort the wave's
+        // pairs by bin so each run of equal bins takes its slots with one
+        // atomic. Chunks of large triangles (many pairs) have mostly
+        // distinct bins and skip the sort. `total` is wave-uniform.
+        if (total <= kSortedBinningPairs) item = wave_sort_u32(item, lane);
+        const uint32_t bin = item >> 6;
+        const bool ok = item != 0xFFFFFFFFu;
+        const uint32_t prev = __shfl_up(item, 1);
+        const bool head = ok && (lane == 0 || (prev >> 6) != bin);
+        const unsigned long long heads = __ballot(head);
+        const uint32_t nvalid = (uint32_t)__popcll(__ballot(ok));
+        const unsigned long long later = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+        const uint32_t runEnd = later ? (uint32_t)__ffsll((long long)later) - 1 : nvalid;
+        uint32_t got = 0;
+        if (head) got = atomicAdd(&p.bin_counts[bin], runEnd - lane);
+        // first lane of this lane's run: highest head at or below it
+        const unsigned long long upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+        const uint32_t runStart = upto ? 63u - (uint32_t)__clzll((long long)upto) : 0u;
+        const uint32_t slot = __shfl(got, (int)runStart) + (lane - runStart);
+        if (ok) {
+            const uint32_t k = item & 63u;
+            if (slot < p.bin_capacity) p.bin_lists[(size_t)bin * p.bin_capacity + slot] = sm.ridx[k];
+            else sm.spill[k] = 1;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    const bool spilled = valid && sm.spill[lane] != 0;
+    unsigned long long spm = __ballot(spilled);
+    if (spm) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&p.scratch->num_spill, (uint32_t)__popcll(spm));
+        base = __shfl(base, 0);
+        if (spilled) p.spill[base + mbcnt(spm)] = ridx;
+    }
+}
+
+// Binning of the meshlet records, then clipping of the queued triangles
+// (one wave each; the fan triangles go straight into the bins from the
+// wave's registers).
 __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
-    __shared__ uint32_t s_start[4][64], s_org[4][64], s_bw[4][64], s_spill[4][64];
+    __shared__ BinScratch s_bins[4];
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
-    const uint32_t nchunks = num_chunks(p, 0);
+    const uint32_t nchunks = num_chunks(p, 0, false);
     const uint32_t waves = gridDim.x * 4;
+    const uint32_t first = blockIdx.x * 4 + wid;
     unsigned long long entries = 0;
-    for (uint32_t c = blockIdx.x * 4 + wid; c < nchunks; c += waves) {
+    for (uint32_t c = first; c < nchunks; c += waves) {
         const Chunk ck = get_chunk(p, 0, c);
         if (ck.count == 0) continue;
         const bool valid = lane < ck.count;
-        const uint32_t r = ck.base + lane;
-        uint32_t nb = 0, org = 0, bw = 0, pitch = 0;
-        if (valid) {
-            const TriRecord rec = p.records[r];
-            const ViewParams& V = view_of(p, rec.flags >> 16);
-            uint32_t bmin, bmax;
-            render_bbox(rec.pos[0], rec.pos[1], rec.pos[2], V.width / 2, V.height / 2, bmin, bmax);
-            uint32_t bx0 = (uint32_t)lo16(bmin) >> p.bin_shift, by0 = (uint32_t)hi16(bmin) >> p.bin_shift;
-            uint32_t bx1 = (uint32_t)(lo16(bmax) - 1) >> p.bin_shift, by1 = (uint32_t)(hi16(bmax) - 1) >> p.bin_shift;
-            bw = bx1 - bx0 + 1;
-            nb = bw * (by1 - by0 + 1);
-            // first bin of the record's box, in the global bin numbering
-            org = V.bin_base + by0 * V.bins_x + bx0;
-            pitch = V.bins_x;
-        }
-        uint32_t incl = nb;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            uint32_t o = __shfl_up(incl, d);
-            if ((int)lane >= d) incl += o;
-        }
-        const uint32_t total = __shfl(incl, 63);
-        entries += total;
-        s_start[wid][lane] = valid ? incl - nb : 0xFFFFFFFFu;
-        s_org[wid][lane] = org;
-        s_bw[wid][lane] = bw | (pitch << 16);
-        s_spill[wid][lane] = 0;
-        __builtin_amdgcn_wave_barrier();
-        for (uint32_t base = 0; base < total; base += 64) {
-            const uint32_t j = base + lane;
-            // (bin << 6 | record lane) of this lane's pair; the sentinel sorts last
-            uint32_t item = 0xFFFFFFFFu;
-            if (j < total) {
-                uint32_t k = 0;
-#pragma unroll
-                for (uint32_t step = 32; step > 0; step >>= 1) {
-                    if (s_start[wid][k + step] <= j) k += step;
-                }
-                const uint32_t q = j - s_start[wid][k];
-                const uint32_t w = s_bw[wid][k] & 0xFFFFu, pitch_k = s_bw[wid][k] >> 16;
-                const uint32_t dy = q / w, dx = q - dy * w;
-                item = ((s_org[wid][k] + dy * pitch_k + dx) << 6) | k;
-            }
-            // Small triangles of one meshlet share few bins: sort the wave's
-            // pairs by bin so each run of equal bins takes its slots with one
-            // atomic. Chunks of large triangles (many pairs) have mostly
-            // distinct bins and skip the sort. `total` is wave-uniform.
-            if (total <= kSortedBinningPairs) item = wave_sort_u32(item, lane);
-            const uint32_t bin = item >> 6;
-            const bool valid = item != 0xFFFFFFFFu;
-            const uint32_t prev = __shfl_up(item, 1);
-            const bool head = valid && (lane == 0 || (prev >> 6) != bin);
-            const unsigned long long heads = __ballot(head);
-            const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
-            const unsigned long long later = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
-            const uint32_t runEnd = later ? (uint32_t)__ffsll((long long)later) - 1 : nvalid;
-            uint32_t got = 0;
-            if (head) got = atomicAdd(&p.bin_counts[bin], runEnd - lane);
-            // first lane of this lane's run: highest head at or below it
-            const unsigned long long upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
-            const uint32_t runStart = upto ? 63u - (uint32_t)__clzll((long long)upto) : 0u;
-            const uint32_t slot = __shfl(got, (int)runStart) + (lane - runStart);
-            if (valid) {
-                const uint32_t k = item & 63u;
-                if (slot < p.bin_capacity) p.bin_lists[(size_t)bin * p.bin_capacity + slot] = ck.base + k;
-                else s_spill[wid][k] = 1;
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        const bool spilled = valid && s_spill[wid][lane] != 0;
-        unsigned long long sm = __ballot(spilled);
-        if (sm) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(&p.scratch->num_spill, (uint32_t)__popcll(sm));
-            base = __shfl(base, 0);
-            if (spilled) p.spill[base + mbcnt(sm)] = r;
-        }
-        __builtin_amdgcn_wave_barrier();
+        TriRecord rec;
+        if (valid) rec = p.records[ck.base + lane];
+        bin_records(p, valid, rec, ck.base + lane, lane, s_bins[wid], entries);
+    }
+    const uint32_t nclip = min(p.scratch->num_clip, p.clip_capacity);
+    for (uint32_t q = first; q < nclip; q += waves) {
+        TriRecord rec;
+        uint32_t ridx = 0;
+        const bool valid = clip_one(p, q, lane, rec, ridx);
+        if (__ballot(valid)) bin_records(p, valid, rec, ridx, lane, s_bins[wid], entries);
     }
     if (counting(p) && lane == 0 && entries) atomicAdd(&p.counters->bin_entries, entries);
 }
@@ -871,9 +911,14 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
     const uint32_t W = (uint32_t)V.width, H = (uint32_t)V.height;
     const uint32_t x0 = bx * BS, y0 = by * BS;
     const uint32_t x1 = min(x0 + BS, W), y1 = min(y0 + BS, H);
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+    // The first batch of list entries is loaded together with the count
+    // (entries past the count are never used; the read stays inside the
+    // bin's list), which takes one dependent load off every bin.
+    const uint32_t* list = p.bin_lists + (size_t)bin * p.bin_capacity;
+    const uint32_t first_entry = tid < p.bin_capacity ? list[tid] : 0u;
     const uint32_t n = min(p.bin_counts[bin], p.bin_capacity);
     if (n == 0 && !V.fresh) return;  // uniform across the block
-    const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
     const int hw = V.width / 2, hh = V.height / 2;
     const unsigned long long clearKey = (unsigned long long)fbits(V.clear_depth) << 32;
     const uint32_t rowElems = (x1 - x0) * 4;   // elements of one 4-row strip of tiles
@@ -895,7 +940,7 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
         uint32_t ntiles = 0;
         TriLds t;
         if (e < n) {
-            const uint32_t r = p.bin_lists[(size_t)bin * p.bin_capacity + e];
+            const uint32_t r = base == 0 ? first_entry : list[e];
             uint32_t bmin, bmax;
             load_tri(p.records[r], hw, hh, t, bmin, bmax);
             uint32_t cx0 = max((uint32_t)lo16(bmin), x0), cy0 = max((uint32_t)hi16(bmin), y0);
@@ -1087,7 +1132,7 @@ __global__ void k_decode(const unsigned long long* keys, int width, int height, 
     if (depth) depth[o] = bitsf(db);
 }
 
-void launch_front(const DrawParams& p, hipStream_t s, Profiler* prof) {
+void launch_front(const DrawParams& p, hipStream_t s, Profiler* prof, bool separate_clip) {
     uint32_t zb = min((p.total_bins + 255) / 256, 64u);
     hipLaunchKernelGGL(k_begin_draw, dim3(max(zb, 1u)), dim3(256), 0, s, p.scratch, p.bin_counts, p.total_bins);
     if (p.total_meshlets) {     // a batch of empty views still clears through the bin raster
@@ -1100,7 +1145,7 @@ void launch_front(const DrawParams& p, hipStream_t s, Profiler* prof) {
                            dim3(256), 0, s,
                            p);
     }
-    {
+    if (separate_clip) {
         Profiler::Scope sc(prof, "clip", s);
         hipLaunchKernelGGL(k_clip, dim3(256), dim3(256), 0, s, p);
     }
@@ -1138,7 +1183,7 @@ hipError_t launch_cull(const DevMeshlets& m, uint32_t first, uint32_t count, con
 }
 
 hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s, Profiler* prof) {
-    launch_front(p, s, prof);
+    launch_front(p, s, prof, true);
     {
         Profiler::Scope sc(prof, "raster_direct", s);
         uint32_t blocks = min(max((2 * p.total_meshlets + 3) / 4 + 16, 64u), resident_blocks<k_raster_direct, 256>());
@@ -1148,7 +1193,7 @@ hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s, Profiler* prof
 }
 
 hipError_t launch_draw_binned(const DrawParams& p, hipStream_t s, Profiler* prof) {
-    launch_front(p, s, prof);
+    launch_front(p, s, prof, false);   // the scatter kernel clips
     {
         Profiler::Scope sc(prof, "bin_scatter", s);
         // about one chunk per wave: two per meshlet plus room for clipped records
