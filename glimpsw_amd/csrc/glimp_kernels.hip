@@ -197,10 +197,12 @@ __device__ __forceinline__ bool counting(const DrawParams& p) { return (p.flags 
 // meshlet numbering. Survivors are compacted into visible_list so later
 // stages launch work only for them: a block covers kCullPerBlock consecutive
 // meshlets and claims its slice of the list with one atomic.
-constexpr uint32_t kCullPerThread = 4;
-constexpr uint32_t kCullPerBlock = 256 * kCullPerThread;
-
+// PerThread meshlets per thread: 1 for small draws (more blocks in flight),
+// 4 for large batches (fewer atomics on the survivor count).
+template <uint32_t PerThread>
 __global__ void __launch_bounds__(256) k_cull(DrawParams p) {
+    constexpr uint32_t kCullPerThread = PerThread;
+    constexpr uint32_t kCullPerBlock = 256 * kCullPerThread;
     __shared__ uint32_t s_wave[4];
     __shared__ uint32_t s_base, s_view0;
     const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
@@ -1137,7 +1139,10 @@ void launch_front(const DrawParams& p, hipStream_t s, Profiler* prof, bool separ
     hipLaunchKernelGGL(k_begin_draw, dim3(max(zb, 1u)), dim3(256), 0, s, p.scratch, p.bin_counts, p.total_bins);
     if (p.total_meshlets) {     // a batch of empty views still clears through the bin raster
         Profiler::Scope sc(prof, "cull", s);
-        hipLaunchKernelGGL(k_cull, dim3((p.total_meshlets + kCullPerBlock - 1) / kCullPerBlock), dim3(256), 0, s, p);
+        if (p.total_meshlets <= 256u * 1024u)
+            hipLaunchKernelGGL(k_cull<1>, dim3((p.total_meshlets + 255) / 256), dim3(256), 0, s, p);
+        else
+            hipLaunchKernelGGL(k_cull<4>, dim3((p.total_meshlets + 1023) / 1024), dim3(256), 0, s, p);
     }
     if (p.total_meshlets) {
         Profiler::Scope sc(prof, "mesh_setup", s);
