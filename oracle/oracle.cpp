@@ -278,6 +278,7 @@ struct EdgeVars {  // Rasterizer.h:162-176
     int32_t A01[16], A12[16], A20[16];
     int32_t B01[16], B12[16], B20[16];
     float Z0[16], Z10[16], Z20[16];
+    float W0[16], W0S[16], W1S[16], W2S[16];
 };
 
 int32_t compute_edge(int32_t a, int32_t x, int32_t b, int32_t y) {  // Rasterizer.cpp:291-295
@@ -310,6 +311,10 @@ void setup_edges(EdgeVars& e, const TrianglePacket& t, int hw, int hh) {
         e.Z0[l] = t.Z0[l];
         e.Z10[l] = (t.Z1[l] - t.Z0[l]) * rcpArea;
         e.Z20[l] = (t.Z2[l] - t.Z0[l]) * rcpArea;
+        e.W0[l] = t.W0[l];
+        e.W0S[l] = t.W0[l] * rcpArea;
+        e.W1S[l] = t.W1[l] * rcpArea;
+        e.W2S[l] = t.W2[l] * rcpArea;
     }
 }
 
@@ -323,9 +328,86 @@ struct DrawState {
     orc_counters* ctr;
 };
 
+void sample_level16(const orc_texture* tex, const float* u, const float* v, int32_t layer, const int32_t* mip,
+                    int32_t wrap, bool magLinear, bool minLinear, const bool* active, uint32_t* out);
+
+// What FS_EncodeSurfaceId<true> (Shading.cpp:309-331) needs besides
+// coverage and depth: the triangle's vertex UVs, its material's texture and
+// alpha cutoff, and for clipped triangles the barycentric remap of the fan
+// triangle onto the source triangle (TriangleClipData, Rasterizer.cpp:461-464).
+struct FragmentCtx {
+    bool alpha_test;
+    float uv[3][2];
+    const orc_texture* tex;
+    uint32_t cutoff;
+    bool clipped;
+    float ru[3], rv[3];
+};
+
+// Perspective-correct barycentrics of one fragment (Rasterizer.h:293-319):
+// from the integer edge values u = E1, v = E2 and the triangle's 1/w terms.
+void fragment_bary(float u, float v, float W0, float W0S, float W1S, float W2S, float& bu, float& bv) {
+    float pw0 = fmaf(u + v, -W0S, W0);
+    float w = fmaf(u, W1S, fmaf(v, W2S, pw0));
+    float r = rcp14(w);
+    r = r * fmaf(-w, r, 2.0f);
+    // `u *= W1S * rcpW` is reassociated by the fast-math build to (u * W1S) * rcpW
+    bu = (u * W1S) * r;
+    bv = (v * W2S) * r;
+}
+
+// The alpha test of FS_EncodeSurfaceId<true> on one 4x4 tile: UVs of all 16
+// pixels from their barycentrics (BaryLerp, Rasterizer.h:100-102),
+// SampleImplicitLod (Texture.h:403-410: derivatives from pixel pairs of the
+// tile, mip level less the 8 fraction bits, one filter per tile) and
+// color >= cutoff << 24. Returns the passing pixels.
+void alpha_uv_tile(const FragmentCtx& fc, float slu, float slv, const float* bu, const float* bv, float* U,
+                   float* V, int32_t* mip) {
+    float su[16], sv[16];
+    for (int l = 0; l < 16; l++) {
+        float u = bu[l], v = bv[l];
+        if (fc.clipped) {
+            float cu = fmaf(u, fc.ru[1], fmaf(v, fc.ru[2], fc.ru[0]));
+            float cv = fmaf(u, fc.rv[1], fmaf(v, fc.rv[2], fc.rv[0]));
+            u = cu;
+            v = cv;
+        }
+        float b0 = 1.0f - (u + v);   // `1 - u - v`, reassociated by the fast-math build
+        U[l] = fmaf(fc.uv[0][0], b0, fmaf(fc.uv[1][0], u, fc.uv[2][0] * v));
+        V[l] = fmaf(fc.uv[0][1], b0, fmaf(fc.uv[1][1], u, fc.uv[2][1] * v));
+        su[l] = U[l] * slu;
+        sv[l] = V[l] * slv;
+    }
+    for (int l = 0; l < 16; l++) {
+        int a = l & ~1, b = l | 1;                   // dFdx: pixel pairs in a row
+        int c = l & ~4, d = l | 4;                   // dFdy: row pairs
+        float gx = su[b] - su[a], gy = sv[b] - sv[a];
+        float gz = su[d] - su[c], gw = sv[d] - sv[c];
+        float dx = fmaf(gx, gx, gy * gy), dy = fmaf(gz, gz, gw * gw);
+        float mx = (dx > dy) ? dx : dy;
+        mip[l] = (wrap_sub((int32_t)fbits(mx), 127 << 23) >> 24) - 8;
+    }
+}
+
+uint16_t alpha_test_tile(const FragmentCtx& fc, const float* bu, const float* bv) {
+    float U[16], V[16];
+    int32_t mip[16];
+    alpha_uv_tile(fc, fc.tex->scale_lerp_u, fc.tex->scale_lerp_v, bu, bv, U, V, mip);
+    uint32_t color[16];
+    const bool all[16] = { true, true, true, true, true, true, true, true,
+                           true, true, true, true, true, true, true, true };
+    sample_level16(fc.tex, U, V, 0, mip, 0, true, false, all, color);
+    uint16_t pass = 0;
+    for (int l = 0; l < 16; l++)
+        if (color[l] >= (fc.cutoff << 24)) pass |= (uint16_t)(1u << l);
+    return pass;
+}
+
 // Rasterizer::DrawTriangle (Rasterizer.h:250-328) with FS_EncodeSurfaceId
-// without alpha test (Shading.cpp:309-331) inlined.
-void draw_triangle(DrawState& st, const EdgeVars& e, int l, uint32_t bmin, uint32_t bmax, uint32_t surfaceId) {
+// (Shading.cpp:309-331) inlined: per 4x4 tile the depth test, then, for
+// alpha-tested materials, the alpha test.
+void draw_triangle(DrawState& st, const EdgeVars& e, int l, uint32_t bmin, uint32_t bmax, uint32_t surfaceId,
+                   const FragmentCtx& fc) {
     orc_framebuffer* fb = st.fb;
     uint32_t W = fb->width;
     uint32_t* ids = fb->data;
@@ -333,19 +415,41 @@ void draw_triangle(DrawState& st, const EdgeVars& e, int l, uint32_t bmin, uint3
     int minX = lo16(bmin), minY = hi16(bmin), maxX = lo16(bmax), maxY = hi16(bmax);
     for (int ty = minY; ty < maxY; ty += 4) {
         for (int tx = minX; tx < maxX; tx += 4) {
+            int32_t E1[16], E2[16];
+            float depth[16];
+            uint16_t mask = 0;
             for (int p = 0; p < 16; p++) {
                 int x = tx + (p & 3), y = ty + (p >> 2);
                 int32_t w0 = wrap_add(wrap_add(e.Edge0[l], wrap_mul(e.A12[l], x)), wrap_mul(e.B12[l], y));
                 int32_t w1 = wrap_add(wrap_add(e.Edge1[l], wrap_mul(e.A20[l], x)), wrap_mul(e.B20[l], y));
                 int32_t w2 = wrap_add(wrap_add(e.Edge2[l], wrap_mul(e.A01[l], x)), wrap_mul(e.B01[l], y));
+                E1[p] = w1;
+                E2[p] = w2;
+                depth[p] = fmaf((float)w1, e.Z10[l], fmaf((float)w2, e.Z20[l], e.Z0[l]));
                 if ((w0 | w1 | w2) < 0) continue;
+                mask |= (uint16_t)(1u << p);
                 st.ctr->fragments_covered++;
                 if (st.opts && st.opts->coverage_count) st.opts->coverage_count[(size_t)y * W + x]++;
-                float u = (float)w1, v = (float)w2;
-                float depth = fmaf(u, e.Z10[l], fmaf(v, e.Z20[l], e.Z0[l]));
+            }
+            if (!mask) continue;
+            uint16_t pass = 0;
+            for (int p = 0; p < 16; p++) {
+                if (!((mask >> p) & 1)) continue;
+                int x = tx + (p & 3), y = ty + (p >> 2);
+                if (depth[p] > depthBuf[pixel_offset(W, (uint32_t)x, (uint32_t)y)]) pass |= (uint16_t)(1u << p);
+            }
+            if (!pass) continue;
+            if (fc.alpha_test) {
+                float bu[16], bv[16];
+                for (int p = 0; p < 16; p++)
+                    fragment_bary((float)E1[p], (float)E2[p], e.W0[l], e.W0S[l], e.W1S[l], e.W2S[l], bu[p], bv[p]);
+                pass &= alpha_test_tile(fc, bu, bv);
+            }
+            for (int p = 0; p < 16; p++) {
+                if (!((pass >> p) & 1)) continue;
+                int x = tx + (p & 3), y = ty + (p >> 2);
                 uint32_t off = pixel_offset(W, (uint32_t)x, (uint32_t)y);
-                if (!(depth > depthBuf[off])) continue;
-                depthBuf[off] = depth;
+                depthBuf[off] = depth[p];
                 ids[off] = surfaceId;
                 st.ctr->fragments_passed++;
             }
@@ -361,6 +465,7 @@ struct Clipper {
     float Vertices[6][16 * 3 + 16];
     V4 Out[3][8][16];
     uint8_t PrimId[8][16];
+    float ClipU[8][16][3], ClipV[8][16][3];   // TriangleClipData::ClippedU/V
     uint16_t Masks[8];
     TrianglePacket Tris[8];
     uint32_t OutIdx = 0;
@@ -429,6 +534,13 @@ struct Clipper {
                 for (int k = 0; k < 3; k++)
                     Out[k][pi][lane] = { Vertices[0][ii[k]], Vertices[1][ii[k]], Vertices[2][ii[k]], Vertices[3][ii[k]] };
                 PrimId[pi][lane] = (uint8_t)prim;
+                const float u0 = Vertices[4][i0], v0 = Vertices[5][i0];
+                ClipU[pi][lane][0] = u0;
+                ClipU[pi][lane][1] = Vertices[4][i1] - u0;
+                ClipU[pi][lane][2] = Vertices[4][i2] - u0;
+                ClipV[pi][lane][0] = v0;
+                ClipV[pi][lane][1] = Vertices[5][i1] - v0;
+                ClipV[pi][lane][2] = Vertices[5][i2] - v0;
                 if (lane == 15) flush(hw, hh, cullMode, false);
                 if (++OutIdx >= 128) return;
             }
@@ -905,6 +1017,27 @@ void orc_draw_meshlets(orc_framebuffer* fb, uint32_t count, const orc_shading_ct
         if (mesh.PrimCount == 0) continue;
         ctr->triangles_processed += mesh.PrimCount;
         uint32_t meshletId = ctx->meshlet_offset + meshIdx;
+        // fragment shader inputs (FS_EncodeSurfaceId<AlphaTest>)
+        FragmentCtx fc = {};
+        const uint32_t matId = ctx->meshlets->material_id[meshletId];
+        if (mesh.FragmentShaderId == 1 && matId != UINT32_MAX && ctx->materials &&
+            ctx->materials[matId].texture >= 0) {
+            fc.alpha_test = true;
+            fc.tex = &ctx->textures[ctx->materials[matId].texture];
+            fc.cutoff = ctx->materials[matId].alpha_cutoff;
+        }
+        auto fragment_ctx = [&](uint32_t prim) {
+            FragmentCtx f = fc;
+            if (f.alpha_test) {
+                for (int k = 0; k < 3; k++) {
+                    uint32_t vi = mesh.Indices[k][prim];
+                    uint32_t tc = ctx->meshlets->texcoords[(size_t)meshletId * 64 + vi];
+                    f.uv[k][0] = half_to_float((uint16_t)(tc & 0xFFFF));
+                    f.uv[k][1] = half_to_float((uint16_t)(tc >> 16));
+                }
+            }
+            return f;
+        };
 
         for (uint32_t primOffset = 0; primOffset < mesh.PrimCount; primOffset += 16) {
             V4 v[3][16];
@@ -933,7 +1066,8 @@ void orc_draw_meshlets(orc_framebuffer* fb, uint32_t count, const orc_shading_ct
                         if (!((accept >> l) & 1)) continue;
                         uint32_t bmin, bmax;
                         render_bbox(tris, l, hw, hh, bmin, bmax);
-                        draw_triangle(st, e, l, bmin, bmax, meshletId * 128 + primOffset + (uint32_t)l);
+                        draw_triangle(st, e, l, bmin, bmax, meshletId * 128 + primOffset + (uint32_t)l,
+                                      fragment_ctx(primOffset + (uint32_t)l));
                         ctr->triangles_rasterized++;
                     }
                 }
@@ -955,7 +1089,13 @@ void orc_draw_meshlets(orc_framebuffer* fb, uint32_t count, const orc_shading_ct
                         uint32_t bmin, bmax;
                         render_bbox(tris, l, hw, hh, bmin, bmax);
                         uint32_t prim = clipper.PrimId[i / 16][l];
-                        draw_triangle(st, e, l, bmin, bmax, meshletId * 128 + primOffset + prim);
+                        FragmentCtx f = fragment_ctx(primOffset + prim);
+                        f.clipped = true;
+                        for (int k = 0; k < 3; k++) {
+                            f.ru[k] = clipper.ClipU[i / 16][l][k];
+                            f.rv[k] = clipper.ClipV[i / 16][l][k];
+                        }
+                        draw_triangle(st, e, l, bmin, bmax, meshletId * 128 + primOffset + prim, f);
                         ctr->triangles_rasterized++;
                     }
                 }
@@ -1186,6 +1326,35 @@ void orc_resolve_lit(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_
 }
 
 // ---------------------------------------------------------------- differentiable resolve
+
+void orc_fragment16(const int32_t* e1, const int32_t* e2, const float* coef, float* depth, float* bu, float* bv) {
+    for (int l = 0; l < 16; l++) {
+        float u = (float)e1[l], v = (float)e2[l];
+        depth[l] = fmaf(u, coef[1], fmaf(v, coef[2], coef[0]));
+        fragment_bary(u, v, coef[3], coef[4], coef[5], coef[6], bu[l], bv[l]);
+    }
+}
+
+void orc_alpha_uv16(const int32_t* e1, const int32_t* e2, const float* coef, const float* remap, const float* uv,
+                    float slu, float slv, float* U, float* V, int32_t* mip) {
+    FragmentCtx fc = {};
+    fc.alpha_test = true;
+    for (int k = 0; k < 3; k++) {
+        fc.uv[k][0] = uv[k * 2];
+        fc.uv[k][1] = uv[k * 2 + 1];
+    }
+    if (remap) {
+        fc.clipped = true;
+        for (int k = 0; k < 3; k++) {
+            fc.ru[k] = remap[k];
+            fc.rv[k] = remap[3 + k];
+        }
+    }
+    float bu[16], bv[16];
+    for (int l = 0; l < 16; l++)
+        fragment_bary((float)e1[l], (float)e2[l], coef[3], coef[4], coef[5], coef[6], bu[l], bv[l]);
+    alpha_uv_tile(fc, slu, slv, bu, bv, U, V, mip);
+}
 
 void orc_bary_f64(const float* clip, double su, double sv, double* bary) {
     double px[3], py[3], iw[3];

@@ -133,6 +133,17 @@ int32_t orc_clip_triangle(const float* v /*[3][4]*/, uint32_t outcodes, float gb
 void orc_setup16(const float* clip /*[3][4][16]*/, int32_t w, int32_t h, int32_t cull_mode,
                  int32_t* X, int32_t* Y, float* Z, float* W, float* det, uint16_t* mask);
 
+/* Per-fragment depth and perspective-correct barycentrics of 16 fragments
+ * (Rasterizer.h:293-319) from the edge values E1, E2 and
+ * coef = {Z0, Z10, Z20, W0, W0S, W1S, W2S}. */
+void orc_fragment16(const int32_t* e1, const int32_t* e2, const float* coef, float* depth, float* bu, float* bv);
+/* The alpha test's UVs and implicit mip levels for one 4x4 tile
+ * (FS_EncodeSurfaceId<true>, Shading.cpp:316-327; SampleImplicitLod,
+ * Texture.h:403-410): remap = {ClippedU[3], ClippedV[3]} or NULL,
+ * uv = vertex UVs [3][2], slu/slv = ScaleLerpU/V. */
+void orc_alpha_uv16(const int32_t* e1, const int32_t* e2, const float* coef, const float* remap, const float* uv,
+                    float slu, float slv, float* U, float* V, int32_t* mip);
+
 /* ------------------------------------------------------------ differentiable resolve
  *
  * Not a reference operation: the texture-sampling resolve made

@@ -110,6 +110,44 @@ void probe_fragment(const int32_t* e1, const int32_t* e2, const float* coef, int
     }
 }
 
+// Alpha-test UVs of one 4x4 tile (Shading.cpp:316-327 with
+// Rasterizer.h:293-319 and Texture.h:260-280, 403-410): perspective-correct
+// barycentrics, the clipped-triangle remap, BaryLerp of the vertex UVs, the
+// lerp-scaled UVs and their tile derivatives, and the implicit mip level.
+// coef: W0, W0S, W1S, W2S; remap: ClippedU[3], ClippedV[3] or null; uv[3][2].
+void probe_alpha_uv(const int32_t* e1, const int32_t* e2, const float* coef, const float* remap, const float* uv,
+                    float slu, float slv, float* U, float* V, int32_t* mip) {
+    vf W0 = coef[0], W0S = coef[1], W1S = coef[2], W2S = coef[3];
+    vf u = __builtin_convertvector(ldi(e1), vf);
+    vf v = __builtin_convertvector(ldi(e2), vf);
+    vf pw0 = vfma(u + v, -W0S, W0);
+    vf w = vfma(u, W1S, vfma(v, W2S, pw0));
+    vf r = (vf)_mm512_rcp14_ps((__m512)w);
+    r *= vfma(-w, r, 2.0f);
+    u *= W1S * r;
+    v *= W2S * r;
+    if (remap) {
+        vf cu = vfma(u, (vf)remap[1], vfma(v, (vf)remap[2], (vf)remap[0]));
+        vf cv = vfma(u, (vf)remap[4], vfma(v, (vf)remap[5], (vf)remap[3]));
+        u = cu, v = cv;
+    }
+    vf b0 = 1 - u - v;
+    vf Ux = vfma((vf)uv[0], b0, vfma((vf)uv[2], u, (vf)uv[4] * v));
+    vf Vy = vfma((vf)uv[1], b0, vfma((vf)uv[3], u, (vf)uv[5] * v));
+    st(U, Ux);
+    st(V, Vy);
+    __m512 su = (__m512)(Ux * slu), sv = (__m512)(Vy * slv);
+    auto ddx = [](__m512 p) { return _mm512_sub_ps(_mm512_shuffle_ps(p, p, 0b11110101), _mm512_shuffle_ps(p, p, 0b10100000)); };
+    auto ddy = [](__m512 p) {
+        return _mm512_sub_ps(_mm512_shuffle_f32x4(p, p, 0b11110101), _mm512_shuffle_f32x4(p, p, 0b10100000));
+    };
+    vf gx = (vf)ddx(su), gy = (vf)ddx(sv), gz = (vf)ddy(su), gw = (vf)ddy(sv);
+    vf dx = vfma(gx, gx, gy * gy), dy = vfma(gz, gz, gw * gw);
+    vf mx = (vf)_mm512_max_ps((__m512)dx, (__m512)dy);
+    vi m = ((((vi)mx) - (127 << 23)) >> 23) >> 1;
+    sti(mip, m - 8);
+}
+
 // Screen UV of a 4x4 tile in a full-screen pass (Rasterizer.h:226-237).
 void probe_tile_uv(const int32_t* px, int n, float size, float* out) {
     float scale = 2.0f / size;

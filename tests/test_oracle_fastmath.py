@@ -181,3 +181,47 @@ def test_exhaustive_approx_tables():
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "rcp14 mismatches 0, rsqrt14 mismatches 0" in r.stdout
+
+
+def _fragment_inputs(rng):
+    # a plausible triangle: edge values over a tile, 1/w terms scaled by 16/area
+    e1 = rng.integers(-2000, 60000, 16).astype(np.int32)
+    e2 = rng.integers(-2000, 60000, 16).astype(np.int32)
+    w = rng.uniform(0.05, 2.0, 3).astype(np.float32)
+    rcp_area = np.float32(16.0) / np.float32(rng.integers(1 << 10, 1 << 24))
+    coef = np.array([rng.uniform(0, 1), rng.uniform(-1, 1) * 1e-5, rng.uniform(-1, 1) * 1e-5,
+                     w[0], w[0] * rcp_area, w[1] * rcp_area, w[2] * rcp_area], np.float32)
+    return e1, e2, coef
+
+
+def test_fragment_barycentrics(probe):
+    rng = np.random.default_rng(8)
+    L = oracle.lib()
+    for _ in range(64):
+        e1, e2, coef = _fragment_inputs(rng)
+        depth = np.zeros(16, np.float32); bu = np.zeros_like(depth); bv = np.zeros_like(depth)
+        probe.probe_fragment(P(e1), P(e2), P(coef), 16, P(depth), P(bu), P(bv))
+        d2 = np.zeros(16, np.float32); u2 = np.zeros_like(d2); v2 = np.zeros_like(d2)
+        L.orc_fragment16(P(e1), P(e2), P(coef), P(d2), P(u2), P(v2))
+        for a, b in ((d2, depth), (u2, bu), (v2, bv)):
+            assert (_bits(a) == _bits(b)).all()
+
+
+@pytest.mark.parametrize("clipped", [False, True])
+def test_alpha_test_uvs_and_mip(probe, clipped):
+    rng = np.random.default_rng(9 + clipped)
+    L = oracle.lib()
+    for _ in range(64):
+        e1, e2, coef = _fragment_inputs(rng)
+        w4 = np.ascontiguousarray(coef[3:7])
+        remap = rng.uniform(-0.5, 1.5, 6).astype(np.float32) if clipped else None
+        uv = rng.uniform(0, 8, 6).astype(np.float32)
+        U = np.zeros(16, np.float32); V = np.zeros_like(U); mip = np.zeros(16, np.int32)
+        rp = P(remap) if clipped else None
+        probe.probe_alpha_uv(P(e1), P(e2), P(w4), rp, P(uv), C.c_float(256 * 256.0), C.c_float(256 * 256.0),
+                             P(U), P(V), P(mip))
+        U2 = np.zeros(16, np.float32); V2 = np.zeros_like(U2); mip2 = np.zeros(16, np.int32)
+        L.orc_alpha_uv16(P(e1), P(e2), P(coef), rp, P(uv), C.c_float(256 * 256.0), C.c_float(256 * 256.0),
+                         P(U2), P(V2), P(mip2))
+        assert (_bits(U2) == _bits(U)).all() and (_bits(V2) == _bits(V)).all()
+        assert (mip2 == mip).all()
