@@ -82,10 +82,12 @@ struct Scratch {
     size_t bin_lists_cap = 0;
     uint32_t* spill = nullptr;
     size_t spill_cap = 0;
+    AlphaRecord* alpha = nullptr;      // only once a draw has alpha-tested materials
+    size_t alpha_cap = 0;
     DrawScratch* scratch = nullptr;
 
     void release() {
-        void* ptrs[] = { records, clip, visible_list, meshlet_counts, bin_counts, bin_lists, spill, scratch };
+        void* ptrs[] = { records, clip, visible_list, meshlet_counts, bin_counts, bin_lists, spill, alpha, scratch };
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         *this = Scratch();
@@ -128,6 +130,7 @@ struct glimp_scene {
     uint32_t num_materials = 0;
     DevTexture* textures = nullptr;
     uint32_t num_textures = 0;
+    bool has_alpha = false;            // a material runs the alpha test
 };
 
 struct glimp_framebuffer {
@@ -336,6 +339,8 @@ int glimp_scene_set_materials(glimp_scene* s, const glimp_material* mats, uint32
     HIP_TRY(upload(&s->textures, dt.data(), dt.size()));
     s->num_materials = count;
     s->num_textures = num_textures;
+    s->has_alpha = false;
+    for (const auto& m : dm) s->has_alpha = s->has_alpha || (m.fs_id == 1 && m.texture >= 0);
     return GLIMP_OK;
 }
 
@@ -480,6 +485,7 @@ void make_view(glimp_framebuffer* f, glimp_scene* s, uint32_t meshlet_offset, ui
     V.m = s->m;
     V.materials = s->materials;
     V.num_materials = s->num_materials;
+    V.textures = s->textures;
     V.meshlet_offset = meshlet_offset;
     V.meshlet_count = meshlet_count;
     V.seq_base = f->next_seq;
@@ -519,7 +525,8 @@ uint64_t count_bins(const std::vector<ViewParams>& views, uint32_t bin_size) {
     return n;
 }
 
-int prepare_scratch(Scratch& S, size_t total_meshlets, size_t total_bins, const LaunchShape& sh, bool binned) {
+int prepare_scratch(Scratch& S, size_t total_meshlets, size_t total_bins, const LaunchShape& sh, bool binned,
+                    bool alpha) {
     const size_t clipq = clip_capacity_for(total_meshlets);
     const size_t recs = total_meshlets * 128 + clipq * 8;
     int rc = grow_one(S.records, S.records_cap, recs);
@@ -529,6 +536,7 @@ int prepare_scratch(Scratch& S, size_t total_meshlets, size_t total_bins, const 
     if (rc == GLIMP_OK) rc = grow_one(S.bin_counts, S.bin_counts_cap, std::max<size_t>(total_bins, 1));
     if (rc == GLIMP_OK && binned) rc = grow_one(S.bin_lists, S.bin_lists_cap, total_bins * sh.bin_capacity);
     if (rc == GLIMP_OK && binned) rc = grow_one(S.spill, S.spill_cap, recs);
+    if (rc == GLIMP_OK && alpha) rc = grow_one(S.alpha, S.alpha_cap, S.records_cap);
     if (rc == GLIMP_OK && !S.scratch) {
         HIP_TRY(dev_alloc(&S.scratch, 1));
         HIP_TRY(hipMemset(S.scratch, 0, sizeof(DrawScratch)));
@@ -537,13 +545,14 @@ int prepare_scratch(Scratch& S, size_t total_meshlets, size_t total_bins, const 
 }
 
 void fill_params(glimp_context* c, const Scratch& S, size_t total_meshlets, uint32_t total_bins,
-                 const LaunchShape& sh, uint32_t flags, DrawParams& p) {
+                 const LaunchShape& sh, uint32_t flags, bool alpha, DrawParams& p) {
     p.total_meshlets = (uint32_t)total_meshlets;
     p.total_bins = total_bins;
     p.flags = flags;
     p.tables = c->approx();
     p.visible_list = S.visible_list;
     p.records = S.records;
+    p.alpha = alpha ? S.alpha : nullptr;
     p.meshlet_counts = S.meshlet_counts;
     p.clip_record_base = (uint32_t)(total_meshlets * 128);
     p.clip_record_capacity = (uint32_t)std::min<size_t>(S.records_cap - total_meshlets * 128, 0x7FFFFFFFu);
@@ -589,11 +598,11 @@ int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, 
     std::vector<ViewParams> views(1);
     make_view(f, s, d->meshlet_offset, d->meshlet_count, d->object_to_clip, d->frustum_planes, d->flags, views[0]);
     const uint32_t total_bins = direct ? 0 : layout_bins(views, sh.bin_size);
-    int rc = prepare_scratch(c->draw_scratch, d->meshlet_count, total_bins, sh, !direct);
+    int rc = prepare_scratch(c->draw_scratch, d->meshlet_count, total_bins, sh, !direct, s->has_alpha);
     if (rc != GLIMP_OK) return rc;
     DrawParams p;
     memset(&p, 0, sizeof(p));
-    fill_params(c, c->draw_scratch, d->meshlet_count, total_bins, sh, d->flags, p);
+    fill_params(c, c->draw_scratch, d->meshlet_count, total_bins, sh, d->flags, s->has_alpha, p);
     p.num_views = 1;
     p.views = nullptr;
     p.view = views[0];
@@ -615,6 +624,7 @@ struct glimp_batch {
     size_t total_meshlets = 0;
     uint32_t total_bins = 0;
     LaunchShape shape;
+    bool alpha = false;                // a view's scene has alpha-tested materials
     Scratch scratch;
     hipStream_t stream = nullptr;      // graph replays run here, chained to the caller's stream by events
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -637,7 +647,9 @@ int layout_batch(glimp_batch* b) {
     if (count_bins(probe, b->shape.bin_size) >= kMaxBins)
         return fail(GLIMP_E_LIMIT, "glimp_batch: more than 2^26 screen bins; use a larger bin size or fewer views");
     b->total_bins = layout_bins(b->views, b->shape.bin_size);
-    int rc = prepare_scratch(b->scratch, b->total_meshlets, b->total_bins, b->shape, true);
+    b->alpha = false;
+    for (const auto& d : b->descs) b->alpha = b->alpha || d.scene->has_alpha;
+    int rc = prepare_scratch(b->scratch, b->total_meshlets, b->total_bins, b->shape, true, b->alpha);
     if (rc != GLIMP_OK) return rc;
     if (!b->d_views) HIP_TRY(dev_alloc(&b->d_views, b->views.size()));
     HIP_TRY(hipMemcpy(b->d_views, b->views.data(), b->views.size() * sizeof(ViewParams), hipMemcpyHostToDevice));
@@ -647,7 +659,7 @@ int layout_batch(glimp_batch* b) {
 
 void batch_params(glimp_batch* b, DrawParams& p) {
     memset(&p, 0, sizeof(p));
-    fill_params(b->ctx, b->scratch, b->total_meshlets, b->total_bins, b->shape, 0, p);
+    fill_params(b->ctx, b->scratch, b->total_meshlets, b->total_bins, b->shape, 0, b->alpha, p);
     p.num_views = (uint32_t)b->views.size();
     p.views = b->d_views;
 }

@@ -21,11 +21,19 @@
 // rank being its position in that draw order; all depths that can pass are
 // positive, so the largest key is the reference's surviving fragment and the
 // result does not depend on the order the GPU produces fragments in.
+//
+// Alpha test. Meshlets whose material has an alpha cutoff below 255 run
+// FS_EncodeSurfaceId<true> (Shading.cpp:309-331): setup writes a 64-byte
+// AlphaRecord beside each of their records, and the raster kernels' Alpha
+// variants drop fragments whose texel fails the cutoff. Which fragments pass
+// is a function of the triangle and its 4x4 tile alone, so the key maximum
+// still reproduces the reference's sequential result.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 
 #include "internal.h"
+#include "sampler.h"
 
 namespace glimp {
 
@@ -82,9 +90,11 @@ __device__ __forceinline__ void render_bbox(uint32_t p0, uint32_t p1, uint32_t p
 
 // TrianglePacket::Setup (Rasterizer.cpp:256-289) for one triangle plus the
 // float part of TriangleEdgeVars::Setup (Rasterizer.cpp:296-329). Fills the
-// record's positions and depth plane; returns whether the triangle is kept.
+// record's positions and depth plane and `wt` with the perspective terms
+// {W0, W0*s, W1*s, W2*s}, s = 16/area (Rasterizer.cpp:324-328); returns
+// whether the triangle is kept.
 __device__ bool setup_triangle(const ApproxTables& T, const float v[3][4], int hw, int hh, int cull_mode,
-                               TriRecord& rec) {
+                               TriRecord& rec, float (&wt)[4]) {
     float rw[3], x[3], y[3], z[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -122,6 +132,10 @@ __device__ bool setup_triangle(const ApproxTables& T, const float v[3][4], int h
     rec.z0 = z[0];
     rec.z10 = (z[1] - z[0]) * rcpArea;
     rec.z20 = (z[2] - z[0]) * rcpArea;
+    wt[0] = rw[0];
+    wt[1] = rw[0] * rcpArea;
+    wt[2] = rw[1] * rcpArea;
+    wt[3] = rw[2] * rcpArea;
     return true;
 }
 
@@ -317,6 +331,8 @@ __device__ __forceinline__ MeshletFetch fetch_meshlet(const DrawParams& p, uint3
     return f;
 }
 
+// `Alpha`: the launch has alpha-tested materials (p.alpha is set).
+template <bool Alpha>
 __device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, float (&lds)[4][64],
                               const MeshletFetch& f) {
     const uint32_t g = f.g, v = f.v;           // global meshlet number, view
@@ -325,12 +341,16 @@ __device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, fl
     const int hw = V.width / 2, hh = V.height / 2;
 
     const uint32_t nt = f.nt;
-    int cull_mode = 1, fs_id = 0;
+    int cull_mode = 1;
+    bool alpha = false;            // FragmentShaderId 1 with a texture to test against
+    uint32_t alpha_tex = 0;
     {
         const uint32_t mat = f.mat;
         if (mat != 0xFFFFFFFFu && mat < V.num_materials) {
-            cull_mode = V.materials[mat].cull_mode;
-            fs_id = V.materials[mat].fs_id;
+            const DevMaterial dm = V.materials[mat];
+            cull_mode = dm.cull_mode;
+            alpha = Alpha && dm.fs_id == 1 && dm.texture >= 0;
+            alpha_tex = (uint32_t)dm.texture | ((uint32_t)dm.alpha_cutoff << 24);
         }
         // ShadeMeshlet (Shading.cpp:295-298): lane = vertex
         __builtin_amdgcn_wave_barrier();   // the previous meshlet's reads are done
@@ -377,14 +397,30 @@ __device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, fl
         }
         const uint32_t packet = t >> 4, plane = t & 15;
         TriRecord rec;
+        float wt[4];
         bool keep = false;
         if (accept) {
-            keep = setup_triangle(p.tables, vv, hw, hh, cull_mode, rec);
+            keep = setup_triangle(p.tables, vv, hw, hh, cull_mode, rec, wt);
             rec.rank = make_rank(seq, packet, 0, plane, 0);
-            rec.flags = (uint32_t)fs_id | flags_hi;
+            rec.flags = (alpha ? kRecAlpha : 0u) | flags_hi;
         }
         unsigned long long km = __ballot(keep);
-        if (keep) out[kept + mbcnt(km)] = rec;
+        if (keep) {
+            const uint32_t slot = kept + mbcnt(km);
+            out[slot] = rec;
+            if (alpha) {
+                AlphaRecord a;
+                a.w0 = wt[0]; a.w0s = wt[1]; a.w1s = wt[2]; a.w2s = wt[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+                    a.uv[k] = V.m.texcoords[(size_t)f.mi * 64 + ((f.idx[round] >> (8 * k)) & 63u)];
+                a.tex = alpha_tex;
+#pragma unroll
+                for (int k = 0; k < 3; k++) a.ru[k] = a.rv[k] = 0.0f;
+                a.pad[0] = a.pad[1] = 0;
+                p.alpha[(size_t)i * 128 + slot] = a;
+            }
+        }
         kept += (uint32_t)__popcll(km);
         unsigned long long cm = __ballot(nontrivial);
         if (cm) {
@@ -398,7 +434,7 @@ __device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, fl
                 if (slot < p.clip_capacity) {
                     ClipEntry ce;
                     ce.rank_base = make_rank(seq, packet, 1, plane, 0);
-                    ce.info = partial | ((uint32_t)cull_mode << 8) | ((uint32_t)fs_id << 10);
+                    ce.info = partial | ((uint32_t)cull_mode << 8) | ((alpha ? 1u : 0u) << 10);
 #pragma unroll
                     for (int k = 0; k < 3; k++)
 #pragma unroll
@@ -422,6 +458,7 @@ __device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, fl
     }
 }
 
+template <bool Alpha>
 __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
     __shared__ float lds_pos[4][4][64];
     const uint32_t wid = threadIdx.x / kWave;
@@ -435,7 +472,7 @@ __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
         const uint32_t next = i + waves;
         MeshletFetch nxt;
         if (next < nvis) nxt = fetch_meshlet(p, next, lane);
-        setup_meshlet(p, i, lane, lds_pos[wid], cur);
+        setup_meshlet<Alpha>(p, i, lane, lds_pos[wid], cur);
         if (next >= nvis) break;
         cur = nxt;
         i = next;
@@ -464,14 +501,23 @@ __device__ __forceinline__ float lane_get(float v, uint32_t src) {
 // Clips queued triangle q with the whole wave. Lanes whose fan triangle
 // survives setup return true with the record and its index; the records
 // are also stored after the meshlet slots.
+// With `Alpha` the barycentric attributes travel with the positions.
+template <bool Alpha>
 __device__ bool clip_one(const DrawParams& p, uint32_t q, uint32_t lane, TriRecord& out, uint32_t& out_index) {
+    constexpr int NC = Alpha ? 6 : 4;   // clipped channels
     {
         const ClipEntry& ce = p.clip_queue[q];
         const ViewParams& VP = view_of(p, ce.view);
         const int hw = VP.width / 2, hh = VP.height / 2;
-        float V[4];
+        // x, y, z, w and the barycentric attributes (u, v) of the source
+        // triangle's vertices, (0,0), (1,0), (0,1) (Rasterizer.cpp:399-408)
+        float V[NC];
 #pragma unroll
         for (int c = 0; c < 4; c++) V[c] = lane < 3 ? ce.v[lane][c] : 0.0f;
+        if constexpr (Alpha) {
+            V[4] = lane == 1 ? 1.0f : 0.0f;
+            V[5] = lane == 2 ? 1.0f : 0.0f;
+        }
         uint32_t vc = 3, generated = 0;
         const uint32_t codes = ce.info & 0x3Fu;
         for (uint32_t plane = 0; plane < 6; plane++) {
@@ -481,9 +527,9 @@ __device__ bool clip_one(const DrawParams& p, uint32_t q, uint32_t lane, TriReco
             const bool neg = plane & 1u;
             const bool active = lane < vc;
             const uint32_t nxt = (lane + 1 < vc) ? lane + 1 : 0;
-            float Vn[4];
+            float Vn[NC];
 #pragma unroll
-            for (int c = 0; c < 4; c++) Vn[c] = lane_get(V[c], nxt);
+            for (int c = 0; c < NC; c++) Vn[c] = lane_get(V[c], nxt);
             float ca = comp == 0 ? V[0] : (comp == 1 ? V[1] : V[2]);
             float cb = comp == 0 ? Vn[0] : (comp == 1 ? Vn[1] : Vn[2]);
             if (neg) { ca = -ca; cb = -cb; }
@@ -505,11 +551,11 @@ __device__ bool clip_one(const DrawParams& p, uint32_t q, uint32_t lane, TriReco
             }
             const uint32_t outCount = __shfl(incl, 63);
             const uint32_t pos = incl - cnt;
-            float I[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+            float I[NC] = {};
             if (make) {
                 const float t = da / (da - db);
 #pragma unroll
-                for (int c = 0; c < 4; c++) I[c] = __builtin_fmaf(Vn[c], t, __builtin_fmaf(-t, V[c], V[c]));
+                for (int c = 0; c < NC; c++) I[c] = __builtin_fmaf(Vn[c], t, __builtin_fmaf(-t, V[c], V[c]));
             }
             if (vc < 3) break;
             // output lane j pulls from source lane src(j) = #{i : incl_i <= j}
@@ -520,7 +566,7 @@ __device__ bool clip_one(const DrawParams& p, uint32_t q, uint32_t lane, TriReco
             const int srcIn = __builtin_amdgcn_ds_bpermute((int)(src << 2), in ? 1 : 0);
             const bool isVert = srcIn && lane == srcPos;
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
+            for (int c = 0; c < NC; c++) {
                 const float a = lane_get(V[c], src), b = lane_get(I[c], src);
                 V[c] = isVert ? a : b;
             }
@@ -538,12 +584,24 @@ __device__ bool clip_one(const DrawParams& p, uint32_t q, uint32_t lane, TriReco
             tri[1][c] = lane_get(V[c], min(f + 1, 63u));
             tri[2][c] = lane_get(V[c], min(f + 2, 63u));
         }
+        // TriangleClipData of the fan triangle (Rasterizer.cpp:461-464)
+        float u0 = 0, v0 = 0, ru1 = 0, rv1 = 0, ru2 = 0, rv2 = 0;
+        if constexpr (Alpha) {
+            u0 = lane_get(V[4], 0);
+            v0 = lane_get(V[5], 0);
+            ru1 = lane_get(V[4], min(f + 1, 63u)) - u0;
+            rv1 = lane_get(V[5], min(f + 1, 63u)) - v0;
+            ru2 = lane_get(V[4], min(f + 2, 63u)) - u0;
+            rv2 = lane_get(V[5], min(f + 2, 63u)) - v0;
+        }
+        const bool alpha = Alpha && ((ce.info >> 10) & 1u);
         TriRecord rec;
+        float wt[4];
         bool keep = false;
         if (f + 2 < vc && f < 8) {
-            keep = setup_triangle(p.tables, tri, hw, hh, cull_mode, rec);
+            keep = setup_triangle(p.tables, tri, hw, hh, cull_mode, rec, wt);
             rec.rank = ce.rank_base | f;
-            rec.flags = ((ce.info >> 10) & 3u) | (ce.view << 16);
+            rec.flags = (alpha ? kRecAlpha : 0u) | (ce.view << 16);
         }
         const unsigned long long km = __ballot(keep);
         if (!km) return false;
@@ -558,12 +616,31 @@ __device__ bool clip_one(const DrawParams& p, uint32_t q, uint32_t lane, TriReco
             return false;
         }
         p.records[p.clip_record_base + slot] = rec;
+        if (alpha) {
+            // the source triangle's UVs and material, from the meshlet and
+            // triangle the rank names
+            const uint32_t seq = ce.rank_base >> 11;
+            const uint32_t t = ((ce.rank_base >> 8) & 7u) * 16 + ((ce.rank_base >> 3) & 15u);
+            const uint32_t mi = VP.meshlet_offset + (seq - VP.seq_base);
+            const DevMaterial dm = VP.materials[VP.m.material_id[mi]];
+            AlphaRecord a;
+            a.w0 = wt[0]; a.w0s = wt[1]; a.w1s = wt[2]; a.w2s = wt[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                a.uv[k] = VP.m.texcoords[(size_t)mi * 64 + (VP.m.indices[(size_t)mi * 384 + k * 128 + t] & 63u)];
+            a.tex = (uint32_t)dm.texture | ((uint32_t)dm.alpha_cutoff << 24);
+            a.ru[0] = u0; a.ru[1] = ru1; a.ru[2] = ru2;
+            a.rv[0] = v0; a.rv[1] = rv1; a.rv[2] = rv2;
+            a.pad[0] = a.pad[1] = 0;
+            p.alpha[p.clip_record_base + slot] = a;
+        }
         out = rec;
         out_index = p.clip_record_base + slot;
         return true;
     }
 }
 
+template <bool Alpha>
 __global__ void __launch_bounds__(256) k_clip(DrawParams p) {
     const uint32_t n = min(p.scratch->num_clip, p.clip_capacity);
     const uint32_t lane = lane_id();
@@ -571,7 +648,7 @@ __global__ void __launch_bounds__(256) k_clip(DrawParams p) {
     for (uint32_t q = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; q < n; q += waves) {
         TriRecord rec;
         uint32_t idx;
-        (void)clip_one(p, q, lane, rec, idx);
+        (void)clip_one<Alpha>(p, q, lane, rec, idx);
     }
 }
 
@@ -657,15 +734,72 @@ __device__ __forceinline__ bool shade_pixel(const TriLds& T, uint32_t x, uint32_
     return true;
 }
 
+// The alpha test of FS_EncodeSurfaceId<true> (Shading.cpp:316-327) for the
+// 16 lanes of one 4x4 tile (lane & 15 = x + 4y inside the tile). All 16
+// lanes must be active: the screen derivatives and the filter choice are
+// taken across them. (e1, e2) are the lane's edge values, the u and v of
+// DrawTriangle: perspective-correct barycentrics (Rasterizer.h:298-314),
+// the clipped triangle's remap (Rasterizer.h:306-312), BaryLerp of the
+// vertex UVs, SampleImplicitLod (Texture.h:403-410: derivatives from the
+// tile's pixel pairs, mip level less the 8 fraction bits, minification
+// anywhere in the tile selects the min filter), then color >= cutoff << 24.
+// Operation order follows the reference's optimised build: `u *= W1S * rcpW`
+// is evaluated as (u * W1S) * rcpW and `1 - u - v` as 1 - (u + v).
+__device__ __forceinline__ bool alpha_test_tile(const ApproxTables& T, const AlphaRecord& A, bool clipped,
+                                                const DevTexture* textures, int32_t e1, int32_t e2, uint32_t lane) {
+    const float u = (float)e1, v = (float)e2;
+    const float pw0 = __builtin_fmaf(u + v, -A.w0s, A.w0);
+    const float w = __builtin_fmaf(u, A.w1s, __builtin_fmaf(v, A.w2s, pw0));
+    float r = rcp14(T, w);
+    r = r * __builtin_fmaf(-w, r, 2.0f);
+    float bu = (u * A.w1s) * r, bv = (v * A.w2s) * r;
+    if (clipped) {
+        const float cu = __builtin_fmaf(bu, A.ru[1], __builtin_fmaf(bv, A.ru[2], A.ru[0]));
+        const float cv = __builtin_fmaf(bu, A.rv[1], __builtin_fmaf(bv, A.rv[2], A.rv[0]));
+        bu = cu;
+        bv = cv;
+    }
+    const float b0 = 1.0f - (bu + bv);
+    float tu[3], tv[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        tu[k] = half_to_float(A.uv[k] & 0xFFFFu);
+        tv[k] = half_to_float(A.uv[k] >> 16);
+    }
+    const float U = __builtin_fmaf(tu[0], b0, __builtin_fmaf(tu[1], bu, tu[2] * bv));
+    const float V = __builtin_fmaf(tv[0], b0, __builtin_fmaf(tv[1], bu, tv[2] * bv));
+    const DevTexture& tex = textures[A.tex & 0xFFFFFFu];
+    const float su = U * tex.scale_lerp_u, sv = V * tex.scale_lerp_v;
+    // dFdx pairs pixels (2i, 2i+1) of a row, dFdy rows (r, r+1) of a pair
+    const float sux = __shfl_xor(su, 1), svx = __shfl_xor(sv, 1);
+    const float suy = __shfl_xor(su, 4), svy = __shfl_xor(sv, 4);
+    const bool odd = (lane & 1u) != 0, low = (lane & 4u) != 0;
+    const float gx = odd ? su - sux : sux - su, gy = odd ? sv - svx : svx - sv;
+    const float gz = low ? su - suy : suy - su, gw = low ? sv - svy : svy - sv;
+    const float dx = __builtin_fmaf(gx, gx, gy * gy), dy = __builtin_fmaf(gz, gz, gw * gw);
+    const float mx = (dx > dy) ? dx : dy;
+    const int32_t mip = ((int32_t)(fbits(mx) - (127u << 23)) >> 24) - 8;
+    const unsigned long long groupMask = 0xFFFFull << (lane & 48u);
+    const bool minify = (__ballot(mip > 0) & groupMask) != 0;
+    const uint32_t color = sample_level(tex, U, V, mip, !minify);
+    return color >= ((A.tex >> 24) << 24);
+}
+
 // Each wave takes a chunk of up to 64 records (meshlet slots, clipped
 // records, or bin spill entries), lays their render boxes out as one list of
 // 4x4 tiles (exclusive scan of tile counts) and walks it four tiles at a
 // time: lane = pixel, 16 lanes = one tile, so a tile's 16 keys (128 bytes)
 // are one contiguous access. Covered pixels update the key of their view's
-// vis-buffer with atomicMax when the key can still win.
+// vis-buffer with atomicMax when the key can still win. With `Alpha`,
+// tiles of alpha-tested records that have a fragment passing the depth test
+// run the alpha test (the reference tests after the depth test; the test is
+// a function of the triangle and the tile alone, so the order does not
+// change which fragments pass).
+template <bool Alpha>
 __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p, int use_spill) {
     __shared__ TriLds tl[4][64];
     __shared__ uint32_t tview[4][64];
+    __shared__ uint32_t talpha[Alpha ? 4 : 1][64];   // record index of an alpha-tested record, else ~0
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
     const uint32_t nchunks = num_chunks(p, use_spill);
@@ -690,6 +824,7 @@ __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p, int use_spi
             t.tw = tw;
             ntiles = tw * th;
             tview[wid][lane] = v;
+            if constexpr (Alpha) talpha[wid][lane] = (rec.flags & kRecAlpha) ? r : 0xFFFFFFFFu;
         }
         uint32_t incl = ntiles;
 #pragma unroll
@@ -721,10 +856,35 @@ __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p, int use_spi
                 const uint32_t tx = pt - ty * T.tw;
                 const uint32_t x = (uint32_t)lo16(T.org) + tx * 4 + px;
                 const uint32_t y = (uint32_t)hi16(T.org) + ty * 4 + py;
-                float depth;
-                if (x < W && y < (uint32_t)V.height && shade_pixel(T, x, y, depth)) {
-                    covered++;
-                    if (depth > V.clear_depth) {
+                if constexpr (!Alpha) {
+                    float depth;
+                    if (x < W && y < (uint32_t)V.height && shade_pixel(T, x, y, depth)) {
+                        covered++;
+                        if (depth > V.clear_depth) {
+                            unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
+                            unsigned long long* dst = V.keys + pixel_offset(W, x, y);
+                            if (key > __atomic_load_n(dst, __ATOMIC_RELAXED)) atomicMax(dst, key);
+                            passed++;
+                        }
+                    }
+                } else {
+                    // every lane of the tile evaluates its edges (the alpha
+                    // test needs all 16); render boxes lie inside the
+                    // framebuffer, the bounds test only guards the store
+                    const int32_t w0 = edge_at(T.e0, T.a12, T.b12, x, y);
+                    const int32_t w1 = edge_at(T.e1, T.a20, T.b20, x, y);
+                    const int32_t w2 = edge_at(T.e2, T.a01, T.b01, x, y);
+                    const bool cov = x < W && y < (uint32_t)V.height && (w0 | w1 | w2) >= 0;
+                    const float depth = __builtin_fmaf((float)w1, T.z10, __builtin_fmaf((float)w2, T.z20, T.z0));
+                    bool pass = cov && depth > V.clear_depth;
+                    const uint32_t ai = talpha[wid][k];
+                    if (ai != 0xFFFFFFFFu && (__ballot(pass) & (0xFFFFull << (lane & 48u)))) {
+                        const bool ok = alpha_test_tile(p.tables, p.alpha[ai], ((T.keylo >> 7) & 1u) == 0,
+                                                        V.textures, w1, w2, lane);
+                        pass = pass && ok;
+                    }
+                    if (cov) covered++;
+                    if (pass) {
                         unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
                         unsigned long long* dst = V.keys + pixel_offset(W, x, y);
                         if (key > __atomic_load_n(dst, __ATOMIC_RELAXED)) atomicMax(dst, key);
@@ -861,6 +1021,7 @@ __device__ void bin_records(const DrawParams& p, bool valid, const TriRecord& re
 // Binning of the meshlet records, then clipping of the queued triangles
 // (one wave each; the fan triangles go straight into the bins from the
 // wave's registers).
+template <bool Alpha>
 __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
     __shared__ BinScratch s_bins[4];
     const uint32_t wid = threadIdx.x / kWave;
@@ -881,7 +1042,7 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
     for (uint32_t q = first; q < nclip; q += waves) {
         TriRecord rec;
         uint32_t ridx = 0;
-        const bool valid = clip_one(p, q, lane, rec, ridx);
+        const bool valid = clip_one<Alpha>(p, q, lane, rec, ridx);
         if (__ballot(valid)) bin_records(p, valid, rec, ridx, lane, s_bins[wid], entries);
     }
     if (counting(p) && lane == 0 && entries) atomicAdd(&p.counters->bin_entries, entries);
@@ -902,9 +1063,10 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
 // L2 by remapping block b to the (b/8)-th bin of the (b%8)-th eighth of the
 // range measured 4% slower: consecutive blocks then run far-apart bins, and
 // the records they share were already in the MALL.)
-template <int BS, int kThreads>
+template <int BS, int kThreads, bool Alpha>
 __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long (&keys)[BS * BS],
-                           TriLds (&tl)[kThreads], uint32_t (&s_wave_total)[kThreads / kWave]) {
+                           TriLds (&tl)[kThreads], uint32_t (&s_wave_total)[kThreads / kWave],
+                           uint32_t (&talpha)[Alpha ? kThreads : 1]) {
     constexpr int kWaves = kThreads / kWave;
     constexpr int kGroups = kThreads / 16;
     const ViewParams& V = view_of(p, view_of_bin(p, bin));
@@ -947,6 +1109,7 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
             load_tri(p.records[r], hw, hh, t, bmin, bmax);
             uint32_t cx0 = max((uint32_t)lo16(bmin), x0), cy0 = max((uint32_t)hi16(bmin), y0);
             uint32_t cx1 = min((uint32_t)lo16(bmax), x1), cy1 = min((uint32_t)hi16(bmax), y1);
+            if constexpr (Alpha) talpha[tid] = (p.records[r].flags & kRecAlpha) ? r : 0xFFFFFFFFu;
             t.org = 0;
             t.tw = 1;
             if (cx1 > cx0 && cy1 > cy0) {
@@ -986,6 +1149,8 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
                 if (tl[k + step].start <= j) k += step;
             }
             TriLds T = tl[k];
+            uint32_t ai = 0xFFFFFFFFu;
+            if constexpr (Alpha) ai = talpha[k];
             uint32_t next = (k + 1 < (uint32_t)kThreads) ? tl[k + 1].start : 0xFFFFFFFFu;
             uint32_t pt = j - T.start;
             uint32_t ty = pt / T.tw, tx = pt - ty * T.tw;
@@ -1015,18 +1180,37 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
                         next = (k + 1 < (uint32_t)kThreads) ? tl[k + 1].start : 0xFFFFFFFFu;
                     } while (j >= next);
                     T = tl[k];
+                    if constexpr (Alpha) ai = talpha[k];
                     tx = ty = 0;
                     enter();
                 }
                 // DrawTriangle's coverage test and depth (Rasterizer.h:281-296)
-                if ((int32_t)(w0 | w1 | w2) >= 0) {
+                if constexpr (!Alpha) {
+                    if ((int32_t)(w0 | w1 | w2) >= 0) {
+                        const float depth = __builtin_fmaf((float)(int32_t)w1, T.z10,
+                                                           __builtin_fmaf((float)(int32_t)w2, T.z20, T.z0));
+                        covered++;
+                        if (depth > V.clear_depth) {
+                            passed++;
+                            // an LDS max without a returned value: no wait on the
+                            // old key (a read-then-max would stall every step)
+                            const unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
+                            atomicMax(&keys[li], key);
+                        }
+                    }
+                } else {
+                    const bool cov = (int32_t)(w0 | w1 | w2) >= 0;
                     const float depth = __builtin_fmaf((float)(int32_t)w1, T.z10,
                                                        __builtin_fmaf((float)(int32_t)w2, T.z20, T.z0));
-                    covered++;
-                    if (depth > V.clear_depth) {
+                    bool pass = cov && depth > V.clear_depth;
+                    if (ai != 0xFFFFFFFFu && (__ballot(pass) & (0xFFFFull << (lane & 48u)))) {
+                        const bool ok = alpha_test_tile(p.tables, p.alpha[ai], ((T.keylo >> 7) & 1u) == 0,
+                                                        V.textures, (int32_t)w1, (int32_t)w2, lane);
+                        pass = pass && ok;
+                    }
+                    if (cov) covered++;
+                    if (pass) {
                         passed++;
-                        // an LDS max without a returned value: no wait on the
-                        // old key (a read-then-max would stall every step)
                         const unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
                         atomicMax(&keys[li], key);
                     }
@@ -1072,12 +1256,13 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
     }
 }
 
-template <int BS, int kThreads>
+template <int BS, int kThreads, bool Alpha>
 __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
     __shared__ unsigned long long keys[BS * BS];
     __shared__ TriLds tl[kThreads];
     __shared__ uint32_t s_wave_total[kThreads / kWave];
-    raster_bin<BS, kThreads>(p, blockIdx.x, keys, tl, s_wave_total);
+    __shared__ uint32_t talpha[Alpha ? kThreads : 1];
+    raster_bin<BS, kThreads, Alpha>(p, blockIdx.x, keys, tl, s_wave_total, talpha);
 }
 
 __global__ void k_begin_draw(DrawScratch* s, uint32_t* bin_counts, uint32_t num_bins) {
@@ -1146,22 +1331,30 @@ void launch_front(const DrawParams& p, hipStream_t s, Profiler* prof, bool separ
     }
     if (p.total_meshlets) {
         Profiler::Scope sc(prof, "mesh_setup", s);
-        hipLaunchKernelGGL(k_mesh_setup, dim3(min((p.total_meshlets + 3) / 4, resident_blocks<k_mesh_setup, 256>())),
-                           dim3(256), 0, s,
-                           p);
+        const uint32_t want = (p.total_meshlets + 3) / 4;
+        if (p.alpha)
+            hipLaunchKernelGGL(k_mesh_setup<true>, dim3(min(want, resident_blocks<k_mesh_setup<true>, 256>())),
+                               dim3(256), 0, s, p);
+        else
+            hipLaunchKernelGGL(k_mesh_setup<false>, dim3(min(want, resident_blocks<k_mesh_setup<false>, 256>())),
+                               dim3(256), 0, s, p);
     }
     if (separate_clip) {
         Profiler::Scope sc(prof, "clip", s);
-        hipLaunchKernelGGL(k_clip, dim3(256), dim3(256), 0, s, p);
+        if (p.alpha) hipLaunchKernelGGL(k_clip<true>, dim3(256), dim3(256), 0, s, p);
+        else hipLaunchKernelGGL(k_clip<false>, dim3(256), dim3(256), 0, s, p);
     }
 }
 
 }  // namespace
 
 void warm_launch_shapes() {
-    (void)resident_blocks<k_mesh_setup, 256>();
-    (void)resident_blocks<k_raster_direct, 256>();
-    (void)resident_blocks<k_bin_scatter, 256>();
+    (void)resident_blocks<k_mesh_setup<false>, 256>();
+    (void)resident_blocks<k_mesh_setup<true>, 256>();
+    (void)resident_blocks<k_raster_direct<false>, 256>();
+    (void)resident_blocks<k_raster_direct<true>, 256>();
+    (void)resident_blocks<k_bin_scatter<false>, 256>();
+    (void)resident_blocks<k_bin_scatter<true>, 256>();
 }
 
 hipError_t launch_clear(unsigned long long* keys, uint32_t n, unsigned long long key, hipStream_t s) {
@@ -1191,8 +1384,13 @@ hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s, Profiler* prof
     launch_front(p, s, prof, true);
     {
         Profiler::Scope sc(prof, "raster_direct", s);
-        uint32_t blocks = min(max((2 * p.total_meshlets + 3) / 4 + 16, 64u), resident_blocks<k_raster_direct, 256>());
-        hipLaunchKernelGGL(k_raster_direct, dim3(blocks), dim3(256), 0, s, p, 0);
+        const uint32_t want = max((2 * p.total_meshlets + 3) / 4 + 16, 64u);
+        if (p.alpha)
+            hipLaunchKernelGGL(k_raster_direct<true>, dim3(min(want, resident_blocks<k_raster_direct<true>, 256>())),
+                               dim3(256), 0, s, p, 0);
+        else
+            hipLaunchKernelGGL(k_raster_direct<false>, dim3(min(want, resident_blocks<k_raster_direct<false>, 256>())),
+                               dim3(256), 0, s, p, 0);
     }
     return hipGetLastError();
 }
@@ -1202,17 +1400,28 @@ hipError_t launch_draw_binned(const DrawParams& p, hipStream_t s, Profiler* prof
     {
         Profiler::Scope sc(prof, "bin_scatter", s);
         // about one chunk per wave: two per meshlet plus room for clipped records
-        uint32_t blocks = min(max((2 * p.total_meshlets + 3) / 4 + 16, 64u), resident_blocks<k_bin_scatter, 256>());
-        hipLaunchKernelGGL(k_bin_scatter, dim3(blocks), dim3(256), 0, s, p);
+        const uint32_t want = max((2 * p.total_meshlets + 3) / 4 + 16, 64u);
+        if (p.alpha)
+            hipLaunchKernelGGL(k_bin_scatter<true>, dim3(min(want, resident_blocks<k_bin_scatter<true>, 256>())),
+                               dim3(256), 0, s, p);
+        else
+            hipLaunchKernelGGL(k_bin_scatter<false>, dim3(min(want, resident_blocks<k_bin_scatter<false>, 256>())),
+                               dim3(256), 0, s, p);
     }
     {
         Profiler::Scope sc(prof, "bin_raster", s);
         const dim3 grid(p.total_bins);
         const bool small = p.raster_threads == 128;
-#define GLIMP_BIN_CASE(BS)                                                                        \
-    case BS:                                                                                      \
-        if (small) hipLaunchKernelGGL((k_bin_raster<BS, 128>), grid, dim3(128), 0, s, p);         \
-        else hipLaunchKernelGGL((k_bin_raster<BS, 256>), grid, dim3(256), 0, s, p);               \
+        // records of alpha-tested materials need the variant with the alpha test
+#define GLIMP_BIN_CASE(BS)                                                                             \
+    case BS:                                                                                           \
+        if (p.alpha) {                                                                                 \
+            if (small) hipLaunchKernelGGL((k_bin_raster<BS, 128, true>), grid, dim3(128), 0, s, p);    \
+            else hipLaunchKernelGGL((k_bin_raster<BS, 256, true>), grid, dim3(256), 0, s, p);          \
+        } else {                                                                                       \
+            if (small) hipLaunchKernelGGL((k_bin_raster<BS, 128, false>), grid, dim3(128), 0, s, p);   \
+            else hipLaunchKernelGGL((k_bin_raster<BS, 256, false>), grid, dim3(256), 0, s, p);         \
+        }                                                                                              \
         break;
         switch (p.bin_size) {
             GLIMP_BIN_CASE(8)
@@ -1225,7 +1434,8 @@ hipError_t launch_draw_binned(const DrawParams& p, hipStream_t s, Profiler* prof
     }
     {
         Profiler::Scope sc(prof, "raster_spill", s);
-        hipLaunchKernelGGL(k_raster_direct, dim3(256), dim3(256), 0, s, p, 1);
+        if (p.alpha) hipLaunchKernelGGL(k_raster_direct<true>, dim3(256), dim3(256), 0, s, p, 1);
+        else hipLaunchKernelGGL(k_raster_direct<false>, dim3(256), dim3(256), 0, s, p, 1);
     }
     return hipGetLastError();
 }

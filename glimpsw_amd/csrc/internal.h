@@ -49,14 +49,34 @@ struct TriRecord {
     uint32_t pos[3];               // snapped 1/16 px, x | y << 16 (s16 each)
     float z0, z10, z20;            // depth plane (Rasterizer.cpp:321-323)
     uint32_t rank;                 // draw order, see key layout
-    uint32_t flags;                // bits 0-1 fragment entry point, bits 16-31 view index
+    uint32_t flags;                // bit 0 alpha-tested (kRecAlpha), bits 16-31 view index
 };
 static_assert(sizeof(TriRecord) == 32, "record size");
+
+// What the alpha test of FS_EncodeSurfaceId<true> (Shading.cpp:309-331)
+// needs of a record beyond coverage and depth, 64 bytes, at the record's own
+// index in a side array that exists only for draws with alpha-tested
+// materials: the 1/w terms of the perspective correction (W0 and W*/area,
+// TriangleEdgeVars::Setup, Rasterizer.cpp:324-328), the vertex UVs (half2
+// each), the texture and cutoff, and for clipped triangles the remap of the
+// fan triangle's barycentrics onto the source triangle (TriangleClipData
+// ClippedU/V, Rasterizer.cpp:461-464).
+struct AlphaRecord {
+    float w0, w0s, w1s, w2s;
+    uint32_t uv[3];
+    uint32_t tex;                  // texture index | alpha cutoff << 24
+    float ru[3], rv[3];
+    uint32_t pad[2];
+};
+static_assert(sizeof(AlphaRecord) == 64, "alpha record size");
+
+// TriRecord::flags bit 0: the record is alpha-tested.
+constexpr uint32_t kRecAlpha = 1u;
 
 // A triangle that needs guard-band clipping, with its clip-space vertices.
 struct ClipEntry {
     uint32_t rank_base;            // rank with fan index 0
-    uint32_t info;                 // outcodes | cull_mode << 8 | fs_id << 10
+    uint32_t info;                 // outcodes | cull_mode << 8 | alpha-tested << 10
     float v[3][4];
     uint32_t view;                 // view index within the launch
     uint32_t pad;
@@ -113,6 +133,7 @@ struct ViewParams {
     DevMeshlets m;
     const DevMaterial* materials;
     uint32_t num_materials;
+    const DevTexture* textures;    // the scene's textures (alpha test)
     uint32_t meshlet_offset;       // ShadingContext::MeshletOffset
     uint32_t meshlet_count;
     uint32_t meshlet_base;         // first global meshlet slot of the view
@@ -146,6 +167,7 @@ struct DrawParams {
     ApproxTables tables;
     uint2* visible_list;           // (global meshlet number, view) of each meshlet that passed the cull
     TriRecord* records;            // [total_meshlets * 128 + clip_record_capacity]
+    AlphaRecord* alpha;            // same indexing as records; null when no view has alpha-tested materials
     uint32_t* meshlet_counts;      // [total_meshlets] records kept per meshlet
     uint32_t clip_record_base;     // = total_meshlets * 128
     uint32_t clip_record_capacity;

@@ -135,3 +135,18 @@ def noise_texture(size: int, seed: int) -> Texture:
     bits = rng.next_u64(size * size // 2)
     img = bits.view(np.uint32).reshape(size, size) | np.uint32(0xFF000000)
     return create([img])
+
+
+def cutout_texture(size: int, seed: int, period: int = 16) -> Texture:
+    """Seeded noise colour with a smooth alpha pattern for the alpha test:
+    alpha = 255 * (1/2 + 1/2 sin(2 pi x/period) sin(2 pi y/period)), rounded,
+    so every alpha value occurs and a cutoff carves holes with texel-scale
+    gradients around them."""
+    from .scene import SplitMix64
+
+    rng = SplitMix64(seed)
+    rgb = rng.next_u64(size * size // 2).view(np.uint32).reshape(size, size) & np.uint32(0x00FFFFFF)
+    y, x = np.meshgrid(np.arange(size, dtype=np.float64), np.arange(size, dtype=np.float64), indexing="ij")
+    a = np.rint(255.0 * (0.5 + 0.5 * np.sin(2 * np.pi * x / period) * np.sin(2 * np.pi * y / period)))
+    img = rgb | (a.astype(np.uint32) << np.uint32(24))
+    return create([img.astype(np.uint32)])

@@ -139,6 +139,14 @@ template <class MeshFn, class... FragFns>
 struct PipelineOf {
     static_assert(sizeof(MeshFn) == 0, "no device pipeline is built for this shader combination");
 };
+// ShadingContext::VisBufferShader (Shading.cpp:650-654): meshlets whose
+// material has an alpha cutoff below 255 take FS_EncodeSurfaceId<true>.
+template <>
+struct PipelineOf<ShadeMeshlet, FS_EncodeSurfaceId<false>, FS_EncodeSurfaceId<true>> {
+    static constexpr uint32_t value = GLIMP_PIPELINE_VISBUFFER;
+};
+// The table without the alpha-tested entry: in the reference a meshlet with
+// FragmentShaderId 1 has no function in it; here it takes the alpha test.
 template <>
 struct PipelineOf<ShadeMeshlet, FS_EncodeSurfaceId<false>> {
     static constexpr uint32_t value = GLIMP_PIPELINE_VISBUFFER;

@@ -1030,7 +1030,9 @@ void orc_draw_meshlets(orc_framebuffer* fb, uint32_t count, const orc_shading_ct
             FragmentCtx f = fc;
             if (f.alpha_test) {
                 for (int k = 0; k < 3; k++) {
-                    uint32_t vi = mesh.Indices[k][prim];
+                    // indices of a well-formed meshlet are below 64; the mask
+                    // keeps a malformed one inside the meshlet's 64 slots
+                    uint32_t vi = mesh.Indices[k][prim] & 63u;
                     uint32_t tc = ctx->meshlets->texcoords[(size_t)meshletId * 64 + vi];
                     f.uv[k][0] = half_to_float((uint16_t)(tc & 0xFFFF));
                     f.uv[k][1] = half_to_float((uint16_t)(tc >> 16));

@@ -91,7 +91,7 @@ int main(int argc, char** argv) {
 
         auto fb = CreateFramebuffer(gpu, W, H);
         ShadingContext shader(scene);
-        const auto& dispatch = GetDispatchTable<ShadeMeshlet, FS_EncodeSurfaceId<false>>();
+        const auto& dispatch = GetDispatchTable<ShadeMeshlet, FS_EncodeSurfaceId<false>, FS_EncodeSurfaceId<true>>();
         fb.Clear(0, 0.0f);
         shader.MeshletOffset = 0;
         shader.UpdateProj(clip);

@@ -78,3 +78,15 @@ def fullscreen_grid(width: int, height: int, cells: int = 7, jitter_seed: int = 
     bounds[:, :3] = positions.mean(axis=2)
     bounds[:, 3] = 1e6
     return dataclasses.replace(sc, positions=positions, bounds=bounds)
+
+
+def alpha_scene(n: int = 512, seed_offset: int = 0, spread: float = 10.0, near_frac: float = 0.0,
+                cutoffs=(128, 200), tex_size: int = 128):
+    """small_scene with alpha-tested materials: patch i uses material i % 3,
+    0 opaque (checker), 1 double-sided cutout with cutoffs[0], 2 single-sided
+    cutout with cutoffs[1] (a cutoff of 255 turns the alpha test off)."""
+    sc = small_scene(n, seed_offset, spread, near_frac)
+    mats = [(0, False, 255), (1, True, cutoffs[0]), (1, False, cutoffs[1])]
+    texs = [sc.textures[0], textures.cutout_texture(tex_size, S.SEED_BASE + 700 + seed_offset)]
+    mid = (np.arange(n) % 3).astype(np.uint32)
+    return dataclasses.replace(sc, material_id=mid, materials=mats, textures=texs)
