@@ -44,6 +44,44 @@ __device__ __forceinline__ float lerpf(float a, float b, float t) {
     return __builtin_fmaf(t, b, __builtin_fmaf(-t, a, a));
 }
 
+// normalize(mul(ObjectToWorldMat, BaryLerp(bary, a0, a1, a2)))
+// (Shading.cpp:546-554, BaryLerp Rasterizer.h:101-104)
+__device__ __forceinline__ void lerp_to_world(const ApproxTables& T, const float a[3][3], const float bary[3],
+                                              const float* M, float out[3]) {
+    float ai[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+        ai[j] = __builtin_fmaf(a[0][j], bary[0], __builtin_fmaf(a[1][j], bary[1], a[2][j] * bary[2]));
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+        out[j] = __builtin_fmaf(ai[0], M[j], __builtin_fmaf(ai[1], M[3 + j], ai[2] * M[6 + j]));
+    normalize3(T, out);
+}
+
+// Normal mapping of ResolveSurface (Shading.cpp:556-572): bitangent =
+// cross(N, T) with the handedness sign, tangent-space normal from the NMR
+// texel's two 8-bit channels with z = approx_sqrt(1 - x^2 - y^2), result
+// normalize(xT + yB + zN), in the operation order of the reference's
+// optimised build (`c * (1/127.5) - 1` and the sums contracted into FMAs,
+// 1 - (x^2 + y^2) evaluated as 1 + fma(-x, x, -y^2)).
+__device__ __forceinline__ void normal_map(const ApproxTables& T, float n[3], const float t[3], uint32_t handed,
+                                           uint32_t nmr) {
+    float b[3] = { __builtin_fmaf(n[1], t[2], -(n[2] * t[1])), __builtin_fmaf(n[2], t[0], -(n[0] * t[2])),
+                   __builtin_fmaf(n[0], t[1], -(n[1] * t[0])) };
+#pragma unroll
+    for (int k = 0; k < 3; k++) b[k] = bitsf(fbits(b[k]) ^ handed);
+    const float sx = __builtin_fmaf((float)(nmr & 255u), 1.0f / 127.5f, -1.0f);
+    const float sy = __builtin_fmaf((float)((nmr >> 8) & 255u), 1.0f / 127.5f, -1.0f);
+    const float s = 1.0f + __builtin_fmaf(-sx, sx, -(sy * sy));
+    const float sz = rsqrt14(T, s) * s;
+    float r[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) r[k] = __builtin_fmaf(sz, n[k], __builtin_fmaf(sy, b[k], sx * t[k]));
+    const float inv = rsqrt14(T, __builtin_fmaf(r[0], r[0], __builtin_fmaf(r[1], r[1], r[2] * r[2])));
+#pragma unroll
+    for (int k = 0; k < 3; k++) n[k] = r[k] * inv;
+}
+
 __device__ __forceinline__ uint32_t pack_rgba8(float r, float g, float b, float a) {  // Texture.h:55-67
     float c[4] = { r, g, b, a };
     uint32_t out = 0;
@@ -57,6 +95,8 @@ __device__ __forceinline__ uint32_t pack_rgba8(float r, float g, float b, float 
     return out;
 }
 
+// UnpackNormalTangent (Shading.cpp:232-236): normal from bytes 0-1, tangent
+// (shift 16) from bytes 2-3.
 __device__ __forceinline__ void unpack_normal(const ApproxTables& T, uint32_t packed, float n[3]) {
     const float scale = 1.0f / 255;
     float u = (float)(packed & 255) * scale, v = (float)((packed >> 8) & 255) * scale;
@@ -186,8 +226,13 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
             done = true;
         }
     }
-    // SurfaceSampler: Repeat, Mag Linear, Min Nearest (Shading.cpp:6-10)
-    if (myTex) albedo = sample_level(*myTex, U, V, myMip, myLinear);
+    // SurfaceSampler: Repeat, Mag Linear, Min Nearest (Shading.cpp:6-10);
+    // layer 1 holds normal xy, metallic and roughness (Shading.cpp:541-543)
+    uint32_t nmr = 0;
+    if (myTex) {
+        albedo = sample_level(*myTex, U, V, myMip, myLinear);
+        if (p.mode != 0 && myTex->num_layers >= 2) nmr = sample_level(*myTex, U, V, myMip, myLinear, 1);
+    }
 
     uint32_t outc;
     if (p.mode == 0) {
@@ -210,22 +255,31 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
                 uint32_t vi = I[k * 128 + tri];
                 unpack_normal(T, p.m.normal_tangents[(size_t)mi * 64 + vi], nv[k]);
             }
-            float ni[3];
+            lerp_to_world(T, nv, bary, p.object_to_world, N);
+        }
+        // normal mapping when a lane of the tile has a normal texel
+        if (__ballot((nmr & 0xFFFFu) != 0) & groupMask) {
+            if (active) {
+                const uint8_t* I = p.m.indices + (size_t)mi * 384;
+                float tv[3][3], Tw[3];
 #pragma unroll
-            for (int j = 0; j < 3; j++)
-                ni[j] = __builtin_fmaf(nv[0][j], bary[0], __builtin_fmaf(nv[1][j], bary[1], nv[2][j] * bary[2]));
-            const float* M = p.object_to_world;
-#pragma unroll
-            for (int j = 0; j < 3; j++)
-                N[j] = __builtin_fmaf(ni[0], M[j], __builtin_fmaf(ni[1], M[3 + j], ni[2] * M[6 + j]));
-            normalize3(T, N);
+                for (int k = 0; k < 3; k++) unpack_normal(T, p.m.normal_tangents[(size_t)mi * 64 + I[k * 128 + tri]] >> 16, tv[k]);
+                lerp_to_world(T, tv, bary, p.object_to_world, Tw);
+                // handedness of the first vertex (Shading.cpp:501-503)
+                const uint32_t handed = ((p.m.tangent_handedness[mi] >> (I[tri] & 63u)) & 1u) << 31;
+                normal_map(T, N, Tw, handed, nmr);
+            }
+        }
+        if (active) {
             float c4[4];
 #pragma unroll
             for (int j = 0; j < 4; j++) c4[j] = xform(p.inv_screen_proj, j, (float)(int32_t)x, (float)(int32_t)y, depth);
             float rw = frcp(T, c4[3]);
             wpos[0] = c4[0] * rw; wpos[1] = c4[1] * rw; wpos[2] = c4[2] * rw;
             unpack_srgb(albedo, base);
-            const float metal = 0.0f, rough = 0.0f;
+            // SurfaceData::Metallic / Roughness (Shading.cpp:574-577), 0 without an NMR layer
+            const float metal = (float)((nmr >> 16) & 255u) * (1.0f / 255);
+            const float rough = (float)((nmr >> 24) & 255u) * (1.0f / 255);
             aR = fmaxf(rough * rough, 1e-4f);
             const float refl = 0.16f * 0.5f * 0.5f;
 #pragma unroll

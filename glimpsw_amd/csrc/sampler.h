@@ -30,12 +30,13 @@ __device__ __forceinline__ uint32_t lerp16(uint32_t a, uint32_t b, uint32_t t) {
 
 // Texture2D<RGBA8u>::SampleLevel with Repeat wrap (Texture.h:412-459,
 // 506-556); `linear` is the tile's filter choice.
-__device__ inline uint32_t sample_level(const DevTexture& tex, float u, float v, int32_t mipIn, bool linear) {
+__device__ inline uint32_t sample_level(const DevTexture& tex, float u, float v, int32_t mipIn, bool linear,
+                                       uint32_t layer = 0) {
     float su = u * tex.scale_lerp_u, sv = v * tex.scale_lerp_v;
     int32_t ix = cvt_rne(su) & tex.mask_lerp_u;
     int32_t iy = cvt_rne(sv) & tex.mask_lerp_v;
     int32_t m = min(max(mipIn, 0), (int32_t)tex.mip_levels - 1);
-    int32_t offset = (int32_t)tex.mip_offsets[m];
+    int32_t offset = (int32_t)(layer * tex.layer_stride + tex.mip_offsets[m]);
     int32_t stride = (int32_t)tex.row_shift - m;
     int32_t x = ix >> m, y = iy >> m;
     const uint32_t* D = tex.data;

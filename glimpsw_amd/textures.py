@@ -150,3 +150,24 @@ def cutout_texture(size: int, seed: int, period: int = 16) -> Texture:
     a = np.rint(255.0 * (0.5 + 0.5 * np.sin(2 * np.pi * x / period) * np.sin(2 * np.pi * y / period)))
     img = rgb | (a.astype(np.uint32) << np.uint32(24))
     return create([img.astype(np.uint32)])
+
+
+def nmr_texture(size: int, seed: int, period: int = 32) -> Texture:
+    """Two layers as the reference's material textures have them: layer 0
+    seeded noise albedo, layer 1 normal/metallic/roughness (bytes 0-1 the
+    tangent-space normal xy of a bump pattern, inside the unit disc; byte 2
+    metallic and byte 3 roughness, constant over 8x8 blocks)."""
+    from .scene import SplitMix64
+
+    rng = SplitMix64(seed)
+    albedo = rng.next_u64(size * size // 2).view(np.uint32).reshape(size, size) | np.uint32(0xFF000000)
+    y, x = np.meshgrid(np.arange(size, dtype=np.float64), np.arange(size, dtype=np.float64), indexing="ij")
+    nx = 0.6 * np.sin(2 * np.pi * x / period)
+    ny = 0.6 * np.cos(2 * np.pi * y / period)
+    bx = np.rint((nx + 1.0) * 127.5).astype(np.uint32)
+    by = np.rint((ny + 1.0) * 127.5).astype(np.uint32)
+    blocks = rng.next_u64((size // 8) * (size // 8)).astype(np.uint64)
+    mr = (blocks & np.uint64(0xFFFF)).astype(np.uint32).reshape(size // 8, size // 8)
+    mr = np.repeat(np.repeat(mr, 8, axis=0), 8, axis=1)
+    nmr = bx | (by << np.uint32(8)) | (mr << np.uint32(16))
+    return create([albedo.astype(np.uint32), nmr.astype(np.uint32)])

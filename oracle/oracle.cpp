@@ -699,7 +699,10 @@ struct SurfaceLane {
     uint32_t material;
     float bary[3];
     float nrm[3];
+    float tan[3];      // world-space tangent (lit resolve)
+    uint32_t handed;   // 1 << 31 when vertex 0's tangent handedness bit is set
     uint32_t albedo;
+    uint32_t nmr;      // layer 1 texel (normal xy, metallic, roughness), 0 without
 };
 
 // ResolveSurface (Shading.cpp:471-579) for one 4x4 tile: vertex fetch,
@@ -733,17 +736,52 @@ void unmap_octahedron(float u, float v, float n[3]) {
     n[0] = u;
     n[1] = v;
     n[2] = (1.0f - fabsf(u)) - fabsf(v);
-    float t = std::max(-n[2], 0.0f);
+    float t = fmaxf(-n[2], 0.0f);
     n[0] = n[0] - mulsign(t, n[0]);
     n[1] = n[1] - mulsign(t, n[1]);
     normalize3(n);
 }
 
-// UnpackNormalTangent (Shading.cpp:232-236), normal part.
+// UnpackNormalTangent (Shading.cpp:232-236): normal from bytes 0-1, tangent
+// from bytes 2-3.
 void unpack_normal(uint32_t packed, float n[3]) {
     float c[4];
     unpack_rgba8(packed, c);
     unmap_octahedron(c[0], c[1], n);
+}
+void unpack_tangent(uint32_t packed, float t[3]) {
+    float c[4];
+    unpack_rgba8(packed, c);
+    unmap_octahedron(c[2], c[3], t);
+}
+
+// normalize(mul(ObjectToWorldMat, BaryLerp(bary, a0, a1, a2)))
+// (Shading.cpp:546-554, BaryLerp Rasterizer.h:101-104)
+void lerp_to_world(const float a[3][3], const float bary[3], const float* M, float out[3]) {
+    float ai[3];
+    for (int j = 0; j < 3; j++) ai[j] = fmaf(a[0][j], bary[0], fmaf(a[1][j], bary[1], a[2][j] * bary[2]));
+    for (int j = 0; j < 3; j++) out[j] = fmaf(ai[0], M[0 * 3 + j], fmaf(ai[1], M[1 * 3 + j], ai[2] * M[2 * 3 + j]));
+    normalize3(out);
+}
+
+// Normal mapping of ResolveSurface (Shading.cpp:556-572) for one lane:
+// bitangent = cross(N, T) with the handedness sign (SIMD.h:431-437),
+// tangent-space normal from the texel's two 8-bit channels with z =
+// approx_sqrt(1 - x^2 - y^2) (SIMD.h:296), result normalize(xT + yB + zN).
+// The fast-math build contracts `c * (1/127.5) - 1` and the sums into FMAs
+// and evaluates 1 - (x^2 + y^2) as 1 + fma(-x, x, -y^2).
+void normal_map(const float n[3], const float t[3], uint32_t handed, uint32_t nmr, float out[3]) {
+    float b[3] = { fmaf(n[1], t[2], -(n[2] * t[1])), fmaf(n[2], t[0], -(n[0] * t[2])),
+                   fmaf(n[0], t[1], -(n[1] * t[0])) };
+    for (int k = 0; k < 3; k++) b[k] = bitsf(fbits(b[k]) ^ handed);
+    const float sx = fmaf((float)(nmr & 255), 1.0f / 127.5f, -1.0f);
+    const float sy = fmaf((float)((nmr >> 8) & 255), 1.0f / 127.5f, -1.0f);
+    const float s = 1.0f + fmaf(-sx, sx, -(sy * sy));
+    const float sz = rsqrt14(s) * s;
+    float r[3];
+    for (int k = 0; k < 3; k++) r[k] = fmaf(sz, n[k], fmaf(sy, b[k], sx * t[k]));
+    const float inv = rsqrt14(fmaf(r[0], r[0], fmaf(r[1], r[1], r[2] * r[2])));
+    for (int k = 0; k < 3; k++) out[k] = r[k] * inv;
 }
 
 void resolve_surface_tile(const orc_shading_ctx& ctx, const uint32_t* ids, const bool* active,
@@ -755,6 +793,8 @@ void resolve_surface_tile(const orc_shading_ctx& ctx, const uint32_t* ids, const
         SurfaceLane& L = lanes[l];
         L.active = active[l];
         L.albedo = 0;
+        L.nmr = 0;
+        L.handed = 0;
         L.material = UINT32_MAX;
         L.u = L.v = 0;
         gradU[l][0] = gradU[l][1] = gradV[l][0] = gradV[l][1] = 0;
@@ -778,16 +818,17 @@ void resolve_surface_tile(const orc_shading_ctx& ctx, const uint32_t* ids, const
         if (object_to_world) {
             // normalWS = normalize(mul(ObjectToWorldMat, BaryLerp(bary, n0, n1, n2)))
             // (Shading.cpp:546-552, BaryLerp Rasterizer.h:101-104)
-            float nv[3][3];
+            float nv[3][3], tv[3][3];
             for (int k = 0; k < 3; k++) {
                 uint32_t vi = ms.indices[(size_t)mi * 384 + k * 128 + tri];
                 unpack_normal(ms.normal_tangents[(size_t)mi * 64 + vi], nv[k]);
+                unpack_tangent(ms.normal_tangents[(size_t)mi * 64 + vi], tv[k]);
             }
-            float ni[3];
-            for (int j = 0; j < 3; j++) ni[j] = fmaf(nv[0][j], bary[0], fmaf(nv[1][j], bary[1], nv[2][j] * bary[2]));
-            const float* M = object_to_world;
-            for (int j = 0; j < 3; j++) L.nrm[j] = fmaf(ni[0], M[0 * 3 + j], fmaf(ni[1], M[1 * 3 + j], ni[2] * M[2 * 3 + j]));
-            normalize3(L.nrm);
+            lerp_to_world(nv, bary, object_to_world, L.nrm);
+            lerp_to_world(tv, bary, object_to_world, L.tan);
+            // handedness of the first vertex (Shading.cpp:501-503)
+            uint32_t v0 = ms.indices[(size_t)mi * 384 + tri] & 63u;
+            if ((ms.tangent_handedness[mi] >> v0) & 1u) L.handed = 1u << 31;
         }
         float t10x = uv[1][0] - uv[0][0], t10y = uv[1][1] - uv[0][1];
         float t20x = uv[2][0] - uv[0][0], t20y = uv[2][1] - uv[0][1];
@@ -822,14 +863,28 @@ void resolve_surface_tile(const orc_shading_ctx& ctx, const uint32_t* ids, const
             U[l] = lanes[l].u;
             V[l] = lanes[l].v;
         }
-        uint32_t res[16];
+        uint32_t res[16], nmr[16] = {};
         // SurfaceSampler: Repeat, Mag Linear, Min Nearest (Shading.cpp:6-10)
         sample_level16(tex, U, V, 0, mip, 0, true, false, active, res);
+        // the normal/metallic/roughness layer (Shading.cpp:541-543)
+        if (object_to_world && tex->num_layers >= 2) sample_level16(tex, U, V, 1, mip, 0, true, false, active, nmr);
         for (int l = 0; l < 16; l++) {
             if (active[l] && lanes[l].material == mat) {
                 lanes[l].albedo = res[l];
+                lanes[l].nmr = nmr[l];
                 done[l] = true;
             }
+        }
+    }
+    // normal mapping when a lane of the tile has a normal texel
+    // (Shading.cpp:556-572)
+    bool anyNormal = false;
+    for (int l = 0; l < 16; l++) anyNormal |= (lanes[l].nmr & 0xFFFFu) != 0;
+    if (object_to_world && anyNormal) {
+        for (int l = 0; l < 16; l++) {
+            if (!active[l]) continue;
+            float n[3] = { lanes[l].nrm[0], lanes[l].nrm[1], lanes[l].nrm[2] };
+            normal_map(n, lanes[l].tan, lanes[l].handed, lanes[l].nmr, lanes[l].nrm);
         }
     }
 }
@@ -853,19 +908,22 @@ void unpack_srgb(uint32_t packed, float c[4]) {
 
 inline float lerpf(float a, float b, float t) { return fmaf(t, b, fmaf(-t, a, a)); }  // SIMD.h:446
 
+// Float min/max/clamp are maxnum/minnum (SIMD.h:319-331): a NaN operand
+// yields the other one, hence fmaxf/fminf rather than std::max/min.
+
 float light_attenuation(const orc_light& L, const float pos[3]) {  // Shading.cpp:581-600
     if (L.type == 0) return 1.0f;
     float d[3] = { L.position[0] - pos[0], L.position[1] - pos[1], L.position[2] - pos[2] };
     float d2 = dot3(d, d);
     float factor = d2 * L.inv_radius_sq;
-    float smooth = std::max(1.0f - factor * factor, 0.0f);
-    float att = (smooth * smooth) * rcp14(std::max(d2, 1e-4f));
+    float smooth = fmaxf(1.0f - factor * factor, 0.0f);
+    float att = (smooth * smooth) * rcp14(fmaxf(d2, 1e-4f));
     if (L.type == 2) {
         float nd[3] = { d[0], d[1], d[2] };
         normalize3(nd);
         float negDir[3] = { -L.direction[0], -L.direction[1], -L.direction[2] };
         float cd = dot3(negDir, nd);
-        float sa = std::min(std::max(cd * L.spot_scale + L.spot_offset, 0.0f), 1.0f);
+        float sa = fminf(fmaxf(cd * L.spot_scale + L.spot_offset, 0.0f), 1.0f);
         att *= sa * sa;
     }
     return att;
@@ -888,9 +946,10 @@ void lit_tile(const SurfaceLane* lanes, const bool* active, uint32_t x0, uint32_
         wpos[l][0] = c[0] * rw; wpos[l][1] = c[1] * rw; wpos[l][2] = c[2] * rw;
         float srgb[4];
         unpack_srgb(lanes[l].albedo, srgb);
-        metal[l] = 0.0f;                       // no metallic/roughness layer
-        float rough = 0.0f;
-        aR[l] = std::max(rough * rough, 1e-4f);
+        // SurfaceData::Metallic / Roughness (Shading.cpp:574-577), 0 without an NMR layer
+        metal[l] = (float)((lanes[l].nmr >> 16) & 255) * (1.0f / 255);
+        float rough = (float)((lanes[l].nmr >> 24) & 255) * (1.0f / 255);
+        aR[l] = fmaxf(rough * rough, 1e-4f);
         const float refl = 0.16f * 0.5f * 0.5f;
         for (int k = 0; k < 3; k++) {
             base[l][k] = srgb[k];
@@ -930,8 +989,8 @@ void lit_tile(const SurfaceLane* lanes, const bool* active, uint32_t x0, uint32_
             if (!active[l]) continue;
             float Hv[3] = { V[l][0] + Ld[l][0], V[l][1] + Ld[l][1], V[l][2] + Ld[l][2] };
             normalize3(Hv);
-            float NoH = std::min(std::max(dot3(N[l], Hv), 0.0f), 1.0f);
-            float LoH = std::min(std::max(dot3(Ld[l], Hv), 0.0f), 1.0f);
+            float NoH = fminf(fmaxf(dot3(N[l], Hv), 0.0f), 1.0f);
+            float LoH = fminf(fmaxf(dot3(Ld[l], Hv), 0.0f), 1.0f);
             // D_GGX, V_SmithGGXCorrelatedFast, F_Schlick (Shading.cpp:19-32)
             float a = NoH * aR[l];
             float k = aR[l] * rcp14((1.0f - NoH * NoH) + a * a);
@@ -941,7 +1000,7 @@ void lit_tile(const SurfaceLane* lanes, const bool* active, uint32_t x0, uint32_
             float Vis = fdiv(0.5f, lerpf(va, vb, aR[l]));
             float u1 = 1.0f - LoH;
             float f = (u1 * u1) * (u1 * u1) * u1;
-            float w = std::max(NoL[l] * att[l], 0.0f);
+            float w = fmaxf(NoL[l] * att[l], 0.0f);
             for (int c = 0; c < 3; c++) {
                 float F = f + f0[l][c] * (1.0f - f);
                 float Fr = (D * Vis) * F;
@@ -1334,6 +1393,14 @@ void orc_fragment16(const int32_t* e1, const int32_t* e2, const float* coef, flo
         float u = (float)e1[l], v = (float)e2[l];
         depth[l] = fmaf(u, coef[1], fmaf(v, coef[2], coef[0]));
         fragment_bary(u, v, coef[3], coef[4], coef[5], coef[6], bu[l], bv[l]);
+    }
+}
+
+void orc_normal_map16(const float* n, const float* t, const uint32_t* hand, const uint32_t* nmr, float* out) {
+    for (int l = 0; l < 16; l++) {
+        float nn[3] = { n[l], n[16 + l], n[32 + l] }, tt[3] = { t[l], t[16 + l], t[32 + l] }, o[3];
+        normal_map(nn, tt, hand[l], nmr[l], o);
+        for (int k = 0; k < 3; k++) out[k * 16 + l] = o[k];
     }
 }
 

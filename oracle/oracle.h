@@ -137,6 +137,10 @@ void orc_setup16(const float* clip /*[3][4][16]*/, int32_t w, int32_t h, int32_t
  * (Rasterizer.h:293-319) from the edge values E1, E2 and
  * coef = {Z0, Z10, Z20, W0, W0S, W1S, W2S}. */
 void orc_fragment16(const int32_t* e1, const int32_t* e2, const float* coef, float* depth, float* bu, float* bv);
+/* Normal mapping of the lit resolve (Shading.cpp:556-572) for 16 lanes:
+ * n, t = world-space normal and tangent [3][16], hand = 0 or 1<<31,
+ * nmr = layer-1 texels; out [3][16]. */
+void orc_normal_map16(const float* n, const float* t, const uint32_t* hand, const uint32_t* nmr, float* out);
 /* The alpha test's UVs and implicit mip levels for one 4x4 tile
  * (FS_EncodeSurfaceId<true>, Shading.cpp:316-327; SampleImplicitLod,
  * Texture.h:403-410): remap = {ClippedU[3], ClippedV[3]} or NULL,

@@ -148,6 +148,36 @@ void probe_alpha_uv(const int32_t* e1, const int32_t* e2, const float* coef, con
     sti(mip, m - 8);
 }
 
+// Normal mapping of the resolve (Shading.cpp:556-572): bitangent = cross(N, T)
+// (SIMD.h:431-437) with the handedness sign, tangent-space normal from the
+// two 8-bit channels of the NMR texel with z = approx_sqrt(1 - x^2 - y^2)
+// (SIMD.h:296), then normalize(x*T + y*B + z*N) (SIMD.h:439).
+// in: n[3][16], t[3][16] (world-space normal and tangent), hand[16] (0 or
+// 1<<31), nmr[16]; out[3][16]
+void probe_normal_map(const float* n, const float* t, const int32_t* hand, const int32_t* nmr, float* out) {
+    vf nx = ld(n), ny = ld(n + 16), nz = ld(n + 32);
+    vf tx = ld(t), ty = ld(t + 16), tz = ld(t + 32);
+    vf bx = vfma(ny, tz, -nz * ty);
+    vf by = vfma(nz, tx, -nx * tz);
+    vf bz = vfma(nx, ty, -ny * tx);
+    vi h = ldi(hand);
+    bx = (vf)((vi)bx ^ h);
+    by = (vf)((vi)by ^ h);
+    bz = (vf)((vi)bz ^ h);
+    vi p = ldi(nmr);
+    vf sx = __builtin_convertvector(p & 255, vf) * (1.0f / 127.5f) - 1.0f;
+    vf sy = __builtin_convertvector((p >> 8) & 255, vf) * (1.0f / 127.5f) - 1.0f;
+    vf s = 1.0f - (sx * sx + sy * sy);
+    vf sz = (vf)_mm512_rsqrt14_ps((__m512)s) * s;
+    vf rx = sx * tx + sy * bx + sz * nx;
+    vf ry = sx * ty + sy * by + sz * ny;
+    vf rz = sx * tz + sy * bz + sz * nz;
+    vf r = (vf)_mm512_rsqrt14_ps((__m512)vfma(rx, rx, vfma(ry, ry, rz * rz)));
+    st(out, rx * r);
+    st(out + 16, ry * r);
+    st(out + 32, rz * r);
+}
+
 // Screen UV of a 4x4 tile in a full-screen pass (Rasterizer.h:226-237).
 void probe_tile_uv(const int32_t* px, int n, float size, float* out) {
     float scale = 2.0f / size;

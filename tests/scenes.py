@@ -90,3 +90,20 @@ def alpha_scene(n: int = 512, seed_offset: int = 0, spread: float = 10.0, near_f
     texs = [sc.textures[0], textures.cutout_texture(tex_size, S.SEED_BASE + 700 + seed_offset)]
     mid = (np.arange(n) % 3).astype(np.uint32)
     return dataclasses.replace(sc, material_id=mid, materials=mats, textures=texs)
+
+
+def normal_mapped_scene(n: int = 384, seed_offset: int = 0, spread: float = 6.0, near_frac: float = 0.0,
+                        mixed: bool = False):
+    """small_scene whose material texture has a normal/metallic/roughness
+    layer, with seeded tangent handedness bits; `mixed` alternates it with
+    the one-layer checker so tiles see both kinds."""
+    sc = small_scene(n, seed_offset, spread, near_frac)
+    rng = S.SplitMix64(S.SEED_BASE + 1300 + seed_offset)
+    nmr = textures.nmr_texture(128, S.SEED_BASE + 1400 + seed_offset)
+    if mixed:
+        mats, texs = [(0, False, 255), (1, False, 255)], [nmr, sc.textures[0]]
+        mid = (np.arange(n) % 2).astype(np.uint32)
+    else:
+        mats, texs, mid = [(0, False, 255)], [nmr], np.zeros(n, np.uint32)
+    return dataclasses.replace(sc, materials=mats, textures=texs, material_id=mid,
+                               tangent_handedness=rng.next_u64(n).astype(np.uint64))

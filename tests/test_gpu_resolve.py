@@ -95,3 +95,13 @@ def test_resolve_without_draw_is_background(gpu_ctx):
     assert (got == want).all()
     fb.close()
     ds.close()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_normal_mapped_scene(gpu_ctx, mixed):
+    """Textures with a normal/metallic/roughness layer (ResolveSurface,
+    Shading.cpp:541-577), alone and alternating with one-layer textures in
+    the same tiles; seeded tangent handedness bits."""
+    sc = T.normal_mapped_scene(384, seed_offset=3 + mixed, near_frac=0.2, mixed=mixed)
+    view = S.make_view(320, 240, count=sc.count)
+    _check(*_frames(gpu_ctx, sc, view, True), f"normal-mapped mixed={mixed}")

@@ -225,3 +225,32 @@ def test_alpha_test_uvs_and_mip(probe, clipped):
                          P(U2), P(V2), P(mip2))
         assert (_bits(U2) == _bits(U)).all() and (_bits(V2) == _bits(V)).all()
         assert (mip2 == mip).all()
+
+
+def test_normal_mapping(probe):
+    rng = np.random.default_rng(11)
+    L = oracle.lib()
+    for trial in range(64):
+        n = rng.normal(size=(3, 16))
+        n /= np.linalg.norm(n, axis=0)
+        t = rng.normal(size=(3, 16))
+        t -= n * (t * n).sum(axis=0)
+        t /= np.linalg.norm(t, axis=0)
+        n = np.ascontiguousarray(n, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        hand = np.where(rng.random(16) < 0.5, 0, 1 << 31).astype(np.uint32)
+        # normal-map bytes inside the unit disc, plus some outside it (NaN z)
+        if trial % 4 == 3:
+            xy = rng.integers(0, 256, (2, 16))
+        else:
+            ang = rng.uniform(0, 2 * np.pi, 16)
+            rad = rng.uniform(0, 0.99, 16)
+            xy = np.rint((np.stack([np.cos(ang), np.sin(ang)]) * rad + 1) * 127.5).astype(np.int64)
+        nmr = (xy[0] | (xy[1] << 8) | (rng.integers(0, 256, 16) << 16) | (rng.integers(0, 256, 16) << 24))
+        nmr = nmr.astype(np.uint32)
+        out = np.zeros((3, 16), np.float32)
+        probe.probe_normal_map(P(n), P(t), P(hand.view(np.int32)), P(nmr.view(np.int32)), P(out))
+        out2 = np.zeros((3, 16), np.float32)
+        L.orc_normal_map16(P(n), P(t), P(hand), P(nmr), P(out2))
+        same = (_bits(out) == _bits(out2)) | (np.isnan(out) & np.isnan(out2))
+        assert same.all(), (trial, np.argwhere(~same)[:3])
