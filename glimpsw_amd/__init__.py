@@ -106,11 +106,13 @@ class LightDesc(C.Structure):
 class ResolveDesc(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("object_to_clip", C.c_float * 16), ("object_to_world", C.c_float * 9),
                 ("inv_screen_proj", C.c_float * 16), ("view_pos", C.c_float * 3), ("exposure", C.c_float),
-                ("lights", C.POINTER(LightDesc)), ("num_lights", C.c_uint32)]
+                ("lights", C.POINTER(LightDesc)), ("num_lights", C.c_uint32), ("flags", C.c_uint32),
+                ("world_to_clip", C.c_float * 16)]
 
 
 GLIMP_RESOLVE_ALBEDO = 0
 GLIMP_RESOLVE_LIT = 1
+GLIMP_RESOLVE_LIGHT_SPRITES = 0x1
 
 
 def light_structs(lights, cls=None):
@@ -387,6 +389,10 @@ class Framebuffer:
             keep = light_structs(lighting.lights)
             d.lights = C.cast(keep, C.POINTER(LightDesc))
             d.num_lights = len(lighting.lights)
+            if getattr(lighting, "light_sprites", False) and getattr(lighting, "world_to_clip", None) is not None:
+                d.flags |= GLIMP_RESOLVE_LIGHT_SPRITES
+                for i in range(16):
+                    d.world_to_clip[i] = float(lighting.world_to_clip[i])
         else:
             d.exposure = 1.0
             d.object_to_world[0] = d.object_to_world[4] = d.object_to_world[8] = 1.0

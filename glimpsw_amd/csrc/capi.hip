@@ -900,6 +900,52 @@ int glimp_cull_meshlets(glimp_context* c, glimp_scene* s, uint32_t first, uint32
     return GLIMP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+int32_t cvt_trunc(float x) {   // float to int toward zero; out of range and NaN give INT32_MIN
+    if (!(x > -2147483904.0f && x < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)x;
+}
+uint32_t pack_byte(float v) {  // one channel of RGBA8u::Pack: RNE, saturated to [0, 255]
+    const int32_t i = (v >= -2147483648.0f && v < 2147483648.0f) ? (int32_t)std::nearbyint(v) : INT32_MIN;
+    return (uint32_t)std::min(std::max(i, 0), 255);
+}
+
+// Sprite of a point or spot light (Shading.cpp:692-706) in the operation
+// order of the reference's optimised build: clip position as one FMA chain
+// from the translation column, divisions by w as a multiply by 1/w, `/ 30`
+// as a multiply by 1/30; lights outside the clip volume get none.
+void place_sprite(const float* m, uint32_t W, uint32_t H, ResolveLight& L) {
+    L.sprite = 0;
+    float c[4];
+    for (int r = 0; r < 4; r++)
+        c[r] = std::fma(m[8 + r], L.position[2], std::fma(m[4 + r], L.position[1], std::fma(m[r], L.position[0], m[12 + r])));
+    const float mx = std::fmax(std::fmax(std::fabs(c[0]), std::fabs(c[1])), std::fabs(c[2]));
+    if (mx > c[3]) return;
+    const float rw = 1.0f / c[3];
+    const float radius = ((float)std::max(W, H) * (1.0f / 30.0f)) * rw;
+    const float cx = std::fma(c[0] * 0.5f, rw, 0.5f) * (float)W;
+    const float cy = std::fma(c[1] * 0.5f, rw, 0.5f) * (float)H;
+    L.depth = c[2] * rw;
+    L.tile_x0 = std::max(cvt_trunc(cx - radius), 0) & ~3;
+    L.tile_y0 = std::max(cvt_trunc(cy - radius), 0) & ~3;
+    L.tile_x1 = std::min(cvt_trunc(cx + radius), (int32_t)W);
+    L.tile_y1 = std::min(cvt_trunc(cy + radius), (int32_t)H);
+    L.half_x = 0.5f - cx;
+    L.half_y = 0.5f - cy;
+    L.r2 = radius * radius;
+    L.inv_r2 = 1.0f / L.r2;
+    L.rgb = pack_byte(L.color[0] * 255.0f) | (pack_byte(L.color[1] * 255.0f) << 8) |
+            (pack_byte(L.color[2] * 255.0f) << 16);
+    L.sprite = 1;
+}
+
+}  // namespace
+
+extern "C" {
+
 int glimp_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const glimp_resolve_desc* d, void* stream) {
     if (!c || !f || !sc || !d) return fail(GLIMP_E_INVALID, "glimp_resolve: NULL argument");
     if (d->mode > GLIMP_RESOLVE_LIT) return fail(GLIMP_E_INVALID, "glimp_resolve: unknown mode");
@@ -941,7 +987,14 @@ int glimp_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const
     p.kx = 2.0f / sw;
     p.ky = -2.0f / sh;
     p.num_lights = d->num_lights;
-    for (uint32_t i = 0; i < d->num_lights; i++) memcpy(&p.lights[i], &d->lights[i], sizeof(ResolveLight));
+    static_assert(sizeof(glimp_light) == kLightAbiBytes && offsetof(ResolveLight, sprite) == kLightAbiBytes,
+                  "ResolveLight starts with glimp_light");
+    for (uint32_t i = 0; i < d->num_lights; i++) {
+        memcpy(&p.lights[i], &d->lights[i], sizeof(glimp_light));
+        if (d->mode == GLIMP_RESOLVE_LIT && (d->flags & GLIMP_RESOLVE_LIGHT_SPRITES) && d->lights[i].type != 0)
+            place_sprite(d->world_to_clip, f->width, f->height, p.lights[i]);
+        p.sprites |= p.lights[i].sprite;
+    }
     p.mode = (int)d->mode;
     p.out = f->color;
     p.linear = 1;

@@ -17,7 +17,15 @@ struct ResolveLight {              // Light (Scene.h:52-75), precomputed terms i
     float color[3];
     float intensity;
     float inv_radius_sq, spot_scale, spot_offset;
+    // its sprite (Shading.cpp:692-724), placed on the host; sprite = 0: none
+    uint32_t sprite;
+    int32_t tile_x0, tile_y0, tile_x1, tile_y1;   // tile bases x0 <= x < x1, y0 <= y < y1
+    float depth;                   // the light's depth
+    float half_x, half_y;          // 0.5 - centre
+    float r2, inv_r2;              // radius^2 and 1 / radius^2
+    uint32_t rgb;                  // RGBA8u::Pack of the colour, alpha byte 0
 };
+constexpr size_t kLightAbiBytes = 56;   // the glimp_light prefix
 
 struct ResolveParams {
     DevMeshlets m;
@@ -37,6 +45,7 @@ struct ResolveParams {
     float exposure, light_exposure;
     float scale_x, scale_y, center_x, center_y, kx, ky;  // screen UV and derivative steps
     uint32_t num_lights;
+    uint32_t sprites;              // a light has a sprite
     ResolveLight lights[kMaxResolveLights];
     int mode;                      // 0 albedo (ResolveDebug BaseColor), 1 lit (Resolve)
     uint32_t* out;                 // RGBA8 per pixel

@@ -82,6 +82,15 @@ __device__ __forceinline__ void normal_map(const ApproxTables& T, float n[3], co
     for (int k = 0; k < 3; k++) n[k] = r[k] * inv;
 }
 
+// AlphaBlendU8 (Shading.cpp:238-246): source over in 8.8 fixed point.
+__device__ __forceinline__ uint32_t alpha_blend_u8(uint32_t bg, uint32_t fg) {
+    uint32_t t = (fg >> 1) & 0x7F800000u;
+    t |= (t >> 16) | 0x00400040u;
+    const uint32_t rb = lerp16(bg & 0x00FF00FFu, fg & 0x00FF00FFu, t);
+    const uint32_t ag = lerp16((bg >> 8) & 0x00FF00FFu, (fg >> 8) & 0x00FF00FFu, t);
+    return rb | (ag << 8);
+}
+
 __device__ __forceinline__ uint32_t pack_rgba8(float r, float g, float b, float a) {  // Texture.h:55-67
     float c[4] = { r, g, b, a };
     uint32_t out = 0;
@@ -336,6 +345,26 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
             cc[k] = fdiv(T, v, v + 0.155f) * 1.019f;
         }
         outc = pack_rgba8(cc[0], cc[1], cc[2], 1.0f);
+        // Light sprites (Shading.cpp:689-727), blended in light order over the
+        // pixel's colour: the reference's pass after the image changes each
+        // pixel through its own tile only, so it fuses into the store. A tile
+        // of a light's range takes the light when one of its pixels lies
+        // inside the disc and behind the light; its pixels behind the light
+        // get colour + alpha 255 (1 - a^2), a = 1 - (-d / r^2).
+        if (p.sprites) {
+            const int32_t bx = (int32_t)(tx * 4), by = (int32_t)(ty * 4);
+            for (uint32_t li = 0; li < p.num_lights; li++) {
+                const ResolveLight& L = p.lights[li];
+                if (!L.sprite || bx < L.tile_x0 || bx >= L.tile_x1 || by < L.tile_y0 || by >= L.tile_y1) continue;
+                const float rx = (float)(int32_t)x + L.half_x, ry = (float)(int32_t)y + L.half_y;
+                const float d = __builtin_fmaf(rx, rx, ry * ry) - L.r2;
+                const bool front = L.depth > depth;
+                if (!(__ballot(inside && d < 0.0f && front) & groupMask)) continue;
+                const float a = __builtin_fmaf(d, L.inv_r2, 1.0f);
+                const int32_t al = min(max(cvt_rne(__builtin_fmaf(-(a * a), 255.0f, 255.0f)), 0), 255);
+                if (front) outc = alpha_blend_u8(outc, L.rgb | ((uint32_t)al << 24));
+            }
+        }
     }
     if (inside) p.out[p.linear ? (size_t)y * W + x : (size_t)t] = outc;
 }

@@ -396,6 +396,8 @@ class Lighting:
     view_pos: tuple
     object_to_world: np.ndarray    # (9,) mat3 column-major
     inv_screen_proj: np.ndarray    # (16,)
+    world_to_clip: np.ndarray | None = None   # (16,) for the light sprites
+    light_sprites: bool = True     # Resolve draws point/spot lights (Shading.cpp:689-727)
 
 
 def default_lighting(view: View) -> Lighting:
@@ -407,4 +409,5 @@ def default_lighting(view: View) -> Lighting:
     Vinv = np.linalg.inv(view.view.astype(np.float64).reshape(4, 4).T)
     eye = tuple(Vinv[:3, 3].astype(np.float32).tolist())
     return Lighting(lights, 1.0, eye, np.eye(3, dtype=np.float32).reshape(9),
-                    inverse_screen_proj(view.object_to_clip, view.width, view.height))
+                    inverse_screen_proj(view.object_to_clip, view.width, view.height),
+                    np.asarray(view.object_to_clip, np.float32))

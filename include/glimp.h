@@ -262,8 +262,11 @@ int glimp_diff_backward_views(glimp_context* ctx, glimp_scene* scene, const glim
 /* ------------------------------------------------------------ resolve */
 
 #define GLIMP_RESOLVE_ALBEDO 0u   /* ResolveDebug(DebugLayer::BaseColor) (Shading.cpp:734-773) */
-#define GLIMP_RESOLVE_LIT 1u      /* Resolve (Shading.cpp:658-689), without skybox and light sprites */
+#define GLIMP_RESOLVE_LIT 1u      /* Resolve (Shading.cpp:658-727), without the skybox */
 #define GLIMP_MAX_LIGHTS 8u
+/* glimp_resolve_desc.flags: Resolve draws its point and spot lights as
+ * sprites over the image (Shading.cpp:689-727); needs world_to_clip. */
+#define GLIMP_RESOLVE_LIGHT_SPRITES 0x1u
 
 /* Light (Scene.h:52-75) with its precomputed terms. */
 typedef struct glimp_light {
@@ -284,6 +287,8 @@ typedef struct glimp_resolve_desc {
     float exposure;                    /* ShadingContext::Exposure */
     const glimp_light* lights;         /* host array (ShadingContext::Lights) */
     uint32_t num_lights;               /* <= GLIMP_MAX_LIGHTS */
+    uint32_t flags;                    /* GLIMP_RESOLVE_LIGHT_SPRITES (lit mode) */
+    float world_to_clip[16];           /* ShadingContext::WorldToClipMat, column-major (light sprites) */
 } glimp_resolve_desc;
 
 /* ShadingContext::Resolve / ResolveDebug(BaseColor) (Shading.cpp:658-773):

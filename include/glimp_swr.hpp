@@ -163,6 +163,8 @@ enum class DebugLayer { None, BaseColor };
 struct Lighting {
     float object_to_world[9] = { 1, 0, 0, 0, 1, 0, 0, 0, 1 };
     float inv_screen_proj[16] = {};
+    float world_to_clip[16] = {};   // ShadingContext::WorldToClipMat (light sprites)
+    bool light_sprites = true;      // Resolve draws point and spot lights (Shading.cpp:689-727)
     float view_pos[3] = {};
     float exposure = 1.0f;
     std::vector<glimp_light> lights;
@@ -257,6 +259,8 @@ inline void ShadingContext::Resolve(Rasterizer& rast, Framebuffer& fb, const Lig
     d.exposure = L.exposure;
     d.lights = L.lights.data();
     d.num_lights = (uint32_t)L.lights.size();
+    d.flags = L.light_sprites ? GLIMP_RESOLVE_LIGHT_SPRITES : 0u;
+    std::memcpy(d.world_to_clip, L.world_to_clip, sizeof(d.world_to_clip));
     check(glimp_resolve(rast.GetContext().get(), fb.get(), scene_->get(), &d, stream), "glimp_resolve");
     fb.resolved_ = true;
 }

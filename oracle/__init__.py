@@ -79,7 +79,8 @@ class Light(C.Structure):
 class ResolveParams(C.Structure):
     _fields_ = [("inv_screen_proj", C.c_float * 16), ("object_to_clip", C.c_float * 16),
                 ("object_to_world", C.c_float * 9), ("view_pos", C.c_float * 3), ("exposure", C.c_float),
-                ("lights", C.POINTER(Light)), ("num_lights", C.c_uint32)]
+                ("lights", C.POINTER(Light)), ("num_lights", C.c_uint32), ("light_sprites", C.c_uint32),
+                ("world_to_clip", C.c_float * 16)]
 
 
 _lib = None
@@ -249,6 +250,10 @@ def resolve_lit(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip, li
     lights = light_structs(lighting.lights, Light)
     rp.lights = C.cast(lights, C.POINTER(Light))
     rp.num_lights = len(lighting.lights)
+    if getattr(lighting, "light_sprites", False):
+        rp.light_sprites = 1
+        for i in range(16):
+            rp.world_to_clip[i] = float(lighting.world_to_clip[i])
     lib().orc_resolve_lit(C.byref(fb.fb), C.byref(ctx), C.byref(rp))
     return fb.pixels(0)
 

@@ -1028,6 +1028,101 @@ uint32_t pack_rgba8(float r, float g, float b, float a) {
     return out;
 }
 
+// ---------------------------------------------------------------- light sprites
+
+// cvttps2dq: truncation, out of range (and NaN) to INT32_MIN.
+int32_t cvt_trunc(float x) {
+    if (!(x > -2147483904.0f && x < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)x;
+}
+
+// Placement of a light's sprite (Shading.cpp:692-706) as the reference's
+// optimised build computes it: glm's mat4 * vec4 becomes one FMA chain from
+// the translation column, the divisions by w a multiply by 1/w (one IEEE
+// division) and `/ 30` a multiply by 1/30.
+struct Sprite {
+    bool on;
+    float depth, cx, cy, radius;
+    int32_t x0, y0, x1, y1;        // tile range: x0, y0 multiples of 4
+};
+
+Sprite sprite_of(const float* m, const float pos[3], uint32_t W, uint32_t H) {
+    Sprite s = {};
+    float c[4];
+    for (int r = 0; r < 4; r++) c[r] = fmaf(m[8 + r], pos[2], fmaf(m[4 + r], pos[1], fmaf(m[r], pos[0], m[12 + r])));
+    float mx = fmaxf(fmaxf(fabsf(c[0]), fabsf(c[1])), fabsf(c[2]));
+    if (mx > c[3]) return s;
+    const float rw = 1.0f / c[3];
+    s.depth = c[2] * rw;
+    s.radius = ((float)std::max(W, H) * (1.0f / 30.0f)) * rw;
+    s.cx = fmaf(c[0] * 0.5f, rw, 0.5f) * (float)W;
+    s.cy = fmaf(c[1] * 0.5f, rw, 0.5f) * (float)H;
+    s.x0 = std::max(cvt_trunc(s.cx - s.radius), 0) & ~3;
+    s.y0 = std::max(cvt_trunc(s.cy - s.radius), 0) & ~3;
+    s.x1 = std::min(cvt_trunc(s.cx + s.radius), (int32_t)W);
+    s.y1 = std::min(cvt_trunc(s.cy + s.radius), (int32_t)H);
+    s.on = true;
+    return s;
+}
+
+uint32_t pack_byte(float v) {  // one channel of RGBA8u::Pack
+    int32_t i = cvt_rne(v);
+    i = std::min(std::max(i, -32768), 32767);
+    return (uint32_t)std::min(std::max(i, 0), 255);
+}
+
+// Sprite colour of tile (x, y) (Shading.cpp:712-724): squared distance to the
+// centre less r^2, a = 1 - (-d / r^2) evaluated as fma(d, 1/r^2, 1), alpha
+// byte RNE(255 - 255 a^2), the light colour in the other bytes.
+void sprite_tile(const Sprite& s, const float color[3], int32_t x, int32_t y, float* dist, uint32_t* fg) {
+    const float hx = 0.5f - s.cx, hy = 0.5f - s.cy;
+    const float r2 = s.radius * s.radius, inv = 1.0f / r2;
+    const uint32_t rgb = pack_byte(color[0] * 255.0f) | (pack_byte(color[1] * 255.0f) << 8) |
+                         (pack_byte(color[2] * 255.0f) << 16);
+    for (int l = 0; l < 16; l++) {
+        const float rx = (float)(x + (l & 3)) + hx, ry = (float)(y + (l >> 2)) + hy;
+        const float d = fmaf(rx, rx, ry * ry) - r2;
+        const float a = fmaf(d, inv, 1.0f);
+        dist[l] = d;
+        fg[l] = rgb | (pack_byte(fmaf(-(a * a), 255.0f, 255.0f)) << 24);
+    }
+}
+
+// AlphaBlendU8 (Shading.cpp:238-246): source over in 8.8 fixed point.
+uint32_t alpha_blend_u8(uint32_t bg, uint32_t fg) {
+    uint32_t t = (fg >> 1) & 0x7F800000u;
+    t |= (t >> 16) | 0x00400040u;
+    uint32_t rb = lerp16(bg & 0x00FF00FFu, fg & 0x00FF00FFu, t);
+    uint32_t ag = lerp16((bg >> 8) & 0x00FF00FFu, (fg >> 8) & 0x00FF00FFu, t);
+    return rb | (ag << 8);
+}
+
+// The light sprites Resolve draws over its output (Shading.cpp:689-727):
+// for each point or spot light in front of the camera, every 4x4 tile of the
+// sprite's range with a pixel inside the disc and behind the light blends
+// the light colour over the pixels behind the light.
+void draw_light_sprites(uint32_t* color, const float* depth, uint32_t W, uint32_t H, const orc_resolve_params* p) {
+    for (uint32_t li = 0; li < p->num_lights; li++) {
+        const orc_light& L = p->lights[li];
+        if (L.type == 0) continue;
+        const Sprite s = sprite_of(p->world_to_clip, L.position, W, H);
+        if (!s.on) continue;
+        for (int32_t y = s.y0; y < s.y1; y += 4) {
+            for (int32_t x = s.x0; x < s.x1; x += 4) {
+                const uint32_t off = pixel_offset(W, (uint32_t)x, (uint32_t)y);
+                float d[16];
+                uint32_t fg[16];
+                sprite_tile(s, L.color, x, y, d, fg);
+                bool any = false;
+                for (int l = 0; l < 16; l++) any |= d[l] < 0.0f && s.depth > depth[off + l];
+                if (!any) continue;
+                for (int l = 0; l < 16; l++)
+                    if (s.depth > depth[off + l]) color[off + l] = alpha_blend_u8(color[off + l], fg[l]);
+            }
+        }
+    }
+}
+
 }  // namespace
 
 // ====================================================================== API
@@ -1383,6 +1478,7 @@ void orc_resolve_lit(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_
             }
         }
     }
+    if (p->light_sprites) draw_light_sprites(out.data(), depth, W, H, p);
     memcpy(fb->data, out.data(), (size_t)W * H * 4);
 }
 
@@ -1403,6 +1499,23 @@ void orc_normal_map16(const float* n, const float* t, const uint32_t* hand, cons
         for (int k = 0; k < 3; k++) out[k * 16 + l] = o[k];
     }
 }
+
+int32_t orc_sprite_place(const float* m, const float* pos, uint32_t W, uint32_t H, float* out, int32_t* tiles) {
+    const Sprite s = sprite_of(m, pos, W, H);
+    if (!s.on) return 0;
+    out[0] = s.depth; out[1] = s.cx; out[2] = s.cy; out[3] = s.radius;
+    tiles[0] = s.x0; tiles[1] = s.y0; tiles[2] = s.x1; tiles[3] = s.y1;
+    return 1;
+}
+
+void orc_sprite_tile(int32_t x, int32_t y, float cx, float cy, float radius, const float* color, float* dist,
+                     uint32_t* fg) {
+    Sprite s = {};
+    s.cx = cx; s.cy = cy; s.radius = radius;
+    sprite_tile(s, color, x, y, dist, fg);
+}
+
+uint32_t orc_alpha_blend_u8(uint32_t bg, uint32_t fg) { return alpha_blend_u8(bg, fg); }
 
 void orc_alpha_uv16(const int32_t* e1, const int32_t* e2, const float* coef, const float* remap, const float* uv,
                     float slu, float slv, float* U, float* V, int32_t* mip) {

@@ -96,6 +96,8 @@ typedef struct orc_resolve_params {
     float exposure;
     const orc_light* lights;
     uint32_t num_lights;
+    uint32_t light_sprites;           /* draw point/spot lights over the image (Shading.cpp:689-727) */
+    float world_to_clip[16];          /* ShadingContext::WorldToClipMat (light sprites) */
 } orc_resolve_params;
 
 /* Tables reproducing VRCP14PS / VRSQRT14PS; must be loaded before any call. */
@@ -141,6 +143,13 @@ void orc_fragment16(const int32_t* e1, const int32_t* e2, const float* coef, flo
  * n, t = world-space normal and tangent [3][16], hand = 0 or 1<<31,
  * nmr = layer-1 texels; out [3][16]. */
 void orc_normal_map16(const float* n, const float* t, const uint32_t* hand, const uint32_t* nmr, float* out);
+/* Light sprite pieces (Shading.cpp:692-724): placement (out = depth, centre
+ * x, y, radius; tiles = x0, y0, x1, y1; returns 0 when culled), one tile's
+ * distances and colours, and AlphaBlendU8 (Shading.cpp:238-246). */
+int32_t orc_sprite_place(const float* m, const float* pos, uint32_t W, uint32_t H, float* out, int32_t* tiles);
+void orc_sprite_tile(int32_t x, int32_t y, float cx, float cy, float radius, const float* color, float* dist,
+                     uint32_t* fg);
+uint32_t orc_alpha_blend_u8(uint32_t bg, uint32_t fg);
 /* The alpha test's UVs and implicit mip levels for one 4x4 tile
  * (FS_EncodeSurfaceId<true>, Shading.cpp:316-327; SampleImplicitLod,
  * Texture.h:403-410): remap = {ClippedU[3], ClippedV[3]} or NULL,

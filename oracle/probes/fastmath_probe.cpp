@@ -178,6 +178,61 @@ void probe_normal_map(const float* n, const float* t, const int32_t* hand, const
     st(out + 32, rz * r);
 }
 
+// Light sprite placement of the resolve (Shading.cpp:692-706), scalar float
+// math as glm 1.0.1 writes it: mat4 * vec4 = (m0*v.x + m1*v.y) + (m2*v.z +
+// m3*v.w) per component, then the clip test, depth, screen position, radius
+// and the tile range. m: column-major 4x4; out: depth, centre x, centre y,
+// radius; tiles: start x, start y, end x, end y; returns 0 when culled.
+int probe_sprite_host(const float* m, const float* pos, int W, int H, float* out, int32_t* tiles) {
+    float c[4];
+    for (int r = 0; r < 4; r++) {
+        float mul0 = m[0 * 4 + r] * pos[0], mul1 = m[1 * 4 + r] * pos[1];
+        float mul2 = m[2 * 4 + r] * pos[2], mul3 = m[3 * 4 + r] * 1.0f;
+        float add0 = mul0 + mul1, add1 = mul2 + mul3;
+        c[r] = add0 + add1;
+    }
+    float mx = __builtin_fmaxf(__builtin_fmaxf(__builtin_fabsf(c[0]), __builtin_fabsf(c[1])), __builtin_fabsf(c[2]));
+    if (mx > c[3]) return 0;
+    float depth = c[2] / c[3];
+    float sx = c[0] / c[3], sy = c[1] / c[3];
+    sx = sx * 0.5f + 0.5f;
+    sy = sy * 0.5f + 0.5f;
+    float radius = ((float)(W > H ? W : H) / 30.0f) / c[3];
+    float cx = sx * (float)W, cy = sy * (float)H;
+    int x0 = (int)(cx - radius), y0 = (int)(cy - radius);
+    int x1 = (int)(cx + radius), y1 = (int)(cy + radius);
+    tiles[0] = (x0 > 0 ? x0 : 0) & ~3;
+    tiles[1] = (y0 > 0 ? y0 : 0) & ~3;
+    tiles[2] = x1 < W ? x1 : W;
+    tiles[3] = y1 < H ? y1 : H;
+    out[0] = depth; out[1] = cx; out[2] = cy; out[3] = radius;
+    return 1;
+}
+
+// Light sprite colour of one 4x4 tile (Shading.cpp:712-724): distance to the
+// centre, alpha 1 - a^2 with a = 1 - (-d / r^2), packed with the light colour
+// (RGBA8u::Pack, Texture.h:55-67). out: dist[16] floats, fg[16] packed.
+void probe_sprite_tile(int x, int y, float cx, float cy, float radius, const float* color, float* dist,
+                       int32_t* fg) {
+    vi lane = { 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15 };
+    vi tx = x + (lane & 3), ty = y + (lane >> 2);
+    vf rx = __builtin_convertvector(tx, vf) + 0.5f - cx;
+    vf ry = __builtin_convertvector(ty, vf) + 0.5f - cy;
+    vf d = vfma(rx, rx, ry * ry) - radius * radius;
+    vf a = 1 - (-d / (radius * radius));
+    vf alpha = 1 - a * a;
+    vi r = rne((vf)color[0] * 255.0f), g = rne((vf)color[1] * 255.0f), b = rne((vf)color[2] * 255.0f);
+    vi al = rne(alpha * 255.0f);
+    auto sat = [](vi v) {
+        v = v < -32768 ? -32768 : v;
+        v = v > 32767 ? 32767 : v;
+        v = v < 0 ? 0 : v;
+        return v > 255 ? 255 : v;
+    };
+    st(dist, d);
+    sti(fg, sat(r) | (sat(g) << 8) | (sat(b) << 16) | (sat(al) << 24));
+}
+
 // Screen UV of a 4x4 tile in a full-screen pass (Rasterizer.h:226-237).
 void probe_tile_uv(const int32_t* px, int n, float size, float* out) {
     float scale = 2.0f / size;

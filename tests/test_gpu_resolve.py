@@ -105,3 +105,32 @@ def test_normal_mapped_scene(gpu_ctx, mixed):
     sc = T.normal_mapped_scene(384, seed_offset=3 + mixed, near_frac=0.2, mixed=mixed)
     view = S.make_view(320, 240, count=sc.count)
     _check(*_frames(gpu_ctx, sc, view, True), f"normal-mapped mixed={mixed}")
+
+
+def test_light_sprites(gpu_ctx):
+    """Point and spot light sprites over the lit image (Shading.cpp:689-727):
+    a light in front of the surfaces near it and one partly hidden behind a
+    patch, as a point and as a spot light."""
+    import glimpsw_amd as g
+    from tests.test_oracle_sprites import _light_positions
+
+    sc, view = S.checker_scene(width=1280, height=960)
+    ds = g.Scene(gpu_ctx, sc)
+    fb = g.render_frame(gpu_ctx, ds, view)
+    ofb, _, b = oracle.render_frame(sc, view)
+    positions = list(_light_positions(sc, view, ofb.depth()))
+    lighting = S.default_lighting(view)
+    for pos in positions:
+        for kind in (1, 2):
+            lighting.lights[1] = S.Light(kind, position=pos, direction=(0.0, 0.0, -1.0), color=(1.2, 0.7, 0.3),
+                                         intensity=4000.0, radius=30.0)
+            fb.resolve(ds, view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting)
+            got = fb.read_color()                       # the vis-buffer stays; resolve writes colour only
+            ofb2, _, _ = oracle.render_frame(sc, view, b)
+            want = oracle.resolve_lit(ofb2, b, view.object_to_clip, lighting)
+            gl = got.view(np.uint8).reshape(*got.shape, 4).astype(int)
+            ol = want.view(np.uint8).reshape(*want.shape, 4).astype(int)
+            print(f"[resolve] sprite at {pos} type {kind}: max diff {np.abs(gl - ol).max()}", flush=True)
+            assert np.abs(gl - ol).max() <= 1
+    fb.close()
+    ds.close()

@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 import oracle
+from glimpsw_amd import scene as S
 
 f32p = C.POINTER(C.c_float)
 i32p = C.POINTER(C.c_int32)
@@ -254,3 +255,54 @@ def test_normal_mapping(probe):
         L.orc_normal_map16(P(n), P(t), P(hand), P(nmr), P(out2))
         same = (_bits(out) == _bits(out2)) | (np.isnan(out) & np.isnan(out2))
         assert same.all(), (trial, np.argwhere(~same)[:3])
+
+
+def test_light_sprite_placement_and_tile(probe):
+    """The sprite placement (glm mat4 * vec4, divisions by w, radius, tile
+    range) and one tile's distances and colours, against the probe."""
+    rng = np.random.default_rng(12)
+    L = oracle.lib()
+    L.orc_sprite_place.restype = C.c_int32
+    placed = 0
+    for trial in range(200):
+        view = S.make_view(int(rng.choice([320, 640, 1920])), int(rng.choice([240, 480, 1080])),
+                           view=S.view_matrix((0.0, 0.0, 0.0), rng.uniform(-0.2, 0.2), rng.uniform(-0.2, 0.2)))
+        m = np.ascontiguousarray(view.object_to_clip, np.float32)
+        z = -rng.uniform(0.005, 12.0)      # mostly in front of the camera, some past the edges
+        pos = np.array([rng.uniform(-0.8, 0.8) * -z, rng.uniform(-0.7, 0.7) * -z, z], np.float32)
+        W, H = view.width, view.height
+        o1 = np.zeros(4, np.float32); t1 = np.zeros(4, np.int32)
+        o2 = np.zeros(4, np.float32); t2 = np.zeros(4, np.int32)
+        r1 = probe.probe_sprite_host(P(m), P(pos), W, H, P(o1), P(t1))
+        r2 = L.orc_sprite_place(P(m), P(pos), C.c_uint32(W), C.c_uint32(H), P(o2), P(t2))
+        assert r1 == r2
+        if not r1:
+            continue
+        placed += 1
+        assert (_bits(o1) == _bits(o2)).all() and (t1 == t2).all(), (trial, o1, o2, t1, t2)
+        color = rng.uniform(0, 1.5, 3).astype(np.float32)
+        for _ in range(4):
+            x = int(rng.integers(t1[0], max(t1[2], t1[0] + 1))) & ~3
+            y = int(rng.integers(t1[1], max(t1[3], t1[1] + 1))) & ~3
+            d1 = np.zeros(16, np.float32); f1 = np.zeros(16, np.int32)
+            d2 = np.zeros(16, np.float32); f2 = np.zeros(16, np.uint32)
+            probe.probe_sprite_tile(x, y, C.c_float(o1[1]), C.c_float(o1[2]), C.c_float(o1[3]), P(color), P(d1), P(f1))
+            L.orc_sprite_tile(x, y, C.c_float(o1[1]), C.c_float(o1[2]), C.c_float(o1[3]), P(color), P(d2), P(f2))
+            assert (_bits(d1) == _bits(d2)).all() and (f1.view(np.uint32) == f2).all()
+    assert placed > 50
+
+
+def test_alpha_blend_u8_against_float():
+    """AlphaBlendU8 is source-over in 8.8 fixed point: within one unit of
+    bg + (fg - bg) * alpha/255 per channel, exact at alpha 0 and 255... up to
+    the 1/256 step of its weights."""
+    rng = np.random.default_rng(13)
+    L = oracle.lib()
+    L.orc_alpha_blend_u8.restype = C.c_uint32
+    for _ in range(2000):
+        bg = int(rng.integers(0, 1 << 32)); fg = int(rng.integers(0, 1 << 32))
+        out = L.orc_alpha_blend_u8(C.c_uint32(bg), C.c_uint32(fg))
+        a = (fg >> 24) / 255.0
+        for k in range(4):
+            b, f, o = (bg >> 8 * k) & 255, (fg >> 8 * k) & 255, (out >> 8 * k) & 255
+            assert abs(o - (b + (f - b) * a)) <= 1.5, (hex(bg), hex(fg), k, o)
