@@ -66,9 +66,9 @@ def algorithmic_bytes(stats: dict) -> dict:
 
 # profiler scope name -> kernel symbol substring in rocprofv3 output
 KERNEL_SYMBOLS = {
-    "clear": "k_clear(", "cull": "k_cull<", "mesh_setup": "k_mesh_setup(", "clip": "k_clip(",
-    "bin_scatter": "k_bin_scatter(", "bin_raster": "k_bin_raster<", "raster_spill": "k_raster_direct(",
-    "resolve_lit": "k_resolve(",
+    "clear": "k_clear(", "cull": "k_cull<", "mesh_setup": "k_mesh_setup<", "clip": "k_clip<",
+    "bin_scatter": "k_bin_scatter<", "bin_raster": "k_bin_raster<", "raster_spill": "k_raster_direct<",
+    "resolve_lit": "k_resolve<1>",
 }
 
 

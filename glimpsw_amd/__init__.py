@@ -112,6 +112,10 @@ class ResolveDesc(C.Structure):
 
 GLIMP_RESOLVE_ALBEDO = 0
 GLIMP_RESOLVE_LIT = 1
+GLIMP_RESOLVE_NORMALS = 2
+GLIMP_RESOLVE_METALLIC_ROUGHNESS = 3
+GLIMP_RESOLVE_MESHLET_ID = 4
+GLIMP_RESOLVE_TRIANGLE_ID = 5
 GLIMP_RESOLVE_LIGHT_SPRITES = 0x1
 
 
@@ -372,7 +376,9 @@ class Framebuffer:
         return ids, depth
 
     def resolve(self, scene: Scene, object_to_clip, mode: int = GLIMP_RESOLVE_ALBEDO, lighting=None, stream=None):
-        """ShadingContext::Resolve (lit) or ResolveDebug(BaseColor) (albedo)."""
+        """ShadingContext::Resolve (lit) or ResolveDebug (the other modes);
+        ``lighting`` supplies object_to_world for the Normals and
+        MetallicRoughness layers as well."""
         d = ResolveDesc()
         d.mode = mode
         for i in range(16):

@@ -948,7 +948,7 @@ extern "C" {
 
 int glimp_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const glimp_resolve_desc* d, void* stream) {
     if (!c || !f || !sc || !d) return fail(GLIMP_E_INVALID, "glimp_resolve: NULL argument");
-    if (d->mode > GLIMP_RESOLVE_LIT) return fail(GLIMP_E_INVALID, "glimp_resolve: unknown mode");
+    if (d->mode > GLIMP_RESOLVE_TRIANGLE_ID) return fail(GLIMP_E_INVALID, "glimp_resolve: unknown mode");
     if (d->num_lights > GLIMP_MAX_LIGHTS || (d->num_lights && !d->lights))
         return fail(GLIMP_E_LIMIT, "glimp_resolve: at most GLIMP_MAX_LIGHTS lights");
     HIP_TRY(hipSetDevice(c->device));

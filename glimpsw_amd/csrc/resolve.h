@@ -47,7 +47,7 @@ struct ResolveParams {
     uint32_t num_lights;
     uint32_t sprites;              // a light has a sprite
     ResolveLight lights[kMaxResolveLights];
-    int mode;                      // 0 albedo (ResolveDebug BaseColor), 1 lit (Resolve)
+    int mode;                      // GLIMP_RESOLVE_*: 0 BaseColor, 1 lit (Resolve), 2-5 other debug layers
     uint32_t* out;                 // RGBA8 per pixel
     int linear;
 };

@@ -6,8 +6,10 @@
 // their screen derivatives analytically (IntersectTriangle,
 // Shading.cpp:417-464), interpolate UVs, pick the mip level from the UV
 // gradients and sample the material texture (Texture.h:412-575); in lit mode
-// also interpolate the vertex normal and evaluate the lights
-// (Shading.cpp:602-645), tonemap and pack.
+// also interpolate the vertex normal (normal-mapped with a second texture
+// layer) and evaluate the lights (Shading.cpp:602-645), tonemap, pack and
+// blend the light sprites. The other debug layers (normals, metallic /
+// roughness, hashed meshlet and triangle ids) share the surface code.
 //
 // The reference shades a 4x4 tile per SIMD step; two decisions are taken
 // per tile: the texture filter (minification anywhere in the tile selects the
@@ -148,6 +150,41 @@ __device__ float light_attenuation(const ApproxTables& T, const ResolveLight& L,
     return att;
 }
 
+// World-space surface normal of ResolveSurface (Shading.cpp:546-572): the
+// interpolated vertex normal, normal-mapped when a lane of the tile has a
+// normal texel. Called by every lane of the wave (the tile decision is a
+// ballot); `active` lanes get their normal.
+__device__ __forceinline__ void surface_normal(const ResolveParams& p, bool active, uint32_t mi, uint32_t tri,
+                                               const float bary[3], uint32_t nmr, unsigned long long groupMask,
+                                               float N[3]) {
+    const ApproxTables& T = p.tables;
+    if (active) {
+        const uint8_t* I = p.m.indices + (size_t)mi * 384;
+        float nv[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            uint32_t vi = I[k * 128 + tri];
+            unpack_normal(T, p.m.normal_tangents[(size_t)mi * 64 + vi], nv[k]);
+        }
+        lerp_to_world(T, nv, bary, p.object_to_world, N);
+    }
+    if (__ballot((nmr & 0xFFFFu) != 0) & groupMask) {
+        if (active) {
+            const uint8_t* I = p.m.indices + (size_t)mi * 384;
+            float tv[3][3], Tw[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) unpack_normal(T, p.m.normal_tangents[(size_t)mi * 64 + I[k * 128 + tri]] >> 16, tv[k]);
+            lerp_to_world(T, tv, bary, p.object_to_world, Tw);
+            // handedness of the first vertex (Shading.cpp:501-503)
+            const uint32_t handed = ((p.m.tangent_handedness[mi] >> (I[tri] & 63u)) & 1u) << 31;
+            normal_map(T, N, Tw, handed, nmr);
+        }
+    }
+}
+
+// Mode: GLIMP_RESOLVE_* (0 BaseColor, 1 lit, 2 Normals, 3 MetallicRoughness,
+// 4 MeshletId, 5 TriangleId); each output is its own instantiation.
+template <int Mode>
 __global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t W = (uint32_t)p.width, H = (uint32_t)p.height;
@@ -166,10 +203,10 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
     const bool active = inside && !(depth <= 0.0f);
 
     // surface of the pixel
-    float uv0[2] = { 0, 0 }, gradU[2] = { 0, 0 }, gradV[2] = { 0, 0 }, U = 0, V = 0, bary[3] = { 0, 0, 0 };
+    float gradU[2] = { 0, 0 }, gradV[2] = { 0, 0 }, U = 0, V = 0, bary[3] = { 0, 0, 0 };
     uint32_t material = 0xFFFFFFFFu, mi = 0, tri = 0;
-    if (active) {
-        decode_surface(key, p.draws, p.num_draws, mi, tri);
+    if (active) decode_surface(key, p.draws, p.num_draws, mi, tri);
+    if (Mode < 4 && active) {
         const uint8_t* I = p.m.indices + (size_t)mi * 384;
         const float* P = p.m.positions + (size_t)mi * 192;
         float c[3][4], uv[3][2];
@@ -196,7 +233,6 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
         gradV[0] = __builtin_fmaf(t20y, ddx[2], t10y * ddx[1]);
         gradU[1] = __builtin_fmaf(t20x, ddy[2], t10x * ddy[1]);
         gradV[1] = __builtin_fmaf(t20y, ddy[2], t10y * ddy[1]);
-        (void)uv0;
     }
 
     // material waterfall per 4x4 tile (Shading.cpp:532-545): each pass
@@ -206,7 +242,7 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
     const DevTexture* myTex = nullptr;
     int32_t myMip = 0;
     bool myLinear = false;
-    bool done = !active || material == 0xFFFFFFFFu || material >= p.num_materials;
+    bool done = Mode >= 4 || !active || material == 0xFFFFFFFFu || material >= p.num_materials;
     for (;;) {
         const unsigned long long todo = __ballot(!done);
         if (!todo) break;
@@ -240,45 +276,43 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
     uint32_t nmr = 0;
     if (myTex) {
         albedo = sample_level(*myTex, U, V, myMip, myLinear);
-        if (p.mode != 0 && myTex->num_layers >= 2) nmr = sample_level(*myTex, U, V, myMip, myLinear, 1);
+        if (Mode >= 1 && Mode <= 3 && myTex->num_layers >= 2) nmr = sample_level(*myTex, U, V, myMip, myLinear, 1);
     }
 
     uint32_t outc;
-    if (p.mode == 0) {
-        // ResolveDebug(BaseColor) (Shading.cpp:744-771)
+    if constexpr (Mode != 1) {
+        // ResolveDebug (Shading.cpp:734-773): the layer's colour, the
+        // background checker where nothing was drawn
         const uint32_t x0 = tx * 4, y0 = ty * 4;
-        uint32_t bg = ((x0 ^ y0) & 4u) ? 0xFFA0A0A0u : 0xFFFFFFFFu;
+        const uint32_t bg = ((x0 ^ y0) & 4u) ? 0xFFA0A0A0u : 0xFFFFFFFFu;
         const float s = 1.0f / 255;
-        outc = active ? pack_rgba8((float)(albedo & 255) * s, (float)((albedo >> 8) & 255) * s,
-                                   (float)((albedo >> 16) & 255) * s, 1.0f)
-                      : bg;
+        float c[3] = { 0, 0, 0 };
+        if constexpr (Mode == 0) {
+            c[0] = (float)(albedo & 255) * s;
+            c[1] = (float)((albedo >> 8) & 255) * s;
+            c[2] = (float)((albedo >> 16) & 255) * s;
+        } else if constexpr (Mode == 2) {
+            float N[3] = { 0, 0, 0 };
+            surface_normal(p, active, mi, tri, bary, nmr, groupMask, N);
+#pragma unroll
+            for (int k = 0; k < 3; k++) c[k] = __builtin_fmaf(N[k], 0.5f, 0.5f);
+        } else if constexpr (Mode == 3) {
+            c[0] = (float)((nmr >> 16) & 255u) * s;
+            c[1] = (float)((nmr >> 24) & 255u) * s;
+        } else {
+            // RGBA8u::Unpack of the hashed surface id (MeshletId: id / 128)
+            const uint32_t sid = mi * 128u + tri;
+            const uint32_t h = (Mode == 4 ? sid / 128u : sid) * 123456789u;
+            c[0] = (float)(h & 255) * s;
+            c[1] = (float)((h >> 8) & 255) * s;
+            c[2] = (float)((h >> 16) & 255) * s;
+        }
+        outc = active ? pack_rgba8(c[0], c[1], c[2], 1.0f) : bg;
     } else {
         float color[3] = { 0, 0, 0 };
         float N[3] = { 0, 0, 0 }, Vd[3] = { 0, 0, 0 }, wpos[3] = { 0, 0, 0 }, base[3] = { 0, 0, 0 };
         float f0[3] = { 0, 0, 0 }, diffuse[3] = { 0, 0, 0 }, NoV = 0, aR = 1e-4f;
-        if (active) {
-            const uint8_t* I = p.m.indices + (size_t)mi * 384;
-            float nv[3][3];
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                uint32_t vi = I[k * 128 + tri];
-                unpack_normal(T, p.m.normal_tangents[(size_t)mi * 64 + vi], nv[k]);
-            }
-            lerp_to_world(T, nv, bary, p.object_to_world, N);
-        }
-        // normal mapping when a lane of the tile has a normal texel
-        if (__ballot((nmr & 0xFFFFu) != 0) & groupMask) {
-            if (active) {
-                const uint8_t* I = p.m.indices + (size_t)mi * 384;
-                float tv[3][3], Tw[3];
-#pragma unroll
-                for (int k = 0; k < 3; k++) unpack_normal(T, p.m.normal_tangents[(size_t)mi * 64 + I[k * 128 + tri]] >> 16, tv[k]);
-                lerp_to_world(T, tv, bary, p.object_to_world, Tw);
-                // handedness of the first vertex (Shading.cpp:501-503)
-                const uint32_t handed = ((p.m.tangent_handedness[mi] >> (I[tri] & 63u)) & 1u) << 31;
-                normal_map(T, N, Tw, handed, nmr);
-            }
-        }
+        surface_normal(p, active, mi, tri, bary, nmr, groupMask, N);
         if (active) {
             float c4[4];
 #pragma unroll
@@ -373,8 +407,19 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
 
 hipError_t launch_resolve(const ResolveParams& p, hipStream_t s, Profiler* prof) {
     uint32_t n = (uint32_t)p.width * (uint32_t)p.height;
-    Profiler::Scope sc(prof, p.mode == 0 ? "resolve_albedo" : "resolve_lit", s);
-    hipLaunchKernelGGL(k_resolve, dim3((n + 255) / 256), dim3(256), 0, s, p);
+    static const char* const names[6] = { "resolve_albedo", "resolve_lit", "resolve_normals",
+                                          "resolve_metallic_roughness", "resolve_meshlet_id", "resolve_triangle_id" };
+    if (p.mode < 0 || p.mode > 5) return hipErrorInvalidValue;
+    Profiler::Scope sc(prof, names[p.mode], s);
+    const dim3 grid((n + 255) / 256), block(256);
+    switch (p.mode) {
+    case 0: hipLaunchKernelGGL(k_resolve<0>, grid, block, 0, s, p); break;
+    case 1: hipLaunchKernelGGL(k_resolve<1>, grid, block, 0, s, p); break;
+    case 2: hipLaunchKernelGGL(k_resolve<2>, grid, block, 0, s, p); break;
+    case 3: hipLaunchKernelGGL(k_resolve<3>, grid, block, 0, s, p); break;
+    case 4: hipLaunchKernelGGL(k_resolve<4>, grid, block, 0, s, p); break;
+    default: hipLaunchKernelGGL(k_resolve<5>, grid, block, 0, s, p); break;
+    }
     return hipGetLastError();
 }
 

@@ -263,6 +263,10 @@ int glimp_diff_backward_views(glimp_context* ctx, glimp_scene* scene, const glim
 
 #define GLIMP_RESOLVE_ALBEDO 0u   /* ResolveDebug(DebugLayer::BaseColor) (Shading.cpp:734-773) */
 #define GLIMP_RESOLVE_LIT 1u      /* Resolve (Shading.cpp:658-727), without the skybox */
+#define GLIMP_RESOLVE_NORMALS 2u             /* ResolveDebug(DebugLayer::Normals) */
+#define GLIMP_RESOLVE_METALLIC_ROUGHNESS 3u  /* ResolveDebug(DebugLayer::MetallicRoughness) */
+#define GLIMP_RESOLVE_MESHLET_ID 4u          /* ResolveDebug(DebugLayer::MeshletId) */
+#define GLIMP_RESOLVE_TRIANGLE_ID 5u         /* ResolveDebug(DebugLayer::TriangleId) */
 #define GLIMP_MAX_LIGHTS 8u
 /* glimp_resolve_desc.flags: Resolve draws its point and spot lights as
  * sprites over the image (Shading.cpp:689-727); needs world_to_clip. */
@@ -291,7 +295,7 @@ typedef struct glimp_resolve_desc {
     float world_to_clip[16];           /* ShadingContext::WorldToClipMat, column-major (light sprites) */
 } glimp_resolve_desc;
 
-/* ShadingContext::Resolve / ResolveDebug(BaseColor) (Shading.cpp:658-773):
+/* ShadingContext::Resolve / ResolveDebug (Shading.cpp:658-773):
  * writes RGBA8 per pixel into the framebuffer's color buffer (row-major),
  * which glimp_framebuffer_color / glimp_framebuffer_read_color expose.
  * Asynchronous. */

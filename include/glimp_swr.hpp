@@ -158,7 +158,9 @@ const ShaderDispatchTable& GetDispatchTable() {
     return table;
 }
 
-enum class DebugLayer { None, BaseColor };
+// DebugLayer (Shading.h:8); the overdraw layers need FS_Overdraw, a pipeline
+// this library does not build.
+enum class DebugLayer { None, BaseColor, Normals, MetallicRoughness, MeshletId, TriangleId };
 
 struct Lighting {
     float object_to_world[9] = { 1, 0, 0, 0, 1, 0, 0, 0, 1 };
@@ -179,6 +181,7 @@ public:
     explicit ShadingContext(Scene& scene) : scene_(&scene) {}
 
     float ObjectToClipMat[16] = {};
+    float ObjectToWorldMat[9] = { 1, 0, 0, 0, 1, 0, 0, 0, 1 };   // ResolveDebug normals
     uint32_t MeshletOffset = 0;
 
     // UpdateProj (Shading.h:35-39) with the product already formed.
@@ -199,7 +202,7 @@ public:
     }
     // Resolve (Shading.cpp:658-732).
     void Resolve(Rasterizer& rast, Framebuffer& fb, const Lighting& L, void* stream = nullptr);
-    // ResolveDebug(BaseColor) (Shading.cpp:734-773).
+    // ResolveDebug (Shading.cpp:734-773).
     void ResolveDebug(Rasterizer& rast, Framebuffer& fb, DebugLayer layer, void* stream = nullptr);
 
     Scene* GetScene() const { return scene_; }
@@ -266,12 +269,18 @@ inline void ShadingContext::Resolve(Rasterizer& rast, Framebuffer& fb, const Lig
 }
 
 inline void ShadingContext::ResolveDebug(Rasterizer& rast, Framebuffer& fb, DebugLayer layer, void* stream) {
-    if (layer != DebugLayer::BaseColor) throw Error(GLIMP_E_UNSUPPORTED, "ResolveDebug: only BaseColor is built");
     glimp_resolve_desc d;
     std::memset(&d, 0, sizeof(d));
-    d.mode = GLIMP_RESOLVE_ALBEDO;
+    switch (layer) {
+    case DebugLayer::BaseColor: d.mode = GLIMP_RESOLVE_ALBEDO; break;
+    case DebugLayer::Normals: d.mode = GLIMP_RESOLVE_NORMALS; break;
+    case DebugLayer::MetallicRoughness: d.mode = GLIMP_RESOLVE_METALLIC_ROUGHNESS; break;
+    case DebugLayer::MeshletId: d.mode = GLIMP_RESOLVE_MESHLET_ID; break;
+    case DebugLayer::TriangleId: d.mode = GLIMP_RESOLVE_TRIANGLE_ID; break;
+    default: throw Error(GLIMP_E_UNSUPPORTED, "ResolveDebug: no such layer");
+    }
     std::memcpy(d.object_to_clip, ObjectToClipMat, sizeof(d.object_to_clip));
-    d.object_to_world[0] = d.object_to_world[4] = d.object_to_world[8] = 1.0f;
+    std::memcpy(d.object_to_world, ObjectToWorldMat, sizeof(d.object_to_world));
     d.exposure = 1.0f;
     check(glimp_resolve(rast.GetContext().get(), fb.get(), scene_->get(), &d, stream), "glimp_resolve");
     fb.resolved_ = true;

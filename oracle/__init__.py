@@ -220,6 +220,18 @@ def resolve_albedo(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip,
     return fb.pixels(0)
 
 
+DEBUG_BASE_COLOR, DEBUG_NORMALS, DEBUG_METALLIC_ROUGHNESS, DEBUG_MESHLET_ID, DEBUG_TRIANGLE_ID = range(5)
+
+
+def resolve_debug(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip, layer: int,
+                  object_to_world=None, meshlet_offset: int = 0):
+    """ResolveDebug of one layer into layer 0; returns row-major RGBA8."""
+    ctx = binding.ctx(object_to_clip, meshlet_offset)
+    o2w = np.ascontiguousarray(object_to_world if object_to_world is not None else np.eye(3).reshape(9), np.float32)
+    lib().orc_resolve_debug(C.byref(fb.fb), C.byref(ctx), C.c_uint32(layer), _ptr(o2w, f32p))
+    return fb.pixels(0)
+
+
 def render_frame(scene, view, binding=None, cull_first: bool = True, clipping: bool = True):
     """Clear, cull, draw: the reference frame sequence (Main.cpp:213-240)."""
     binding = binding or SceneBinding(scene)

@@ -1398,7 +1398,10 @@ int32_t orc_clip_triangle(const float* v /*[3][4]*/, uint32_t outcodes, float gb
 }
 
 // ResolveDebug(DebugLayer::BaseColor) (Shading.cpp:734-773).
-void orc_resolve_albedo(orc_framebuffer* fb, const orc_shading_ctx* ctx) {
+// ResolveDebug (Shading.cpp:734-773) for the surface layers (BaseColor,
+// Normals, MetallicRoughness) and the id layers (MeshletId, TriangleId);
+// uncovered pixels get the background checker.
+void orc_resolve_debug(orc_framebuffer* fb, const orc_shading_ctx* ctx, uint32_t layer, const float* object_to_world) {
     uint32_t W = fb->width, H = fb->height;
     float sw = (float)W, sh = (float)H;
     float scaleX = 2.0f / sw, scaleY = 2.0f / sh;
@@ -1406,6 +1409,8 @@ void orc_resolve_albedo(orc_framebuffer* fb, const orc_shading_ctx* ctx) {
     const float* depth = (const float*)(fb->data + fb->layer_stride);
     std::vector<uint32_t> out(fb->layer_stride);
     memcpy(out.data(), fb->data, (size_t)W * H * 4);
+    const bool surface = layer <= ORC_DEBUG_METALLIC_ROUGHNESS;
+    const float* o2w = (layer == ORC_DEBUG_NORMALS || layer == ORC_DEBUG_METALLIC_ROUGHNESS) ? object_to_world : nullptr;
     for (uint32_t y0 = 0; y0 < H; y0 += 4) {
         for (uint32_t x0 = 0; x0 < W; x0 += 4) {
             uint32_t off = pixel_offset(W, x0, y0);
@@ -1421,23 +1426,39 @@ void orc_resolve_albedo(orc_framebuffer* fb, const orc_shading_ctx* ctx) {
             SurfaceLane lanes[16];
             bool anyActive = false;
             for (int l = 0; l < 16; l++) anyActive |= active[l];
-            if (anyActive) resolve_surface_tile(*ctx, ids, active, sx, sy, sw, sh, lanes);
+            if (anyActive && surface) resolve_surface_tile(*ctx, ids, active, sx, sy, sw, sh, lanes, o2w);
             uint32_t bg = ((x0 ^ y0) & 4) ? 0xFFA0A0A0u : 0xFFFFFFFFu;
             for (int l = 0; l < 16; l++) {
                 if (!active[l]) { out[off + l] = bg; continue; }
-                uint32_t a = lanes[l].albedo;
-                // Unpack * (1/255) then Pack: the round trip returns the bytes
-                float r = (float)(a & 255) * (1.0f / 255), g = (float)((a >> 8) & 255) * (1.0f / 255);
-                float b = (float)((a >> 16) & 255) * (1.0f / 255);
-                out[off + l] = pack_rgba8(r, g, b, 1.0f);
+                float c[4];
+                if (layer == ORC_DEBUG_BASE_COLOR) {
+                    unpack_rgba8(lanes[l].albedo, c);
+                } else if (layer == ORC_DEBUG_NORMALS) {
+                    // Normal * 0.5 + 0.5 (one FMA in the fast-math build; x * 0.5 is exact)
+                    for (int k = 0; k < 3; k++) c[k] = fmaf(lanes[l].nrm[k], 0.5f, 0.5f);
+                } else if (layer == ORC_DEBUG_METALLIC_ROUGHNESS) {
+                    c[0] = (float)((lanes[l].nmr >> 16) & 255) * (1.0f / 255);
+                    c[1] = (float)((lanes[l].nmr >> 24) & 255) * (1.0f / 255);
+                    c[2] = 0.0f;
+                } else {
+                    // RGBA8u::Unpack of the hashed id (MeshletId: id / 128)
+                    uint32_t v = layer == ORC_DEBUG_MESHLET_ID ? ids[l] / 128u : ids[l];
+                    unpack_rgba8(v * 123456789u, c);
+                }
+                out[off + l] = pack_rgba8(c[0], c[1], c[2], 1.0f);
             }
         }
     }
     memcpy(fb->data, out.data(), (size_t)W * H * 4);
 }
 
-// Resolve (Shading.cpp:658-689) without skybox and light sprites: covered
-// pixels get EvalLighting of their surface, uncovered pixels are black.
+void orc_resolve_albedo(orc_framebuffer* fb, const orc_shading_ctx* ctx) {
+    orc_resolve_debug(fb, ctx, ORC_DEBUG_BASE_COLOR, nullptr);
+}
+
+// Resolve (Shading.cpp:658-727) without the skybox: covered pixels get
+// EvalLighting of their surface, uncovered pixels are black; then the light
+// sprites when asked for.
 void orc_resolve_lit(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_resolve_params* p) {
     uint32_t W = fb->width, H = fb->height;
     float sw = (float)W, sh = (float)H;

@@ -118,6 +118,14 @@ uint32_t orc_cull_meshlets(uint16_t* bitmap, const orc_meshlets* m, uint32_t fir
 
 /* ResolveDebug(BaseColor): layer 0 becomes RGBA8 albedo, sky = checker bg. */
 void orc_resolve_albedo(orc_framebuffer* fb, const orc_shading_ctx* ctx);
+/* ResolveDebug (Shading.cpp:734-773): layer ORC_DEBUG_*; object_to_world
+ * (mat3) for the Normals and MetallicRoughness layers. */
+#define ORC_DEBUG_BASE_COLOR 0u
+#define ORC_DEBUG_NORMALS 1u
+#define ORC_DEBUG_METALLIC_ROUGHNESS 2u
+#define ORC_DEBUG_MESHLET_ID 3u
+#define ORC_DEBUG_TRIANGLE_ID 4u
+void orc_resolve_debug(orc_framebuffer* fb, const orc_shading_ctx* ctx, uint32_t layer, const float* object_to_world);
 /* Resolve(): lit, tonemapped RGBA8 into layer 0. */
 void orc_resolve_lit(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_resolve_params* p);
 

@@ -134,3 +134,30 @@ def test_light_sprites(gpu_ctx):
             assert np.abs(gl - ol).max() <= 1
     fb.close()
     ds.close()
+
+
+@pytest.mark.parametrize("layer", ["normals", "metallic_roughness", "meshlet_id", "triangle_id"])
+def test_debug_layers(gpu_ctx, layer):
+    """The other ResolveDebug layers (Shading.cpp:734-773) on a normal-mapped
+    scene mixing one- and two-layer textures, bit-exact with the oracle."""
+    import glimpsw_amd as g
+
+    modes = {"normals": (g.GLIMP_RESOLVE_NORMALS, oracle.DEBUG_NORMALS),
+             "metallic_roughness": (g.GLIMP_RESOLVE_METALLIC_ROUGHNESS, oracle.DEBUG_METALLIC_ROUGHNESS),
+             "meshlet_id": (g.GLIMP_RESOLVE_MESHLET_ID, oracle.DEBUG_MESHLET_ID),
+             "triangle_id": (g.GLIMP_RESOLVE_TRIANGLE_ID, oracle.DEBUG_TRIANGLE_ID)}
+    gmode, omode = modes[layer]
+    sc = T.normal_mapped_scene(384, seed_offset=6, near_frac=0.2, mixed=True)
+    view = S.make_view(320, 240, count=sc.count)
+    lighting = S.default_lighting(view)
+    ds = g.Scene(gpu_ctx, sc)
+    fb = g.render_frame(gpu_ctx, ds, view)
+    fb.resolve(ds, view.object_to_clip, gmode, lighting)
+    got = fb.read_color()
+    fb.close()
+    ds.close()
+    ofb, _, b = oracle.render_frame(sc, view)
+    want = oracle.resolve_debug(ofb, b, view.object_to_clip, omode, lighting.object_to_world)
+    bad = got != want
+    print(f"[resolve] debug layer {layer}: {bad.sum()} pixels differ", flush=True)
+    assert not bad.any(), np.argwhere(bad)[:3].tolist()
