@@ -49,10 +49,16 @@ class MaterialDesc(C.Structure):
                 ("pad", C.c_uint8 * 2)]
 
 
+class OcclusionDesc(C.Structure):
+    _fields_ = [("depth_map", C.c_void_p), ("object_to_prev_view", C.c_float * 16),
+                ("object_to_world_scale", C.c_float), ("proj_factors", C.c_float * 3)]
+
+
 class DrawDesc(C.Structure):
     _fields_ = [("meshlet_offset", C.c_uint32), ("meshlet_count", C.c_uint32),
                 ("object_to_clip", C.c_float * 16), ("frustum_planes", f32p), ("flags", C.c_uint32),
-                ("pipeline", C.c_uint32), ("bin_size", C.c_uint32), ("bin_capacity", C.c_uint32)]
+                ("pipeline", C.c_uint32), ("bin_size", C.c_uint32), ("bin_capacity", C.c_uint32),
+                ("occlusion", C.POINTER(OcclusionDesc))]
 
 
 class ViewDesc(C.Structure):
@@ -154,6 +160,12 @@ _SIGNATURES = {
     "glimp_draw_meshlets": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(DrawDesc), C.c_void_p]),
     "glimp_cull_meshlets": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, f32p, u16p,
                                       C.POINTER(C.c_uint32), C.c_void_p]),
+    "glimp_cull_meshlets_occlusion": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, f32p,
+                                                C.POINTER(OcclusionDesc), u16p, C.POINTER(C.c_uint32), C.c_void_p]),
+    "glimp_hiz_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "glimp_hiz_destroy": (None, [C.c_void_p]),
+    "glimp_hiz_build": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "glimp_hiz_read": (C.c_int, [C.c_void_p, f32p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.c_void_p]),
     "glimp_get_counters": (C.c_int, [C.c_void_p, C.POINTER(CountersStruct)]),
     "glimp_reset_counters": (C.c_int, [C.c_void_p]),
     "glimp_resolve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(ResolveDesc), C.c_void_p]),
@@ -332,6 +344,61 @@ class Scene:
                "glimp_cull_meshlets")
         return bitmap[: (count + 15) // 16], int(vis.value)
 
+    def cull_occlusion(self, planes: np.ndarray, first: int, count: int, occlusion: "Occlusion", stream=None):
+        """CullMeshlets with the occlusion branch (HiZ test of the frustum survivors)."""
+        bitmap = np.zeros(max(1, (count + 15) // 16), dtype=np.uint16)
+        vis = C.c_uint32()
+        planes = np.ascontiguousarray(planes, dtype=np.float32)
+        _check(self.ctx.lib.glimp_cull_meshlets_occlusion(self.ctx.handle, self.handle, first, count,
+                                                          _ptr(planes, f32p), C.byref(occlusion.desc),
+                                                          _ptr(bitmap, u16p), C.byref(vis), _stream_ptr(stream)),
+               "glimp_cull_meshlets_occlusion")
+        return bitmap[: (count + 15) // 16], int(vis.value)
+
+
+class DepthMap:
+    """The previous frame's min-depth pyramid for occlusion culling
+    (glimp_hiz_* in include/glimp.h)."""
+
+    def __init__(self, ctx: "Context", width: int, height: int):
+        self.ctx = ctx
+        self.width, self.height = width, height
+        h = C.c_void_p()
+        _check(ctx.lib.glimp_hiz_create(ctx.handle, width, height, C.byref(h)), "glimp_hiz_create")
+        self.handle = h.value
+
+    def build(self, fb: "Framebuffer", stream=None):
+        _check(self.ctx.lib.glimp_hiz_build(self.handle, fb.handle, _stream_ptr(stream)), "glimp_hiz_build")
+
+    def read(self, stream=None):
+        levels, floats = C.c_uint32(), C.c_uint64()
+        _check(self.ctx.lib.glimp_hiz_read(self.handle, None, C.byref(levels), C.byref(floats), None), "glimp_hiz_read")
+        out = np.zeros(max(int(floats.value), 1), np.float32)
+        _check(self.ctx.lib.glimp_hiz_read(self.handle, _ptr(out, f32p), None, None, _stream_ptr(stream)),
+               "glimp_hiz_read")
+        return out[: int(floats.value)], int(levels.value)
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.glimp_hiz_destroy(self.handle)
+            self.handle = None
+
+
+class Occlusion:
+    """Occlusion inputs of one cull or draw: the depth map and the terms
+    CullMeshlets derives from its matrices (Shading.cpp:779-795)."""
+
+    def __init__(self, depth_map: DepthMap, object_to_prev_view, object_to_world_scale: float, proj_factors):
+        self.depth_map = depth_map
+        d = OcclusionDesc()
+        d.depth_map = depth_map.handle
+        for i in range(16):
+            d.object_to_prev_view[i] = float(object_to_prev_view[i])
+        d.object_to_world_scale = float(object_to_world_scale)
+        for i in range(3):
+            d.proj_factors[i] = float(proj_factors[i])
+        self.desc = d
+
 
 class Framebuffer:
     """Device vis-buffer of 64-bit depth-test keys (include/glimp.h)."""
@@ -353,7 +420,7 @@ class Framebuffer:
                "glimp_framebuffer_clear")
 
     def draw(self, scene: Scene, object_to_clip, count: int, offset: int = 0, planes=None, flags: int = 0,
-             stream=None, bin_size: int = 0, bin_capacity: int = 0):
+             stream=None, bin_size: int = 0, bin_capacity: int = 0, occlusion: "Occlusion | None" = None):
         d = DrawDesc()
         d.meshlet_offset, d.meshlet_count = offset, count
         for i in range(16):
@@ -365,6 +432,8 @@ class Framebuffer:
         d.flags = flags
         d.pipeline = GLIMP_PIPELINE_VISBUFFER
         d.bin_size, d.bin_capacity = bin_size, bin_capacity
+        if occlusion is not None:
+            d.occlusion = C.pointer(occlusion.desc)
         _check(self.ctx.lib.glimp_draw_meshlets(self.ctx.handle, self.handle, scene.handle, C.byref(d),
                                                 _stream_ptr(stream)), "glimp_draw_meshlets")
 

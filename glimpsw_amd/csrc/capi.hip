@@ -464,6 +464,48 @@ int glimp_framebuffer_read(glimp_framebuffer* f, uint32_t* ids, float* depth, in
 
 }  // extern "C"
 
+struct glimp_hiz {
+    glimp_context* ctx = nullptr;
+    Occlusion layout = {};             // depth map pointer, sizes and level table
+    float* data = nullptr;
+    size_t floats = 0;
+};
+
+namespace {
+
+// Depth-map layout for a width x height frame: the reference's level count
+// (power-of-two sides above (size - 1) / 2, levels while both sides are at
+// least 4, Main.cpp:54-56 and Texture.h:606-611) and levels 0 .. count - 2
+// stored row-major.
+size_t hiz_layout(uint32_t W, uint32_t H, Occlusion& o) {
+    auto half = [](uint32_t v) { uint32_t h = (v - 1) / 2; return h ? 1u << (32 - __builtin_clz(h)) : 1u; };
+    const uint32_t hw = half(W), hh = half(H);
+    uint32_t levels = 0;
+    for (; levels < 16; levels++)
+        if ((hw >> levels) < 4 || (hh >> levels) < 4) break;
+    o.mip_levels = levels;
+    o.width = W;
+    o.height = H;
+    size_t off = 0;
+    for (uint32_t m = 0; m + 1 < levels; m++) {
+        o.level_offset[m] = (uint32_t)off;
+        o.level_width[m] = ((W - 1) >> (m + 1)) + 1;
+        off += (size_t)o.level_width[m] * (((H - 1) >> (m + 1)) + 1);
+    }
+    return off;
+}
+
+int make_occlusion(const glimp_occlusion* d, const std::string& who, Occlusion& o) {
+    if (!d->depth_map) return fail(GLIMP_E_INVALID, who + ": occlusion without a depth map");
+    o = d->depth_map->layout;
+    memcpy(o.m, d->object_to_prev_view, sizeof(o.m));
+    o.scale = d->object_to_world_scale;
+    memcpy(o.proj, d->proj_factors, sizeof(o.proj));
+    return GLIMP_OK;
+}
+
+}  // namespace
+
 namespace {
 
 struct LaunchShape {
@@ -480,7 +522,7 @@ size_t clip_capacity_for(size_t meshlets) { return std::max<size_t>(4096, meshle
 
 // One view of a draw (ShadingContext + DrawMeshlets arguments).
 void make_view(glimp_framebuffer* f, glimp_scene* s, uint32_t meshlet_offset, uint32_t meshlet_count,
-               const float* clip, const float* planes, uint32_t flags, ViewParams& V) {
+               const float* clip, const float* planes, uint32_t flags, ViewParams& V, const Occlusion* occ = nullptr) {
     memset(&V, 0, sizeof(V));
     V.m = s->m;
     V.materials = s->materials;
@@ -497,6 +539,7 @@ void make_view(glimp_framebuffer* f, glimp_scene* s, uint32_t meshlet_offset, ui
     V.clipping = !(flags & GLIMP_DRAW_NO_CLIPPING);
     V.use_cull = planes != nullptr;
     if (planes) memcpy(V.planes, planes, sizeof(V.planes));
+    if (planes && occ) V.occ = *occ;
     memcpy(V.clip, clip, sizeof(V.clip));
     V.clear_depth = f->clear_depth;
     V.fresh = f->pending_clear ? 1 : 0;
@@ -595,8 +638,15 @@ int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, 
     hipStream_t st = (hipStream_t)stream;
     if (direct) HIP_TRY(flush_clear(f, st));
 
+    Occlusion occ;
+    if (d->occlusion) {
+        if (!d->frustum_planes) return fail(GLIMP_E_INVALID, "glimp_draw_meshlets: occlusion needs frustum planes");
+        int orc = make_occlusion(d->occlusion, "glimp_draw_meshlets", occ);
+        if (orc != GLIMP_OK) return orc;
+    }
     std::vector<ViewParams> views(1);
-    make_view(f, s, d->meshlet_offset, d->meshlet_count, d->object_to_clip, d->frustum_planes, d->flags, views[0]);
+    make_view(f, s, d->meshlet_offset, d->meshlet_count, d->object_to_clip, d->frustum_planes, d->flags, views[0],
+              d->occlusion ? &occ : nullptr);
     const uint32_t total_bins = direct ? 0 : layout_bins(views, sh.bin_size);
     int rc = prepare_scratch(c->draw_scratch, d->meshlet_count, total_bins, sh, !direct, s->has_alpha);
     if (rc != GLIMP_OK) return rc;
@@ -881,22 +931,103 @@ int glimp_batch_autotune(glimp_batch* b, void* stream, glimp_batch_shape* chosen
     return GLIMP_OK;
 }
 
-int glimp_cull_meshlets(glimp_context* c, glimp_scene* s, uint32_t first, uint32_t count, const float planes[20],
-                        uint16_t* bitmap, uint32_t* visible, void* stream) {
-    if (!c || !s || !planes) return fail(GLIMP_E_INVALID, "glimp_cull_meshlets: NULL argument");
-    if ((uint64_t)first + count > s->m.count) return fail(GLIMP_E_INVALID, "glimp_cull_meshlets: range outside the scene");
+}  // extern "C"
+
+namespace {
+
+int cull_common(glimp_context* c, glimp_scene* s, uint32_t first, uint32_t count, const float planes[20],
+                const glimp_occlusion* occ, uint16_t* bitmap, uint32_t* visible, void* stream,
+                const std::string& who) {
+    if (!c || !s || !planes) return fail(GLIMP_E_INVALID, who + ": NULL argument");
+    if ((uint64_t)first + count > s->m.count) return fail(GLIMP_E_INVALID, who + ": range outside the scene");
+    Occlusion o;
+    if (occ) {
+        int rc = make_occlusion(occ, who, o);
+        if (rc != GLIMP_OK) return rc;
+    }
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t st = (hipStream_t)stream;
     size_t words = (count + 15) / 16;
     int rc = grow_one(c->bitmap, c->bitmap_cap, words);
     if (rc != GLIMP_OK) return rc;
     HIP_TRY(hipMemsetAsync(c->visible, 0, 4, st));
-    if (count) HIP_TRY(launch_cull(s->m, first, count, planes, c->bitmap, c->visible, st));
+    if (count) HIP_TRY(launch_cull(s->m, first, count, planes, occ ? &o : nullptr, c->approx(), c->bitmap, c->visible, st));
     if (bitmap && words) HIP_TRY(hipMemcpyAsync(bitmap, c->bitmap, words * 2, hipMemcpyDeviceToHost, st));
     uint32_t v = 0;
     HIP_TRY(hipMemcpyAsync(&v, c->visible, 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (visible) *visible = v;
+    return GLIMP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int glimp_cull_meshlets(glimp_context* c, glimp_scene* s, uint32_t first, uint32_t count, const float planes[20],
+                        uint16_t* bitmap, uint32_t* visible, void* stream) {
+    return cull_common(c, s, first, count, planes, nullptr, bitmap, visible, stream, "glimp_cull_meshlets");
+}
+
+int glimp_cull_meshlets_occlusion(glimp_context* c, glimp_scene* s, uint32_t first, uint32_t count,
+                                  const float planes[20], const glimp_occlusion* occ, uint16_t* bitmap,
+                                  uint32_t* visible, void* stream) {
+    if (!occ) return fail(GLIMP_E_INVALID, "glimp_cull_meshlets_occlusion: NULL occlusion");
+    return cull_common(c, s, first, count, planes, occ, bitmap, visible, stream, "glimp_cull_meshlets_occlusion");
+}
+
+int glimp_hiz_create(glimp_context* c, uint32_t width, uint32_t height, glimp_hiz** out) {
+    if (!c || !out) return fail(GLIMP_E_INVALID, "glimp_hiz_create: NULL argument");
+    *out = nullptr;
+    if (width == 0 || height == 0 || width % 4 || height % 4 || width > GLIMP_MAX_RENDER_SIZE ||
+        height > GLIMP_MAX_RENDER_SIZE)
+        return fail(GLIMP_E_INVALID, "glimp_hiz_create: sizes must be multiples of 4 up to GLIMP_MAX_RENDER_SIZE");
+    glimp_hiz* h = new (std::nothrow) glimp_hiz();
+    if (!h) return fail(GLIMP_E_NOMEM, "glimp_hiz_create: out of host memory");
+    h->ctx = c;
+    h->floats = hiz_layout(width, height, h->layout);
+    if (h->layout.mip_levels < 2) {
+        delete h;
+        return fail(GLIMP_E_INVALID, "glimp_hiz_create: framebuffer too small for a depth map (needs 2 levels)");
+    }
+    hipError_t e = hipSetDevice(c->device);
+    if (e == hipSuccess) e = dev_alloc(&h->data, h->floats);
+    if (e != hipSuccess) {
+        delete h;
+        return fail(GLIMP_E_HIP, std::string("glimp_hiz_create: ") + hipGetErrorString(e));
+    }
+    h->layout.depth_map = h->data;
+    *out = h;
+    return GLIMP_OK;
+}
+
+void glimp_hiz_destroy(glimp_hiz* h) {
+    if (!h) return;
+    (void)hipSetDevice(h->ctx->device);
+    if (h->data) (void)hipFree(h->data);
+    delete h;
+}
+
+int glimp_hiz_build(glimp_hiz* h, glimp_framebuffer* f, void* stream) {
+    if (!h || !f) return fail(GLIMP_E_INVALID, "glimp_hiz_build: NULL argument");
+    if (f->width != h->layout.width || f->height != h->layout.height)
+        return fail(GLIMP_E_INVALID, "glimp_hiz_build: framebuffer and depth map sizes differ");
+    HIP_TRY(hipSetDevice(h->ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(flush_clear(f, s));
+    HIP_TRY(launch_hiz_build(f->keys, h->layout, h->data, s));
+    return GLIMP_OK;
+}
+
+int glimp_hiz_read(glimp_hiz* h, float* out, uint32_t* levels, uint64_t* floats, void* stream) {
+    if (!h) return fail(GLIMP_E_INVALID, "glimp_hiz_read: NULL depth map");
+    if (levels) *levels = h->layout.mip_levels;
+    if (floats) *floats = h->floats;
+    if (!out) return GLIMP_OK;
+    HIP_TRY(hipSetDevice(h->ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipMemcpyAsync(out, h->data, h->floats * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return GLIMP_OK;
 }
 

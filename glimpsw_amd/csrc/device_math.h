@@ -145,6 +145,12 @@ __device__ __forceinline__ int32_t cvt_rne(float x) {
     return (int32_t)__builtin_rintf(x);
 }
 
+// cvttps2dq: truncation toward zero; out of range and NaN give INT32_MIN.
+__device__ __forceinline__ int32_t cvt_trunc(float x) {
+    if (!(x > -2147483904.0f && x < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)x;
+}
+
 // Packed signed 16-bit pairs: x in the low half, y in the high half.
 __device__ __forceinline__ int32_t lo16(uint32_t v) { return (int32_t)(int16_t)(v & 0xFFFFu); }
 __device__ __forceinline__ int32_t hi16(uint32_t v) { return (int32_t)(int16_t)(v >> 16); }

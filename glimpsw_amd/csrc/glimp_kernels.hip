@@ -207,6 +207,49 @@ __device__ __forceinline__ bool counting(const DrawParams& p) { return (p.flags 
 
 // ---------------------------------------------------------------- cull
 
+// ProjectSphere (Shading.cpp:264-279) and the occlusion test of
+// CullMeshlets (Shading.cpp:811-840) for one bound sphere that passed the
+// frustum test, in the operation order of the reference's optimised build:
+// the sphere's screen box picks the depth map level whose texels are about
+// half the box, and the meshlet stays visible when the sphere's nearest
+// depth lies in front of the farthest depth under the box. Spheres reaching
+// the near plane are kept.
+__device__ __forceinline__ bool hiz_visible(const ApproxTables& T, const Occlusion& o, float4 b) {
+    const float* m = o.m;
+    const float cx = __builtin_fmaf(b.x, m[0], __builtin_fmaf(b.y, m[4], __builtin_fmaf(b.z, m[8], m[12])));
+    const float cy = __builtin_fmaf(b.x, m[1], __builtin_fmaf(b.y, m[5], __builtin_fmaf(b.z, m[9], m[13])));
+    const float cz = __builtin_fmaf(b.x, m[2], __builtin_fmaf(b.y, m[6], __builtin_fmaf(b.z, m[10], m[14])));
+    const float r = b.w * o.scale;
+    const float nz = -cz;
+    if (!((r + o.proj[0]) <= nz)) return true;
+    const float crx = cx * r, cry = cy * r, crz = r * nz;
+    const float ax = __builtin_fmaf(cx, cx, -(r * r)) + cz * cz, ay = __builtin_fmaf(cy, cy, -(r * r)) + cz * cz;
+    const float vx = ax * rsqrt14(T, ax), vy = ay * rsqrt14(T, ay);
+    const float sxh = o.proj[1] * 0.5f, syh = o.proj[2] * 0.5f;
+    const float b0 = __builtin_fmaf((vx * cx - crz) * rcp14(T, vx * nz + crx), sxh, 0.5f);
+    const float b1 = __builtin_fmaf((vy * cy + crz) * rcp14(T, vy * nz - cry), syh, 0.5f);
+    const float b2 = __builtin_fmaf((vx * cx + crz) * rcp14(T, vx * nz - crx), sxh, 0.5f);
+    const float b3 = __builtin_fmaf((vy * cy - crz) * rcp14(T, vy * nz + cry), syh, 0.5f);
+    const float fw = (float)o.width, fh = (float)o.height;
+    const float sx = (b2 - b0) * fw, sy = (b3 - b1) * fh;
+    const float mx = (sx > sy) ? sx : sy;
+    int32_t lv = (int32_t)(fbits(mx) - (127u << 23)) >> 23;
+    lv = min(max(lv, 1), (int32_t)o.mip_levels - 1);
+    const int32_t x0 = max(cvt_trunc(b0 * fw), 0) >> lv;
+    const int32_t y0 = max(cvt_trunc(b1 * fh), 0) >> lv;
+    const int32_t x1 = min(cvt_trunc(b2 * fw), (int32_t)o.width - 1) >> lv;
+    const int32_t y1 = min(cvt_trunc(b3 * fh), (int32_t)o.height - 1) >> lv;
+    const float dsphere = fdiv(T, o.proj[0], -(cz + r));
+    // level lv - 1 of the depth map covers 2^lv pixels per texel; the range
+    // lies inside it (x1 <= (width - 1) >> lv, x0 >= 0)
+    const float* D = o.depth_map + o.level_offset[lv - 1];
+    const int32_t w = (int32_t)o.level_width[lv - 1];
+    float vis = 3.40282347e38f;
+    for (int32_t y = y0; y <= y1; y++)
+        for (int32_t x = x0; x <= x1; x++) vis = fminf(vis, D[y * w + x]);
+    return dsphere > vis;
+}
+
 // Frustum test of bound spheres (Shading.cpp:797-809) over the global
 // meshlet numbering. Survivors are compacted into visible_list so later
 // stages launch work only for them: a block covers kCullPerBlock consecutive
@@ -245,6 +288,7 @@ __global__ void __launch_bounds__(256) k_cull(DrawParams p) {
                     float d = __builtin_fmaf(b.x, P[0], __builtin_fmaf(b.y, P[1], b.z * P[2])) + P[3];
                     ok = ok && (d > -b.w);
                 }
+                if (ok && V.occ.depth_map) ok = hiz_visible(p.tables, V.occ, b);
             }
         }
         vis[k] = ok;
@@ -272,8 +316,8 @@ __global__ void __launch_bounds__(256) k_cull(DrawParams p) {
 }
 
 // Standalone cull of one meshlet range (ShadingContext::CullMeshlets).
-__global__ void k_cull_range(DevMeshlets m, uint32_t first, uint32_t count, Planes5 pl, uint16_t* bitmap,
-                             uint32_t* visible) {
+__global__ void k_cull_range(DevMeshlets m, uint32_t first, uint32_t count, Planes5 pl, Occlusion occ,
+                             ApproxTables tables, uint16_t* bitmap, uint32_t* visible) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool vis = false;
     if (i < count) {
@@ -285,6 +329,7 @@ __global__ void k_cull_range(DevMeshlets m, uint32_t first, uint32_t count, Plan
             float d = __builtin_fmaf(b.x, P[0], __builtin_fmaf(b.y, P[1], b.z * P[2])) + P[3];
             vis = vis && (d > -b.w);
         }
+        if (vis && occ.depth_map) vis = hiz_visible(tables, occ, b);
     }
     unsigned long long mask = __ballot(vis);
     uint32_t lane = lane_id();
@@ -293,6 +338,34 @@ __global__ void k_cull_range(DevMeshlets m, uint32_t first, uint32_t count, Plan
         uint32_t c = __popcll(mask);
         if (c) atomicAdd(visible, c);
     }
+}
+
+// texutil::DownsampleDepth (ImageHelpers.cpp:149-247), one level per launch:
+// level 0 texel = minimum depth of its 2x2 pixels of the vis-buffer, level
+// lv texel = minimum of its 2x2 texels of level lv - 1; pixels and texels
+// outside the frame count as FLT_MAX. Thread = texel.
+__global__ void __launch_bounds__(256) k_hiz_level(const unsigned long long* keys, uint32_t W, uint32_t H,
+                                                   uint32_t lv, const float* prev, uint32_t prev_w, float* out,
+                                                   uint32_t w, uint32_t h) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= w * h) return;
+    const uint32_t j = t / w, i = t - j * w;
+    float v = 3.40282347e38f;
+    if (lv == 0) {
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t x = 2 * i + (k & 1), y = 2 * j + (k >> 1);
+            if (x < W && y < H) v = fminf(v, bitsf((uint32_t)(keys[pixel_offset(W, x, y)] >> 32)));
+        }
+    } else {
+        const uint32_t ph = ((H - 1) >> lv) + 1;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t x = 2 * i + (k & 1), y = 2 * j + (k >> 1);
+            if (x < prev_w && y < ph) v = fminf(v, prev[y * prev_w + x]);
+        }
+    }
+    out[j * w + i] = v;
 }
 
 // ---------------------------------------------------------------- mesh + setup
@@ -1372,11 +1445,27 @@ hipError_t launch_decode(const unsigned long long* keys, int width, int height, 
     return hipGetLastError();
 }
 
-hipError_t launch_cull(const DevMeshlets& m, uint32_t first, uint32_t count, const float* planes, uint16_t* bitmap,
-                       uint32_t* visible, hipStream_t s) {
+hipError_t launch_cull(const DevMeshlets& m, uint32_t first, uint32_t count, const float* planes,
+                       const Occlusion* occ, const ApproxTables& tables, uint16_t* bitmap, uint32_t* visible,
+                       hipStream_t s) {
     Planes5 pl;
     for (int i = 0; i < 20; i++) pl.p[i] = planes[i];
-    hipLaunchKernelGGL(k_cull_range, dim3((count + 255) / 256), dim3(256), 0, s, m, first, count, pl, bitmap, visible);
+    Occlusion o = {};
+    if (occ) o = *occ;
+    hipLaunchKernelGGL(k_cull_range, dim3((count + 255) / 256), dim3(256), 0, s, m, first, count, pl, o, tables,
+                       bitmap, visible);
+    return hipGetLastError();
+}
+
+hipError_t launch_hiz_build(const unsigned long long* keys, const Occlusion& occ, float* depth_map, hipStream_t s) {
+    for (uint32_t lv = 0; lv + 1 < occ.mip_levels; lv++) {
+        const uint32_t w = occ.level_width[lv];
+        const uint32_t h = ((occ.height - 1) >> (lv + 1)) + 1;
+        const uint32_t n = w * h;
+        hipLaunchKernelGGL(k_hiz_level, dim3((n + 255) / 256), dim3(256), 0, s, keys, occ.width, occ.height, lv,
+                           lv ? depth_map + occ.level_offset[lv - 1] : nullptr, lv ? occ.level_width[lv - 1] : 0u,
+                           depth_map + occ.level_offset[lv], w, h);
+    }
     return hipGetLastError();
 }
 

@@ -126,6 +126,21 @@ struct DrawScratch {
     uint32_t pad[3];
 };
 
+// Occlusion inputs of CullMeshlets (Shading.cpp:775-869): the previous
+// frame's depth map, a min-depth pyramid whose level m texel covers 2^(m+1)
+// pixels (texutil::DownsampleDepth), and the sphere transform terms.
+// depth_map == nullptr: no occlusion test.
+struct Occlusion {
+    const float* depth_map;        // levels 0 .. mip_levels - 2, row-major, concatenated
+    uint32_t mip_levels;           // of the reference's depth texture (CreateTexture2D rule)
+    uint32_t width, height;        // frame size
+    uint32_t level_offset[16];     // in floats
+    uint32_t level_width[16];
+    float m[16];                   // objectToPrevViewMat, column-major
+    float scale;                   // objectToWorldScale
+    float proj[3];                 // projFactors: znear, f/aspect, -f
+};
+
 // One view of a draw or batch: which meshlets of which scene are drawn into
 // which framebuffer with which matrix. A single DrawMeshlets call is a batch
 // of one view; a batch renders many views with one launch per stage.
@@ -147,6 +162,7 @@ struct ViewParams {
     unsigned long long* keys;      // the view's vis-buffer, 4x4 tiles
     float clip[16];
     float planes[20];
+    Occlusion occ;                 // with use_cull: HiZ test of the frustum survivors
 };
 
 // Launch-wide parameters. Meshlets of all views are numbered globally
@@ -194,6 +210,9 @@ hipError_t launch_decode(const unsigned long long* keys, int width, int height, 
                          uint32_t clear_color, const DrawEntry* draws, uint32_t num_draws, uint32_t* ids,
                          float* depth, int linear, hipStream_t s);
 hipError_t launch_cull(const DevMeshlets& m, uint32_t first, uint32_t count, const float* planes,
-                       uint16_t* bitmap, uint32_t* visible, hipStream_t s);
+                       const Occlusion* occ, const ApproxTables& tables, uint16_t* bitmap, uint32_t* visible,
+                       hipStream_t s);
+// Min-depth pyramid of a vis-buffer into occ.depth_map (all levels).
+hipError_t launch_hiz_build(const unsigned long long* keys, const Occlusion& occ, float* depth_map, hipStream_t s);
 
 }  // namespace glimp

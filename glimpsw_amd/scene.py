@@ -411,3 +411,17 @@ def default_lighting(view: View) -> Lighting:
     return Lighting(lights, 1.0, eye, np.eye(3, dtype=np.float32).reshape(9),
                     inverse_screen_proj(view.object_to_clip, view.width, view.height),
                     np.asarray(view.object_to_clip, np.float32))
+
+
+def occlusion_terms(proj_mat: np.ndarray, prev_view_mat: np.ndarray, model_mat: np.ndarray | None = None):
+    """The terms CullMeshlets derives from its matrices (Shading.cpp:779-795),
+    computed in float64 and rounded once: objectToPrevViewMat = prevView *
+    model (column-major), objectToWorldScale = |model column 0| and
+    projFactors = (proj[3][2], proj[0][0], proj[1][1])."""
+    M = np.eye(4) if model_mat is None else model_mat.astype(np.float64).reshape(4, 4).T
+    Vp = prev_view_mat.astype(np.float64).reshape(4, 4).T
+    o2pv = (Vp @ M).T.reshape(16).astype(np.float32)
+    scale = np.float32(np.linalg.norm(M[:3, 0]))
+    P = proj_mat.astype(np.float64).reshape(16)
+    proj = np.array([P[14], P[0], P[5]], np.float32)
+    return o2pv, scale, proj

@@ -143,6 +143,7 @@ typedef struct glimp_draw_desc {
     uint32_t pipeline;                 /* GLIMP_PIPELINE_* */
     uint32_t bin_size;                 /* screen bin side in pixels: 8, 16, 32 or 64; 0 = default */
     uint32_t bin_capacity;             /* records per bin before spilling, 0 = default */
+    const struct glimp_occlusion* occlusion; /* with frustum_planes: HiZ test of the survivors, or NULL */
 } glimp_draw_desc;
 
 /* Rasterizer::DrawMeshlets(fb, count, {VisBufferShader, &ctx})
@@ -157,6 +158,40 @@ int glimp_draw_meshlets(glimp_context* ctx, glimp_framebuffer* fb, glimp_scene* 
  * count in *visible. Synchronous. */
 int glimp_cull_meshlets(glimp_context* ctx, glimp_scene* scene, uint32_t first, uint32_t count,
                         const float planes[20], uint16_t* bitmap, uint32_t* visible, void* stream);
+
+/* The previous frame's depth map for occlusion culling: a min-depth pyramid
+ * of a width x height framebuffer with the level count of
+ * CreateTexture2D<R32f>(halfW, halfH, 16) (Main.cpp:54-56, Texture.h:599-636);
+ * level m texel covers 2^(m+1) x 2^(m+1) pixels. */
+typedef struct glimp_hiz glimp_hiz;
+int glimp_hiz_create(glimp_context* ctx, uint32_t width, uint32_t height, glimp_hiz** out);
+void glimp_hiz_destroy(glimp_hiz* hiz);
+/* texutil::DownsampleDepth (ImageHelpers.cpp:241-245): rebuilds the pyramid
+ * from fb's depth (same size as the map). Asynchronous. */
+int glimp_hiz_build(glimp_hiz* hiz, glimp_framebuffer* fb, void* stream);
+/* Copies the sampled levels (0 .. levels - 2, row-major, level m with
+ * ((width-1) >> (m+1)) + 1 columns, concatenated) to host `out` (NULL: only
+ * the sizes); *levels = the map's level count, *floats = the total.
+ * Synchronous. */
+int glimp_hiz_read(glimp_hiz* hiz, float* out, uint32_t* levels, uint64_t* floats, void* stream);
+
+/* Occlusion inputs of CullMeshlets (Shading.cpp:775-869): the depth map of
+ * the previous frame and the terms the reference derives from its matrices:
+ * objectToPrevViewMat = prevViewMat * modelMat, objectToWorldScale =
+ * length(modelMat[0]), projFactors = (projMat[3][2], projMat[0][0],
+ * projMat[1][1]). The frame size is the depth map's. */
+typedef struct glimp_occlusion {
+    const glimp_hiz* depth_map;
+    float object_to_prev_view[16];     /* column-major */
+    float object_to_world_scale;
+    float proj_factors[3];             /* znear, f/aspect, -f */
+} glimp_occlusion;
+
+/* CullMeshlets with the occlusion branch (Shading.cpp:811-840): frustum test,
+ * then the HiZ test of the survivors. As glimp_cull_meshlets. */
+int glimp_cull_meshlets_occlusion(glimp_context* ctx, glimp_scene* scene, uint32_t first, uint32_t count,
+                                  const float planes[20], const glimp_occlusion* occlusion, uint16_t* bitmap,
+                                  uint32_t* visible, void* stream);
 
 /* ------------------------------------------------------------ batches */
 

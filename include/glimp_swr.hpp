@@ -97,6 +97,23 @@ private:
     glimp_texture* h_ = nullptr;
 };
 
+// The previous frame's depth map for occlusion culling (Main.cpp:54-56,
+// 257-260): Build() is texutil::DownsampleDepth.
+class DepthMap {
+public:
+    DepthMap(Context& ctx, uint32_t width, uint32_t height) {
+        check(glimp_hiz_create(ctx.get(), width, height, &h_), "glimp_hiz_create");
+    }
+    ~DepthMap() { glimp_hiz_destroy(h_); }
+    DepthMap(const DepthMap&) = delete;
+    DepthMap& operator=(const DepthMap&) = delete;
+    void Build(Framebuffer& fb, void* stream = nullptr) { check(glimp_hiz_build(h_, fb.get(), stream), "glimp_hiz_build"); }
+    glimp_hiz* get() const { return h_; }
+
+private:
+    glimp_hiz* h_ = nullptr;
+};
+
 // Scene::Meshlets + Materials (Scene.h:92-128), resident on the device.
 class Scene {
 public:
@@ -198,6 +215,17 @@ public:
         uint32_t visible = 0;
         check(glimp_cull_meshlets(ctx.get(), scene_->get(), first, count, planes, bitmap, &visible, stream),
               "glimp_cull_meshlets");
+        return visible;
+    }
+    // CullMeshlets with the occlusion branch (Shading.cpp:811-840): the terms
+    // the reference derives from projMat, modelMat and prevViewMat are passed
+    // in `occ` (see glimp_occlusion).
+    uint32_t CullMeshlets(Context& ctx, uint16_t* bitmap, uint32_t first, uint32_t count, const float planes[20],
+                          const glimp_occlusion& occ, void* stream = nullptr) {
+        uint32_t visible = 0;
+        check(glimp_cull_meshlets_occlusion(ctx.get(), scene_->get(), first, count, planes, &occ, bitmap, &visible,
+                                            stream),
+              "glimp_cull_meshlets_occlusion");
         return visible;
     }
     // Resolve (Shading.cpp:658-732).

@@ -214,6 +214,54 @@ def cull(binding: SceneBinding, planes: np.ndarray, first: int, count: int):
     return bitmap, int(n)
 
 
+class Occlusion(C.Structure):
+    _fields_ = [("depth_map", f32p), ("mip_levels", C.c_uint32), ("object_to_prev_view", C.c_float * 16),
+                ("object_to_world_scale", C.c_float), ("proj", C.c_float * 3), ("frame_size", C.c_float * 2)]
+
+
+def hiz_levels(width: int, height: int) -> int:
+    """Mip levels of the reference's depth map for a framebuffer."""
+    lib().orc_hiz_levels.restype = C.c_uint32
+    return int(lib().orc_hiz_levels(C.c_uint32(width), C.c_uint32(height)))
+
+
+def hiz_level_dims(width: int, height: int):
+    return [(((width - 1) >> (m + 1)) + 1, ((height - 1) >> (m + 1)) + 1)
+            for m in range(max(hiz_levels(width, height) - 1, 0))]
+
+
+def hiz_build(fb: OracleFramebuffer) -> np.ndarray:
+    """DownsampleDepth: the min-depth pyramid, levels concatenated row-major."""
+    n = sum(w * h for w, h in hiz_level_dims(fb.width, fb.height))
+    out = np.zeros(max(n, 1), np.float32)
+    lib().orc_hiz_build(C.byref(fb.fb), _ptr(out, f32p))
+    return out[:n]
+
+
+def occlusion(depth_map: np.ndarray, width: int, height: int, object_to_prev_view, scale: float, proj, keep: list):
+    o = Occlusion()
+    dm = np.ascontiguousarray(depth_map, np.float32)
+    keep.append(dm)
+    o.depth_map = _ptr(dm, f32p)
+    o.mip_levels = hiz_levels(width, height)
+    for i in range(16):
+        o.object_to_prev_view[i] = float(object_to_prev_view[i])
+    o.object_to_world_scale = float(scale)
+    for i in range(3):
+        o.proj[i] = float(proj[i])
+    o.frame_size[0], o.frame_size[1] = float(width), float(height)
+    return o
+
+
+def cull_occlusion(binding: SceneBinding, planes: np.ndarray, first: int, count: int, occ: Occlusion):
+    bitmap = np.zeros((count + 15) // 16, dtype=np.uint16)
+    planes = np.ascontiguousarray(planes, dtype=np.float32)
+    lib().orc_cull_meshlets_occlusion.restype = C.c_uint32
+    n = lib().orc_cull_meshlets_occlusion(_ptr(bitmap, u16p), C.byref(binding.meshlets), C.c_uint32(first),
+                                          C.c_uint32(count), _ptr(planes, f32p), C.byref(occ))
+    return bitmap, int(n)
+
+
 def resolve_albedo(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip, meshlet_offset: int = 0):
     ctx = binding.ctx(object_to_clip, meshlet_offset)
     lib().orc_resolve_albedo(C.byref(fb.fb), C.byref(ctx))

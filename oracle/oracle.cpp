@@ -14,6 +14,7 @@
 // structure so draw order and those tile-wide decisions are reproduced.
 #include "oracle.h"
 
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1264,6 +1265,95 @@ void orc_draw_meshlets(orc_framebuffer* fb, uint32_t count, const orc_shading_ct
 // computed by the caller (Gribb-Hartmann, Shading.cpp:783-791).
 uint32_t orc_cull_meshlets(uint16_t* bitmap, const orc_meshlets* m, uint32_t first, uint32_t count,
                            const float planes[20]) {
+    return orc_cull_meshlets_occlusion(bitmap, m, first, count, planes, nullptr);
+}
+
+uint32_t orc_hiz_levels(uint32_t width, uint32_t height) {
+    // CreateFramebuffer's depth map (Main.cpp:54-56): power-of-two sides
+    // above (size - 1) / 2, CreateTexture2D's level rule (Texture.h:606-611)
+    auto half = [](uint32_t v) { uint32_t h = (v - 1) / 2; return h ? 1u << (32 - __builtin_clz(h)) : 1u; };
+    uint32_t hw = half(width), hh = half(height), mip = 0;
+    for (; mip < 16; mip++)
+        if ((hw >> mip) < 4 || (hh >> mip) < 4) break;
+    return mip;
+}
+
+void orc_hiz_build(const orc_framebuffer* fb, float* out) {
+    // texutil::DownsampleDepth (ImageHelpers.cpp:149-247): level m texel (i, j)
+    // is the minimum depth over pixels [i, i+1) x [j, j+1) * 2^(m+1) inside
+    // the framebuffer (pixels outside count as FLT_MAX). The depth map's
+    // top level is never sampled, so levels 0 .. orc_hiz_levels - 2 are kept.
+    const uint32_t W = fb->width, H = fb->height;
+    const float* depth = (const float*)(fb->data + fb->layer_stride);
+    const uint32_t L = orc_hiz_levels(W, H);
+    size_t off = 0;
+    for (uint32_t m = 0; m + 1 < L; m++) {
+        const uint32_t s = 1u << (m + 1);
+        const uint32_t w = ((W - 1) >> (m + 1)) + 1, h = ((H - 1) >> (m + 1)) + 1;
+        for (uint32_t j = 0; j < h; j++)
+            for (uint32_t i = 0; i < w; i++) {
+                float v = FLT_MAX;
+                for (uint32_t y = j * s; y < std::min((j + 1) * s, H); y++)
+                    for (uint32_t x = i * s; x < std::min((i + 1) * s, W); x++)
+                        v = std::min(v, depth[pixel_offset(W, x, y)]);
+                out[off + (size_t)j * w + i] = v;
+            }
+        off += (size_t)w * h;
+    }
+}
+
+// ProjectSphere (Shading.cpp:264-279) and the occlusion test of
+// CullMeshlets (Shading.cpp:811-840) for one bound sphere, in the operation
+// order of the reference's optimised build. Returns whether the meshlet
+// stays visible.
+static void hiz_sphere(const float* b, const orc_occlusion* o, bool& proj, float bb[4], int32_t& lv, int32_t rect[4],
+                float& dsphere) {
+    const float* m = o->object_to_prev_view;
+    const float cx = fmaf(b[0], m[0], fmaf(b[1], m[4], fmaf(b[2], m[8], m[12])));
+    const float cy = fmaf(b[0], m[1], fmaf(b[1], m[5], fmaf(b[2], m[9], m[13])));
+    const float cz = fmaf(b[0], m[2], fmaf(b[1], m[6], fmaf(b[2], m[10], m[14])));
+    const float r = b[3] * o->object_to_world_scale;
+    const float nz = -cz;
+    proj = (r + o->proj[0]) <= nz;
+    const float crx = cx * r, cry = cy * r, crz = r * nz;
+    const float ax = fmaf(cx, cx, -(r * r)) + cz * cz, ay = fmaf(cy, cy, -(r * r)) + cz * cz;
+    const float vx = ax * rsqrt14(ax), vy = ay * rsqrt14(ay);
+    const float sxh = o->proj[1] * 0.5f, syh = o->proj[2] * 0.5f;
+    bb[0] = fmaf((vx * cx - crz) * rcp14(vx * nz + crx), sxh, 0.5f);
+    bb[1] = fmaf((vy * cy + crz) * rcp14(vy * nz - cry), syh, 0.5f);
+    bb[2] = fmaf((vx * cx + crz) * rcp14(vx * nz - crx), sxh, 0.5f);
+    bb[3] = fmaf((vy * cy - crz) * rcp14(vy * nz + cry), syh, 0.5f);
+    const float fw = o->frame_size[0], fh = o->frame_size[1];
+    const float sx = (bb[2] - bb[0]) * fw, sy = (bb[3] - bb[1]) * fh;
+    const float mx = (sx > sy) ? sx : sy;           // vmaxps: second operand on NaN
+    lv = wrap_sub((int32_t)fbits(mx), 127 << 23) >> 23;
+    lv = std::min(std::max(lv, 1), (int32_t)o->mip_levels - 1);
+    rect[0] = std::max(cvt_trunc(bb[0] * fw), 0) >> lv;
+    rect[1] = std::max(cvt_trunc(bb[1] * fh), 0) >> lv;
+    rect[2] = std::min(cvt_trunc(bb[2] * fw), cvt_trunc(fw) - 1) >> lv;
+    rect[3] = std::min(cvt_trunc(bb[3] * fh), cvt_trunc(fh) - 1) >> lv;
+    dsphere = fdiv(o->proj[0], -(cz + r));
+}
+
+static bool hiz_visible(const float* b, const orc_occlusion* o) {
+    bool proj;
+    float bb[4], dsphere;
+    int32_t lv, rect[4];
+    hiz_sphere(b, o, proj, bb, lv, rect, dsphere);
+    if (!proj) return true;
+    // the depth map's level lv - 1 (its texels cover 2^lv pixels)
+    const uint32_t W = (uint32_t)o->frame_size[0], H = (uint32_t)o->frame_size[1];
+    size_t off = 0;
+    for (int32_t k = 0; k < lv - 1; k++) off += (size_t)(((W - 1) >> (k + 1)) + 1) * (((H - 1) >> (k + 1)) + 1);
+    const uint32_t w = ((W - 1) >> lv) + 1;
+    float vis = FLT_MAX;
+    for (int32_t y = rect[1]; y <= rect[3]; y++)
+        for (int32_t x = rect[0]; x <= rect[2]; x++) vis = std::min(vis, o->depth_map[off + (size_t)y * w + x]);
+    return dsphere > vis;
+}
+
+uint32_t orc_cull_meshlets_occlusion(uint16_t* bitmap, const orc_meshlets* m, uint32_t first, uint32_t count,
+                                     const float planes[20], const orc_occlusion* occ) {
     uint32_t visible = 0;
     for (uint32_t off = 0; off < count; off += 16) {
         uint16_t mask = 0;
@@ -1275,11 +1365,28 @@ uint32_t orc_cull_meshlets(uint16_t* bitmap, const orc_meshlets* m, uint32_t fir
                 float d = fmaf(b[0], P[0], fmaf(b[1], P[1], b[2] * P[2])) + P[3];
                 vis = vis && (d > -b[3]);
             }
+            if (vis && occ) vis = hiz_visible(b, occ);
             if (vis) { mask |= (uint16_t)(1u << l); visible++; }
         }
         bitmap[off / 16] = mask;
     }
     return visible;
+}
+
+void orc_hiz_sphere16(const float* sphere, const orc_occlusion* o, int32_t* mask, float* bb, int32_t* mip,
+                      int32_t* rect, float* depth) {
+    for (int l = 0; l < 16; l++) {
+        const float b[4] = { sphere[l], sphere[16 + l], sphere[32 + l], sphere[48 + l] };
+        bool proj;
+        float q[4];
+        int32_t r[4];
+        hiz_sphere(b, o, proj, q, mip[l], r, depth[l]);
+        mask[l] = proj ? -1 : 0;
+        for (int k = 0; k < 4; k++) {
+            bb[k * 16 + l] = q[k];
+            rect[k * 16 + l] = r[k];
+        }
+    }
 }
 
 void orc_sample_level16(const orc_texture* tex, const float* u, const float* v, int32_t layer,

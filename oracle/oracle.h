@@ -113,6 +113,25 @@ void orc_draw_meshlets(orc_framebuffer* fb, uint32_t count, const orc_shading_ct
                        const orc_raster_opts* opts, orc_counters* counters);
 
 /* Frustum cull of bound spheres against 5 normalised planes (xyz,w each). */
+/* Occlusion inputs of CullMeshlets (Shading.cpp:775-869): the previous
+ * frame's depth map (levels 0 .. mip_levels - 2 of orc_hiz_build, row-major,
+ * level m of ((W-1) >> (m+1)) + 1 columns), objectToPrevViewMat
+ * (column-major), objectToWorldScale, projFactors (znear, f/aspect, -f) and
+ * the frame size. */
+typedef struct orc_occlusion {
+    const float* depth_map;
+    uint32_t mip_levels;
+    float object_to_prev_view[16];
+    float object_to_world_scale;
+    float proj[3];
+    float frame_size[2];
+} orc_occlusion;
+uint32_t orc_hiz_levels(uint32_t width, uint32_t height);
+void orc_hiz_build(const orc_framebuffer* fb, float* out);
+uint32_t orc_cull_meshlets_occlusion(uint16_t* bitmap, const orc_meshlets* m, uint32_t first, uint32_t count,
+                                     const float planes[20], const orc_occlusion* occ);
+void orc_hiz_sphere16(const float* sphere, const orc_occlusion* o, int32_t* mask, float* bb, int32_t* mip,
+                      int32_t* rect, float* depth);
 uint32_t orc_cull_meshlets(uint16_t* bitmap, const orc_meshlets* m, uint32_t first, uint32_t count,
                            const float planes[20]);
 

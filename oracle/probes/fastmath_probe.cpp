@@ -233,6 +233,55 @@ void probe_sprite_tile(int x, int y, float cx, float cy, float radius, const flo
     sti(fg, sat(r) | (sat(g) << 8) | (sat(b) << 16) | (sat(al) << 24));
 }
 
+// Occlusion part of CullMeshlets (Shading.cpp:811-832) with ProjectSphere
+// (Shading.cpp:264-279) for 16 bound spheres: view-space centre (simd::mul,
+// SIMD.h:457-464), projected box, mip level, texel range and sphere depth.
+// in: sphere[4][16] (x, y, z, r), m column-major object-to-previous-view,
+// scale, proj (znear, f/ar, -f), fs (frame size), mipLevels;
+// out: mask[16], bb[4][16], mip[16], rect[4][16] (x0, y0, x1, y1), depth[16]
+void probe_hiz_sphere(const float* sphere, const float* m, float scale, const float* proj, const float* fs,
+                      int mipLevels, int32_t* mask, float* bb, int32_t* mip, int32_t* rect, float* depth) {
+    vf px = ld(sphere), py = ld(sphere + 16), pz = ld(sphere + 32), pr = ld(sphere + 48);
+    vf one = 1.0f;
+    vf cx = vfma(px, m[0], vfma(py, m[4], vfma(pz, m[8], one * m[12])));
+    vf cy = vfma(px, m[1], vfma(py, m[5], vfma(pz, m[9], one * m[13])));
+    vf cz = vfma(px, m[2], vfma(py, m[6], vfma(pz, m[10], one * m[14])));
+    vf r = pr * scale;
+    // ProjectSphere
+    vf c_z = cz * -1.0f;
+    vi msk = c_z >= r + proj[0];
+    vf crx = cx * r, cry = cy * r, crz = c_z * r;
+    vf vx = (vf)_mm512_rsqrt14_ps((__m512)(cx * cx + c_z * c_z - r * r)) * (cx * cx + c_z * c_z - r * r);
+    vf vy = (vf)_mm512_rsqrt14_ps((__m512)(cy * cy + c_z * c_z - r * r)) * (cy * cy + c_z * c_z - r * r);
+    vf rcp0 = (vf)_mm512_rcp14_ps((__m512)(vx * c_z + crx));
+    vf rcp1 = (vf)_mm512_rcp14_ps((__m512)(vy * c_z - cry));
+    vf rcp2 = (vf)_mm512_rcp14_ps((__m512)(vx * c_z - crx));
+    vf rcp3 = (vf)_mm512_rcp14_ps((__m512)(vy * c_z + cry));
+    vf b0 = (vx * cx - crz) * rcp0 * (proj[1] * 0.5f) + 0.5f;
+    vf b1 = (vy * cy + crz) * rcp1 * (proj[2] * 0.5f) + 0.5f;
+    vf b2 = (vx * cx + crz) * rcp2 * (proj[1] * 0.5f) + 0.5f;
+    vf b3 = (vy * cy - crz) * rcp3 * (proj[2] * 0.5f) + 0.5f;
+    vf sx = (b2 - b0) * fs[0], sy = (b3 - b1) * fs[1];
+    vf mx = (vf)_mm512_max_ps((__m512)sx, (__m512)sy);
+    vi lv = (((vi)mx) - (127 << 23)) >> 23;
+    lv = lv < 1 ? 1 : lv;
+    lv = lv > mipLevels - 1 ? mipLevels - 1 : lv;
+    vi x0 = __builtin_convertvector(b0 * fs[0], vi);
+    vi y0 = __builtin_convertvector(b1 * fs[1], vi);
+    vi x1 = __builtin_convertvector(b2 * fs[0], vi);
+    vi y1 = __builtin_convertvector(b3 * fs[1], vi);
+    x0 = (x0 < 0 ? 0 : x0) >> lv;
+    y0 = (y0 < 0 ? 0 : y0) >> lv;
+    x1 = (x1 > (int)fs[0] - 1 ? (int)fs[0] - 1 : x1) >> lv;
+    y1 = (y1 > (int)fs[1] - 1 ? (int)fs[1] - 1 : y1) >> lv;
+    vf d = proj[0] / (-cz - r);
+    sti(mask, msk);
+    st(bb, b0); st(bb + 16, b1); st(bb + 32, b2); st(bb + 48, b3);
+    sti(mip, lv);
+    sti(rect, x0); sti(rect + 16, y0); sti(rect + 32, x1); sti(rect + 48, y1);
+    st(depth, d);
+}
+
 // Screen UV of a 4x4 tile in a full-screen pass (Rasterizer.h:226-237).
 void probe_tile_uv(const int32_t* px, int n, float size, float* out) {
     float scale = 2.0f / size;

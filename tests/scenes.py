@@ -107,3 +107,25 @@ def normal_mapped_scene(n: int = 384, seed_offset: int = 0, spread: float = 6.0,
         mats, texs, mid = [(0, False, 255)], [nmr], np.zeros(n, np.uint32)
     return dataclasses.replace(sc, materials=mats, textures=texs, material_id=mid,
                                tangent_handedness=rng.next_u64(n).astype(np.uint64))
+
+
+def occluder_scene(n: int = 600, seed_offset: int = 0):
+    """Patches spread in front of a camera at the origin (depth 3 to 20,
+    inside a widened frustum) behind a large double-sided wall at depth 2.5
+    that covers the left half of the view: occlusion culling has meshlets
+    to remove and meshlets that stay."""
+    rng = S.SplitMix64(S.SEED_BASE + 1500 + seed_offset)
+    depth = rng.uniform(n, 3.0, 20.0)
+    ndc = rng.uniform(2 * n, -1.1, 1.1).reshape(n, 2)
+    f = 1.0 / np.tan(np.radians(30.0))
+    centers = np.stack([ndc[:, 0] * depth * (4 / 3) / f, ndc[:, 1] * depth / f, -depth], axis=1)
+    R = S._random_rotations(rng, n)
+    scales = rng.uniform(n, 0.3, 1.5)
+    wall_c = np.array([[-1.2, 0.0, -2.5]])
+    centers = np.concatenate([centers, wall_c])
+    R = np.concatenate([R, np.eye(3)[None]])
+    scales = np.concatenate([scales, [2.6]])
+    mats = [(0, False, 255), (0, True, 255)]
+    mid = np.zeros(n + 1, np.uint32)
+    mid[-1] = 1
+    return S.build_patches(centers, R, scales, mid, mats, [textures.checker_texture(256)])

@@ -306,3 +306,40 @@ def test_alpha_blend_u8_against_float():
         for k in range(4):
             b, f, o = (bg >> 8 * k) & 255, (fg >> 8 * k) & 255, (out >> 8 * k) & 255
             assert abs(o - (b + (f - b) * a)) <= 1.5, (hex(bg), hex(fg), k, o)
+
+
+def test_hiz_sphere_projection(probe):
+    """ProjectSphere and the occlusion test's box, mip level, texel range and
+    sphere depth (Shading.cpp:264-279, 811-832) against the probe."""
+    rng = np.random.default_rng(14)
+    L = oracle.lib()
+    projected = 0
+    for trial in range(100):
+        W, H = int(rng.choice([320, 640, 1920])), int(rng.choice([240, 480, 1080]))
+        Pm = S.proj_matrix(60.0, W / H)
+        V = S.view_matrix(tuple(rng.uniform(-1, 1, 3)), rng.uniform(-1, 1), rng.uniform(-0.5, 0.5))
+        eye_fwd = np.linalg.inv(V.astype(np.float64).reshape(4, 4).T)
+        ctr = eye_fwd[:3, 3][:, None] + eye_fwd[:3, :3] @ np.stack(
+            [rng.uniform(-4, 4, 16), rng.uniform(-3, 3, 16), -rng.uniform(-1, 12, 16)])
+        sphere = np.ascontiguousarray(np.concatenate([ctr, rng.uniform(0.05, 2.0, (1, 16))]), np.float32)
+        scale = np.float32(rng.uniform(0.5, 2.0))
+        proj = np.array([Pm[14], Pm[0], Pm[5]], np.float32)
+        fs = np.array([W, H], np.float32)
+        levels = oracle.hiz_levels(W, H)
+        keep = []
+        occ = oracle.occlusion(np.zeros(1, np.float32), W, H, V, scale, proj, keep)
+        outs1 = [np.zeros(16, np.int32), np.zeros(64, np.float32), np.zeros(16, np.int32), np.zeros(64, np.int32),
+                 np.zeros(16, np.float32)]
+        outs2 = [np.zeros_like(a) for a in outs1]
+        Vf = np.ascontiguousarray(V, np.float32)
+        probe.probe_hiz_sphere(P(sphere), P(Vf), C.c_float(scale), P(proj), P(fs), levels, *[P(a) for a in outs1])
+        L.orc_hiz_sphere16(P(sphere), C.byref(occ), *[P(a) for a in outs2])
+        mask = outs1[0] != 0
+        projected += int(mask.sum())
+        assert (outs1[0] == outs2[0]).all()
+        for a, b in zip(outs1[1:], outs2[1:]):
+            if a.dtype == np.float32:
+                assert (_bits(a) == _bits(b)).all(), trial
+            else:
+                assert (a == b).all(), trial
+    assert projected > 500
