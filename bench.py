@@ -291,16 +291,17 @@ def main() -> int:
                 args.bin_capacity)
         fb.resolve(ds, view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting, stream)
 
-    # the same frame with clear + cull + draw replayed as one captured graph
-    # (a batch of one view); the vis-buffer is identical (tests/test_gpu_batch.py)
+    # the same frame (clear, cull, draw, lit resolve) replayed as one captured
+    # graph: a batch of one view with its resolve; vis-buffer and colour are
+    # identical to the separate calls (tests/test_gpu_batch.py)
     graph = None
     if not args.direct and not args.frame_calls:
         shape = {"bin_size": args.bin_size or 16, "bin_capacity": args.bin_capacity or 1024}
         graph = g.Batch(ctx, [(ds, fb, view, planes, flags)], shape=shape)
+        graph.set_resolve([(view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting)])
 
     def frame_graph():
         graph.launch(True, stream)
-        fb.resolve(ds, view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting, stream)
 
     def run(fn, steps):
         torch.cuda.synchronize()
@@ -369,7 +370,7 @@ def main() -> int:
         "warmup": args.warmup,
         "ms_per_step": round(ms, 4),
         "fps": round(1e3 / ms, 1),
-        "frame": "graph replay of clear+cull+draw, then resolve" if graph is not None else "separate calls",
+        "frame": "one graph replay of clear+cull+draw+resolve" if graph is not None else "separate calls",
         "ms_per_step_separate_calls": round(ms_calls, 4),
         "higher_is_better": True,
         "scaling": "weak",

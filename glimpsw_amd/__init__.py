@@ -177,6 +177,7 @@ _SIGNATURES = {
     "glimp_batch_set_shape": (C.c_int, [C.c_void_p, C.POINTER(BatchShape)]),
     "glimp_batch_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "glimp_batch_launch_per_view": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "glimp_batch_set_resolve": (C.c_int, [C.c_void_p, C.POINTER(ResolveDesc)]),
     "glimp_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BatchShape), C.POINTER(C.c_float)]),
     "glimp_diff_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(DiffDesc), C.c_void_p,
                                      C.c_void_p]),
@@ -400,6 +401,36 @@ class Occlusion:
         self.desc = d
 
 
+def resolve_desc(object_to_clip, mode: int = GLIMP_RESOLVE_ALBEDO, lighting=None):
+    """A glimp_resolve_desc (and the light array it points to, to keep
+    alive): ``lighting`` supplies the lit terms and object_to_world; its
+    light sprites need ``world_to_clip``."""
+    d = ResolveDesc()
+    d.mode = mode
+    for i in range(16):
+        d.object_to_clip[i] = float(object_to_clip[i])
+    keep = None
+    if lighting is not None:
+        for i in range(9):
+            d.object_to_world[i] = float(lighting.object_to_world[i])
+        for i in range(16):
+            d.inv_screen_proj[i] = float(lighting.inv_screen_proj[i])
+        for i in range(3):
+            d.view_pos[i] = float(lighting.view_pos[i])
+        d.exposure = lighting.exposure
+        keep = light_structs(lighting.lights)
+        d.lights = C.cast(keep, C.POINTER(LightDesc))
+        d.num_lights = len(lighting.lights)
+        if getattr(lighting, "light_sprites", False) and getattr(lighting, "world_to_clip", None) is not None:
+            d.flags |= GLIMP_RESOLVE_LIGHT_SPRITES
+            for i in range(16):
+                d.world_to_clip[i] = float(lighting.world_to_clip[i])
+    else:
+        d.exposure = 1.0
+        d.object_to_world[0] = d.object_to_world[4] = d.object_to_world[8] = 1.0
+    return d, keep
+
+
 class Framebuffer:
     """Device vis-buffer of 64-bit depth-test keys (include/glimp.h)."""
 
@@ -448,29 +479,7 @@ class Framebuffer:
         """ShadingContext::Resolve (lit) or ResolveDebug (the other modes);
         ``lighting`` supplies object_to_world for the Normals and
         MetallicRoughness layers as well."""
-        d = ResolveDesc()
-        d.mode = mode
-        for i in range(16):
-            d.object_to_clip[i] = float(object_to_clip[i])
-        keep = None
-        if lighting is not None:
-            for i in range(9):
-                d.object_to_world[i] = float(lighting.object_to_world[i])
-            for i in range(16):
-                d.inv_screen_proj[i] = float(lighting.inv_screen_proj[i])
-            for i in range(3):
-                d.view_pos[i] = float(lighting.view_pos[i])
-            d.exposure = lighting.exposure
-            keep = light_structs(lighting.lights)
-            d.lights = C.cast(keep, C.POINTER(LightDesc))
-            d.num_lights = len(lighting.lights)
-            if getattr(lighting, "light_sprites", False) and getattr(lighting, "world_to_clip", None) is not None:
-                d.flags |= GLIMP_RESOLVE_LIGHT_SPRITES
-                for i in range(16):
-                    d.world_to_clip[i] = float(lighting.world_to_clip[i])
-        else:
-            d.exposure = 1.0
-            d.object_to_world[0] = d.object_to_world[4] = d.object_to_world[8] = 1.0
+        d, keep = resolve_desc(object_to_clip, mode, lighting)
         _check(self.ctx.lib.glimp_resolve(self.ctx.handle, self.handle, scene.handle, C.byref(d), _stream_ptr(stream)),
                "glimp_resolve")
 
@@ -557,6 +566,21 @@ class Batch:
 
     def launch(self, graph: bool = True, stream=None):
         _check(self.ctx.lib.glimp_batch_launch(self.handle, int(graph), _stream_ptr(stream)), "glimp_batch_launch")
+
+    def set_resolve(self, resolves):
+        """Resolve every view after its draw, inside the batch's launch and
+        graph: ``resolves`` holds one ``(object_to_clip, mode, lighting)`` per
+        view, or is None to remove them."""
+        if resolves is None:
+            _check(self.ctx.lib.glimp_batch_set_resolve(self.handle, None), "glimp_batch_set_resolve")
+            return
+        arr = (ResolveDesc * len(resolves))()
+        keep = []
+        for i, (o2c, mode, lighting) in enumerate(resolves):
+            d, k = resolve_desc(o2c, mode, lighting)
+            arr[i] = d
+            keep.append(k)
+        _check(self.ctx.lib.glimp_batch_set_resolve(self.handle, arr), "glimp_batch_set_resolve")
 
     def launch_per_view(self, stream=None):
         _check(self.ctx.lib.glimp_batch_launch_per_view(self.handle, _stream_ptr(stream)),

@@ -676,13 +676,20 @@ struct glimp_batch {
     LaunchShape shape;
     bool alpha = false;                // a view's scene has alpha-tested materials
     Scratch scratch;
-    hipStream_t stream = nullptr;      // graph replays run here, chained to the caller's stream by events
-    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipStream_t stream = nullptr;      // capture stream (replays run on the caller's stream)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // resolves after the draw (glimp_batch_set_resolve), with copies of
+    // their light arrays and the batch's own one-entry draw tables
+    std::vector<glimp_resolve_desc> resolves;
+    std::vector<std::vector<glimp_light>> resolve_lights;
+    DrawEntry* d_tables = nullptr;
 };
 
 namespace {
+
+int build_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const glimp_resolve_desc* d, hipStream_t s,
+                  ResolveParams& p, const std::string& who);
 
 void drop_graph(glimp_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -786,8 +793,6 @@ int glimp_batch_create(glimp_context* c, const glimp_view_desc* views, uint32_t 
     int rc = layout_batch(b);
     hipError_t e = hipSuccess;
     if (rc == GLIMP_OK) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming);
     if (rc == GLIMP_OK && e != hipSuccess) rc = fail(GLIMP_E_HIP, std::string("glimp_batch_create: ") + hipGetErrorString(e));
     if (rc != GLIMP_OK) {
         glimp_batch_destroy(b);
@@ -800,12 +805,13 @@ int glimp_batch_create(glimp_context* c, const glimp_view_desc* views, uint32_t 
 void glimp_batch_destroy(glimp_batch* b) {
     if (!b) return;
     (void)hipSetDevice(b->ctx->device);
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    // replays run on the callers' streams: wait for the device before the
+    // graph and the buffers it uses go away
+    (void)hipDeviceSynchronize();
     drop_graph(b);
     if (b->d_views) (void)hipFree(b->d_views);
+    if (b->d_tables) (void)hipFree(b->d_tables);
     b->scratch.release();
-    if (b->ev_in) (void)hipEventDestroy(b->ev_in);
-    if (b->ev_out) (void)hipEventDestroy(b->ev_out);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -830,11 +836,38 @@ int glimp_batch_launch(glimp_batch* b, int use_graph, void* stream) {
     if (!use_graph) {
         HIP_TRY(launch_draw_binned(p, user, &b->ctx->prof));
         mark_drawn(b);
+        for (size_t v = 0; v < b->resolves.size(); v++) {
+            int rc = glimp_resolve(b->ctx, b->descs[v].framebuffer, b->descs[v].scene, &b->resolves[v], stream);
+            if (rc != GLIMP_OK) return rc;
+        }
         return GLIMP_OK;
     }
     if (!b->exec) {
+        // The captured resolves read each view as the batch leaves it: its
+        // keys, its clear depth and a one-entry draw table owned by the batch
+        // (the framebuffer's own table may change between replays).
+        std::vector<ResolveParams> rps(b->resolves.size());
+        if (!b->resolves.empty()) {
+            mark_drawn(b);
+            if (!b->d_tables) {
+                std::vector<DrawEntry> t(b->descs.size());
+                for (size_t v = 0; v < t.size(); v++)
+                    t[v] = DrawEntry{ 0, b->descs[v].meshlet_count, b->descs[v].meshlet_offset, 0 };
+                HIP_TRY(upload(&b->d_tables, t.data(), t.size()));
+            }
+            for (size_t v = 0; v < rps.size(); v++) {
+                const glimp_view_desc& d = b->descs[v];
+                // on the caller's stream: after work in flight that reads the framebuffer
+                int rc = build_resolve(b->ctx, d.framebuffer, d.scene, &b->resolves[v], user, rps[v],
+                                       "glimp_batch_launch");
+                if (rc != GLIMP_OK) return rc;
+                rps[v].draws = b->d_tables + v;
+                rps[v].num_draws = 1;
+            }
+        }
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         hipError_t le = launch_draw_binned(p, b->stream, nullptr);
+        for (size_t v = 0; v < rps.size() && le == hipSuccess; v++) le = launch_resolve(rps[v], b->stream, nullptr);
         hipGraph_t g = nullptr;
         hipError_t ce = hipStreamEndCapture(b->stream, &g);
         if (le != hipSuccess || ce != hipSuccess) {
@@ -845,13 +878,31 @@ int glimp_batch_launch(glimp_batch* b, int use_graph, void* stream) {
         b->graph = g;
         HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
-    Profiler::Scope sc(&b->ctx->prof, "batch_graph", b->stream);
-    HIP_TRY(hipEventRecord(b->ev_in, user));
-    HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_in, 0));
-    HIP_TRY(hipGraphLaunch(b->exec, b->stream));
-    HIP_TRY(hipEventRecord(b->ev_out, b->stream));
-    HIP_TRY(hipStreamWaitEvent(user, b->ev_out, 0));
+    Profiler::Scope sc(&b->ctx->prof, "batch_graph", user);
+    HIP_TRY(hipGraphLaunch(b->exec, user));
     mark_drawn(b);
+    return GLIMP_OK;
+}
+
+int glimp_batch_set_resolve(glimp_batch* b, const glimp_resolve_desc* descs) {
+    if (!b) return fail(GLIMP_E_INVALID, "glimp_batch_set_resolve: NULL batch");
+    std::vector<glimp_resolve_desc> r;
+    std::vector<std::vector<glimp_light>> lights;
+    if (descs) {
+        r.assign(descs, descs + b->descs.size());
+        lights.resize(r.size());
+        for (size_t v = 0; v < r.size(); v++) {
+            if (r[v].mode > GLIMP_RESOLVE_TRIANGLE_ID)
+                return fail(GLIMP_E_INVALID, "glimp_batch_set_resolve: unknown mode");
+            if (r[v].num_lights > GLIMP_MAX_LIGHTS || (r[v].num_lights && !r[v].lights))
+                return fail(GLIMP_E_LIMIT, "glimp_batch_set_resolve: at most GLIMP_MAX_LIGHTS lights");
+            lights[v].assign(r[v].lights, r[v].lights + r[v].num_lights);
+            r[v].lights = lights[v].data();
+        }
+    }
+    b->resolves.swap(r);
+    b->resolve_lights.swap(lights);
+    drop_graph(b);
     return GLIMP_OK;
 }
 
@@ -871,6 +922,10 @@ int glimp_batch_launch_per_view(glimp_batch* b, void* stream) {
         dd.bin_size = b->shape.bin_size;
         dd.bin_capacity = b->shape.bin_capacity;
         rc = glimp_draw_meshlets(b->ctx, d.framebuffer, d.scene, &dd, stream);
+        if (rc != GLIMP_OK) return rc;
+    }
+    for (size_t v = 0; v < b->resolves.size(); v++) {
+        int rc = glimp_resolve(b->ctx, b->descs[v].framebuffer, b->descs[v].scene, &b->resolves[v], stream);
         if (rc != GLIMP_OK) return rc;
     }
     return GLIMP_OK;
@@ -1073,20 +1128,15 @@ void place_sprite(const float* m, uint32_t W, uint32_t H, ResolveLight& L) {
     L.sprite = 1;
 }
 
-}  // namespace
-
-extern "C" {
-
-int glimp_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const glimp_resolve_desc* d, void* stream) {
-    if (!c || !f || !sc || !d) return fail(GLIMP_E_INVALID, "glimp_resolve: NULL argument");
-    if (d->mode > GLIMP_RESOLVE_TRIANGLE_ID) return fail(GLIMP_E_INVALID, "glimp_resolve: unknown mode");
+// The launch of a resolve of framebuffer f as it stands (its clear applied):
+// checks the description, allocates the colour buffer, uploads the draw
+// table and fills the kernel parameters. Launches nothing.
+int build_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const glimp_resolve_desc* d, hipStream_t s,
+                  ResolveParams& p, const std::string& who) {
+    if (d->mode > GLIMP_RESOLVE_TRIANGLE_ID) return fail(GLIMP_E_INVALID, who + ": unknown mode");
     if (d->num_lights > GLIMP_MAX_LIGHTS || (d->num_lights && !d->lights))
-        return fail(GLIMP_E_LIMIT, "glimp_resolve: at most GLIMP_MAX_LIGHTS lights");
-    HIP_TRY(hipSetDevice(c->device));
-    hipStream_t s = (hipStream_t)stream;
-    HIP_TRY(flush_clear(f, s));
+        return fail(GLIMP_E_LIMIT, who + ": at most GLIMP_MAX_LIGHTS lights");
     if (!f->color) HIP_TRY(dev_alloc(&f->color, (size_t)f->width * f->height));
-    ResolveParams p;
     memset(&p, 0, sizeof(p));
     p.m = sc->m;
     p.materials = sc->materials;
@@ -1129,6 +1179,21 @@ int glimp_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const
     p.mode = (int)d->mode;
     p.out = f->color;
     p.linear = 1;
+    return GLIMP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int glimp_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const glimp_resolve_desc* d, void* stream) {
+    if (!c || !f || !sc || !d) return fail(GLIMP_E_INVALID, "glimp_resolve: NULL argument");
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(flush_clear(f, s));
+    ResolveParams p;
+    int rc = build_resolve(c, f, sc, d, s, p, "glimp_resolve");
+    if (rc != GLIMP_OK) return rc;
     HIP_TRY(launch_resolve(p, s, &c->prof));
     return GLIMP_OK;
 }

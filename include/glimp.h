@@ -226,11 +226,14 @@ int glimp_batch_create(glimp_context* ctx, const glimp_view_desc* views, uint32_
 void glimp_batch_destroy(glimp_batch* batch);
 int glimp_batch_set_shape(glimp_batch* batch, const glimp_batch_shape* shape);
 /* Clears each view's framebuffer and draws its meshlets, one launch per
- * stage for all views; use_graph != 0 replays the batch as one hipGraph
- * (captured on first use, re-captured after a shape change). Asynchronous. */
+ * stage for all views, then the resolves if set; use_graph != 0 replays the
+ * batch as one hipGraph on `stream` (captured on first use, re-captured
+ * after a shape or resolve change). Asynchronous. The framebuffers and
+ * scenes must outlive the batch. */
 int glimp_batch_launch(glimp_batch* batch, int use_graph, void* stream);
-/* The same work as glimp_framebuffer_clear + glimp_draw_meshlets per view,
- * view after view (for comparison with the batched launch). */
+/* The same work as glimp_framebuffer_clear + glimp_draw_meshlets (and
+ * glimp_resolve when set) per view, view after view (for comparison with
+ * the batched launch). */
 int glimp_batch_launch_per_view(glimp_batch* batch, void* stream);
 /* Autotuner: times the candidate shapes (bin 8-64, capacity 256-4096,
  * 128/256-thread bin raster) on this batch, keeps the fastest and returns
@@ -336,6 +339,12 @@ typedef struct glimp_resolve_desc {
  * Asynchronous. */
 int glimp_resolve(glimp_context* ctx, glimp_framebuffer* fb, glimp_scene* scene, const glimp_resolve_desc* desc,
                   void* stream);
+/* Batches (above): adds a resolve of every view after its draw, in the same launch and the
+ * same captured graph: descs holds one glimp_resolve description per view
+ * of the batch, in view order; descs = NULL removes them. Light arrays are
+ * copied. The replay of a frame (clear, cull, draw, resolve) is then one
+ * graph launch. */
+int glimp_batch_set_resolve(glimp_batch* batch, const glimp_resolve_desc* descs);
 /* Device pointer to the resolved RGBA8 buffer (width*height u32, row-major). */
 int glimp_framebuffer_color(glimp_framebuffer* fb, uint32_t** color);
 /* Copies the resolved RGBA8 buffer to host memory (row-major). Synchronous. */

@@ -173,3 +173,36 @@ def test_batch_rejects_shared_framebuffer(gpu_ctx, small):
         g.Batch(gpu_ctx, items)
     with pytest.raises(g.GlimpError):
         g.Batch(gpu_ctx, st.items[:2], shape={"bin_size": 12, "bin_capacity": 64})
+
+
+def test_batch_with_resolve(gpu_ctx, small):
+    """Resolves added to the batch run inside its launch and graph: every
+    view's colour equals draw + resolve view by view, for lit (with light
+    sprites) and albedo resolves, on every replay, with eager draws and
+    resolves of the same framebuffers between the replays."""
+    scenes, views, st = small
+    g = st.g
+    b = g.Batch(gpu_ctx, st.items)
+    res = []
+    for i, (si, v) in enumerate(views):
+        lighting = S.default_lighting(v)
+        res.append((v.object_to_clip, g.GLIMP_RESOLVE_LIT if i % 2 == 0 else g.GLIMP_RESOLVE_ALBEDO, lighting))
+    b.set_resolve(res)
+    b.launch_per_view()
+    want = [fb.read_color() for fb in st.fbs]
+    assert any((w != want[0][0, 0]).any() for w in want)
+    for r in range(3):
+        b.launch(graph=True)
+        got = [fb.read_color() for fb in st.fbs]
+        for i, (w, gt) in enumerate(zip(want, got)):
+            assert (w == gt).all(), f"replay {r}: view {i} colour differs at {int((w != gt).sum())} pixels"
+        for (ds, fb, v), (o2c, mode, lighting) in list(zip(st.items, res))[:3]:
+            fb.clear(0x123, 0.0)
+            fb.draw(ds, v.object_to_clip, v.meshlet_count // 2, 0)
+            fb.draw(ds, v.object_to_clip, v.meshlet_count - v.meshlet_count // 2, v.meshlet_count // 2)
+            fb.resolve(ds, o2c, mode, lighting)
+    b.launch(graph=False)
+    got = [fb.read_color() for fb in st.fbs]
+    assert all((w == gt).all() for w, gt in zip(want, got))
+    b.set_resolve(None)
+    b.close()
