@@ -516,8 +516,8 @@ struct LaunchShape {
 
 // Clipped triangles are rare (the reference measures < 0.1%, README.md:93);
 // the clip queue holds up to two per meshlet and every queued triangle may
-// produce 8 fan triangles. A fuller queue sets the sticky overflow flag,
-// which glimp_framebuffer_read reports.
+// produce 8 fan triangles. What a full queue drops is counted in the
+// context's `dropped` counter (glimp_get_counters).
 size_t clip_capacity_for(size_t meshlets) { return std::max<size_t>(4096, meshlets * 2); }
 
 // One view of a draw (ShadingContext + DrawMeshlets arguments).
@@ -576,7 +576,11 @@ int prepare_scratch(Scratch& S, size_t total_meshlets, size_t total_bins, const 
     if (rc == GLIMP_OK) rc = grow_one(S.clip, S.clip_cap, clipq);
     if (rc == GLIMP_OK) rc = grow_one(S.visible_list, S.visible_list_cap, std::max<size_t>(total_meshlets, 1));
     if (rc == GLIMP_OK) rc = grow_one(S.meshlet_counts, S.meshlet_counts_cap, total_meshlets);
+    uint32_t* const old_counts = S.bin_counts;
     if (rc == GLIMP_OK) rc = grow_one(S.bin_counts, S.bin_counts_cap, std::max<size_t>(total_bins, 1));
+    // draws leave the bin counts cleared; new ones start cleared
+    if (rc == GLIMP_OK && S.bin_counts != old_counts)
+        HIP_TRY(hipMemset(S.bin_counts, 0, S.bin_counts_cap * sizeof(uint32_t)));
     if (rc == GLIMP_OK && binned) rc = grow_one(S.bin_lists, S.bin_lists_cap, total_bins * sh.bin_capacity);
     if (rc == GLIMP_OK && binned) rc = grow_one(S.spill, S.spill_cap, recs);
     if (rc == GLIMP_OK && alpha) rc = grow_one(S.alpha, S.alpha_cap, S.records_cap);
@@ -598,9 +602,13 @@ void fill_params(glimp_context* c, const Scratch& S, size_t total_meshlets, uint
     p.alpha = alpha ? S.alpha : nullptr;
     p.meshlet_counts = S.meshlet_counts;
     p.clip_record_base = (uint32_t)(total_meshlets * 128);
-    p.clip_record_capacity = (uint32_t)std::min<size_t>(S.records_cap - total_meshlets * 128, 0x7FFFFFFFu);
+    // the draw's own capacities (not what earlier, larger draws allocated), so
+    // what a full queue drops does not depend on the context's history
+    const size_t clipq = clip_capacity_for(total_meshlets);
+    p.clip_record_capacity =
+        (uint32_t)std::min<size_t>(std::min(clipq * 8, S.records_cap - total_meshlets * 128), 0x7FFFFFFFu);
     p.clip_queue = S.clip;
-    p.clip_capacity = (uint32_t)std::min<size_t>(S.clip_cap, 0x7FFFFFFFu);
+    p.clip_capacity = (uint32_t)std::min<size_t>(std::min(clipq, S.clip_cap), 0x7FFFFFFFu);
     p.scratch = S.scratch;
     p.counters = c->counters;
     p.bin_size = sh.bin_size;
@@ -1406,6 +1414,7 @@ int glimp_get_counters(glimp_context* c, glimp_counters* out) {
     out->fragments_passed = h.fragments_passed;
     out->setup_records = h.setup_records;
     out->bin_entries = h.bin_entries;
+    out->dropped = h.dropped;
     return GLIMP_OK;
 }
 

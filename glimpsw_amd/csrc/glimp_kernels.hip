@@ -264,6 +264,13 @@ __global__ void __launch_bounds__(256) k_cull(DrawParams p) {
     __shared__ uint32_t s_base, s_view0;
     const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
     const uint32_t first = blockIdx.x * kCullPerBlock;
+    // counters the stages after the cull fill; the previous draw's last
+    // readers are done (num_visible is cleared after its last reader)
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        p.scratch->num_clip = 0;
+        p.scratch->num_records = 0;
+        p.scratch->num_spill = 0;
+    }
     // the block's meshlets are consecutive: one search for the first view,
     // then each thread walks forward from it
     if (threadIdx.x == 0) s_view0 = view_of_meshlet(p, first);
@@ -516,7 +523,7 @@ __device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, fl
                     ce.pad = 0;
                     p.clip_queue[slot] = ce;
                 } else {
-                    p.scratch->overflow = 1;
+                    atomicAdd(&p.counters->dropped, 1ull);
                 }
             }
         }
@@ -685,7 +692,7 @@ __device__ bool clip_one(const DrawParams& p, uint32_t q, uint32_t lane, TriReco
         if (!keep) return false;
         const uint32_t slot = base + mbcnt(km);
         if (slot >= p.clip_record_capacity) {
-            p.scratch->overflow = 1;
+            atomicAdd(&p.counters->dropped, 1ull);
             return false;
         }
         p.records[p.clip_record_base + slot] = rec;
@@ -1154,7 +1161,11 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
     // bin's list), which takes one dependent load off every bin.
     const uint32_t* list = p.bin_lists + (size_t)bin * p.bin_capacity;
     const uint32_t first_entry = tid < p.bin_capacity ? list[tid] : 0u;
-    const uint32_t n = min(p.bin_counts[bin], p.bin_capacity);
+    const uint32_t count = p.bin_counts[bin];
+    const uint32_t n = min(count, p.bin_capacity);
+    // the visible count has had its last reader (the scatter): leave it
+    // cleared for the next draw's cull
+    if (bin == 0 && tid == 0) p.scratch->num_visible = 0;
     if (n == 0 && !V.fresh) return;  // uniform across the block
     const int hw = V.width / 2, hh = V.height / 2;
     const unsigned long long clearKey = (unsigned long long)fbits(V.clear_depth) << 32;
@@ -1316,6 +1327,9 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
         uint32_t strip = i / (BS * 4), rem = i % (BS * 4);
         if (rem < rowElems && y0 + strip * 4 < y1) gkeys[(size_t)(y0 + strip * 4) * W + x0 * 4 + rem] = keys[i];
     }
+    // the bin's count, read by every thread before the first barrier of the
+    // record loop (count > 0 means the loop ran), is cleared for the next draw
+    if (tid == 0 && count) p.bin_counts[bin] = 0;
     if (countFrags) {
         for (int d = 32; d > 0; d >>= 1) {
             covered += __shfl_xor(covered, d);
@@ -1338,16 +1352,14 @@ __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
     raster_bin<BS, kThreads, Alpha>(p, blockIdx.x, keys, tl, s_wave_total, talpha);
 }
 
-__global__ void k_begin_draw(DrawScratch* s, uint32_t* bin_counts, uint32_t num_bins) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        s->num_records = 0;
-        s->num_clip = 0;
-        s->num_visible = 0;
-        s->overflow = 0;
-        s->num_spill = 0;
-    }
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < num_bins; i += gridDim.x * blockDim.x)
-        bin_counts[i] = 0;
+// Clears every scratch counter: at the end of a direct draw (the binned
+// path clears num_visible in the bin raster and the others in the cull)
+// and in place of the cull of a launch without meshlets.
+__global__ void k_reset_counters(DrawScratch* s) {
+    s->num_records = 0;
+    s->num_clip = 0;
+    s->num_visible = 0;
+    s->num_spill = 0;
 }
 
 __global__ void k_clear(unsigned long long* keys, uint32_t n, unsigned long long key) {
@@ -1393,8 +1405,7 @@ __global__ void k_decode(const unsigned long long* keys, int width, int height, 
 }
 
 void launch_front(const DrawParams& p, hipStream_t s, Profiler* prof, bool separate_clip) {
-    uint32_t zb = min((p.total_bins + 255) / 256, 64u);
-    hipLaunchKernelGGL(k_begin_draw, dim3(max(zb, 1u)), dim3(256), 0, s, p.scratch, p.bin_counts, p.total_bins);
+    if (!p.total_meshlets) hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, s, p.scratch);
     if (p.total_meshlets) {     // a batch of empty views still clears through the bin raster
         Profiler::Scope sc(prof, "cull", s);
         if (p.total_meshlets <= 256u * 1024u)
@@ -1481,6 +1492,7 @@ hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s, Profiler* prof
             hipLaunchKernelGGL(k_raster_direct<false>, dim3(min(want, resident_blocks<k_raster_direct<false>, 256>())),
                                dim3(256), 0, s, p, 0);
     }
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, s, p.scratch);
     return hipGetLastError();
 }
 

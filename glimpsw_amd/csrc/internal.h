@@ -114,14 +114,18 @@ struct Counters {                  // device counters, u64 each
     unsigned long long fragments_passed;
     unsigned long long setup_records;
     unsigned long long bin_entries;
+    unsigned long long dropped;    // clipped triangles or fan records lost to a full queue (always counted)
 };
 
-// Per-draw scratch counters, cleared by the first kernel of each draw.
+// Per-draw scratch counters, cleared without a launch of their own:
+// num_visible is zero when a draw starts (allocation clears it, the bin
+// raster clears it after its last reader, a direct draw ends with a reset),
+// and the cull clears the others, which only later stages fill.
 struct DrawScratch {
     uint32_t num_records;          // records appended by the clipper
     uint32_t num_clip;
     uint32_t num_visible;
-    uint32_t overflow;             // a record or clip queue was full
+    uint32_t pad0;
     uint32_t num_spill;            // records that did not fit a bin
     uint32_t pad[3];
 };

@@ -360,6 +360,8 @@ typedef struct glimp_counters {
     uint64_t fragments_passed;
     uint64_t setup_records;            /* triangles that left setup */
     uint64_t bin_entries;              /* (triangle, bin) pairs */
+    uint64_t dropped;                  /* queued clip triangles or clipped records lost to a full queue
+                                        * (counted always; non-zero means the frame is incomplete) */
 } glimp_counters;
 int glimp_get_counters(glimp_context* ctx, glimp_counters* out); /* synchronous */
 int glimp_reset_counters(glimp_context* ctx);

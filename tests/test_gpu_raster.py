@@ -7,6 +7,7 @@ import pytest
 
 import oracle
 from glimpsw_amd import scene as S
+from glimpsw_amd import textures
 from tests import scenes as T
 
 pytestmark = pytest.mark.gpu
@@ -181,6 +182,28 @@ def test_counters_match_oracle(gpu_ctx):
     c = gpu_ctx.counters()
     for k in ("triangles_processed", "triangles_rasterized", "triangles_clipped", "fragments_covered"):
         assert c[k] == o[2][k], (k, c[k], o[2][k])
+    assert c["dropped"] == 0
+
+
+def test_full_clip_queue_is_counted(gpu_ctx):
+    """600 identical small patches through the eye, sheared so the near
+    plane cuts a diagonal of each (7,200 clipped triangles in the oracle),
+    queue more clipped triangles than the clip queue holds (4096 or two per
+    meshlet): the draw completes and the loss shows in the `dropped` counter;
+    the next, ordinary draw (whose scratch counters the full one must leave
+    cleared) is exact again."""
+    n = 600
+    R = np.tile(np.array([[0.7, 0.0, 0.0], [0.0, 0.7, 0.0], [0.7, 0.7, 1.0]]), (n, 1, 1))
+    sc = S.build_patches(np.tile([[0.0, 0.0, -S.NEAR_Z]], (n, 1)), R, np.full(n, 0.05), np.zeros(n, np.uint32),
+                         [(0, True, 255)], [textures.checker_texture(64)])
+    view = S.make_view(320, 240, count=n)
+    gpu_ctx.reset_counters()
+    _gpu_frame(gpu_ctx, sc, view, cull=False)
+    c = gpu_ctx.counters()
+    print(f"[raster] full clip queue: dropped {c['dropped']}", flush=True)
+    assert c["dropped"] > 0
+    sc2, view2 = S.checker_scene()
+    _assert_same(*_gpu_frame(gpu_ctx, sc2, view2), *_oracle_frame(sc2, view2)[:2], "after a full clip queue")
 
 
 @pytest.mark.slow
