@@ -303,9 +303,12 @@ def main() -> int:
     # graph: a batch of one view with its resolve; vis-buffer and colour are
     # identical to the separate calls (tests/test_gpu_batch.py)
     graph = None
+    frame_shape = None
     if not args.direct and not args.frame_calls:
         shape = {"bin_size": args.bin_size or 16, "bin_capacity": args.bin_capacity or 1024}
         graph = g.Batch(ctx, [(ds, fb, view, planes, flags)], shape=shape)
+        if not args.bin_size and not args.bin_capacity:
+            frame_shape, _ = graph.autotune(stream)     # the launch shape, as for any batch
         graph.set_resolve([(view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting)])
 
     def frame_graph():
@@ -395,6 +398,7 @@ def main() -> int:
         "ms_per_step": round(ms, 4),
         "fps": round(1e3 / ms, 1),
         "frame": frame_mode,
+        "frame_shape": frame_shape,
         "frame_trial_ms": trial,
         "ms_per_step_separate_calls": round(ms_calls, 4),
         "higher_is_better": True,
