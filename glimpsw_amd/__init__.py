@@ -19,6 +19,8 @@ GLIMP_DRAW_NO_GUARDBAND = 0x2
 GLIMP_DRAW_DIRECT = 0x4
 GLIMP_DRAW_COUNTERS = 0x8
 GLIMP_PIPELINE_VISBUFFER = 0
+GLIMP_PIPELINE_DEFERRED = 1
+GLIMP_PIPELINE_OVERDRAW = 2
 MAX_RENDER_SIZE = 2896
 
 u8p = C.POINTER(C.c_uint8)
@@ -122,6 +124,8 @@ GLIMP_RESOLVE_NORMALS = 2
 GLIMP_RESOLVE_METALLIC_ROUGHNESS = 3
 GLIMP_RESOLVE_MESHLET_ID = 4
 GLIMP_RESOLVE_TRIANGLE_ID = 5
+GLIMP_RESOLVE_OVERDRAW_PIXEL = 6
+GLIMP_RESOLVE_OVERDRAW_QUAD = 7
 GLIMP_RESOLVE_LIGHT_SPRITES = 0x1
 
 
@@ -156,6 +160,7 @@ _SIGNATURES = {
     "glimp_framebuffer_destroy": (None, [C.c_void_p]),
     "glimp_framebuffer_clear": (C.c_int, [C.c_void_p, C.c_uint32, C.c_float, C.c_void_p]),
     "glimp_framebuffer_read": (C.c_int, [C.c_void_p, u32p, f32p, C.c_int, C.c_void_p]),
+"glimp_framebuffer_read_layer": (C.c_int, [C.c_void_p, C.c_uint32, u32p, C.c_int, C.c_void_p]),
     "glimp_framebuffer_keys": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]),
     "glimp_draw_meshlets": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(DrawDesc), C.c_void_p]),
     "glimp_cull_meshlets": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, f32p, u16p,
@@ -451,7 +456,8 @@ class Framebuffer:
                "glimp_framebuffer_clear")
 
     def draw(self, scene: Scene, object_to_clip, count: int, offset: int = 0, planes=None, flags: int = 0,
-             stream=None, bin_size: int = 0, bin_capacity: int = 0, occlusion: "Occlusion | None" = None):
+             stream=None, bin_size: int = 0, bin_capacity: int = 0, occlusion: "Occlusion | None" = None,
+             pipeline: int = GLIMP_PIPELINE_VISBUFFER):
         d = DrawDesc()
         d.meshlet_offset, d.meshlet_count = offset, count
         for i in range(16):
@@ -461,7 +467,7 @@ class Framebuffer:
             keep = np.ascontiguousarray(planes, dtype=np.float32)
             d.frustum_planes = _ptr(keep, f32p)
         d.flags = flags
-        d.pipeline = GLIMP_PIPELINE_VISBUFFER
+        d.pipeline = pipeline
         d.bin_size, d.bin_capacity = bin_size, bin_capacity
         if occlusion is not None:
             d.occlusion = C.pointer(occlusion.desc)
@@ -474,6 +480,13 @@ class Framebuffer:
         _check(self.ctx.lib.glimp_framebuffer_read(self.handle, _ptr(ids, u32p), _ptr(depth, f32p), int(linear),
                                                    _stream_ptr(stream)), "glimp_framebuffer_read")
         return ids, depth
+
+    def read_layer(self, layer: int, linear: bool = True, stream=None) -> np.ndarray:
+        """One framebuffer layer as 32-bit words (glimp_framebuffer_read_layer)."""
+        out = np.zeros((self.height, self.width), dtype=np.uint32)
+        _check(self.ctx.lib.glimp_framebuffer_read_layer(self.handle, layer, _ptr(out, u32p), int(linear),
+                                                         _stream_ptr(stream)), "glimp_framebuffer_read_layer")
+        return out
 
     def resolve(self, scene: Scene, object_to_clip, mode: int = GLIMP_RESOLVE_ALBEDO, lighting=None, stream=None):
         """ShadingContext::Resolve (lit) or ResolveDebug (the other modes);

@@ -146,6 +146,12 @@ struct glimp_framebuffer {
     size_t d_draws_cap = 0;
     std::vector<DrawEntry> uploaded;   // contents of d_draws
     uint32_t* color = nullptr;         // resolved RGBA8, row-major
+    // The pipeline whose output the framebuffer holds (GLIMP_PIPELINE_*), or
+    // -1 when nothing was drawn since the clear. Overdraw keeps its counts
+    // in `keys` and its depth layer in `aux`; the deferred pipeline keeps
+    // its colour and normal/MR layers in `aux` (2 * width * height words).
+    int content = -1;
+    uint32_t* aux = nullptr;
 };
 
 namespace {
@@ -373,6 +379,7 @@ void glimp_framebuffer_destroy(glimp_framebuffer* f) {
     if (f->keys) (void)hipFree(f->keys);
     if (f->d_draws) (void)hipFree(f->d_draws);
     if (f->color) (void)hipFree(f->color);
+    if (f->aux) (void)hipFree(f->aux);
     delete f;
 }
 
@@ -394,6 +401,7 @@ int glimp_framebuffer_clear(glimp_framebuffer* f, uint32_t color, float depth, v
     f->next_seq = 0;
     f->draws.clear();
     f->pending_clear = true;
+    f->content = -1;
     return GLIMP_OK;
 }
 
@@ -435,11 +443,36 @@ int upload_draws(glimp_framebuffer* f, hipStream_t s, uint32_t* count) {
 
 }  // namespace
 
+}  // extern "C"
+
+namespace {
+
+// Counter words and depth of an overdraw framebuffer (host arrays, either
+// may be NULL).
+int read_overdraw(glimp_framebuffer* f, uint32_t* words, uint32_t* depth, int linear, hipStream_t s) {
+    const size_t n = (size_t)f->width * f->height;
+    uint32_t* d_out = nullptr;
+    HIP_TRY(dev_alloc(&d_out, 2 * n));
+    hipError_t e = launch_decode_overdraw(f->keys, f->aux, (uint32_t)n, f->clear_color, (int)f->width, d_out,
+                                          d_out + n, linear, s);
+    if (e == hipSuccess && words) e = hipMemcpyAsync(words, d_out, n * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && depth) e = hipMemcpyAsync(depth, d_out + n, n * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return fail(GLIMP_E_HIP, std::string("glimp_framebuffer_read: ") + hipGetErrorString(e));
+    return GLIMP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int glimp_framebuffer_read(glimp_framebuffer* f, uint32_t* ids, float* depth, int linear, void* stream) {
     if (!f) return fail(GLIMP_E_INVALID, "glimp_framebuffer_read: NULL framebuffer");
     HIP_TRY(hipSetDevice(f->ctx->device));
     hipStream_t s = (hipStream_t)stream;
     size_t n = (size_t)f->width * f->height;
+    if (f->content == (int)GLIMP_PIPELINE_OVERDRAW) return read_overdraw(f, ids, (uint32_t*)depth, linear, s);
     HIP_TRY(flush_clear(f, s));
     uint32_t ndraws = 0;
     int rc = upload_draws(f, s, &ndraws);
@@ -460,6 +493,13 @@ int glimp_framebuffer_read(glimp_framebuffer* f, uint32_t* ids, float* depth, in
     if (d_depth) (void)hipFree(d_depth);
     if (e != hipSuccess) return fail(GLIMP_E_HIP, std::string("glimp_framebuffer_read: ") + hipGetErrorString(e));
     return GLIMP_OK;
+}
+
+int glimp_framebuffer_read_layer(glimp_framebuffer* f, uint32_t layer, uint32_t* out, int linear, void* stream) {
+    if (!f || !out) return fail(GLIMP_E_INVALID, "glimp_framebuffer_read_layer: NULL argument");
+    if (layer == 0) return glimp_framebuffer_read(f, out, nullptr, linear, stream);
+    if (layer == 1) return glimp_framebuffer_read(f, nullptr, (float*)out, linear, stream);
+    return fail(GLIMP_E_INVALID, "glimp_framebuffer_read_layer: the framebuffer has no such layer");
 }
 
 }  // extern "C"
@@ -629,8 +669,10 @@ extern "C" {
 int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, const glimp_draw_desc* d,
                         void* stream) {
     if (!c || !f || !s || !d) return fail(GLIMP_E_INVALID, "glimp_draw_meshlets: NULL argument");
-    if (d->pipeline != GLIMP_PIPELINE_VISBUFFER)
+    if (d->pipeline != GLIMP_PIPELINE_VISBUFFER && d->pipeline != GLIMP_PIPELINE_OVERDRAW)
         return fail(GLIMP_E_UNSUPPORTED, "glimp_draw_meshlets: unknown pipeline");
+    if (f->content >= 0 && f->content != (int)d->pipeline)
+        return fail(GLIMP_E_INVALID, "glimp_draw_meshlets: the framebuffer holds another pipeline's output; clear it first");
     if ((uint64_t)d->meshlet_offset + d->meshlet_count > s->m.count)
         return fail(GLIMP_E_INVALID, "glimp_draw_meshlets: meshlet range outside the scene");
     if ((uint64_t)f->next_seq + d->meshlet_count > GLIMP_MAX_DRAWN_MESHLETS)
@@ -642,8 +684,19 @@ int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, 
         return fail(GLIMP_E_INVALID, "glimp_draw_meshlets: bin_size must be 8, 16, 32 or 64");
     if (d->meshlet_count == 0) return GLIMP_OK;
     HIP_TRY(hipSetDevice(c->device));
-    const bool direct = (d->flags & GLIMP_DRAW_DIRECT) != 0;
+    const bool overdraw = d->pipeline == GLIMP_PIPELINE_OVERDRAW;
+    const bool direct = overdraw || (d->flags & GLIMP_DRAW_DIRECT) != 0;
     hipStream_t st = (hipStream_t)stream;
+    if (overdraw && f->content != (int)GLIMP_PIPELINE_OVERDRAW) {
+        // first overdraw draw since the clear: zero counts, depth = clear depth
+        const size_t n = (size_t)f->width * f->height;
+        if (!f->aux) HIP_TRY(dev_alloc(&f->aux, 2 * n));
+        uint32_t db;
+        memcpy(&db, &f->clear_depth, 4);
+        HIP_TRY(launch_clear(f->keys, (uint32_t)n, 0ull, st));
+        HIP_TRY(launch_fill32(f->aux, (uint32_t)n, db, st));
+        f->pending_clear = false;
+    }
     if (direct) HIP_TRY(flush_clear(f, st));
 
     Occlusion occ;
@@ -656,17 +709,21 @@ int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, 
     make_view(f, s, d->meshlet_offset, d->meshlet_count, d->object_to_clip, d->frustum_planes, d->flags, views[0],
               d->occlusion ? &occ : nullptr);
     const uint32_t total_bins = direct ? 0 : layout_bins(views, sh.bin_size);
-    int rc = prepare_scratch(c->draw_scratch, d->meshlet_count, total_bins, sh, !direct, s->has_alpha);
+    const bool alpha = s->has_alpha && !overdraw;    // FS_Overdraw has no alpha test
+    int rc = prepare_scratch(c->draw_scratch, d->meshlet_count, total_bins, sh, !direct, alpha);
     if (rc != GLIMP_OK) return rc;
     DrawParams p;
     memset(&p, 0, sizeof(p));
-    fill_params(c, c->draw_scratch, d->meshlet_count, total_bins, sh, d->flags, s->has_alpha, p);
+    fill_params(c, c->draw_scratch, d->meshlet_count, total_bins, sh, d->flags, alpha, p);
     p.num_views = 1;
     p.views = nullptr;
     p.view = views[0];
-    if (direct) HIP_TRY(launch_draw_direct(p, st, &c->prof));
+    p.view.aux = f->aux;
+    if (overdraw) HIP_TRY(launch_draw_overdraw(p, st, &c->prof));
+    else if (direct) HIP_TRY(launch_draw_direct(p, st, &c->prof));
     else HIP_TRY(launch_draw_binned(p, st, &c->prof));
     f->pending_clear = false;
+    f->content = (int)d->pipeline;
     f->draws.push_back(DrawEntry{ f->next_seq, d->meshlet_count, d->meshlet_offset, 0 });
     f->next_seq += d->meshlet_count;
     return GLIMP_OK;
@@ -740,6 +797,7 @@ void mark_drawn(glimp_batch* b) {
         f->clear_color = d.clear_color;
         f->clear_depth = d.clear_depth == 0.0f ? 0.0f : d.clear_depth;
         f->pending_clear = false;
+        f->content = (int)GLIMP_PIPELINE_VISBUFFER;
         f->draws.assign(1, DrawEntry{ 0, d.meshlet_count, d.meshlet_offset, 0 });
         f->next_seq = d.meshlet_count;
     }
@@ -1075,6 +1133,8 @@ int glimp_hiz_build(glimp_hiz* h, glimp_framebuffer* f, void* stream) {
     if (!h || !f) return fail(GLIMP_E_INVALID, "glimp_hiz_build: NULL argument");
     if (f->width != h->layout.width || f->height != h->layout.height)
         return fail(GLIMP_E_INVALID, "glimp_hiz_build: framebuffer and depth map sizes differ");
+    if (f->content > (int)GLIMP_PIPELINE_VISBUFFER)
+        return fail(GLIMP_E_UNSUPPORTED, "glimp_hiz_build: only vis-buffer framebuffers");
     HIP_TRY(hipSetDevice(h->ctx->device));
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(flush_clear(f, s));
@@ -1198,6 +1258,17 @@ int glimp_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const
     if (!c || !f || !sc || !d) return fail(GLIMP_E_INVALID, "glimp_resolve: NULL argument");
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
+    const bool od_mode = d->mode == GLIMP_RESOLVE_OVERDRAW_PIXEL || d->mode == GLIMP_RESOLVE_OVERDRAW_QUAD;
+    const bool od_fb = f->content == (int)GLIMP_PIPELINE_OVERDRAW;
+    if (od_mode != od_fb)
+        return fail(GLIMP_E_INVALID, od_mode ? "glimp_resolve: the overdraw layers need an overdraw framebuffer"
+                                             : "glimp_resolve: an overdraw framebuffer has only the overdraw layers");
+    if (od_mode) {
+        if (!f->color) HIP_TRY(dev_alloc(&f->color, (size_t)f->width * f->height));
+        HIP_TRY(launch_resolve_overdraw(f->keys, f->aux, (int)f->width, (int)f->height, f->clear_color,
+                                        d->mode == GLIMP_RESOLVE_OVERDRAW_QUAD, f->color, s, &c->prof));
+        return GLIMP_OK;
+    }
     HIP_TRY(flush_clear(f, s));
     ResolveParams p;
     int rc = build_resolve(c, f, sc, d, s, p, "glimp_resolve");

@@ -987,6 +987,112 @@ __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p, int use_spi
     }
 }
 
+// The overdraw pipeline: Rasterizer::DrawTriangle with FS_Overdraw
+// (Rasterizer.h:281-324, Shading.cpp:333-342). Every 4x4 tile of a
+// triangle's render box with a covered pixel adds one to each of its 16
+// pixels: to the covered count when the pixel is covered, else to the
+// quad-helper count; and raises each pixel's depth to the triangle's plane
+// depth there (extrapolated for uncovered pixels) when that is greater.
+// Counts and maxima commute, so the result does not depend on the order in
+// which triangles are drawn. Counts are 32-bit halves of one 64-bit word
+// per pixel (saturated to 16 bits when read); the depth maximum is a 32-bit
+// max on the float's bits, taken only for positive depths, which order like
+// their bits (the stored depth starts at the clear depth, >= +0).
+__global__ void __launch_bounds__(256) k_raster_overdraw(DrawParams p) {
+    __shared__ TriLds tl[4][64];
+    __shared__ uint32_t tview[4][64];
+    const uint32_t wid = threadIdx.x / kWave;
+    const uint32_t lane = lane_id();
+    const uint32_t nchunks = num_chunks(p, 0);
+    const bool countFrags = counting(p);
+    const uint32_t waves = gridDim.x * 4;
+    const unsigned long long groupMask = 0xFFFFull << (lane & 48u);
+
+    for (uint32_t c = blockIdx.x * 4 + wid; c < nchunks; c += waves) {
+        const Chunk ck = get_chunk(p, 0, c);
+        if (ck.count == 0) continue;
+        uint32_t ntiles = 0;
+        TriLds t;
+        if (lane < ck.count) {
+            const uint32_t r = ck.list ? ck.list[ck.base + lane] : ck.base + lane;
+            const TriRecord rec = p.records[r];
+            const uint32_t v = rec.flags >> 16;
+            const ViewParams& V = view_of(p, v);
+            uint32_t bmin, bmax;
+            load_tri(rec, V.width / 2, V.height / 2, t, bmin, bmax);
+            t.org = bmin;
+            const uint32_t tw = (uint32_t)(lo16(bmax) - lo16(bmin)) >> 2;
+            const uint32_t th = (uint32_t)(hi16(bmax) - hi16(bmin)) >> 2;
+            t.tw = tw;
+            ntiles = tw * th;
+            tview[wid][lane] = v;
+        }
+        uint32_t incl = ntiles;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t o = __shfl_up(incl, d);
+            if ((int)lane >= d) incl += o;
+        }
+        const uint32_t total = __shfl(incl, 63);
+        t.start = incl - ntiles;
+        if (lane < ck.count) tl[wid][lane] = t;
+        else tl[wid][lane].start = 0xFFFFFFFFu;
+        __builtin_amdgcn_wave_barrier();
+
+        uint32_t covered = 0;
+        const uint32_t sub = lane >> 4, px = lane & 3, py = (lane >> 2) & 3;
+        for (uint32_t base = 0; base < total; base += 4) {
+            const uint32_t j = base + sub;
+            bool cov = false, inside = false;
+            float depth = 0.0f;
+            uint32_t off = 0;
+            unsigned long long* keys = nullptr;
+            uint32_t* dbuf = nullptr;
+            if (j < total) {
+                uint32_t k = 0;
+#pragma unroll
+                for (uint32_t step = 32; step > 0; step >>= 1) {
+                    if (tl[wid][k + step].start <= j) k += step;
+                }
+                const TriLds& T = tl[wid][k];
+                const ViewParams& V = view_of(p, tview[wid][k]);
+                const uint32_t W = (uint32_t)V.width;
+                const uint32_t pt = j - T.start;
+                const uint32_t ty = pt / T.tw;
+                const uint32_t tx = pt - ty * T.tw;
+                const uint32_t x = (uint32_t)lo16(T.org) + tx * 4 + px;
+                const uint32_t y = (uint32_t)hi16(T.org) + ty * 4 + py;
+                const int32_t w0 = edge_at(T.e0, T.a12, T.b12, x, y);
+                const int32_t w1 = edge_at(T.e1, T.a20, T.b20, x, y);
+                const int32_t w2 = edge_at(T.e2, T.a01, T.b01, x, y);
+                // render boxes lie inside the framebuffer; the bounds test
+                // only guards the stores
+                inside = x < W && y < (uint32_t)V.height;
+                cov = inside && (w0 | w1 | w2) >= 0;
+                depth = __builtin_fmaf((float)w1, T.z10, __builtin_fmaf((float)w2, T.z20, T.z0));
+                off = pixel_offset(W, x, y);
+                keys = V.keys;
+                dbuf = V.aux;
+            }
+            if (!(__ballot(cov) & groupMask) || !inside) continue;
+            atomicAdd(keys + off, cov ? (1ull << 32) : 1ull);
+            if (depth > 0.0f) {
+                const uint32_t bits = fbits(depth);
+                if (bits > __atomic_load_n(dbuf + off, __ATOMIC_RELAXED)) atomicMax(dbuf + off, bits);
+            }
+            covered += cov ? 1u : 0u;
+        }
+        if (countFrags) {
+            for (int d = 32; d > 0; d >>= 1) covered += __shfl_xor(covered, d);
+            if (lane == 0) {
+                atomicAdd(&p.counters->fragments_covered, (unsigned long long)covered);
+                atomicAdd(&p.counters->fragments_passed, (unsigned long long)covered);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 // ---------------------------------------------------------------- binning
 
 constexpr uint32_t kSortedBinningPairs = 128;
@@ -1404,6 +1510,32 @@ __global__ void k_decode(const unsigned long long* keys, int width, int height, 
     if (depth) depth[o] = bitsf(db);
 }
 
+__global__ void k_fill32(uint32_t* buf, uint32_t n, uint32_t value) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) buf[i] = value;
+}
+
+// Overdraw counts and depth -> the reference's layers: the counter word
+// with each 16-bit half the clear colour's half plus the count, saturated
+// (FS_Overdraw's _mm512_adds_epu16, Shading.cpp:336-337), and the depth.
+__global__ void k_decode_overdraw(const unsigned long long* counts, const uint32_t* depth, uint32_t n,
+                                  uint32_t clear_color, int width, uint32_t* words, uint32_t* depth_out, int linear) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long c = counts[i];
+    const uint32_t hi = (uint32_t)min((unsigned long long)(clear_color >> 16) + (c >> 32), 0xFFFFull);
+    const uint32_t lo = (uint32_t)min((unsigned long long)(clear_color & 0xFFFFu) + (c & 0xFFFFFFFFull), 0xFFFFull);
+    uint32_t o = i;
+    if (linear) {
+        const uint32_t W = (uint32_t)width;
+        const uint32_t tileRow = i / (W * 4u);
+        const uint32_t rem = i - tileRow * W * 4u;
+        const uint32_t tx = rem >> 4, q = rem & 15u;
+        o = (tileRow * 4 + (q >> 2)) * W + tx * 4 + (q & 3u);
+    }
+    if (words) words[o] = (hi << 16) | lo;
+    if (depth_out) depth_out[o] = depth[i];
+}
+
 void launch_front(const DrawParams& p, hipStream_t s, Profiler* prof, bool separate_clip) {
     if (!p.total_meshlets) hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, s, p.scratch);
     if (p.total_meshlets) {     // a batch of empty views still clears through the bin raster
@@ -1437,6 +1569,7 @@ void warm_launch_shapes() {
     (void)resident_blocks<k_mesh_setup<true>, 256>();
     (void)resident_blocks<k_raster_direct<false>, 256>();
     (void)resident_blocks<k_raster_direct<true>, 256>();
+    (void)resident_blocks<k_raster_overdraw, 256>();
     (void)resident_blocks<k_bin_scatter<false>, 256>();
     (void)resident_blocks<k_bin_scatter<true>, 256>();
 }
@@ -1444,6 +1577,32 @@ void warm_launch_shapes() {
 hipError_t launch_clear(unsigned long long* keys, uint32_t n, unsigned long long key, hipStream_t s) {
     uint32_t blocks = min((n + 255) / 256, 8192u);
     hipLaunchKernelGGL(k_clear, dim3(blocks), dim3(256), 0, s, keys, n, key);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill32(uint32_t* buf, uint32_t n, uint32_t value, hipStream_t s) {
+    uint32_t blocks = min((n + 255) / 256, 8192u);
+    hipLaunchKernelGGL(k_fill32, dim3(blocks), dim3(256), 0, s, buf, n, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_decode_overdraw(const unsigned long long* counts, const uint32_t* depth, uint32_t n,
+                                  uint32_t clear_color, int width, uint32_t* words, uint32_t* depth_out, int linear,
+                                  hipStream_t s) {
+    hipLaunchKernelGGL(k_decode_overdraw, dim3((n + 255) / 256), dim3(256), 0, s, counts, depth, n, clear_color,
+                       width, words, depth_out, linear);
+    return hipGetLastError();
+}
+
+hipError_t launch_draw_overdraw(const DrawParams& p, hipStream_t s, Profiler* prof) {
+    launch_front(p, s, prof, true);
+    {
+        Profiler::Scope sc(prof, "raster_overdraw", s);
+        const uint32_t want = max((2 * p.total_meshlets + 3) / 4 + 16, 64u);
+        hipLaunchKernelGGL(k_raster_overdraw, dim3(min(want, resident_blocks<k_raster_overdraw, 256>())), dim3(256), 0,
+                           s, p);
+    }
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, s, p.scratch);
     return hipGetLastError();
 }
 

@@ -164,6 +164,7 @@ struct ViewParams {
     int use_cull, clipping, fresh;
     float clear_depth;
     unsigned long long* keys;      // the view's vis-buffer, 4x4 tiles
+    uint32_t* aux;                 // overdraw: the depth layer; deferred: colour layer, then normal/MR layer
     float clip[16];
     float planes[20];
     Occlusion occ;                 // with use_cull: HiZ test of the frustum survivors
@@ -209,7 +210,15 @@ struct DrawParams {
 void warm_launch_shapes();
 hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s, Profiler* prof);
 hipError_t launch_draw_binned(const DrawParams& p, hipStream_t s, Profiler* prof);
+// The overdraw pipeline (FS_Overdraw): per pixel the keys hold the covered
+// count in the high and the quad-helper count in the low 32 bits, and
+// V.aux the depth layer.
+hipError_t launch_draw_overdraw(const DrawParams& p, hipStream_t s, Profiler* prof);
+hipError_t launch_decode_overdraw(const unsigned long long* counts, const uint32_t* depth, uint32_t n,
+                                  uint32_t clear_color, int width, uint32_t* words, uint32_t* depth_out, int linear,
+                                  hipStream_t s);
 hipError_t launch_clear(unsigned long long* keys, uint32_t n, unsigned long long key, hipStream_t s);
+hipError_t launch_fill32(uint32_t* buf, uint32_t n, uint32_t value, hipStream_t s);
 hipError_t launch_decode(const unsigned long long* keys, int width, int height, unsigned long long clear_key,
                          uint32_t clear_color, const DrawEntry* draws, uint32_t num_draws, uint32_t* ids,
                          float* depth, int linear, hipStream_t s);

@@ -53,5 +53,9 @@ struct ResolveParams {
 };
 
 hipError_t launch_resolve(const ResolveParams& p, hipStream_t s, Profiler* prof);
+// ResolveDebug(OverdrawPixel / OverdrawQuad) of an overdraw framebuffer
+// (counts and depth as launch_draw_overdraw leaves them) into row-major RGBA8.
+hipError_t launch_resolve_overdraw(const unsigned long long* counts, const uint32_t* depth, int width, int height,
+                                   uint32_t clear_color, int quad, uint32_t* out, hipStream_t s, Profiler* prof);
 
 }  // namespace glimp

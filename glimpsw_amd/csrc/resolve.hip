@@ -403,7 +403,73 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
     if (inside) p.out[p.linear ? (size_t)y * W + x : (size_t)t] = outc;
 }
 
+// Coefficients of ColormapTurbo (Shading.cpp:249-260): Google's published
+// degree-5 polynomial fit of the Turbo colormap, one row per channel,
+// constant term first; double literals rounded to float.
+__constant__ float kTurbo[18] = {
+    0.13572138, 4.61539260,  -42.66032258, 132.13108234, -152.94239396, 59.28637943,
+    0.09140261, 2.19418839,  4.84296658,   -14.18503333, 4.27729857,    2.82956604,
+    0.10667330, 12.64194608, -60.58204836, 110.36276771, -89.90310912,  27.34824973,
+};
+
+// ResolveDebug's overdraw layers (Shading.cpp:759-766) on a framebuffer
+// drawn with the overdraw pipeline: ColormapTurbo of pixels / 30
+// (OverdrawPixel) or (pixels + helpers * 0.5) / 30 (OverdrawQuad), packed;
+// pixels with depth <= 0 get the background checker. The counter word is
+// the clear colour's 16-bit halves plus the counts, saturated. Operation
+// order of the reference's optimised build: / 30 as * (1/30), and the
+// innermost x * c5 folded into s * (c5 * (1/30)) with s the undivided sum;
+// the rest of the Horner chain is FMAs.
+template <bool Quad>
+__global__ void __launch_bounds__(256) k_resolve_overdraw(const unsigned long long* counts, const uint32_t* depth,
+                                                          int width, int height, uint32_t clear_color, uint32_t* out) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t W = (uint32_t)width, H = (uint32_t)height;
+    if (t >= W * H) return;
+    const uint32_t tile = t >> 4, q = t & 15u;
+    uint32_t tx;
+    const uint32_t ty = div_small(tile, W / 4, tx);
+    const uint32_t x = tx * 4 + (q & 3u), y = ty * 4 + (q >> 2);
+    uint32_t outc;
+    if (bitsf(depth[t]) <= 0.0f) {
+        outc = ((tx ^ ty) & 1u) ? 0xFFA0A0A0u : 0xFFFFFFFFu;
+    } else {
+        const unsigned long long c = counts[t];
+        const uint32_t hi = (uint32_t)min((unsigned long long)(clear_color >> 16) + (c >> 32), 0xFFFFull);
+        const uint32_t lo = (uint32_t)min((unsigned long long)(clear_color & 0xFFFFu) + (c & 0xFFFFFFFFull), 0xFFFFull);
+        const float pix = (float)hi;
+        const float sum = Quad ? pix + (float)lo * 0.5f : pix;
+        const float r = 1.0f / 30.0f;
+        const float xs = sum * r;
+        float rgb[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float* cf = kTurbo + 6 * k;
+            float v = __builtin_fmaf(sum, cf[5] * r, cf[4]);
+            v = __builtin_fmaf(xs, v, cf[3]);
+            v = __builtin_fmaf(xs, v, cf[2]);
+            v = __builtin_fmaf(xs, v, cf[1]);
+            rgb[k] = __builtin_fmaf(xs, v, cf[0]);
+        }
+        outc = pack_rgba8(rgb[0], rgb[1], rgb[2], 1.0f);
+    }
+    out[(size_t)y * W + x] = outc;
+}
+
 }  // namespace
+
+hipError_t launch_resolve_overdraw(const unsigned long long* counts, const uint32_t* depth, int width, int height,
+                                   uint32_t clear_color, int quad, uint32_t* out, hipStream_t s, Profiler* prof) {
+    const uint32_t n = (uint32_t)width * (uint32_t)height;
+    Profiler::Scope sc(prof, quad ? "resolve_overdraw_quad" : "resolve_overdraw_pixel", s);
+    if (quad)
+        hipLaunchKernelGGL(k_resolve_overdraw<true>, dim3((n + 255) / 256), dim3(256), 0, s, counts, depth, width,
+                           height, clear_color, out);
+    else
+        hipLaunchKernelGGL(k_resolve_overdraw<false>, dim3((n + 255) / 256), dim3(256), 0, s, counts, depth, width,
+                           height, clear_color, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_resolve(const ResolveParams& p, hipStream_t s, Profiler* prof) {
     uint32_t n = (uint32_t)p.width * (uint32_t)p.height;

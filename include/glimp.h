@@ -120,6 +120,13 @@ int glimp_framebuffer_clear(glimp_framebuffer* fb, uint32_t color, float depth, 
  * Synchronous. */
 int glimp_framebuffer_read(glimp_framebuffer* fb, uint32_t* ids, float* depth, int linear, void* stream);
 
+/* Framebuffer::GetPixels of one layer (ImageHelpers.cpp:109-147) as 32-bit
+ * words: layer 0 = surface ids (vis-buffer), counters (overdraw) or base
+ * colour (deferred); layer 1 = depth bits; layer 2 = the deferred
+ * pipeline's packed normal / metallic / roughness. Row-major (linear != 0)
+ * or 4x4-tiled host array of width*height words. Synchronous. */
+int glimp_framebuffer_read_layer(glimp_framebuffer* fb, uint32_t layer, uint32_t* out, int linear, void* stream);
+
 /* Device pointer to the raw 64-bit key buffer (4x4 tiles) and its size. */
 int glimp_framebuffer_keys(glimp_framebuffer* fb, void** keys, size_t* bytes);
 
@@ -133,6 +140,11 @@ int glimp_framebuffer_keys(glimp_framebuffer* fb, void** keys, size_t* bytes);
 /* Pre-instantiated shader pipelines: the device counterpart of a dispatch
  * table built by GetDispatchTable<MeshFn, FragFn...> (Rasterizer.h:342-364). */
 #define GLIMP_PIPELINE_VISBUFFER 0u   /* ShadingContext::VisBufferShader (Shading.cpp:650-654) */
+#define GLIMP_PIPELINE_OVERDRAW 2u    /* ShadingContext::OverdrawShader: FS_Overdraw (Shading.cpp:333-342, 656).
+                                       * Layer 0 holds per pixel the covered count (high 16 bits) and the
+                                       * quad-helper count (low 16 bits) added to the clear colour
+                                       * (saturating), layer 1 the maximum plane depth of the tiles drawn.
+                                       * Always rasterized with per-fragment atomics (GLIMP_DRAW_DIRECT). */
 
 typedef struct glimp_draw_desc {
     uint32_t meshlet_offset;           /* ShadingContext::MeshletOffset */
@@ -149,7 +161,9 @@ typedef struct glimp_draw_desc {
 /* Rasterizer::DrawMeshlets(fb, count, {VisBufferShader, &ctx})
  * (Rasterizer.h:213), preceded by ShadingContext::CullMeshlets' frustum
  * test (Shading.cpp:775-869) when planes are given. Depth-test ties resolve
- * in the reference's single-thread draw order. Asynchronous. */
+ * in the reference's single-thread draw order. A framebuffer holds the
+ * output of one pipeline between clears: drawing with another pipeline
+ * before clearing fails with GLIMP_E_INVALID. Asynchronous. */
 int glimp_draw_meshlets(glimp_context* ctx, glimp_framebuffer* fb, glimp_scene* scene,
                         const glimp_draw_desc* desc, void* stream);
 
@@ -305,6 +319,8 @@ int glimp_diff_backward_views(glimp_context* ctx, glimp_scene* scene, const glim
 #define GLIMP_RESOLVE_METALLIC_ROUGHNESS 3u  /* ResolveDebug(DebugLayer::MetallicRoughness) */
 #define GLIMP_RESOLVE_MESHLET_ID 4u          /* ResolveDebug(DebugLayer::MeshletId) */
 #define GLIMP_RESOLVE_TRIANGLE_ID 5u         /* ResolveDebug(DebugLayer::TriangleId) */
+#define GLIMP_RESOLVE_OVERDRAW_PIXEL 6u      /* ResolveDebug(DebugLayer::OverdrawPixel); overdraw framebuffers */
+#define GLIMP_RESOLVE_OVERDRAW_QUAD 7u       /* ResolveDebug(DebugLayer::OverdrawQuad); overdraw framebuffers */
 #define GLIMP_MAX_LIGHTS 8u
 /* glimp_resolve_desc.flags: Resolve draws its point and spot lights as
  * sprites over the image (Shading.cpp:689-727); needs world_to_clip. */

@@ -56,18 +56,16 @@ public:
     void Clear(uint32_t color, float depth, void* stream = nullptr) {
         check(glimp_framebuffer_clear(h_, color, depth, stream), "glimp_framebuffer_clear");
     }
-    // GetPixels (ImageHelpers.cpp:109-147): layer 0 = surface ids (or the
-    // resolved RGBA8 colour after a resolve), layer 1 = depth bits;
-    // `stride` in pixels.
+    // GetPixels (ImageHelpers.cpp:109-147): layer 0 = surface ids, overdraw
+    // counters or base colour (or the resolved RGBA8 colour after a
+    // resolve), layer 1 = depth bits, layer 2 = the deferred pipeline's
+    // normal / metallic / roughness; `stride` in pixels.
     void GetPixels(uint32_t layer, uint32_t* dest, uint32_t stride, void* stream = nullptr) {
         std::vector<uint32_t> tmp((size_t)Width * Height);
         if (layer == 0 && resolved_) {
             check(glimp_framebuffer_read_color(h_, tmp.data(), stream), "glimp_framebuffer_read_color");
-        } else if (layer == 0) {
-            check(glimp_framebuffer_read(h_, tmp.data(), nullptr, 1, stream), "glimp_framebuffer_read");
         } else {
-            check(glimp_framebuffer_read(h_, nullptr, reinterpret_cast<float*>(tmp.data()), 1, stream),
-                  "glimp_framebuffer_read");
+            check(glimp_framebuffer_read_layer(h_, layer, tmp.data(), 1, stream), "glimp_framebuffer_read_layer");
         }
         for (uint32_t y = 0; y < Height; y++) std::memcpy(dest + (size_t)y * stride, &tmp[(size_t)y * Width], Width * 4);
     }
@@ -147,6 +145,7 @@ private:
 // as pipelines, and the tags select one.
 struct ShadeMeshlet {};                            // ShadingContext::ShadeMeshlet (Shading.cpp:281-307)
 template <bool AlphaTest> struct FS_EncodeSurfaceId {};  // Shading.cpp:309-331
+struct FS_Overdraw {};                             // Shading.cpp:333-342
 
 struct ShaderDispatchTable {
     uint32_t pipeline;
@@ -169,15 +168,21 @@ struct PipelineOf<ShadeMeshlet, FS_EncodeSurfaceId<false>> {
     static constexpr uint32_t value = GLIMP_PIPELINE_VISBUFFER;
 };
 
+// ShadingContext::OverdrawShader (Shading.cpp:656).
+template <>
+struct PipelineOf<ShadeMeshlet, FS_Overdraw> {
+    static constexpr uint32_t value = GLIMP_PIPELINE_OVERDRAW;
+};
+
 template <class MeshFn, class... FragFns>
 const ShaderDispatchTable& GetDispatchTable() {
     static const ShaderDispatchTable table{ PipelineOf<MeshFn, FragFns...>::value };
     return table;
 }
 
-// DebugLayer (Shading.h:8); the overdraw layers need FS_Overdraw, a pipeline
-// this library does not build.
-enum class DebugLayer { None, BaseColor, Normals, MetallicRoughness, MeshletId, TriangleId };
+// DebugLayer (Shading.h:8); the overdraw layers read a framebuffer drawn
+// with the overdraw table, the others a vis-buffer.
+enum class DebugLayer { None, BaseColor, Normals, MetallicRoughness, MeshletId, TriangleId, OverdrawPixel, OverdrawQuad };
 
 struct Lighting {
     float object_to_world[9] = { 1, 0, 0, 0, 1, 0, 0, 0, 1 };
@@ -305,6 +310,8 @@ inline void ShadingContext::ResolveDebug(Rasterizer& rast, Framebuffer& fb, Debu
     case DebugLayer::MetallicRoughness: d.mode = GLIMP_RESOLVE_METALLIC_ROUGHNESS; break;
     case DebugLayer::MeshletId: d.mode = GLIMP_RESOLVE_MESHLET_ID; break;
     case DebugLayer::TriangleId: d.mode = GLIMP_RESOLVE_TRIANGLE_ID; break;
+    case DebugLayer::OverdrawPixel: d.mode = GLIMP_RESOLVE_OVERDRAW_PIXEL; break;
+    case DebugLayer::OverdrawQuad: d.mode = GLIMP_RESOLVE_OVERDRAW_QUAD; break;
     default: throw Error(GLIMP_E_UNSUPPORTED, "ResolveDebug: no such layer");
     }
     std::memcpy(d.object_to_clip, ObjectToClipMat, sizeof(d.object_to_clip));

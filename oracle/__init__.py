@@ -57,8 +57,11 @@ class Framebuffer(C.Structure):
 
 
 class RasterOpts(C.Structure):
-    _fields_ = [("enable_clipping", C.c_uint8), ("enable_guardband", C.c_uint8), ("pad", C.c_uint8 * 2),
-                ("coverage_count", u32p)]
+    _fields_ = [("enable_clipping", C.c_uint8), ("enable_guardband", C.c_uint8), ("pipeline", C.c_uint8),
+                ("pad", C.c_uint8), ("coverage_count", u32p)]
+
+
+PIPELINE_VISBUFFER, PIPELINE_DEFERRED, PIPELINE_OVERDRAW = 0, 1, 2
 
 
 class Counters(C.Structure):
@@ -197,9 +200,10 @@ class OracleFramebuffer:
 
 
 def draw(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip, count: int, meshlet_offset: int = 0,
-         cull_bitmap=None, clipping: bool = True, guardband: bool = True, coverage: bool = False):
+         cull_bitmap=None, clipping: bool = True, guardband: bool = True, coverage: bool = False,
+         pipeline: int = PIPELINE_VISBUFFER):
     cov = np.zeros((fb.height, fb.width), dtype=np.uint32) if coverage else None
-    opts = RasterOpts(int(clipping), int(guardband), (C.c_uint8 * 2)(), _ptr(cov, u32p) if cov is not None else u32p())
+    opts = RasterOpts(int(clipping), int(guardband), pipeline, 0, _ptr(cov, u32p) if cov is not None else u32p())
     ctr = Counters()
     ctx = binding.ctx(object_to_clip, meshlet_offset, cull_bitmap)
     lib().orc_draw_meshlets(C.byref(fb.fb), C.c_uint32(count), C.byref(ctx), C.byref(opts), C.byref(ctr))
@@ -268,7 +272,8 @@ def resolve_albedo(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip,
     return fb.pixels(0)
 
 
-DEBUG_BASE_COLOR, DEBUG_NORMALS, DEBUG_METALLIC_ROUGHNESS, DEBUG_MESHLET_ID, DEBUG_TRIANGLE_ID = range(5)
+(DEBUG_BASE_COLOR, DEBUG_NORMALS, DEBUG_METALLIC_ROUGHNESS, DEBUG_MESHLET_ID, DEBUG_TRIANGLE_ID,
+ DEBUG_OVERDRAW_PIXEL, DEBUG_OVERDRAW_QUAD) = range(7)
 
 
 def resolve_debug(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip, layer: int,

@@ -404,9 +404,35 @@ uint16_t alpha_test_tile(const FragmentCtx& fc, const float* bu, const float* bv
     return pass;
 }
 
+// FS_Overdraw (Shading.cpp:333-342) on one 4x4 tile with at least one
+// covered pixel: every pixel's counter gains 1 in its high 16 bits when
+// covered, else 1 in its low 16 bits (saturating 16-bit adds), and every
+// pixel's depth becomes max(plane depth, stored depth) -- uncovered pixels
+// included, with the plane's value extrapolated to them. The max is
+// vmaxps(depth, stored): the stored value unless the new one is greater.
+void overdraw_tile(DrawState& st, int tx, int ty, uint16_t mask, const float* depth) {
+    orc_framebuffer* fb = st.fb;
+    uint32_t* counters = fb->data;
+    float* depthBuf = (float*)(fb->data + fb->layer_stride);
+    for (int p = 0; p < 16; p++) {
+        uint32_t off = pixel_offset(fb->width, (uint32_t)(tx + (p & 3)), (uint32_t)(ty + (p >> 2)));
+        uint32_t c = counters[off];
+        uint32_t hi = c >> 16, lo = c & 0xFFFFu;
+        if ((mask >> p) & 1) {
+            hi = std::min(hi + 1u, 0xFFFFu);
+            st.ctr->fragments_passed++;
+        } else {
+            lo = std::min(lo + 1u, 0xFFFFu);
+        }
+        counters[off] = (hi << 16) | lo;
+        if (depth[p] > depthBuf[off]) depthBuf[off] = depth[p];
+    }
+}
+
 // Rasterizer::DrawTriangle (Rasterizer.h:250-328) with FS_EncodeSurfaceId
 // (Shading.cpp:309-331) inlined: per 4x4 tile the depth test, then, for
-// alpha-tested materials, the alpha test.
+// alpha-tested materials, the alpha test. With the overdraw table the
+// fragment shader is FS_Overdraw instead.
 void draw_triangle(DrawState& st, const EdgeVars& e, int l, uint32_t bmin, uint32_t bmax, uint32_t surfaceId,
                    const FragmentCtx& fc) {
     orc_framebuffer* fb = st.fb;
@@ -433,6 +459,10 @@ void draw_triangle(DrawState& st, const EdgeVars& e, int l, uint32_t bmin, uint3
                 if (st.opts && st.opts->coverage_count) st.opts->coverage_count[(size_t)y * W + x]++;
             }
             if (!mask) continue;
+            if (st.opts && st.opts->pipeline == ORC_PIPELINE_OVERDRAW) {
+                overdraw_tile(st, tx, ty, mask, depth);
+                continue;
+            }
             uint16_t pass = 0;
             for (int p = 0; p < 16; p++) {
                 if (!((mask >> p) & 1)) continue;
@@ -1504,6 +1534,36 @@ int32_t orc_clip_triangle(const float* v /*[3][4]*/, uint32_t outcodes, float gb
     return (int32_t)c.OutIdx;
 }
 
+// Coefficients of ColormapTurbo (Shading.cpp:249-260): Google's published
+// degree-5 polynomial fit of the Turbo colormap, one row per channel,
+// constant term first; double literals rounded to float.
+static const float kTurbo[18] = {
+    0.13572138, 4.61539260,  -42.66032258, 132.13108234, -152.94239396, 59.28637943,
+    0.09140261, 2.19418839,  4.84296658,   -14.18503333, 4.27729857,    2.82956604,
+    0.10667330, 12.64194608, -60.58204836, 110.36276771, -89.90310912,  27.34824973,
+};
+
+// The overdraw debug colour (ResolveDebug, Shading.cpp:759-766) of one
+// counter word (pixel count << 16 | quad-helper count): ColormapTurbo of
+// pixels / 30 (OverdrawPixel) or (pixels + helpers * 0.5) / 30
+// (OverdrawQuad). The fast-math build divides by multiplying with 1/30 and
+// folds the innermost x * c5 into s * (c5 * (1/30)), s the undivided count;
+// the rest of the Horner chain is FMAs.
+static void overdraw_colour(uint32_t data, bool quad, float rgb[3]) {
+    const float pix = (float)(data >> 16);
+    const float s = quad ? pix + (float)(data & 0xFFFFu) * 0.5f : pix;
+    const float r = 1.0f / 30.0f;
+    const float x = s * r;
+    for (int k = 0; k < 3; k++) {
+        const float* c = kTurbo + 6 * k;
+        float t = fmaf(s, c[5] * r, c[4]);
+        t = fmaf(x, t, c[3]);
+        t = fmaf(x, t, c[2]);
+        t = fmaf(x, t, c[1]);
+        rgb[k] = fmaf(x, t, c[0]);
+    }
+}
+
 // ResolveDebug(DebugLayer::BaseColor) (Shading.cpp:734-773).
 // ResolveDebug (Shading.cpp:734-773) for the surface layers (BaseColor,
 // Normals, MetallicRoughness) and the id layers (MeshletId, TriangleId);
@@ -1547,6 +1607,8 @@ void orc_resolve_debug(orc_framebuffer* fb, const orc_shading_ctx* ctx, uint32_t
                     c[0] = (float)((lanes[l].nmr >> 16) & 255) * (1.0f / 255);
                     c[1] = (float)((lanes[l].nmr >> 24) & 255) * (1.0f / 255);
                     c[2] = 0.0f;
+                } else if (layer == ORC_DEBUG_OVERDRAW_PIXEL || layer == ORC_DEBUG_OVERDRAW_QUAD) {
+                    overdraw_colour(ids[l], layer == ORC_DEBUG_OVERDRAW_QUAD, c);
                 } else {
                     // RGBA8u::Unpack of the hashed id (MeshletId: id / 128)
                     uint32_t v = layer == ORC_DEBUG_MESHLET_ID ? ids[l] / 128u : ids[l];
@@ -1557,6 +1619,14 @@ void orc_resolve_debug(orc_framebuffer* fb, const orc_shading_ctx* ctx, uint32_t
         }
     }
     memcpy(fb->data, out.data(), (size_t)W * H * 4);
+}
+
+void orc_overdraw_colour16(const uint32_t* data, int32_t quad, float* out) {
+    for (int l = 0; l < 16; l++) {
+        float c[3];
+        overdraw_colour(data[l], quad != 0, c);
+        for (int k = 0; k < 3; k++) out[k * 16 + l] = c[k];
+    }
 }
 
 void orc_resolve_albedo(orc_framebuffer* fb, const orc_shading_ctx* ctx) {

@@ -67,9 +67,15 @@ typedef struct orc_framebuffer {
 typedef struct orc_raster_opts {
     uint8_t enable_clipping;
     uint8_t enable_guardband;
-    uint8_t pad[2];
+    uint8_t pipeline;                 /* ORC_PIPELINE_*: the fragment shader of the dispatch table */
+    uint8_t pad;
     uint32_t* coverage_count;         /* optional [height][width]: covered samples per pixel, before depth test */
 } orc_raster_opts;
+
+/* Dispatch tables of ShadingContext (Shading.cpp:650-656). */
+#define ORC_PIPELINE_VISBUFFER 0u     /* FS_EncodeSurfaceId<false/true> */
+#define ORC_PIPELINE_DEFERRED 1u      /* FS_EncodeGBuffer (three layers) */
+#define ORC_PIPELINE_OVERDRAW 2u      /* FS_Overdraw */
 
 typedef struct orc_counters {
     uint64_t triangles_processed;
@@ -144,7 +150,11 @@ void orc_resolve_albedo(orc_framebuffer* fb, const orc_shading_ctx* ctx);
 #define ORC_DEBUG_METALLIC_ROUGHNESS 2u
 #define ORC_DEBUG_MESHLET_ID 3u
 #define ORC_DEBUG_TRIANGLE_ID 4u
+#define ORC_DEBUG_OVERDRAW_PIXEL 5u   /* framebuffer drawn with ORC_PIPELINE_OVERDRAW */
+#define ORC_DEBUG_OVERDRAW_QUAD 6u
 void orc_resolve_debug(orc_framebuffer* fb, const orc_shading_ctx* ctx, uint32_t layer, const float* object_to_world);
+/* The overdraw layers' colour for 16 counter words: out = rgb [3][16]. */
+void orc_overdraw_colour16(const uint32_t* data, int32_t quad, float* out);
 /* Resolve(): lit, tonemapped RGBA8 into layer 0. */
 void orc_resolve_lit(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_resolve_params* p);
 

@@ -359,3 +359,33 @@ void probe_uvgrad(const float* t, const float* bary, const float* ddx, const flo
 }
 
 }  // extern "C"
+
+// Overdraw debug colours (ResolveDebug, Shading.cpp:759-766, with
+// ColormapTurbo, Shading.cpp:249-260: the published polynomial fit of the
+// Turbo colormap, Horner form). in: counter words (pixel count << 16 |
+// quad-helper count); quad = 0: pixels / 30, else (pixels + helpers * 0.5) / 30.
+// out: rgb[3][n].
+static const float kTurbo[18] = {   // double literals rounded to float, as a float array of them is
+    0.13572138, 4.61539260,  -42.66032258, 132.13108234, -152.94239396, 59.28637943,
+    0.09140261, 2.19418839,  4.84296658,   -14.18503333, 4.27729857,    2.82956604,
+    0.10667330, 12.64194608, -60.58204836, 110.36276771, -89.90310912,  27.34824973,
+};
+extern "C" void probe_overdraw_colour(const uint32_t* data, int quad, float* out, int n) {
+    typedef uint32_t vu __attribute__((ext_vector_type(16)));
+    for (int i = 0; i < n; i += 16) {
+        vu d;
+        memcpy(&d, data + i, 64);
+        vf pix = __builtin_convertvector(d >> 16, vf);
+        vf x;
+        if (quad) {
+            vf frag = __builtin_convertvector(d & 0xFFFFu, vf);
+            x = (pix + frag * 0.5f) / 30.0f;
+        } else {
+            x = pix / 30.0f;
+        }
+        for (int k = 0; k < 3; k++) {
+            const float* c = kTurbo + 6 * k;
+            st(out + k * n + i, x * (x * (x * (x * (x * c[5] + c[4]) + c[3]) + c[2]) + c[1]) + c[0]);
+        }
+    }
+}

@@ -106,6 +106,18 @@ int main(int argc, char** argv) {
         // visible count through the standalone cull entry point
         std::vector<uint16_t> bitmap((count + 15) / 16);
         uint32_t visible = shader.CullMeshlets(gpu, bitmap.data(), 0, count, planes);
+        // the same frame through OverdrawShader (Main.cpp:204-213: clear to 0,
+        // then ResolveDebug of an overdraw layer)
+        std::vector<uint32_t> odWords((size_t)W * H), odDepth((size_t)W * H), odColor((size_t)W * H);
+        fb.Clear(0, 0.0f);
+        rast.DrawMeshlets(fb, count, { GetDispatchTable<ShadeMeshlet, FS_Overdraw>(), &shader });
+        fb.GetPixels(0, odWords.data(), W);
+        fb.GetPixels(1, odDepth.data(), W);
+        shader.ResolveDebug(rast, fb, DebugLayer::OverdrawQuad);
+        fb.GetPixels(0, odColor.data(), W);
+        save(dir + "/od_words.bin", odWords);
+        save(dir + "/od_depth.bin", odDepth);
+        save(dir + "/od_color.bin", odColor);
         save(dir + "/ids.bin", ids);
         save(dir + "/depth.bin", depth);
         save(dir + "/color.bin", color);

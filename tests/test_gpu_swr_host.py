@@ -53,3 +53,13 @@ def test_swr_host_program(tmp_path):
     assert (bitmap == want_bitmap).all()
     assert f"{visible} visible" in r.stdout
     assert (color == oracle.resolve_albedo(ofb, b, view.object_to_clip)).all()
+    # the overdraw table over the same frame
+    bm, _ = oracle.cull(b, view.frustum_planes(), 0, sc.count)
+    od = oracle.OracleFramebuffer(view.width, view.height)
+    od.clear(0, 0.0)
+    oracle.draw(od, b, view.object_to_clip, sc.count, 0, bm, pipeline=oracle.PIPELINE_OVERDRAW)
+    words = np.fromfile(tmp_path / "od_words.bin", np.uint32).reshape(view.height, view.width)
+    odepth = np.fromfile(tmp_path / "od_depth.bin", np.uint32).reshape(view.height, view.width)
+    ocolor = np.fromfile(tmp_path / "od_color.bin", np.uint32).reshape(view.height, view.width)
+    assert (words == od.pixels(0)).all() and (odepth == od.pixels(1)).all()
+    assert (ocolor == oracle.resolve_debug(od, b, view.object_to_clip, oracle.DEBUG_OVERDRAW_QUAD)).all()

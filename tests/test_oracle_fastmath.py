@@ -343,3 +343,24 @@ def test_hiz_sphere_projection(probe):
             else:
                 assert (a == b).all(), trial
     assert projected > 500
+
+
+@pytest.mark.parametrize("quad", [0, 1])
+def test_overdraw_colormap(probe, quad):
+    """ColormapTurbo of the overdraw counts (Shading.cpp:249-260, 759-766):
+    every pixel count and helper count up to 300 each, plus random words."""
+    pix, frag = np.meshgrid(np.arange(301, dtype=np.uint32), np.arange(301, dtype=np.uint32))
+    rng = np.random.default_rng(7)
+    words = np.concatenate([(pix << 16 | frag).ravel(), rng.integers(0, 2 ** 32, 4096, dtype=np.uint64).astype(np.uint32)])
+    words = np.ascontiguousarray(words[: len(words) // 16 * 16])
+    n = len(words)
+    want = np.zeros(3 * n, np.float32)
+    probe.probe_overdraw_colour(words.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_int(quad), P(want), C.c_int(n))
+    want = want.reshape(3, n)
+    got = np.zeros((3, n), np.float32)
+    L = oracle.lib()
+    for i in range(0, n, 16):
+        o = np.zeros(48, np.float32)
+        L.orc_overdraw_colour16(words[i:i + 16].ctypes.data_as(C.POINTER(C.c_uint32)), C.c_int32(quad), P(o))
+        got[:, i:i + 16] = o.reshape(3, 16)
+    assert (_bits(got) == _bits(want)).all()
