@@ -60,7 +60,7 @@ class DrawDesc(C.Structure):
     _fields_ = [("meshlet_offset", C.c_uint32), ("meshlet_count", C.c_uint32),
                 ("object_to_clip", C.c_float * 16), ("frustum_planes", f32p), ("flags", C.c_uint32),
                 ("pipeline", C.c_uint32), ("bin_size", C.c_uint32), ("bin_capacity", C.c_uint32),
-                ("occlusion", C.POINTER(OcclusionDesc))]
+                ("occlusion", C.POINTER(OcclusionDesc)), ("object_to_world", C.c_float * 9)]
 
 
 class ViewDesc(C.Structure):
@@ -457,8 +457,14 @@ class Framebuffer:
 
     def draw(self, scene: Scene, object_to_clip, count: int, offset: int = 0, planes=None, flags: int = 0,
              stream=None, bin_size: int = 0, bin_capacity: int = 0, occlusion: "Occlusion | None" = None,
-             pipeline: int = GLIMP_PIPELINE_VISBUFFER):
+             pipeline: int = GLIMP_PIPELINE_VISBUFFER, object_to_world=None):
+        """Rasterizer::DrawMeshlets with the pipeline's dispatch table;
+        ``object_to_world`` (mat3, column-major, default identity) is the
+        deferred pipeline's ObjectToWorldMat."""
         d = DrawDesc()
+        o2w = np.eye(3).reshape(9) if object_to_world is None else np.asarray(object_to_world).reshape(9)
+        for i in range(9):
+            d.object_to_world[i] = float(o2w[i])
         d.meshlet_offset, d.meshlet_count = offset, count
         for i in range(16):
             d.object_to_clip[i] = float(object_to_clip[i])

@@ -131,6 +131,10 @@ struct glimp_scene {
     DevTexture* textures = nullptr;
     uint32_t num_textures = 0;
     bool has_alpha = false;            // a material runs the alpha test
+    std::vector<uint32_t> material_id; // host copy of the meshlets' material ids
+    // bad[i] = meshlets before i without a textured material (the deferred
+    // pipeline samples every meshlet's texture)
+    std::vector<uint32_t> untextured_before;
 };
 
 struct glimp_framebuffer {
@@ -257,6 +261,9 @@ int glimp_scene_create(glimp_context* c, const glimp_meshlets_desc* d, glimp_sce
         glimp_scene_destroy(s);
         return fail(GLIMP_E_HIP, std::string("glimp_scene_create: ") + hipGetErrorString(e));
     }
+    s->material_id.assign(d->material_id, d->material_id + n);
+    s->untextured_before.resize(n + 1);
+    for (size_t i = 0; i <= n; i++) s->untextured_before[i] = (uint32_t)i;   // no materials yet
     *out = s;
     return GLIMP_OK;
 }
@@ -347,6 +354,13 @@ int glimp_scene_set_materials(glimp_scene* s, const glimp_material* mats, uint32
     s->num_textures = num_textures;
     s->has_alpha = false;
     for (const auto& m : dm) s->has_alpha = s->has_alpha || (m.fs_id == 1 && m.texture >= 0);
+    uint32_t bad = 0;
+    for (size_t i = 0; i < s->material_id.size(); i++) {
+        s->untextured_before[i] = bad;
+        const uint32_t id = s->material_id[i];
+        bad += (id >= count || dm[id].texture < 0) ? 1u : 0u;
+    }
+    s->untextured_before[s->material_id.size()] = bad;
     return GLIMP_OK;
 }
 
@@ -463,6 +477,19 @@ int read_overdraw(glimp_framebuffer* f, uint32_t* words, uint32_t* depth, int li
     return GLIMP_OK;
 }
 
+// Layer `which` (0 colour, 1 normal/MR) of a deferred framebuffer.
+int read_aux_layer(glimp_framebuffer* f, uint32_t which, uint32_t* out, int linear, hipStream_t s) {
+    const size_t n = (size_t)f->width * f->height;
+    uint32_t* d_out = nullptr;
+    HIP_TRY(dev_alloc(&d_out, n));
+    hipError_t e = launch_detile32(f->aux + which * n, (int)f->width, (uint32_t)n, d_out, linear, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, n * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return fail(GLIMP_E_HIP, std::string("glimp_framebuffer_read: ") + hipGetErrorString(e));
+    return GLIMP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -473,6 +500,12 @@ int glimp_framebuffer_read(glimp_framebuffer* f, uint32_t* ids, float* depth, in
     hipStream_t s = (hipStream_t)stream;
     size_t n = (size_t)f->width * f->height;
     if (f->content == (int)GLIMP_PIPELINE_OVERDRAW) return read_overdraw(f, ids, (uint32_t*)depth, linear, s);
+    if (f->content == (int)GLIMP_PIPELINE_DEFERRED && ids) {
+        // layer 0 is the G-buffer's colour layer; depth still comes from the keys
+        int rc = read_aux_layer(f, 0, ids, linear, s);
+        if (rc != GLIMP_OK || !depth) return rc;
+        ids = nullptr;
+    }
     HIP_TRY(flush_clear(f, s));
     uint32_t ndraws = 0;
     int rc = upload_draws(f, s, &ndraws);
@@ -499,6 +532,10 @@ int glimp_framebuffer_read_layer(glimp_framebuffer* f, uint32_t layer, uint32_t*
     if (!f || !out) return fail(GLIMP_E_INVALID, "glimp_framebuffer_read_layer: NULL argument");
     if (layer == 0) return glimp_framebuffer_read(f, out, nullptr, linear, stream);
     if (layer == 1) return glimp_framebuffer_read(f, nullptr, (float*)out, linear, stream);
+    if (layer == 2 && f->aux && f->content != (int)GLIMP_PIPELINE_OVERDRAW) {
+        HIP_TRY(hipSetDevice(f->ctx->device));
+        return read_aux_layer(f, 1, out, linear, (hipStream_t)stream);
+    }
     return fail(GLIMP_E_INVALID, "glimp_framebuffer_read_layer: the framebuffer has no such layer");
 }
 
@@ -669,7 +706,8 @@ extern "C" {
 int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, const glimp_draw_desc* d,
                         void* stream) {
     if (!c || !f || !s || !d) return fail(GLIMP_E_INVALID, "glimp_draw_meshlets: NULL argument");
-    if (d->pipeline != GLIMP_PIPELINE_VISBUFFER && d->pipeline != GLIMP_PIPELINE_OVERDRAW)
+    if (d->pipeline != GLIMP_PIPELINE_VISBUFFER && d->pipeline != GLIMP_PIPELINE_OVERDRAW &&
+        d->pipeline != GLIMP_PIPELINE_DEFERRED)
         return fail(GLIMP_E_UNSUPPORTED, "glimp_draw_meshlets: unknown pipeline");
     if (f->content >= 0 && f->content != (int)d->pipeline)
         return fail(GLIMP_E_INVALID, "glimp_draw_meshlets: the framebuffer holds another pipeline's output; clear it first");
@@ -685,8 +723,26 @@ int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, 
     if (d->meshlet_count == 0) return GLIMP_OK;
     HIP_TRY(hipSetDevice(c->device));
     const bool overdraw = d->pipeline == GLIMP_PIPELINE_OVERDRAW;
-    const bool direct = overdraw || (d->flags & GLIMP_DRAW_DIRECT) != 0;
+    const bool deferred = d->pipeline == GLIMP_PIPELINE_DEFERRED;
+    const bool direct = overdraw || deferred || (d->flags & GLIMP_DRAW_DIRECT) != 0;
     hipStream_t st = (hipStream_t)stream;
+    if (deferred) {
+        // FS_EncodeGBuffer samples the texture of every meshlet's material
+        // (Shading.cpp:357-366); without one the reference reads through a
+        // null texture, and a meshlet without a material leaves the colour of
+        // whichever earlier fragment passed the depth test, which one pass
+        // per draw cannot reproduce: such ranges are refused
+        if (s->untextured_before[d->meshlet_offset + d->meshlet_count] != s->untextured_before[d->meshlet_offset])
+            return fail(GLIMP_E_UNSUPPORTED,
+                        "glimp_draw_meshlets: the deferred pipeline needs a textured material on every meshlet");
+        const size_t n = (size_t)f->width * f->height;
+        if (!f->aux) {
+            HIP_TRY(dev_alloc(&f->aux, 2 * n));
+            HIP_TRY(hipMemsetAsync(f->aux, 0, 2 * n * 4, st));
+        }
+        if (f->content != (int)GLIMP_PIPELINE_DEFERRED)   // Clear's colour; layer 2 keeps its contents
+            HIP_TRY(launch_fill32(f->aux, (uint32_t)n, f->clear_color, st));
+    }
     if (overdraw && f->content != (int)GLIMP_PIPELINE_OVERDRAW) {
         // first overdraw draw since the clear: zero counts, depth = clear depth
         const size_t n = (size_t)f->width * f->height;
@@ -709,7 +765,7 @@ int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, 
     make_view(f, s, d->meshlet_offset, d->meshlet_count, d->object_to_clip, d->frustum_planes, d->flags, views[0],
               d->occlusion ? &occ : nullptr);
     const uint32_t total_bins = direct ? 0 : layout_bins(views, sh.bin_size);
-    const bool alpha = s->has_alpha && !overdraw;    // FS_Overdraw has no alpha test
+    const bool alpha = (s->has_alpha && !overdraw) || deferred;    // FS_Overdraw has no alpha test
     int rc = prepare_scratch(c->draw_scratch, d->meshlet_count, total_bins, sh, !direct, alpha);
     if (rc != GLIMP_OK) return rc;
     DrawParams p;
@@ -719,7 +775,10 @@ int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, 
     p.views = nullptr;
     p.view = views[0];
     p.view.aux = f->aux;
+    memcpy(p.view.o2w, d->object_to_world, sizeof(p.view.o2w));
+    p.deferred = deferred ? 1u : 0u;
     if (overdraw) HIP_TRY(launch_draw_overdraw(p, st, &c->prof));
+    else if (deferred) HIP_TRY(launch_draw_deferred(p, st, &c->prof));
     else if (direct) HIP_TRY(launch_draw_direct(p, st, &c->prof));
     else HIP_TRY(launch_draw_binned(p, st, &c->prof));
     f->pending_clear = false;
@@ -1133,8 +1192,8 @@ int glimp_hiz_build(glimp_hiz* h, glimp_framebuffer* f, void* stream) {
     if (!h || !f) return fail(GLIMP_E_INVALID, "glimp_hiz_build: NULL argument");
     if (f->width != h->layout.width || f->height != h->layout.height)
         return fail(GLIMP_E_INVALID, "glimp_hiz_build: framebuffer and depth map sizes differ");
-    if (f->content > (int)GLIMP_PIPELINE_VISBUFFER)
-        return fail(GLIMP_E_UNSUPPORTED, "glimp_hiz_build: only vis-buffer framebuffers");
+    if (f->content == (int)GLIMP_PIPELINE_OVERDRAW)
+        return fail(GLIMP_E_UNSUPPORTED, "glimp_hiz_build: an overdraw framebuffer has no depth-test depth");
     HIP_TRY(hipSetDevice(h->ctx->device));
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(flush_clear(f, s));
@@ -1260,6 +1319,8 @@ int glimp_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const
     hipStream_t s = (hipStream_t)stream;
     const bool od_mode = d->mode == GLIMP_RESOLVE_OVERDRAW_PIXEL || d->mode == GLIMP_RESOLVE_OVERDRAW_QUAD;
     const bool od_fb = f->content == (int)GLIMP_PIPELINE_OVERDRAW;
+    if (f->content == (int)GLIMP_PIPELINE_DEFERRED)
+        return fail(GLIMP_E_INVALID, "glimp_resolve: a deferred framebuffer holds G-buffer layers, not surface ids");
     if (od_mode != od_fb)
         return fail(GLIMP_E_INVALID, od_mode ? "glimp_resolve: the overdraw layers need an overdraw framebuffer"
                                              : "glimp_resolve: an overdraw framebuffer has only the overdraw layers");

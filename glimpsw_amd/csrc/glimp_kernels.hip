@@ -33,6 +33,7 @@
 #include <algorithm>
 
 #include "internal.h"
+#include "normals.h"
 #include "sampler.h"
 
 namespace glimp {
@@ -429,7 +430,9 @@ __device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, fl
         if (mat != 0xFFFFFFFFu && mat < V.num_materials) {
             const DevMaterial dm = V.materials[mat];
             cull_mode = dm.cull_mode;
-            alpha = Alpha && dm.fs_id == 1 && dm.texture >= 0;
+            // FS_EncodeSurfaceId<true> for FragmentShaderId 1; FS_EncodeGBuffer
+            // tests every textured material (Shading.cpp:357-366)
+            alpha = Alpha && (dm.fs_id == 1 || p.deferred) && dm.texture >= 0;
             alpha_tex = (uint32_t)dm.texture | ((uint32_t)dm.alpha_cutoff << 24);
         }
         // ShadeMeshlet (Shading.cpp:295-298): lane = vertex
@@ -814,19 +817,26 @@ __device__ __forceinline__ bool shade_pixel(const TriLds& T, uint32_t x, uint32_
     return true;
 }
 
-// The alpha test of FS_EncodeSurfaceId<true> (Shading.cpp:316-327) for the
-// 16 lanes of one 4x4 tile (lane & 15 = x + 4y inside the tile). All 16
+// UVs of one fragment of a 4x4 tile as FS_EncodeSurfaceId<true> and
+// FS_EncodeGBuffer compute them (lane & 15 = x + 4y inside the tile). All 16
 // lanes must be active: the screen derivatives and the filter choice are
 // taken across them. (e1, e2) are the lane's edge values, the u and v of
 // DrawTriangle: perspective-correct barycentrics (Rasterizer.h:298-314),
 // the clipped triangle's remap (Rasterizer.h:306-312), BaryLerp of the
-// vertex UVs, SampleImplicitLod (Texture.h:403-410: derivatives from the
-// tile's pixel pairs, mip level less the 8 fraction bits, minification
-// anywhere in the tile selects the min filter), then color >= cutoff << 24.
-// Operation order follows the reference's optimised build: `u *= W1S * rcpW`
-// is evaluated as (u * W1S) * rcpW and `1 - u - v` as 1 - (u + v).
-__device__ __forceinline__ bool alpha_test_tile(const ApproxTables& T, const AlphaRecord& A, bool clipped,
-                                                const DevTexture* textures, int32_t e1, int32_t e2, uint32_t lane) {
+// vertex UVs and SampleImplicitLod's level (Texture.h:403-410: derivatives
+// from the tile's pixel pairs, mip level less the 8 fraction bits,
+// minification anywhere in the tile selects the min filter). Operation
+// order follows the reference's optimised build: `u *= W1S * rcpW` is
+// evaluated as (u * W1S) * rcpW and `1 - u - v` as 1 - (u + v).
+struct TileUV {
+    float U, V;
+    int32_t mip;
+    bool minify;
+    float b[3];                    // Bary = {1 - u - v, u, v}
+};
+
+__device__ __forceinline__ TileUV tile_uv(const ApproxTables& T, const AlphaRecord& A, bool clipped,
+                                          const DevTexture& tex, int32_t e1, int32_t e2, uint32_t lane) {
     const float u = (float)e1, v = (float)e2;
     const float pw0 = __builtin_fmaf(u + v, -A.w0s, A.w0);
     const float w = __builtin_fmaf(u, A.w1s, __builtin_fmaf(v, A.w2s, pw0));
@@ -839,17 +849,20 @@ __device__ __forceinline__ bool alpha_test_tile(const ApproxTables& T, const Alp
         bu = cu;
         bv = cv;
     }
+    TileUV o;
     const float b0 = 1.0f - (bu + bv);
+    o.b[0] = b0;
+    o.b[1] = bu;
+    o.b[2] = bv;
     float tu[3], tv[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         tu[k] = half_to_float(A.uv[k] & 0xFFFFu);
         tv[k] = half_to_float(A.uv[k] >> 16);
     }
-    const float U = __builtin_fmaf(tu[0], b0, __builtin_fmaf(tu[1], bu, tu[2] * bv));
-    const float V = __builtin_fmaf(tv[0], b0, __builtin_fmaf(tv[1], bu, tv[2] * bv));
-    const DevTexture& tex = textures[A.tex & 0xFFFFFFu];
-    const float su = U * tex.scale_lerp_u, sv = V * tex.scale_lerp_v;
+    o.U = __builtin_fmaf(tu[0], b0, __builtin_fmaf(tu[1], bu, tu[2] * bv));
+    o.V = __builtin_fmaf(tv[0], b0, __builtin_fmaf(tv[1], bu, tv[2] * bv));
+    const float su = o.U * tex.scale_lerp_u, sv = o.V * tex.scale_lerp_v;
     // dFdx pairs pixels (2i, 2i+1) of a row, dFdy rows (r, r+1) of a pair
     const float sux = __shfl_xor(su, 1), svx = __shfl_xor(sv, 1);
     const float suy = __shfl_xor(su, 4), svy = __shfl_xor(sv, 4);
@@ -858,10 +871,20 @@ __device__ __forceinline__ bool alpha_test_tile(const ApproxTables& T, const Alp
     const float gz = low ? su - suy : suy - su, gw = low ? sv - svy : svy - sv;
     const float dx = __builtin_fmaf(gx, gx, gy * gy), dy = __builtin_fmaf(gz, gz, gw * gw);
     const float mx = (dx > dy) ? dx : dy;
-    const int32_t mip = ((int32_t)(fbits(mx) - (127u << 23)) >> 24) - 8;
+    o.mip = ((int32_t)(fbits(mx) - (127u << 23)) >> 24) - 8;
     const unsigned long long groupMask = 0xFFFFull << (lane & 48u);
-    const bool minify = (__ballot(mip > 0) & groupMask) != 0;
-    const uint32_t color = sample_level(tex, U, V, mip, !minify);
+    o.minify = (__ballot(o.mip > 0) & groupMask) != 0;
+    return o;
+}
+
+// The alpha test of FS_EncodeSurfaceId<true> (Shading.cpp:316-327) for the
+// 16 lanes of one 4x4 tile: SampleImplicitLod at the fragment's UV, then
+// color >= cutoff << 24.
+__device__ __forceinline__ bool alpha_test_tile(const ApproxTables& T, const AlphaRecord& A, bool clipped,
+                                                const DevTexture* textures, int32_t e1, int32_t e2, uint32_t lane) {
+    const DevTexture& tex = textures[A.tex & 0xFFFFFFu];
+    const TileUV t = tile_uv(T, A, clipped, tex, e1, e2, lane);
+    const uint32_t color = sample_level(tex, t.U, t.V, t.mip, !t.minify);
     return color >= ((A.tex >> 24) << 24);
 }
 
@@ -1087,6 +1110,160 @@ __global__ void __launch_bounds__(256) k_raster_overdraw(DrawParams p) {
             if (lane == 0) {
                 atomicAdd(&p.counters->fragments_covered, (unsigned long long)covered);
                 atomicAdd(&p.counters->fragments_passed, (unsigned long long)covered);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// cvttps2udq: truncation; NaN and results outside [0, 2^32) give 0xFFFFFFFF.
+__device__ __forceinline__ uint32_t cvt_trunc_u32(float x) {
+    if (!(x > -1.0f && x < 4294967296.0f)) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+
+// The layer-2 word of FS_EncodeGBuffer (Shading.cpp:397-401): MapOctahedron
+// (Texture.h:282-288) of the remapped normal, each coordinate as
+// conv<uint32_t>(oct * 1023 + 0.5) in 10 bits, then the NMR texel's
+// metallic and roughness bytes cut to 6 bits. The optimised build folds
+// (c * 0.5 + 0.5) * 1023 + 0.5 into fma(c, 511.5, 512), sums the absolute
+// values left to right, takes 1/sum from VRCP14PS alone and evaluates
+// max(-z * w, 0) as zero unless positive.
+__device__ __forceinline__ uint32_t gbuffer_pack(const ApproxTables& T, const float n[3], uint32_t nmr) {
+    const float w = rcp14(T, (fabsf(n[0]) + fabsf(n[1])) + fabsf(n[2]));
+    const float mz = -n[2] * w;
+    const float t = mz > 0.0f ? mz : 0.0f;
+    const float u = __builtin_fmaf(n[0], w, mulsign(t, n[0]));
+    const float v = __builtin_fmaf(n[1], w, mulsign(t, n[1]));
+    uint32_t r = cvt_trunc_u32(__builtin_fmaf(u, 511.5f, 512.0f));
+    r |= cvt_trunc_u32(__builtin_fmaf(v, 511.5f, 512.0f)) << 10;
+    r |= ((nmr >> 18) & 0x3Fu) << 20;
+    r |= ((nmr >> 26) & 0x3Fu) << 26;
+    return r;
+}
+
+// FS_EncodeGBuffer's attribute stores (Shading.cpp:357-401) as a second pass
+// over the draw's records, after the vis-buffer raster has settled which
+// fragment wins each pixel (depth test, draw order and the alpha test,
+// which FS_EncodeGBuffer applies to every material). A fragment wins when
+// its key equals the pixel's final key; keys are unique per fragment, so
+// exactly one fragment per covered pixel writes, and the result is the
+// reference's last write. A tile with a winning fragment computes UVs and
+// levels on all 16 lanes (derivatives and the filter choice span the tile),
+// samples the base colour and, for two-layer textures, the NMR texel, and
+// the winning lanes write the colour and the packed normal / metallic /
+// roughness word (zero for one-layer textures). Meshlets of a deferred
+// draw all have textured materials (checked on the host), so every record
+// has its alpha record.
+__global__ void __launch_bounds__(256) k_gbuffer(DrawParams p) {
+    __shared__ TriLds tl[4][64];
+    __shared__ uint32_t tview[4][64];
+    __shared__ uint32_t talpha[4][64];
+    const uint32_t wid = threadIdx.x / kWave;
+    const uint32_t lane = lane_id();
+    const uint32_t nchunks = num_chunks(p, 0);
+    const uint32_t waves = gridDim.x * 4;
+    const unsigned long long groupMask = 0xFFFFull << (lane & 48u);
+    const ApproxTables& AT = p.tables;
+
+    for (uint32_t c = blockIdx.x * 4 + wid; c < nchunks; c += waves) {
+        const Chunk ck = get_chunk(p, 0, c);
+        if (ck.count == 0) continue;
+        uint32_t ntiles = 0;
+        TriLds t;
+        if (lane < ck.count) {
+            const uint32_t r = ck.list ? ck.list[ck.base + lane] : ck.base + lane;
+            const TriRecord rec = p.records[r];
+            const uint32_t v = rec.flags >> 16;
+            const ViewParams& V = view_of(p, v);
+            uint32_t bmin, bmax;
+            load_tri(rec, V.width / 2, V.height / 2, t, bmin, bmax);
+            t.org = bmin;
+            const uint32_t tw = (uint32_t)(lo16(bmax) - lo16(bmin)) >> 2;
+            const uint32_t th = (uint32_t)(hi16(bmax) - hi16(bmin)) >> 2;
+            t.tw = tw;
+            ntiles = tw * th;
+            tview[wid][lane] = v;
+            talpha[wid][lane] = (rec.flags & kRecAlpha) ? r : 0xFFFFFFFFu;
+        }
+        uint32_t incl = ntiles;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t o = __shfl_up(incl, d);
+            if ((int)lane >= d) incl += o;
+        }
+        const uint32_t total = __shfl(incl, 63);
+        t.start = incl - ntiles;
+        if (lane < ck.count) tl[wid][lane] = t;
+        else tl[wid][lane].start = 0xFFFFFFFFu;
+        __builtin_amdgcn_wave_barrier();
+
+        const uint32_t sub = lane >> 4, px = lane & 3, py = (lane >> 2) & 3;
+        for (uint32_t base = 0; base < total; base += 4) {
+            const uint32_t j = base + sub;
+            bool match = false;
+            uint32_t k = 0, off = 0;
+            int32_t w1 = 0, w2 = 0;
+            if (j < total) {
+#pragma unroll
+                for (uint32_t step = 32; step > 0; step >>= 1) {
+                    if (tl[wid][k + step].start <= j) k += step;
+                }
+                const TriLds& T = tl[wid][k];
+                const ViewParams& V = view_of(p, tview[wid][k]);
+                const uint32_t W = (uint32_t)V.width;
+                const uint32_t pt = j - T.start;
+                const uint32_t ty = pt / T.tw;
+                const uint32_t tx = pt - ty * T.tw;
+                const uint32_t x = (uint32_t)lo16(T.org) + tx * 4 + px;
+                const uint32_t y = (uint32_t)hi16(T.org) + ty * 4 + py;
+                const int32_t w0 = edge_at(T.e0, T.a12, T.b12, x, y);
+                w1 = edge_at(T.e1, T.a20, T.b20, x, y);
+                w2 = edge_at(T.e2, T.a01, T.b01, x, y);
+                const float depth = __builtin_fmaf((float)w1, T.z10, __builtin_fmaf((float)w2, T.z20, T.z0));
+                if (x < W && y < (uint32_t)V.height && (w0 | w1 | w2) >= 0 && depth > V.clear_depth) {
+                    off = pixel_offset(W, x, y);
+                    const unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
+                    match = V.keys[off] == key;
+                }
+            }
+            // uniform per 16-lane group: the group shares j, hence the record
+            if (!(__ballot(match) & groupMask)) continue;
+            const uint32_t ai = talpha[wid][k];
+            if (ai == 0xFFFFFFFFu) continue;
+            const TriLds& T = tl[wid][k];
+            const ViewParams& V = view_of(p, tview[wid][k]);
+            const AlphaRecord& A = p.alpha[ai];
+            const DevTexture& tex = V.textures[A.tex & 0xFFFFFFu];
+            const TileUV uv = tile_uv(AT, A, ((T.keylo >> 7) & 1u) == 0, tex, w1, w2, lane);
+            const uint32_t baseColor = sample_level(tex, uv.U, uv.V, uv.mip, !uv.minify);
+            uint32_t ch2 = 0;
+            if (tex.num_layers >= 2) {
+                const uint32_t nmr = sample_level(tex, uv.U, uv.V, uv.mip, !uv.minify, 1);
+                if (match) {
+                    // the fragment's triangle: meshlet and primitive from its rank
+                    const uint32_t rank = ~T.keylo;
+                    const uint32_t prim = ((rank >> 8) & 7u) * 16u + ((rank >> 3) & 15u);
+                    const uint32_t mi = V.meshlet_offset + ((rank >> 11) - V.seq_base);
+                    const uint8_t* I = V.m.indices + (size_t)mi * 384;
+                    float nv[3][3], tv[3][3];
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        const uint32_t nt = V.m.normal_tangents[(size_t)mi * 64 + (I[q * 128 + prim] & 63u)];
+                        unpack_normal(AT, nt, nv[q]);
+                        unpack_normal(AT, nt >> 16, tv[q]);
+                    }
+                    const uint32_t handed = (uint32_t)((V.m.tangent_handedness[mi] >> (I[prim] & 63u)) & 1u) << 31;
+                    float N[3], Tw[3];
+                    lerp_to_world(AT, nv, uv.b, V.o2w, N);
+                    lerp_to_world(AT, tv, uv.b, V.o2w, Tw);
+                    normal_map(AT, N, Tw, handed, nmr);
+                    ch2 = gbuffer_pack(AT, N, nmr);
+                }
+            }
+            if (match) {
+                V.aux[off] = baseColor;
+                V.aux[(size_t)V.width * V.height + off] = ch2;
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -1510,6 +1687,20 @@ __global__ void k_decode(const unsigned long long* keys, int width, int height, 
     if (depth) depth[o] = bitsf(db);
 }
 
+__global__ void k_detile32(const uint32_t* src, int width, uint32_t n, uint32_t* out, int linear) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t o = i;
+    if (linear) {
+        const uint32_t W = (uint32_t)width;
+        const uint32_t tileRow = i / (W * 4u);
+        const uint32_t rem = i - tileRow * W * 4u;
+        const uint32_t tx = rem >> 4, q = rem & 15u;
+        o = (tileRow * 4 + (q >> 2)) * W + tx * 4 + (q & 3u);
+    }
+    out[o] = src[i];
+}
+
 __global__ void k_fill32(uint32_t* buf, uint32_t n, uint32_t value) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) buf[i] = value;
 }
@@ -1570,6 +1761,7 @@ void warm_launch_shapes() {
     (void)resident_blocks<k_raster_direct<false>, 256>();
     (void)resident_blocks<k_raster_direct<true>, 256>();
     (void)resident_blocks<k_raster_overdraw, 256>();
+    (void)resident_blocks<k_gbuffer, 256>();
     (void)resident_blocks<k_bin_scatter<false>, 256>();
     (void)resident_blocks<k_bin_scatter<true>, 256>();
 }
@@ -1603,6 +1795,27 @@ hipError_t launch_draw_overdraw(const DrawParams& p, hipStream_t s, Profiler* pr
                            s, p);
     }
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, s, p.scratch);
+    return hipGetLastError();
+}
+
+hipError_t launch_draw_deferred(const DrawParams& p, hipStream_t s, Profiler* prof) {
+    launch_front(p, s, prof, true);
+    const uint32_t want = max((2 * p.total_meshlets + 3) / 4 + 16, 64u);
+    {
+        Profiler::Scope sc(prof, "raster_direct", s);
+        hipLaunchKernelGGL(k_raster_direct<true>, dim3(min(want, resident_blocks<k_raster_direct<true>, 256>())),
+                           dim3(256), 0, s, p, 0);
+    }
+    {
+        Profiler::Scope sc(prof, "gbuffer", s);
+        hipLaunchKernelGGL(k_gbuffer, dim3(min(want, resident_blocks<k_gbuffer, 256>())), dim3(256), 0, s, p);
+    }
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, s, p.scratch);
+    return hipGetLastError();
+}
+
+hipError_t launch_detile32(const uint32_t* src, int width, uint32_t n, uint32_t* out, int linear, hipStream_t s) {
+    hipLaunchKernelGGL(k_detile32, dim3((n + 255) / 256), dim3(256), 0, s, src, width, n, out, linear);
     return hipGetLastError();
 }
 

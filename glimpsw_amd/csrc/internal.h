@@ -165,6 +165,7 @@ struct ViewParams {
     float clear_depth;
     unsigned long long* keys;      // the view's vis-buffer, 4x4 tiles
     uint32_t* aux;                 // overdraw: the depth layer; deferred: colour layer, then normal/MR layer
+    float o2w[9];                  // ShadingContext::ObjectToWorldMat (deferred normals), mat3 column-major
     float clip[16];
     float planes[20];
     Occlusion occ;                 // with use_cull: HiZ test of the frustum survivors
@@ -189,6 +190,7 @@ struct DrawParams {
     uint2* visible_list;           // (global meshlet number, view) of each meshlet that passed the cull
     TriRecord* records;            // [total_meshlets * 128 + clip_record_capacity]
     AlphaRecord* alpha;            // same indexing as records; null when no view has alpha-tested materials
+    uint32_t deferred;             // FS_EncodeGBuffer: every textured material runs the alpha test
     uint32_t* meshlet_counts;      // [total_meshlets] records kept per meshlet
     uint32_t clip_record_base;     // = total_meshlets * 128
     uint32_t clip_record_capacity;
@@ -217,6 +219,12 @@ hipError_t launch_draw_overdraw(const DrawParams& p, hipStream_t s, Profiler* pr
 hipError_t launch_decode_overdraw(const unsigned long long* counts, const uint32_t* depth, uint32_t n,
                                   uint32_t clear_color, int width, uint32_t* words, uint32_t* depth_out, int linear,
                                   hipStream_t s);
+// The deferred pipeline (FS_EncodeGBuffer): the vis-buffer raster with the
+// alpha test on every textured material, then the G-buffer encode of each
+// pixel's winning fragment into V.aux (colour layer, then layer 2).
+hipError_t launch_draw_deferred(const DrawParams& p, hipStream_t s, Profiler* prof);
+// 4x4-tiled 32-bit layer -> host order (row-major when linear).
+hipError_t launch_detile32(const uint32_t* src, int width, uint32_t n, uint32_t* out, int linear, hipStream_t s);
 hipError_t launch_clear(unsigned long long* keys, uint32_t n, unsigned long long key, hipStream_t s);
 hipError_t launch_fill32(uint32_t* buf, uint32_t n, uint32_t value, hipStream_t s);
 hipError_t launch_decode(const unsigned long long* keys, int width, int height, unsigned long long clear_key,

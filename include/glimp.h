@@ -140,6 +140,12 @@ int glimp_framebuffer_keys(glimp_framebuffer* fb, void** keys, size_t* bytes);
 /* Pre-instantiated shader pipelines: the device counterpart of a dispatch
  * table built by GetDispatchTable<MeshFn, FragFn...> (Rasterizer.h:342-364). */
 #define GLIMP_PIPELINE_VISBUFFER 0u   /* ShadingContext::VisBufferShader (Shading.cpp:650-654) */
+#define GLIMP_PIPELINE_DEFERRED 1u    /* ShadingContext::DeferredShader: FS_EncodeGBuffer (Shading.cpp:344-407, 655).
+                                       * Layer 0 holds the base colour, layer 1 the depth, layer 2 the
+                                       * octahedral normal (2 x 10 bits) and the metallic / roughness bytes
+                                       * cut to 6 bits; a clear leaves layer 2 alone. Every drawn meshlet
+                                       * needs a material with a texture (GLIMP_E_UNSUPPORTED otherwise).
+                                       * Always rasterized with per-fragment atomics (GLIMP_DRAW_DIRECT). */
 #define GLIMP_PIPELINE_OVERDRAW 2u    /* ShadingContext::OverdrawShader: FS_Overdraw (Shading.cpp:333-342, 656).
                                        * Layer 0 holds per pixel the covered count (high 16 bits) and the
                                        * quad-helper count (low 16 bits) added to the clear colour
@@ -156,6 +162,7 @@ typedef struct glimp_draw_desc {
     uint32_t bin_size;                 /* screen bin side in pixels: 8, 16, 32 or 64; 0 = default */
     uint32_t bin_capacity;             /* records per bin before spilling, 0 = default */
     const struct glimp_occlusion* occlusion; /* with frustum_planes: HiZ test of the survivors, or NULL */
+    float object_to_world[9];          /* ShadingContext::ObjectToWorldMat, mat3 column-major (deferred) */
 } glimp_draw_desc;
 
 /* Rasterizer::DrawMeshlets(fb, count, {VisBufferShader, &ctx})

@@ -146,6 +146,7 @@ private:
 struct ShadeMeshlet {};                            // ShadingContext::ShadeMeshlet (Shading.cpp:281-307)
 template <bool AlphaTest> struct FS_EncodeSurfaceId {};  // Shading.cpp:309-331
 struct FS_Overdraw {};                             // Shading.cpp:333-342
+struct FS_EncodeGBuffer {};                        // Shading.cpp:344-407
 
 struct ShaderDispatchTable {
     uint32_t pipeline;
@@ -168,6 +169,11 @@ struct PipelineOf<ShadeMeshlet, FS_EncodeSurfaceId<false>> {
     static constexpr uint32_t value = GLIMP_PIPELINE_VISBUFFER;
 };
 
+// ShadingContext::DeferredShader (Shading.cpp:655).
+template <>
+struct PipelineOf<ShadeMeshlet, FS_EncodeGBuffer> {
+    static constexpr uint32_t value = GLIMP_PIPELINE_DEFERRED;
+};
 // ShadingContext::OverdrawShader (Shading.cpp:656).
 template <>
 struct PipelineOf<ShadeMeshlet, FS_Overdraw> {
@@ -273,6 +279,7 @@ public:
         d.frustum_planes = sc.Planes();
         d.flags = (EnableClipping ? 0u : GLIMP_DRAW_NO_CLIPPING) | (EnableGuardband ? 0u : GLIMP_DRAW_NO_GUARDBAND);
         d.pipeline = shader.Dispatch.pipeline;
+        std::memcpy(d.object_to_world, sc.ObjectToWorldMat, sizeof(d.object_to_world));
         d.bin_size = BinSize;
         d.bin_capacity = BinCapacity;
         check(glimp_draw_meshlets(ctx_->get(), fb.get(), sc.GetScene()->get(), &d, stream), "glimp_draw_meshlets");

@@ -48,7 +48,7 @@ class Material(C.Structure):
 class ShadingCtx(C.Structure):
     _fields_ = [("meshlets", C.POINTER(Meshlets)), ("materials", C.POINTER(Material)),
                 ("textures", C.POINTER(Texture)), ("meshlet_offset", C.c_uint32),
-                ("cull_bitmap", u16p), ("object_to_clip", C.c_float * 16)]
+                ("cull_bitmap", u16p), ("object_to_clip", C.c_float * 16), ("object_to_world", C.c_float * 9)]
 
 
 class Framebuffer(C.Structure):
@@ -94,8 +94,15 @@ def _ptr(a: np.ndarray, t):
     return a.ctypes.data_as(t)
 
 
+def _stale(target: str, *sources: str) -> bool:
+    if not os.path.exists(target):
+        return True
+    t = os.path.getmtime(target)
+    return any(os.path.getmtime(os.path.join(HERE, s)) > t for s in sources)
+
+
 def build(force: bool = False) -> None:
-    if force or not os.path.exists(LIB_PATH):
+    if force or _stale(LIB_PATH, "oracle.cpp", "oracle.h"):
         subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
 
 
@@ -124,7 +131,7 @@ def lib():
 
 def probe_lib():
     """The fast-math probe library (clang, reference flags); needs AVX-512."""
-    if not os.path.exists(PROBE_PATH):
+    if _stale(PROBE_PATH, os.path.join("probes", "fastmath_probe.cpp")):
         subprocess.run(["make", "-s", "-C", HERE, "probe"], check=True)
     return C.CDLL(PROBE_PATH)
 
@@ -148,8 +155,11 @@ class SceneBinding:
         for i, (tex, ds, ac) in enumerate(s.materials):
             self.materials[i] = Material(tex, int(ds), ac)
 
-    def ctx(self, object_to_clip, meshlet_offset=0, cull_bitmap=None):
+    def ctx(self, object_to_clip, meshlet_offset=0, cull_bitmap=None, object_to_world=None):
         c = ShadingCtx()
+        o2w = np.eye(3).reshape(9) if object_to_world is None else np.asarray(object_to_world).reshape(9)
+        for i in range(9):
+            c.object_to_world[i] = float(o2w[i])
         c.meshlets = C.pointer(self.meshlets)
         c.materials = C.cast(self.materials, C.POINTER(Material))
         c.textures = C.cast(self.textures, C.POINTER(Texture))
@@ -201,11 +211,13 @@ class OracleFramebuffer:
 
 def draw(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip, count: int, meshlet_offset: int = 0,
          cull_bitmap=None, clipping: bool = True, guardband: bool = True, coverage: bool = False,
-         pipeline: int = PIPELINE_VISBUFFER):
+         pipeline: int = PIPELINE_VISBUFFER, object_to_world=None):
     cov = np.zeros((fb.height, fb.width), dtype=np.uint32) if coverage else None
     opts = RasterOpts(int(clipping), int(guardband), pipeline, 0, _ptr(cov, u32p) if cov is not None else u32p())
     ctr = Counters()
-    ctx = binding.ctx(object_to_clip, meshlet_offset, cull_bitmap)
+    if pipeline == PIPELINE_DEFERRED:
+        assert fb.fb.num_layers >= 3, "the deferred pipeline writes three layers"
+    ctx = binding.ctx(object_to_clip, meshlet_offset, cull_bitmap, object_to_world)
     lib().orc_draw_meshlets(C.byref(fb.fb), C.c_uint32(count), C.byref(ctx), C.byref(opts), C.byref(ctr))
     return ctr.as_dict(), cov
 

@@ -343,6 +343,12 @@ struct FragmentCtx {
     uint32_t cutoff;
     bool clipped;
     float ru[3], rv[3];
+    // FS_EncodeGBuffer: whether the meshlet has a material, the triangle's
+    // vertex normal/tangent words and the handedness bit of its first vertex
+    bool material;
+    uint32_t nt[3];
+    uint32_t handed;
+    const float* object_to_world;
 };
 
 // Perspective-correct barycentrics of one fragment (Rasterizer.h:293-319):
@@ -363,7 +369,7 @@ void fragment_bary(float u, float v, float W0, float W0S, float W1S, float W2S, 
 // tile, mip level less the 8 fraction bits, one filter per tile) and
 // color >= cutoff << 24. Returns the passing pixels.
 void alpha_uv_tile(const FragmentCtx& fc, float slu, float slv, const float* bu, const float* bv, float* U,
-                   float* V, int32_t* mip) {
+                   float* V, int32_t* mip, float (*bary)[16] = nullptr) {
     float su[16], sv[16];
     for (int l = 0; l < 16; l++) {
         float u = bu[l], v = bv[l];
@@ -374,6 +380,11 @@ void alpha_uv_tile(const FragmentCtx& fc, float slu, float slv, const float* bu,
             v = cv;
         }
         float b0 = 1.0f - (u + v);   // `1 - u - v`, reassociated by the fast-math build
+        if (bary) {
+            bary[0][l] = b0;
+            bary[1][l] = u;
+            bary[2][l] = v;
+        }
         U[l] = fmaf(fc.uv[0][0], b0, fmaf(fc.uv[1][0], u, fc.uv[2][0] * v));
         V[l] = fmaf(fc.uv[0][1], b0, fmaf(fc.uv[1][1], u, fc.uv[2][1] * v));
         su[l] = U[l] * slu;
@@ -403,6 +414,11 @@ uint16_t alpha_test_tile(const FragmentCtx& fc, const float* bu, const float* bv
         if (color[l] >= (fc.cutoff << 24)) pass |= (uint16_t)(1u << l);
     return pass;
 }
+
+// FS_EncodeGBuffer (Shading.cpp:344-407) on one 4x4 tile after the depth
+// test; defined with the surface helpers below.
+void gbuffer_tile(DrawState& st, const EdgeVars& e, int l, int tx, int ty, uint16_t pass, const int32_t* E1,
+                  const int32_t* E2, const float* depth, const FragmentCtx& fc);
 
 // FS_Overdraw (Shading.cpp:333-342) on one 4x4 tile with at least one
 // covered pixel: every pixel's counter gains 1 in its high 16 bits when
@@ -470,6 +486,10 @@ void draw_triangle(DrawState& st, const EdgeVars& e, int l, uint32_t bmin, uint3
                 if (depth[p] > depthBuf[pixel_offset(W, (uint32_t)x, (uint32_t)y)]) pass |= (uint16_t)(1u << p);
             }
             if (!pass) continue;
+            if (st.opts && st.opts->pipeline == ORC_PIPELINE_DEFERRED) {
+                gbuffer_tile(st, e, l, tx, ty, pass, E1, E2, depth, fc);
+                continue;
+            }
             if (fc.alpha_test) {
                 float bu[16], bv[16];
                 for (int p = 0; p < 16; p++)
@@ -813,6 +833,85 @@ void normal_map(const float n[3], const float t[3], uint32_t handed, uint32_t nm
     for (int k = 0; k < 3; k++) r[k] = fmaf(sz, n[k], fmaf(sy, b[k], sx * t[k]));
     const float inv = rsqrt14(fmaf(r[0], r[0], fmaf(r[1], r[1], r[2] * r[2])));
     for (int k = 0; k < 3; k++) out[k] = r[k] * inv;
+}
+
+// cvttps2udq: truncation; NaN and results outside [0, 2^32) give 0xFFFFFFFF.
+uint32_t cvt_trunc_u32(float x) {
+    if (!(x > -1.0f && x < 4294967296.0f)) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+
+// The layer-2 word of FS_EncodeGBuffer (Shading.cpp:397-401): MapOctahedron
+// (Texture.h:282-288) of the remapped normal, each coordinate as
+// conv<uint32_t>(oct * 1023 + 0.5) in 10 bits, then the NMR texel's
+// metallic and roughness bytes cut to 6 bits. The fast-math build folds
+// (c * 0.5 + 0.5) * 1023 + 0.5 into fma(c, 511.5, 512), sums the absolute
+// values left to right, takes 1/sum from VRCP14PS alone and evaluates
+// max(-z * w, 0) as vmaxps (zero unless positive).
+uint32_t gbuffer_pack(const float n[3], uint32_t nmr) {
+    const float w = rcp14((fabsf(n[0]) + fabsf(n[1])) + fabsf(n[2]));
+    const float mz = -n[2] * w;
+    const float t = mz > 0.0f ? mz : 0.0f;
+    const float u = fmaf(n[0], w, mulsign(t, n[0]));
+    const float v = fmaf(n[1], w, mulsign(t, n[1]));
+    uint32_t r = cvt_trunc_u32(fmaf(u, 511.5f, 512.0f));
+    r |= cvt_trunc_u32(fmaf(v, 511.5f, 512.0f)) << 10;
+    r |= ((nmr >> 18) & 0x3Fu) << 20;
+    r |= ((nmr >> 26) & 0x3Fu) << 26;
+    return r;
+}
+
+void gbuffer_tile(DrawState& st, const EdgeVars& e, int l, int tx, int ty, uint16_t pass, const int32_t* E1,
+                  const int32_t* E2, const float* depth, const FragmentCtx& fc) {
+    orc_framebuffer* fb = st.fb;
+    const uint32_t W = fb->width;
+    float* depthBuf = (float*)(fb->data + fb->layer_stride);
+    uint32_t* color = fb->data;
+    uint32_t* ch2Buf = fb->data + 2 * (size_t)fb->layer_stride;
+    uint32_t base[16] = {}, ch2[16] = {};
+    if (fc.material && fc.tex) {
+        // BaryLerp of the vertex UVs and SampleImplicitLod, as the alpha test
+        float bu[16], bv[16], U[16], V[16], bary[3][16];
+        int32_t mip[16];
+        for (int p = 0; p < 16; p++)
+            fragment_bary((float)E1[p], (float)E2[p], e.W0[l], e.W0S[l], e.W1S[l], e.W2S[l], bu[p], bv[p]);
+        alpha_uv_tile(fc, fc.tex->scale_lerp_u, fc.tex->scale_lerp_v, bu, bv, U, V, mip, bary);
+        const bool all[16] = { true, true, true, true, true, true, true, true,
+                               true, true, true, true, true, true, true, true };
+        sample_level16(fc.tex, U, V, 0, mip, 0, true, false, all, base);
+        for (int p = 0; p < 16; p++)
+            if (base[p] < (fc.cutoff << 24)) pass &= (uint16_t)~(1u << p);
+        if (!pass) return;
+        if (fc.tex->num_layers >= 2) {
+            uint32_t nmr[16];
+            sample_level16(fc.tex, U, V, 1, mip, 0, true, false, all, nmr);
+            float nv[3][3], tv[3][3];
+            for (int k = 0; k < 3; k++) {
+                unpack_normal(fc.nt[k], nv[k]);
+                unpack_tangent(fc.nt[k], tv[k]);
+            }
+            for (int p = 0; p < 16; p++) {
+                if (!((pass >> p) & 1)) continue;
+                const float b[3] = { bary[0][p], bary[1][p], bary[2][p] };
+                float N[3], T[3], R[3];
+                lerp_to_world(nv, b, fc.object_to_world, N);
+                lerp_to_world(tv, b, fc.object_to_world, T);
+                normal_map(N, T, fc.handed, nmr[p], R);
+                ch2[p] = gbuffer_pack(R, nmr[p]);
+            }
+        }
+    }
+    for (int p = 0; p < 16; p++) {
+        if (!((pass >> p) & 1)) continue;
+        const uint32_t off = pixel_offset(W, (uint32_t)(tx + (p & 3)), (uint32_t)(ty + (p >> 2)));
+        depthBuf[off] = depth[p];
+        // a meshlet without a material writes depth only (Shading.cpp:352-355)
+        if (fc.material && fc.tex) {
+            color[off] = base[p];
+            ch2Buf[off] = ch2[p];
+        }
+        st.ctr->fragments_passed++;
+    }
 }
 
 void resolve_surface_tile(const orc_shading_ctx& ctx, const uint32_t* ids, const bool* active,
@@ -1205,15 +1304,25 @@ void orc_draw_meshlets(orc_framebuffer* fb, uint32_t count, const orc_shading_ct
         // fragment shader inputs (FS_EncodeSurfaceId<AlphaTest>)
         FragmentCtx fc = {};
         const uint32_t matId = ctx->meshlets->material_id[meshletId];
-        if (mesh.FragmentShaderId == 1 && matId != UINT32_MAX && ctx->materials &&
-            ctx->materials[matId].texture >= 0) {
+        const bool deferred = opts && opts->pipeline == ORC_PIPELINE_DEFERRED;
+        fc.material = matId != UINT32_MAX;
+        fc.object_to_world = ctx->object_to_world;
+        if (deferred && matId != UINT32_MAX && ctx->materials && ctx->materials[matId].texture >= 0) {
+            // FS_EncodeGBuffer samples the base colour of every material
+            // (Shading.cpp:357-366); a material without a texture would be
+            // sampled through a null pointer there and is drawn here with
+            // depth only
+            fc.tex = &ctx->textures[ctx->materials[matId].texture];
+            fc.cutoff = ctx->materials[matId].alpha_cutoff;
+        } else if (!deferred && mesh.FragmentShaderId == 1 && matId != UINT32_MAX && ctx->materials &&
+                   ctx->materials[matId].texture >= 0) {
             fc.alpha_test = true;
             fc.tex = &ctx->textures[ctx->materials[matId].texture];
             fc.cutoff = ctx->materials[matId].alpha_cutoff;
         }
         auto fragment_ctx = [&](uint32_t prim) {
             FragmentCtx f = fc;
-            if (f.alpha_test) {
+            if (f.alpha_test || f.tex) {
                 for (int k = 0; k < 3; k++) {
                     // indices of a well-formed meshlet are below 64; the mask
                     // keeps a malformed one inside the meshlet's 64 slots
@@ -1221,7 +1330,10 @@ void orc_draw_meshlets(orc_framebuffer* fb, uint32_t count, const orc_shading_ct
                     uint32_t tc = ctx->meshlets->texcoords[(size_t)meshletId * 64 + vi];
                     f.uv[k][0] = half_to_float((uint16_t)(tc & 0xFFFF));
                     f.uv[k][1] = half_to_float((uint16_t)(tc >> 16));
+                    f.nt[k] = ctx->meshlets->normal_tangents[(size_t)meshletId * 64 + vi];
                 }
+                const uint32_t v0 = mesh.Indices[0][prim] & 63u;
+                f.handed = (uint32_t)((ctx->meshlets->tangent_handedness[meshletId] >> v0) & 1u) << 31;
             }
             return f;
         };
@@ -1619,6 +1731,13 @@ void orc_resolve_debug(orc_framebuffer* fb, const orc_shading_ctx* ctx, uint32_t
         }
     }
     memcpy(fb->data, out.data(), (size_t)W * H * 4);
+}
+
+void orc_gbuffer_pack16(const float* n, const uint32_t* nmr, uint32_t* out) {
+    for (int l = 0; l < 16; l++) {
+        const float v[3] = { n[l], n[16 + l], n[32 + l] };
+        out[l] = gbuffer_pack(v, nmr[l]);
+    }
 }
 
 void orc_overdraw_colour16(const uint32_t* data, int32_t quad, float* out) {

@@ -57,6 +57,7 @@ typedef struct orc_shading_ctx {
     uint32_t meshlet_offset;
     const uint16_t* cull_bitmap;      /* 1 bit per meshlet, 16 per word; NULL = all visible */
     float object_to_clip[16];         /* column-major, m[c*4+r] */
+    float object_to_world[9];         /* mat3 column-major (the deferred pipeline's normals) */
 } orc_shading_ctx;
 
 typedef struct orc_framebuffer {
@@ -153,6 +154,9 @@ void orc_resolve_albedo(orc_framebuffer* fb, const orc_shading_ctx* ctx);
 #define ORC_DEBUG_OVERDRAW_PIXEL 5u   /* framebuffer drawn with ORC_PIPELINE_OVERDRAW */
 #define ORC_DEBUG_OVERDRAW_QUAD 6u
 void orc_resolve_debug(orc_framebuffer* fb, const orc_shading_ctx* ctx, uint32_t layer, const float* object_to_world);
+/* FS_EncodeGBuffer's layer-2 word for 16 lanes: n = remapped world normal
+ * [3][16], nmr = layer-1 texels. */
+void orc_gbuffer_pack16(const float* n, const uint32_t* nmr, uint32_t* out);
 /* The overdraw layers' colour for 16 counter words: out = rgb [3][16]. */
 void orc_overdraw_colour16(const uint32_t* data, int32_t quad, float* out);
 /* Resolve(): lit, tonemapped RGBA8 into layer 0. */

@@ -389,3 +389,28 @@ extern "C" void probe_overdraw_colour(const uint32_t* data, int quad, float* out
         }
     }
 }
+
+// G-buffer packing of FS_EncodeGBuffer (Shading.cpp:397-401): MapOctahedron
+// (Texture.h:282-288) of the remapped normal, two 10-bit channels
+// conv<uint32_t>(oct * 1023 + 0.5) (truncating), and the NMR texel's
+// metallic / roughness bytes cut to 6 bits. n: [3][16]; nmr: [16]; out: [16].
+extern "C" void probe_gbuffer_pack(const float* n, const int32_t* nmr, int32_t* out) {
+    typedef uint32_t vu __attribute__((ext_vector_type(16)));
+    vf x = ld(n), y = ld(n + 16), z = ld(n + 32);
+    vf w = (vf)_mm512_rcp14_ps((__m512)(__builtin_elementwise_abs(x) + __builtin_elementwise_abs(y) +
+                                        __builtin_elementwise_abs(z)));
+    vf t = __builtin_elementwise_maxnum(-z * w, (vf)0.0f);
+    vf sx = (vf)((vu)t ^ ((vu)x & 0x80000000u));
+    vf sy = (vf)((vu)t ^ ((vu)y & 0x80000000u));
+    vf u = vfma(x, w, sx);
+    vf v = vfma(y, w, sy);
+    u = u * 0.5f + 0.5f;
+    v = v * 0.5f + 0.5f;
+    vu p;
+    memcpy(&p, nmr, 64);
+    vu r = __builtin_convertvector(u * 1023.0f + 0.5f, vu);
+    r |= __builtin_convertvector(v * 1023.0f + 0.5f, vu) << 10;
+    r |= (p >> 18 & 0x3F) << 20;
+    r |= (p >> 26 & 0x3F) << 26;
+    memcpy(out, &r, 64);
+}

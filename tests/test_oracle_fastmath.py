@@ -364,3 +364,25 @@ def test_overdraw_colormap(probe, quad):
         L.orc_overdraw_colour16(words[i:i + 16].ctypes.data_as(C.POINTER(C.c_uint32)), C.c_int32(quad), P(o))
         got[:, i:i + 16] = o.reshape(3, 16)
     assert (_bits(got) == _bits(want)).all()
+
+
+def test_gbuffer_pack(probe):
+    """MapOctahedron and the packing of FS_EncodeGBuffer's layer 2
+    (Shading.cpp:397-401, Texture.h:282-288) on random unit-ish normals,
+    axis-aligned and zero-component normals and random NMR texels."""
+    rng = np.random.default_rng(11)
+    n = rng.normal(size=(4096, 3)).astype(np.float32)
+    n /= np.linalg.norm(n, axis=1, keepdims=True).astype(np.float32)
+    n[:64] = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [-1, 0, 0], [0, -1, 0], [0, 0, -1], [0.6, 0, -0.8],
+                       [0, -0.6, -0.8]] * 8, np.float32)
+    n[64:128] *= 1.0003                      # slightly long normals push oct past 1
+    nmr = rng.integers(0, 2 ** 32, 4096, dtype=np.uint64).astype(np.uint32)
+    L = oracle.lib()
+    u32p = C.POINTER(C.c_uint32)
+    for i in range(0, 4096, 16):
+        blk = np.ascontiguousarray(n[i:i + 16].T)
+        want = np.zeros(16, np.int32)
+        probe.probe_gbuffer_pack(P(blk), P(nmr[i:i + 16].view(np.int32).copy()), P(want))
+        got = np.zeros(16, np.uint32)
+        L.orc_gbuffer_pack16(P(blk), nmr[i:i + 16].ctypes.data_as(u32p), got.ctypes.data_as(u32p))
+        assert (got == want.view(np.uint32)).all(), i
