@@ -43,7 +43,7 @@ class MeshletsDesc(C.Structure):
 class TextureDesc(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("mip_levels", C.c_uint32),
                 ("num_layers", C.c_uint32), ("row_shift", C.c_uint32), ("layer_stride", C.c_uint32),
-                ("mip_offsets", C.c_uint32 * 16), ("texels", u32p)]
+                ("mip_offsets", C.c_uint32 * 16), ("texels", u32p), ("format", C.c_uint32)]
 
 
 class MaterialDesc(C.Structure):
@@ -115,7 +115,7 @@ class ResolveDesc(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("object_to_clip", C.c_float * 16), ("object_to_world", C.c_float * 9),
                 ("inv_screen_proj", C.c_float * 16), ("view_pos", C.c_float * 3), ("exposure", C.c_float),
                 ("lights", C.POINTER(LightDesc)), ("num_lights", C.c_uint32), ("flags", C.c_uint32),
-                ("world_to_clip", C.c_float * 16)]
+                ("world_to_clip", C.c_float * 16), ("skybox", C.c_void_p)]
 
 
 GLIMP_RESOLVE_ALBEDO = 0
@@ -302,6 +302,7 @@ class Texture:
             d.mip_offsets[i] = int(tex.mip_offsets[i])
         data = np.ascontiguousarray(tex.data, dtype=np.uint32)
         d.texels = _ptr(data, u32p)
+        d.format = int(getattr(tex, "format", 0))
         h = C.c_void_p()
         _check(ctx.lib.glimp_texture_create(ctx.handle, C.byref(d), C.byref(h)), "glimp_texture_create")
         self.handle = h
@@ -406,12 +407,15 @@ class Occlusion:
         self.desc = d
 
 
-def resolve_desc(object_to_clip, mode: int = GLIMP_RESOLVE_ALBEDO, lighting=None):
+def resolve_desc(object_to_clip, mode: int = GLIMP_RESOLVE_ALBEDO, lighting=None, skybox: "Texture | None" = None):
     """A glimp_resolve_desc (and the light array it points to, to keep
     alive): ``lighting`` supplies the lit terms and object_to_world; its
-    light sprites need ``world_to_clip``."""
+    light sprites need ``world_to_clip``; ``skybox`` is a device
+    R11G11B10f texture for the uncovered pixels of the lit resolve."""
     d = ResolveDesc()
     d.mode = mode
+    if skybox is not None:
+        d.skybox = skybox.handle
     for i in range(16):
         d.object_to_clip[i] = float(object_to_clip[i])
     keep = None
@@ -494,11 +498,12 @@ class Framebuffer:
                                                          _stream_ptr(stream)), "glimp_framebuffer_read_layer")
         return out
 
-    def resolve(self, scene: Scene, object_to_clip, mode: int = GLIMP_RESOLVE_ALBEDO, lighting=None, stream=None):
+    def resolve(self, scene: Scene, object_to_clip, mode: int = GLIMP_RESOLVE_ALBEDO, lighting=None, stream=None,
+                skybox: "Texture | None" = None):
         """ShadingContext::Resolve (lit) or ResolveDebug (the other modes);
         ``lighting`` supplies object_to_world for the Normals and
         MetallicRoughness layers as well."""
-        d, keep = resolve_desc(object_to_clip, mode, lighting)
+        d, keep = resolve_desc(object_to_clip, mode, lighting, skybox)
         _check(self.ctx.lib.glimp_resolve(self.ctx.handle, self.handle, scene.handle, C.byref(d), _stream_ptr(stream)),
                "glimp_resolve")
 

@@ -120,6 +120,7 @@ struct glimp_texture {
     glimp_context* ctx = nullptr;
     DevTexture hdr = {};
     uint32_t* data = nullptr;
+    uint32_t format = GLIMP_TEXEL_RGBA8;
 };
 
 struct glimp_scene {
@@ -283,10 +284,13 @@ int glimp_texture_create(glimp_context* c, const glimp_texture_desc* d, glimp_te
     auto pow2 = [](uint32_t v) { return v && !(v & (v - 1)); };
     if (!pow2(d->width) || !pow2(d->height) || d->mip_levels < 1 || d->mip_levels > 16 || d->num_layers < 1)
         return fail(GLIMP_E_INVALID, "glimp_texture_create: sizes must be powers of two, 1-16 mips");
+    if (d->format != GLIMP_TEXEL_RGBA8 && d->format != GLIMP_TEXEL_R11G11B10F)
+        return fail(GLIMP_E_INVALID, "glimp_texture_create: unknown texel format");
     HIP_TRY(hipSetDevice(c->device));
     glimp_texture* t = new (std::nothrow) glimp_texture();
     if (!t) return fail(GLIMP_E_NOMEM, "glimp_texture_create: out of host memory");
     t->ctx = c;
+    t->format = d->format;
     size_t n = (size_t)d->layer_stride * d->num_layers;
     // 64 texels of slack: bilinear reads the right neighbour of the last
     // column at offset +8 even when its weight is zero (Texture.h:527-531)
@@ -342,6 +346,8 @@ int glimp_scene_set_materials(glimp_scene* s, const glimp_material* mats, uint32
     std::vector<DevTexture> dt(num_textures);
     for (uint32_t i = 0; i < num_textures; i++) {
         if (!textures[i]) return fail(GLIMP_E_INVALID, "glimp_scene_set_materials: NULL texture");
+        if (textures[i]->format != GLIMP_TEXEL_RGBA8)
+            return fail(GLIMP_E_INVALID, "glimp_scene_set_materials: material textures are RGBA8");
         dt[i] = textures[i]->hdr;
     }
     if (s->materials) (void)hipFree(s->materials);
@@ -1263,6 +1269,8 @@ int build_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const
     if (d->mode > GLIMP_RESOLVE_TRIANGLE_ID) return fail(GLIMP_E_INVALID, who + ": unknown mode");
     if (d->num_lights > GLIMP_MAX_LIGHTS || (d->num_lights && !d->lights))
         return fail(GLIMP_E_LIMIT, who + ": at most GLIMP_MAX_LIGHTS lights");
+    if (d->mode == GLIMP_RESOLVE_LIT && d->skybox && d->skybox->format != GLIMP_TEXEL_R11G11B10F)
+        return fail(GLIMP_E_INVALID, who + ": the skybox must be an R11G11B10f texture");
     if (!f->color) HIP_TRY(dev_alloc(&f->color, (size_t)f->width * f->height));
     memset(&p, 0, sizeof(p));
     p.m = sc->m;
@@ -1302,6 +1310,10 @@ int build_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const
         if (d->mode == GLIMP_RESOLVE_LIT && (d->flags & GLIMP_RESOLVE_LIGHT_SPRITES) && d->lights[i].type != 0)
             place_sprite(d->world_to_clip, f->width, f->height, p.lights[i]);
         p.sprites |= p.lights[i].sprite;
+    }
+    if (d->mode == GLIMP_RESOLVE_LIT && d->skybox) {
+        p.sky = d->skybox->hdr;
+        p.has_sky = 1;
     }
     p.mode = (int)d->mode;
     p.out = f->color;

@@ -47,6 +47,8 @@ struct ResolveParams {
     uint32_t num_lights;
     uint32_t sprites;              // a light has a sprite
     ResolveLight lights[kMaxResolveLights];
+    DevTexture sky;                // ShadingContext::SkyboxTex (R11G11B10f), when has_sky
+    uint32_t has_sky;
     int mode;                      // GLIMP_RESOLVE_*: 0 BaseColor, 1 lit (Resolve), 2-5 other debug layers
     uint32_t* out;                 // RGBA8 per pixel
     int linear;

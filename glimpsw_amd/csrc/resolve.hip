@@ -301,9 +301,20 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveParams p) {
             }
         }
         float cc[3];
+        // sky: the view direction through the pixel at depth 1
+        // (Shading.cpp:663-667, 675-678)
+        float sky[3] = { 0, 0, 0 };
+        if (!active && inside && p.has_sky) {
+            float c4[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) c4[j] = xform(p.inv_screen_proj, j, (float)(int32_t)x, (float)(int32_t)y, 1.0f);
+            const float rw = frcp(T, c4[3]);
+            const float dir[3] = { c4[0] * rw - p.view_pos[0], c4[1] * rw - p.view_pos[1], c4[2] * rw - p.view_pos[2] };
+            sample_sky(T, p.sky, dir, sky);
+        }
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            float v = active ? (color[k] + base[k] * 0.05f) * p.exposure : 0.0f;
+            float v = active ? (color[k] + base[k] * 0.05f) * p.exposure : sky[k] * p.exposure;
             cc[k] = fdiv(T, v, v + 0.155f) * 1.019f;
         }
         outc = pack_rgba8(cc[0], cc[1], cc[2], 1.0f);

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "device_math.h"
 #include "internal.h"
 
 namespace glimp {
@@ -57,6 +58,51 @@ __device__ inline uint32_t sample_level(const DevTexture& tex, float u, float v,
     uint32_t rb2 = lerp16(d01 & 0x00FF00FFu, d11 & 0x00FF00FFu, fx);
     uint32_t ga2 = lerp16((d01 >> 8) & 0x00FF00FFu, (d11 >> 8) & 0x00FF00FFu, fx);
     return lerp16(rb1, rb2, fy) | (lerp16(ga1, ga2, fy) << 8);
+}
+
+// SkyboxTex->SampleOctLevel<EnvSampler>(dir, 1) (Shading.cpp:677,
+// Texture.h:467-480, EnvSampler Shading.cpp:11-15): MapOctahedron of the
+// view direction (1/(|x|+|y|+|z|) from VRCP14PS alone, max(-z * w, 0) zero
+// unless positive, c * 0.5 + 0.5 one FMA), ClampToEdge, level 1 (the mip
+// fraction is 0), the float bilinear filter of SampleLinear on
+// R11G11B10f texels (Texture.h:557-571, 157-182) with rowA + (rowB - rowA)
+// * fy evaluated as fma(fy, fma(fx, c11 - c01, c01 - rowA), rowA).
+__device__ inline void sample_sky(const ApproxTables& T, const DevTexture& tex, const float d[3], float out[3]) {
+    const float w = rcp14(T, (fabsf(d[0]) + fabsf(d[1])) + fabsf(d[2]));
+    const float mz = -d[2] * w;
+    const float t = mz > 0.0f ? mz : 0.0f;
+    const float sx = bitsf(fbits(t) ^ (fbits(d[0]) & 0x80000000u));
+    const float sy = bitsf(fbits(t) ^ (fbits(d[1]) & 0x80000000u));
+    const float u = __builtin_fmaf(__builtin_fmaf(d[0], w, sx), 0.5f, 0.5f);
+    const float v = __builtin_fmaf(__builtin_fmaf(d[1], w, sy), 0.5f, 0.5f);
+    int32_t ix = min(max(cvt_rne(u * tex.scale_lerp_u), 0), tex.mask_lerp_u);
+    int32_t iy = min(max(cvt_rne(v * tex.scale_lerp_v), 0), tex.mask_lerp_v);
+    const int32_t m = min(1, (int32_t)tex.mip_levels - 1);
+    const int32_t stride = (int32_t)tex.row_shift - m;
+    const int32_t offset = (int32_t)tex.mip_offsets[m];
+    ix >>= m;
+    iy >>= m;
+    const int32_t xf = max(ix - 127, 0), yf = max(iy - 127, 0);
+    const int32_t tx = xf >> 8, ty = yf >> 8;
+    const bool inX = ((tx + 1) << m) < (int32_t)tex.width;
+    const bool inY = ((ty + 1) << m) < (int32_t)tex.height;
+    const int32_t i00 = offset + (int32_t)texel_offset(tx, ty, stride);
+    const int32_t i01 = offset + (int32_t)texel_offset(tx, ty + (inY ? 1 : 0), stride);
+    const uint32_t q[4] = { tex.data[i00], tex.data[i00 + 8], tex.data[i01], tex.data[i01 + 8] };
+    const float fx = inX ? (float)(xf & 255) * (1.0f / 256) : 0.0f;
+    const float fy = (float)(yf & 255) * (1.0f / 256);
+    float c[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        c[k][0] = bitsf((((q[k] >> 21) << 17) & 0x0FFE0000u) + 0x38000000u);
+        c[k][1] = bitsf((((q[k] >> 10) << 17) & 0x0FFE0000u) + 0x38000000u);
+        c[k][2] = bitsf(((q[k] << 18) & 0x0FFC0000u) + 0x38000000u);
+    }
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) {
+        const float rowA = __builtin_fmaf(fx, c[1][ch] - c[0][ch], c[0][ch]);
+        out[ch] = __builtin_fmaf(fy, __builtin_fmaf(fx, c[3][ch] - c[2][ch], c[2][ch] - rowA), rowA);
+    }
 }
 
 }  // namespace glimp

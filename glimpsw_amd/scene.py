@@ -398,6 +398,7 @@ class Lighting:
     inv_screen_proj: np.ndarray    # (16,)
     world_to_clip: np.ndarray | None = None   # (16,) for the light sprites
     light_sprites: bool = True     # Resolve draws point/spot lights (Shading.cpp:689-727)
+    skybox: object = None          # host R11G11B10f octahedral texture (ShadingContext::SkyboxTex), or None
 
 
 def default_lighting(view: View) -> Lighting:

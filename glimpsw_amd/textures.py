@@ -31,6 +31,7 @@ class Texture:
     layer_stride: int
     mip_offsets: np.ndarray   # (16,) u32
     data: np.ndarray          # u32, layer_stride * num_layers + 64 slack texels
+    format: int = 0           # 0 RGBA8u, 1 R11G11B10f (HDR, the skybox)
 
     @property
     def scale_u(self) -> float:
@@ -99,8 +100,45 @@ def next_mip(level: np.ndarray) -> np.ndarray:
     return pack_f32(avg)
 
 
-def create(levels0: list, max_levels: int = 16) -> Texture:
-    """levels0: one (h, w) u32 RGBA8 array per layer (level 0)."""
+def pack_r11g11b10(c: np.ndarray) -> np.ndarray:
+    """R11G11B10f::Pack (Texture.h:135-157): clamp to [2^-15, the largest
+    value], then cut the float's bits (no rounding)."""
+    c = c.astype(np.float32)
+    lo = np.float32(1.0 / (1 << 15))
+    r = np.minimum(np.maximum(c[..., 0], lo), np.float32(130048.0)).view(np.uint32)
+    g = np.minimum(np.maximum(c[..., 1], lo), np.float32(130048.0)).view(np.uint32)
+    b = np.minimum(np.maximum(c[..., 2], lo), np.float32(129024.0)).view(np.uint32)
+    pr = ((r >> 17) & 0x3FFF) - 0x1C00
+    pg = ((g >> 17) & 0x3FFF) - 0x1C00
+    pb = ((b >> 18) & 0x1FFF) - 0x0E00
+    return ((pr << 21) | (pg << 10) | pb).astype(np.uint32)
+
+
+def unpack_r11g11b10(t: np.ndarray) -> np.ndarray:
+    """R11G11B10f::Unpack (Texture.h:157-182)."""
+    t = t.astype(np.uint32)
+    r = ((((t >> 21) << 17) & 0x0FFE0000) + 0x38000000).astype(np.uint32).view(np.float32)
+    g = ((((t >> 10) << 17) & 0x0FFE0000) + 0x38000000).astype(np.uint32).view(np.float32)
+    b = (((t << 18) & 0x0FFC0000) + 0x38000000).astype(np.uint32).view(np.float32)
+    return np.stack([r, g, b], axis=-1)
+
+
+def next_mip_hdr(level: np.ndarray) -> np.ndarray:
+    c = unpack_r11g11b10(level)
+    c00, c10 = c[0::2, 0::2], c[0::2, 1::2]
+    c01, c11 = c[1::2, 0::2], c[1::2, 1::2]
+    return pack_r11g11b10((((c00 + c10) + c01) + c11) * np.float32(0.25))
+
+
+def hdr_texture(rgb: np.ndarray, max_levels: int = 16) -> Texture:
+    """An R11G11B10f texture (HdrTexture2D, Texture.h:216) from float RGB
+    level 0 of shape (h, w, 3), mips by the same box filter."""
+    return create([pack_r11g11b10(rgb)], max_levels, fmt=1)
+
+
+def create(levels0: list, max_levels: int = 16, fmt: int = 0) -> Texture:
+    """levels0: one (h, w) u32 array per layer (level 0), RGBA8 texels
+    (fmt 0) or R11G11B10f texels (fmt 1)."""
     h, w = levels0[0].shape
     row_shift, offsets, layer_stride, nmips = layout(w, h, max_levels)
     data = np.zeros(layer_stride * len(levels0) + 64, dtype=np.uint32)
@@ -112,8 +150,8 @@ def create(levels0: list, max_levels: int = 16) -> Texture:
             off = tiled_offset(xs, ys, row_shift - m)
             data[layer * layer_stride + int(offsets[m]) + off] = cur
             if m + 1 < nmips:
-                cur = next_mip(cur)
-    return Texture(w, h, nmips, len(levels0), row_shift, layer_stride, offsets, data)
+                cur = next_mip(cur) if fmt == 0 else next_mip_hdr(cur)
+    return Texture(w, h, nmips, len(levels0), row_shift, layer_stride, offsets, data, fmt)
 
 
 def checker_texture(size: int = 256) -> Texture:
@@ -171,3 +209,39 @@ def nmr_texture(size: int, seed: int, period: int = 32) -> Texture:
     mr = np.repeat(np.repeat(mr, 8, axis=0), 8, axis=1)
     nmr = bx | (by << np.uint32(8)) | (mr << np.uint32(16))
     return create([albedo.astype(np.uint32), nmr.astype(np.uint32)])
+
+
+def unmap_octahedron64(u: np.ndarray, v: np.ndarray) -> np.ndarray:
+    """Float64 inverse of the octahedral map (Texture.h:289-296): (u, v) in
+    [0, 1] -> unit direction."""
+    x, y = u * 2 - 1, v * 2 - 1
+    z = 1 - np.abs(x) - np.abs(y)
+    t = np.maximum(-z, 0)
+    x = x - np.where(x < 0, -t, t)
+    y = y - np.where(y < 0, -t, t)
+    d = np.stack([x, y, z], axis=-1)
+    return d / np.linalg.norm(d, axis=-1, keepdims=True)
+
+
+def sky_rgb(d: np.ndarray) -> np.ndarray:
+    """A smooth HDR sky over directions d (..., 3): a blue gradient by
+    elevation, a warm horizon band and a bright sun (values up to ~40)."""
+    sun = np.array([0.3, 0.55, -0.78])
+    sun = sun / np.linalg.norm(sun)
+    up = np.clip(d[..., 1], -1, 1)
+    base = np.stack([0.35 + 0.25 * up, 0.55 + 0.3 * up, 0.9 + 0.6 * up], axis=-1).clip(0.02)
+    band = np.exp(-8 * up ** 2)[..., None] * np.array([1.6, 0.9, 0.4])
+    s = np.clip((d * sun).sum(axis=-1), -1, 1)
+    glow = (np.maximum(s, 0) ** 64)[..., None] * np.array([40.0, 34.0, 26.0])
+    return base + band + glow
+
+
+def sky_texture(size: int = 256) -> Texture:
+    """An octahedral HDR sky map (the layout LoadOctahedronFromPanoramaHDR
+    produces, ImageHelpers.cpp:73-105) with its mips, built from sky_rgb
+    with a +-30% checker of 2x2-texel cells so that every mip level looks
+    different from its neighbours."""
+    j, i = np.meshgrid(np.arange(size), np.arange(size), indexing="ij")
+    d = unmap_octahedron64((i + 0.5) / size, (j + 0.5) / size)
+    cells = 1.0 + 0.3 * ((((i >> 1) + (j >> 1)) & 1) * 2 - 1)
+    return hdr_texture((sky_rgb(d) * cells[..., None]).astype(np.float32))

@@ -84,11 +84,14 @@ void glimp_scene_destroy(glimp_scene* scene);
  * (Texture.h:599-636): TiledY8 texels, mip offsets in texels, layers
  * `layer_stride` texels apart. `texels` (host) holds
  * layer_stride * num_layers entries. */
+#define GLIMP_TEXEL_RGBA8 0u          /* pixfmt::RGBA8u (Texture.h:24-68): material textures */
+#define GLIMP_TEXEL_R11G11B10F 1u     /* pixfmt::R11G11B10f (Texture.h:135-185): HdrTexture2D, the skybox */
 typedef struct glimp_texture_desc {
     uint32_t width, height, mip_levels, num_layers;
     uint32_t row_shift, layer_stride;
     uint32_t mip_offsets[16];
     const uint32_t* texels;
+    uint32_t format;                   /* GLIMP_TEXEL_* */
 } glimp_texture_desc;
 int glimp_texture_create(glimp_context* ctx, const glimp_texture_desc* desc, glimp_texture** out);
 void glimp_texture_destroy(glimp_texture* tex);
@@ -354,6 +357,8 @@ typedef struct glimp_resolve_desc {
     uint32_t num_lights;               /* <= GLIMP_MAX_LIGHTS */
     uint32_t flags;                    /* GLIMP_RESOLVE_LIGHT_SPRITES (lit mode) */
     float world_to_clip[16];           /* ShadingContext::WorldToClipMat, column-major (light sprites) */
+    const glimp_texture* skybox;       /* ShadingContext::SkyboxTex (lit mode): an R11G11B10f octahedral map
+                                        * for the pixels nothing covers (Shading.cpp:675-678), or NULL */
 } glimp_resolve_desc;
 
 /* ShadingContext::Resolve / ResolveDebug (Shading.cpp:658-773):

@@ -198,6 +198,7 @@ struct Lighting {
     float view_pos[3] = {};
     float exposure = 1.0f;
     std::vector<glimp_light> lights;
+    const Texture2D* SkyboxTex = nullptr;   // ShadingContext::SkyboxTex: an R11G11B10f octahedral map
 };
 
 class Rasterizer;
@@ -303,6 +304,7 @@ inline void ShadingContext::Resolve(Rasterizer& rast, Framebuffer& fb, const Lig
     d.lights = L.lights.data();
     d.num_lights = (uint32_t)L.lights.size();
     d.flags = L.light_sprites ? GLIMP_RESOLVE_LIGHT_SPRITES : 0u;
+    d.skybox = L.SkyboxTex ? L.SkyboxTex->get() : nullptr;
     std::memcpy(d.world_to_clip, L.world_to_clip, sizeof(d.world_to_clip));
     check(glimp_resolve(rast.GetContext().get(), fb.get(), scene_->get(), &d, stream), "glimp_resolve");
     fb.resolved_ = true;

@@ -83,7 +83,7 @@ class ResolveParams(C.Structure):
     _fields_ = [("inv_screen_proj", C.c_float * 16), ("object_to_clip", C.c_float * 16),
                 ("object_to_world", C.c_float * 9), ("view_pos", C.c_float * 3), ("exposure", C.c_float),
                 ("lights", C.POINTER(Light)), ("num_lights", C.c_uint32), ("light_sprites", C.c_uint32),
-                ("world_to_clip", C.c_float * 16)]
+                ("world_to_clip", C.c_float * 16), ("skybox", C.POINTER(Texture))]
 
 
 _lib = None
@@ -331,6 +331,10 @@ def resolve_lit(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip, li
         rp.light_sprites = 1
         for i in range(16):
             rp.world_to_clip[i] = float(lighting.world_to_clip[i])
+    sky = getattr(lighting, "skybox", None)
+    if sky is not None:
+        sky_struct = texture_struct(sky)
+        rp.skybox = C.pointer(sky_struct)
     lib().orc_resolve_lit(C.byref(fb.fb), C.byref(ctx), C.byref(rp))
     return fb.pixels(0)
 

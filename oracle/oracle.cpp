@@ -1158,6 +1158,72 @@ uint32_t pack_rgba8(float r, float g, float b, float a) {
     return out;
 }
 
+// ---------------------------------------------------------------- skybox
+
+// MapOctahedron (Texture.h:282-288) followed by SampleLevel's scaling and
+// rounding (Texture.h:412-416): 1/(|x| + |y| + |z|) from VRCP14PS alone,
+// max(-z * w, 0) as vmaxps, (c * 0.5 + 0.5) contracted into one FMA.
+void sky_coords(const float d[3], float slu, float slv, int32_t& ix, int32_t& iy) {
+    const float w = rcp14((fabsf(d[0]) + fabsf(d[1])) + fabsf(d[2]));
+    const float mz = -d[2] * w;
+    const float t = mz > 0.0f ? mz : 0.0f;
+    const float u = fmaf(fmaf(d[0], w, mulsign(t, d[0])), 0.5f, 0.5f);
+    const float v = fmaf(fmaf(d[1], w, mulsign(t, d[1])), 0.5f, 0.5f);
+    ix = cvt_rne(u * slu);
+    iy = cvt_rne(v * slv);
+}
+
+// R11G11B10f::Unpack (Texture.h:157-182): exponent and mantissa moved into
+// a float by bit operations (no denormal, NaN or infinity handling).
+void unpack_r11g11b10(uint32_t x, float c[3]) {
+    c[0] = bitsf((((x >> 21) << 17) & 0x0FFE0000u) + 0x38000000u);
+    c[1] = bitsf((((x >> 10) << 17) & 0x0FFE0000u) + 0x38000000u);
+    c[2] = bitsf(((x << 18) & 0x0FFC0000u) + 0x38000000u);
+}
+
+// SampleLinear's float path (Texture.h:557-571) on four texels: 8-bit
+// fractions times 1/256, the x weight zeroed past the last column; the
+// optimised build evaluates rowA + (rowB - rowA) * fy as
+// fma(fy, fma(fx, c11 - c01, c01 - rowA), rowA).
+void env_bilinear(const uint32_t d[4], int32_t ixf, int32_t iyf, bool inx, float out[3]) {
+    float fx = (float)(ixf & 255) * (1.0f / 256);
+    const float fy = (float)(iyf & 255) * (1.0f / 256);
+    if (!inx) fx = 0.0f;
+    float c[4][3];
+    for (int k = 0; k < 4; k++) unpack_r11g11b10(d[k], c[k]);
+    for (int ch = 0; ch < 3; ch++) {
+        const float rowA = fmaf(fx, c[1][ch] - c[0][ch], c[0][ch]);
+        out[ch] = fmaf(fy, fmaf(fx, c[3][ch] - c[2][ch], c[2][ch] - rowA), rowA);
+    }
+}
+
+// SkyboxTex->SampleOctLevel<EnvSampler>(dir, 1) (Shading.cpp:677,
+// Texture.h:467-480): level 1 (the mip fraction is 0), ClampToEdge, linear
+// filter, layer 0 of an R11G11B10f texture in TiledY8 layout.
+void sky_sample(const orc_texture* t, const float dir[3], float out[3]) {
+    int32_t ix, iy;
+    sky_coords(dir, t->scale_lerp_u, t->scale_lerp_v, ix, iy);
+    ix = std::min(std::max(ix, 0), t->mask_lerp_u);
+    iy = std::min(std::max(iy, 0), t->mask_lerp_v);
+    const int32_t mip = std::min(1, (int32_t)t->mip_levels - 1);
+    int32_t stride = (int32_t)t->row_shift;
+    uint32_t offset = 0;
+    if (mip > 0) {
+        ix >>= mip;
+        iy >>= mip;
+        stride -= mip;
+        offset = t->mip_offsets[mip];
+    }
+    const int32_t ixf = std::max(ix - 127, 0), iyf = std::max(iy - 127, 0);
+    const int32_t tx = ixf >> 8, ty = iyf >> 8;
+    const bool inx = ((tx + 1) << mip) < (int32_t)t->width;
+    const bool iny = ((ty + 1) << mip) < (int32_t)t->height;
+    const uint32_t o0 = offset + texel_offset(tx, ty, stride);
+    const uint32_t o1 = offset + texel_offset(tx, ty + (iny ? 1 : 0), stride);
+    const uint32_t d[4] = { t->data[o0], t->data[o0 + 8], t->data[o1], t->data[o1 + 8] };
+    env_bilinear(d, ixf, iyf, inx, out);
+}
+
 // ---------------------------------------------------------------- light sprites
 
 // cvttps2dq: truncation, out of range (and NaN) to INT32_MIN.
@@ -1733,6 +1799,22 @@ void orc_resolve_debug(orc_framebuffer* fb, const orc_shading_ctx* ctx, uint32_t
     memcpy(fb->data, out.data(), (size_t)W * H * 4);
 }
 
+void orc_sky_coords16(const float* dir, float slu, float slv, int32_t* ix, int32_t* iy) {
+    for (int l = 0; l < 16; l++) {
+        const float d[3] = { dir[l], dir[16 + l], dir[32 + l] };
+        sky_coords(d, slu, slv, ix[l], iy[l]);
+    }
+}
+
+void orc_env_bilinear16(const uint32_t* d, const int32_t* ixf, const int32_t* iyf, const int32_t* inx, float* out) {
+    for (int l = 0; l < 16; l++) {
+        const uint32_t t[4] = { d[l], d[16 + l], d[32 + l], d[48 + l] };
+        float c[3];
+        env_bilinear(t, ixf[l], iyf[l], inx[l] != 0, c);
+        for (int k = 0; k < 3; k++) out[16 * k + l] = c[k];
+    }
+}
+
 void orc_gbuffer_pack16(const float* n, const uint32_t* nmr, uint32_t* out) {
     for (int l = 0; l < 16; l++) {
         const float v[3] = { n[l], n[16 + l], n[32 + l] };
@@ -1785,10 +1867,22 @@ void orc_resolve_lit(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_
                 lit_tile(lanes, active, x0, y0, depth + off, p, lightExposure, color);
             }
             for (int l = 0; l < 16; l++) {
+                if (active[l] || !p->skybox) continue;
+                // sky: the view direction through the pixel at depth 1
+                // (Shading.cpp:663-667, 675-678)
+                float px = (float)(int32_t)(x0 + (l & 3)), py = (float)(int32_t)(y0 + (l >> 2));
+                float c4[4];
+                transform(p->inv_screen_proj, px, py, 1.0f, c4);
+                const float rw = frcp(c4[3]);
+                const float dir[3] = { c4[0] * rw - p->view_pos[0], c4[1] * rw - p->view_pos[1],
+                                       c4[2] * rw - p->view_pos[2] };
+                sky_sample(p->skybox, dir, color[l]);
+            }
+            for (int l = 0; l < 16; l++) {
                 float c[3];
                 for (int k = 0; k < 3; k++) {
                     // Tonemap_Unreal(x * Exposure) (Shading.cpp:221-226, 680)
-                    float x = active[l] ? color[l][k] * exposure : 0.0f;
+                    float x = (active[l] || p->skybox) ? color[l][k] * exposure : 0.0f;
                     c[k] = fdiv(x, x + 0.155f) * 1.019f;
                 }
                 out[(size_t)off + l] = pack_rgba8(c[0], c[1], c[2], 1.0f);

@@ -105,6 +105,7 @@ typedef struct orc_resolve_params {
     uint32_t num_lights;
     uint32_t light_sprites;           /* draw point/spot lights over the image (Shading.cpp:689-727) */
     float world_to_clip[16];          /* ShadingContext::WorldToClipMat (light sprites) */
+    const orc_texture* skybox;        /* ShadingContext::SkyboxTex: R11G11B10f octahedral map, or NULL */
 } orc_resolve_params;
 
 /* Tables reproducing VRCP14PS / VRSQRT14PS; must be loaded before any call. */
@@ -154,6 +155,13 @@ void orc_resolve_albedo(orc_framebuffer* fb, const orc_shading_ctx* ctx);
 #define ORC_DEBUG_OVERDRAW_PIXEL 5u   /* framebuffer drawn with ORC_PIPELINE_OVERDRAW */
 #define ORC_DEBUG_OVERDRAW_QUAD 6u
 void orc_resolve_debug(orc_framebuffer* fb, const orc_shading_ctx* ctx, uint32_t layer, const float* object_to_world);
+/* The skybox lookup of Resolve (Shading.cpp:675-678) for 16 lanes:
+ * texel coordinates of SampleLevel from view directions dir [3][16]
+ * (ScaleLerpU/V slu, slv), and the R11G11B10f bilinear filter of four
+ * texels d [4][16] (00, 10, 01, 11) at fixed-point coordinates ixf, iyf
+ * with the x-inbound mask inx: out = rgb [3][16]. */
+void orc_sky_coords16(const float* dir, float slu, float slv, int32_t* ix, int32_t* iy);
+void orc_env_bilinear16(const uint32_t* d, const int32_t* ixf, const int32_t* iyf, const int32_t* inx, float* out);
 /* FS_EncodeGBuffer's layer-2 word for 16 lanes: n = remapped world normal
  * [3][16], nmr = layer-1 texels. */
 void orc_gbuffer_pack16(const float* n, const uint32_t* nmr, uint32_t* out);

@@ -414,3 +414,49 @@ extern "C" void probe_gbuffer_pack(const float* n, const int32_t* nmr, int32_t* 
     r |= (p >> 26 & 0x3F) << 26;
     memcpy(out, &r, 64);
 }
+
+// Skybox coordinates of Resolve (Shading.cpp:675-678): SampleOctLevel
+// (Texture.h:467-480) maps the view direction with MapOctahedron
+// (Texture.h:282-288) and SampleLevel scales and rounds (Texture.h:412-416).
+// dir: [3][16]; slu, slv: ScaleLerpU/V (run-time values); out ix, iy: [16].
+extern "C" void probe_sky_coords(const float* dir, float slu, float slv, int32_t* ix, int32_t* iy) {
+    typedef uint32_t vu __attribute__((ext_vector_type(16)));
+    vf x = ld(dir), y = ld(dir + 16), z = ld(dir + 32);
+    vf w = (vf)_mm512_rcp14_ps((__m512)(__builtin_elementwise_abs(x) + __builtin_elementwise_abs(y) +
+                                        __builtin_elementwise_abs(z)));
+    vf t = __builtin_elementwise_maxnum(-z * w, (vf)0.0f);
+    vf u = vfma(x, w, (vf)((vu)t ^ ((vu)x & 0x80000000u)));
+    vf v = vfma(y, w, (vf)((vu)t ^ ((vu)y & 0x80000000u)));
+    u = u * 0.5f + 0.5f;
+    v = v * 0.5f + 0.5f;
+    sti(ix, rne(u * slu));
+    sti(iy, rne(v * slv));
+}
+
+// Float bilinear filter of SampleLinear for R11G11B10f texels
+// (Texture.h:504-573, 135-185): 8-bit fractions scaled by 1/256, the x weight
+// zeroed past the last column, F11/F11/F10 unpacked by bit moves, two row
+// lerps and one column lerp. d: [4][16] texels 00, 10, 01, 11; ixf, iyf:
+// fixed-point coordinates; inx: x inbound mask; out: rgb [3][16].
+extern "C" void probe_env_bilinear(const uint32_t* d, const int32_t* ixf, const int32_t* iyf, const int32_t* inx,
+                                   float* out) {
+    typedef uint32_t vu __attribute__((ext_vector_type(16)));
+    vu t[4];
+    for (int k = 0; k < 4; k++) memcpy(&t[k], d + 16 * k, 64);
+    vi fxi = ldi(ixf), fyi = ldi(iyf), in = ldi(inx);
+    const float fracScale = 1.0f / 256;
+    vf fx = __builtin_convertvector(fxi & 255, vf) * fracScale;
+    vf fy = __builtin_convertvector(fyi & 255, vf) * fracScale;
+    fx = in ? fx : (vf)0.0f;
+    for (int c = 0; c < 3; c++) {
+        vf col[4];
+        for (int k = 0; k < 4; k++) {
+            vu x = c == 0 ? t[k] >> 21 : c == 1 ? t[k] >> 10 : t[k];
+            vu b = c == 2 ? ((x << 18) & 0x0FFC0000u) + 0x38000000u : ((x << 17) & 0x0FFE0000u) + 0x38000000u;
+            col[k] = (vf)b;
+        }
+        vf rowA = col[0] + (col[1] - col[0]) * fx;
+        vf rowB = col[2] + (col[3] - col[2]) * fx;
+        st(out + 16 * c, rowA + (rowB - rowA) * fy);
+    }
+}

@@ -161,3 +161,37 @@ def test_debug_layers(gpu_ctx, layer):
     bad = got != want
     print(f"[resolve] debug layer {layer}: {bad.sum()} pixels differ", flush=True)
     assert not bad.any(), np.argwhere(bad)[:3].tolist()
+
+
+@pytest.mark.parametrize("size", [(320, 240, 128), (1920, 1080, 256)])
+def test_skybox(gpu_ctx, size):
+    """The lit resolve with an R11G11B10f octahedral skybox for the pixels
+    nothing covers (Shading.cpp:659-678): sky pixels bit-exact, covered
+    pixels within 1/255, with the light sprites on."""
+    import glimpsw_amd as g
+    from glimpsw_amd import textures
+
+    W, H, tsize = size
+    sc, view = S.checker_scene(1024 if W > 1000 else 192, W, H)
+    sky = textures.sky_texture(tsize)
+    ds = g.Scene(gpu_ctx, sc)
+    dsky = g.Texture(gpu_ctx, sky)
+    fb = g.render_frame(gpu_ctx, ds, view)
+    lighting = S.default_lighting(view)
+    fb.resolve(ds, view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting, skybox=dsky)
+    got = fb.read_color()
+    ids, depth = fb.read()
+    fb.close()
+    dsky.close()
+    ds.close()
+    lighting.skybox = sky
+    ofb, _, b = oracle.render_frame(sc, view)
+    want = oracle.resolve_lit(ofb, b, view.object_to_clip, lighting)
+    skyp = depth <= 0
+    print(f"[resolve] skybox {W}x{H}: {skyp.sum()} sky pixels", flush=True)
+    assert skyp.sum() > 1000
+    bad = (got != want) & skyp
+    assert not bad.any(), f"{bad.sum()} sky pixels differ, first {np.argwhere(bad)[:3].tolist()}"
+    gl = got.view(np.uint8).reshape(*got.shape, 4).astype(int)
+    ol = want.view(np.uint8).reshape(*want.shape, 4).astype(int)
+    assert np.abs(gl - ol).max() <= 1

@@ -386,3 +386,38 @@ def test_gbuffer_pack(probe):
         got = np.zeros(16, np.uint32)
         L.orc_gbuffer_pack16(P(blk), nmr[i:i + 16].ctypes.data_as(u32p), got.ctypes.data_as(u32p))
         assert (got == want.view(np.uint32)).all(), i
+
+
+def test_sky_coordinates(probe):
+    """MapOctahedron of unnormalised view directions and SampleLevel's
+    scaled, rounded coordinates (Texture.h:282-288, 412-416)."""
+    rng = np.random.default_rng(12)
+    d = rng.normal(size=(4096, 3)).astype(np.float32) * rng.uniform(0.01, 50, (4096, 1)).astype(np.float32)
+    d[:16] = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [-1, 0, 0], [0, -1, 0], [0, 0, -1], [1, 1, -1], [-1, -1, 1]] * 2,
+                      np.float32)
+    L = oracle.lib()
+    for slu, slv in ((256.0 * 128, 256.0 * 64), (256.0 * 1024, 256.0 * 1024)):
+        for i in range(0, 4096, 16):
+            blk = np.ascontiguousarray(d[i:i + 16].T)
+            wx, wy = np.zeros(16, np.int32), np.zeros(16, np.int32)
+            probe.probe_sky_coords(P(blk), C.c_float(slu), C.c_float(slv), P(wx), P(wy))
+            gx, gy = np.zeros(16, np.int32), np.zeros(16, np.int32)
+            L.orc_sky_coords16(P(blk), C.c_float(slu), C.c_float(slv), P(gx), P(gy))
+            assert (gx == wx).all() and (gy == wy).all(), i
+
+
+def test_env_bilinear(probe):
+    """SampleLinear's float filter on R11G11B10f texels (Texture.h:557-571)."""
+    rng = np.random.default_rng(13)
+    L = oracle.lib()
+    u32p = C.POINTER(C.c_uint32)
+    for i in range(256):
+        d = rng.integers(0, 2 ** 32, 64, dtype=np.uint64).astype(np.uint32)
+        ixf = rng.integers(0, 1 << 20, 16).astype(np.int32)
+        iyf = rng.integers(0, 1 << 20, 16).astype(np.int32)
+        inx = rng.integers(0, 2, 16).astype(np.int32) * -1
+        want = np.zeros(48, np.float32)
+        probe.probe_env_bilinear(d.ctypes.data_as(u32p), P(ixf), P(iyf), P(inx), P(want))
+        got = np.zeros(48, np.float32)
+        L.orc_env_bilinear16(d.ctypes.data_as(u32p), P(ixf), P(iyf), P(inx), P(got))
+        assert (_bits(got) == _bits(want)).all(), i
