@@ -66,7 +66,8 @@ class DrawDesc(C.Structure):
 class ViewDesc(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("framebuffer", C.c_void_p), ("meshlet_offset", C.c_uint32),
                 ("meshlet_count", C.c_uint32), ("object_to_clip", C.c_float * 16), ("frustum_planes", f32p),
-                ("flags", C.c_uint32), ("clear_color", C.c_uint32), ("clear_depth", C.c_float)]
+                ("flags", C.c_uint32), ("clear_color", C.c_uint32), ("clear_depth", C.c_float),
+                ("occlusion", C.POINTER(OcclusionDesc))]
 
 
 class BatchShape(C.Structure):
@@ -553,6 +554,7 @@ class Batch:
             sc, fb, vw = v[0], v[1], v[2]
             planes = v[3] if len(v) > 3 else vw.frustum_planes()
             flags = v[4] if len(v) > 4 else 0
+            occlusion = v[5] if len(v) > 5 else None
             d = arr[i]
             d.scene, d.framebuffer = sc.handle, fb.handle
             d.meshlet_offset, d.meshlet_count = vw.meshlet_offset, vw.meshlet_count
@@ -564,6 +566,9 @@ class Batch:
                 d.frustum_planes = _ptr(pl, f32p)
             d.flags = flags
             d.clear_color, d.clear_depth = clear_color, clear_depth
+            if occlusion is not None:
+                d.occlusion = C.pointer(occlusion.desc)
+                self._keep.append(occlusion)
         sh = None
         if shape is not None:
             sh = BatchShape(shape["bin_size"], shape["bin_capacity"], shape.get("raster_threads", 256))

@@ -799,6 +799,7 @@ int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, 
 struct glimp_batch {
     glimp_context* ctx = nullptr;
     std::vector<glimp_view_desc> descs;
+    std::vector<glimp_occlusion> occlusion;   // per view, copied (descs point here)
     std::vector<ViewParams> views;
     ViewParams* d_views = nullptr;
     size_t total_meshlets = 0;
@@ -886,6 +887,10 @@ int glimp_batch_create(glimp_context* c, const glimp_view_desc* views, uint32_t 
             return fail(GLIMP_E_INVALID, "glimp_batch_create: meshlet range outside the scene");
         if (d.meshlet_count > GLIMP_MAX_DRAWN_MESHLETS) return fail(GLIMP_E_LIMIT, "glimp_batch_create: view too large");
         if (!(d.clear_depth >= 0.0f)) return fail(GLIMP_E_INVALID, "glimp_batch_create: clear depth must be >= 0");
+        if (d.occlusion && !d.frustum_planes)
+            return fail(GLIMP_E_INVALID, "glimp_batch_create: occlusion needs frustum planes");
+        if (d.occlusion && !d.occlusion->depth_map)
+            return fail(GLIMP_E_INVALID, "glimp_batch_create: occlusion without a depth map");
         for (auto* f : seen)
             if (f == d.framebuffer) return fail(GLIMP_E_INVALID, "glimp_batch_create: views must not share a framebuffer");
         seen.push_back(d.framebuffer);
@@ -904,10 +909,18 @@ int glimp_batch_create(glimp_context* c, const glimp_view_desc* views, uint32_t 
         b->shape.raster_threads = shape->raster_threads;
     }
     b->descs.assign(views, views + count);
+    b->occlusion.resize(count);
+    for (uint32_t i = 0; i < count; i++) {
+        if (!views[i].occlusion) continue;
+        b->occlusion[i] = *views[i].occlusion;
+        b->descs[i].occlusion = &b->occlusion[i];
+    }
     b->views.resize(count);
     size_t base = 0;
     for (uint32_t i = 0; i < count; i++) {
-        const glimp_view_desc& d = views[i];
+        const glimp_view_desc& d = b->descs[i];
+        Occlusion occ;
+        if (d.occlusion) (void)make_occlusion(d.occlusion, "glimp_batch_create", occ);
         glimp_framebuffer tmp;     // the view starts from a cleared framebuffer
         tmp.width = d.framebuffer->width;
         tmp.height = d.framebuffer->height;
@@ -916,7 +929,7 @@ int glimp_batch_create(glimp_context* c, const glimp_view_desc* views, uint32_t 
         tmp.pending_clear = true;
         tmp.next_seq = 0;
         make_view(&tmp, d.scene, d.meshlet_offset, d.meshlet_count, d.object_to_clip, d.frustum_planes, d.flags,
-                  b->views[i]);
+                  b->views[i], d.occlusion ? &occ : nullptr);
         b->views[i].meshlet_base = (uint32_t)base;
         base += d.meshlet_count;
     }
@@ -1050,6 +1063,7 @@ int glimp_batch_launch_per_view(glimp_batch* b, void* stream) {
         dd.frustum_planes = d.frustum_planes;
         dd.flags = d.flags & (GLIMP_DRAW_NO_CLIPPING | GLIMP_DRAW_NO_GUARDBAND);
         dd.pipeline = GLIMP_PIPELINE_VISBUFFER;
+        dd.occlusion = d.occlusion;
         dd.bin_size = b->shape.bin_size;
         dd.bin_capacity = b->shape.bin_capacity;
         rc = glimp_draw_meshlets(b->ctx, d.framebuffer, d.scene, &dd, stream);

@@ -236,6 +236,8 @@ typedef struct glimp_view_desc {
     uint32_t flags;                    /* GLIMP_DRAW_NO_CLIPPING / GLIMP_DRAW_NO_GUARDBAND */
     uint32_t clear_color;
     float clear_depth;                 /* >= 0 */
+    const struct glimp_occlusion* occlusion; /* with frustum_planes: HiZ test of the survivors, or NULL;
+                                        * copied at creation, its depth map read at every launch */
 } glimp_view_desc;
 
 /* Launch shape: screen bin side, records per bin, bin-raster workgroup size. */
