@@ -77,6 +77,11 @@ class BatchShape(C.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class MeshletRange(C.Structure):
+    _fields_ = [("vertex_offset", C.c_uint32), ("triangle_offset", C.c_uint32), ("vertex_count", C.c_uint32),
+                ("triangle_count", C.c_uint32)]
+
+
 class DiffTextureDesc(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("mip_levels", C.c_uint32), ("pad", C.c_uint32),
                 ("level_offset", C.c_uint64 * 16)]
@@ -194,6 +199,11 @@ _SIGNATURES = {
     "glimp_diff_backward_views": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DiffDesc), C.POINTER(DiffViewDesc),
                                             C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_void_p]),
+    "glimp_build_meshlets_bound": (C.c_size_t, [C.c_size_t, C.c_size_t, C.c_size_t]),
+    "glimp_build_meshlets": (C.c_int, [u32p, C.c_size_t, f32p, C.c_size_t, C.c_size_t, C.c_uint32, C.c_uint32,
+                                       C.POINTER(MeshletRange), u32p, C.POINTER(C.c_uint8), C.POINTER(C.c_size_t)]),
+    "glimp_meshlet_bounds": (C.c_int, [u32p, C.c_uint32, f32p, C.c_size_t, f32p]),
+    "glimp_pack_normal_tangents": (C.c_int, [f32p, f32p, C.c_size_t, u32p]),
     "glimp_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "glimp_profile_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_uint32, C.POINTER(C.c_uint32), C.c_int]),
 }

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 GEN = os.path.join(HERE, "_gen")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libglimp.so")
-SOURCES = ["glimp_kernels.hip", "resolve.hip", "diff.hip", "capi.hip"]
+SOURCES = ["glimp_kernels.hip", "resolve.hip", "diff.hip", "capi.hip", "meshlets.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GLIMP_ARCH", "gfx950")
 

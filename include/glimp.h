@@ -217,6 +217,41 @@ int glimp_cull_meshlets_occlusion(glimp_context* ctx, glimp_scene* scene, uint32
                                   const float planes[20], const glimp_occlusion* occlusion, uint16_t* bitmap,
                                   uint32_t* visible, void* stream);
 
+/* ------------------------------------------------------------ import
+ *
+ * Host-side helpers of Scene::ImportGltf (Scene.cpp:199-279); no device
+ * work. The reference uses meshoptimizer v1.1 there, which this library
+ * does not contain: glimp_build_meshlets is its own greedy builder with the
+ * same limits and output layout (meshopt_buildMeshlets, Scene.cpp:214-217),
+ * so meshlet partitions differ from the reference's. */
+typedef struct glimp_meshlet_range {
+    uint32_t vertex_offset;            /* into meshlet_vertices */
+    uint32_t triangle_offset;          /* into meshlet_triangles (3 u8 corners per triangle) */
+    uint32_t vertex_count, triangle_count;
+} glimp_meshlet_range;
+/* meshopt_buildMeshletsBound (Scene.cpp:197): the most meshlets
+ * glimp_build_meshlets can write for index_count indices. */
+size_t glimp_build_meshlets_bound(size_t index_count, size_t max_vertices, size_t max_triangles);
+/* meshopt_buildMeshlets (Scene.cpp:214-217): partitions a triangle list into
+ * meshlets of at most max_vertices (<= 256) vertices and max_triangles
+ * (<= 512) triangles; `meshlets` holds glimp_build_meshlets_bound entries,
+ * meshlet_vertices and meshlet_triangles index_count entries each. Every
+ * input triangle lands in exactly one meshlet. */
+int glimp_build_meshlets(const uint32_t* indices, size_t index_count, const float* positions, size_t vertex_count,
+                         size_t vertex_stride, uint32_t max_vertices, uint32_t max_triangles,
+                         glimp_meshlet_range* meshlets, uint32_t* meshlet_vertices, uint8_t* meshlet_triangles,
+                         size_t* meshlet_count);
+/* meshopt_computeMeshletBounds' sphere (Scene.cpp:226-232): a bounding
+ * sphere (centre, radius) of the meshlet's vertices that contains each of
+ * them in float arithmetic. */
+int glimp_meshlet_bounds(const uint32_t* meshlet_vertices, uint32_t vertex_count, const float* positions,
+                         size_t vertex_stride, float sphere[4]);
+/* Meshlet::NormalTangents of ImportGltf (Scene.cpp:266-274):
+ * packUnorm4x8(MapOctahedron(normal), MapOctahedron(tangent.xyz)) per vertex,
+ * the octahedral map evaluated as the reference's optimised build does;
+ * tangents ([count][4]) may be NULL (zero bytes). */
+int glimp_pack_normal_tangents(const float* normals, const float* tangents, size_t count, uint32_t* packed);
+
 /* ------------------------------------------------------------ batches */
 
 /* Many views (scene, meshlet range, matrix, framebuffer) rendered together:

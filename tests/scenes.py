@@ -129,3 +129,25 @@ def occluder_scene(n: int = 600, seed_offset: int = 0):
     mid = np.zeros(n + 1, np.uint32)
     mid[-1] = 1
     return S.build_patches(centers, R, scales, mid, mats, [textures.checker_texture(256)])
+
+
+def torus_mesh(R: float = 1.0, r: float = 0.35, nu: int = 96, nv: int = 48):
+    """An indexed torus around the y axis: positions, unit normals, tangents
+    (xyz along u, w = +1 on one half of the ring and -1 on the other), UVs
+    and a triangle list (two triangles per grid cell, shared vertices)."""
+    u = np.arange(nu) * (2 * np.pi / nu)
+    v = np.arange(nv) * (2 * np.pi / nv)
+    U, V = np.meshgrid(u, v, indexing="ij")
+    cx, cz = np.cos(U), np.sin(U)
+    pos = np.stack([(R + r * np.cos(V)) * cx, r * np.sin(V), (R + r * np.cos(V)) * cz], axis=-1).reshape(-1, 3)
+    nrm = np.stack([np.cos(V) * cx, np.sin(V), np.cos(V) * cz], axis=-1).reshape(-1, 3)
+    tan = np.stack([-np.sin(U), np.zeros_like(U), np.cos(U), np.where(U < np.pi, 1.0, -1.0)], axis=-1).reshape(-1, 4)
+    uv = np.stack([U / (2 * np.pi) * 4, V / (2 * np.pi) * 2], axis=-1).reshape(-1, 2)
+    i, j = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    a = i * nv + j
+    b = ((i + 1) % nu) * nv + j
+    c = ((i + 1) % nu) * nv + (j + 1) % nv
+    d = i * nv + (j + 1) % nv
+    tris = np.concatenate([np.stack([a, b, c], -1).reshape(-1, 3), np.stack([a, c, d], -1).reshape(-1, 3)])
+    return (pos.astype(np.float32), nrm.astype(np.float32), tan.astype(np.float32), uv.astype(np.float32),
+            tris.astype(np.uint32))
