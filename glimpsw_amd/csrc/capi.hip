@@ -853,7 +853,9 @@ void batch_params(glimp_batch* b, DrawParams& p) {
 }
 
 bool valid_shape(const glimp_batch_shape& s) {
-    return valid_bin_size(s.bin_size) && s.bin_capacity >= 1 && (s.raster_threads == 128 || s.raster_threads == 256);
+    const bool wide = s.raster_threads == 512 || s.raster_threads == 1024;    // 32- and 64-pixel bins only
+    return valid_bin_size(s.bin_size) && s.bin_capacity >= 1 &&
+           (s.raster_threads == 128 || s.raster_threads == 256 || (wide && s.bin_size >= 32));
 }
 
 // Framebuffer state after a batch launch: cleared, then one draw.
@@ -1082,7 +1084,7 @@ int glimp_batch_autotune(glimp_batch* b, void* stream, glimp_batch_shape* chosen
     hipStream_t st = (hipStream_t)stream;
     const uint32_t sizes[] = { 8, 16, 32, 64 };
     const uint32_t caps[] = { 256, 512, 1024, 2048, 4096 };
-    const uint32_t threads[] = { 128, 256 };
+    const uint32_t threads[] = { 128, 256, 512, 1024 };
     const size_t kListBudget = (size_t)2 << 30;   // bytes of bin lists a candidate may use
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
@@ -1098,6 +1100,7 @@ int glimp_batch_autotune(glimp_batch* b, void* stream, glimp_batch_shape* chosen
                 const uint64_t bins = count_bins(b->views, bs);
                 if (bins >= kMaxBins || bins * cap * 4 > kListBudget) continue;
                 glimp_batch_shape sh = { bs, cap, th };
+                if (!valid_shape(sh)) continue;
                 rc = glimp_batch_set_shape(b, &sh);
                 if (rc != GLIMP_OK) break;
                 for (int i = 0; i < 2 && rc == GLIMP_OK; i++) rc = glimp_batch_launch(b, 0, stream);

@@ -1727,6 +1727,41 @@ __global__ void k_decode_overdraw(const unsigned long long* counts, const uint32
     if (depth_out) depth_out[o] = depth[i];
 }
 
+// The bin raster for the launch's bin size and workgroup size: 128 or 256
+// threads for every bin size, 512 or 1024 for 32- and 64-pixel bins (a
+// crowded large bin then has more waves to spread its tiles over).
+template <int BS, bool Alpha>
+bool launch_bin_raster_bs(const DrawParams& p, dim3 grid, hipStream_t s) {
+    switch (p.raster_threads) {
+    case 128: hipLaunchKernelGGL((k_bin_raster<BS, 128, Alpha>), grid, dim3(128), 0, s, p); return true;
+    case 256: hipLaunchKernelGGL((k_bin_raster<BS, 256, Alpha>), grid, dim3(256), 0, s, p); return true;
+    case 512:
+        if constexpr (BS >= 32) {
+            hipLaunchKernelGGL((k_bin_raster<BS, 512, Alpha>), grid, dim3(512), 0, s, p);
+            return true;
+        }
+        return false;
+    case 1024:
+        if constexpr (BS >= 32) {
+            hipLaunchKernelGGL((k_bin_raster<BS, 1024, Alpha>), grid, dim3(1024), 0, s, p);
+            return true;
+        }
+        return false;
+    default: return false;
+    }
+}
+
+template <bool Alpha>
+bool launch_bin_raster(const DrawParams& p, dim3 grid, hipStream_t s) {
+    switch (p.bin_size) {
+    case 8: return launch_bin_raster_bs<8, Alpha>(p, grid, s);
+    case 16: return launch_bin_raster_bs<16, Alpha>(p, grid, s);
+    case 32: return launch_bin_raster_bs<32, Alpha>(p, grid, s);
+    case 64: return launch_bin_raster_bs<64, Alpha>(p, grid, s);
+    default: return false;
+    }
+}
+
 void launch_front(const DrawParams& p, hipStream_t s, Profiler* prof, bool separate_clip) {
     if (!p.total_meshlets) hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, s, p.scratch);
     if (p.total_meshlets) {     // a batch of empty views still clears through the bin raster
@@ -1884,26 +1919,12 @@ hipError_t launch_draw_binned(const DrawParams& p, hipStream_t s, Profiler* prof
     {
         Profiler::Scope sc(prof, "bin_raster", s);
         const dim3 grid(p.total_bins);
-        const bool small = p.raster_threads == 128;
         // records of alpha-tested materials need the variant with the alpha test
-#define GLIMP_BIN_CASE(BS)                                                                             \
-    case BS:                                                                                           \
-        if (p.alpha) {                                                                                 \
-            if (small) hipLaunchKernelGGL((k_bin_raster<BS, 128, true>), grid, dim3(128), 0, s, p);    \
-            else hipLaunchKernelGGL((k_bin_raster<BS, 256, true>), grid, dim3(256), 0, s, p);          \
-        } else {                                                                                       \
-            if (small) hipLaunchKernelGGL((k_bin_raster<BS, 128, false>), grid, dim3(128), 0, s, p);   \
-            else hipLaunchKernelGGL((k_bin_raster<BS, 256, false>), grid, dim3(256), 0, s, p);         \
-        }                                                                                              \
-        break;
-        switch (p.bin_size) {
-            GLIMP_BIN_CASE(8)
-            GLIMP_BIN_CASE(16)
-            GLIMP_BIN_CASE(32)
-            GLIMP_BIN_CASE(64)
-        default: return hipErrorInvalidValue;
+        if (p.alpha) {
+            if (!launch_bin_raster<true>(p, grid, s)) return hipErrorInvalidValue;
+        } else {
+            if (!launch_bin_raster<false>(p, grid, s)) return hipErrorInvalidValue;
         }
-#undef GLIMP_BIN_CASE
     }
     {
         Profiler::Scope sc(prof, "raster_spill", s);

@@ -199,7 +199,7 @@ struct DrawParams {
     DrawScratch* scratch;
     Counters* counters;
     uint32_t bin_size, bin_shift, bin_capacity;
-    uint32_t raster_threads;       // bin raster workgroup size: 128 or 256
+    uint32_t raster_threads;       // bin raster workgroup size: 128 or 256 (512, 1024 for 32/64-pixel bins)
     uint32_t* bin_counts;          // [total_bins]
     uint32_t* bin_lists;           // [total_bins][bin_capacity]
     uint32_t* spill;               // [clip_record_base + clip_record_capacity]

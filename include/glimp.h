@@ -279,7 +279,7 @@ typedef struct glimp_view_desc {
 typedef struct glimp_batch_shape {
     uint32_t bin_size;                 /* 8, 16, 32 or 64 */
     uint32_t bin_capacity;             /* records per bin before spilling */
-    uint32_t raster_threads;           /* 128 or 256 */
+    uint32_t raster_threads;           /* 128 or 256; also 512 or 1024 for 32- and 64-pixel bins */
 } glimp_batch_shape;
 
 int glimp_batch_create(glimp_context* ctx, const glimp_view_desc* views, uint32_t count,

@@ -110,9 +110,11 @@ def test_plain_then_alpha_draws(gpu_ctx):
     _assert_same(got, (ofb.ids(), ofb.depth()), "opaque then cutout draw")
 
 
-def test_alpha_batch(gpu_ctx):
+@pytest.mark.parametrize("shape", [None, {"bin_size": 32, "bin_capacity": 512, "raster_threads": 1024},
+                                   {"bin_size": 64, "bin_capacity": 4096, "raster_threads": 512}])
+def test_alpha_batch(gpu_ctx, shape):
     """A batch mixing alpha-tested and opaque scenes equals its per-view
-    draws and the oracle."""
+    draws and the oracle (default shape and the wide bin rasters)."""
     import glimpsw_amd as g
 
     scenes = [T.alpha_scene(300, seed_offset=7, spread=6.0, near_frac=0.2), T.small_scene(300, seed_offset=8)]
@@ -124,7 +126,7 @@ def test_alpha_batch(gpu_ctx):
         views.append((i % 2, v))
     fbs = [g.Framebuffer(gpu_ctx, v.width, v.height) for _, v in views]
     items = [(dsc[si], fb, v) for (si, v), fb in zip(views, fbs)]
-    b = g.Batch(gpu_ctx, items)
+    b = g.Batch(gpu_ctx, items, shape=shape)
     b.launch_per_view()
     want = [fb.read() for fb in fbs]
     b.launch(graph=True)

@@ -100,6 +100,7 @@ def test_graph_replays_with_work_between(gpu_ctx, small):
 
 @pytest.mark.parametrize("bin_size,capacity,threads", [
     (8, 256, 128), (16, 64, 256), (32, 4096, 256), (64, 2048, 128), (64, 3, 256),
+    (32, 4096, 512), (32, 1024, 1024), (64, 2048, 1024), (32, 3, 1024),
 ])
 def test_batch_shapes(gpu_ctx, small, bin_size, capacity, threads):
     scenes, views, st = small
@@ -124,7 +125,7 @@ def test_autotune_keeps_results(gpu_ctx, small):
     print(f"[batch] autotune -> {shape} {ms:.3f} ms", flush=True)
     assert shape["bin_size"] in (8, 16, 32, 64)
     assert 256 <= shape["bin_capacity"] <= 4096
-    assert shape["raster_threads"] in (128, 256)
+    assert shape["raster_threads"] in (128, 256, 512, 1024)
     assert ms > 0.0
     b.launch(graph=True)
     _same(want, st.read_all(), "autotuned")
