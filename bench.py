@@ -30,6 +30,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Vector-ALU issue rate: 256 CUs x 4 SIMDs, a wave64 VALU instruction occupies
+# a 16-lane SIMD for 4 cycles at the 2.4 GHz peak clock.
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
 
 
 def algorithmic_bytes(stats: dict) -> dict:
@@ -112,7 +115,10 @@ def measure_traffic() -> dict | None:
     loop, each scaled by what the counter reads for a device copy of known
     size."""
     copy_bytes = 1 << 28
-    frame = ["--steps", "3", "--warmup", "1", "--no-cpu-baseline", "--no-batch", "--no-traffic", "--no-counts"]
+    # the separate-call frames' launch shape (the library default) for the
+    # batch frames too, so that each kernel has one instantiation
+    frame = ["--steps", "3", "--warmup", "1", "--no-cpu-baseline", "--no-batch", "--no-traffic", "--no-counts",
+             "--bin-size", "16", "--bin-capacity", "1024", "--raster-threads", "256"]
     copy = ["--calibrate-copy", str(copy_bytes)]
     fetch = _rocprof_counter("FETCH_SIZE", frame, "frame")
     write = _rocprof_counter("WRITE_SIZE", frame, "frame")
@@ -135,7 +141,23 @@ def measure_traffic() -> dict | None:
         fv, wv = pick(fetch, sub), pick(write, sub)
         if fv is not None and wv is not None:
             out[name] = int(fv * 1024.0 * f_scale + wv * 1024.0 * w_scale)
+    # vector-ALU instructions per launch (wave instructions): the bound these
+    # kernels actually meet
+    valu = _rocprof_counter("SQ_INSTS_VALU", frame, "frame")
+    if valu:
+        out["valu"] = {name: int(pick(valu, sub)) for name, sub in KERNEL_SYMBOLS.items() if pick(valu, sub)}
     return out
+
+
+def _valu_roofline(traffic, name: str, avg_us) -> dict | None:
+    """The kernel's vector-ALU instructions per launch against the VALU issue
+    rate: the time they need at best, and that time over the measured one."""
+    if not traffic or not avg_us or name not in traffic.get("valu", {}):
+        return None
+    insts = traffic["valu"][name]
+    bound_us = insts / VALU_PEAK_WAVE_INSTS * 1e6
+    return {"wave_insts": insts, "peak_wave_insts_per_s": VALU_PEAK_WAVE_INSTS, "bound_us": round(bound_us, 2),
+            "frac": round(bound_us / avg_us, 4)}
 
 
 def cpu_baseline(scene, view, seconds: float) -> dict:
@@ -234,6 +256,7 @@ def main() -> int:
     ap.add_argument("--direct", action="store_true", help="global-atomic raster instead of tile bins")
     ap.add_argument("--bin-size", type=int, default=0, help="screen bin side (8/16/32/64), 0 = library default")
     ap.add_argument("--bin-capacity", type=int, default=0, help="records per bin, 0 = library default")
+    ap.add_argument("--raster-threads", type=int, default=0, help="bin raster workgroup (128/256/512/1024), 0 = 256")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batching measurement")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM traffic passes")
     ap.add_argument("--no-counts", action="store_true", help="skip the counting pass (frame kernels only)")
@@ -305,9 +328,11 @@ def main() -> int:
     graph = None
     frame_shape = None
     if not args.direct and not args.frame_calls:
-        shape = {"bin_size": args.bin_size or 16, "bin_capacity": args.bin_capacity or 1024}
+        shape = {"bin_size": args.bin_size or 16, "bin_capacity": args.bin_capacity or 1024,
+                 "raster_threads": args.raster_threads or 256}
         graph = g.Batch(ctx, [(ds, fb, view, planes, flags)], shape=shape)
-        if not args.bin_size and not args.bin_capacity:
+        frame_shape = shape
+        if not args.bin_size and not args.bin_capacity and not args.raster_threads:
             frame_shape, _ = graph.autotune(stream)     # the launch shape, as for any batch
         graph.set_resolve([(view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting)])
 
@@ -384,7 +409,8 @@ def main() -> int:
         b = alg.get(name, 0)
         per_kernel[name] = {"avg_us": round(avg_ms * 1e3, 2), "launches": launches, "alg_bytes": b,
                             "GBps": round(b / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 else None,
-                            "hbm_bytes": traffic.get(name) if traffic else None}
+                            "hbm_bytes": traffic.get(name) if traffic else None,
+                            "valu": _valu_roofline(traffic, name, round(avg_ms * 1e3, 2))}
     dom = max(kern.items(), key=lambda kv: kv[1][1])[0]
     dk = per_kernel[dom]
     tris = scene.triangles()
@@ -407,11 +433,16 @@ def main() -> int:
         "dtype": "int32+fp32",
         "data": "synthetic",
         "config": {"workload": "10,240 grid-patch meshlets (1,003,520 tris), 1920x1080, depth complexity ~3, "
-                               "16 materials 2048^2 RGBA8", "raster": "direct" if args.direct else f"binned {args.bin_size or 16}x{args.bin_size or 16}",
+                               "16 materials 2048^2 RGBA8",
+                   "raster": "direct" if args.direct else (
+                       f"binned {frame_shape['bin_size']}x{frame_shape['bin_size']}, {frame_shape['bin_capacity']} "
+                       f"records per bin, {frame_shape['raster_threads']}-thread bin raster" if frame_shape else
+                       f"binned {args.bin_size or 16}x{args.bin_size or 16}"),
                    "submitted_tris": tris},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dk["GBps"] / HBM_PEAK_GBS, 4) if dk["GBps"] else None,
                      "traffic": traffic.get(dom) if traffic else None,
+                     "valu": _valu_roofline(traffic, dom, dk["avg_us"]),
                      "alg_bytes_per_launch": dk["alg_bytes"], "avg_us": dk["avg_us"]},
         "traffic_calibration": ({"fetch_scale": traffic["fetch_scale"], "write_scale": traffic["write_scale"]}
                                 if traffic else None),

@@ -57,6 +57,7 @@ def compile_cmd() -> list[str]:
         "-Wno-unused-command-line-argument",
         "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-I", GEN,
         *srcs, "-o", LIB,
+        "-L", "/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib",
     ]
 
 

@@ -14,6 +14,8 @@
 #include "resolve.h"
 #include "diff.h"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 // 128 (A, B) pairs of the piecewise-linear VRCP14 / VRSQRT14 mantissa
 // tables (device_math.h), generated at build time from
 // glimpsw_amd/data/approx14_pwl.txt and checked there against approx14.bin.
@@ -25,6 +27,16 @@ static_assert(sizeof(kApproxPwl) == 256 * 4, "approximation pairs");
 using namespace glimp;
 
 namespace {
+
+// A roctx range around an entry point: the host-side zones of the
+// reference's Tracy instrumentation (Rasterizer.cpp:494-612, Main.cpp:98-263),
+// visible to rocprofv3 --marker-trace; inert without a tool attached.
+struct Zone {
+    explicit Zone(const char* name) { roctxRangePushA(name); }
+    ~Zone() { roctxRangePop(); }
+    Zone(const Zone&) = delete;
+    Zone& operator=(const Zone&) = delete;
+};
 
 thread_local std::string g_error;
 
@@ -711,6 +723,7 @@ extern "C" {
 
 int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, const glimp_draw_desc* d,
                         void* stream) {
+    Zone zone("glimp_draw_meshlets");
     if (!c || !f || !s || !d) return fail(GLIMP_E_INVALID, "glimp_draw_meshlets: NULL argument");
     if (d->pipeline != GLIMP_PIPELINE_VISBUFFER && d->pipeline != GLIMP_PIPELINE_OVERDRAW &&
         d->pipeline != GLIMP_PIPELINE_DEFERRED)
@@ -974,6 +987,7 @@ int glimp_batch_set_shape(glimp_batch* b, const glimp_batch_shape* shape) {
 }
 
 int glimp_batch_launch(glimp_batch* b, int use_graph, void* stream) {
+    Zone zone("glimp_batch_launch");
     if (!b) return fail(GLIMP_E_INVALID, "glimp_batch_launch: NULL batch");
     HIP_TRY(hipSetDevice(b->ctx->device));
     hipStream_t user = (hipStream_t)stream;
@@ -1169,6 +1183,7 @@ extern "C" {
 
 int glimp_cull_meshlets(glimp_context* c, glimp_scene* s, uint32_t first, uint32_t count, const float planes[20],
                         uint16_t* bitmap, uint32_t* visible, void* stream) {
+    Zone zone("glimp_cull_meshlets");
     return cull_common(c, s, first, count, planes, nullptr, bitmap, visible, stream, "glimp_cull_meshlets");
 }
 
@@ -1212,6 +1227,7 @@ void glimp_hiz_destroy(glimp_hiz* h) {
 }
 
 int glimp_hiz_build(glimp_hiz* h, glimp_framebuffer* f, void* stream) {
+    Zone zone("glimp_hiz_build");
     if (!h || !f) return fail(GLIMP_E_INVALID, "glimp_hiz_build: NULL argument");
     if (f->width != h->layout.width || f->height != h->layout.height)
         return fail(GLIMP_E_INVALID, "glimp_hiz_build: framebuffer and depth map sizes differ");
@@ -1343,6 +1359,7 @@ int build_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const
 extern "C" {
 
 int glimp_resolve(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const glimp_resolve_desc* d, void* stream) {
+    Zone zone("glimp_resolve");
     if (!c || !f || !sc || !d) return fail(GLIMP_E_INVALID, "glimp_resolve: NULL argument");
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
@@ -1516,6 +1533,7 @@ extern "C" {
 
 int glimp_diff_forward(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const glimp_diff_desc* d, float* out,
                        void* stream) {
+    Zone zone("glimp_diff_forward");
     if (!d) return fail(GLIMP_E_INVALID, "glimp_diff_forward: NULL argument");
     glimp_diff_view v;
     v.framebuffer = f;
@@ -1525,6 +1543,7 @@ int glimp_diff_forward(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, 
 
 int glimp_diff_backward(glimp_context* c, glimp_framebuffer* f, glimp_scene* sc, const glimp_diff_desc* d,
                         const float* grad_out, float* grad_texels, float* grad_uv, float* grad_color, void* stream) {
+    Zone zone("glimp_diff_backward");
     if (!d) return fail(GLIMP_E_INVALID, "glimp_diff_backward: NULL argument");
     glimp_diff_view v;
     v.framebuffer = f;
