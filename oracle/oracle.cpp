@@ -1330,6 +1330,10 @@ void orc_load_tables(const uint16_t* tables) {
     memcpy(g_rsq, tables + 65536, sizeof(g_rsq));
 }
 float orc_rcp14(float x) { return rcp14(x); }
+void orc_approx_n(int32_t op, const float* a, const float* b, float* out, uint64_t n) {
+    for (uint64_t i = 0; i < n; i++)
+        out[i] = op == 0 ? rcp14(a[i]) : op == 1 ? rsqrt14(a[i]) : fdiv(a[i], b[i]);
+}
 float orc_rsqrt14(float x) { return rsqrt14(x); }
 float orc_div(float a, float b) { return fdiv(a, b); }
 int32_t orc_cvt_rne(float x) { return cvt_rne(x); }

@@ -114,6 +114,8 @@ float orc_rcp14(float x);
 float orc_rsqrt14(float x);
 float orc_div(float a, float b);      /* vector a/b under -ffast-math */
 int32_t orc_cvt_rne(float x);         /* cvtps2dq */
+/* op 0: rcp14(a), 1: rsqrt14(a), 2: fast-math a / b, over n elements. */
+void orc_approx_n(int32_t op, const float* a, const float* b, float* out, uint64_t n);
 
 void orc_fb_clear(orc_framebuffer* fb, uint32_t color, float depth);
 void orc_get_pixels(const orc_framebuffer* fb, uint32_t layer, uint32_t* dest, uint32_t stride);
