@@ -91,6 +91,8 @@ struct Scratch {
     uint32_t* bin_counts = nullptr;
     size_t bin_counts_cap = 0;
     uint32_t* bin_lists = nullptr;
+    uint32_t* bin_order = nullptr;
+    size_t bin_order_cap = 0;
     size_t bin_lists_cap = 0;
     uint32_t* spill = nullptr;
     size_t spill_cap = 0;
@@ -99,7 +101,8 @@ struct Scratch {
     DrawScratch* scratch = nullptr;
 
     void release() {
-        void* ptrs[] = { records, clip, visible_list, meshlet_counts, bin_counts, bin_lists, spill, alpha, scratch };
+        void* ptrs[] = { records, clip, visible_list, meshlet_counts, bin_counts, bin_lists, bin_order, spill, alpha,
+                         scratch };
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         *this = Scratch();
@@ -677,6 +680,7 @@ int prepare_scratch(Scratch& S, size_t total_meshlets, size_t total_bins, const 
     if (rc == GLIMP_OK && S.bin_counts != old_counts)
         HIP_TRY(hipMemset(S.bin_counts, 0, S.bin_counts_cap * sizeof(uint32_t)));
     if (rc == GLIMP_OK && binned) rc = grow_one(S.bin_lists, S.bin_lists_cap, total_bins * sh.bin_capacity);
+    if (rc == GLIMP_OK && binned) rc = grow_one(S.bin_order, S.bin_order_cap, total_bins * kBinBuckets);
     if (rc == GLIMP_OK && binned) rc = grow_one(S.spill, S.spill_cap, recs);
     if (rc == GLIMP_OK && alpha) rc = grow_one(S.alpha, S.alpha_cap, S.records_cap);
     if (rc == GLIMP_OK && !S.scratch) {
@@ -712,6 +716,7 @@ void fill_params(glimp_context* c, const Scratch& S, size_t total_meshlets, uint
     p.raster_threads = sh.raster_threads;
     p.bin_counts = S.bin_counts;
     p.bin_lists = S.bin_lists;
+    p.bin_order = S.bin_order;
     p.spill = S.spill;
 }
 

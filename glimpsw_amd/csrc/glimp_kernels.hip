@@ -267,6 +267,7 @@ __global__ void __launch_bounds__(256) k_cull(DrawParams p) {
     const uint32_t first = blockIdx.x * kCullPerBlock;
     // counters the stages after the cull fill; the previous draw's last
     // readers are done (num_visible is cleared after its last reader)
+    if (blockIdx.x == 0 && threadIdx.x < kBinBuckets) p.scratch->bucket[threadIdx.x] = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         p.scratch->num_clip = 0;
         p.scratch->num_records = 0;
@@ -1626,13 +1627,57 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
     }
 }
 
+// Orders the bins for the raster, most records first: bin b goes to class
+// c = bit length of its record count (0 for an empty bin) and takes a slot
+// in that class's list. Work per bin grows with its records, and crowded
+// bins (at 32-pixel bins the largest hold about four times the mean) would
+// otherwise finish last on an idle machine.
+__global__ void __launch_bounds__(256) k_bin_order(DrawParams p) {
+    __shared__ uint32_t s_count[kBinBuckets], s_base[kBinBuckets];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t bin = blockIdx.x * 256 + tid;
+    if (tid < kBinBuckets) s_count[tid] = 0;
+    __syncthreads();
+    uint32_t cls = 0, rank = 0;
+    const bool valid = bin < p.total_bins;
+    if (valid) {
+        const uint32_t n = min(p.bin_counts[bin], p.bin_capacity);
+        cls = n ? min(kBinBuckets - 1, 32u - (uint32_t)__clz(n)) : 0u;
+        rank = atomicAdd(&s_count[cls], 1u);
+    }
+    __syncthreads();
+    if (tid < kBinBuckets) s_base[tid] = s_count[tid] ? atomicAdd(&p.scratch->bucket[tid], s_count[tid]) : 0u;
+    __syncthreads();
+    if (valid) p.bin_order[(size_t)cls * p.total_bins + s_base[cls] + rank] = bin;
+}
+
+// One block per bin; block i rasterizes the i-th bin of the heavy-first
+// order, so the blocks dispatched first (the hardware starts blocks in
+// index order) get the crowded bins. (Taking the position from a ticket
+// counter instead measured much slower: every block start then waits for one
+// contended atomic.)
 template <int BS, int kThreads, bool Alpha>
 __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
     __shared__ unsigned long long keys[BS * BS];
     __shared__ TriLds tl[kThreads];
     __shared__ uint32_t s_wave_total[kThreads / kWave];
     __shared__ uint32_t talpha[Alpha ? kThreads : 1];
-    raster_bin<BS, kThreads, Alpha>(p, blockIdx.x, keys, tl, s_wave_total, talpha);
+    __shared__ uint32_t s_bin;
+    if (threadIdx.x == 0) {
+        uint32_t item = blockIdx.x;
+        uint32_t bin = item;        // (every position is in a class)
+        for (int c = (int)kBinBuckets - 1; c >= 0; c--) {
+            const uint32_t n = p.scratch->bucket[c];
+            if (item < n) {
+                bin = p.bin_order[(size_t)c * p.total_bins + item];
+                break;
+            }
+            item -= n;
+        }
+        s_bin = bin;
+    }
+    __syncthreads();
+    raster_bin<BS, kThreads, Alpha>(p, s_bin, keys, tl, s_wave_total, talpha);
 }
 
 // Clears every scratch counter: at the end of a direct draw (the binned
@@ -1643,6 +1688,7 @@ __global__ void k_reset_counters(DrawScratch* s) {
     s->num_clip = 0;
     s->num_visible = 0;
     s->num_spill = 0;
+    for (uint32_t c = 0; c < kBinBuckets; c++) s->bucket[c] = 0;
 }
 
 __global__ void k_clear(unsigned long long* keys, uint32_t n, unsigned long long key) {
@@ -1730,26 +1776,32 @@ __global__ void k_decode_overdraw(const unsigned long long* counts, const uint32
 // The bin raster for the launch's bin size and workgroup size: 128 or 256
 // threads for every bin size, 512 or 1024 for 32- and 64-pixel bins (a
 // crowded large bin then has more waves to spread its tiles over).
+template <int BS, int T, bool Alpha>
+void launch_bin_raster_one(const DrawParams& p, hipStream_t s) {
+    hipLaunchKernelGGL((k_bin_raster<BS, T, Alpha>), dim3(p.total_bins), dim3(T), 0, s, p);
+}
+
 template <int BS, bool Alpha>
-bool launch_bin_raster_bs(const DrawParams& p, dim3 grid, hipStream_t s) {
+bool launch_bin_raster_bs(const DrawParams& p, dim3, hipStream_t s) {
     switch (p.raster_threads) {
-    case 128: hipLaunchKernelGGL((k_bin_raster<BS, 128, Alpha>), grid, dim3(128), 0, s, p); return true;
-    case 256: hipLaunchKernelGGL((k_bin_raster<BS, 256, Alpha>), grid, dim3(256), 0, s, p); return true;
+    case 128: launch_bin_raster_one<BS, 128, Alpha>(p, s); return true;
+    case 256: launch_bin_raster_one<BS, 256, Alpha>(p, s); return true;
     case 512:
         if constexpr (BS >= 32) {
-            hipLaunchKernelGGL((k_bin_raster<BS, 512, Alpha>), grid, dim3(512), 0, s, p);
+            launch_bin_raster_one<BS, 512, Alpha>(p, s);
             return true;
         }
         return false;
     case 1024:
         if constexpr (BS >= 32) {
-            hipLaunchKernelGGL((k_bin_raster<BS, 1024, Alpha>), grid, dim3(1024), 0, s, p);
+            launch_bin_raster_one<BS, 1024, Alpha>(p, s);
             return true;
         }
         return false;
     default: return false;
     }
 }
+
 
 template <bool Alpha>
 bool launch_bin_raster(const DrawParams& p, dim3 grid, hipStream_t s) {
@@ -1915,6 +1967,10 @@ hipError_t launch_draw_binned(const DrawParams& p, hipStream_t s, Profiler* prof
         else
             hipLaunchKernelGGL(k_bin_scatter<false>, dim3(min(want, resident_blocks<k_bin_scatter<false>, 256>())),
                                dim3(256), 0, s, p);
+    }
+    {
+        Profiler::Scope sc(prof, "bin_order", s);
+        hipLaunchKernelGGL(k_bin_order, dim3((p.total_bins + 255) / 256), dim3(256), 0, s, p);
     }
     {
         Profiler::Scope sc(prof, "bin_raster", s);

@@ -117,6 +117,8 @@ struct Counters {                  // device counters, u64 each
     unsigned long long dropped;    // clipped triangles or fan records lost to a full queue (always counted)
 };
 
+constexpr uint32_t kBinBuckets = 16;
+
 // Per-draw scratch counters, cleared without a launch of their own:
 // num_visible is zero when a draw starts (allocation clears it, the bin
 // raster clears it after its last reader, a direct draw ends with a reset),
@@ -128,6 +130,7 @@ struct DrawScratch {
     uint32_t pad0;
     uint32_t num_spill;            // records that did not fit a bin
     uint32_t pad[3];
+    uint32_t bucket[kBinBuckets];  // bins per record-count class (log2), for heavy-first raster order
 };
 
 // Occlusion inputs of CullMeshlets (Shading.cpp:775-869): the previous
@@ -202,6 +205,7 @@ struct DrawParams {
     uint32_t raster_threads;       // bin raster workgroup size: 128 or 256 (512, 1024 for 32/64-pixel bins)
     uint32_t* bin_counts;          // [total_bins]
     uint32_t* bin_lists;           // [total_bins][bin_capacity]
+    uint32_t* bin_order;           // [kBinBuckets][total_bins]: bins by record-count class
     uint32_t* spill;               // [clip_record_base + clip_record_capacity]
 };
 
