@@ -907,7 +907,7 @@ __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p, int use_spi
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
     const uint32_t nchunks = num_chunks(p, use_spill);
-    const bool countFrags = counting(p) && !use_spill;
+    const bool countFrags = counting(p);
     const uint32_t waves = gridDim.x * 4;
 
     for (uint32_t c = blockIdx.x * 4 + wid; c < nchunks; c += waves) {
@@ -1325,7 +1325,12 @@ __device__ void bin_records(const DrawParams& p, bool valid, const TriRecord& re
     const uint32_t total = __shfl(incl, 63);
     entries += total;
     __builtin_amdgcn_wave_barrier();   // the previous call's reads of `sm` are done
-    sm.start[lane] = valid ? incl - nb : 0xFFFFFFFFu;
+    // Starts must not decrease with the lane for the search below: an
+    // invalid lane (nb = 0) gets the empty range at the running total. The
+    // valid lanes of a clipped triangle's fan need not be contiguous (setup
+    // culls some fan triangles), so a sentinel start here would hide the
+    // lanes after it.
+    sm.start[lane] = incl - nb;
     sm.org[lane] = org;
     sm.bw[lane] = bw | (pitch << 16);
     sm.spill[lane] = 0;
@@ -1378,7 +1383,10 @@ __device__ void bin_records(const DrawParams& p, bool valid, const TriRecord& re
         uint32_t base = 0;
         if (lane == 0) base = atomicAdd(&p.scratch->num_spill, (uint32_t)__popcll(spm));
         base = __shfl(base, 0);
-        if (spilled) p.spill[base + mbcnt(spm)] = ridx;
+        if (spilled) {
+            p.spill[base + mbcnt(spm)] = ridx;
+            p.records[ridx].flags = rec.flags | kRecSpilled;
+        }
     }
 }
 
@@ -1477,10 +1485,14 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
             load_tri(p.records[r], hw, hh, t, bmin, bmax);
             uint32_t cx0 = max((uint32_t)lo16(bmin), x0), cy0 = max((uint32_t)hi16(bmin), y0);
             uint32_t cx1 = min((uint32_t)lo16(bmax), x1), cy1 = min((uint32_t)hi16(bmax), y1);
-            if constexpr (Alpha) talpha[tid] = (p.records[r].flags & kRecAlpha) ? r : 0xFFFFFFFFu;
+            const uint32_t rflags = (Alpha || countFrags) ? p.records[r].flags : 0u;
+            if constexpr (Alpha) talpha[tid] = (rflags & kRecAlpha) ? r : 0xFFFFFFFFu;
             t.org = 0;
             t.tw = 1;
-            if (cx1 > cx0 && cy1 > cy0) {
+            // with fragment counting on, a spilled record is left to the
+            // spill raster, which draws and counts all of it (the result is
+            // the same: that raster runs after this one and keeps the max)
+            if (cx1 > cx0 && cy1 > cy0 && !(rflags & kRecSpilled)) {
                 t.org = (cx0 - x0) | ((cy0 - y0) << 16);
                 t.tw = (cx1 - cx0) >> 2;
                 ntiles = t.tw * ((cy1 - cy0) >> 2);

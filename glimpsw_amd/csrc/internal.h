@@ -49,7 +49,7 @@ struct TriRecord {
     uint32_t pos[3];               // snapped 1/16 px, x | y << 16 (s16 each)
     float z0, z10, z20;            // depth plane (Rasterizer.cpp:321-323)
     uint32_t rank;                 // draw order, see key layout
-    uint32_t flags;                // bit 0 alpha-tested (kRecAlpha), bits 16-31 view index
+    uint32_t flags;                // bit 0 alpha-tested (kRecAlpha), bit 1 spilled (kRecSpilled), bits 16-31 view index
 };
 static_assert(sizeof(TriRecord) == 32, "record size");
 
@@ -72,6 +72,10 @@ static_assert(sizeof(AlphaRecord) == 64, "alpha record size");
 
 // TriRecord::flags bit 0: the record is alpha-tested.
 constexpr uint32_t kRecAlpha = 1u;
+// TriRecord::flags bit 1: a bin the record covers was full, so the spill
+// raster draws the whole record; with fragment counting on, only that
+// raster draws and counts it (each fragment counted once).
+constexpr uint32_t kRecSpilled = 2u;
 
 // A triangle that needs guard-band clipping, with its clip-space vertices.
 struct ClipEntry {

@@ -171,14 +171,17 @@ def test_empty_draw_and_clear_values(gpu_ctx):
     ds.close()
 
 
-def test_counters_match_oracle(gpu_ctx):
+@pytest.mark.parametrize("bin_capacity", [0, 4])
+def test_counters_match_oracle(gpu_ctx, bin_capacity):
+    """Capacity 4 overflows most bins: the spilled triangles' fragments are
+    then counted by the spill raster alone, still once each."""
     import glimpsw_amd as g
 
     sc = T.small_scene(384, seed_offset=2, spread=6.0, near_frac=0.25)
     view = S.make_view(320, 240, count=sc.count)
     o = _oracle_frame(sc, view)
     gpu_ctx.reset_counters()
-    _gpu_frame(gpu_ctx, sc, view, flags=g.GLIMP_DRAW_COUNTERS)
+    _gpu_frame(gpu_ctx, sc, view, flags=g.GLIMP_DRAW_COUNTERS, bin_size=16, bin_capacity=bin_capacity)
     c = gpu_ctx.counters()
     for k in ("triangles_processed", "triangles_rasterized", "triangles_clipped", "fragments_covered"):
         assert c[k] == o[2][k], (k, c[k], o[2][k])
