@@ -674,10 +674,11 @@ int prepare_scratch(Scratch& S, size_t total_meshlets, size_t total_bins, const 
     if (rc == GLIMP_OK) rc = grow_one(S.clip, S.clip_cap, clipq);
     if (rc == GLIMP_OK) rc = grow_one(S.visible_list, S.visible_list_cap, std::max<size_t>(total_meshlets, 1));
     if (rc == GLIMP_OK) rc = grow_one(S.meshlet_counts, S.meshlet_counts_cap, total_meshlets);
-    uint32_t* const old_counts = S.bin_counts;
+    const size_t old_counts_cap = S.bin_counts_cap;
     if (rc == GLIMP_OK) rc = grow_one(S.bin_counts, S.bin_counts_cap, std::max<size_t>(total_bins, 1));
-    // draws leave the bin counts cleared; new ones start cleared
-    if (rc == GLIMP_OK && S.bin_counts != old_counts)
+    // draws leave the bin counts cleared; a regrown buffer starts cleared (the
+    // test is on the capacity: hipMalloc may hand back the freed address)
+    if (rc == GLIMP_OK && S.bin_counts_cap != old_counts_cap)
         HIP_TRY(hipMemset(S.bin_counts, 0, S.bin_counts_cap * sizeof(uint32_t)));
     if (rc == GLIMP_OK && binned) rc = grow_one(S.bin_lists, S.bin_lists_cap, total_bins * sh.bin_capacity);
     if (rc == GLIMP_OK && binned) rc = grow_one(S.bin_order, S.bin_order_cap, total_bins * kBinBuckets);

@@ -264,3 +264,34 @@ def test_mixed_direct_and_binned_draws(gpu_ctx):
     _assert_same(ids, depth, o[0], o[1], "mixed")
     fb.close()
     ds.close()
+
+
+def test_growing_bin_grid_starts_cleared():
+    """Draws on one context with ever more bins: each draw regrows the bin
+    count buffer, which must start at zero even when the allocator hands back
+    memory just freed with other contents (here a framebuffer full of keys,
+    freed right before)."""
+    import glimpsw_amd as g
+
+    ctx = g.Context(0)
+    try:
+        sc = T.small_scene(256, seed_offset=7, spread=6.0, near_frac=0.1)
+        ds = g.Scene(ctx, sc)
+        for w, h in ((64, 48), (128, 96), (200, 180), (320, 240), (480, 368)):
+            junk = g.Framebuffer(ctx, w * 2, h * 2)
+            junk.clear(0, 0.0)
+            jv = S.make_view(w * 2, h * 2, count=sc.count)
+            junk.draw(ds, jv.object_to_clip, sc.count, 0, jv.frustum_planes())
+            junk.read()
+            junk.close()
+            view = S.make_view(w, h, count=sc.count)
+            o = _oracle_frame(sc, view)
+            fb = g.Framebuffer(ctx, w, h)
+            fb.clear(0, 0.0)
+            fb.draw(ds, view.object_to_clip, sc.count, 0, view.frustum_planes(), 0, bin_size=8, bin_capacity=64)
+            ids, depth = fb.read()
+            fb.close()
+            _assert_same(ids, depth, o[0], o[1], f"{w}x{h}")
+        ds.close()
+    finally:
+        ctx.close()
