@@ -110,3 +110,17 @@ def test_random_scenes_in_batches(gpu_ctx, shape):
                 fb.close()
             for ds in scenes:
                 ds.close()
+
+
+def test_random_scenes_deferred_and_overdraw(gpu_ctx):
+    """Every third random case through the deferred pipeline (three G-buffer
+    layers) and the overdraw pipeline (counts, depth maximum, counters and
+    both overdraw resolves), against the oracle."""
+    from tests import test_gpu_deferred as D
+    from tests import test_gpu_overdraw as O
+
+    for seed in range(0, len(SEEDS), 3):
+        case = R.make_case(seed)
+        gpu_ctx.reset_counters()
+        D._check(f"random seed {seed}", gpu_ctx, case.scene, case.view, frames=[(0x0BADF00D, case.draws)])
+        O._check(f"random seed {seed}", gpu_ctx, case.scene, case.view, 0, tuple(case.draws))
