@@ -618,6 +618,16 @@ struct LaunchShape {
 // context's `dropped` counter (glimp_get_counters).
 size_t clip_capacity_for(size_t meshlets) { return std::max<size_t>(4096, meshlets * 2); }
 
+// A batch's views share one queue: as large as theirs drawn one at a time
+// would be together, up to 2^20 triangles (64 MiB of queue, 320 MiB of fan
+// records), and never smaller than the single-draw rule for all its meshlets.
+constexpr size_t kBatchClipCap = size_t(1) << 20;
+size_t batch_clip_capacity(const std::vector<glimp_view_desc>& views, size_t total_meshlets) {
+    size_t sum = 0;
+    for (const auto& d : views) sum = std::min(sum + clip_capacity_for(d.meshlet_count), kBatchClipCap);
+    return std::max(clip_capacity_for(total_meshlets), sum);
+}
+
 // One view of a draw (ShadingContext + DrawMeshlets arguments).
 void make_view(glimp_framebuffer* f, glimp_scene* s, uint32_t meshlet_offset, uint32_t meshlet_count,
                const float* clip, const float* planes, uint32_t flags, ViewParams& V, const Occlusion* occ = nullptr) {
@@ -666,9 +676,8 @@ uint64_t count_bins(const std::vector<ViewParams>& views, uint32_t bin_size) {
     return n;
 }
 
-int prepare_scratch(Scratch& S, size_t total_meshlets, size_t total_bins, const LaunchShape& sh, bool binned,
-                    bool alpha) {
-    const size_t clipq = clip_capacity_for(total_meshlets);
+int prepare_scratch(Scratch& S, size_t total_meshlets, size_t clipq, size_t total_bins, const LaunchShape& sh,
+                    bool binned, bool alpha) {
     const size_t recs = total_meshlets * 128 + clipq * 8;
     int rc = grow_one(S.records, S.records_cap, recs);
     if (rc == GLIMP_OK) rc = grow_one(S.clip, S.clip_cap, clipq);
@@ -691,7 +700,7 @@ int prepare_scratch(Scratch& S, size_t total_meshlets, size_t total_bins, const 
     return rc;
 }
 
-void fill_params(glimp_context* c, const Scratch& S, size_t total_meshlets, uint32_t total_bins,
+void fill_params(glimp_context* c, const Scratch& S, size_t total_meshlets, size_t clipq, uint32_t total_bins,
                  const LaunchShape& sh, uint32_t flags, bool alpha, DrawParams& p) {
     p.total_meshlets = (uint32_t)total_meshlets;
     p.total_bins = total_bins;
@@ -704,7 +713,6 @@ void fill_params(glimp_context* c, const Scratch& S, size_t total_meshlets, uint
     p.clip_record_base = (uint32_t)(total_meshlets * 128);
     // the draw's own capacities (not what earlier, larger draws allocated), so
     // what a full queue drops does not depend on the context's history
-    const size_t clipq = clip_capacity_for(total_meshlets);
     p.clip_record_capacity =
         (uint32_t)std::min<size_t>(std::min(clipq * 8, S.records_cap - total_meshlets * 128), 0x7FFFFFFFu);
     p.clip_queue = S.clip;
@@ -791,11 +799,12 @@ int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, 
               d->occlusion ? &occ : nullptr);
     const uint32_t total_bins = direct ? 0 : layout_bins(views, sh.bin_size);
     const bool alpha = (s->has_alpha && !overdraw) || deferred;    // FS_Overdraw has no alpha test
-    int rc = prepare_scratch(c->draw_scratch, d->meshlet_count, total_bins, sh, !direct, alpha);
+    const size_t clipq = clip_capacity_for(d->meshlet_count);
+    int rc = prepare_scratch(c->draw_scratch, d->meshlet_count, clipq, total_bins, sh, !direct, alpha);
     if (rc != GLIMP_OK) return rc;
     DrawParams p;
     memset(&p, 0, sizeof(p));
-    fill_params(c, c->draw_scratch, d->meshlet_count, total_bins, sh, d->flags, alpha, p);
+    fill_params(c, c->draw_scratch, d->meshlet_count, clipq, total_bins, sh, d->flags, alpha, p);
     p.num_views = 1;
     p.views = nullptr;
     p.view = views[0];
@@ -822,6 +831,7 @@ struct glimp_batch {
     std::vector<ViewParams> views;
     ViewParams* d_views = nullptr;
     size_t total_meshlets = 0;
+    size_t clip_triangles = 0;         // clip queue capacity (batch_clip_capacity)
     uint32_t total_bins = 0;
     LaunchShape shape;
     bool alpha = false;                // a view's scene has alpha-tested materials
@@ -856,7 +866,8 @@ int layout_batch(glimp_batch* b) {
     b->total_bins = layout_bins(b->views, b->shape.bin_size);
     b->alpha = false;
     for (const auto& d : b->descs) b->alpha = b->alpha || d.scene->has_alpha;
-    int rc = prepare_scratch(b->scratch, b->total_meshlets, b->total_bins, b->shape, true, b->alpha);
+    int rc = prepare_scratch(b->scratch, b->total_meshlets, b->clip_triangles, b->total_bins, b->shape, true,
+                             b->alpha);
     if (rc != GLIMP_OK) return rc;
     if (!b->d_views) HIP_TRY(dev_alloc(&b->d_views, b->views.size()));
     HIP_TRY(hipMemcpy(b->d_views, b->views.data(), b->views.size() * sizeof(ViewParams), hipMemcpyHostToDevice));
@@ -866,7 +877,7 @@ int layout_batch(glimp_batch* b) {
 
 void batch_params(glimp_batch* b, DrawParams& p) {
     memset(&p, 0, sizeof(p));
-    fill_params(b->ctx, b->scratch, b->total_meshlets, b->total_bins, b->shape, 0, b->alpha, p);
+    fill_params(b->ctx, b->scratch, b->total_meshlets, b->clip_triangles, b->total_bins, b->shape, 0, b->alpha, p);
     p.num_views = (uint32_t)b->views.size();
     p.views = b->d_views;
 }
@@ -955,6 +966,7 @@ int glimp_batch_create(glimp_context* c, const glimp_view_desc* views, uint32_t 
         base += d.meshlet_count;
     }
     b->total_meshlets = base;
+    b->clip_triangles = batch_clip_capacity(b->descs, base);
     int rc = layout_batch(b);
     hipError_t e = hipSuccess;
     if (rc == GLIMP_OK) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);

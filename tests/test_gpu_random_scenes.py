@@ -68,10 +68,11 @@ def test_random_scenes(gpu_ctx):
 def test_random_scenes_in_batches(gpu_ctx, shape):
     """The same random cases as views of batches (one draw each, every view
     its own scene, framebuffer size, cull planes and clipping flags),
-    launched eagerly and replayed as a graph. The cases go into 12 batches
-    of 5 views (views k, k+12, ...) so that no batch queues more than its
-    4096 clipped triangles (the sixty cases together queue about 11,800,
-    and a batch shares one clip queue among its views)."""
+    launched eagerly and replayed as a graph: first all sixty in one batch
+    (together they queue about 11,800 clipped triangles, more than the 4,096
+    of one small draw's queue; a batch's shared queue is as large as its
+    views' queues drawn one at a time), then 12 batches of 5 views (views k,
+    k+12, ...)."""
     import dataclasses
 
     import glimpsw_amd as g
@@ -79,8 +80,7 @@ def test_random_scenes_in_batches(gpu_ctx, shape):
     cases = [dataclasses.replace(c, draws=[(0, c.scene.count)], clear_depth=0.0)
              for c in (R.make_case(s) for s in SEEDS)]
     wants = [_oracle(c)[:2] for c in cases]
-    for k in range(12):
-        sel = list(range(k, len(cases), 12))
+    for sel in [list(range(len(cases)))] + [list(range(k, len(cases), 12)) for k in range(12)]:
         items, scenes, fbs = [], [], []
         try:
             for i in sel:
