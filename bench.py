@@ -61,7 +61,6 @@ def algorithmic_bytes(stats: dict) -> dict:
         "bin_scatter": 32 * S + 4 * P + 64 * Cq,
         # entries and records in, every pixel's key written once
         "bin_raster": 4 * P + 32 * S + 8 * N,
-        "raster_spill": 0,
         # keys in, RGBA8 out; vertex data (positions, indices, UVs, normals)
         # of the visible meshlets once; one RGBA8 texel per covered pixel
         # (the mip level is chosen for about one texel per pixel)
@@ -72,7 +71,7 @@ def algorithmic_bytes(stats: dict) -> dict:
 # profiler scope name -> kernel symbol substring in rocprofv3 output
 KERNEL_SYMBOLS = {
     "clear": "k_clear(", "cull": "k_cull<", "mesh_setup": "k_mesh_setup<", "clip": "k_clip<",
-    "bin_scatter": "k_bin_scatter<", "bin_raster": "k_bin_raster<", "raster_spill": "k_raster_direct<",
+    "bin_scatter": "k_bin_scatter<", "bin_raster": "k_bin_raster<",
     "resolve_lit": "k_resolve<1>",
 }
 

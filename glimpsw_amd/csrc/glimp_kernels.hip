@@ -10,10 +10,12 @@
 //                 meshlet's own slots, guard-band cases go to a clip queue
 //   k_clip        one wave per queued triangle: wave-parallel
 //                 Sutherland-Hodgman, fan triangulation, setup
-//   k_bin_scatter (record, screen bin) pairs into fixed-capacity bins
-//   k_bin_raster  one workgroup per bin: LDS-resident 64-bit keys
-//   k_raster_direct  records that found a bin full (or, without binning,
-//                 all records): global 64-bit atomicMax
+//   k_bin_scatter (record, screen bin) pairs into fixed-capacity bins; a
+//                 record that finds a bin full goes to the spill list
+//   k_bin_order   bins ordered by record count, most first
+//   k_bin_raster  one workgroup per bin: LDS-resident 64-bit keys, from the
+//                 bin's list and the spilled records that overlap the bin
+//   k_raster_direct  without binning, all records: global 64-bit atomicMax
 //
 // Depth test. The reference keeps the nearest fragment (reverse-Z, strictly
 // greater depth wins, Shading.cpp:313) and, with one thread, resolves ties in
@@ -740,30 +742,21 @@ __global__ void __launch_bounds__(256) k_clip(DrawParams p) {
 // ---------------------------------------------------------------- record chunks
 
 // Chunks of up to 64 records for a wave: chunk c < 2*num_visible is half
-// c&1 of visible meshlet slot c>>1; later chunks cover the clipped records;
-// in spill mode chunk c covers spill-list entries [64c, 64c+64).
+// c&1 of visible meshlet slot c>>1; later chunks cover the clipped records.
 struct Chunk {
     uint32_t base, count;
     const uint32_t* list;      // non-null: record index = list[base + i]
 };
 
-__device__ __forceinline__ uint32_t num_chunks(const DrawParams& p, int spill, bool with_clipped = true) {
-    if (spill) return (min(p.scratch->num_spill, p.clip_record_base + p.clip_record_capacity) + 63) / 64;
+__device__ __forceinline__ uint32_t num_chunks(const DrawParams& p, bool with_clipped = true) {
     if (!with_clipped) return 2 * p.scratch->num_visible;
     uint32_t nclip = min(p.scratch->num_records, p.clip_record_capacity);
     return 2 * p.scratch->num_visible + (nclip + 63) / 64;
 }
 
-__device__ __forceinline__ Chunk get_chunk(const DrawParams& p, int spill, uint32_t c) {
+__device__ __forceinline__ Chunk get_chunk(const DrawParams& p, uint32_t c) {
     Chunk k;
     k.list = nullptr;
-    if (spill) {
-        uint32_t n = min(p.scratch->num_spill, p.clip_record_base + p.clip_record_capacity);
-        k.base = c * 64;
-        k.count = min(64u, n - k.base);
-        k.list = p.spill;
-        return k;
-    }
     const uint32_t nvis = p.scratch->num_visible;
     if (c < 2 * nvis) {
         uint32_t i = c >> 1, half = c & 1;
@@ -889,8 +882,8 @@ __device__ __forceinline__ bool alpha_test_tile(const ApproxTables& T, const Alp
     return color >= ((A.tex >> 24) << 24);
 }
 
-// Each wave takes a chunk of up to 64 records (meshlet slots, clipped
-// records, or bin spill entries), lays their render boxes out as one list of
+// Each wave takes a chunk of up to 64 records (meshlet slots or clipped
+// records), lays their render boxes out as one list of
 // 4x4 tiles (exclusive scan of tile counts) and walks it four tiles at a
 // time: lane = pixel, 16 lanes = one tile, so a tile's 16 keys (128 bytes)
 // are one contiguous access. Covered pixels update the key of their view's
@@ -900,18 +893,18 @@ __device__ __forceinline__ bool alpha_test_tile(const ApproxTables& T, const Alp
 // a function of the triangle and the tile alone, so the order does not
 // change which fragments pass).
 template <bool Alpha>
-__global__ void __launch_bounds__(256) k_raster_direct(DrawParams p, int use_spill) {
+__global__ void __launch_bounds__(256) k_raster_direct(DrawParams p) {
     __shared__ TriLds tl[4][64];
     __shared__ uint32_t tview[4][64];
     __shared__ uint32_t talpha[Alpha ? 4 : 1][64];   // record index of an alpha-tested record, else ~0
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
-    const uint32_t nchunks = num_chunks(p, use_spill);
+    const uint32_t nchunks = num_chunks(p);
     const bool countFrags = counting(p);
     const uint32_t waves = gridDim.x * 4;
 
     for (uint32_t c = blockIdx.x * 4 + wid; c < nchunks; c += waves) {
-        const Chunk ck = get_chunk(p, use_spill, c);
+        const Chunk ck = get_chunk(p, c);
         if (ck.count == 0) continue;
         uint32_t ntiles = 0;
         TriLds t;
@@ -1027,13 +1020,13 @@ __global__ void __launch_bounds__(256) k_raster_overdraw(DrawParams p) {
     __shared__ uint32_t tview[4][64];
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
-    const uint32_t nchunks = num_chunks(p, 0);
+    const uint32_t nchunks = num_chunks(p);
     const bool countFrags = counting(p);
     const uint32_t waves = gridDim.x * 4;
     const unsigned long long groupMask = 0xFFFFull << (lane & 48u);
 
     for (uint32_t c = blockIdx.x * 4 + wid; c < nchunks; c += waves) {
-        const Chunk ck = get_chunk(p, 0, c);
+        const Chunk ck = get_chunk(p, c);
         if (ck.count == 0) continue;
         uint32_t ntiles = 0;
         TriLds t;
@@ -1162,13 +1155,13 @@ __global__ void __launch_bounds__(256) k_gbuffer(DrawParams p) {
     __shared__ uint32_t talpha[4][64];
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
-    const uint32_t nchunks = num_chunks(p, 0);
+    const uint32_t nchunks = num_chunks(p);
     const uint32_t waves = gridDim.x * 4;
     const unsigned long long groupMask = 0xFFFFull << (lane & 48u);
     const ApproxTables& AT = p.tables;
 
     for (uint32_t c = blockIdx.x * 4 + wid; c < nchunks; c += waves) {
-        const Chunk ck = get_chunk(p, 0, c);
+        const Chunk ck = get_chunk(p, c);
         if (ck.count == 0) continue;
         uint32_t ntiles = 0;
         TriLds t;
@@ -1398,12 +1391,12 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
     __shared__ BinScratch s_bins[4];
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
-    const uint32_t nchunks = num_chunks(p, 0, false);
+    const uint32_t nchunks = num_chunks(p, false);
     const uint32_t waves = gridDim.x * 4;
     const uint32_t first = blockIdx.x * 4 + wid;
     unsigned long long entries = 0;
     for (uint32_t c = first; c < nchunks; c += waves) {
-        const Chunk ck = get_chunk(p, 0, c);
+        const Chunk ck = get_chunk(p, c);
         if (ck.count == 0) continue;
         const bool valid = lane < ck.count;
         TriRecord rec;
@@ -1441,7 +1434,8 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
                            uint32_t (&talpha)[Alpha ? kThreads : 1]) {
     constexpr int kWaves = kThreads / kWave;
     constexpr int kGroups = kThreads / 16;
-    const ViewParams& V = view_of(p, view_of_bin(p, bin));
+    const uint32_t vidx = view_of_bin(p, bin);
+    const ViewParams& V = view_of(p, vidx);
     const uint32_t lbin = bin - V.bin_base;
     const uint32_t bx = lbin % V.bins_x, by = lbin / V.bins_x;
     const uint32_t W = (uint32_t)V.width, H = (uint32_t)V.height;
@@ -1458,7 +1452,13 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
     // the visible count has had its last reader (the scatter): leave it
     // cleared for the next draw's cull
     if (bin == 0 && tid == 0) p.scratch->num_visible = 0;
-    if (n == 0 && !V.fresh) return;  // uniform across the block
+    // Records that overflowed a bin (the spill list, usually empty) are drawn
+    // by every bin block they overlap, after the bin's own list; their list
+    // entries elsewhere are skipped, so each is drawn (and counted) once per
+    // pixel.
+    const uint32_t ns = min(p.scratch->num_spill, p.clip_record_base + p.clip_record_capacity);
+    const uint32_t nall = n + ns;
+    if (nall == 0 && !V.fresh) return;  // uniform across the block
     const int hw = V.width / 2, hh = V.height / 2;
     const unsigned long long clearKey = (unsigned long long)fbits(V.clear_depth) << 32;
     const uint32_t rowElems = (x1 - x0) * 4;   // elements of one 4-row strip of tiles
@@ -1475,24 +1475,25 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
     const bool countFrags = counting(p);
     uint32_t covered = 0, passed = 0;
     const uint32_t g = tid >> 4, px = tid & 3, py = (tid >> 2) & 3;
-    for (uint32_t base = 0; base < n; base += kThreads) {
+    for (uint32_t base = 0; base < nall; base += kThreads) {
         const uint32_t e = base + tid;
         uint32_t ntiles = 0;
         TriLds t;
-        if (e < n) {
-            const uint32_t r = base == 0 ? first_entry : list[e];
+        if (e < nall) {
+            const bool fromSpill = e >= n;
+            const uint32_t r = fromSpill ? p.spill[e - n] : (base == 0 ? first_entry : list[e]);
             uint32_t bmin, bmax;
             load_tri(p.records[r], hw, hh, t, bmin, bmax);
             uint32_t cx0 = max((uint32_t)lo16(bmin), x0), cy0 = max((uint32_t)hi16(bmin), y0);
             uint32_t cx1 = min((uint32_t)lo16(bmax), x1), cy1 = min((uint32_t)hi16(bmax), y1);
-            const uint32_t rflags = (Alpha || countFrags) ? p.records[r].flags : 0u;
+            const uint32_t rflags = p.records[r].flags;
             if constexpr (Alpha) talpha[tid] = (rflags & kRecAlpha) ? r : 0xFFFFFFFFu;
             t.org = 0;
             t.tw = 1;
-            // with fragment counting on, a spilled record is left to the
-            // spill raster, which draws and counts all of it (the result is
-            // the same: that raster runs after this one and keeps the max)
-            if (cx1 > cx0 && cy1 > cy0 && !(rflags & kRecSpilled)) {
+            // a spilled record comes from the spill list (of any view: only
+            // this bin's view draws it), never from a bin list
+            const bool skip = fromSpill ? (rflags >> 16) != vidx : (rflags & kRecSpilled) != 0;
+            if (cx1 > cx0 && cy1 > cy0 && !skip) {
                 t.org = (cx0 - x0) | ((cy0 - y0) << 16);
                 t.tw = (cx1 - cx0) >> 2;
                 ntiles = t.tw * ((cy1 - cy0) >> 2);
@@ -1515,7 +1516,7 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
         }
         t.start = prev + incl - ntiles;
         tl[tid] = t;
-        if (e >= n) tl[tid].start = 0xFFFFFFFFu;
+        if (e >= nall) tl[tid].start = 0xFFFFFFFFu;
         __syncthreads();
 
         // group g walks tiles [j, j1) of the block's tile list
@@ -1903,7 +1904,7 @@ hipError_t launch_draw_deferred(const DrawParams& p, hipStream_t s, Profiler* pr
     {
         Profiler::Scope sc(prof, "raster_direct", s);
         hipLaunchKernelGGL(k_raster_direct<true>, dim3(min(want, resident_blocks<k_raster_direct<true>, 256>())),
-                           dim3(256), 0, s, p, 0);
+                           dim3(256), 0, s, p);
     }
     {
         Profiler::Scope sc(prof, "gbuffer", s);
@@ -1958,10 +1959,10 @@ hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s, Profiler* prof
         const uint32_t want = max((2 * p.total_meshlets + 3) / 4 + 16, 64u);
         if (p.alpha)
             hipLaunchKernelGGL(k_raster_direct<true>, dim3(min(want, resident_blocks<k_raster_direct<true>, 256>())),
-                               dim3(256), 0, s, p, 0);
+                               dim3(256), 0, s, p);
         else
             hipLaunchKernelGGL(k_raster_direct<false>, dim3(min(want, resident_blocks<k_raster_direct<false>, 256>())),
-                               dim3(256), 0, s, p, 0);
+                               dim3(256), 0, s, p);
     }
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, s, p.scratch);
     return hipGetLastError();
@@ -1993,11 +1994,6 @@ hipError_t launch_draw_binned(const DrawParams& p, hipStream_t s, Profiler* prof
         } else {
             if (!launch_bin_raster<false>(p, grid, s)) return hipErrorInvalidValue;
         }
-    }
-    {
-        Profiler::Scope sc(prof, "raster_spill", s);
-        if (p.alpha) hipLaunchKernelGGL(k_raster_direct<true>, dim3(256), dim3(256), 0, s, p, 1);
-        else hipLaunchKernelGGL(k_raster_direct<false>, dim3(256), dim3(256), 0, s, p, 1);
     }
     return hipGetLastError();
 }

@@ -72,9 +72,9 @@ static_assert(sizeof(AlphaRecord) == 64, "alpha record size");
 
 // TriRecord::flags bit 0: the record is alpha-tested.
 constexpr uint32_t kRecAlpha = 1u;
-// TriRecord::flags bit 1: a bin the record covers was full, so the spill
-// raster draws the whole record; with fragment counting on, only that
-// raster draws and counts it (each fragment counted once).
+// TriRecord::flags bit 1: a bin the record covers was full, so the record is
+// on the spill list, from which every bin it overlaps draws it; bin lists
+// that hold it skip it (each fragment drawn and counted once).
 constexpr uint32_t kRecSpilled = 2u;
 
 // A triangle that needs guard-band clipping, with its clip-space vertices.
