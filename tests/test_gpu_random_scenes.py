@@ -14,7 +14,7 @@ from tests import random_scenes as R
 
 pytestmark = pytest.mark.gpu
 
-SEEDS = range(60)
+SEEDS = range(120)
 
 
 def _oracle(case):
@@ -68,11 +68,10 @@ def test_random_scenes(gpu_ctx):
 def test_random_scenes_in_batches(gpu_ctx, shape):
     """The same random cases as views of batches (one draw each, every view
     its own scene, framebuffer size, cull planes and clipping flags),
-    launched eagerly and replayed as a graph: first all sixty in one batch
-    (together they queue about 11,800 clipped triangles, more than the 4,096
-    of one small draw's queue; a batch's shared queue is as large as its
-    views' queues drawn one at a time), then 12 batches of 5 views (views k,
-    k+12, ...)."""
+    launched eagerly and replayed as a graph: first all of them in one batch
+    (together they queue far more clipped triangles than the 4,096 of one
+    small draw's queue; a batch's shared queue is as large as its views'
+    queues drawn one at a time), then 12 batches of every twelfth view."""
     import dataclasses
 
     import glimpsw_amd as g
