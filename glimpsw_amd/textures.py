@@ -93,11 +93,18 @@ def pack_f32(c: np.ndarray) -> np.ndarray:
 
 
 def next_mip(level: np.ndarray) -> np.ndarray:
-    c = unpack_f32(level)
-    c00, c10 = c[0::2, 0::2], c[0::2, 1::2]
-    c01, c11 = c[1::2, 0::2], c[1::2, 1::2]
-    avg = (((c00 + c10) + c01) + c11) * np.float32(0.25)
-    return pack_f32(avg)
+    """GenerateMip's box filter for RGBA8u (Texture.h:577-596) in the operation
+    order the reference's optimised build gives `Pack((Unpack(c00) + ... +
+    Unpack(c11)) * 0.25f)`: the 1/255 of Unpack, the 0.25 and the 255 of Pack
+    fold into one multiply by 0.25 and the four bytes of a channel are summed
+    as integers, so each channel is round-half-even(sum / 4)."""
+    t = level.astype(np.uint32)
+    q = (t[0::2, 0::2], t[0::2, 1::2], t[1::2, 0::2], t[1::2, 1::2])
+    out = np.zeros(q[0].shape, np.uint32)
+    for k in range(4):
+        s = sum(((x >> np.uint32(8 * k)) & np.uint32(255)) for x in q).astype(np.float32)
+        out |= np.clip(np.rint(s * np.float32(0.25)), 0, 255).astype(np.uint32) << np.uint32(8 * k)
+    return out
 
 
 def pack_r11g11b10(c: np.ndarray) -> np.ndarray:

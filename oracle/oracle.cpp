@@ -1062,18 +1062,11 @@ float light_attenuation(const orc_light& L, const float pos[3]) {  // Shading.cp
 // EvalLighting for one 4x4 tile (Shading.cpp:602-645): Lambert diffuse,
 // GGX/Smith/Schlick specular, 5% ambient. The per-light skips are taken over
 // the tile's covered lanes, as the reference takes them over its 16 lanes.
-void lit_tile(const SurfaceLane* lanes, const bool* active, uint32_t x0, uint32_t y0, const float* tileDepth,
-              const orc_resolve_params* p, float exposure, float color[16][3]) {
+void lit_tile_at(const SurfaceLane* lanes, const bool* active, const float (*wpos)[3], const orc_resolve_params* p,
+                 float exposure, float color[16][3]) {
     float base[16][3], f0[16][3], diffuse[16][3], N[16][3], V[16][3], NoV[16], aR[16], metal[16];
-    float wpos[16][3];
     for (int l = 0; l < 16; l++) {
         if (!active[l]) continue;
-        // world position from the inverse screen projection (Shading.cpp:659-667)
-        float px = (float)(int32_t)(x0 + (l & 3)), py = (float)(int32_t)(y0 + (l >> 2));
-        float c[4];
-        transform(p->inv_screen_proj, px, py, tileDepth[l], c);
-        float rw = frcp(c[3]);
-        wpos[l][0] = c[0] * rw; wpos[l][1] = c[1] * rw; wpos[l][2] = c[2] * rw;
         float srgb[4];
         unpack_srgb(lanes[l].albedo, srgb);
         // SurfaceData::Metallic / Roughness (Shading.cpp:574-577), 0 without an NMR layer
@@ -1143,6 +1136,21 @@ void lit_tile(const SurfaceLane* lanes, const bool* active, uint32_t x0, uint32_
         if (!active[l]) continue;
         for (int k = 0; k < 3; k++) color[l][k] = color[l][k] + base[l][k] * 0.05f;
     }
+}
+
+void lit_tile(const SurfaceLane* lanes, const bool* active, uint32_t x0, uint32_t y0, const float* tileDepth,
+              const orc_resolve_params* p, float exposure, float color[16][3]) {
+    float wpos[16][3] = {};
+    for (int l = 0; l < 16; l++) {
+        if (!active[l]) continue;
+        // world position from the inverse screen projection (Shading.cpp:659-667)
+        float px = (float)(int32_t)(x0 + (l & 3)), py = (float)(int32_t)(y0 + (l >> 2));
+        float c[4];
+        transform(p->inv_screen_proj, px, py, tileDepth[l], c);
+        float rw = frcp(c[3]);
+        wpos[l][0] = c[0] * rw; wpos[l][1] = c[1] * rw; wpos[l][2] = c[2] * rw;
+    }
+    lit_tile_at(lanes, active, wpos, p, exposure, color);
 }
 
 // RGBA8u::Pack of (r,g,b,a) (Texture.h:55-67): RNE, saturating packs.
@@ -1895,6 +1903,37 @@ void orc_resolve_lit(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_
     }
     if (p->light_sprites) draw_light_sprites(out.data(), depth, W, H, p);
     memcpy(fb->data, out.data(), (size_t)W * H * 4);
+}
+
+void orc_eval_lighting16(const uint32_t* albedo, const uint32_t* nmr, const float* normal, const float* wpos,
+                         const uint8_t* active, const orc_resolve_params* p, uint32_t* out) {
+    SurfaceLane lanes[16] = {};
+    bool act[16];
+    float wp[16][3];
+    for (int l = 0; l < 16; l++) {
+        act[l] = active[l] != 0;
+        lanes[l].albedo = albedo[l];
+        lanes[l].nmr = nmr[l];
+        for (int k = 0; k < 3; k++) {
+            lanes[l].nrm[k] = normal[k * 16 + l];
+            wp[l][k] = wpos[k * 16 + l];
+        }
+    }
+    float color[16][3] = {};
+    lit_tile_at(lanes, act, wp, p, p->exposure * 0.001f, color);
+    for (int l = 0; l < 16; l++) {
+        float c[3];
+        for (int k = 0; k < 3; k++) {
+            // Tonemap_Unreal(x * Exposure) (Shading.cpp:221-226, 680)
+            float x = act[l] ? color[l][k] * p->exposure : 0.0f;
+            c[k] = fdiv(x, x + 0.155f) * 1.019f;
+        }
+        out[l] = pack_rgba8(c[0], c[1], c[2], 1.0f);
+    }
+}
+
+void orc_unpack_srgb_n(const uint32_t* packed, float* out, uint64_t n) {
+    for (uint64_t i = 0; i < n; i++) unpack_srgb(packed[i], out + 4 * i);
 }
 
 // ---------------------------------------------------------------- differentiable resolve

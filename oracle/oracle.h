@@ -171,6 +171,15 @@ void orc_gbuffer_pack16(const float* n, const uint32_t* nmr, uint32_t* out);
 void orc_overdraw_colour16(const uint32_t* data, int32_t quad, float* out);
 /* Resolve(): lit, tonemapped RGBA8 into layer 0. */
 void orc_resolve_lit(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_resolve_params* p);
+/* The lit colour of one 4x4 tile at given surfaces (Shading.cpp:602-645,
+ * 221-226, 680): albedo and layer-1 (normal/metallic/roughness) texels,
+ * world normals [3][16] and world positions [3][16] of the 16 lanes, lanes
+ * with active[l] != 0 lit, from p's lights, view position and exposure; out =
+ * Tonemap_Unreal(colour * exposure) packed to RGBA8 (inactive lanes 0). */
+void orc_eval_lighting16(const uint32_t* albedo, const uint32_t* nmr, const float* normal, const float* wpos,
+                         const uint8_t* active, const orc_resolve_params* p, uint32_t* out);
+/* RGBA8u::UnpackSrgb (Texture.h:37-54) of n texels: out [n][4]. */
+void orc_unpack_srgb_n(const uint32_t* packed, float* out, uint64_t n);
 
 /* Sampler entry for tests: 16 lanes = one 4x4 tile (Texture.h:412-459). */
 void orc_sample_level16(const orc_texture* tex, const float* u, const float* v, int32_t layer,

@@ -460,3 +460,64 @@ extern "C" void probe_env_bilinear(const uint32_t* d, const int32_t* ixf, const 
         st(out + 16 * c, rowA + (rowB - rowA) * fy);
     }
 }
+
+// GenerateMip's box filter (Texture.h:577-596): four texels unpacked, averaged
+// as (c00 + c10 + c01 + c11) * 0.25f on four-component vectors, packed back.
+typedef uint32_t vu __attribute__((ext_vector_type(16)));
+struct vf4 { vf x, y, z, w; };
+static inline vu ldu(const uint32_t* p) { vu v; memcpy(&v, p, 64); return v; }
+static inline void stu(uint32_t* p, vu v) { memcpy(p, &v, 64); }
+static inline vf4 add4(vf4 a, vf4 b) { return { a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w }; }
+static inline vf4 mul4(vf4 a, float s) { return { a.x * s, a.y * s, a.z * s, a.w * s }; }
+
+// RGBA8u::Unpack (Texture.h:28-36)
+static inline vf4 unpack_rgba8(vu p) {
+    const float scale = 1.0f / 255;
+    return { __builtin_convertvector((p >> 0) & 255, vf) * scale, __builtin_convertvector((p >> 8) & 255, vf) * scale,
+             __builtin_convertvector((p >> 16) & 255, vf) * scale, __builtin_convertvector((p >> 24) & 255, vf) * scale };
+}
+// RGBA8u::Pack (Texture.h:55-67): RNE conversion, then the signed 16-bit and
+// unsigned 8-bit saturations of the two packs
+static inline vu pack_chan(vf v) {
+    vi i = rne(v * 255.0f);
+    i = i < -32768 ? -32768 : i;
+    i = i > 32767 ? 32767 : i;
+    i = i < 0 ? 0 : i;
+    i = i > 255 ? 255 : i;
+    return (vu)i;
+}
+
+extern "C" void probe_mip_rgba8(const uint32_t* c00, const uint32_t* c10, const uint32_t* c01, const uint32_t* c11,
+                                uint32_t* out, int n) {
+    for (int i = 0; i < n; i += 16) {
+        vf4 avg = mul4(add4(add4(add4(unpack_rgba8(ldu(c00 + i)), unpack_rgba8(ldu(c10 + i))),
+                                 unpack_rgba8(ldu(c01 + i))), unpack_rgba8(ldu(c11 + i))), 0.25f);
+        stu(out + i, pack_chan(avg.x) | (pack_chan(avg.y) << 8) | (pack_chan(avg.z) << 16) | (pack_chan(avg.w) << 24));
+    }
+}
+
+// The same filter over R11G11B10f texels (Texture.h:135-185): bit-level
+// unpack to float, average, clamp and cut the float's bits.
+static inline vf unpack_f11(vu x) { x = x << 17; return (vf)((x & 0x0FFE0000u) + 0x38000000u); }
+static inline vf unpack_f10(vu x) { x = x << 18; return (vf)((x & 0x0FFC0000u) + 0x38000000u); }
+static inline vu pack_f(vf x, float hi, int shift, uint32_t mask, uint32_t bias) {
+    x = x > (1.0f / (1 << 15)) ? x : (1.0f / (1 << 15));
+    x = x < hi ? x : hi;
+    return (((vu)x >> shift) & mask) - bias;
+}
+
+extern "C" void probe_mip_r11g11b10(const uint32_t* c00, const uint32_t* c10, const uint32_t* c01,
+                                    const uint32_t* c11, uint32_t* out, int n) {
+    for (int i = 0; i < n; i += 16) {
+        vu p[4] = { ldu(c00 + i), ldu(c10 + i), ldu(c01 + i), ldu(c11 + i) };
+        vf r = unpack_f11(p[0] >> 21), g = unpack_f11(p[0] >> 10), b = unpack_f10(p[0]);
+        for (int k = 1; k < 4; k++) {
+            r = r + unpack_f11(p[k] >> 21);
+            g = g + unpack_f11(p[k] >> 10);
+            b = b + unpack_f10(p[k]);
+        }
+        r = r * 0.25f; g = g * 0.25f; b = b * 0.25f;
+        stu(out + i, (pack_f(r, 130048.0f, 17, 0x3FFF, 0x1C00) << 21) | (pack_f(g, 130048.0f, 17, 0x3FFF, 0x1C00) << 10) |
+                         pack_f(b, 129024.0f, 18, 0x1FFF, 0x0E00));
+    }
+}

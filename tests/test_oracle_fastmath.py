@@ -421,3 +421,61 @@ def test_env_bilinear(probe):
         got = np.zeros(48, np.float32)
         L.orc_env_bilinear16(d.ctypes.data_as(u32p), P(ixf), P(iyf), P(inx), P(got))
         assert (_bits(got) == _bits(want)).all(), i
+
+
+def _mip_inputs(rng, n):
+    """n quads of texels: every (a, b) byte pair against random (c, d) pairs
+    in the four channels, then random texels."""
+    a, b = np.meshgrid(np.arange(256, dtype=np.uint32), np.arange(256, dtype=np.uint32), indexing="ij")
+    a, b = np.tile(a.reshape(-1), 16), np.tile(b.reshape(-1), 16)
+    c = rng.integers(0, 256, size=a.size, dtype=np.uint32)
+    d = rng.integers(0, 256, size=a.size, dtype=np.uint32)
+    quad = np.stack([a, b, c, d])                                       # [4, 2^20] one channel each
+    shuffled = [np.roll(quad, s, axis=0) for s in range(4)]             # the same values in every position
+    texels = sum(q << np.uint32(8 * k) for k, q in enumerate(shuffled))
+    rand = rng.integers(0, 1 << 32, size=(4, n), dtype=np.uint64).astype(np.uint32)
+    return np.concatenate([texels.astype(np.uint32), rand], axis=1)
+
+
+def _as_level(quads):
+    """[4, n] (c00, c10, c01, c11) -> a (2, 2n) level whose 2x2 blocks are the quads."""
+    n = quads.shape[1]
+    lv = np.zeros((2, 2 * n), np.uint32)
+    lv[0, 0::2], lv[0, 1::2], lv[1, 0::2], lv[1, 1::2] = quads
+    return lv
+
+
+def test_generate_mip_rgba8(probe):
+    """GenerateMip's box filter for RGBA8u (Texture.h:577-596) as the
+    reference's flags compile it against textures.next_mip, which builds the
+    mip chains both the GPU and the oracle sample: bit for bit over every
+    byte pair in the first two positions (random partners) in every channel,
+    and 2^20 random quads."""
+    from glimpsw_amd import textures
+
+    quads = _mip_inputs(np.random.default_rng(3), 1 << 20)
+    n = quads.shape[1]
+    out = np.zeros(n, np.uint32)
+    q = [np.ascontiguousarray(quads[k]) for k in range(4)]
+    u32 = C.POINTER(C.c_uint32)
+    probe.probe_mip_rgba8(*[x.ctypes.data_as(u32) for x in q], out.ctypes.data_as(u32), n)
+    got = textures.next_mip(_as_level(quads))[0]
+    bad = got != out
+    assert not bad.any(), f"{bad.sum()} quads differ, first {quads[:, np.nonzero(bad)[0][0]]}"
+
+
+def test_generate_mip_r11g11b10(probe):
+    """The same filter over R11G11B10f texels (the skybox's mips) against
+    textures.next_mip_hdr, bit for bit on 2^21 random quads."""
+    from glimpsw_amd import textures
+
+    rng = np.random.default_rng(4)
+    quads = rng.integers(0, 1 << 32, size=(4, 1 << 21), dtype=np.uint64).astype(np.uint32)
+    n = quads.shape[1]
+    out = np.zeros(n, np.uint32)
+    q = [np.ascontiguousarray(quads[k]) for k in range(4)]
+    u32 = C.POINTER(C.c_uint32)
+    probe.probe_mip_r11g11b10(*[x.ctypes.data_as(u32) for x in q], out.ctypes.data_as(u32), n)
+    got = textures.next_mip_hdr(_as_level(quads))[0]
+    bad = got != out
+    assert not bad.any(), f"{bad.sum()} quads differ"
