@@ -4,15 +4,26 @@
 One step = one frame of the seeded 1080p scene with 10,240 grid-patch
 meshlets (1,003,520 triangles, 16 materials): clear, frustum cull, mesh +
 setup, guard-band clip and raster into the 64-bit vis-buffer, then the
-lit resolve. Inputs are resident in HBM before the timed region; the frame is
-launched as a batch of one view, replayed as one hipGraph or launched kernel
-by kernel, whichever a short untimed trial finds faster. A second measurement
-renders the 64-view batching workload as one batch launch (graph and eager)
-and as per-view launches (views/s, same vis-buffers).
+lit resolve. Inputs are resident in HBM before the timed region. The frame is
+a batch of one view at the launch shape the batch autotuner picks, launched
+as one hipGraph replay or kernel by kernel, whichever a short untimed trial
+finds faster. A second measurement renders the 64-view batching workload as
+one batch launch (graph and eager) and as per-view launches (views/s, same
+vis-buffers).
 
-Prints one JSON line: Mtri/s (submitted triangles per second) and fps, the
-roofline of the dominant kernel from HIP-event kernel times taken inside the
-timed region, and the CPU oracle timed on a bounded sample of the same frame.
+Prints one JSON line:
+ - value: Mtri/s of submitted triangles per frame time; value_processed:
+   Mtri/s of the TrianglesProcessed counter (triangles of meshlets that pass
+   the cull, Rasterizer.cpp:545); fps;
+ - kernels: each kernel's average duration over the timed steps of the same
+   frame at the same launch shape, from events the runtime stamps at the
+   dispatch's own start and end (hipExtLaunchKernel), with its algorithmic
+   bytes, and (from rocprofv3 counter passes in child processes, at the same
+   shape) HBM bytes and vector-ALU instructions;
+ - roofline: the dominant kernel's algorithmic bytes over its duration
+   against the 8 TB/s HBM peak;
+ - cpu_baseline: the CPU oracle on a bounded sample of the same frame, on one
+   host core and on all the host threads the box grants (row bands).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -41,10 +52,11 @@ def algorithmic_bytes(stats: dict) -> dict:
     frame (meshlets visible, records, clip entries, bin entries, pixels).
     Data that several work items share (a record in several bins, a vertex
     of several pixels) counts once."""
-    m_all, m_vis, t_vis = stats["meshlets"], stats["visible"], stats["tris_visible"]
-    S, Cq, F, N = stats["records"], stats["clipped"], stats["fragments"], stats["pixels"]
+    m_all, m_vis = stats["meshlets"], stats["visible"]
+    S, Cq, N = stats["records"], stats["clipped"], stats["pixels"]
     P = stats.get("bin_entries", 0)
     ncov = stats.get("covered_pixels", 0)
+    F = stats["fragments"]
     # per visible meshlet: positions 768 B, indices 384 B, count + material 5 B
     meshlet_in = m_vis * (768 + 384 + 5)
     return {
@@ -68,10 +80,10 @@ def algorithmic_bytes(stats: dict) -> dict:
     }
 
 
-# profiler scope name -> kernel symbol substring in rocprofv3 output
+# profiler name -> kernel symbol substring in rocprofv3 output
 KERNEL_SYMBOLS = {
     "clear": "k_clear(", "cull": "k_cull<", "mesh_setup": "k_mesh_setup<", "clip": "k_clip<",
-    "bin_scatter": "k_bin_scatter<", "bin_raster": "k_bin_raster<",
+    "bin_scatter": "k_bin_scatter<", "bin_order": "k_bin_order<", "bin_raster": "k_bin_raster<",
     "resolve_lit": "k_resolve<1>",
 }
 
@@ -108,43 +120,62 @@ def _rocprof_counter(counter: str, child_args: list, tag: str) -> dict | None:
     return {k: sums[k] / counts[k] for k in sums}
 
 
-def measure_traffic() -> dict | None:
+def _pick(d, sub):
+    vals = [v for k, v in d.items() if sub in k]
+    return vals[0] if vals else None
+
+
+def measure_traffic(shape: dict | None) -> dict | None:
     """HBM bytes per launch of each kernel of the frame: FETCH_SIZE and
     WRITE_SIZE (KiB per dispatch) from two rocprofv3 runs of a short frame
-    loop, each scaled by what the counter reads for a device copy of known
-    size."""
-    copy_bytes = 1 << 28
-    # the separate-call frames' launch shape (the library default) for the
-    # batch frames too, so that each kernel has one instantiation
-    frame = ["--steps", "3", "--warmup", "1", "--no-cpu-baseline", "--no-batch", "--no-traffic", "--no-counts",
-             "--bin-size", "16", "--bin-capacity", "1024", "--raster-threads", "256"]
-    copy = ["--calibrate-copy", str(copy_bytes)]
+    loop at the headline launch shape, each scaled by what the counter reads
+    for a device copy of known size. Two more known-bytes kernels check that
+    scale: a read-only reduction and a gather of 64-byte rows in random order
+    (the access pattern of the resolve's vertex and texel fetches)."""
+    n_bytes = 1 << 28
+    frame = ["--steps", "3", "--warmup", "1", "--no-cpu-baseline", "--no-batch", "--no-traffic", "--no-counts"]
+    if shape:
+        frame += ["--bin-size", str(shape["bin_size"]), "--bin-capacity", str(shape["bin_capacity"]),
+                  "--raster-threads", str(shape["raster_threads"])]
     fetch = _rocprof_counter("FETCH_SIZE", frame, "frame")
     write = _rocprof_counter("WRITE_SIZE", frame, "frame")
-    cf = _rocprof_counter("FETCH_SIZE", copy, "copy")
-    cw = _rocprof_counter("WRITE_SIZE", copy, "copy")
-    if not (fetch and write and cf and cw):
+    cal = {}
+    for kind in ("copy", "read", "gather"):
+        cal[kind] = (_rocprof_counter("FETCH_SIZE", [f"--calibrate-{kind}", str(n_bytes)], kind),
+                     _rocprof_counter("WRITE_SIZE", [f"--calibrate-{kind}", str(n_bytes)], kind))
+    if not (fetch and write and cal["copy"][0] and cal["copy"][1]):
         return None
-
-    def pick(d, sub):
-        vals = [v for k, v in d.items() if sub in k]
-        return vals[0] if vals else None
-
-    kf, kw = pick(cf, "copyBuffer"), pick(cw, "copyBuffer")
+    kf, kw = _pick(cal["copy"][0], "copyBuffer"), _pick(cal["copy"][1], "copyBuffer")
     if not kf or not kw:
         return None
-    f_scale = copy_bytes / (kf * 1024.0)      # bytes per counted KiB
-    w_scale = copy_bytes / (kw * 1024.0)
+    f_scale = n_bytes / (kf * 1024.0)      # bytes per counted KiB
+    w_scale = n_bytes / (kw * 1024.0)
     out = {"fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4)}
+
+    def largest(d):
+        return max(d.values()) if d else None
+
+    # the same scales applied to the other known-bytes kernels: the bytes they
+    # must read (and write), over what the scaled counters say
+    checks = {}
+    rf = largest(cal["read"][0])
+    if rf:
+        checks["read_fetch"] = round(n_bytes / (rf * 1024.0 * f_scale), 4)
+    gf, gw = largest(cal["gather"][0]), largest(cal["gather"][1])
+    if gf:
+        # rows of 64 bytes plus one 8-byte index per row read; the rows written
+        checks["gather_fetch"] = round((n_bytes + n_bytes // 8) / (gf * 1024.0 * f_scale), 4)
+    if gw:
+        checks["gather_write"] = round(n_bytes / (gw * 1024.0 * w_scale), 4)
+    out["scale_checks"] = checks
     for name, sub in KERNEL_SYMBOLS.items():
-        fv, wv = pick(fetch, sub), pick(write, sub)
+        fv, wv = _pick(fetch, sub), _pick(write, sub)
         if fv is not None and wv is not None:
             out[name] = int(fv * 1024.0 * f_scale + wv * 1024.0 * w_scale)
-    # vector-ALU instructions per launch (wave instructions): the bound these
-    # kernels actually meet
+    # vector-ALU instructions per launch (wave instructions)
     valu = _rocprof_counter("SQ_INSTS_VALU", frame, "frame")
     if valu:
-        out["valu"] = {name: int(pick(valu, sub)) for name, sub in KERNEL_SYMBOLS.items() if pick(valu, sub)}
+        out["valu"] = {name: int(_pick(valu, sub)) for name, sub in KERNEL_SYMBOLS.items() if _pick(valu, sub)}
     return out
 
 
@@ -159,30 +190,62 @@ def _valu_roofline(traffic, name: str, avg_us) -> dict | None:
             "frac": round(bound_us / avg_us, 4)}
 
 
+def host_threads() -> int:
+    """Host threads this process may use: the CPUs it is allowed to run on,
+    capped by OMP_NUM_THREADS when that is set (the GPU box grants 16)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(n, int(cap))) if cap and cap.isdigit() else n
+
+
 def cpu_baseline(scene, view, seconds: float) -> dict:
+    """The CPU oracle (a scalar C++ restatement of the reference's algorithm,
+    -O2 without fast-math) on the same frame: clear + cull + draw + lit
+    resolve, on one host core, then on all granted host threads with the
+    frame split into row bands (bit-identical to the one-core frame), each for
+    about `seconds`."""
+    from concurrent.futures import ThreadPoolExecutor
+
     import oracle
     from glimpsw_amd import scene as S
 
     b = oracle.SceneBinding(scene)
-    frames = 0
-    t0 = time.perf_counter()
     lighting = S.default_lighting(view)
-    while True:
-        fb, ctr, _ = oracle.render_frame(scene, view, b)
-        oracle.resolve_lit(fb, b, view.object_to_clip, lighting)
-        frames += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
     tris = scene.triangles()
+
+    def timed(fn):
+        fn()                                   # warm (tables, page faults)
+        frames, t0 = 0, time.perf_counter()
+        while True:
+            fn()
+            frames += 1
+            el = time.perf_counter() - t0
+            if el >= seconds:
+                return frames, el
+
+    def one_core():
+        fb, _, _ = oracle.render_frame(scene, view, b)
+        oracle.resolve_lit(fb, b, view.object_to_clip, lighting)
+
+    f1, e1 = timed(one_core)
+    threads = host_threads()
+    # one band per thread: every band sets up all visible meshlets, so more
+    # bands cost more than the better balance gains
+    bands = oracle.row_bands(view.height, threads)
+    with ThreadPoolExecutor(threads) as pool:
+        fn, en = timed(lambda: oracle.render_frame_threads(scene, view, b, lighting, pool, bands))
     return {
-        "value": round(tris * frames / el / 1e6, 3),
+        "value": round(tris * f1 / e1 / 1e6, 3),
         "unit": "Mtri/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{frames} frames of the same scene (clear+cull+draw+lit resolve, {tris} submitted triangles each) "
-                  f"in {el:.1f} s on one host core, scalar C++ oracle",
-        "fps": round(frames / el, 3),
+        "sample": f"{f1} frames of the same scene (clear+cull+draw+lit resolve, {tris} submitted triangles each) "
+                  f"in {e1:.1f} s on one host core, scalar C++ oracle",
+        "fps": round(f1 / e1, 3),
+        "all_cores": {"value": round(tris * fn / en / 1e6, 3), "unit": "Mtri/s", "cores": threads,
+                      "fps": round(fn / en, 3),
+                      "sample": f"{fn} frames in {en:.1f} s, {len(bands)} row bands over {threads} threads "
+                                f"(cull once, then every band's draw and lit resolve; bit-identical to one core)"},
     }
 
 
@@ -199,7 +262,9 @@ def bench_batch(ctx, stream, steps: int, warmup: int) -> dict:
     dscenes = [g.Scene(ctx, sc) for sc in scenes]
     fbs = [g.Framebuffer(ctx, v.width, v.height) for _, v in views]
     b = g.Batch(ctx, [(dscenes[si], fb, v) for (si, v), fb in zip(views, fbs)])
+    t_tune = time.perf_counter()
     shape, tune_ms = b.autotune(stream)
+    t_tune = time.perf_counter() - t_tune
 
     def timed(fn):
         for _ in range(warmup):
@@ -216,6 +281,13 @@ def bench_batch(ctx, stream, steps: int, warmup: int) -> dict:
     ms_eager = timed(lambda: b.launch(False, stream))
     same_eager = all((wi == gi).all() for (wi, _), (gi, _) in zip(want, [fb.read() for fb in fbs]))
     ms_batch = min(ms_graph, ms_eager)
+    # per-kernel times of the eager batch at the tuned shape
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    for _ in range(steps):
+        b.launch(False, stream)
+    kern = ctx.profile_read(reset=True)
+    ctx.profile(False)
     ms_per_view = timed(lambda: b.launch_per_view(stream))
     got = [fb.read() for fb in fbs]
     identical = all((wi == gi).all() and (wd.view(np.uint32) == gd.view(np.uint32)).all()
@@ -237,12 +309,83 @@ def bench_batch(ctx, stream, steps: int, warmup: int) -> dict:
         "ms_per_batch_graph": round(ms_graph, 4),
         "ms_per_batch_eager": round(ms_eager, 4),
         "ms_per_batch_per_view_launch": round(ms_per_view, 4),
+        "kernels_eager_us": {k: round(t / max(1, c) * 1e3, 2) for k, (c, t) in kern.items()},
         "submitted_tris": tris,
         "Mtri_per_s": round(tris / (ms_batch * 1e-3) / 1e6, 1),
         "autotuned_shape": shape,
         "autotune_ms": round(tune_ms, 4),
+        "autotune_wall_s": round(t_tune, 2),
         "identical_vis_buffers": bool(identical and same_eager),
     }
+
+
+def bench_clip_frame(ctx, stream, shape: dict, steps: int, warmup: int) -> dict:
+    """The 1080p frame with its near-plane crossers reaching into the frustum
+    (frame_1080p(crossers_inside=True): about ten times the clipped triangles
+    of the headline frame, each covering much of the screen), at the headline
+    launch shape: the frame time, the clipped-triangle count and the bin
+    scatter's duration, which includes the clipping."""
+    import torch
+    import glimpsw_amd as g
+    from glimpsw_amd import scene as S
+
+    scene, view = S.frame_1080p(crossers_inside=True)
+    ds = g.Scene(ctx, scene)
+    fb = g.Framebuffer(ctx, view.width, view.height)
+    planes = view.frustum_planes()
+    b = g.Batch(ctx, [(ds, fb, view, planes, 0)], shape=shape)
+    b.set_resolve([(view.object_to_clip, g.GLIMP_RESOLVE_LIT, S.default_lighting(view))])
+    for _ in range(warmup):
+        b.launch(True, stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.launch(True, stream)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    for _ in range(steps):
+        b.launch(False, stream)
+    kern = ctx.profile_read(reset=True)
+    ctx.profile(False)
+    ctx.reset_counters()
+    fb.clear(0, 0.0, stream)
+    fb.draw(ds, view.object_to_clip, view.meshlet_count, 0, planes, g.GLIMP_DRAW_COUNTERS, stream,
+            shape["bin_size"], shape["bin_capacity"])
+    c = ctx.counters()
+    b.close()
+    fb.close()
+    ds.close()
+    return {"workload": "the 1080p frame with its near-plane crossers 0.5-0.7 of their size off the view axis",
+            "ms_per_frame_graph": round(ms, 4), "Mtri_per_s": round(scene.triangles() / (ms * 1e-3) / 1e6, 1),
+            "clipped": int(c["triangles_clipped"]), "dropped": int(c["dropped"]),
+            "fragments_per_pixel": round(c["fragments_covered"] / (view.width * view.height), 2),
+            "kernels_eager_us": {k: round(t / max(1, n) * 1e3, 2) for k, (n, t) in kern.items()}}
+
+
+def _calibrate(kind: str, n_bytes: int) -> None:
+    """Known-bytes kernels for the HBM counter calibration (10 launches each):
+    copy (reads and writes n_bytes), read (a sum over n_bytes) and gather
+    (n_bytes of 64-byte rows read in a random order through an index, written
+    in order)."""
+    import torch
+
+    src = torch.empty(n_bytes // 4, dtype=torch.int32, device="cuda").fill_(1)
+    if kind == "copy":
+        dst = torch.empty_like(src)
+        fn = lambda: dst.copy_(src)                                    # noqa: E731
+    elif kind == "read":
+        fn = lambda: src.sum()                                         # noqa: E731
+    else:
+        rows = src.view(-1, 16)
+        idx = torch.randperm(rows.shape[0], device="cuda")
+        fn = lambda: torch.index_select(rows, 0, idx)                  # noqa: E731
+    torch.cuda.synchronize()
+    for _ in range(10):
+        fn()
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate": kind, "bytes": n_bytes, "launches": 10}))
 
 
 def main() -> int:
@@ -250,33 +393,25 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="seconds of each CPU baseline leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--direct", action="store_true", help="global-atomic raster instead of tile bins")
-    ap.add_argument("--bin-size", type=int, default=0, help="screen bin side (8/16/32/64), 0 = library default")
-    ap.add_argument("--bin-capacity", type=int, default=0, help="records per bin, 0 = library default")
-    ap.add_argument("--raster-threads", type=int, default=0, help="bin raster workgroup (128/256/512/1024), 0 = 256")
+    ap.add_argument("--bin-size", type=int, default=0, help="screen bin side (8/16/32/64), 0 = autotuned")
+    ap.add_argument("--bin-capacity", type=int, default=0, help="records per bin, 0 = autotuned")
+    ap.add_argument("--raster-threads", type=int, default=0, help="bin raster workgroup (128/256/512/1024)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batching measurement")
-    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM traffic passes")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--no-counts", action="store_true", help="skip the counting pass (frame kernels only)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the 64-view batching measurement (its JSON object)")
-    ap.add_argument("--frame-calls", action="store_true",
-                    help="time the frame as separate clear/draw/resolve calls instead of a graph replay")
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
-                    help="only run a device copy of BYTES bytes 10 times (HBM counter calibration)")
+    for kind in ("copy", "read", "gather"):
+        ap.add_argument(f"--calibrate-{kind}", type=int, default=0, metavar="BYTES",
+                        help=f"only run the {kind} calibration kernel over BYTES bytes 10 times")
     args = ap.parse_args()
-    if args.calibrate_copy:
-        import torch
-
-        src = torch.empty(args.calibrate_copy, dtype=torch.uint8, device="cuda").fill_(1)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        for _ in range(10):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes": args.calibrate_copy, "launches": 10}))
-        return 0
+    for kind in ("copy", "read", "gather"):
+        if getattr(args, f"calibrate_{kind}"):
+            _calibrate(kind, getattr(args, f"calibrate_{kind}"))
+            return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; the path has no multi-GPU mode "
                                    "(views or meshlets would shard without exchange except a min-reduce "
@@ -293,10 +428,6 @@ def main() -> int:
         ctx.close()
         return 0
 
-    # HBM traffic from counter passes in child processes, before this
-    # process touches the GPU
-    traffic = None if args.no_traffic else measure_traffic()
-
     import torch
     import glimpsw_amd as g
     from glimpsw_amd import scene as S
@@ -311,37 +442,7 @@ def main() -> int:
     fb = g.Framebuffer(ctx, view.width, view.height)
     planes = view.frustum_planes()
     flags = g.GLIMP_DRAW_DIRECT if args.direct else 0
-
     lighting = S.default_lighting(view)
-
-    def frame_calls():
-        # the reference's call sequence: Clear, DrawMeshlets (with the cull), Resolve
-        fb.clear(0, 0.0, stream)
-        fb.draw(ds, view.object_to_clip, view.meshlet_count, 0, planes, flags, stream, args.bin_size,
-                args.bin_capacity)
-        fb.resolve(ds, view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting, stream)
-
-    # the same frame (clear, cull, draw, lit resolve) replayed as one captured
-    # graph: a batch of one view with its resolve; vis-buffer and colour are
-    # identical to the separate calls (tests/test_gpu_batch.py)
-    graph = None
-    frame_shape = None
-    if not args.direct and not args.frame_calls:
-        shape = {"bin_size": args.bin_size or 16, "bin_capacity": args.bin_capacity or 1024,
-                 "raster_threads": args.raster_threads or 256}
-        graph = g.Batch(ctx, [(ds, fb, view, planes, flags)], shape=shape)
-        frame_shape = shape
-        if not args.bin_size and not args.bin_capacity and not args.raster_threads:
-            frame_shape, _ = graph.autotune(stream)     # the launch shape, as for any batch
-        graph.set_resolve([(view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting)])
-
-    def frame_graph():
-        graph.launch(True, stream)
-
-    def frame_batch_eager():
-        # the same batch launched kernel by kernel: the queue stays full
-        # without the graph's per-launch boundary
-        graph.launch(False, stream)
 
     def run(fn, steps):
         torch.cuda.synchronize()
@@ -351,46 +452,65 @@ def main() -> int:
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) * 1e3 / steps
 
-    # per-kernel times: HIP events around every launch of the separate-call
-    # frame, on the launch stream, over the timed steps
-    for _ in range(args.warmup):
-        frame_calls()
-    ctx.profile(True)
-    ctx.profile_read(reset=True)
-    ms_calls = run(frame_calls, args.steps)
-    kern = ctx.profile_read(reset=True)
-    ctx.profile(False)
-    # the headline frame time, without per-kernel events: the faster of the
-    # graph replay and the eager batch launch (a short untimed trial of
-    # each), then exactly `steps` timed frames of it
+    # The frame (clear, cull, draw, lit resolve) as a batch of one view with
+    # its resolve: replayed as one captured graph or launched kernel by
+    # kernel. Vis-buffer and colour equal the separate API calls
+    # (tests/test_gpu_batch.py).
+    batch, shape = None, None
+    if not args.direct:
+        shape = {"bin_size": args.bin_size or 16, "bin_capacity": args.bin_capacity or 1024,
+                 "raster_threads": args.raster_threads or 256}
+        batch = g.Batch(ctx, [(ds, fb, view, planes, flags)], shape=shape)
+        if not args.bin_size and not args.bin_capacity and not args.raster_threads:
+            shape, _ = batch.autotune(stream)
+        batch.set_resolve([(view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting)])
+
+    def frame_calls():
+        # the reference's call sequence: Clear, DrawMeshlets (with the cull), Resolve
+        fb.clear(0, 0.0, stream)
+        fb.draw(ds, view.object_to_clip, view.meshlet_count, 0, planes, flags, stream,
+                shape["bin_size"] if shape else args.bin_size, shape["bin_capacity"] if shape else args.bin_capacity)
+        fb.resolve(ds, view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting, stream)
+
     frame, frame_mode, trial = frame_calls, "separate calls", {}
-    if graph is not None:
-        for name, fn in (("graph", frame_graph), ("eager", frame_batch_eager)):
+    if batch is not None:
+        modes = {"graph": lambda: batch.launch(True, stream), "eager": lambda: batch.launch(False, stream)}
+        for name, fn in modes.items():
             for _ in range(args.warmup):
                 fn()
             trial[name] = round(run(fn, 20), 4)
-        if trial["graph"] <= trial["eager"]:
-            frame, frame_mode = frame_graph, "one graph replay of clear+cull+draw+resolve"
-        else:
-            frame, frame_mode = frame_batch_eager, "one eager batch launch of clear+cull+draw+resolve"
+        best = min(trial, key=trial.get)
+        frame = modes[best]
+        frame_mode = f"one {'graph replay' if best == 'graph' else 'eager batch launch'} of clear+cull+draw+resolve"
     for _ in range(args.warmup):
         frame()
-    ms = run(frame, args.steps)
+    ms = run(frame, args.steps)                    # the headline: no events, no counters
+
+    # per-kernel durations of the same frame at the same shape: the eager
+    # launch of the same kernels with dispatch-stamped events
+    per_kernel_fn = (lambda: batch.launch(False, stream)) if batch is not None else frame_calls
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    ms_profiled = run(per_kernel_fn, args.steps)
+    kern = ctx.profile_read(reset=True)
+    ctx.profile(False)
+    ms_calls = run(frame_calls, args.steps)
 
     if args.no_counts:
-        print(json.dumps({"ms_per_step": round(ms, 4), "kernels": {k: v for k, v in kern.items()}}))
-        if graph is not None:
-            graph.close()
+        print(json.dumps({"ms_per_step": round(ms, 4), "shape": shape,
+                          "kernels_us": {k: round(t / max(1, c) * 1e3, 2) for k, (c, t) in kern.items()}}))
+        if batch is not None:
+            batch.close()
         fb.close()
         ds.close()
         ctx.close()
         return 0
 
-    # counted quantities of one frame (separate, untimed run)
+    # counted quantities of one frame at the same shape (separate, untimed run)
     ctx.reset_counters()
     fb.clear(0, 0.0, stream)
     fb.draw(ds, view.object_to_clip, view.meshlet_count, 0, planes, flags | g.GLIMP_DRAW_COUNTERS, stream,
-            args.bin_size, args.bin_capacity)
+            shape["bin_size"] if shape else 0, shape["bin_capacity"] if shape else 0)
     c = ctx.counters()
     _, visible = ds.cull(planes, 0, view.meshlet_count)
     stats = {
@@ -398,18 +518,26 @@ def main() -> int:
         "records": int(c["triangles_rasterized"]), "clipped": int(c["triangles_clipped"]),
         "fragments": int(c["fragments_covered"]), "pixels": view.width * view.height,
         "bin_entries": int(c["bin_entries"]), "bins_rasterized": int(c["bins_rasterized"]),
+        "tile_lanes": int(c["tile_lanes"]), "dropped": int(c["dropped"]),
     }
+    if stats["tile_lanes"]:
+        stats["lane_utilisation"] = round(stats["fragments"] / stats["tile_lanes"], 4)
     _, depth = fb.read()
     stats["covered_pixels"] = int((depth > 0).sum())
     alg = algorithmic_bytes(stats)
+
+    # counter passes in child processes at the headline shape (this process
+    # keeps its context but is idle meanwhile)
+    traffic = None if args.no_traffic else measure_traffic(shape)
+
     per_kernel = {}
     for name, (launches, total) in kern.items():
-        avg_ms = total / max(1, launches)
+        avg_us = total / max(1, launches) * 1e3
         b = alg.get(name, 0)
-        per_kernel[name] = {"avg_us": round(avg_ms * 1e3, 2), "launches": launches, "alg_bytes": b,
-                            "GBps": round(b / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 else None,
+        per_kernel[name] = {"avg_us": round(avg_us, 2), "launches": launches, "alg_bytes": b,
+                            "GBps": round(b / (avg_us * 1e-6) / 1e9, 1) if avg_us > 0 and b else None,
                             "hbm_bytes": traffic.get(name) if traffic else None,
-                            "valu": _valu_roofline(traffic, name, round(avg_ms * 1e3, 2))}
+                            "valu": _valu_roofline(traffic, name, avg_us)}
     dom = max(kern.items(), key=lambda kv: kv[1][1])[0]
     dk = per_kernel[dom]
     tris = scene.triangles()
@@ -422,9 +550,14 @@ def main() -> int:
         "warmup": args.warmup,
         "ms_per_step": round(ms, 4),
         "fps": round(1e3 / ms, 1),
+        "value_processed": round(stats["tris_visible"] / (ms * 1e-3) / 1e6, 2),
+        "value_definitions": {"value": "submitted triangles / frame time",
+                              "value_processed": "TrianglesProcessed (triangles of meshlets passing the cull, "
+                                                 "Rasterizer.cpp:545) / frame time"},
         "frame": frame_mode,
-        "frame_shape": frame_shape,
+        "frame_shape": shape,
         "frame_trial_ms": trial,
+        "ms_per_step_profiled": round(ms_profiled, 4),
         "ms_per_step_separate_calls": round(ms_calls, 4),
         "higher_is_better": True,
         "scaling": "weak",
@@ -434,30 +567,30 @@ def main() -> int:
         "config": {"workload": "10,240 grid-patch meshlets (1,003,520 tris), 1920x1080, depth complexity ~3, "
                                "16 materials 2048^2 RGBA8",
                    "raster": "direct" if args.direct else (
-                       f"binned {frame_shape['bin_size']}x{frame_shape['bin_size']}, {frame_shape['bin_capacity']} "
-                       f"records per bin, {frame_shape['raster_threads']}-thread bin raster" if frame_shape else
-                       f"binned {args.bin_size or 16}x{args.bin_size or 16}"),
+                       f"binned {shape['bin_size']}x{shape['bin_size']}, {shape['bin_capacity']} "
+                       f"records per bin, {shape['raster_threads']}-thread bin raster"),
                    "submitted_tris": tris},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dk["GBps"] / HBM_PEAK_GBS, 4) if dk["GBps"] else None,
                      "traffic": traffic.get(dom) if traffic else None,
                      "valu": _valu_roofline(traffic, dom, dk["avg_us"]),
-                     "alg_bytes_per_launch": dk["alg_bytes"], "avg_us": dk["avg_us"]},
-        "traffic_calibration": ({"fetch_scale": traffic["fetch_scale"], "write_scale": traffic["write_scale"]}
+                     "alg_bytes_per_launch": dk["alg_bytes"], "avg_us": dk["avg_us"],
+                     "timing": "dispatch-stamped events (hipExtLaunchKernel) over the timed steps of the eager "
+                               "frame at the headline shape"},
+        "traffic_calibration": ({k: traffic[k] for k in ("fetch_scale", "write_scale", "scale_checks")}
                                 if traffic else None),
         "kernels": per_kernel,
         "frame_counts": stats,
         "scene_gen_s": round(t_gen, 2),
     }
+    if shape is not None:
+        result["clip_frame"] = bench_clip_frame(ctx, stream, shape, max(5, args.steps // 5), 3)
     if not args.no_batch:
         result["batch"] = bench_batch(ctx, stream, max(5, args.steps // 5), 3)
-    if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(scene, view, args.cpu_seconds)
-    else:
-        result["cpu_baseline"] = None
+    result["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(scene, view, args.cpu_seconds)
     print(json.dumps(result))
-    if graph is not None:
-        graph.close()
+    if batch is not None:
+        batch.close()
     fb.close()
     ds.close()
     ctx.close()

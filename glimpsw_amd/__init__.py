@@ -101,7 +101,8 @@ class CountersStruct(C.Structure):
     _fields_ = [("triangles_processed", C.c_uint64), ("triangles_rasterized", C.c_uint64),
                 ("triangles_clipped", C.c_uint64), ("bins_rasterized", C.c_uint64),
                 ("fragments_covered", C.c_uint64), ("fragments_passed", C.c_uint64),
-                ("setup_records", C.c_uint64), ("bin_entries", C.c_uint64), ("dropped", C.c_uint64)]
+                ("setup_records", C.c_uint64), ("bin_entries", C.c_uint64), ("dropped", C.c_uint64),
+                ("tile_lanes", C.c_uint64)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -90,19 +90,23 @@ struct Scratch {
     size_t meshlet_counts_cap = 0;
     uint32_t* bin_counts = nullptr;
     size_t bin_counts_cap = 0;
+    uint32_t* bin_flags = nullptr;     // zero between draws, like bin_counts
+    size_t bin_flags_cap = 0;
+    uint2* clip_overflow = nullptr;
+    size_t clip_overflow_cap = 0;
     uint32_t* bin_lists = nullptr;
     uint32_t* bin_order = nullptr;
     size_t bin_order_cap = 0;
     size_t bin_lists_cap = 0;
-    uint32_t* spill = nullptr;
+    uint32_t* spill = nullptr;         // overflow list (record indices)
     size_t spill_cap = 0;
     AlphaRecord* alpha = nullptr;      // only once a draw has alpha-tested materials
     size_t alpha_cap = 0;
     DrawScratch* scratch = nullptr;
 
     void release() {
-        void* ptrs[] = { records, clip, visible_list, meshlet_counts, bin_counts, bin_lists, bin_order, spill, alpha,
-                         scratch };
+        void* ptrs[] = { records, clip, visible_list, meshlet_counts, bin_counts, bin_flags, clip_overflow, bin_lists,
+                         bin_order, spill, alpha, scratch };
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         *this = Scratch();
@@ -157,6 +161,8 @@ struct glimp_framebuffer {
     glimp_context* ctx = nullptr;
     uint32_t width = 0, height = 0;
     unsigned long long* keys = nullptr;
+    // keys of bin-overflow fragments, zero between draws (ViewParams::spill_keys)
+    unsigned long long* spill_keys = nullptr;
     uint32_t clear_color = 0;
     float clear_depth = 0.0f;
     bool pending_clear = false;        // cleared, keys not yet written
@@ -400,7 +406,11 @@ int glimp_framebuffer_create(glimp_context* c, uint32_t width, uint32_t height, 
     f->height = height;
     hipError_t e = dev_alloc(&f->keys, (size_t)width * height);
     if (e == hipSuccess) e = hipMemset(f->keys, 0, (size_t)width * height * 8);
+    if (e == hipSuccess) e = dev_alloc(&f->spill_keys, (size_t)width * height);
+    if (e == hipSuccess) e = hipMemset(f->spill_keys, 0, (size_t)width * height * 8);
     if (e != hipSuccess) {
+        if (f->keys) (void)hipFree(f->keys);
+        if (f->spill_keys) (void)hipFree(f->spill_keys);
         delete f;
         return fail(GLIMP_E_HIP, std::string("glimp_framebuffer_create: ") + hipGetErrorString(e));
     }
@@ -412,6 +422,7 @@ void glimp_framebuffer_destroy(glimp_framebuffer* f) {
     if (!f) return;
     (void)hipSetDevice(f->ctx->device);
     if (f->keys) (void)hipFree(f->keys);
+    if (f->spill_keys) (void)hipFree(f->spill_keys);
     if (f->d_draws) (void)hipFree(f->d_draws);
     if (f->color) (void)hipFree(f->color);
     if (f->aux) (void)hipFree(f->aux);
@@ -652,6 +663,7 @@ void make_view(glimp_framebuffer* f, glimp_scene* s, uint32_t meshlet_offset, ui
     V.clear_depth = f->clear_depth;
     V.fresh = f->pending_clear ? 1 : 0;
     V.keys = f->keys;
+    V.spill_keys = f->spill_keys;
 }
 
 // Lays out bins for `views` (bin_base, bins_x/y) and returns the total.
@@ -676,6 +688,9 @@ uint64_t count_bins(const std::vector<ViewParams>& views, uint32_t bin_size) {
     return n;
 }
 
+// The overflow list takes each record at most once: one entry per record slot.
+size_t spill_capacity_for(size_t record_slots) { return record_slots; }
+
 int prepare_scratch(Scratch& S, size_t total_meshlets, size_t clipq, size_t total_bins, const LaunchShape& sh,
                     bool binned, bool alpha) {
     const size_t recs = total_meshlets * 128 + clipq * 8;
@@ -683,15 +698,20 @@ int prepare_scratch(Scratch& S, size_t total_meshlets, size_t clipq, size_t tota
     if (rc == GLIMP_OK) rc = grow_one(S.clip, S.clip_cap, clipq);
     if (rc == GLIMP_OK) rc = grow_one(S.visible_list, S.visible_list_cap, std::max<size_t>(total_meshlets, 1));
     if (rc == GLIMP_OK) rc = grow_one(S.meshlet_counts, S.meshlet_counts_cap, total_meshlets);
+    if (rc == GLIMP_OK) rc = grow_one(S.clip_overflow, S.clip_overflow_cap, std::max<size_t>(total_meshlets, 1));
     const size_t old_counts_cap = S.bin_counts_cap;
     if (rc == GLIMP_OK) rc = grow_one(S.bin_counts, S.bin_counts_cap, std::max<size_t>(total_bins, 1));
     // draws leave the bin counts cleared; a regrown buffer starts cleared (the
     // test is on the capacity: hipMalloc may hand back the freed address)
     if (rc == GLIMP_OK && S.bin_counts_cap != old_counts_cap)
         HIP_TRY(hipMemset(S.bin_counts, 0, S.bin_counts_cap * sizeof(uint32_t)));
+    const size_t old_flags_cap = S.bin_flags_cap;
+    if (rc == GLIMP_OK && binned) rc = grow_one(S.bin_flags, S.bin_flags_cap, std::max<size_t>(total_bins, 1));
+    if (rc == GLIMP_OK && S.bin_flags_cap != old_flags_cap)
+        HIP_TRY(hipMemset(S.bin_flags, 0, S.bin_flags_cap * sizeof(uint32_t)));
     if (rc == GLIMP_OK && binned) rc = grow_one(S.bin_lists, S.bin_lists_cap, total_bins * sh.bin_capacity);
     if (rc == GLIMP_OK && binned) rc = grow_one(S.bin_order, S.bin_order_cap, total_bins * kBinBuckets);
-    if (rc == GLIMP_OK && binned) rc = grow_one(S.spill, S.spill_cap, recs);
+    if (rc == GLIMP_OK && binned) rc = grow_one(S.spill, S.spill_cap, spill_capacity_for(recs));
     if (rc == GLIMP_OK && alpha) rc = grow_one(S.alpha, S.alpha_cap, S.records_cap);
     if (rc == GLIMP_OK && !S.scratch) {
         HIP_TRY(dev_alloc(&S.scratch, 1));
@@ -727,6 +747,10 @@ void fill_params(glimp_context* c, const Scratch& S, size_t total_meshlets, size
     p.bin_lists = S.bin_lists;
     p.bin_order = S.bin_order;
     p.spill = S.spill;
+    p.spill_capacity = (uint32_t)std::min(spill_capacity_for(total_meshlets * 128 + clipq * 8), S.spill_cap);
+    p.bin_flags = S.bin_flags;
+    p.clip_overflow = S.clip_overflow;
+    p.exact_overflow = 1u;          // vis-buffer pipeline; the overdraw and deferred draws turn it off
 }
 
 bool valid_bin_size(uint32_t b) { return b == 8 || b == 16 || b == 32 || b == 64; }
@@ -811,6 +835,9 @@ int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, 
     p.view.aux = f->aux;
     memcpy(p.view.o2w, d->object_to_world, sizeof(p.view.o2w));
     p.deferred = deferred ? 1u : 0u;
+    // clipped triangles without queue or record room are drawn at once only by
+    // the vis-buffer pipeline; the others count them in `dropped`
+    p.exact_overflow = (overdraw || deferred) ? 0u : 1u;
     if (overdraw) HIP_TRY(launch_draw_overdraw(p, st, &c->prof));
     else if (deferred) HIP_TRY(launch_draw_deferred(p, st, &c->prof));
     else if (direct) HIP_TRY(launch_draw_direct(p, st, &c->prof));
@@ -957,6 +984,7 @@ int glimp_batch_create(glimp_context* c, const glimp_view_desc* views, uint32_t 
         tmp.width = d.framebuffer->width;
         tmp.height = d.framebuffer->height;
         tmp.keys = d.framebuffer->keys;
+        tmp.spill_keys = d.framebuffer->spill_keys;
         tmp.clear_depth = d.clear_depth == 0.0f ? 0.0f : d.clear_depth;
         tmp.pending_clear = true;
         tmp.next_seq = 0;
@@ -967,6 +995,13 @@ int glimp_batch_create(glimp_context* c, const glimp_view_desc* views, uint32_t 
     }
     b->total_meshlets = base;
     b->clip_triangles = batch_clip_capacity(b->descs, base);
+    // record indices (meshlet slots of 128, then 8 fan records per queued
+    // triangle) are 32-bit and must stay below 2^31
+    if (b->total_meshlets * 128 + b->clip_triangles * 8 >= (size_t(1) << 31)) {
+        glimp_batch_destroy(b);
+        return fail(GLIMP_E_LIMIT, "glimp_batch_create: the views hold too many meshlets together "
+                                   "(meshlets * 128 + clip queue * 8 must stay below 2^31)");
+    }
     int rc = layout_batch(b);
     hipError_t e = hipSuccess;
     if (rc == GLIMP_OK) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
@@ -1123,9 +1158,21 @@ int glimp_batch_autotune(glimp_batch* b, void* stream, glimp_batch_shape* chosen
     HIP_TRY(hipEventCreate(&e1));
     glimp_batch_shape best = { b->shape.bin_size, b->shape.bin_capacity, b->shape.raster_threads };
     float bestMs = 1e30f;
+    bool found = false;
     const bool prof = b->ctx->prof.enabled;
     b->ctx->prof.enabled = false;
     int rc = GLIMP_OK;
+    auto timed = [&](int reps, float& ms) -> int {
+        (void)hipEventRecord(e0, st);
+        int r = GLIMP_OK;
+        for (int i = 0; i < reps && r == GLIMP_OK; i++) r = glimp_batch_launch(b, 0, stream);
+        (void)hipEventRecord(e1, st);
+        if (r != GLIMP_OK) return r;
+        (void)hipEventSynchronize(e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)reps;
+        return GLIMP_OK;
+    };
     for (uint32_t bs : sizes) {
         for (uint32_t cap : caps) {
             for (uint32_t th : threads) {
@@ -1135,20 +1182,29 @@ int glimp_batch_autotune(glimp_batch* b, void* stream, glimp_batch_shape* chosen
                 if (!valid_shape(sh)) continue;
                 rc = glimp_batch_set_shape(b, &sh);
                 if (rc != GLIMP_OK) break;
-                for (int i = 0; i < 2 && rc == GLIMP_OK; i++) rc = glimp_batch_launch(b, 0, stream);
+                // One probe launch: a shape whose bins overflow (their pairs
+                // take the slower overflow raster) or that drops anything is
+                // rejected, and one slower than twice the best so far is
+                // abandoned before the timed launches.
+                Counters before;
+                HIP_TRY(hipMemcpy(&before, b->ctx->counters, sizeof(before), hipMemcpyDeviceToHost));
+                float probe = 0.0f;
+                rc = timed(1, probe);
                 if (rc != GLIMP_OK) break;
-                const int reps = 5;
-                (void)hipEventRecord(e0, st);
-                for (int i = 0; i < reps && rc == GLIMP_OK; i++) rc = glimp_batch_launch(b, 0, stream);
-                (void)hipEventRecord(e1, st);
-                if (rc != GLIMP_OK) break;
+                Counters after;
+                DrawScratch ds;
+                HIP_TRY(hipMemcpy(&after, b->ctx->counters, sizeof(after), hipMemcpyDeviceToHost));
+                HIP_TRY(hipMemcpy(&ds, b->scratch.scratch, sizeof(ds), hipMemcpyDeviceToHost));
+                if (ds.num_spill != 0 || after.dropped != before.dropped) continue;
+                if (found && probe > 2.0f * bestMs) continue;
                 float ms = 0.0f;
-                (void)hipEventSynchronize(e1);
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                ms /= reps;
+                rc = timed(1, ms);                 // warm-up
+                if (rc == GLIMP_OK) rc = timed(5, ms);
+                if (rc != GLIMP_OK) break;
                 if (ms < bestMs) {
                     bestMs = ms;
                     best = sh;
+                    found = true;
                 }
             }
             if (rc != GLIMP_OK) break;
@@ -1159,8 +1215,20 @@ int glimp_batch_autotune(glimp_batch* b, void* stream, glimp_batch_shape* chosen
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc != GLIMP_OK) return rc;
+    if (!found) {
+        // every shape overflowed a bin or dropped something: keep the largest
+        // capacity at the batch's bin size, which overflows least
+        best.bin_capacity = caps[sizeof(caps) / sizeof(caps[0]) - 1];
+        if (!valid_shape(best)) best.raster_threads = 256;
+    }
     rc = glimp_batch_set_shape(b, &best);
     if (rc != GLIMP_OK) return rc;
+    if (!found) {
+        float ms = 0.0f;
+        rc = timed(1, ms);
+        if (rc != GLIMP_OK) return rc;
+        bestMs = ms;
+    }
     if (chosen) *chosen = best;
     if (best_ms) *best_ms = bestMs;
     return GLIMP_OK;
@@ -1613,6 +1681,7 @@ int glimp_get_counters(glimp_context* c, glimp_counters* out) {
     out->setup_records = h.setup_records;
     out->bin_entries = h.bin_entries;
     out->dropped = h.dropped;
+    out->tile_lanes = h.tile_lanes;
     return GLIMP_OK;
 }
 

@@ -493,8 +493,7 @@ void diff_workspace_release(DiffWorkspace& w) {
 
 hipError_t launch_diff_forward(const DiffParams& p, hipStream_t s, Profiler* prof) {
     const uint32_t n = p.total_pixels;
-    Profiler::Scope sc(prof, "diff_forward", s);
-    hipLaunchKernelGGL(k_diff_forward, dim3((n + 255) / 256), dim3(256), 0, s, p);
+    launch_kernel(prof, "diff_forward", k_diff_forward, dim3((n + 255) / 256), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 
@@ -516,12 +515,9 @@ hipError_t launch_diff_backward(DiffParams p, DiffWorkspace& w, uint64_t num_tex
         if (e == hipSuccess) e = fill_zero(grad_color, (size_t)num_vertices * 4, s);
         if (e != hipSuccess) return e;
     }
-    {
-        Profiler::Scope sc(prof, "diff_records", s);
-        hipLaunchKernelGGL(k_diff_records, dim3((n + 255) / 256), dim3(256), 0, s, p);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
+    launch_kernel(prof, "diff_records", k_diff_records, dim3((n + 255) / 256), dim3(256), 0, s, p);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
     {
         Profiler::Scope sc(prof, "diff_reduce_texels", s);
         e = reduce_records<1>(w, w.tex_keys, w.tex_vals, n * 4, p.tex_sentinel, TexelOut{ grad_texels }, s);

@@ -11,10 +11,12 @@
 //   k_clip        one wave per queued triangle: wave-parallel
 //                 Sutherland-Hodgman, fan triangulation, setup
 //   k_bin_scatter (record, screen bin) pairs into fixed-capacity bins; a
-//                 record that finds a bin full goes to the spill list
-//   k_bin_order   bins ordered by record count, most first
-//   k_bin_raster  one workgroup per bin: LDS-resident 64-bit keys, from the
-//                 bin's list and the spilled records that overlap the bin
+//                 record with a pair that finds its bin full goes to the
+//                 overflow list
+//   k_bin_order   bins ordered by record count, most first; extra blocks
+//                 draw the overflow list into the views' spill keys
+//   k_bin_raster  one workgroup per bin: LDS-resident 64-bit keys from the
+//                 bin's list, merged with its spill keys when marked
 //   k_raster_direct  without binning, all records: global 64-bit atomicMax
 //
 // Depth test. The reference keeps the nearest fragment (reverse-Z, strictly
@@ -274,6 +276,7 @@ __global__ void __launch_bounds__(256) k_cull(DrawParams p) {
         p.scratch->num_clip = 0;
         p.scratch->num_records = 0;
         p.scratch->num_spill = 0;
+        p.scratch->num_clip_overflow = 0;
     }
     // the block's meshlets are consecutive: one search for the first view,
     // then each thread walks forward from it
@@ -415,6 +418,39 @@ __device__ __forceinline__ MeshletFetch fetch_meshlet(const DrawParams& p, uint3
     return f;
 }
 
+// A triangle to clip, wave-uniform: the clip queue's entry, or the values
+// setup broadcasts for a triangle that found the queue full.
+struct ClipSrc {
+    uint32_t rank_base, info, view;
+    float v[3][4];
+};
+
+__device__ __forceinline__ ClipSrc clip_src(const ClipEntry& ce) {
+    ClipSrc c;
+    c.rank_base = ce.rank_base;
+    c.info = ce.info;
+    c.view = ce.view;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) c.v[k][j] = ce.v[k][j];
+    return c;
+}
+
+// A triangle that found the clip queue full, kept in a free record slot of
+// its own meshlet (slot 127 - j for the meshlet's j-th such triangle: kept
+// records and clipped triangles together never exceed the meshlet's 128
+// slots). Its clip-space vertices are recomputed from the meshlet.
+struct ClipRef {
+    uint32_t rank_base;
+    uint32_t info;                 // as ClipEntry::info
+    uint32_t meshlet;              // meshlet index in the scene
+    uint32_t tri;                  // triangle index in the meshlet
+    uint32_t view;
+    uint32_t pad[3];
+};
+static_assert(sizeof(ClipRef) == sizeof(TriRecord), "a clip reference takes one record slot");
+
 // `Alpha`: the launch has alpha-tested materials (p.alpha is set).
 template <bool Alpha>
 __device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, float (&lds)[4][64],
@@ -447,7 +483,7 @@ __device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, fl
     const uint32_t seq = V.seq_base + w;
     TriRecord* out = p.records + (size_t)i * 128;
     const uint32_t flags_hi = v << 16;
-    uint32_t kept = 0, clipped = 0;
+    uint32_t kept = 0, clipped = 0, overflowed = 0;
 
     for (uint32_t round = 0; round < 2; round++) {
         if (round * 64 >= nt) break;
@@ -515,27 +551,50 @@ __device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, fl
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(&p.scratch->num_clip, n);
             base = __shfl(base, 0);
-            if (nontrivial) {
-                uint32_t slot = base + mbcnt(cm);
-                if (slot < p.clip_capacity) {
-                    ClipEntry ce;
-                    ce.rank_base = make_rank(seq, packet, 1, plane, 0);
-                    ce.info = partial | ((uint32_t)cull_mode << 8) | ((alpha ? 1u : 0u) << 10);
+            const uint32_t info = partial | ((uint32_t)cull_mode << 8) | ((alpha ? 1u : 0u) << 10);
+            const bool queued = nontrivial && base + mbcnt(cm) < p.clip_capacity;
+            if (queued) {
+                ClipEntry ce;
+                ce.rank_base = make_rank(seq, packet, 1, plane, 0);
+                ce.info = info;
 #pragma unroll
-                    for (int k = 0; k < 3; k++)
+                for (int k = 0; k < 3; k++)
 #pragma unroll
-                        for (int c = 0; c < 4; c++) ce.v[k][c] = vv[k][c];
-                    ce.view = v;
-                    ce.pad = 0;
-                    p.clip_queue[slot] = ce;
+                    for (int c = 0; c < 4; c++) ce.v[k][c] = vv[k][c];
+                ce.view = v;
+                ce.pad = 0;
+                p.clip_queue[base + mbcnt(cm)] = ce;
+            }
+            // Triangles that found the queue full go into free slots at the
+            // top of the meshlet's own record block, to be clipped after the
+            // binning (pipelines without that pass count them as dropped).
+            const unsigned long long om = __ballot(nontrivial && !queued);
+            if (om) {
+                if (!p.exact_overflow) {
+                    if (lane == 0) atomicAdd(&p.counters->dropped, (unsigned long long)__popcll(om));
                 } else {
-                    atomicAdd(&p.counters->dropped, 1ull);
+                    if (nontrivial && !queued) {
+                        ClipRef r;
+                        r.rank_base = make_rank(seq, packet, 1, plane, 0);
+                        r.info = info;
+                        r.meshlet = f.mi;
+                        r.tri = t;
+                        r.view = v;
+                        r.pad[0] = r.pad[1] = r.pad[2] = 0;
+                        reinterpret_cast<ClipRef*>(out)[127 - (overflowed + mbcnt(om))] = r;
+                    }
+                    overflowed += (uint32_t)__popcll(om);
                 }
             }
         }
     }
     if (lane == 0) {
         p.meshlet_counts[i] = kept;
+        if (overflowed) {
+            // listed once; at most one entry per visible meshlet
+            const uint32_t k = atomicAdd(&p.scratch->num_clip_overflow, 1u);
+            p.clip_overflow[k] = make_uint2(i, overflowed);
+        }
         if (counting(p)) {
             atomicAdd(&p.counters->triangles_processed, (unsigned long long)nt);
             atomicAdd(&p.counters->triangles_rasterized, (unsigned long long)kept);
@@ -584,145 +643,216 @@ __device__ __forceinline__ float lane_get(float v, uint32_t src) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute((int)(src << 2), __float_as_int(v)));
 }
 
-// Clips queued triangle q with the whole wave. Lanes whose fan triangle
-// survives setup return true with the record and its index; the records
-// are also stored after the meshlet slots.
-// With `Alpha` the barycentric attributes travel with the positions.
 template <bool Alpha>
-__device__ bool clip_one(const DrawParams& p, uint32_t q, uint32_t lane, TriRecord& out, uint32_t& out_index) {
+__device__ void raster_now(const DrawParams& p, const TriRecord& rec, const AlphaRecord& A, uint32_t lane);
+
+// Clips one triangle with the whole wave. With `Store`, lanes whose fan
+// triangle survives setup return true with the record and its index, the
+// record stored after the meshlet slots (the region holds 8 records per
+// queue entry, as many as one triangle can yield, so it cannot fill up).
+// Without `Store` (triangles that found the queue full) the surviving fan
+// triangles are drawn at once by the wave (raster_now).
+// With `Alpha` the barycentric attributes travel with the positions.
+template <bool Alpha, bool Store>
+__device__ bool clip_one(const DrawParams& p, const ClipSrc& ce, uint32_t lane, TriRecord& out,
+                         uint32_t& out_index) {
     constexpr int NC = Alpha ? 6 : 4;   // clipped channels
-    {
-        const ClipEntry& ce = p.clip_queue[q];
-        const ViewParams& VP = view_of(p, ce.view);
-        const int hw = VP.width / 2, hh = VP.height / 2;
-        // x, y, z, w and the barycentric attributes (u, v) of the source
-        // triangle's vertices, (0,0), (1,0), (0,1) (Rasterizer.cpp:399-408)
-        float V[NC];
+    const ViewParams& VP = view_of(p, ce.view);
+    const int hw = VP.width / 2, hh = VP.height / 2;
+    // x, y, z, w and the barycentric attributes (u, v) of the source
+    // triangle's vertices, (0,0), (1,0), (0,1) (Rasterizer.cpp:399-408)
+    float V[NC];
 #pragma unroll
-        for (int c = 0; c < 4; c++) V[c] = lane < 3 ? ce.v[lane][c] : 0.0f;
-        if constexpr (Alpha) {
-            V[4] = lane == 1 ? 1.0f : 0.0f;
-            V[5] = lane == 2 ? 1.0f : 0.0f;
+    for (int c = 0; c < 4; c++) V[c] = lane == 0 ? ce.v[0][c] : (lane == 1 ? ce.v[1][c] : (lane == 2 ? ce.v[2][c] : 0.0f));
+    if constexpr (Alpha) {
+        V[4] = lane == 1 ? 1.0f : 0.0f;
+        V[5] = lane == 2 ? 1.0f : 0.0f;
+    }
+    uint32_t vc = 3, generated = 0;
+    const uint32_t codes = ce.info & 0x3Fu;
+    for (uint32_t plane = 0; plane < 6; plane++) {
+        if (!((codes >> plane) & 1u)) continue;
+        const float scale = plane < 2 ? VP.guard_x : (plane < 4 ? VP.guard_y : 1.0f);
+        const uint32_t comp = plane >> 1;
+        const bool neg = plane & 1u;
+        const bool active = lane < vc;
+        const uint32_t nxt = (lane + 1 < vc) ? lane + 1 : 0;
+        float Vn[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) Vn[c] = lane_get(V[c], nxt);
+        float ca = comp == 0 ? V[0] : (comp == 1 ? V[1] : V[2]);
+        float cb = comp == 0 ? Vn[0] : (comp == 1 ? Vn[1] : Vn[2]);
+        if (neg) { ca = -ca; cb = -cb; }
+        const float da = __builtin_fmaf(V[3], scale, ca);
+        const float db = __builtin_fmaf(Vn[3], scale, cb);
+        const bool in = active && da >= 0.0f;
+        const bool cross = active && ((da >= 0.0f) != (db >= 0.0f));
+        // crossings are created in polygon order while room is left
+        const unsigned long long xm = __ballot(cross);
+        const uint32_t xrank = mbcnt(xm);
+        const bool make = cross && (generated + xrank < 16);
+        const uint32_t made = min((uint32_t)__popcll(xm), 16u - generated);
+        const uint32_t cnt = (in ? 1u : 0u) + (make ? 1u : 0u);
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t o = __shfl_up(incl, d);
+            if ((int)lane >= d) incl += o;
         }
-        uint32_t vc = 3, generated = 0;
-        const uint32_t codes = ce.info & 0x3Fu;
-        for (uint32_t plane = 0; plane < 6; plane++) {
-            if (!((codes >> plane) & 1u)) continue;
-            const float scale = plane < 2 ? VP.guard_x : (plane < 4 ? VP.guard_y : 1.0f);
-            const uint32_t comp = plane >> 1;
-            const bool neg = plane & 1u;
-            const bool active = lane < vc;
-            const uint32_t nxt = (lane + 1 < vc) ? lane + 1 : 0;
-            float Vn[NC];
+        const uint32_t outCount = __shfl(incl, 63);
+        const uint32_t pos = incl - cnt;
+        float I[NC] = {};
+        if (make) {
+            const float t = da / (da - db);
 #pragma unroll
-            for (int c = 0; c < NC; c++) Vn[c] = lane_get(V[c], nxt);
-            float ca = comp == 0 ? V[0] : (comp == 1 ? V[1] : V[2]);
-            float cb = comp == 0 ? Vn[0] : (comp == 1 ? Vn[1] : Vn[2]);
-            if (neg) { ca = -ca; cb = -cb; }
-            const float da = __builtin_fmaf(V[3], scale, ca);
-            const float db = __builtin_fmaf(Vn[3], scale, cb);
-            const bool in = active && da >= 0.0f;
-            const bool cross = active && ((da >= 0.0f) != (db >= 0.0f));
-            // crossings are created in polygon order while room is left
-            const unsigned long long xm = __ballot(cross);
-            const uint32_t xrank = mbcnt(xm);
-            const bool make = cross && (generated + xrank < 16);
-            const uint32_t made = min((uint32_t)__popcll(xm), 16u - generated);
-            const uint32_t cnt = (in ? 1u : 0u) + (make ? 1u : 0u);
-            uint32_t incl = cnt;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                uint32_t o = __shfl_up(incl, d);
-                if ((int)lane >= d) incl += o;
-            }
-            const uint32_t outCount = __shfl(incl, 63);
-            const uint32_t pos = incl - cnt;
-            float I[NC] = {};
-            if (make) {
-                const float t = da / (da - db);
-#pragma unroll
-                for (int c = 0; c < NC; c++) I[c] = __builtin_fmaf(Vn[c], t, __builtin_fmaf(-t, V[c], V[c]));
-            }
-            if (vc < 3) break;
-            // output lane j pulls from source lane src(j) = #{i : incl_i <= j}
-            uint32_t src = 0;
-            for (uint32_t k = 0; k < vc; k++) src += (__shfl(incl, (int)k) <= lane) ? 1u : 0u;
-            src = min(src, 63u);
-            const uint32_t srcPos = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)pos);
-            const int srcIn = __builtin_amdgcn_ds_bpermute((int)(src << 2), in ? 1 : 0);
-            const bool isVert = srcIn && lane == srcPos;
-#pragma unroll
-            for (int c = 0; c < NC; c++) {
-                const float a = lane_get(V[c], src), b = lane_get(I[c], src);
-                V[c] = isVert ? a : b;
-            }
-            generated += made;
-            vc = outCount;
+            for (int c = 0; c < NC; c++) I[c] = __builtin_fmaf(Vn[c], t, __builtin_fmaf(-t, V[c], V[c]));
         }
-        if (vc < 3) return false;
-        const int cull_mode = (ce.info >> 8) & 3;
-        // fan triangle f = (v0, v[f+1], v[f+2]) on lane f
-        const uint32_t f = lane;
-        float tri[3][4];
+        if (vc < 3) break;
+        // output lane j pulls from source lane src(j) = #{i : incl_i <= j}
+        uint32_t src = 0;
+        for (uint32_t k = 0; k < vc; k++) src += (__shfl(incl, (int)k) <= lane) ? 1u : 0u;
+        src = min(src, 63u);
+        const uint32_t srcPos = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)pos);
+        const int srcIn = __builtin_amdgcn_ds_bpermute((int)(src << 2), in ? 1 : 0);
+        const bool isVert = srcIn && lane == srcPos;
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            tri[0][c] = lane_get(V[c], 0);
-            tri[1][c] = lane_get(V[c], min(f + 1, 63u));
-            tri[2][c] = lane_get(V[c], min(f + 2, 63u));
+        for (int c = 0; c < NC; c++) {
+            const float a = lane_get(V[c], src), b = lane_get(I[c], src);
+            V[c] = isVert ? a : b;
         }
-        // TriangleClipData of the fan triangle (Rasterizer.cpp:461-464)
-        float u0 = 0, v0 = 0, ru1 = 0, rv1 = 0, ru2 = 0, rv2 = 0;
-        if constexpr (Alpha) {
-            u0 = lane_get(V[4], 0);
-            v0 = lane_get(V[5], 0);
-            ru1 = lane_get(V[4], min(f + 1, 63u)) - u0;
-            rv1 = lane_get(V[5], min(f + 1, 63u)) - v0;
-            ru2 = lane_get(V[4], min(f + 2, 63u)) - u0;
-            rv2 = lane_get(V[5], min(f + 2, 63u)) - v0;
-        }
-        const bool alpha = Alpha && ((ce.info >> 10) & 1u);
-        TriRecord rec;
-        float wt[4];
-        bool keep = false;
-        if (f + 2 < vc && f < 8) {
-            keep = setup_triangle(p.tables, tri, hw, hh, cull_mode, rec, wt);
-            rec.rank = ce.rank_base | f;
-            rec.flags = (alpha ? kRecAlpha : 0u) | (ce.view << 16);
-        }
-        const unsigned long long km = __ballot(keep);
-        if (!km) return false;
+        generated += made;
+        vc = outCount;
+    }
+    if (vc < 3) return false;
+    const int cull_mode = (ce.info >> 8) & 3;
+    // fan triangle f = (v0, v[f+1], v[f+2]) on lane f
+    const uint32_t f = lane;
+    float tri[3][4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        tri[0][c] = lane_get(V[c], 0);
+        tri[1][c] = lane_get(V[c], min(f + 1, 63u));
+        tri[2][c] = lane_get(V[c], min(f + 2, 63u));
+    }
+    // TriangleClipData of the fan triangle (Rasterizer.cpp:461-464)
+    float u0 = 0, v0 = 0, ru1 = 0, rv1 = 0, ru2 = 0, rv2 = 0;
+    if constexpr (Alpha) {
+        u0 = lane_get(V[4], 0);
+        v0 = lane_get(V[5], 0);
+        ru1 = lane_get(V[4], min(f + 1, 63u)) - u0;
+        rv1 = lane_get(V[5], min(f + 1, 63u)) - v0;
+        ru2 = lane_get(V[4], min(f + 2, 63u)) - u0;
+        rv2 = lane_get(V[5], min(f + 2, 63u)) - v0;
+    }
+    const bool alpha = Alpha && ((ce.info >> 10) & 1u);
+    TriRecord rec;
+    float wt[4];
+    bool keep = false;
+    if (f + 2 < vc && f < 8) {
+        keep = setup_triangle(p.tables, tri, hw, hh, cull_mode, rec, wt);
+        rec.rank = ce.rank_base | f;
+        rec.flags = (alpha ? kRecAlpha : 0u) | (ce.view << 16);
+    }
+    const unsigned long long km = __ballot(keep);
+    if (!km) return false;
+    if (lane == 0 && counting(p)) atomicAdd(&p.counters->triangles_rasterized, (unsigned long long)__popcll(km));
+    AlphaRecord a = {};
+    if (alpha && keep) {
+        // the source triangle's UVs and material, from the meshlet and
+        // triangle the rank names
+        const uint32_t seq = ce.rank_base >> 11;
+        const uint32_t t = ((ce.rank_base >> 8) & 7u) * 16 + ((ce.rank_base >> 3) & 15u);
+        const uint32_t mi = VP.meshlet_offset + (seq - VP.seq_base);
+        const DevMaterial dm = VP.materials[VP.m.material_id[mi]];
+        a.w0 = wt[0]; a.w0s = wt[1]; a.w1s = wt[2]; a.w2s = wt[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            a.uv[k] = VP.m.texcoords[(size_t)mi * 64 + (VP.m.indices[(size_t)mi * 384 + k * 128 + t] & 63u)];
+        a.tex = (uint32_t)dm.texture | ((uint32_t)dm.alpha_cutoff << 24);
+        a.ru[0] = u0; a.ru[1] = ru1; a.ru[2] = ru2;
+        a.rv[0] = v0; a.rv[1] = rv1; a.rv[2] = rv2;
+    }
+    if constexpr (Store) {
         uint32_t base = 0;
         if (lane == 0) base = atomicAdd(&p.scratch->num_records, (uint32_t)__popcll(km));
         base = __shfl(base, 0);
-        if (lane == 0 && counting(p)) atomicAdd(&p.counters->triangles_rasterized, (unsigned long long)__popcll(km));
-        if (!keep) return false;
         const uint32_t slot = base + mbcnt(km);
-        if (slot >= p.clip_record_capacity) {
+        if (!keep) return false;
+        if (slot >= p.clip_record_capacity) {     // not reached: see above
             atomicAdd(&p.counters->dropped, 1ull);
             return false;
         }
         p.records[p.clip_record_base + slot] = rec;
-        if (alpha) {
-            // the source triangle's UVs and material, from the meshlet and
-            // triangle the rank names
-            const uint32_t seq = ce.rank_base >> 11;
-            const uint32_t t = ((ce.rank_base >> 8) & 7u) * 16 + ((ce.rank_base >> 3) & 15u);
-            const uint32_t mi = VP.meshlet_offset + (seq - VP.seq_base);
-            const DevMaterial dm = VP.materials[VP.m.material_id[mi]];
-            AlphaRecord a;
-            a.w0 = wt[0]; a.w0s = wt[1]; a.w1s = wt[2]; a.w2s = wt[3];
-#pragma unroll
-            for (int k = 0; k < 3; k++)
-                a.uv[k] = VP.m.texcoords[(size_t)mi * 64 + (VP.m.indices[(size_t)mi * 384 + k * 128 + t] & 63u)];
-            a.tex = (uint32_t)dm.texture | ((uint32_t)dm.alpha_cutoff << 24);
-            a.ru[0] = u0; a.ru[1] = ru1; a.ru[2] = ru2;
-            a.rv[0] = v0; a.rv[1] = rv1; a.rv[2] = rv2;
-            a.pad[0] = a.pad[1] = 0;
-            p.alpha[p.clip_record_base + slot] = a;
-        }
+        if (alpha) p.alpha[p.clip_record_base + slot] = a;
         out = rec;
         out_index = p.clip_record_base + slot;
         return true;
+    } else {
+        // one fan triangle at a time, broadcast to the wave
+        unsigned long long nm = km;
+        while (nm) {
+            const int l = __ffsll((long long)nm) - 1;
+            nm &= nm - 1;
+            TriRecord r;
+#pragma unroll
+            for (int k = 0; k < 3; k++) r.pos[k] = (uint32_t)__shfl((int)rec.pos[k], l);
+            r.z0 = __shfl(rec.z0, l);
+            r.z10 = __shfl(rec.z10, l);
+            r.z20 = __shfl(rec.z20, l);
+            r.rank = (uint32_t)__shfl((int)rec.rank, l);
+            r.flags = (uint32_t)__shfl((int)rec.flags, l);
+            AlphaRecord ar = {};
+            if constexpr (Alpha) {
+                if (r.flags & kRecAlpha) {
+                    ar.w0 = __shfl(a.w0, l); ar.w0s = __shfl(a.w0s, l);
+                    ar.w1s = __shfl(a.w1s, l); ar.w2s = __shfl(a.w2s, l);
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        ar.uv[k] = (uint32_t)__shfl((int)a.uv[k], l);
+                        ar.ru[k] = __shfl(a.ru[k], l);
+                        ar.rv[k] = __shfl(a.rv[k], l);
+                    }
+                    ar.tex = (uint32_t)__shfl((int)a.tex, l);
+                }
+            }
+            raster_now<Alpha>(p, r, ar, lane);
+        }
+        return false;
+    }
+}
+
+// The triangles setup could not queue (ClipRef): one wave per listed
+// meshlet clips its references in order and draws their fan triangles at
+// once. Runs after setup, before the bin raster.
+template <bool Alpha>
+__device__ void clip_overflow(const DrawParams& p, uint32_t first_wave, uint32_t waves) {
+    const uint32_t lane = lane_id();
+    const uint32_t n = p.scratch->num_clip_overflow;
+    for (uint32_t k = first_wave; k < n; k += waves) {
+        const uint2 e = p.clip_overflow[k];      // (visible slot, references)
+        const ClipRef* refs = reinterpret_cast<const ClipRef*>(p.records + (size_t)e.x * 128);
+        for (uint32_t j = 0; j < e.y; j++) {
+            const ClipRef r = refs[127 - j];
+            const ViewParams& V = view_of(p, r.view);
+            ClipSrc cs;
+            cs.rank_base = r.rank_base;
+            cs.info = r.info;
+            cs.view = r.view;
+            // the vertices as setup transformed them (ShadeMeshlet and the
+            // gather_preload64 wrap of the index)
+            const float* P = V.m.positions + (size_t)r.meshlet * 192;
+            const uint8_t* I = V.m.indices + (size_t)r.meshlet * 384;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const uint32_t ix = I[k * 128 + r.tri];
+                const uint32_t vi = ix < 32 ? ix : 32 + (ix & 31);
+#pragma unroll
+                for (int c = 0; c < 4; c++) cs.v[k][c] = xform(V.clip, c, P[vi], P[64 + vi], P[128 + vi]);
+            }
+            TriRecord unused;
+            uint32_t unused_index;
+            (void)clip_one<Alpha, false>(p, cs, lane, unused, unused_index);
+        }
     }
 }
 
@@ -734,8 +864,10 @@ __global__ void __launch_bounds__(256) k_clip(DrawParams p) {
     for (uint32_t q = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; q < n; q += waves) {
         TriRecord rec;
         uint32_t idx;
-        (void)clip_one<Alpha>(p, q, lane, rec, idx);
+        (void)clip_one<Alpha, true>(p, clip_src(p.clip_queue[q]), lane, rec, idx);
     }
+    // then the triangles that found the queue full (direct route)
+    clip_overflow<Alpha>(p, blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave, waves);
 }
 
 
@@ -880,6 +1012,68 @@ __device__ __forceinline__ bool alpha_test_tile(const ApproxTables& T, const Alp
     const TileUV t = tile_uv(T, A, clipped, tex, e1, e2, lane);
     const uint32_t color = sample_level(tex, t.U, t.V, t.mip, !t.minify);
     return color >= ((A.tex >> 24) << 24);
+}
+
+// Draws one record with the whole wave, four 4x4 tiles at a time (lane =
+// pixel), for a clipped fan triangle that found no record slot: into the
+// view's spill keys when the launch bins (marking each bin the record passes
+// a fragment in, so that bin's raster merges its spill keys) or straight
+// into the vis-buffer on the direct route. `rec` and `A` are wave-uniform.
+template <bool Alpha>
+__device__ void raster_now(const DrawParams& p, const TriRecord& rec, const AlphaRecord& A, uint32_t lane) {
+    const ViewParams& V = view_of(p, rec.flags >> 16);
+    const uint32_t W = (uint32_t)V.width;
+    TriLds T;
+    uint32_t bmin, bmax;
+    load_tri(rec, V.width / 2, V.height / 2, T, bmin, bmax);
+    const uint32_t x0 = (uint32_t)lo16(bmin), y0 = (uint32_t)hi16(bmin);
+    const uint32_t tw = ((uint32_t)lo16(bmax) - x0) >> 2, total = tw * (((uint32_t)hi16(bmax) - y0) >> 2);
+    const bool binned = p.total_bins != 0;
+    unsigned long long* dst_keys = binned ? V.spill_keys : V.keys;
+    const bool alpha = Alpha && (rec.flags & kRecAlpha) != 0;
+    const float clearDepth = V.clear_depth;        // in a register: see raster_bin
+    const uint32_t sub = lane >> 4, px = lane & 3, py = (lane >> 2) & 3;
+    uint32_t covered = 0, passed = 0;
+    for (uint32_t base = 0; base < total; base += 4) {
+        const uint32_t j = base + sub;
+        if (j >= total) continue;                    // uniform across each 16-lane group
+        const uint32_t ty = j / tw, tx = j - ty * tw;
+        const uint32_t x = x0 + tx * 4 + px, y = y0 + ty * 4 + py;
+        const int32_t w0 = edge_at(T.e0, T.a12, T.b12, x, y);
+        const int32_t w1 = edge_at(T.e1, T.a20, T.b20, x, y);
+        const int32_t w2 = edge_at(T.e2, T.a01, T.b01, x, y);
+        const bool cov = (w0 | w1 | w2) >= 0;
+        const float depth = __builtin_fmaf((float)w1, T.z10, __builtin_fmaf((float)w2, T.z20, T.z0));
+        bool pass = cov && depth > clearDepth;
+        if constexpr (Alpha) {
+            if (alpha && (__ballot(pass) & (0xFFFFull << (lane & 48u)))) {
+                const bool ok = alpha_test_tile(p.tables, A, ((T.keylo >> 7) & 1u) == 0, V.textures, w1, w2, lane);
+                pass = pass && ok;
+            }
+        }
+        if (cov) covered++;
+        if (pass) {
+            passed++;
+            const unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
+            unsigned long long* dst = dst_keys + pixel_offset(W, x, y);
+            if (key > __atomic_load_n(dst, __ATOMIC_RELAXED)) atomicMax(dst, key);
+        }
+        // the ballot is taken by every lane of the group (a ballot behind the
+        // lane test would see the group's first lane only)
+        const unsigned long long passing = __ballot(pass) & (0xFFFFull << (lane & 48u));
+        if (binned && (lane & 15u) == 0 && passing)
+            p.bin_flags[V.bin_base + ((y0 + ty * 4) >> p.bin_shift) * V.bins_x + ((x0 + tx * 4) >> p.bin_shift)] = 1u;
+    }
+    if (counting(p)) {
+        for (int d = 32; d > 0; d >>= 1) {
+            covered += __shfl_xor(covered, d);
+            passed += __shfl_xor(passed, d);
+        }
+        if (lane == 0) {
+            if (covered) atomicAdd(&p.counters->fragments_covered, (unsigned long long)covered);
+            if (passed) atomicAdd(&p.counters->fragments_passed, (unsigned long long)passed);
+        }
+    }
 }
 
 // Each wave takes a chunk of up to 64 records (meshlet slots or clipped
@@ -1289,7 +1483,7 @@ __device__ __forceinline__ uint32_t wave_sort_u32(uint32_t v, uint32_t lane) {
 // an exclusive scan and inserts the pairs in parallel, so a triangle
 // spanning many bins costs one pass of the wave, not a serial loop.
 struct BinScratch {                     // one wave's slice of the block's LDS
-    uint32_t start[64], org[64], bw[64], spill[64], ridx[64];
+    uint32_t start[64], org[64], bw[64], ridx[64], spill[64];
 };
 
 // Inserts the wave's records (one per lane where `valid`) into the screen
@@ -1326,8 +1520,8 @@ __device__ void bin_records(const DrawParams& p, bool valid, const TriRecord& re
     sm.start[lane] = incl - nb;
     sm.org[lane] = org;
     sm.bw[lane] = bw | (pitch << 16);
-    sm.spill[lane] = 0;
     sm.ridx[lane] = ridx;
+    sm.spill[lane] = 0;
     __builtin_amdgcn_wave_barrier();
     for (uint32_t base = 0; base < total; base += 64) {
         const uint32_t j = base + lane;
@@ -1369,9 +1563,12 @@ __device__ void bin_records(const DrawParams& p, bool valid, const TriRecord& re
             else sm.spill[k] = 1;
         }
     }
+    // A record with a pair past its bin's capacity goes to the overflow list
+    // once and is flagged, so the bins that did take it skip it: the overflow
+    // raster draws it over its whole render box.
     __builtin_amdgcn_wave_barrier();
     const bool spilled = valid && sm.spill[lane] != 0;
-    unsigned long long spm = __ballot(spilled);
+    const unsigned long long spm = __ballot(spilled);
     if (spm) {
         uint32_t base = 0;
         if (lane == 0) base = atomicAdd(&p.scratch->num_spill, (uint32_t)__popcll(spm));
@@ -1391,23 +1588,29 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
     __shared__ BinScratch s_bins[4];
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
-    const uint32_t nchunks = num_chunks(p, false);
+    const uint32_t nvis = p.scratch->num_visible;
     const uint32_t waves = gridDim.x * 4;
     const uint32_t first = blockIdx.x * 4 + wid;
     unsigned long long entries = 0;
-    for (uint32_t c = first; c < nchunks; c += waves) {
-        const Chunk ck = get_chunk(p, c);
-        if (ck.count == 0) continue;
-        const bool valid = lane < ck.count;
-        TriRecord rec;
-        if (valid) rec = p.records[ck.base + lane];
-        bin_records(p, valid, rec, ck.base + lane, lane, s_bins[wid], entries);
+    // one visible meshlet's records at a time (up to two rounds of 64)
+    for (uint32_t i = first; i < nvis; i += waves) {
+        const uint32_t cnt = p.meshlet_counts[i];
+        for (uint32_t half = 0; half * 64 < cnt; half++) {
+            const uint32_t base = i * 128 + half * 64;
+            const bool valid = half * 64 + lane < cnt;
+            TriRecord rec;
+            if (valid) rec = p.records[base + lane];
+            bin_records(p, valid, rec, base + lane, lane, s_bins[wid], entries);
+        }
     }
+    // Queued triangles go to the last waves of the grid, which have no
+    // meshlet when the grid holds more waves than visible meshlets: clipping
+    // then runs beside the binning instead of after it.
     const uint32_t nclip = min(p.scratch->num_clip, p.clip_capacity);
-    for (uint32_t q = first; q < nclip; q += waves) {
+    for (uint32_t q = waves - 1 - first; q < nclip; q += waves) {
         TriRecord rec;
         uint32_t ridx = 0;
-        const bool valid = clip_one<Alpha>(p, q, lane, rec, ridx);
+        const bool valid = clip_one<Alpha, true>(p, clip_src(p.clip_queue[q]), lane, rec, ridx);
         if (__ballot(valid)) bin_records(p, valid, rec, ridx, lane, s_bins[wid], entries);
     }
     if (counting(p) && lane == 0 && entries) atomicAdd(&p.counters->bin_entries, entries);
@@ -1442,25 +1645,32 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
     const uint32_t x0 = bx * BS, y0 = by * BS;
     const uint32_t x1 = min(x0 + BS, W), y1 = min(y0 + BS, H);
     const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
-    // The first batch of list entries is loaded together with the count
-    // (entries past the count are never used; the read stays inside the
-    // bin's list), which takes one dependent load off every bin.
+    // The first kPrefetchEntries list entries are loaded together with the
+    // count (entries past the count are never used; the read stays inside
+    // the bin's list), which takes one dependent load off most bins; the
+    // rest wait for the count.
+    constexpr uint32_t kPrefetchEntries = kThreads < 128 ? kThreads : 128;
     const uint32_t* list = p.bin_lists + (size_t)bin * p.bin_capacity;
-    const uint32_t first_entry = tid < p.bin_capacity ? list[tid] : 0u;
+    const uint32_t first_entry = tid < min(p.bin_capacity, kPrefetchEntries) ? list[tid] : 0u;
     const uint32_t count = p.bin_counts[bin];
+    const uint32_t flagged = p.bin_flags[bin];
     const uint32_t n = min(count, p.bin_capacity);
+    // records on the overflow list and the fan triangles of triangles that
+    // found the clip queue full were drawn into the view's spill keys, which
+    // flagged the bins they reached
+    const bool merge = flagged != 0;
     // the visible count has had its last reader (the scatter): leave it
     // cleared for the next draw's cull
     if (bin == 0 && tid == 0) p.scratch->num_visible = 0;
-    // Records that overflowed a bin (the spill list, usually empty) are drawn
-    // by every bin block they overlap, after the bin's own list; their list
-    // entries elsewhere are skipped, so each is drawn (and counted) once per
-    // pixel.
-    const uint32_t ns = min(p.scratch->num_spill, p.clip_record_base + p.clip_record_capacity);
-    const uint32_t nall = n + ns;
-    if (nall == 0 && !V.fresh) return;  // uniform across the block
+    if (tid == 0 && flagged) p.bin_flags[bin] = 0u;
+    if (n == 0 && !merge && !V.fresh) return;  // uniform across the block
     const int hw = V.width / 2, hh = V.height / 2;
-    const unsigned long long clearKey = (unsigned long long)fbits(V.clear_depth) << 32;
+    // view fields the tile loop needs, in registers: the compiler does not
+    // move a load of the view table (global memory for batches) across the
+    // LDS atomics, so a field read inside the loop is a load and a wait per
+    // tile
+    const float clearDepth = V.clear_depth;
+    const unsigned long long clearKey = (unsigned long long)fbits(clearDepth) << 32;
     const uint32_t rowElems = (x1 - x0) * 4;   // elements of one 4-row strip of tiles
     unsigned long long* gkeys = V.keys;
 
@@ -1473,27 +1683,24 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
     }
 
     const bool countFrags = counting(p);
-    uint32_t covered = 0, passed = 0;
+    uint32_t covered = 0, passed = 0, tiles = 0;
     const uint32_t g = tid >> 4, px = tid & 3, py = (tid >> 2) & 3;
-    for (uint32_t base = 0; base < nall; base += kThreads) {
+    for (uint32_t base = 0; base < n; base += kThreads) {
         const uint32_t e = base + tid;
         uint32_t ntiles = 0;
         TriLds t;
-        if (e < nall) {
-            const bool fromSpill = e >= n;
-            const uint32_t r = fromSpill ? p.spill[e - n] : (base == 0 ? first_entry : list[e]);
+        if (e < n) {
+            const uint32_t r = (base == 0 && e < kPrefetchEntries) ? first_entry : list[e];
+            const TriRecord rec = p.records[r];
             uint32_t bmin, bmax;
-            load_tri(p.records[r], hw, hh, t, bmin, bmax);
+            load_tri(rec, hw, hh, t, bmin, bmax);
             uint32_t cx0 = max((uint32_t)lo16(bmin), x0), cy0 = max((uint32_t)hi16(bmin), y0);
             uint32_t cx1 = min((uint32_t)lo16(bmax), x1), cy1 = min((uint32_t)hi16(bmax), y1);
-            const uint32_t rflags = p.records[r].flags;
-            if constexpr (Alpha) talpha[tid] = (rflags & kRecAlpha) ? r : 0xFFFFFFFFu;
+            if constexpr (Alpha) talpha[tid] = (rec.flags & kRecAlpha) ? r : 0xFFFFFFFFu;
             t.org = 0;
             t.tw = 1;
-            // a spilled record comes from the spill list (of any view: only
-            // this bin's view draws it), never from a bin list
-            const bool skip = fromSpill ? (rflags >> 16) != vidx : (rflags & kRecSpilled) != 0;
-            if (cx1 > cx0 && cy1 > cy0 && !skip) {
+            // a record on the overflow list is drawn by the overflow raster
+            if (cx1 > cx0 && cy1 > cy0 && !(rec.flags & kRecSpilled)) {
                 t.org = (cx0 - x0) | ((cy0 - y0) << 16);
                 t.tw = (cx1 - cx0) >> 2;
                 ntiles = t.tw * ((cy1 - cy0) >> 2);
@@ -1516,13 +1723,14 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
         }
         t.start = prev + incl - ntiles;
         tl[tid] = t;
-        if (e >= nall) tl[tid].start = 0xFFFFFFFFu;
+        if (e >= n) tl[tid].start = 0xFFFFFFFFu;
         __syncthreads();
 
         // group g walks tiles [j, j1) of the block's tile list
         const uint32_t per = (total + kGroups - 1) / kGroups;
         uint32_t j = min(g * per, total);
         const uint32_t j1 = min(j + per, total);
+        if (countFrags && j < j1) tiles += j1 - j;
         if (j < j1) {
             uint32_t k = 0;
 #pragma unroll
@@ -1570,9 +1778,9 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
                     if ((int32_t)(w0 | w1 | w2) >= 0) {
                         const float depth = __builtin_fmaf((float)(int32_t)w1, T.z10,
                                                            __builtin_fmaf((float)(int32_t)w2, T.z20, T.z0));
-                        covered++;
-                        if (depth > V.clear_depth) {
-                            passed++;
+                        if (countFrags) covered++;
+                        if (depth > clearDepth) {
+                            if (countFrags) passed++;
                             // an LDS max without a returned value: no wait on the
                             // old key (a read-then-max would stall every step)
                             const unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
@@ -1583,15 +1791,15 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
                     const bool cov = (int32_t)(w0 | w1 | w2) >= 0;
                     const float depth = __builtin_fmaf((float)(int32_t)w1, T.z10,
                                                        __builtin_fmaf((float)(int32_t)w2, T.z20, T.z0));
-                    bool pass = cov && depth > V.clear_depth;
+                    bool pass = cov && depth > clearDepth;
                     if (ai != 0xFFFFFFFFu && (__ballot(pass) & (0xFFFFull << (lane & 48u)))) {
                         const bool ok = alpha_test_tile(p.tables, p.alpha[ai], ((T.keylo >> 7) & 1u) == 0,
                                                         V.textures, (int32_t)w1, (int32_t)w2, lane);
                         pass = pass && ok;
                     }
-                    if (cov) covered++;
+                    if (countFrags && cov) covered++;
                     if (pass) {
-                        passed++;
+                        if (countFrags) passed++;
                         const unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
                         atomicMax(&keys[li], key);
                     }
@@ -1619,10 +1827,23 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
     }
 
     // each thread writes back the keys it initialised, so the next bin's
-    // initialisation needs no extra barrier
+    // initialisation needs no extra barrier; an overflowing bin takes the
+    // maximum with the spill keys of its pixels and zeroes them for the next
+    // draw
     for (uint32_t i = tid; i < BS * BS; i += kThreads) {
         uint32_t strip = i / (BS * 4), rem = i % (BS * 4);
-        if (rem < rowElems && y0 + strip * 4 < y1) gkeys[(size_t)(y0 + strip * 4) * W + x0 * 4 + rem] = keys[i];
+        if (rem < rowElems && y0 + strip * 4 < y1) {
+            const size_t gi = (size_t)(y0 + strip * 4) * W + x0 * 4 + rem;
+            unsigned long long val = keys[i];
+            if (merge) {
+                const unsigned long long sk = V.spill_keys[gi];
+                if (sk) {
+                    val = sk > val ? sk : val;
+                    V.spill_keys[gi] = 0ull;
+                }
+            }
+            gkeys[gi] = val;
+        }
     }
     // the bin's count, read by every thread before the first barrier of the
     // record loop (count > 0 means the loop ran), is cleared for the next draw
@@ -1636,18 +1857,53 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
             atomicAdd(&p.counters->fragments_covered, (unsigned long long)covered);
             atomicAdd(&p.counters->fragments_passed, (unsigned long long)passed);
         }
+        // one lane of each 16-lane group counted the group's tiles
+        if ((tid & 15u) == 0 && tiles) atomicAdd(&p.counters->tile_lanes, 16ull * tiles);
         if (tid == 0 && n) atomicAdd(&p.counters->bins_rasterized, 1ull);
     }
 }
+
+// The overflow raster: one wave per record on the overflow list draws it
+// over its whole render box into the view's spill keys (raster_now), marking
+// the bins it passes a fragment in; the bins' rasters skip the record and
+// merge the spill keys. The work is that of the listed records' areas,
+// however many bins or views the launch has.
+template <bool Alpha>
+__device__ void overflow_raster(const DrawParams& p, uint32_t first_wave, uint32_t waves) {
+    const uint32_t lane = lane_id();
+    const uint32_t ns = min(p.scratch->num_spill, p.spill_capacity);
+    for (uint32_t q = first_wave; q < ns; q += waves) {
+        const uint32_t r = p.spill[q];
+        const TriRecord rec = p.records[r];
+        AlphaRecord a = {};
+        if constexpr (Alpha) {
+            if (rec.flags & kRecAlpha) a = p.alpha[r];
+        }
+        raster_now<Alpha>(p, rec, a, lane);
+    }
+}
+
+// Blocks of the bin-order launch that run the overflow raster (they exit at
+// once when no pair overflowed, so an empty list costs no launch of its own).
+constexpr uint32_t kOverflowBlocks = 256;
 
 // Orders the bins for the raster, most records first: bin b goes to class
 // c = bit length of its record count (0 for an empty bin) and takes a slot
 // in that class's list. Work per bin grows with its records, and crowded
 // bins (at 32-pixel bins the largest hold about four times the mean) would
-// otherwise finish last on an idle machine.
+// otherwise finish last on an idle machine. Blocks past the bins' run the
+// overflow raster.
+template <bool Alpha>
 __global__ void __launch_bounds__(256) k_bin_order(DrawParams p) {
     __shared__ uint32_t s_count[kBinBuckets], s_base[kBinBuckets];
     const uint32_t tid = threadIdx.x;
+    const uint32_t order_blocks = (p.total_bins + 255) / 256;
+    if (blockIdx.x >= order_blocks) {
+        const uint32_t w = (blockIdx.x - order_blocks) * 4 + tid / kWave;
+        clip_overflow<Alpha>(p, w, kOverflowBlocks * 4);
+        overflow_raster<Alpha>(p, w, kOverflowBlocks * 4);
+        return;
+    }
     const uint32_t bin = blockIdx.x * 256 + tid;
     if (tid < kBinBuckets) s_count[tid] = 0;
     __syncthreads();
@@ -1701,6 +1957,7 @@ __global__ void k_reset_counters(DrawScratch* s) {
     s->num_clip = 0;
     s->num_visible = 0;
     s->num_spill = 0;
+    s->num_clip_overflow = 0;
     for (uint32_t c = 0; c < kBinBuckets; c++) s->bucket[c] = 0;
 }
 
@@ -1790,24 +2047,24 @@ __global__ void k_decode_overdraw(const unsigned long long* counts, const uint32
 // threads for every bin size, 512 or 1024 for 32- and 64-pixel bins (a
 // crowded large bin then has more waves to spread its tiles over).
 template <int BS, int T, bool Alpha>
-void launch_bin_raster_one(const DrawParams& p, hipStream_t s) {
-    hipLaunchKernelGGL((k_bin_raster<BS, T, Alpha>), dim3(p.total_bins), dim3(T), 0, s, p);
+void launch_bin_raster_one(const DrawParams& p, hipStream_t s, Profiler* prof) {
+    launch_kernel(prof, "bin_raster", k_bin_raster<BS, T, Alpha>, dim3(p.total_bins), dim3(T), 0, s, p);
 }
 
 template <int BS, bool Alpha>
-bool launch_bin_raster_bs(const DrawParams& p, dim3, hipStream_t s) {
+bool launch_bin_raster_bs(const DrawParams& p, hipStream_t s, Profiler* prof) {
     switch (p.raster_threads) {
-    case 128: launch_bin_raster_one<BS, 128, Alpha>(p, s); return true;
-    case 256: launch_bin_raster_one<BS, 256, Alpha>(p, s); return true;
+    case 128: launch_bin_raster_one<BS, 128, Alpha>(p, s, prof); return true;
+    case 256: launch_bin_raster_one<BS, 256, Alpha>(p, s, prof); return true;
     case 512:
         if constexpr (BS >= 32) {
-            launch_bin_raster_one<BS, 512, Alpha>(p, s);
+            launch_bin_raster_one<BS, 512, Alpha>(p, s, prof);
             return true;
         }
         return false;
     case 1024:
         if constexpr (BS >= 32) {
-            launch_bin_raster_one<BS, 1024, Alpha>(p, s);
+            launch_bin_raster_one<BS, 1024, Alpha>(p, s, prof);
             return true;
         }
         return false;
@@ -1817,12 +2074,12 @@ bool launch_bin_raster_bs(const DrawParams& p, dim3, hipStream_t s) {
 
 
 template <bool Alpha>
-bool launch_bin_raster(const DrawParams& p, dim3 grid, hipStream_t s) {
+bool launch_bin_raster(const DrawParams& p, hipStream_t s, Profiler* prof) {
     switch (p.bin_size) {
-    case 8: return launch_bin_raster_bs<8, Alpha>(p, grid, s);
-    case 16: return launch_bin_raster_bs<16, Alpha>(p, grid, s);
-    case 32: return launch_bin_raster_bs<32, Alpha>(p, grid, s);
-    case 64: return launch_bin_raster_bs<64, Alpha>(p, grid, s);
+    case 8: return launch_bin_raster_bs<8, Alpha>(p, s, prof);
+    case 16: return launch_bin_raster_bs<16, Alpha>(p, s, prof);
+    case 32: return launch_bin_raster_bs<32, Alpha>(p, s, prof);
+    case 64: return launch_bin_raster_bs<64, Alpha>(p, s, prof);
     default: return false;
     }
 }
@@ -1830,26 +2087,23 @@ bool launch_bin_raster(const DrawParams& p, dim3 grid, hipStream_t s) {
 void launch_front(const DrawParams& p, hipStream_t s, Profiler* prof, bool separate_clip) {
     if (!p.total_meshlets) hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, s, p.scratch);
     if (p.total_meshlets) {     // a batch of empty views still clears through the bin raster
-        Profiler::Scope sc(prof, "cull", s);
         if (p.total_meshlets <= 256u * 1024u)
-            hipLaunchKernelGGL(k_cull<1>, dim3((p.total_meshlets + 255) / 256), dim3(256), 0, s, p);
+            launch_kernel(prof, "cull", k_cull<1>, dim3((p.total_meshlets + 255) / 256), dim3(256), 0, s, p);
         else
-            hipLaunchKernelGGL(k_cull<4>, dim3((p.total_meshlets + 1023) / 1024), dim3(256), 0, s, p);
+            launch_kernel(prof, "cull", k_cull<4>, dim3((p.total_meshlets + 1023) / 1024), dim3(256), 0, s, p);
     }
     if (p.total_meshlets) {
-        Profiler::Scope sc(prof, "mesh_setup", s);
         const uint32_t want = (p.total_meshlets + 3) / 4;
         if (p.alpha)
-            hipLaunchKernelGGL(k_mesh_setup<true>, dim3(min(want, resident_blocks<k_mesh_setup<true>, 256>())),
-                               dim3(256), 0, s, p);
+            launch_kernel(prof, "mesh_setup", k_mesh_setup<true>,
+                          dim3(min(want, resident_blocks<k_mesh_setup<true>, 256>())), dim3(256), 0, s, p);
         else
-            hipLaunchKernelGGL(k_mesh_setup<false>, dim3(min(want, resident_blocks<k_mesh_setup<false>, 256>())),
-                               dim3(256), 0, s, p);
+            launch_kernel(prof, "mesh_setup", k_mesh_setup<false>,
+                          dim3(min(want, resident_blocks<k_mesh_setup<false>, 256>())), dim3(256), 0, s, p);
     }
     if (separate_clip) {
-        Profiler::Scope sc(prof, "clip", s);
-        if (p.alpha) hipLaunchKernelGGL(k_clip<true>, dim3(256), dim3(256), 0, s, p);
-        else hipLaunchKernelGGL(k_clip<false>, dim3(256), dim3(256), 0, s, p);
+        if (p.alpha) launch_kernel(prof, "clip", k_clip<true>, dim3(256), dim3(256), 0, s, p);
+        else launch_kernel(prof, "clip", k_clip<false>, dim3(256), dim3(256), 0, s, p);
     }
 }
 
@@ -1864,6 +2118,11 @@ void warm_launch_shapes() {
     (void)resident_blocks<k_gbuffer, 256>();
     (void)resident_blocks<k_bin_scatter<false>, 256>();
     (void)resident_blocks<k_bin_scatter<true>, 256>();
+}
+
+hipError_t launch_reset_counters(DrawScratch* scratch, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, s, scratch);
+    return hipGetLastError();
 }
 
 hipError_t launch_clear(unsigned long long* keys, uint32_t n, unsigned long long key, hipStream_t s) {
@@ -1889,10 +2148,9 @@ hipError_t launch_decode_overdraw(const unsigned long long* counts, const uint32
 hipError_t launch_draw_overdraw(const DrawParams& p, hipStream_t s, Profiler* prof) {
     launch_front(p, s, prof, true);
     {
-        Profiler::Scope sc(prof, "raster_overdraw", s);
         const uint32_t want = max((2 * p.total_meshlets + 3) / 4 + 16, 64u);
-        hipLaunchKernelGGL(k_raster_overdraw, dim3(min(want, resident_blocks<k_raster_overdraw, 256>())), dim3(256), 0,
-                           s, p);
+        launch_kernel(prof, "raster_overdraw", k_raster_overdraw,
+                      dim3(min(want, resident_blocks<k_raster_overdraw, 256>())), dim3(256), 0, s, p);
     }
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, s, p.scratch);
     return hipGetLastError();
@@ -1901,15 +2159,9 @@ hipError_t launch_draw_overdraw(const DrawParams& p, hipStream_t s, Profiler* pr
 hipError_t launch_draw_deferred(const DrawParams& p, hipStream_t s, Profiler* prof) {
     launch_front(p, s, prof, true);
     const uint32_t want = max((2 * p.total_meshlets + 3) / 4 + 16, 64u);
-    {
-        Profiler::Scope sc(prof, "raster_direct", s);
-        hipLaunchKernelGGL(k_raster_direct<true>, dim3(min(want, resident_blocks<k_raster_direct<true>, 256>())),
-                           dim3(256), 0, s, p);
-    }
-    {
-        Profiler::Scope sc(prof, "gbuffer", s);
-        hipLaunchKernelGGL(k_gbuffer, dim3(min(want, resident_blocks<k_gbuffer, 256>())), dim3(256), 0, s, p);
-    }
+    launch_kernel(prof, "raster_direct", k_raster_direct<true>,
+                  dim3(min(want, resident_blocks<k_raster_direct<true>, 256>())), dim3(256), 0, s, p);
+    launch_kernel(prof, "gbuffer", k_gbuffer, dim3(min(want, resident_blocks<k_gbuffer, 256>())), dim3(256), 0, s, p);
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, s, p.scratch);
     return hipGetLastError();
 }
@@ -1955,14 +2207,13 @@ hipError_t launch_hiz_build(const unsigned long long* keys, const Occlusion& occ
 hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s, Profiler* prof) {
     launch_front(p, s, prof, true);
     {
-        Profiler::Scope sc(prof, "raster_direct", s);
         const uint32_t want = max((2 * p.total_meshlets + 3) / 4 + 16, 64u);
         if (p.alpha)
-            hipLaunchKernelGGL(k_raster_direct<true>, dim3(min(want, resident_blocks<k_raster_direct<true>, 256>())),
-                               dim3(256), 0, s, p);
+            launch_kernel(prof, "raster_direct", k_raster_direct<true>,
+                          dim3(min(want, resident_blocks<k_raster_direct<true>, 256>())), dim3(256), 0, s, p);
         else
-            hipLaunchKernelGGL(k_raster_direct<false>, dim3(min(want, resident_blocks<k_raster_direct<false>, 256>())),
-                               dim3(256), 0, s, p);
+            launch_kernel(prof, "raster_direct", k_raster_direct<false>,
+                          dim3(min(want, resident_blocks<k_raster_direct<false>, 256>())), dim3(256), 0, s, p);
     }
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, s, p.scratch);
     return hipGetLastError();
@@ -1971,29 +2222,25 @@ hipError_t launch_draw_direct(const DrawParams& p, hipStream_t s, Profiler* prof
 hipError_t launch_draw_binned(const DrawParams& p, hipStream_t s, Profiler* prof) {
     launch_front(p, s, prof, false);   // the scatter kernel clips
     {
-        Profiler::Scope sc(prof, "bin_scatter", s);
         // about one chunk per wave: two per meshlet plus room for clipped records
         const uint32_t want = max((2 * p.total_meshlets + 3) / 4 + 16, 64u);
         if (p.alpha)
-            hipLaunchKernelGGL(k_bin_scatter<true>, dim3(min(want, resident_blocks<k_bin_scatter<true>, 256>())),
-                               dim3(256), 0, s, p);
+            launch_kernel(prof, "bin_scatter", k_bin_scatter<true>,
+                          dim3(min(want, resident_blocks<k_bin_scatter<true>, 256>())), dim3(256), 0, s, p);
         else
-            hipLaunchKernelGGL(k_bin_scatter<false>, dim3(min(want, resident_blocks<k_bin_scatter<false>, 256>())),
-                               dim3(256), 0, s, p);
+            launch_kernel(prof, "bin_scatter", k_bin_scatter<false>,
+                          dim3(min(want, resident_blocks<k_bin_scatter<false>, 256>())), dim3(256), 0, s, p);
     }
     {
-        Profiler::Scope sc(prof, "bin_order", s);
-        hipLaunchKernelGGL(k_bin_order, dim3((p.total_bins + 255) / 256), dim3(256), 0, s, p);
+        const dim3 grid((p.total_bins + 255) / 256 + kOverflowBlocks);
+        if (p.alpha) launch_kernel(prof, "bin_order", k_bin_order<true>, grid, dim3(256), 0, s, p);
+        else launch_kernel(prof, "bin_order", k_bin_order<false>, grid, dim3(256), 0, s, p);
     }
-    {
-        Profiler::Scope sc(prof, "bin_raster", s);
-        const dim3 grid(p.total_bins);
-        // records of alpha-tested materials need the variant with the alpha test
-        if (p.alpha) {
-            if (!launch_bin_raster<true>(p, grid, s)) return hipErrorInvalidValue;
-        } else {
-            if (!launch_bin_raster<false>(p, grid, s)) return hipErrorInvalidValue;
-        }
+    // records of alpha-tested materials need the variant with the alpha test
+    if (p.alpha) {
+        if (!launch_bin_raster<true>(p, s, prof)) return hipErrorInvalidValue;
+    } else {
+        if (!launch_bin_raster<false>(p, s, prof)) return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

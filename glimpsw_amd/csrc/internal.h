@@ -49,7 +49,7 @@ struct TriRecord {
     uint32_t pos[3];               // snapped 1/16 px, x | y << 16 (s16 each)
     float z0, z10, z20;            // depth plane (Rasterizer.cpp:321-323)
     uint32_t rank;                 // draw order, see key layout
-    uint32_t flags;                // bit 0 alpha-tested (kRecAlpha), bit 1 spilled (kRecSpilled), bits 16-31 view index
+    uint32_t flags;                // bit 0 alpha-tested (kRecAlpha), bit 1 on the overflow list (kRecSpilled), bits 16-31 view index
 };
 static_assert(sizeof(TriRecord) == 32, "record size");
 
@@ -73,8 +73,8 @@ static_assert(sizeof(AlphaRecord) == 64, "alpha record size");
 // TriRecord::flags bit 0: the record is alpha-tested.
 constexpr uint32_t kRecAlpha = 1u;
 // TriRecord::flags bit 1: a bin the record covers was full, so the record is
-// on the spill list, from which every bin it overlaps draws it; bin lists
-// that hold it skip it (each fragment drawn and counted once).
+// on the overflow list and drawn over its whole render box by the overflow
+// raster; the bin lists that hold it skip it (each fragment drawn once).
 constexpr uint32_t kRecSpilled = 2u;
 
 // A triangle that needs guard-band clipping, with its clip-space vertices.
@@ -118,7 +118,8 @@ struct Counters {                  // device counters, u64 each
     unsigned long long fragments_passed;
     unsigned long long setup_records;
     unsigned long long bin_entries;
-    unsigned long long dropped;    // clipped triangles or fan records lost to a full queue (always counted)
+    unsigned long long dropped;    // clipped triangles, fan records or overflow pairs lost to a full queue (always counted)
+    unsigned long long tile_lanes; // 16 per 4x4 tile the bin raster and the overflow raster evaluate
 };
 
 constexpr uint32_t kBinBuckets = 16;
@@ -132,8 +133,9 @@ struct DrawScratch {
     uint32_t num_clip;
     uint32_t num_visible;
     uint32_t pad0;
-    uint32_t num_spill;            // records that did not fit a bin
-    uint32_t pad[3];
+    uint32_t num_spill;            // records on the overflow list
+    uint32_t num_clip_overflow;    // meshlets with triangles that found the clip queue full
+    uint32_t pad[2];
     uint32_t bucket[kBinBuckets];  // bins per record-count class (log2), for heavy-first raster order
 };
 
@@ -171,6 +173,12 @@ struct ViewParams {
     int use_cull, clipping, fresh;
     float clear_depth;
     unsigned long long* keys;      // the view's vis-buffer, 4x4 tiles
+    // Same layout as `keys`, all zero between draws: records that overflowed
+    // a bin (and fan triangles that found no record slot) are drawn here with
+    // global atomic maxima, marking the bins they reach; the raster of a
+    // marked bin takes the maximum with its own keys and zeroes its pixels
+    // again.
+    unsigned long long* spill_keys;
     uint32_t* aux;                 // overdraw: the depth layer; deferred: colour layer, then normal/MR layer
     float o2w[9];                  // ShadingContext::ObjectToWorldMat (deferred normals), mat3 column-major
     float clip[16];
@@ -182,10 +190,12 @@ struct ViewParams {
 // (view v owns slots [meshlet_base, meshlet_base + meshlet_count)), and so
 // are the screen bins (view v owns [bin_base, bin_base + bins_x*bins_y)).
 // Bins are bin_size x bin_size pixel squares (a multiple of the 4x4 tile);
-// each bin holds up to bin_capacity record indices. A record that finds a
-// bin full is appended to the spill list and rasterized with global atomics
-// after the bins; the depth-test max is idempotent, so fragments rasterized
-// twice leave the same result.
+// each bin holds up to bin_capacity record indices. A record with a pair
+// that finds its bin full is appended to the overflow list instead (once)
+// and flagged; the overflow raster draws it over its render box into the
+// view's spill_keys, marking the bins it reaches, whose rasters skip it and
+// merge the spill keys. Each fragment is drawn once, and the depth-test
+// maximum does not depend on the order.
 struct DrawParams {
     uint32_t num_views;
     const ViewParams* views;       // device array, or null: the inline view
@@ -210,7 +220,18 @@ struct DrawParams {
     uint32_t* bin_counts;          // [total_bins]
     uint32_t* bin_lists;           // [total_bins][bin_capacity]
     uint32_t* bin_order;           // [kBinBuckets][total_bins]: bins by record-count class
-    uint32_t* spill;               // [clip_record_base + clip_record_capacity]
+    uint32_t* spill;               // overflow list: record indices, one per record slot at most
+    uint32_t spill_capacity;
+    // [total_bins], zero between draws: set for a bin when a record of the
+    // overflow list or a fan triangle of a triangle that found the clip queue
+    // full was drawn into its spill keys
+    uint32_t* bin_flags;
+    // Triangles that find the clip queue full are kept as references in free
+    // slots of their meshlet's record block and listed here as (visible slot,
+    // count), [total_meshlets]; they are clipped and drawn after the binning
+    // (the vis-buffer pipeline; the others count them in `dropped`).
+    uint2* clip_overflow;
+    uint32_t exact_overflow;
 };
 
 // Launchers (glimp_kernels.hip). `max_meshlets` bounds the meshlets of
@@ -234,6 +255,8 @@ hipError_t launch_draw_deferred(const DrawParams& p, hipStream_t s, Profiler* pr
 // 4x4-tiled 32-bit layer -> host order (row-major when linear).
 hipError_t launch_detile32(const uint32_t* src, int width, uint32_t n, uint32_t* out, int linear, hipStream_t s);
 hipError_t launch_clear(unsigned long long* keys, uint32_t n, unsigned long long key, hipStream_t s);
+// Zeroes a draw's scratch counters (one thread).
+hipError_t launch_reset_counters(DrawScratch* scratch, hipStream_t s);
 hipError_t launch_fill32(uint32_t* buf, uint32_t n, uint32_t value, hipStream_t s);
 hipError_t launch_decode(const unsigned long long* keys, int width, int height, unsigned long long clear_key,
                          uint32_t clear_color, const DrawEntry* draws, uint32_t num_draws, uint32_t* ids,

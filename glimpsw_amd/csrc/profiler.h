@@ -1,11 +1,16 @@
 // Per-kernel timing with HIP events on the launch stream.
 //
-// When enabled, every kernel launch of a draw or resolve is bracketed by two
-// events recorded on the stream it runs on; read() waits for them and sums
-// the elapsed time per kernel name. This is the device counterpart of the
-// reference's DrawTime/ResolveTime perf counters (Rasterizer.h:366-379).
+// When enabled, every kernel of a draw or resolve is launched with
+// hipExtLaunchKernel and a start and a stop event that the runtime stamps at
+// the dispatch's own start and end, so a kernel's time is its execution (what
+// rocprofv3 --kernel-trace reports), not the gap between markers recorded
+// around it; collect() waits for them and sums the elapsed time per kernel
+// name. Launches that are not single kernels use a Scope of two events on the
+// stream. This is the device counterpart of the reference's
+// DrawTime/ResolveTime perf counters (Rasterizer.h:366-379).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <map>
@@ -55,6 +60,17 @@ public:
 
     void end(size_t slot, hipStream_t s) { (void)hipEventRecord(pending_[slot].b, s); }
 
+    // Events for one hipExtLaunchKernel dispatch of kernel `name`.
+    void dispatch_events(const char* name, hipEvent_t& a, hipEvent_t& b) {
+        Pending p;
+        p.name = name;
+        p.a = acquire();
+        p.b = acquire();
+        pending_.push_back(p);
+        a = p.a;
+        b = p.b;
+    }
+
     // Waits for all recorded events and folds them into the statistics.
     hipError_t collect() {
         for (auto& p : pending_) {
@@ -96,5 +112,19 @@ private:
         return e;
     }
 };
+
+// Launches `kernel` on `s`; timed per dispatch when `prof` is enabled (never
+// during stream capture: captured launches pass no profiler).
+template <typename F, typename... Args>
+inline void launch_kernel(Profiler* prof, const char* name, F kernel, dim3 grid, dim3 block, uint32_t shmem,
+                          hipStream_t s, Args... args) {
+    if (prof && prof->enabled) {
+        hipEvent_t a, b;
+        prof->dispatch_events(name, a, b);
+        hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, a, b, 0u, args...);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
+    }
+}
 
 }  // namespace glimp

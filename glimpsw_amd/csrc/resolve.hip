@@ -400,13 +400,12 @@ __global__ void __launch_bounds__(256) k_resolve_overdraw(const unsigned long lo
 hipError_t launch_resolve_overdraw(const unsigned long long* counts, const uint32_t* depth, int width, int height,
                                    uint32_t clear_color, int quad, uint32_t* out, hipStream_t s, Profiler* prof) {
     const uint32_t n = (uint32_t)width * (uint32_t)height;
-    Profiler::Scope sc(prof, quad ? "resolve_overdraw_quad" : "resolve_overdraw_pixel", s);
     if (quad)
-        hipLaunchKernelGGL(k_resolve_overdraw<true>, dim3((n + 255) / 256), dim3(256), 0, s, counts, depth, width,
-                           height, clear_color, out);
+        launch_kernel(prof, "resolve_overdraw_quad", k_resolve_overdraw<true>, dim3((n + 255) / 256), dim3(256), 0, s,
+                      counts, depth, width, height, clear_color, out);
     else
-        hipLaunchKernelGGL(k_resolve_overdraw<false>, dim3((n + 255) / 256), dim3(256), 0, s, counts, depth, width,
-                           height, clear_color, out);
+        launch_kernel(prof, "resolve_overdraw_pixel", k_resolve_overdraw<false>, dim3((n + 255) / 256), dim3(256), 0,
+                      s, counts, depth, width, height, clear_color, out);
     return hipGetLastError();
 }
 
@@ -415,15 +414,15 @@ hipError_t launch_resolve(const ResolveParams& p, hipStream_t s, Profiler* prof)
     static const char* const names[6] = { "resolve_albedo", "resolve_lit", "resolve_normals",
                                           "resolve_metallic_roughness", "resolve_meshlet_id", "resolve_triangle_id" };
     if (p.mode < 0 || p.mode > 5) return hipErrorInvalidValue;
-    Profiler::Scope sc(prof, names[p.mode], s);
     const dim3 grid((n + 255) / 256), block(256);
+    const char* name = names[p.mode];
     switch (p.mode) {
-    case 0: hipLaunchKernelGGL(k_resolve<0>, grid, block, 0, s, p); break;
-    case 1: hipLaunchKernelGGL(k_resolve<1>, grid, block, 0, s, p); break;
-    case 2: hipLaunchKernelGGL(k_resolve<2>, grid, block, 0, s, p); break;
-    case 3: hipLaunchKernelGGL(k_resolve<3>, grid, block, 0, s, p); break;
-    case 4: hipLaunchKernelGGL(k_resolve<4>, grid, block, 0, s, p); break;
-    default: hipLaunchKernelGGL(k_resolve<5>, grid, block, 0, s, p); break;
+    case 0: launch_kernel(prof, name, k_resolve<0>, grid, block, 0, s, p); break;
+    case 1: launch_kernel(prof, name, k_resolve<1>, grid, block, 0, s, p); break;
+    case 2: launch_kernel(prof, name, k_resolve<2>, grid, block, 0, s, p); break;
+    case 3: launch_kernel(prof, name, k_resolve<3>, grid, block, 0, s, p); break;
+    case 4: launch_kernel(prof, name, k_resolve<4>, grid, block, 0, s, p); break;
+    default: launch_kernel(prof, name, k_resolve<5>, grid, block, 0, s, p); break;
     }
     return hipGetLastError();
 }

@@ -255,12 +255,20 @@ def checker_scene(num_meshlets: int = 512, width: int = 640, height: int = 480) 
 
 
 def frame_1080p(num_meshlets: int = 10240, width: int = 1920, height: int = 1080, tex_size: int = 2048,
-            num_materials: int = 16) -> tuple[Scene, View]:
+            num_materials: int = 16, crossers_inside: bool = False) -> tuple[Scene, View]:
     """Single 1080p view of ~1M triangles: per-patch triangle area
     log-uniform in [0.5, 500] px, patches spread over a widened frustum (so
     culling and outcode rejection see work), random orientation (about half
     back-facing) and 5% of patches straddling the camera plane to exercise
-    near-plane clipping; 16 seeded noise materials with mips."""
+    near-plane clipping; 16 seeded noise materials with mips.
+
+    The straddling patches sit beside the eye, their centres 0.85-1.05 of
+    their size off the view axis, so most of their triangles are rejected or
+    accepted without clipping and the depth complexity stays about 3. With
+    `crossers_inside` they sit 0.5-0.7 of their size off the axis, reaching
+    into the frustum: about ten times the clipped triangles, each covering a
+    large part of the screen (depth complexity about 18); the rest of the
+    scene is the same."""
     rng = SplitMix64(SEED_BASE + 1)
     n = num_meshlets
     aspect = width / height
@@ -277,7 +285,8 @@ def frame_1080p(num_meshlets: int = 10240, width: int = 1920, height: int = 1080
     k = int(near_sel.sum())
     s_near = rng.uniform(k, 0.5, 2.0)
     side = np.where(rng.uniform(k) < 0.5, -1.0, 1.0)
-    cx = side * rng.uniform(k, 0.85, 1.05) * s_near
+    off = (0.5, 0.7) if crossers_inside else (0.85, 1.05)
+    cx = side * rng.uniform(k, off[0], off[1]) * s_near
     cy = rng.uniform(k, -0.4, 0.4) * s_near
     cz = rng.uniform(k, -0.3, 0.3) * s_near - NEAR_Z
     centers[near_sel] = np.stack([cx, cy, cz], axis=1)

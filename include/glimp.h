@@ -177,6 +177,7 @@ typedef struct glimp_draw_desc {
 int glimp_draw_meshlets(glimp_context* ctx, glimp_framebuffer* fb, glimp_scene* scene,
                         const glimp_draw_desc* desc, void* stream);
 
+
 /* ShadingContext::CullMeshlets frustum branch (Shading.cpp:775-869): writes
  * one bit per meshlet, 16 per word (host `bitmap`), returns the visible
  * count in *visible. Synchronous. */
@@ -297,8 +298,12 @@ int glimp_batch_launch(glimp_batch* batch, int use_graph, void* stream);
  * the batched launch). */
 int glimp_batch_launch_per_view(glimp_batch* batch, void* stream);
 /* Autotuner: times the candidate shapes (bin 8-64, capacity 256-4096,
- * 128/256-thread bin raster) on this batch, keeps the fastest and returns
- * it with its time per launch. Synchronous. */
+ * 128/256-thread bin raster, 512/1024 for 32/64-pixel bins) on this batch,
+ * keeps the fastest and returns it with its time per launch. Each candidate
+ * is probed with one launch first: shapes whose bins overflow or that drop
+ * anything are rejected, and one slower than twice the best so far is not
+ * timed further. If every shape is rejected the batch's bin size is kept with
+ * the largest capacity. Synchronous. */
 int glimp_batch_autotune(glimp_batch* batch, void* stream, glimp_batch_shape* chosen, float* best_ms);
 
 /* ------------------------------------------------------------ differentiable resolve */
@@ -425,8 +430,11 @@ typedef struct glimp_counters {
     uint64_t fragments_passed;
     uint64_t setup_records;            /* triangles that left setup */
     uint64_t bin_entries;              /* (triangle, bin) pairs */
-    uint64_t dropped;                  /* queued clip triangles or clipped records lost to a full queue
-                                        * (counted always; non-zero means the frame is incomplete) */
+    uint64_t dropped;                  /* queued clip triangles, clipped records or bin-overflow pairs lost
+                                        * to a full queue (counted always; non-zero means the frame is
+                                        * incomplete) */
+    uint64_t tile_lanes;               /* 16 per 4x4 tile the bin raster evaluates (lane utilisation:
+                                        * fragments_covered / tile_lanes) */
 } glimp_counters;
 int glimp_get_counters(glimp_context* ctx, glimp_counters* out); /* synchronous */
 int glimp_reset_counters(glimp_context* ctx);

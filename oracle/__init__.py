@@ -211,14 +211,18 @@ class OracleFramebuffer:
 
 def draw(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip, count: int, meshlet_offset: int = 0,
          cull_bitmap=None, clipping: bool = True, guardband: bool = True, coverage: bool = False,
-         pipeline: int = PIPELINE_VISBUFFER, object_to_world=None):
+         pipeline: int = PIPELINE_VISBUFFER, object_to_world=None, rows=None):
+    """DrawMeshlets into `fb`; with `rows` = (row0, row1) (multiples of 4)
+    only fragments in those rows are drawn and counted."""
     cov = np.zeros((fb.height, fb.width), dtype=np.uint32) if coverage else None
     opts = RasterOpts(int(clipping), int(guardband), pipeline, 0, _ptr(cov, u32p) if cov is not None else u32p())
     ctr = Counters()
     if pipeline == PIPELINE_DEFERRED:
         assert fb.fb.num_layers >= 3, "the deferred pipeline writes three layers"
     ctx = binding.ctx(object_to_clip, meshlet_offset, cull_bitmap, object_to_world)
-    lib().orc_draw_meshlets(C.byref(fb.fb), C.c_uint32(count), C.byref(ctx), C.byref(opts), C.byref(ctr))
+    r0, r1 = rows if rows is not None else (0, fb.height)
+    lib().orc_draw_meshlets_rows(C.byref(fb.fb), C.c_uint32(count), C.byref(ctx), C.byref(opts), C.byref(ctr),
+                                 C.c_uint32(r0), C.c_uint32(r1))
     return ctr.as_dict(), cov
 
 
@@ -310,8 +314,11 @@ def render_frame(scene, view, binding=None, cull_first: bool = True, clipping: b
     return fb, ctr, binding
 
 
-def resolve_lit(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip, lighting, meshlet_offset: int = 0):
-    """Resolve (lit, tonemapped) into layer 0; returns row-major RGBA8."""
+def resolve_lit(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip, lighting, meshlet_offset: int = 0,
+                rows=None):
+    """Resolve (lit, tonemapped) into layer 0; returns row-major RGBA8. With
+    `rows` = (row0, row1) only those rows are resolved and nothing is read
+    back (returns None)."""
     from glimpsw_amd import light_structs
 
     ctx = binding.ctx(object_to_clip, meshlet_offset)
@@ -335,6 +342,10 @@ def resolve_lit(fb: OracleFramebuffer, binding: SceneBinding, object_to_clip, li
     if sky is not None:
         sky_struct = texture_struct(sky)
         rp.skybox = C.pointer(sky_struct)
+    if rows is not None:
+        lib().orc_resolve_lit_rows(C.byref(fb.fb), C.byref(ctx), C.byref(rp), C.c_uint32(rows[0]),
+                                   C.c_uint32(rows[1]))
+        return None
     lib().orc_resolve_lit(C.byref(fb.fb), C.byref(ctx), C.byref(rp))
     return fb.pixels(0)
 
@@ -416,3 +427,26 @@ def bary_f64(clip, su: float, sv: float) -> np.ndarray:
     out = np.zeros(3, dtype=np.float64)
     lib().orc_bary_f64(_ptr(c, f32p), C.c_double(su), C.c_double(sv), _ptr(out, f64p))
     return out
+
+
+def row_bands(height: int, parts: int) -> list:
+    """`parts` row ranges of about equal height covering [0, height), cut on
+    4-row tile boundaries (empty ranges dropped)."""
+    tiles = height // 4
+    cuts = [4 * ((tiles * k) // parts) for k in range(parts + 1)]
+    return [(a, b) for a, b in zip(cuts, cuts[1:]) if b > a]
+
+
+def render_frame_threads(scene, view, binding, lighting, pool, bands):
+    """The frame of render_frame + resolve_lit split over host threads by row
+    bands: the cull once, then every band's draw and every band's lit resolve
+    on `pool` (a concurrent.futures executor). Bit-identical to the one-thread
+    frame; returns the framebuffer."""
+    fb = OracleFramebuffer(view.width, view.height)
+    fb.clear(0, 0.0)
+    bitmap, _ = cull(binding, view.frustum_planes(), view.meshlet_offset, view.meshlet_count)
+    list(pool.map(lambda r: draw(fb, binding, view.object_to_clip, view.meshlet_count, view.meshlet_offset, bitmap,
+                                 rows=r), bands))
+    list(pool.map(lambda r: resolve_lit(fb, binding, view.object_to_clip, lighting, view.meshlet_offset, rows=r),
+                  bands))
+    return fb

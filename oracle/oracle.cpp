@@ -327,6 +327,7 @@ struct DrawState {
     orc_framebuffer* fb;
     const orc_raster_opts* opts;
     orc_counters* ctr;
+    int row0, row1;                // rows drawn: [row0, row1), multiples of 4 (the whole frame by default)
 };
 
 void sample_level16(const orc_texture* tex, const float* u, const float* v, int32_t layer, const int32_t* mip,
@@ -455,7 +456,8 @@ void draw_triangle(DrawState& st, const EdgeVars& e, int l, uint32_t bmin, uint3
     uint32_t W = fb->width;
     uint32_t* ids = fb->data;
     float* depthBuf = (float*)(fb->data + fb->layer_stride);
-    int minX = lo16(bmin), minY = hi16(bmin), maxX = lo16(bmax), maxY = hi16(bmax);
+    int minX = lo16(bmin), minY = std::max((int)hi16(bmin), st.row0), maxX = lo16(bmax),
+        maxY = std::min((int)hi16(bmax), st.row1);
     for (int ty = minY; ty < maxY; ty += 4) {
         for (int tx = minX; tx < maxX; tx += 4) {
             int32_t E1[16], E2[16];
@@ -1305,13 +1307,14 @@ uint32_t alpha_blend_u8(uint32_t bg, uint32_t fg) {
 // for each point or spot light in front of the camera, every 4x4 tile of the
 // sprite's range with a pixel inside the disc and behind the light blends
 // the light colour over the pixels behind the light.
-void draw_light_sprites(uint32_t* color, const float* depth, uint32_t W, uint32_t H, const orc_resolve_params* p) {
+void draw_light_sprites(uint32_t* color, const float* depth, uint32_t W, uint32_t H, const orc_resolve_params* p,
+                        int32_t row0, int32_t row1) {
     for (uint32_t li = 0; li < p->num_lights; li++) {
         const orc_light& L = p->lights[li];
         if (L.type == 0) continue;
         const Sprite s = sprite_of(p->world_to_clip, L.position, W, H);
         if (!s.on) continue;
-        for (int32_t y = s.y0; y < s.y1; y += 4) {
+        for (int32_t y = std::max(s.y0, row0); y < std::min(s.y1, row1); y += 4) {
             for (int32_t x = s.x0; x < s.x1; x += 4) {
                 const uint32_t off = pixel_offset(W, (uint32_t)x, (uint32_t)y);
                 float d[16];
@@ -1362,9 +1365,14 @@ void orc_get_pixels(const orc_framebuffer* fb, uint32_t layer, uint32_t* dest, u
 
 void orc_draw_meshlets(orc_framebuffer* fb, uint32_t count, const orc_shading_ctx* ctx,
                        const orc_raster_opts* opts, orc_counters* counters) {
+    orc_draw_meshlets_rows(fb, count, ctx, opts, counters, 0, fb->height);
+}
+
+void orc_draw_meshlets_rows(orc_framebuffer* fb, uint32_t count, const orc_shading_ctx* ctx,
+                            const orc_raster_opts* opts, orc_counters* counters, uint32_t row0, uint32_t row1) {
     orc_counters localCtr = {};
     orc_counters* ctr = counters ? counters : &localCtr;
-    DrawState st = { fb, opts, ctr };
+    DrawState st = { fb, opts, ctr, (int)row0, (int)row1 };
     int hw = (int)fb->width / 2, hh = (int)fb->height / 2;
     bool guard = !opts || opts->enable_guardband;
     bool clipping = !opts || opts->enable_clipping;
@@ -1850,6 +1858,11 @@ void orc_resolve_albedo(orc_framebuffer* fb, const orc_shading_ctx* ctx) {
 // EvalLighting of their surface, uncovered pixels are black; then the light
 // sprites when asked for.
 void orc_resolve_lit(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_resolve_params* p) {
+    orc_resolve_lit_rows(fb, ctx, p, 0, fb->height);
+}
+
+void orc_resolve_lit_rows(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_resolve_params* p, uint32_t row0,
+                          uint32_t row1) {
     uint32_t W = fb->width, H = fb->height;
     float sw = (float)W, sh = (float)H;
     float scaleX = 2.0f / sw, scaleY = 2.0f / sh;
@@ -1858,7 +1871,7 @@ void orc_resolve_lit(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_
     std::vector<uint32_t> out((size_t)W * H);
     const float exposure = p->exposure;
     const float lightExposure = p->exposure * 0.001f;
-    for (uint32_t y0 = 0; y0 < H; y0 += 4) {
+    for (uint32_t y0 = row0; y0 < row1; y0 += 4) {
         for (uint32_t x0 = 0; x0 < W; x0 += 4) {
             uint32_t off = pixel_offset(W, x0, y0);
             bool active[16];
@@ -1901,8 +1914,9 @@ void orc_resolve_lit(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_
             }
         }
     }
-    if (p->light_sprites) draw_light_sprites(out.data(), depth, W, H, p);
-    memcpy(fb->data, out.data(), (size_t)W * H * 4);
+    if (p->light_sprites) draw_light_sprites(out.data(), depth, W, fb->height, p, (int32_t)row0, (int32_t)row1);
+    // rows [row0, row1) of 4x4 tiles are one contiguous range of the layer
+    memcpy(fb->data + (size_t)row0 * W, out.data() + (size_t)row0 * W, (size_t)(row1 - row0) * W * 4);
 }
 
 void orc_eval_lighting16(const uint32_t* albedo, const uint32_t* nmr, const float* normal, const float* wpos,

@@ -121,6 +121,12 @@ void orc_fb_clear(orc_framebuffer* fb, uint32_t color, float depth);
 void orc_get_pixels(const orc_framebuffer* fb, uint32_t layer, uint32_t* dest, uint32_t stride);
 void orc_draw_meshlets(orc_framebuffer* fb, uint32_t count, const orc_shading_ctx* ctx,
                        const orc_raster_opts* opts, orc_counters* counters);
+/* The same draw restricted to the framebuffer rows [row0, row1) (multiples
+ * of 4): every meshlet is shaded, set up and clipped, and only fragments in
+ * those rows are drawn and counted. Calls on disjoint row ranges may run on
+ * separate threads and together give the whole frame bit for bit. */
+void orc_draw_meshlets_rows(orc_framebuffer* fb, uint32_t count, const orc_shading_ctx* ctx,
+                            const orc_raster_opts* opts, orc_counters* counters, uint32_t row0, uint32_t row1);
 
 /* Frustum cull of bound spheres against 5 normalised planes (xyz,w each). */
 /* Occlusion inputs of CullMeshlets (Shading.cpp:775-869): the previous
@@ -171,6 +177,10 @@ void orc_gbuffer_pack16(const float* n, const uint32_t* nmr, uint32_t* out);
 void orc_overdraw_colour16(const uint32_t* data, int32_t quad, float* out);
 /* Resolve(): lit, tonemapped RGBA8 into layer 0. */
 void orc_resolve_lit(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_resolve_params* p);
+/* The lit resolve of rows [row0, row1) only (multiples of 4), light sprites
+ * included; calls on disjoint row ranges may run on separate threads. */
+void orc_resolve_lit_rows(orc_framebuffer* fb, const orc_shading_ctx* ctx, const orc_resolve_params* p, uint32_t row0,
+                          uint32_t row1);
 /* The lit colour of one 4x4 tile at given surfaces (Shading.cpp:602-645,
  * 221-226, 680): albedo and layer-1 (normal/metallic/roughness) texels,
  * world normals [3][16] and world positions [3][16] of the 16 lanes, lanes

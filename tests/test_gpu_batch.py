@@ -127,8 +127,10 @@ def test_autotune_keeps_results(gpu_ctx, small):
     assert 256 <= shape["bin_capacity"] <= 4096
     assert shape["raster_threads"] in (128, 256, 512, 1024)
     assert ms > 0.0
+    gpu_ctx.reset_counters()
     b.launch(graph=True)
     _same(want, st.read_all(), "autotuned")
+    assert gpu_ctx.counters()["dropped"] == 0
     b.close()
 
 

@@ -173,8 +173,9 @@ def test_empty_draw_and_clear_values(gpu_ctx):
 
 @pytest.mark.parametrize("bin_capacity", [0, 4])
 def test_counters_match_oracle(gpu_ctx, bin_capacity):
-    """Capacity 4 overflows most bins: the spilled triangles' fragments are
-    then counted by the spill raster alone, still once each."""
+    """Capacity 4 overflows most bins: the (triangle, bin) pairs past a bin's
+    capacity are drawn by the overflow raster instead of the bin raster, and
+    each fragment is still counted once."""
     import glimpsw_amd as g
 
     sc = T.small_scene(384, seed_offset=2, spread=6.0, near_frac=0.25)
@@ -188,23 +189,35 @@ def test_counters_match_oracle(gpu_ctx, bin_capacity):
     assert c["dropped"] == 0
 
 
-def test_full_clip_queue_is_counted(gpu_ctx):
+@pytest.mark.parametrize("direct", [False, True])
+def test_full_clip_queue_is_exact(gpu_ctx, direct):
     """600 identical small patches through the eye, sheared so the near
     plane cuts a diagonal of each (7,200 clipped triangles in the oracle),
     queue more clipped triangles than the clip queue holds (4096 or two per
-    meshlet): the draw completes and the loss shows in the `dropped` counter;
-    the next, ordinary draw (whose scratch counters the full one must leave
-    cleared) is exact again."""
+    meshlet) and produce more fan triangles than the record region holds:
+    the triangles without queue room are clipped by their setup wave, the fan
+    triangles without a record slot are drawn at once, and the frame equals
+    the oracle's bit for bit with nothing dropped, on the binned and the
+    direct route. The next, ordinary draw (whose scratch counters the full
+    one must leave cleared) is exact too."""
+    import glimpsw_amd as g
+
     n = 600
     R = np.tile(np.array([[0.7, 0.0, 0.0], [0.0, 0.7, 0.0], [0.7, 0.7, 1.0]]), (n, 1, 1))
     sc = S.build_patches(np.tile([[0.0, 0.0, -S.NEAR_Z]], (n, 1)), R, np.full(n, 0.05), np.zeros(n, np.uint32),
                          [(0, True, 255)], [textures.checker_texture(64)])
     view = S.make_view(320, 240, count=n)
+    o = _oracle_frame(sc, view, cull=False)
+    assert o[2]["triangles_clipped"] > 4096
     gpu_ctx.reset_counters()
-    _gpu_frame(gpu_ctx, sc, view, cull=False)
+    flags = g.GLIMP_DRAW_COUNTERS | (g.GLIMP_DRAW_DIRECT if direct else 0)
+    got = _gpu_frame(gpu_ctx, sc, view, cull=False, flags=flags)
     c = gpu_ctx.counters()
-    print(f"[raster] full clip queue: dropped {c['dropped']}", flush=True)
-    assert c["dropped"] > 0
+    print(f"[raster] full clip queue: clipped {c['triangles_clipped']}, dropped {c['dropped']}", flush=True)
+    _assert_same(*got, o[0], o[1], "full clip queue")
+    assert c["dropped"] == 0
+    for k in ("triangles_processed", "triangles_rasterized", "triangles_clipped", "fragments_covered"):
+        assert c[k] == o[2][k], (k, c[k], o[2][k])
     sc2, view2 = S.checker_scene()
     _assert_same(*_gpu_frame(gpu_ctx, sc2, view2), *_oracle_frame(sc2, view2)[:2], "after a full clip queue")
 
@@ -215,6 +228,20 @@ def test_frame_1080p(gpu_ctx):
     o = _oracle_frame(sc, view)
     g = _gpu_frame(gpu_ctx, sc, view)
     _assert_same(*g, o[0], o[1], "frame_1080p")
+
+
+@pytest.mark.slow
+def test_frame_1080p_crossers_inside(gpu_ctx):
+    """The 1080p frame with its near-plane crossers reaching into the frustum
+    (hundreds of clipped triangles covering much of the screen), binned at
+    32-pixel bins and on the direct route, bit-exact with the oracle."""
+    import glimpsw_amd as g
+
+    sc, view = S.frame_1080p(crossers_inside=True)
+    o = _oracle_frame(sc, view)
+    assert o[2]["triangles_clipped"] > 300
+    _assert_same(*_gpu_frame(gpu_ctx, sc, view, bin_size=32), o[0], o[1], "crossers inside, binned")
+    _assert_same(*_gpu_frame(gpu_ctx, sc, view, flags=g.GLIMP_DRAW_DIRECT), o[0], o[1], "crossers inside, direct")
 
 
 @pytest.mark.parametrize("bin_size", [8, 16, 32, 64])
@@ -228,12 +255,39 @@ def test_bin_sizes(gpu_ctx, bin_size):
 
 @pytest.mark.parametrize("cap", [1, 3, 16])
 def test_bin_spill_bit_identical(gpu_ctx, cap):
-    """Bins far too small: most records spill to the global-atomic pass and
-    the vis-buffer must not change."""
+    """Bins far too small: most (triangle, bin) pairs overflow to the
+    overflow raster, whose spill keys the bins merge, and the vis-buffer must
+    not change."""
     sc, view = S.checker_scene()
     o = _oracle_frame(sc, view)
     g = _gpu_frame(gpu_ctx, sc, view, bin_size=16, bin_capacity=cap)
     _assert_same(*g, o[0], o[1], f"capacity {cap}")
+
+
+def test_overflow_leaves_spill_keys_cleared(gpu_ctx):
+    """An overflowing draw must leave its framebuffer's spill keys zero: a
+    second scene drawn into the same framebuffer right after (its bins also
+    overflow, so they merge the spill keys) must equal the oracle's frame of
+    that scene alone, at every bin size."""
+    import glimpsw_amd as g
+
+    sc1, view = S.checker_scene()
+    sc2 = T.small_scene(300, seed_offset=5, spread=6.0, near_frac=0.2)
+    view2 = S.make_view(view.width, view.height, count=sc2.count)
+    o = _oracle_frame(sc2, view2)
+    d1, d2 = g.Scene(gpu_ctx, sc1), g.Scene(gpu_ctx, sc2)
+    fb = g.Framebuffer(gpu_ctx, view.width, view.height)
+    for bs in (8, 16, 32, 64):
+        fb.clear(0, 0.0)
+        fb.draw(d1, view.object_to_clip, sc1.count, 0, view.frustum_planes(), 0, bin_size=bs, bin_capacity=2)
+        fb.read()
+        fb.clear(0, 0.0)
+        fb.draw(d2, view2.object_to_clip, sc2.count, 0, view2.frustum_planes(), 0, bin_size=bs, bin_capacity=2)
+        ids, depth = fb.read()
+        _assert_same(ids, depth, o[0], o[1], f"second overflowing draw, bin {bs}")
+    fb.close()
+    d1.close()
+    d2.close()
 
 
 def test_direct_path(gpu_ctx):
