@@ -170,6 +170,7 @@ _SIGNATURES = {
 "glimp_framebuffer_read_layer": (C.c_int, [C.c_void_p, C.c_uint32, u32p, C.c_int, C.c_void_p]),
     "glimp_framebuffer_keys": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]),
     "glimp_draw_meshlets": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(DrawDesc), C.c_void_p]),
+    "glimp_draw_meshlets_static": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(DrawDesc), C.c_void_p]),
     "glimp_cull_meshlets": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, f32p, u16p,
                                       C.POINTER(C.c_uint32), C.c_void_p]),
     "glimp_cull_meshlets_occlusion": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, f32p,

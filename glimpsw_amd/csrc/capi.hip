@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -115,6 +116,55 @@ struct Scratch {
 
 }  // namespace
 
+// ---------------------------------------------------------------- static scratch
+//
+// The scratch of glimp_draw_meshlets_static: every buffer the direct-route
+// draw needs beyond its inputs and outputs (setup records and their alpha
+// side records, the clip queue and its fan records, the visible list, the
+// per-meshlet counts, the clip-overflow list and the draw counters), for up to
+// kStaticMaxMeshlets meshlets per call. Each device holds its own copy for as
+// long as the process runs. Host code never uses the array's name as an
+// address: glimp_create takes this device's address with hipGetSymbolAddress
+// and every use goes through that pointer.
+constexpr size_t kStaticMaxMeshlets = 1024;
+constexpr size_t kStaticClip = 4096;                         // clip queue entries: clip_capacity_for(1024)
+constexpr size_t kStaticRecords = kStaticMaxMeshlets * 128 + kStaticClip * 8;
+constexpr size_t kStaticMaxDraws = 256;                      // draws per framebuffer between clears
+
+constexpr size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+constexpr size_t kStaticRecordsOff = 0;
+constexpr size_t kStaticAlphaOff = kStaticRecordsOff + align256(kStaticRecords * sizeof(TriRecord));
+constexpr size_t kStaticClipOff = kStaticAlphaOff + align256(kStaticRecords * sizeof(AlphaRecord));
+constexpr size_t kStaticVisibleOff = kStaticClipOff + align256(kStaticClip * sizeof(ClipEntry));
+constexpr size_t kStaticCountsOff = kStaticVisibleOff + align256(kStaticMaxMeshlets * sizeof(uint2));
+constexpr size_t kStaticOverflowOff = kStaticCountsOff + align256(kStaticMaxMeshlets * sizeof(uint32_t));
+constexpr size_t kStaticDrawScratchOff = kStaticOverflowOff + align256(kStaticMaxMeshlets * sizeof(uint2));
+constexpr size_t kStaticBytes = kStaticDrawScratchOff + align256(sizeof(DrawScratch));   // about 16 MB
+
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+namespace {
+
+// this process's address of `scratch` on each device, taken once per device
+std::mutex g_static_mutex;
+void* g_static_ptr[64] = {};
+
+// With `device` current: the device's address of `scratch`.
+hipError_t static_scratch_address(int device, void** out) {
+    std::lock_guard<std::mutex> lock(g_static_mutex);
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    if (!g_static_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_static_ptr[device] = ptr;
+    }
+    *out = g_static_ptr[device];
+    return hipSuccess;
+}
+
+}  // namespace
+
 struct glimp_context {
     int device = 0;
     uint32_t* tables = nullptr;
@@ -131,6 +181,8 @@ struct glimp_context {
     DiffView* d_diff_views = nullptr;
     size_t d_diff_views_cap = 0;
     Profiler prof;
+    // this device's copy of the static scratch array (glimp_draw_meshlets_static)
+    unsigned char* static_scratch = nullptr;
 
     ApproxTables approx() const { return ApproxTables{ reinterpret_cast<const uint2*>(tables) }; }
 };
@@ -214,6 +266,9 @@ int glimp_create(glimp_context** out, int device) {
     if (e == hipSuccess) e = dev_alloc(&c->scratch, 1);
     if (e == hipSuccess) e = hipMemset(c->scratch, 0, sizeof(DrawScratch));
     if (e == hipSuccess) e = dev_alloc(&c->visible, 1);
+    void* sp = nullptr;
+    if (e == hipSuccess) e = static_scratch_address(device, &sp);
+    c->static_scratch = static_cast<unsigned char*>(sp);
     if (e != hipSuccess) rc = fail(GLIMP_E_HIP, std::string("glimp_create: ") + hipGetErrorString(e));
     if (rc == GLIMP_OK) warm_launch_shapes();
     if (rc != GLIMP_OK) {
@@ -404,6 +459,9 @@ int glimp_framebuffer_create(glimp_context* c, uint32_t width, uint32_t height, 
     f->ctx = c;
     f->width = width;
     f->height = height;
+    // room for the draw table of glimp_draw_meshlets_static, which must not
+    // allocate (the table only grows past this for the other draw calls)
+    f->draws.reserve(kStaticMaxDraws);
     hipError_t e = dev_alloc(&f->keys, (size_t)width * height);
     if (e == hipSuccess) e = hipMemset(f->keys, 0, (size_t)width * height * 8);
     if (e == hipSuccess) e = dev_alloc(&f->spill_keys, (size_t)width * height);
@@ -845,6 +903,71 @@ int glimp_draw_meshlets(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, 
     f->pending_clear = false;
     f->content = (int)d->pipeline;
     f->draws.push_back(DrawEntry{ f->next_seq, d->meshlet_count, d->meshlet_offset, 0 });
+    f->next_seq += d->meshlet_count;
+    return GLIMP_OK;
+}
+
+int glimp_draw_meshlets_static(glimp_context* c, glimp_framebuffer* f, glimp_scene* s, const glimp_draw_desc* d,
+                               void* stream) {
+    if (!c || !f || !s || !d) return fail(GLIMP_E_INVALID, "glimp_draw_meshlets_static: NULL argument");
+    if (d->pipeline != GLIMP_PIPELINE_VISBUFFER)
+        return fail(GLIMP_E_UNSUPPORTED, "glimp_draw_meshlets_static: only the vis-buffer pipeline");
+    if (f->content >= 0 && f->content != (int)GLIMP_PIPELINE_VISBUFFER)
+        return fail(GLIMP_E_INVALID,
+                    "glimp_draw_meshlets_static: the framebuffer holds another pipeline's output; clear it first");
+    if ((uint64_t)d->meshlet_offset + d->meshlet_count > s->m.count)
+        return fail(GLIMP_E_INVALID, "glimp_draw_meshlets_static: meshlet range outside the scene");
+    if (d->meshlet_count > kStaticMaxMeshlets)
+        return fail(GLIMP_E_LIMIT, "glimp_draw_meshlets_static: more than 1024 meshlets in one call "
+                                   "(the size of the static scratch); split the range or use glimp_draw_meshlets");
+    if ((uint64_t)f->next_seq + d->meshlet_count > GLIMP_MAX_DRAWN_MESHLETS)
+        return fail(GLIMP_E_LIMIT,
+                    "glimp_draw_meshlets_static: more than GLIMP_MAX_DRAWN_MESHLETS meshlets since the last clear");
+    if (f->draws.size() >= f->draws.capacity())
+        return fail(GLIMP_E_LIMIT, "glimp_draw_meshlets_static: more than 256 draws since the last clear");
+    if (!c->static_scratch) return fail(GLIMP_E_HIP, "glimp_draw_meshlets_static: no static scratch on this device");
+    if (d->meshlet_count == 0) return GLIMP_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(flush_clear(f, st));
+    Occlusion occ;
+    if (d->occlusion) {
+        if (!d->frustum_planes) return fail(GLIMP_E_INVALID, "glimp_draw_meshlets_static: occlusion needs frustum planes");
+        int orc = make_occlusion(d->occlusion, "glimp_draw_meshlets_static", occ);
+        if (orc != GLIMP_OK) return orc;
+    }
+    unsigned char* base = c->static_scratch;
+    const size_t count = d->meshlet_count;
+    const size_t clipq = clip_capacity_for(count);
+    DrawParams p;
+    memset(&p, 0, sizeof(p));
+    make_view(f, s, d->meshlet_offset, d->meshlet_count, d->object_to_clip, d->frustum_planes, d->flags, p.view,
+              d->occlusion ? &occ : nullptr);
+    p.num_views = 1;
+    p.views = nullptr;
+    p.total_meshlets = (uint32_t)count;
+    p.total_bins = 0;                                  // the direct route
+    p.flags = d->flags | GLIMP_DRAW_DIRECT;
+    p.tables = c->approx();
+    p.records = reinterpret_cast<TriRecord*>(base + kStaticRecordsOff);
+    p.alpha = s->has_alpha ? reinterpret_cast<AlphaRecord*>(base + kStaticAlphaOff) : nullptr;
+    p.clip_queue = reinterpret_cast<ClipEntry*>(base + kStaticClipOff);
+    p.visible_list = reinterpret_cast<uint2*>(base + kStaticVisibleOff);
+    p.meshlet_counts = reinterpret_cast<uint32_t*>(base + kStaticCountsOff);
+    p.clip_overflow = reinterpret_cast<uint2*>(base + kStaticOverflowOff);
+    p.scratch = reinterpret_cast<DrawScratch*>(base + kStaticDrawScratchOff);
+    p.clip_record_base = (uint32_t)(count * 128);
+    p.clip_record_capacity = (uint32_t)(clipq * 8);
+    p.clip_capacity = (uint32_t)clipq;
+    p.counters = c->counters;
+    p.exact_overflow = 1u;
+    // the array keeps the last call's contents: the counters start at zero
+    // (a kernel, not a memset, so the call can be captured)
+    HIP_TRY(launch_reset_counters(p.scratch, st));
+    HIP_TRY(launch_draw_direct(p, st, nullptr));
+    f->pending_clear = false;
+    f->content = (int)GLIMP_PIPELINE_VISBUFFER;
+    f->draws.push_back(DrawEntry{ f->next_seq, d->meshlet_count, d->meshlet_offset, 0 });   // capacity reserved
     f->next_seq += d->meshlet_count;
     return GLIMP_OK;
 }

@@ -177,6 +177,19 @@ typedef struct glimp_draw_desc {
 int glimp_draw_meshlets(glimp_context* ctx, glimp_framebuffer* fb, glimp_scene* scene,
                         const glimp_draw_desc* desc, void* stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under hipGraph stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The same draw as glimp_draw_meshlets with the
+ * vis-buffer pipeline on the direct route (GLIMP_DRAW_DIRECT: bin_size and
+ * bin_capacity are ignored), with the same results, but every scratch buffer
+ * comes from a static device array of about 16 MB that each GPU holds for the
+ * life of the process: the call makes no allocation. Limits: at most 1024
+ * meshlets per call (GLIMP_E_LIMIT beyond) and 256 draws per framebuffer
+ * between clears. All calls on one GPU share that GPU's array, so calls that
+ * overlap on one GPU (other host threads, or other streams without ordering
+ * between them) race; order them on one stream. Asynchronous. */
+int glimp_draw_meshlets_static(glimp_context* ctx, glimp_framebuffer* fb, glimp_scene* scene,
+                               const glimp_draw_desc* desc, void* stream);
 
 /* ShadingContext::CullMeshlets frustum branch (Shading.cpp:775-869): writes
  * one bit per meshlet, 16 per word (host `bitmap`), returns the visible
