@@ -249,11 +249,11 @@ def cpu_baseline(scene, view, seconds: float) -> dict:
     }
 
 
-def bench_batch(ctx, stream, steps: int, warmup: int) -> dict:
+def bench_batch(ctx, stream, steps: int, warmup: int, shape: dict | None = None) -> dict:
     """Batching workload: 64 views over 8 scenes of 1k-50k meshlets, one
     batch launch per step (hipGraph replay or eager, the faster; shape picked
-    by the autotuner) against the same views drawn one clear+draw call at a
-    time."""
+    by the autotuner unless given) against the same views drawn one
+    clear+draw call at a time."""
     import torch
     import glimpsw_amd as g
     from glimpsw_amd import scene as S
@@ -261,9 +261,11 @@ def bench_batch(ctx, stream, steps: int, warmup: int) -> dict:
     scenes, views = S.batch_views()
     dscenes = [g.Scene(ctx, sc) for sc in scenes]
     fbs = [g.Framebuffer(ctx, v.width, v.height) for _, v in views]
-    b = g.Batch(ctx, [(dscenes[si], fb, v) for (si, v), fb in zip(views, fbs)])
+    b = g.Batch(ctx, [(dscenes[si], fb, v) for (si, v), fb in zip(views, fbs)], shape=shape)
     t_tune = time.perf_counter()
-    shape, tune_ms = b.autotune(stream)
+    tune_ms = None
+    if shape is None:
+        shape, tune_ms = b.autotune(stream)
     t_tune = time.perf_counter() - t_tune
 
     def timed(fn):
@@ -313,7 +315,7 @@ def bench_batch(ctx, stream, steps: int, warmup: int) -> dict:
         "submitted_tris": tris,
         "Mtri_per_s": round(tris / (ms_batch * 1e-3) / 1e6, 1),
         "autotuned_shape": shape,
-        "autotune_ms": round(tune_ms, 4),
+        "autotune_ms": round(tune_ms, 4) if tune_ms is not None else None,
         "autotune_wall_s": round(t_tune, 2),
         "identical_vis_buffers": bool(identical and same_eager),
     }
@@ -404,6 +406,8 @@ def main() -> int:
     ap.add_argument("--no-counts", action="store_true", help="skip the counting pass (frame kernels only)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the 64-view batching measurement (its JSON object)")
+    ap.add_argument("--batch-shape", default="", metavar="BIN,CAPACITY,THREADS",
+                    help="launch shape of the batching measurement instead of the autotuner's")
     for kind in ("copy", "read", "gather"):
         ap.add_argument(f"--calibrate-{kind}", type=int, default=0, metavar="BYTES",
                         help=f"only run the {kind} calibration kernel over BYTES bytes 10 times")
@@ -424,7 +428,11 @@ def main() -> int:
 
         torch.cuda.init()
         ctx = g.Context(0)
-        print(json.dumps(bench_batch(ctx, torch.cuda.current_stream(), max(5, args.steps // 5), 3)))
+        bshape = None
+        if args.batch_shape:
+            bs, cap, th = (int(x) for x in args.batch_shape.split(","))
+            bshape = {"bin_size": bs, "bin_capacity": cap, "raster_threads": th}
+        print(json.dumps(bench_batch(ctx, torch.cuda.current_stream(), max(5, args.steps // 5), 3, bshape)))
         ctx.close()
         return 0
 
