@@ -914,6 +914,22 @@ struct TriLds {
     uint32_t keylo, org, tw, start;
 };
 
+// The bin raster's state of one record, prepared once per (record, bin) so
+// that a 16-lane group stepping onto the record pays only a few multiply-adds:
+// the edge values at the first pixel of the record's render box clipped to
+// the bin, the edge steps, the steps from the last tile of a tile row to the
+// first tile of the next (4B - 4A (tw - 1)), the depth plane, the key's low
+// word, the LDS key index of the box's first tile, the box width in tiles and
+// the record's first tile in the block's tile list.
+struct BinTri {
+    int32_t w0, w1, w2;
+    int32_t a12, a20, a01, b12, b20, b01;
+    int32_t r0, r1, r2;
+    float z0, z10, z20;
+    uint32_t keylo, li0, tw, start, pad;
+};
+static_assert(sizeof(BinTri) == 80, "BinTri is five 16-byte LDS reads");
+
 __device__ __forceinline__ void load_tri(const TriRecord& rec, int hw, int hh, TriLds& t, uint32_t& bmin,
                                          uint32_t& bmax) {
     render_bbox(rec.pos[0], rec.pos[1], rec.pos[2], hw, hh, bmin, bmax);
@@ -1633,7 +1649,7 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
 // the records they share were already in the MALL.)
 template <int BS, int kThreads, bool Alpha>
 __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long (&keys)[BS * BS],
-                           TriLds (&tl)[kThreads], uint32_t (&s_wave_total)[kThreads / kWave],
+                           BinTri (&tl)[kThreads], uint32_t (&s_wave_total)[kThreads / kWave],
                            uint32_t (&talpha)[Alpha ? kThreads : 1]) {
     constexpr int kWaves = kThreads / kWave;
     constexpr int kGroups = kThreads / 16;
@@ -1688,22 +1704,37 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
     for (uint32_t base = 0; base < n; base += kThreads) {
         const uint32_t e = base + tid;
         uint32_t ntiles = 0;
-        TriLds t;
+        BinTri t;
         if (e < n) {
             const uint32_t r = (base == 0 && e < kPrefetchEntries) ? first_entry : list[e];
             const TriRecord rec = p.records[r];
             uint32_t bmin, bmax;
-            load_tri(rec, hw, hh, t, bmin, bmax);
-            uint32_t cx0 = max((uint32_t)lo16(bmin), x0), cy0 = max((uint32_t)hi16(bmin), y0);
-            uint32_t cx1 = min((uint32_t)lo16(bmax), x1), cy1 = min((uint32_t)hi16(bmax), y1);
+            TriLds f;
+            load_tri(rec, hw, hh, f, bmin, bmax);
+            const uint32_t cx0 = max((uint32_t)lo16(bmin), x0), cy0 = max((uint32_t)hi16(bmin), y0);
+            const uint32_t cx1 = min((uint32_t)lo16(bmax), x1), cy1 = min((uint32_t)hi16(bmax), y1);
             if constexpr (Alpha) talpha[tid] = (rec.flags & kRecAlpha) ? r : 0xFFFFFFFFu;
-            t.org = 0;
+            t.a12 = f.a12; t.a20 = f.a20; t.a01 = f.a01;
+            t.b12 = f.b12; t.b20 = f.b20; t.b01 = f.b01;
+            t.z0 = f.z0; t.z10 = f.z10; t.z20 = f.z20;
+            t.keylo = f.keylo;
             t.tw = 1;
             // a record on the overflow list is drawn by the overflow raster
             if (cx1 > cx0 && cy1 > cy0 && !(rec.flags & kRecSpilled)) {
-                t.org = (cx0 - x0) | ((cy0 - y0) << 16);
-                t.tw = (cx1 - cx0) >> 2;
-                ntiles = t.tw * ((cy1 - cy0) >> 2);
+                const uint32_t tw = (cx1 - cx0) >> 2;
+                t.tw = tw;
+                ntiles = tw * ((cy1 - cy0) >> 2);
+                // edge values at pixel (cx0, cy0); sums wrap in 32 bits as a
+                // direct evaluation does (DrawTriangle, Rasterizer.h:268-296)
+                t.w0 = edge_at(f.e0, f.a12, f.b12, cx0, cy0);
+                t.w1 = edge_at(f.e1, f.a20, f.b20, cx0, cy0);
+                t.w2 = edge_at(f.e2, f.a01, f.b01, cx0, cy0);
+                t.r0 = (int32_t)(((uint32_t)f.b12 << 2) - (uint32_t)__mul24(f.a12, (int32_t)(4 * (tw - 1))));
+                t.r1 = (int32_t)(((uint32_t)f.b20 << 2) - (uint32_t)__mul24(f.a20, (int32_t)(4 * (tw - 1))));
+                t.r2 = (int32_t)(((uint32_t)f.b01 << 2) - (uint32_t)__mul24(f.a01, (int32_t)(4 * (tw - 1))));
+                // the box starts on a tile corner: its first tile's index in the
+                // bin's 4x4-tiled LDS layout
+                t.li0 = ((cx0 - x0) << 2) + (cy0 - y0) * BS;
             }
         }
         uint32_t incl = ntiles;
@@ -1737,41 +1768,41 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
             for (uint32_t step = kThreads / 2; step > 0; step >>= 1) {
                 if (tl[k + step].start <= j) k += step;
             }
-            TriLds T = tl[k];
+            BinTri T = tl[k];
             uint32_t ai = 0xFFFFFFFFu;
             if constexpr (Alpha) ai = talpha[k];
             uint32_t next = (k + 1 < (uint32_t)kThreads) ? tl[k + 1].start : 0xFFFFFFFFu;
-            uint32_t pt = j - T.start;
-            uint32_t ty = pt / T.tw, tx = pt - ty * T.tw;
+            const uint32_t pt = j - T.start;
+            const uint32_t ty0 = pt / T.tw;
+            uint32_t tx = pt - ty0 * T.tw;
             // Edge values and the LDS key index of this lane's pixel, stepped
             // from tile to tile (DrawTriangle's stepping, Rasterizer.h:268-
-            // 296): +4A per tile in x; at the end of a row back to the row
-            // start plus 4B. Sums wrap in 32 bits, as a direct evaluation does.
-            uint32_t w0, w1, w2, r0, r1, r2, li, lrow;
-            auto enter = [&]() {
-                const uint32_t lx = (T.org & 0xFFFFu) + px, ly = (T.org >> 16) + ty * 4 + py;
-                r0 = (uint32_t)edge_at(T.e0, T.a12, T.b12, x0 + lx, y0 + ly);
-                r1 = (uint32_t)edge_at(T.e1, T.a20, T.b20, x0 + lx, y0 + ly);
-                r2 = (uint32_t)edge_at(T.e2, T.a01, T.b01, x0 + lx, y0 + ly);
-                w0 = r0 + (uint32_t)__mul24(T.a12, (int32_t)(4 * tx));
-                w1 = r1 + (uint32_t)__mul24(T.a20, (int32_t)(4 * tx));
-                w2 = r2 + (uint32_t)__mul24(T.a01, (int32_t)(4 * tx));
-                // (lx, ly) of the row start in the bin's 4x4-tiled LDS layout;
-                // the triangle's render box starts on a tile corner
-                lrow = ((lx & ~3u) << 2) + (ly & ~3u) * BS + px + (py << 2);
-                li = lrow + tx * 16;
+            // 296): +4A per tile in x, and from the last tile of a row to the
+            // first of the next the record's row step. Sums wrap in 32 bits,
+            // as a direct evaluation does.
+            uint32_t w0, w1, w2, s0, s1, s2, li, liRow;
+            auto enter = [&](uint32_t ty) {
+                const int32_t mx = (int32_t)(px + 4 * tx), my = (int32_t)(py + 4 * ty);
+                w0 = (uint32_t)T.w0 + (uint32_t)__mul24(T.a12, mx) + (uint32_t)__mul24(T.b12, my);
+                w1 = (uint32_t)T.w1 + (uint32_t)__mul24(T.a20, mx) + (uint32_t)__mul24(T.b20, my);
+                w2 = (uint32_t)T.w2 + (uint32_t)__mul24(T.a01, mx) + (uint32_t)__mul24(T.b01, my);
+                s0 = (uint32_t)T.a12 << 2;
+                s1 = (uint32_t)T.a20 << 2;
+                s2 = (uint32_t)T.a01 << 2;
+                li = T.li0 + (tx << 4) + ty * (4 * BS) + px + (py << 2);
+                liRow = 4 * BS - 16 * (T.tw - 1);
             };
-            enter();
+            enter(ty0);
             for (; j < j1; j++) {
-                if (j >= next) {      // step to the triangle owning tile j
+                if (j >= next) {      // step to the record owning tile j
                     do {
                         k++;
                         next = (k + 1 < (uint32_t)kThreads) ? tl[k + 1].start : 0xFFFFFFFFu;
                     } while (j >= next);
                     T = tl[k];
                     if constexpr (Alpha) ai = talpha[k];
-                    tx = ty = 0;
-                    enter();
+                    tx = 0;
+                    enter(0);
                 }
                 // DrawTriangle's coverage test and depth (Rasterizer.h:281-296)
                 if constexpr (!Alpha) {
@@ -1804,23 +1835,13 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
                         atomicMax(&keys[li], key);
                     }
                 }
-                if (++tx == T.tw) {
-                    tx = 0;
-                    ty++;
-                    r0 += (uint32_t)T.b12 << 2;
-                    r1 += (uint32_t)T.b20 << 2;
-                    r2 += (uint32_t)T.b01 << 2;
-                    w0 = r0;
-                    w1 = r1;
-                    w2 = r2;
-                    lrow += 4 * BS;
-                    li = lrow;
-                } else {
-                    w0 += (uint32_t)T.a12 << 2;
-                    w1 += (uint32_t)T.a20 << 2;
-                    w2 += (uint32_t)T.a01 << 2;
-                    li += 16;
-                }
+                // next tile, without a branch
+                const bool rowEnd = ++tx == T.tw;
+                w0 += rowEnd ? (uint32_t)T.r0 : s0;
+                w1 += rowEnd ? (uint32_t)T.r1 : s1;
+                w2 += rowEnd ? (uint32_t)T.r2 : s2;
+                li += rowEnd ? liRow : 16u;
+                tx = rowEnd ? 0u : tx;
             }
         }
         __syncthreads();
@@ -1928,7 +1949,7 @@ __global__ void __launch_bounds__(256) k_bin_order(DrawParams p) {
 template <int BS, int kThreads, bool Alpha>
 __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
     __shared__ unsigned long long keys[BS * BS];
-    __shared__ TriLds tl[kThreads];
+    __shared__ BinTri tl[kThreads];
     __shared__ uint32_t s_wave_total[kThreads / kWave];
     __shared__ uint32_t talpha[Alpha ? kThreads : 1];
     __shared__ uint32_t s_bin;
