@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "internal.h"
 #include "normals.h"
@@ -917,18 +918,37 @@ struct TriLds {
 // The bin raster's state of one record, prepared once per (record, bin) so
 // that a 16-lane group stepping onto the record pays only a few multiply-adds:
 // the edge values at the first pixel of the record's render box clipped to
-// the bin, the edge steps, the steps from the last tile of a tile row to the
-// first tile of the next (4B - 4A (tw - 1)), the depth plane, the key's low
-// word, the LDS key index of the box's first tile, the box width in tiles and
-// the record's first tile in the block's tile list.
-struct BinTri {
+// the bin, the edge steps, the depth plane, the key's low word, the LDS key
+// index of the box's first tile and the record's first tile in the block's
+// tile list. In bins of up to 32x32 pixels (at most 8x8 tiles) only the
+// box's tiles that the triangle can cover are walked, from a list of
+// (record, tile) items; 64-pixel bins walk every tile of the box, with the
+// step from the last tile of a tile row to the first of the next
+// (4B - 4A (tw - 1)).
+struct BinTriMask {
+    int32_t w0, w1, w2;
+    int32_t a12, a20, a01, b12, b20, b01;
+    float z0, z10, z20;
+    uint32_t keylo, li0, start, pad;
+};
+static_assert(sizeof(BinTriMask) == 64, "BinTriMask is four 16-byte LDS reads");
+
+struct BinTriStep {
     int32_t w0, w1, w2;
     int32_t a12, a20, a01, b12, b20, b01;
     int32_t r0, r1, r2;
     float z0, z10, z20;
     uint32_t keylo, li0, tw, start, pad;
 };
-static_assert(sizeof(BinTri) == 80, "BinTri is five 16-byte LDS reads");
+static_assert(sizeof(BinTriStep) == 80, "BinTriStep is five 16-byte LDS reads");
+
+template <int BS>
+using BinTri = typename std::conditional<(BS <= 32), BinTriMask, BinTriStep>::type;
+
+// (record, tile) items of one round of a bin raster chunk: four per thread
+__host__ __device__ constexpr uint32_t bin_items(int BS, int kThreads) {
+    return BS <= 32 ? 4u * (uint32_t)kThreads : 1u;
+}
 
 __device__ __forceinline__ void load_tri(const TriRecord& rec, int hw, int hh, TriLds& t, uint32_t& bmin,
                                          uint32_t& bmax) {
@@ -1649,7 +1669,8 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
 // the records they share were already in the MALL.)
 template <int BS, int kThreads, bool Alpha>
 __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long (&keys)[BS * BS],
-                           BinTri (&tl)[kThreads], uint32_t (&s_wave_total)[kThreads / kWave],
+                           BinTri<BS> (&tl)[kThreads], uint32_t (&items)[bin_items(BS, kThreads)],
+                           uint32_t (&s_wave_total)[kThreads / kWave],
                            uint32_t (&talpha)[Alpha ? kThreads : 1]) {
     constexpr int kWaves = kThreads / kWave;
     constexpr int kGroups = kThreads / 16;
@@ -1701,10 +1722,16 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
     const bool countFrags = counting(p);
     uint32_t covered = 0, passed = 0, tiles = 0;
     const uint32_t g = tid >> 4, px = tid & 3, py = (tid >> 2) & 3;
+    constexpr bool kMasked = BS <= 32;
+    constexpr uint32_t kItems = bin_items(BS, kThreads);
+    constexpr uint32_t kTilesX = BS / 4;                                   // tiles per bin row
+    constexpr uint32_t kTileShift = kTilesX == 2 ? 1u : (kTilesX == 4 ? 2u : 3u);
     for (uint32_t base = 0; base < n; base += kThreads) {
         const uint32_t e = base + tid;
+        const uint32_t cnt = min(n - base, (uint32_t)kThreads);          // records of this chunk
         uint32_t ntiles = 0;
-        BinTri t;
+        unsigned long long tmask = 0;      // candidate tiles of this thread's record
+        BinTri<BS> t;
         if (e < n) {
             const uint32_t r = (base == 0 && e < kPrefetchEntries) ? first_entry : list[e];
             const TriRecord rec = p.records[r];
@@ -1718,23 +1745,68 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
             t.b12 = f.b12; t.b20 = f.b20; t.b01 = f.b01;
             t.z0 = f.z0; t.z10 = f.z10; t.z20 = f.z20;
             t.keylo = f.keylo;
-            t.tw = 1;
+            t.w0 = t.w1 = t.w2 = 0;
+            t.li0 = 0;
+            uint32_t tw = 0, th = 0;
             // a record on the overflow list is drawn by the overflow raster
             if (cx1 > cx0 && cy1 > cy0 && !(rec.flags & kRecSpilled)) {
-                const uint32_t tw = (cx1 - cx0) >> 2;
-                t.tw = tw;
-                ntiles = tw * ((cy1 - cy0) >> 2);
+                tw = (cx1 - cx0) >> 2;
+                th = (cy1 - cy0) >> 2;
                 // edge values at pixel (cx0, cy0); sums wrap in 32 bits as a
                 // direct evaluation does (DrawTriangle, Rasterizer.h:268-296)
                 t.w0 = edge_at(f.e0, f.a12, f.b12, cx0, cy0);
                 t.w1 = edge_at(f.e1, f.a20, f.b20, cx0, cy0);
                 t.w2 = edge_at(f.e2, f.a01, f.b01, cx0, cy0);
-                t.r0 = (int32_t)(((uint32_t)f.b12 << 2) - (uint32_t)__mul24(f.a12, (int32_t)(4 * (tw - 1))));
-                t.r1 = (int32_t)(((uint32_t)f.b20 << 2) - (uint32_t)__mul24(f.a20, (int32_t)(4 * (tw - 1))));
-                t.r2 = (int32_t)(((uint32_t)f.b01 << 2) - (uint32_t)__mul24(f.a01, (int32_t)(4 * (tw - 1))));
                 // the box starts on a tile corner: its first tile's index in the
                 // bin's 4x4-tiled LDS layout
                 t.li0 = ((cx0 - x0) << 2) + (cy0 - y0) * BS;
+            }
+            if constexpr (kMasked) {
+                // The tiles of the box that the triangle can cover, one tile
+                // row at a time. Each edge's largest value over a tile's 16
+                // pixels is V = D + 4A tx + 4B ty, D = e + max(0, 3A) +
+                // max(0, 3B) at the box's first tile; a tile where one V is
+                // negative has no covered pixel. Along a row V >= 0 holds on
+                // one side of q = -(D + 4B ty) / 4A: tx >= q for A > 0, tx <= q
+                // for A < 0. q comes from a float reciprocal (relative error a
+                // few 2^-24, far below the 2^-10 margin taken outward over the
+                // |q| <= 16 that matter), so the mask keeps every tile with a
+                // covered pixel. Edge values inside a render box stay below 2^31
+                // in magnitude (|e| < 2^27, |A|, |B| < 2^17, pixel coordinates
+                // below 2^12), so D + 4B ty is exact.
+                unsigned long long mask = 0;
+                if (tw) {
+                    const int32_t A[3] = {f.a12, f.a20, f.a01}, Bv[3] = {f.b12, f.b20, f.b01};
+                    const int32_t W0[3] = {t.w0, t.w1, t.w2};
+                    int32_t D[3];
+                    float nr[3];
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        D[i] = W0[i] + max(0, 3 * A[i]) + max(0, 3 * Bv[i]);
+                        nr[i] = A[i] != 0 ? -__builtin_amdgcn_rcpf((float)(4 * A[i])) : 0.0f;
+                    }
+                    for (uint32_t ty = 0; ty < th; ty++) {
+                        int32_t lo = 0, hi = (int32_t)tw - 1;
+#pragma unroll
+                        for (int i = 0; i < 3; i++) {
+                            const int32_t R = D[i] + __mul24(Bv[i], (int32_t)(4 * ty));
+                            const float q = fminf(fmaxf((float)R * nr[i], -16.0f), 16.0f);
+                            if (A[i] > 0) lo = max(lo, (int32_t)ceilf(q - 0x1p-10f));
+                            else if (A[i] < 0) hi = min(hi, (int32_t)floorf(q + 0x1p-10f));
+                            else if (R < 0) hi = -1;
+                        }
+                        if (hi >= lo)
+                            mask |= (unsigned long long)((2u << hi) - (1u << lo)) << (ty * kTilesX);
+                    }
+                }
+                tmask = mask;
+                ntiles = (uint32_t)__popcll(mask);
+            } else {
+                ntiles = tw * th;
+                t.tw = tw ? tw : 1u;
+                t.r0 = (int32_t)(((uint32_t)f.b12 << 2) - (uint32_t)__mul24(f.a12, (int32_t)(4 * (t.tw - 1))));
+                t.r1 = (int32_t)(((uint32_t)f.b20 << 2) - (uint32_t)__mul24(f.a20, (int32_t)(4 * (t.tw - 1))));
+                t.r2 = (int32_t)(((uint32_t)f.b01 << 2) - (uint32_t)__mul24(f.a01, (int32_t)(4 * (t.tw - 1))));
             }
         }
         uint32_t incl = ntiles;
@@ -1755,98 +1827,142 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
         t.start = prev + incl - ntiles;
         tl[tid] = t;
         if (e >= n) tl[tid].start = 0xFFFFFFFFu;
-        __syncthreads();
 
-        // group g walks tiles [j, j1) of the block's tile list
-        const uint32_t per = (total + kGroups - 1) / kGroups;
-        uint32_t j = min(g * per, total);
-        const uint32_t j1 = min(j + per, total);
-        if (countFrags && j < j1) tiles += j1 - j;
-        if (j < j1) {
-            uint32_t k = 0;
-#pragma unroll
-            for (uint32_t step = kThreads / 2; step > 0; step >>= 1) {
-                if (tl[k + step].start <= j) k += step;
-            }
-            BinTri T = tl[k];
-            uint32_t ai = 0xFFFFFFFFu;
-            if constexpr (Alpha) ai = talpha[k];
-            uint32_t next = (k + 1 < (uint32_t)kThreads) ? tl[k + 1].start : 0xFFFFFFFFu;
-            const uint32_t pt = j - T.start;
-            const uint32_t ty0 = pt / T.tw;
-            uint32_t tx = pt - ty0 * T.tw;
-            // Edge values and the LDS key index of this lane's pixel, stepped
-            // from tile to tile (DrawTriangle's stepping, Rasterizer.h:268-
-            // 296): +4A per tile in x, and from the last tile of a row to the
-            // first of the next the record's row step. Sums wrap in 32 bits,
-            // as a direct evaluation does.
-            uint32_t w0, w1, w2, s0, s1, s2, li, liRow;
-            auto enter = [&](uint32_t ty) {
-                const int32_t mx = (int32_t)(px + 4 * tx), my = (int32_t)(py + 4 * ty);
-                w0 = (uint32_t)T.w0 + (uint32_t)__mul24(T.a12, mx) + (uint32_t)__mul24(T.b12, my);
-                w1 = (uint32_t)T.w1 + (uint32_t)__mul24(T.a20, mx) + (uint32_t)__mul24(T.b20, my);
-                w2 = (uint32_t)T.w2 + (uint32_t)__mul24(T.a01, mx) + (uint32_t)__mul24(T.b01, my);
-                s0 = (uint32_t)T.a12 << 2;
-                s1 = (uint32_t)T.a20 << 2;
-                s2 = (uint32_t)T.a01 << 2;
-                li = T.li0 + (tx << 4) + ty * (4 * BS) + px + (py << 2);
-                liRow = 4 * BS - 16 * (T.tw - 1);
-            };
-            enter(ty0);
-            for (; j < j1; j++) {
-                if (j >= next) {      // step to the record owning tile j
-                    do {
-                        k++;
-                        next = (k + 1 < (uint32_t)kThreads) ? tl[k + 1].start : 0xFFFFFFFFu;
-                    } while (j >= next);
-                    T = tl[k];
-                    if constexpr (Alpha) ai = talpha[k];
-                    tx = 0;
-                    enter(0);
-                }
-                // DrawTriangle's coverage test and depth (Rasterizer.h:281-296)
-                if constexpr (!Alpha) {
-                    if ((int32_t)(w0 | w1 | w2) >= 0) {
-                        const float depth = __builtin_fmaf((float)(int32_t)w1, T.z10,
-                                                           __builtin_fmaf((float)(int32_t)w2, T.z20, T.z0));
-                        if (countFrags) covered++;
-                        if (depth > clearDepth) {
-                            if (countFrags) passed++;
-                            // an LDS max without a returned value: no wait on the
-                            // old key (a read-then-max would stall every step)
-                            const unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
-                            atomicMax(&keys[li], key);
-                        }
-                    }
-                } else {
-                    const bool cov = (int32_t)(w0 | w1 | w2) >= 0;
+        // DrawTriangle's coverage test and depth (Rasterizer.h:281-296) for
+        // this lane's pixel of a tile of record T, given its edge values and
+        // LDS key index
+        BinTri<BS> T;
+        uint32_t ai = 0xFFFFFFFFu;
+        auto shade = [&](uint32_t w0, uint32_t w1, uint32_t w2, uint32_t li) {
+            if constexpr (!Alpha) {
+                if ((int32_t)(w0 | w1 | w2) >= 0) {
                     const float depth = __builtin_fmaf((float)(int32_t)w1, T.z10,
                                                        __builtin_fmaf((float)(int32_t)w2, T.z20, T.z0));
-                    bool pass = cov && depth > clearDepth;
-                    if (ai != 0xFFFFFFFFu && (__ballot(pass) & (0xFFFFull << (lane & 48u)))) {
-                        const bool ok = alpha_test_tile(p.tables, p.alpha[ai], ((T.keylo >> 7) & 1u) == 0,
-                                                        V.textures, (int32_t)w1, (int32_t)w2, lane);
-                        pass = pass && ok;
-                    }
-                    if (countFrags && cov) covered++;
-                    if (pass) {
+                    if (countFrags) covered++;
+                    if (depth > clearDepth) {
                         if (countFrags) passed++;
+                        // an LDS max without a returned value: no wait on the
+                        // old key (a read-then-max would stall every step)
                         const unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
                         atomicMax(&keys[li], key);
                     }
                 }
-                // next tile, without a branch
-                const bool rowEnd = ++tx == T.tw;
-                w0 += rowEnd ? (uint32_t)T.r0 : s0;
-                w1 += rowEnd ? (uint32_t)T.r1 : s1;
-                w2 += rowEnd ? (uint32_t)T.r2 : s2;
-                li += rowEnd ? liRow : 16u;
-                tx = rowEnd ? 0u : tx;
+            } else {
+                const bool cov = (int32_t)(w0 | w1 | w2) >= 0;
+                const float depth = __builtin_fmaf((float)(int32_t)w1, T.z10,
+                                                   __builtin_fmaf((float)(int32_t)w2, T.z20, T.z0));
+                bool pass = cov && depth > clearDepth;
+                if (ai != 0xFFFFFFFFu && (__ballot(pass) & (0xFFFFull << (lane & 48u)))) {
+                    const bool ok = alpha_test_tile(p.tables, p.alpha[ai], ((T.keylo >> 7) & 1u) == 0,
+                                                    V.textures, (int32_t)w1, (int32_t)w2, lane);
+                    pass = pass && ok;
+                }
+                if (countFrags && cov) covered++;
+                if (pass) {
+                    if (countFrags) passed++;
+                    const unsigned long long key = ((unsigned long long)fbits(depth) << 32) | T.keylo;
+                    atomicMax(&keys[li], key);
+                }
             }
-        }
-        __syncthreads();
-    }
+        };
 
+        if constexpr (kMasked) {
+            // The chunk's candidate tiles as a list of items (record << 6 |
+            // tile bit), written by each record's thread at its start, at
+            // most kItems per round; group g takes items [j, j1) of the round
+            // and reads a record's state only when the record changes.
+            for (uint32_t r0 = 0; r0 < total; r0 += kItems) {
+                if (ntiles && t.start < r0 + kItems && t.start + ntiles > r0) {
+                    unsigned long long m = tmask;
+                    for (uint32_t pos = t.start; m && pos < r0 + kItems; pos++, m &= m - 1ull) {
+                        if (pos >= r0) items[pos - r0] = (tid << 6) | (uint32_t)__builtin_ctzll(m);
+                    }
+                }
+                __syncthreads();
+                const uint32_t nr = min(total - r0, kItems);
+                const uint32_t per = (nr + kGroups - 1) / kGroups;
+                uint32_t j = min(g * per, nr);
+                const uint32_t j1 = min(j + per, nr);
+                if (countFrags && j < j1) tiles += j1 - j;
+                uint32_t kcur = 0xFFFFFFFFu;
+                for (; j < j1; j++) {
+                    const uint32_t it = items[j];
+                    const uint32_t k = it >> 6, b = it & 63u;
+                    if (k != kcur) {
+                        kcur = k;
+                        T = tl[k];
+                        if constexpr (Alpha) ai = talpha[k];
+                    }
+                    const uint32_t tx = b & (kTilesX - 1), ty = b >> kTileShift;
+                    const int32_t mx = (int32_t)(px + 4 * tx), my = (int32_t)(py + 4 * ty);
+                    shade((uint32_t)T.w0 + (uint32_t)__mul24(T.a12, mx) + (uint32_t)__mul24(T.b12, my),
+                          (uint32_t)T.w1 + (uint32_t)__mul24(T.a20, mx) + (uint32_t)__mul24(T.b20, my),
+                          (uint32_t)T.w2 + (uint32_t)__mul24(T.a01, mx) + (uint32_t)__mul24(T.b01, my),
+                          T.li0 + (tx << 4) + ty * (4 * BS) + px + (py << 2));
+                }
+                __syncthreads();
+            }
+        } else {
+            __syncthreads();
+            // group g walks tiles [j, j1) of the block's tile list
+            const uint32_t per = (total + kGroups - 1) / kGroups;
+            uint32_t j = min(g * per, total);
+            const uint32_t j1 = min(j + per, total);
+            if (countFrags && j < j1) tiles += j1 - j;
+            if (j < j1) {
+                uint32_t k = 0;
+#pragma unroll
+                for (uint32_t step = kThreads / 2; step > 0; step >>= 1) {
+                    if (tl[k + step].start <= j) k += step;
+                }
+                T = tl[k];
+                if constexpr (Alpha) ai = talpha[k];
+                uint32_t next = (k + 1 < (uint32_t)kThreads) ? tl[k + 1].start : 0xFFFFFFFFu;
+                const uint32_t pt = j - T.start;
+                const uint32_t ty0 = pt / T.tw;
+                uint32_t tx = pt - ty0 * T.tw;
+                // Edge values and the LDS key index of this lane's pixel, stepped
+                // from tile to tile (DrawTriangle's stepping, Rasterizer.h:268-
+                // 296): +4A per tile in x, and from the last tile of a row to the
+                // first of the next the record's row step. Sums wrap in 32 bits,
+                // as a direct evaluation does.
+                uint32_t w0, w1, w2, s0, s1, s2, li, liRow;
+                auto enter = [&](uint32_t ty) {
+                    const int32_t mx = (int32_t)(px + 4 * tx), my = (int32_t)(py + 4 * ty);
+                    w0 = (uint32_t)T.w0 + (uint32_t)__mul24(T.a12, mx) + (uint32_t)__mul24(T.b12, my);
+                    w1 = (uint32_t)T.w1 + (uint32_t)__mul24(T.a20, mx) + (uint32_t)__mul24(T.b20, my);
+                    w2 = (uint32_t)T.w2 + (uint32_t)__mul24(T.a01, mx) + (uint32_t)__mul24(T.b01, my);
+                    s0 = (uint32_t)T.a12 << 2;
+                    s1 = (uint32_t)T.a20 << 2;
+                    s2 = (uint32_t)T.a01 << 2;
+                    li = T.li0 + (tx << 4) + ty * (4 * BS) + px + (py << 2);
+                    liRow = 4 * BS - 16 * (T.tw - 1);
+                };
+                enter(ty0);
+                for (; j < j1; j++) {
+                    if (j >= next) {      // step to the record owning tile j
+                        do {
+                            k++;
+                            next = (k + 1 < (uint32_t)kThreads) ? tl[k + 1].start : 0xFFFFFFFFu;
+                        } while (j >= next);
+                        T = tl[k];
+                        if constexpr (Alpha) ai = talpha[k];
+                        tx = 0;
+                        enter(0);
+                    }
+                    shade(w0, w1, w2, li);
+                    // next tile, without a branch
+                    const bool rowEnd = ++tx == T.tw;
+                    w0 += rowEnd ? (uint32_t)T.r0 : s0;
+                    w1 += rowEnd ? (uint32_t)T.r1 : s1;
+                    w2 += rowEnd ? (uint32_t)T.r2 : s2;
+                    li += rowEnd ? liRow : 16u;
+                    tx = rowEnd ? 0u : tx;
+                }
+            }
+            __syncthreads();
+        }
+    }
     // each thread writes back the keys it initialised, so the next bin's
     // initialisation needs no extra barrier; an overflowing bin takes the
     // maximum with the spill keys of its pixels and zeroes them for the next
@@ -1949,7 +2065,8 @@ __global__ void __launch_bounds__(256) k_bin_order(DrawParams p) {
 template <int BS, int kThreads, bool Alpha>
 __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
     __shared__ unsigned long long keys[BS * BS];
-    __shared__ BinTri tl[kThreads];
+    __shared__ BinTri<BS> tl[kThreads];
+    __shared__ uint32_t items[bin_items(BS, kThreads)];
     __shared__ uint32_t s_wave_total[kThreads / kWave];
     __shared__ uint32_t talpha[Alpha ? kThreads : 1];
     __shared__ uint32_t s_bin;
@@ -1967,7 +2084,7 @@ __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
         s_bin = bin;
     }
     __syncthreads();
-    raster_bin<BS, kThreads, Alpha>(p, s_bin, keys, tl, s_wave_total, talpha);
+    raster_bin<BS, kThreads, Alpha>(p, s_bin, keys, tl, items, s_wave_total, talpha);
 }
 
 // Clears every scratch counter: at the end of a direct draw (the binned
