@@ -88,9 +88,9 @@ KERNEL_SYMBOLS = {
 }
 
 
-def _rocprof_counter(counter: str, child_args: list, tag: str) -> dict | None:
-    """Runs this script as a child under rocprofv3 --pmc <counter> and returns
-    the per-dispatch average of the counter for every kernel, or None."""
+def _rocprof_counters(counters: list, child_args: list, tag: str) -> dict | None:
+    """Runs this script as a child under rocprofv3 --pmc <counters> (one pass)
+    and returns {kernel: {counter: per-dispatch average}}, or None."""
     import csv
     import shutil
     import subprocess
@@ -100,7 +100,7 @@ def _rocprof_counter(counter: str, child_args: list, tag: str) -> dict | None:
     if not exe:
         return None
     out = tempfile.mkdtemp(prefix="glimp_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
-    cmd = ["timeout", "-s", "KILL", "150", exe, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", tag,
+    cmd = ["timeout", "-s", "KILL", "150", exe, "--pmc", *counters, "--output-format", "csv", "-d", out, "-o", tag,
            "--", sys.executable, os.path.abspath(__file__), *child_args]
     try:
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=170)
@@ -111,13 +111,22 @@ def _rocprof_counter(counter: str, child_args: list, tag: str) -> dict | None:
         for f in files:
             if f.endswith("counter_collection.csv"):
                 for row in csv.DictReader(open(os.path.join(root, f))):
-                    k = row["Kernel_Name"]
-                    sums[k] = sums.get(k, 0.0) + float(row["Counter_Value"])
-                    counts[k] = counts.get(k, 0) + 1
+                    key = (row["Kernel_Name"], row["Counter_Name"])
+                    sums[key] = sums.get(key, 0.0) + float(row["Counter_Value"])
+                    counts[key] = counts.get(key, 0) + 1
     shutil.rmtree(out, ignore_errors=True)
     if r.returncode != 0 and not sums:
         return None
-    return {k: sums[k] / counts[k] for k in sums}
+    res: dict = {}
+    for (k, c), v in sums.items():
+        res.setdefault(k, {})[c] = v / counts[(k, c)]
+    return res
+
+
+def _rocprof_counter(counter: str, child_args: list, tag: str) -> dict | None:
+    """One counter: {kernel: per-dispatch average}, or None."""
+    r = _rocprof_counters([counter], child_args, tag)
+    return None if r is None else {k: v[counter] for k, v in r.items() if counter in v}
 
 
 def _pick(d, sub):
@@ -125,13 +134,25 @@ def _pick(d, sub):
     return vals[0] if vals else None
 
 
+# TCC read requests to memory by size; on gfx950 FETCH_SIZE counts every
+# 128-byte request as 64 bytes (its formula takes the 128-byte count from
+# TCC_BUBBLE, which stays zero), so a copy kernel reads it as half its bytes
+READ_REQUESTS = ("TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum")
+
+
+def _request_bytes(c: dict) -> float:
+    return (32 * c.get("TCC_EA0_RDREQ_32B_sum", 0.0) + 64 * c.get("TCC_EA0_RDREQ_64B_sum", 0.0)
+            + 128 * c.get("TCC_EA0_RDREQ_128B_sum", 0.0))
+
+
 def measure_traffic(shape: dict | None) -> dict | None:
     """HBM bytes per launch of each kernel of the frame: FETCH_SIZE and
     WRITE_SIZE (KiB per dispatch) from two rocprofv3 runs of a short frame
     loop at the headline launch shape, each scaled by what the counter reads
-    for a device copy of known size. Two more known-bytes kernels check that
-    scale: a read-only reduction and a gather of 64-byte rows in random order
-    (the access pattern of the resolve's vertex and texel fetches)."""
+    for a device copy of known size. A third run counts the read requests by
+    size (32, 64 and 128 bytes), which gives the bytes read without a scale;
+    both readings are checked on the copy and on a read-only reduction of
+    known size."""
     n_bytes = 1 << 28
     frame = ["--steps", "3", "--warmup", "1", "--no-cpu-baseline", "--no-batch", "--no-traffic", "--no-counts"]
     if shape:
@@ -139,39 +160,38 @@ def measure_traffic(shape: dict | None) -> dict | None:
                   "--raster-threads", str(shape["raster_threads"])]
     fetch = _rocprof_counter("FETCH_SIZE", frame, "frame")
     write = _rocprof_counter("WRITE_SIZE", frame, "frame")
-    cal = {}
-    for kind in ("copy", "read", "gather"):
-        cal[kind] = (_rocprof_counter("FETCH_SIZE", [f"--calibrate-{kind}", str(n_bytes)], kind),
-                     _rocprof_counter("WRITE_SIZE", [f"--calibrate-{kind}", str(n_bytes)], kind))
-    if not (fetch and write and cal["copy"][0] and cal["copy"][1]):
+    reqs = _rocprof_counters(list(READ_REQUESTS), frame, "frame")
+    copy_args, read_args = ["--calibrate-copy", str(n_bytes)], ["--calibrate-read", str(n_bytes)]
+    cf, cw = _rocprof_counter("FETCH_SIZE", copy_args, "copy"), _rocprof_counter("WRITE_SIZE", copy_args, "copy")
+    creq = _rocprof_counters(list(READ_REQUESTS), copy_args, "copy")
+    rf, rreq = _rocprof_counter("FETCH_SIZE", read_args, "read"), _rocprof_counters(list(READ_REQUESTS), read_args, "read")
+    if not (fetch and write and cf and cw):
         return None
-    kf, kw = _pick(cal["copy"][0], "copyBuffer"), _pick(cal["copy"][1], "copyBuffer")
+    kf, kw = _pick(cf, "copyBuffer"), _pick(cw, "copyBuffer")
     if not kf or not kw:
         return None
     f_scale = n_bytes / (kf * 1024.0)      # bytes per counted KiB
     w_scale = n_bytes / (kw * 1024.0)
     out = {"fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4)}
-
-    def largest(d):
-        return max(d.values()) if d else None
-
-    # the same scales applied to the other known-bytes kernels: the bytes they
-    # must read (and write), over what the scaled counters say
+    # known bytes over each reading: 1.0 is exact
     checks = {}
-    rf = largest(cal["read"][0])
-    if rf:
-        checks["read_fetch"] = round(n_bytes / (rf * 1024.0 * f_scale), 4)
-    gf, gw = largest(cal["gather"][0]), largest(cal["gather"][1])
-    if gf:
-        # rows of 64 bytes plus one 8-byte index per row read; the rows written
-        checks["gather_fetch"] = round((n_bytes + n_bytes // 8) / (gf * 1024.0 * f_scale), 4)
-    if gw:
-        checks["gather_write"] = round(n_bytes / (gw * 1024.0 * w_scale), 4)
+    rfv = _pick(rf, "reduce_kernel") if rf else None
+    if rfv:
+        checks["reduce_fetch_scaled"] = round(n_bytes / (rfv * 1024.0 * f_scale), 4)
+    if creq and _pick(creq, "copyBuffer"):
+        checks["copy_read_requests"] = round(n_bytes / _request_bytes(_pick(creq, "copyBuffer")), 4)
+    if rreq and _pick(rreq, "reduce_kernel"):
+        checks["reduce_read_requests"] = round(n_bytes / _request_bytes(_pick(rreq, "reduce_kernel")), 4)
     out["scale_checks"] = checks
+    by_req = {}
     for name, sub in KERNEL_SYMBOLS.items():
         fv, wv = _pick(fetch, sub), _pick(write, sub)
         if fv is not None and wv is not None:
             out[name] = int(fv * 1024.0 * f_scale + wv * 1024.0 * w_scale)
+            rq = _pick(reqs, sub) if reqs else None
+            if rq:
+                by_req[name] = int(_request_bytes(rq) + wv * 1024.0)
+    out["by_request_size"] = by_req
     # vector-ALU instructions per launch (wave instructions)
     valu = _rocprof_counter("SQ_INSTS_VALU", frame, "frame")
     if valu:
@@ -368,21 +388,17 @@ def bench_clip_frame(ctx, stream, shape: dict, steps: int, warmup: int) -> dict:
 
 def _calibrate(kind: str, n_bytes: int) -> None:
     """Known-bytes kernels for the HBM counter calibration (10 launches each):
-    copy (reads and writes n_bytes), read (a sum over n_bytes) and gather
-    (n_bytes of 64-byte rows read in a random order through an index, written
-    in order)."""
+    copy (a device copy that reads and writes n_bytes) and read (an int64 sum
+    that reads n_bytes)."""
     import torch
 
-    src = torch.empty(n_bytes // 4, dtype=torch.int32, device="cuda").fill_(1)
     if kind == "copy":
+        src = torch.empty(n_bytes // 4, dtype=torch.int32, device="cuda").fill_(1)
         dst = torch.empty_like(src)
         fn = lambda: dst.copy_(src)                                    # noqa: E731
-    elif kind == "read":
-        fn = lambda: src.sum()                                         # noqa: E731
     else:
-        rows = src.view(-1, 16)
-        idx = torch.randperm(rows.shape[0], device="cuda")
-        fn = lambda: torch.index_select(rows, 0, idx)                  # noqa: E731
+        src = torch.empty(n_bytes // 8, dtype=torch.int64, device="cuda").fill_(1)
+        fn = lambda: src.sum()                                         # noqa: E731
     torch.cuda.synchronize()
     for _ in range(10):
         fn()
@@ -408,11 +424,11 @@ def main() -> int:
                     help="only the 64-view batching measurement (its JSON object)")
     ap.add_argument("--batch-shape", default="", metavar="BIN,CAPACITY,THREADS",
                     help="launch shape of the batching measurement instead of the autotuner's")
-    for kind in ("copy", "read", "gather"):
+    for kind in ("copy", "read"):
         ap.add_argument(f"--calibrate-{kind}", type=int, default=0, metavar="BYTES",
                         help=f"only run the {kind} calibration kernel over BYTES bytes 10 times")
     args = ap.parse_args()
-    for kind in ("copy", "read", "gather"):
+    for kind in ("copy", "read"):
         if getattr(args, f"calibrate_{kind}"):
             _calibrate(kind, getattr(args, f"calibrate_{kind}"))
             return 0
@@ -504,6 +520,23 @@ def main() -> int:
     ctx.profile(False)
     ms_calls = run(frame_calls, args.steps)
 
+    # The lit resolve alone: `steps` back-to-back launches over the vis-buffer
+    # the frame left, between two events on the stream it runs on. The
+    # per-dispatch events above stamp each dispatch with extra packets that
+    # add a few microseconds to every kernel; this loop does not.
+    resolve_us = None
+    if not args.direct:
+        resolve = lambda: fb.resolve(ds, view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting, stream)  # noqa: E731
+        for _ in range(args.warmup):
+            resolve()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            resolve()
+        e1.record(stream)
+        e1.synchronize()
+        resolve_us = e0.elapsed_time(e1) * 1e3 / args.steps
+
     if args.no_counts:
         print(json.dumps({"ms_per_step": round(ms, 4), "shape": shape,
                           "kernels_us": {k: round(t / max(1, c) * 1e3, 2) for k, (c, t) in kern.items()}}))
@@ -545,9 +578,19 @@ def main() -> int:
         per_kernel[name] = {"avg_us": round(avg_us, 2), "launches": launches, "alg_bytes": b,
                             "GBps": round(b / (avg_us * 1e-6) / 1e9, 1) if avg_us > 0 and b else None,
                             "hbm_bytes": traffic.get(name) if traffic else None,
+                            "hbm_bytes_by_request": traffic["by_request_size"].get(name) if traffic else None,
                             "valu": _valu_roofline(traffic, name, avg_us)}
     dom = max(kern.items(), key=lambda kv: kv[1][1])[0]
     dk = per_kernel[dom]
+    # the roofline's duration: the dominant kernel's own back-to-back loop
+    # where it has one (the lit resolve), else its dispatch-stamped average
+    roof_us, roof_timing = dk["avg_us"], ("dispatch-stamped events (hipExtLaunchKernel) over the timed steps of "
+                                          "the eager frame at the headline shape")
+    if dom == "resolve_lit" and resolve_us:
+        roof_us = resolve_us
+        roof_timing = (f"{args.steps} back-to-back launches of the frame's lit resolve between two events on its "
+                       "stream")
+    roof_gbps = dk["alg_bytes"] / (roof_us * 1e-6) / 1e9 if dk["alg_bytes"] else None
     tris = scene.triangles()
     result = {
         "metric": "Mtri/s at 1080p/1M tris (frame: clear+cull+mesh/setup+clip+bin+raster into vis-buffer+lit resolve)",
@@ -578,13 +621,14 @@ def main() -> int:
                        f"binned {shape['bin_size']}x{shape['bin_size']}, {shape['bin_capacity']} "
                        f"records per bin, {shape['raster_threads']}-thread bin raster"),
                    "submitted_tris": tris},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(dk["GBps"] / HBM_PEAK_GBS, 4) if dk["GBps"] else None,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(roof_gbps, 1) if roof_gbps else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(roof_gbps / HBM_PEAK_GBS, 4) if roof_gbps else None,
                      "traffic": traffic.get(dom) if traffic else None,
-                     "valu": _valu_roofline(traffic, dom, dk["avg_us"]),
-                     "alg_bytes_per_launch": dk["alg_bytes"], "avg_us": dk["avg_us"],
-                     "timing": "dispatch-stamped events (hipExtLaunchKernel) over the timed steps of the eager "
-                               "frame at the headline shape"},
+                     "traffic_by_request": traffic["by_request_size"].get(dom) if traffic else None,
+                     "valu": _valu_roofline(traffic, dom, roof_us),
+                     "alg_bytes_per_launch": dk["alg_bytes"], "avg_us": round(roof_us, 2),
+                     "avg_us_dispatch_events": dk["avg_us"], "timing": roof_timing},
         "traffic_calibration": ({k: traffic[k] for k in ("fetch_scale", "write_scale", "scale_checks")}
                                 if traffic else None),
         "kernels": per_kernel,
