@@ -485,9 +485,10 @@ def main() -> int:
         shape = {"bin_size": args.bin_size or 16, "bin_capacity": args.bin_capacity or 1024,
                  "raster_threads": args.raster_threads or 256}
         batch = g.Batch(ctx, [(ds, fb, view, planes, flags)], shape=shape)
+        # the autotuner times the whole frame, resolve included
+        batch.set_resolve([(view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting)])
         if not args.bin_size and not args.bin_capacity and not args.raster_threads:
             shape, _ = batch.autotune(stream)
-        batch.set_resolve([(view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting)])
 
     def frame_calls():
         # the reference's call sequence: Clear, DrawMeshlets (with the cull), Resolve

@@ -209,3 +209,44 @@ def test_batch_with_resolve(gpu_ctx, small):
     assert all((w == gt).all() for w, gt in zip(want, got))
     b.set_resolve(None)
     b.close()
+
+
+@pytest.mark.parametrize("shape", [(32, 2048, 256), (32, 1024, 512), (32, 256, 1024), (16, 1024, 128),
+                                   (16, 256, 256), (8, 512, 128), (64, 1024, 256), (64, 4096, 1024)])
+def test_single_view_lit_batch_equals_separate_calls(gpu_ctx, shape):
+    """The bench's frame: a batch of one view with its lit resolve gives the
+    colour (light sprites included) and the vis-buffer of clear + draw +
+    resolve as separate calls, eager and on every graph replay, for every bin
+    size and block shape, including bins whose records overflow (capacity
+    256)."""
+    import glimpsw_amd as g
+
+    scene, view = S.frame_1080p(num_meshlets=2048, width=960, height=540, tex_size=256, num_materials=4)
+    ds = g.Scene(gpu_ctx, scene)
+    fb = g.Framebuffer(gpu_ctx, view.width, view.height)
+    ref = g.Framebuffer(gpu_ctx, view.width, view.height)
+    lighting = S.default_lighting(view)
+    planes = view.frustum_planes()
+    ref.clear(0, 0.0)
+    ref.draw(ds, view.object_to_clip, view.meshlet_count, 0, planes)
+    ref.resolve(ds, view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting)
+    want_c = ref.read_color()
+    want_ids, want_d = ref.read()
+    assert (want_ids != 0).any()
+    bs, cap, th = shape
+    b = g.Batch(gpu_ctx, [(ds, fb, view, planes, 0)], shape={"bin_size": bs, "bin_capacity": cap,
+                                                              "raster_threads": th})
+    b.set_resolve([(view.object_to_clip, g.GLIMP_RESOLVE_LIT, lighting)])
+    print(f"[frame batch] shape {shape}", flush=True)
+    for r, graph in enumerate([False, True, True, True, False]):
+        b.launch(graph=graph)
+        got_c = fb.read_color()
+        ids, d = fb.read()
+        bad = got_c != want_c
+        assert not bad.any(), f"launch {r} (graph={graph}): colour differs at {int(bad.sum())} pixels"
+        assert (ids == want_ids).all() and (d.view(np.uint32) == want_d.view(np.uint32)).all(), \
+            f"launch {r}: vis-buffer differs"
+    b.close()
+    ref.close()
+    fb.close()
+    ds.close()
