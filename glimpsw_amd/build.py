@@ -23,8 +23,9 @@ ARCH = os.environ.get("GLIMP_ARCH", "gfx950")
 
 
 def gen_tables() -> str:
-    """Writes _gen/approx14_pwl.inc from data/approx14_pwl.txt after checking
-    that the pairs reproduce every entry of data/approx14.bin."""
+    """Writes _gen/approx14_pwl.inc (per run A and -B, as u32) from
+    data/approx14_pwl.txt after checking that the pairs reproduce every entry
+    of data/approx14.bin."""
     import numpy as np
 
     os.makedirs(GEN, exist_ok=True)
@@ -41,7 +42,8 @@ def gen_tables() -> str:
         rsq = (pairs[run, 0] - pairs[run, 1] * (idx & 1023)) >> 16
         assert (rsq == table[65536 + odd * 32768:65536 + (odd + 1) * 32768]).all(), \
             "approx14_pwl.txt does not reproduce the rsqrt table"
-    text = "\n".join(f"{int(a)}u, {int(b)}u," for a, b in pairs) + "\n"
+    # (A, -B): the device evaluates A + (-B) * j with one 24-bit multiply-add
+    text = "\n".join(f"{int(a)}u, {int((-b) & 0xFFFFFFFF)}u," for a, b in pairs) + "\n"
     if not os.path.exists(out) or open(out).read() != text:
         with open(out, "w") as f:
             f.write(text)

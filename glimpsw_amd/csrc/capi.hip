@@ -17,7 +17,7 @@
 
 #include <rocprofiler-sdk-roctx/roctx.h>
 
-// 128 (A, B) pairs of the piecewise-linear VRCP14 / VRSQRT14 mantissa
+// 128 (A, -B) pairs of the piecewise-linear VRCP14 / VRSQRT14 mantissa
 // tables (device_math.h), generated at build time from
 // glimpsw_amd/data/approx14_pwl.txt and checked there against approx14.bin.
 static const uint32_t kApproxPwl[] = {

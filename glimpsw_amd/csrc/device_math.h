@@ -24,22 +24,26 @@
 namespace glimp {
 
 struct ApproxTables {
-    const uint2* pwl;         // [128]: 64 rcp runs, then 32 + 32 rsqrt runs (exponent parity 0, 1)
+    // [128] (A, -B) runs: 64 rcp runs, then 32 + 32 rsqrt runs (exponent parity 0, 1)
+    const uint2* pwl;
 };
+
+// Entry j of a run: (A - B * j) >> shift, with -B stored. |B| < 2^17 and
+// j < 2^10: a full-rate 24-bit multiply-add is exact (and wraps like the
+// 32-bit table arithmetic).
+__device__ __forceinline__ uint32_t pwl_entry(uint2 ab, uint32_t j, uint32_t shift) {
+    return ((uint32_t)__mul24((int32_t)ab.y, (int32_t)j) + ab.x) >> shift;
+}
 
 // 16-bit result mantissa of VRCP14PS for the top 16 input mantissa bits.
 __device__ __forceinline__ uint32_t rcp14_mant(const ApproxTables& T, uint32_t idx16) {
-    const uint2 ab = T.pwl[idx16 >> 10];
-    // B < 2^14 and j < 2^10: a full-rate 24-bit multiply is exact
-    return (uint32_t)((int32_t)ab.x - (int32_t)__umul24(ab.y, idx16 & 1023u)) >> 12;
+    return pwl_entry(T.pwl[idx16 >> 10], idx16 & 1023u, 12);
 }
 
 // 16-bit result mantissa of VRSQRT14PS for exponent parity `odd` and the
 // top 15 input mantissa bits.
 __device__ __forceinline__ uint32_t rsqrt14_mant(const ApproxTables& T, uint32_t odd, uint32_t idx15) {
-    const uint2 ab = T.pwl[64 + odd * 32 + (idx15 >> 10)];
-    // B < 2^17 and j < 2^10: a full-rate 24-bit multiply is exact
-    return (ab.x - __umul24(ab.y, idx15 & 1023u)) >> 16;
+    return pwl_entry(T.pwl[64 + odd * 32 + (idx15 >> 10)], idx15 & 1023u, 16);
 }
 
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
@@ -78,17 +82,19 @@ __device__ __forceinline__ float rcp14_special(const ApproxTables& T, float xf) 
     return bitsf(s | ((uint32_t)re << 23) | mant);
 }
 
-// VRCP14PS. Inputs with a biased exponent in [1, 252] have a normal result
-// (exponent 253 - e, or 254 - e for an exact power of two); everything else
-// takes the general path.
+// VRCP14PS. Inputs with a biased exponent e in [1, 252] have a normal
+// result: exponent 253 - e and the table mantissa, or 254 - e and mantissa 0
+// for an exact power of two (a mantissa of 2^16 carries into the
+// exponent); the bits are (253 << 23) - (x's exponent bits) + (mantissa <<
+// 7), then x's sign. Everything else takes the general path.
 __device__ __forceinline__ float rcp14(const ApproxTables& T, float xf) {
     const uint32_t x = fbits(xf);
-    const uint32_t e = (x >> 23) & 255u;
-    if (__builtin_expect(e - 1u >= 252u, 0)) return rcp14_special(T, xf);
-    const uint32_t m = x & 0x7FFFFFu;
-    const uint32_t mant = m ? rcp14_mant(T, m >> 7) << 7 : 0u;
-    const uint32_t re = (m ? 253u : 254u) - e;
-    return bitsf((x & 0x80000000u) | (re << 23) | mant);
+    const uint32_t ex = x & 0x7F800000u;
+    if (__builtin_expect(ex - 0x00800000u >= (252u << 23), 0)) return rcp14_special(T, xf);
+    uint32_t mant = rcp14_mant(T, (x >> 7) & 0xFFFFu);
+    mant = (x & 0x7FFFFFu) ? mant : 0x10000u;
+    const uint32_t r = (0x7E800000u - ex) + (mant << 7);
+    return bitsf((r & 0x7FFFFFFFu) | (x & 0x80000000u));
 }
 
 // Full VRSQRT14PS semantics; rsqrt14 only calls it off the fast path.
@@ -116,17 +122,17 @@ __device__ __forceinline__ float rsqrt14_special(const ApproxTables& T, float xf
     return bitsf(((uint32_t)(126 - half) << 23) | (rsqrt14_mant(T, (uint32_t)oc, m >> 8) << 7));
 }
 
-// VRSQRT14PS. Positive normal inputs take the fast path.
+// VRSQRT14PS. Positive normal inputs take the fast path: for x = 2^u * 1.m
+// the result exponent is 126 - floor(u / 2) = 190 - (e + 1) / 2 (e the biased
+// exponent), the table is chosen by u's parity (bit 23 of x, inverted) and the
+// top mantissa bits, and an even power of two (m = 0, u even) gives mantissa
+// 2^16, which carries into the exponent (127 - u / 2).
 __device__ __forceinline__ float rsqrt14(const ApproxTables& T, float xf) {
     const uint32_t x = fbits(xf);
-    const uint32_t e = x >> 23;                     // sign bit set -> >= 256
-    if (__builtin_expect(e - 1u >= 254u, 0)) return rsqrt14_special(T, xf);
-    const uint32_t m = x & 0x7FFFFFu;
-    const int ue = (int)e - 127;
-    const uint32_t oc = (uint32_t)ue & 1u;
-    const int half = (ue - (int)oc) >> 1;           // exact: ue - oc is even
-    if (m == 0 && oc == 0) return bitsf((uint32_t)(127 - half) << 23);
-    return bitsf(((uint32_t)(126 - half) << 23) | (rsqrt14_mant(T, oc, m >> 8) << 7));
+    if (__builtin_expect(x - 0x00800000u >= (254u << 23), 0)) return rsqrt14_special(T, xf);
+    uint32_t mant = pwl_entry(T.pwl[64 + (((x >> 18) & 63u) ^ 32u)], (x >> 8) & 1023u, 16);
+    mant = (x & 0x00FFFFFFu) == 0x00800000u ? 0x10000u : mant;
+    return bitsf((0x5F000000u - (((x + 0x00800000u) >> 1) & 0x7F800000u)) + (mant << 7));
 }
 
 // Vector a/b as the reference's fast-math build computes it:

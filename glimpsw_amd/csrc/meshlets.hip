@@ -31,8 +31,8 @@
 
 namespace {
 
-// 128 (A, B) pairs of the piecewise-linear VRCP14 mantissa table, as the
-// device uses them (device_math.h).
+// 128 (A, -B) pairs of the piecewise-linear VRCP14 mantissa table, as the
+// device uses them (device_math.h): entry j of a run is (A - B * j) >> 12.
 const uint32_t kPairs[] = {
 #include "approx14_pwl.inc"
 };
@@ -65,8 +65,8 @@ float rcp14(float xf) {
     } else {
         re = 253 - e;
         const uint32_t idx = m >> 7;
-        const uint32_t a = kPairs[2 * (idx >> 10)], b = kPairs[2 * (idx >> 10) + 1];
-        mant = (uint32_t)(((int64_t)a - (int64_t)b * (idx & 1023)) >> 12) << 7;
+        const uint32_t a = kPairs[2 * (idx >> 10)], nb = kPairs[2 * (idx >> 10) + 1];
+        mant = ((a + nb * (idx & 1023)) >> 12) << 7;    // u32 arithmetic wraps as the device's
     }
     if (re >= 255) return bits_to_float(s | 0x7F800000u);
     if (re <= 0) {

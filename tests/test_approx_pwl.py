@@ -19,6 +19,11 @@ def test_pairs_reproduce_tables():
         run = 64 + odd * 32 + (idx >> 10)
         got = (pairs[run, 0] - pairs[run, 1] * (idx & 1023)) >> 16
         assert (got == table[65536 + odd * 32768:65536 + (odd + 1) * 32768]).all()
-    # device arithmetic ranges: rcp in int32, rsqrt in uint32
-    assert pairs[:64, 0].max() < 2 ** 31 and (pairs[:64, 1] * 1023).max() < 2 ** 31
-    assert pairs[64:, 0].max() < 2 ** 32 and (pairs[64:, 0] - pairs[64:, 1] * 1023).min() >= 0
+    # the device's arithmetic (device_math.h: pwl_entry): -B as a signed
+    # 24-bit operand times j, added to A in 32 bits, shifted logically
+    assert (pairs[:, 1] < 2 ** 23).all() and (pairs[:, 0] < 2 ** 32).all()
+    for lo, hi, n, shift, base in ((0, 64, 65536, 12, 0), (64, 96, 32768, 16, 65536), (96, 128, 32768, 16, 98304)):
+        idx = np.arange(n)
+        run = lo + (idx >> 10)
+        dev = ((-pairs[run, 1]) * (idx & 1023) + pairs[run, 0]) & 0xFFFFFFFF
+        assert ((dev >> shift) == table[base:base + n]).all()
