@@ -454,8 +454,8 @@ static_assert(sizeof(ClipRef) == sizeof(TriRecord), "a clip reference takes one 
 
 // `Alpha`: the launch has alpha-tested materials (p.alpha is set).
 template <bool Alpha>
-__device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, float (&lds)[4][64],
-                              const MeshletFetch& f) {
+__device__ __forceinline__ void setup_meshlet(const DrawParams& p, const ApproxTables& AT, uint32_t i, uint32_t lane,
+                                              float (&lds)[4][64], const MeshletFetch& f) {
     const uint32_t g = f.g, v = f.v;           // global meshlet number, view
     const ViewParams& V = view_of(p, v);
     const uint32_t w = g - V.meshlet_base;     // meshlet index within the view
@@ -523,7 +523,7 @@ __device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, fl
         float wt[4];
         bool keep = false;
         if (accept) {
-            keep = setup_triangle(p.tables, vv, hw, hh, cull_mode, rec, wt);
+            keep = setup_triangle(AT, vv, hw, hh, cull_mode, rec, wt);
             rec.rank = make_rank(seq, packet, 0, plane, 0);
             rec.flags = (alpha ? kRecAlpha : 0u) | flags_hi;
         }
@@ -607,6 +607,12 @@ __device__ void setup_meshlet(const DrawParams& p, uint32_t i, uint32_t lane, fl
 template <bool Alpha>
 __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
     __shared__ float lds_pos[4][4][64];
+    // the VRCP14 / VRSQRT14 runs in LDS: triangle setup's reciprocals wait
+    // on an LDS read instead of a cached global one
+    __shared__ uint2 s_pwl[128];
+    if (threadIdx.x < 128) s_pwl[threadIdx.x] = p.tables.pwl[threadIdx.x];
+    __syncthreads();
+    const ApproxTables lds_tables{ s_pwl };
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
     const uint32_t nvis = p.scratch->num_visible;
@@ -618,7 +624,7 @@ __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
         const uint32_t next = i + waves;
         MeshletFetch nxt;
         if (next < nvis) nxt = fetch_meshlet(p, next, lane);
-        setup_meshlet<Alpha>(p, i, lane, lds_pos[wid], cur);
+        setup_meshlet<Alpha>(p, lds_tables, i, lane, lds_pos[wid], cur);
         if (next >= nvis) break;
         cur = nxt;
         i = next;
