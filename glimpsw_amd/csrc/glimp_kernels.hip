@@ -398,6 +398,15 @@ struct MeshletFetch {
     uint32_t idx[2];               // this lane's triangle of each round: v0 | v1 << 8 | v2 << 16
 };
 
+// Device-memory pointers reached through a view (whose table may be a
+// kernel argument or device memory, so the compiler cannot tell): loads
+// through them as global loads, not flat ones (a flat load also counts
+// against the LDS wait counter, so every LDS wait would wait for it too).
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* as_global(const T* ptr) {
+    return (const __attribute__((address_space(1))) T*)ptr;
+}
+
 __device__ __forceinline__ MeshletFetch fetch_meshlet(const DrawParams& p, uint32_t i, uint32_t lane) {
     MeshletFetch f;
     const uint2 gv = p.visible_list[i];
@@ -405,13 +414,13 @@ __device__ __forceinline__ MeshletFetch fetch_meshlet(const DrawParams& p, uint3
     f.v = gv.y;
     const ViewParams& V = view_of(p, f.v);
     f.mi = V.meshlet_offset + (f.g - V.meshlet_base);
-    f.nt = V.m.num_triangles[f.mi];
-    f.mat = V.m.material_id[f.mi];
-    const float* P = V.m.positions + (size_t)f.mi * 192;
+    f.nt = as_global(V.m.num_triangles)[f.mi];
+    f.mat = as_global(V.m.material_id)[f.mi];
+    const auto P = as_global(V.m.positions + (size_t)f.mi * 192);
     f.px = P[lane];
     f.py = P[64 + lane];
     f.pz = P[128 + lane];
-    const uint8_t* I = V.m.indices + (size_t)f.mi * 384;
+    const auto I = as_global(V.m.indices + (size_t)f.mi * 384);
 #pragma unroll
     for (int r = 0; r < 2; r++)
         f.idx[r] = (uint32_t)I[r * 64 + lane] | ((uint32_t)I[128 + r * 64 + lane] << 8) |
