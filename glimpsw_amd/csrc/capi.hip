@@ -1272,8 +1272,13 @@ int glimp_batch_autotune(glimp_batch* b, void* stream, glimp_batch_shape* chosen
     if (!b) return fail(GLIMP_E_INVALID, "glimp_batch_autotune: NULL batch");
     HIP_TRY(hipSetDevice(b->ctx->device));
     hipStream_t st = (hipStream_t)stream;
-    const uint32_t sizes[] = { 8, 16, 32, 64 };
-    const uint32_t caps[] = { 256, 512, 1024, 2048, 4096 };
+    // bin sizes in the order they usually win; capacities from the largest
+    // down, so the first launch of a bin size tells the largest bin's record
+    // count: smaller capacities that would overflow are not launched, and
+    // when even the largest overflows only it is tried (the overflowing
+    // records are drawn by the overflow raster, at a measured cost)
+    const uint32_t sizes[] = { 16, 32, 8, 64 };
+    const uint32_t caps[] = { 4096, 2048, 1024, 512, 256 };
     const uint32_t threads[] = { 128, 256, 512, 1024 };
     const size_t kListBudget = (size_t)2 << 30;   // bytes of bin lists a candidate may use
     hipEvent_t e0, e1;
@@ -1297,7 +1302,11 @@ int glimp_batch_autotune(glimp_batch* b, void* stream, glimp_batch_shape* chosen
         return GLIMP_OK;
     };
     for (uint32_t bs : sizes) {
+        uint32_t maxCount = 0;            // the largest bin at this bin size, once known
+        bool firstOfSize = true;
+        bool skipSize = false;
         for (uint32_t cap : caps) {
+            if (skipSize || (maxCount && cap < maxCount && cap < caps[0])) continue;
             for (uint32_t th : threads) {
                 const uint64_t bins = count_bins(b->views, bs);
                 if (bins >= kMaxBins || bins * cap * 4 > kListBudget) continue;
@@ -1305,10 +1314,10 @@ int glimp_batch_autotune(glimp_batch* b, void* stream, glimp_batch_shape* chosen
                 if (!valid_shape(sh)) continue;
                 rc = glimp_batch_set_shape(b, &sh);
                 if (rc != GLIMP_OK) break;
-                // One probe launch: a shape whose bins overflow (their pairs
-                // take the slower overflow raster) or that drops anything is
-                // rejected, and one slower than twice the best so far is
-                // abandoned before the timed launches.
+                // One probe launch: a shape that drops anything is rejected,
+                // one slower than twice the best so far is abandoned before
+                // the timed launches, and a bin size whose first launch takes
+                // more than four times the best is given up.
                 Counters before;
                 HIP_TRY(hipMemcpy(&before, b->ctx->counters, sizeof(before), hipMemcpyDeviceToHost));
                 float probe = 0.0f;
@@ -1318,7 +1327,14 @@ int glimp_batch_autotune(glimp_batch* b, void* stream, glimp_batch_shape* chosen
                 DrawScratch ds;
                 HIP_TRY(hipMemcpy(&after, b->ctx->counters, sizeof(after), hipMemcpyDeviceToHost));
                 HIP_TRY(hipMemcpy(&ds, b->scratch.scratch, sizeof(ds), hipMemcpyDeviceToHost));
-                if (ds.num_spill != 0 || after.dropped != before.dropped) continue;
+                maxCount = max(maxCount, ds.max_bin_count);
+                const bool slowSize = firstOfSize && found && probe > 4.0f * bestMs;
+                firstOfSize = false;
+                if (slowSize) {
+                    skipSize = true;
+                    break;
+                }
+                if (after.dropped != before.dropped) continue;
                 if (found && probe > 2.0f * bestMs) continue;
                 float ms = 0.0f;
                 rc = timed(1, ms);                 // warm-up
@@ -1339,9 +1355,9 @@ int glimp_batch_autotune(glimp_batch* b, void* stream, glimp_batch_shape* chosen
     (void)hipEventDestroy(e1);
     if (rc != GLIMP_OK) return rc;
     if (!found) {
-        // every shape overflowed a bin or dropped something: keep the largest
-        // capacity at the batch's bin size, which overflows least
-        best.bin_capacity = caps[sizeof(caps) / sizeof(caps[0]) - 1];
+        // every shape dropped something: keep the largest capacity at the
+        // batch's bin size
+        best.bin_capacity = caps[0];
         if (!valid_shape(best)) best.raster_threads = 256;
     }
     rc = glimp_batch_set_shape(b, &best);

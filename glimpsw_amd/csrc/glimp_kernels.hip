@@ -278,6 +278,7 @@ __global__ void __launch_bounds__(256) k_cull(DrawParams p) {
         p.scratch->num_records = 0;
         p.scratch->num_spill = 0;
         p.scratch->num_clip_overflow = 0;
+        p.scratch->max_bin_count = 0;
     }
     // the block's meshlets are consecutive: one search for the first view,
     // then each thread walks forward from it
@@ -2047,7 +2048,7 @@ constexpr uint32_t kOverflowBlocks = 256;
 // overflow raster.
 template <bool Alpha>
 __global__ void __launch_bounds__(256) k_bin_order(DrawParams p) {
-    __shared__ uint32_t s_count[kBinBuckets], s_base[kBinBuckets];
+    __shared__ uint32_t s_count[kBinBuckets], s_base[kBinBuckets], s_max;
     const uint32_t tid = threadIdx.x;
     const uint32_t order_blocks = (p.total_bins + 255) / 256;
     if (blockIdx.x >= order_blocks) {
@@ -2058,15 +2059,23 @@ __global__ void __launch_bounds__(256) k_bin_order(DrawParams p) {
     }
     const uint32_t bin = blockIdx.x * 256 + tid;
     if (tid < kBinBuckets) s_count[tid] = 0;
+    if (tid == 0) s_max = 0;
     __syncthreads();
-    uint32_t cls = 0, rank = 0;
+    uint32_t cls = 0, rank = 0, offered = 0;
     const bool valid = bin < p.total_bins;
     if (valid) {
-        const uint32_t n = min(p.bin_counts[bin], p.bin_capacity);
+        offered = p.bin_counts[bin];
+        const uint32_t n = min(offered, p.bin_capacity);
         cls = n ? min(kBinBuckets - 1, 32u - (uint32_t)__clz(n)) : 0u;
         rank = atomicAdd(&s_count[cls], 1u);
     }
+    // the largest bin, for launch-shape choices: one device atomic per block,
+    // and none when the block's largest is not above the value already there
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) offered = max(offered, (uint32_t)__shfl_xor((int)offered, d));
+    if (lane_id() == 0 && offered) atomicMax(&s_max, offered);
     __syncthreads();
+    if (tid == 0 && s_max > p.scratch->max_bin_count) atomicMax(&p.scratch->max_bin_count, s_max);
     if (tid < kBinBuckets) s_base[tid] = s_count[tid] ? atomicAdd(&p.scratch->bucket[tid], s_count[tid]) : 0u;
     __syncthreads();
     if (valid) p.bin_order[(size_t)cls * p.total_bins + s_base[cls] + rank] = bin;
@@ -2111,6 +2120,7 @@ __global__ void k_reset_counters(DrawScratch* s) {
     s->num_visible = 0;
     s->num_spill = 0;
     s->num_clip_overflow = 0;
+    s->max_bin_count = 0;
     for (uint32_t c = 0; c < kBinBuckets; c++) s->bucket[c] = 0;
 }
 

@@ -135,7 +135,8 @@ struct DrawScratch {
     uint32_t pad0;
     uint32_t num_spill;            // records on the overflow list
     uint32_t num_clip_overflow;    // meshlets with triangles that found the clip queue full
-    uint32_t pad[2];
+    uint32_t max_bin_count;        // the largest bin's record count (pairs offered, kept or not)
+    uint32_t pad;
     uint32_t bucket[kBinBuckets];  // bins per record-count class (log2), for heavy-first raster order
 };
 

@@ -312,11 +312,16 @@ int glimp_batch_launch(glimp_batch* batch, int use_graph, void* stream);
 int glimp_batch_launch_per_view(glimp_batch* batch, void* stream);
 /* Autotuner: times the candidate shapes (bin 8-64, capacity 256-4096,
  * 128/256-thread bin raster, 512/1024 for 32/64-pixel bins) on this batch,
- * keeps the fastest and returns it with its time per launch. Each candidate
- * is probed with one launch first: shapes whose bins overflow or that drop
- * anything are rejected, and one slower than twice the best so far is not
- * timed further. If every shape is rejected the batch's bin size is kept with
- * the largest capacity. Synchronous. */
+ * keeps the fastest and returns it with its time per launch. Bin sizes go
+ * 16, 32, 8, 64 and capacities from the largest down; the first launch of a
+ * bin size gives its largest bin's record count, and smaller capacities
+ * (which would overflow) are not launched; when even the largest overflows,
+ * only it is tried (overflowing records take the overflow raster, whose cost
+ * the timing includes). Each candidate is probed with one launch first:
+ * shapes that drop anything are rejected, one slower than twice the best so
+ * far is not timed further, and a bin size whose first launch takes over
+ * four times the best is given up. If every shape is rejected the batch's
+ * bin size is kept with the largest capacity. Synchronous. */
 int glimp_batch_autotune(glimp_batch* batch, void* stream, glimp_batch_shape* chosen, float* best_ms);
 
 /* ------------------------------------------------------------ differentiable resolve */
