@@ -185,8 +185,14 @@ __device__ __forceinline__ uint32_t pixel_offset(uint32_t W, uint32_t x, uint32_
 
 // ---------------------------------------------------------------- views
 
+// The view table is constant while a launch runs: reading it through the
+// constant address space lets a wave-uniform view index become scalar loads
+// (waited on apart from the vector loads in flight, which a prefetch would
+// otherwise have to finish first).
 __device__ __forceinline__ const ViewParams& view_of(const DrawParams& p, uint32_t v) {
-    return p.views ? p.views[v] : p.view;
+    if (!p.views) return p.view;
+    const __attribute__((address_space(4))) ViewParams* t = (const __attribute__((address_space(4))) ViewParams*)p.views;
+    return *(const ViewParams*)(t + v);
 }
 
 // View owning global meshlet slot g (views are ordered by meshlet_base).
@@ -411,8 +417,9 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* as_global(
 __device__ __forceinline__ MeshletFetch fetch_meshlet(const DrawParams& p, uint32_t i, uint32_t lane) {
     MeshletFetch f;
     const uint2 gv = p.visible_list[i];
-    f.g = gv.x;
-    f.v = gv.y;
+    // one meshlet per wave: its number and view are wave-uniform (scalar)
+    f.g = (uint32_t)__builtin_amdgcn_readfirstlane((int)gv.x);
+    f.v = (uint32_t)__builtin_amdgcn_readfirstlane((int)gv.y);
     const ViewParams& V = view_of(p, f.v);
     f.mi = V.meshlet_offset + (f.g - V.meshlet_base);
     f.nt = as_global(V.m.num_triangles)[f.mi];
