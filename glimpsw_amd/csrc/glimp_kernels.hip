@@ -72,6 +72,37 @@ __device__ __forceinline__ uint32_t mbcnt(unsigned long long mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// Inclusive scan across the wave with DPP moves (no LDS round trips):
+// row_shr 1, 2, 4, 8 within each row of 16 lanes, then row_bcast:15 and
+// row_bcast:31 carry the row totals into the rows above. Lanes a move has no
+// source for, and rows its row mask leaves out, take `identity`.
+template <class Op>
+__device__ __forceinline__ uint32_t wave_scan(uint32_t v, uint32_t identity, Op op) {
+    const int id = (int)identity;
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x111, 0xF, 0xF, false));   // row_shr:1
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x112, 0xF, 0xF, false));   // row_shr:2
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x114, 0xF, 0xF, false));   // row_shr:4
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x118, 0xF, 0xF, false));   // row_shr:8
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x142, 0xA, 0xF, false));   // row_bcast:15
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x143, 0xC, 0xF, false));   // row_bcast:31
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_scan_add(uint32_t v) {
+    return wave_scan(v, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+}
+
+// Wave-uniform minimum / maximum (lane 63 of the inclusive scan).
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+    v = wave_scan(v, 0xFFFFFFFFu, [](uint32_t a, uint32_t b) { return min(a, b); });
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+    v = wave_scan(v, 0u, [](uint32_t a, uint32_t b) { return max(a, b); });
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 // Render bounding box of a snapped triangle (GetBoundingBox and
 // GetRenderBoundingBox, Rasterizer.cpp:331-351), including the carry of the
 // 32-bit add across the packed halves. Returns min/max as packed x | y << 16.
@@ -673,13 +704,15 @@ __device__ void raster_now(const DrawParams& p, const TriRecord& rec, const Alph
 // Clips one triangle with the whole wave. With `Store`, lanes whose fan
 // triangle survives setup return true with the record and its index, the
 // record stored after the meshlet slots (the region holds 8 records per
-// queue entry, as many as one triangle can yield, so it cannot fill up).
+// queue entry, as many as one triangle can yield, so it cannot fill up):
+// packed through a shared counter, or, given `slots`, at slots + fan index
+// (the entry's own 8 slots, no counter).
 // Without `Store` (triangles that found the queue full) the surviving fan
 // triangles are drawn at once by the wave (raster_now).
 // With `Alpha` the barycentric attributes travel with the positions.
 template <bool Alpha, bool Store>
 __device__ bool clip_one(const DrawParams& p, const ClipSrc& ce, uint32_t lane, TriRecord& out,
-                         uint32_t& out_index) {
+                         uint32_t& out_index, uint32_t slots = 0xFFFFFFFFu) {
     constexpr int NC = Alpha ? 6 : 4;   // clipped channels
     const ViewParams& VP = view_of(p, ce.view);
     const int hw = VP.width / 2, hh = VP.height / 2;
@@ -717,13 +750,8 @@ __device__ bool clip_one(const DrawParams& p, const ClipSrc& ce, uint32_t lane, 
         const bool make = cross && (generated + xrank < 16);
         const uint32_t made = min((uint32_t)__popcll(xm), 16u - generated);
         const uint32_t cnt = (in ? 1u : 0u) + (make ? 1u : 0u);
-        uint32_t incl = cnt;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            uint32_t o = __shfl_up(incl, d);
-            if ((int)lane >= d) incl += o;
-        }
-        const uint32_t outCount = __shfl(incl, 63);
+        const uint32_t incl = wave_scan_add(cnt);
+        const uint32_t outCount = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
         const uint32_t pos = incl - cnt;
         float I[NC] = {};
         if (make) {
@@ -734,7 +762,7 @@ __device__ bool clip_one(const DrawParams& p, const ClipSrc& ce, uint32_t lane, 
         if (vc < 3) break;
         // output lane j pulls from source lane src(j) = #{i : incl_i <= j}
         uint32_t src = 0;
-        for (uint32_t k = 0; k < vc; k++) src += (__shfl(incl, (int)k) <= lane) ? 1u : 0u;
+        for (uint32_t k = 0; k < vc; k++) src += ((uint32_t)__builtin_amdgcn_readlane((int)incl, k) <= lane) ? 1u : 0u;
         src = min(src, 63u);
         const uint32_t srcPos = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)pos);
         const int srcIn = __builtin_amdgcn_ds_bpermute((int)(src << 2), in ? 1 : 0);
@@ -797,10 +825,12 @@ __device__ bool clip_one(const DrawParams& p, const ClipSrc& ce, uint32_t lane, 
         a.rv[0] = v0; a.rv[1] = rv1; a.rv[2] = rv2;
     }
     if constexpr (Store) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&p.scratch->num_records, (uint32_t)__popcll(km));
-        base = __shfl(base, 0);
-        const uint32_t slot = base + mbcnt(km);
+        uint32_t slot = slots + f;
+        if (slots == 0xFFFFFFFFu) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&p.scratch->num_records, (uint32_t)__popcll(km));
+            slot = (uint32_t)__builtin_amdgcn_readlane((int)base, 0) + mbcnt(km);
+        }
         if (!keep) return false;
         if (slot >= p.clip_record_capacity) {     // not reached: see above
             atomicAdd(&p.counters->dropped, 1ull);
@@ -1541,8 +1571,18 @@ __device__ __forceinline__ uint32_t wave_sort_u32(uint32_t v, uint32_t lane) {
 // each wave expands a chunk of records into their (record, bin) pairs with
 // an exclusive scan and inserts the pairs in parallel, so a triangle
 // spanning many bins costs one pass of the wave, not a serial loop.
+// Bins of the rectangle a meshlet's records cover, up to this many: their
+// pair counts, then their next free slots, live in LDS.
+constexpr uint32_t kRectBins = 1024;
+
 struct BinScratch {                     // one wave's slice of the block's LDS
-    uint32_t start[64], org[64], bw[64], ridx[64], spill[64];
+    // per record: first pair, first bin, bin-box width | bins per row << 16,
+    // record index, overflow flag (64 records a call in bin_records, 128 in
+    // bin_meshlet)
+    uint32_t start[128], org[128], bw[128], ridx[128], spill[128];
+    uint32_t rect[128];                 // rect bin of the first bin | box width << 16
+    uint32_t marks[64];                 // record starting at each pair of a round, + 1
+    uint32_t slots[kRectBins];          // per rect bin: pairs, then next slot
 };
 
 // Inserts the wave's records (one per lane where `valid`) into the screen
@@ -1639,6 +1679,216 @@ __device__ void bin_records(const DrawParams& p, bool valid, const TriRecord& re
     }
 }
 
+// Per-half minimum of two packed u16 values.
+__device__ __forceinline__ uint32_t min_u16x2(uint32_t a, uint32_t b) {
+    return min(a & 0xFFFFu, b & 0xFFFFu) | (min(a >> 16, b >> 16) << 16);
+}
+
+// Bins a lane walks for its own records; a wave with a larger record box
+// walks its pairs in parallel instead.
+constexpr uint32_t kLanePairs = 16;
+
+// Inserts up to 128 records of one view (record k = lane + 64 h in slot h of
+// each lane) into the screen bins with one device atomic per bin they
+// reach. The records' pairs are counted per bin of the rectangle their bin
+// boxes span, in LDS; each bin with pairs takes its run of slots with one
+// atomicAdd; a second walk over the pairs writes them to those slots. Small
+// triangles of a meshlet share few bins, and a bin counter that every
+// overlapping wave increments once per pair run makes each later atomic on
+// it wait for the earlier ones. Rectangles of more than kRectBins bins go
+// through bin_records, 64 records at a time.
+__device__ void bin_meshlet(const DrawParams& p, const bool (&valid)[2], const TriRecord (&rec)[2],
+                            const uint32_t (&ridx)[2], uint32_t lane, BinScratch& sm,
+                            unsigned long long& entries) {
+    const unsigned long long vm0 = __ballot(valid[0]), vm1 = __ballot(valid[1]);
+    if (!(vm0 | vm1)) return;
+    const int src = __ffsll((long long)(vm0 ? vm0 : vm1)) - 1;
+    const uint32_t flags = (uint32_t)__builtin_amdgcn_readlane((int)(vm0 ? rec[0].flags : rec[1].flags), src);
+    const ViewParams& V = view_of(p, flags >> 16);
+    // bin boxes, x | y << 16; the rectangle's corners (the high corner as the
+    // minimum of the complement) and the largest box, by wave reductions
+    uint32_t lo[2], hi[2], nb[2], rlo = 0xFFFFFFFFu, rhi = 0xFFFFFFFFu, big = 0;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        lo[h] = hi[h] = nb[h] = 0;
+        if (valid[h]) {
+            uint32_t bmin, bmax;
+            render_bbox(rec[h].pos[0], rec[h].pos[1], rec[h].pos[2], V.width / 2, V.height / 2, bmin, bmax);
+            lo[h] = ((uint32_t)lo16(bmin) >> p.bin_shift) | (((uint32_t)hi16(bmin) >> p.bin_shift) << 16);
+            hi[h] = ((uint32_t)(lo16(bmax) - 1) >> p.bin_shift) | (((uint32_t)(hi16(bmax) - 1) >> p.bin_shift) << 16);
+            nb[h] = ((hi[h] & 0xFFFFu) - (lo[h] & 0xFFFFu) + 1) * ((hi[h] >> 16) - (lo[h] >> 16) + 1);
+            rlo = min_u16x2(rlo, lo[h]);
+            rhi = min_u16x2(rhi, ~hi[h]);
+            big = max(big, nb[h]);
+        }
+    }
+    rlo = (uint32_t)__builtin_amdgcn_readlane((int)wave_scan(rlo, 0xFFFFFFFFu, min_u16x2), 63);
+    rhi = ~(uint32_t)__builtin_amdgcn_readlane((int)wave_scan(rhi, 0xFFFFFFFFu, min_u16x2), 63);
+    // with every record's box at most kLanePairs bins each lane walks its
+    // own records' pairs (`steps` steps); otherwise the wave walks all pairs
+    // in parallel, 64 at a time
+    const uint32_t maxnb = wave_max(big);
+    const bool wide = maxnb > kLanePairs;
+    const uint32_t steps = wide ? 0u : maxnb;
+    const uint32_t rx0 = rlo & 0xFFFFu, ry0 = rlo >> 16;
+    const uint32_t rw = (rhi & 0xFFFFu) - rx0 + 1, rh = (rhi >> 16) - ry0 + 1;
+    if (rw * rh > kRectBins) {
+        if (vm0) bin_records(p, valid[0], rec[0], ridx[0], lane, sm, entries);
+        if (vm1) bin_records(p, valid[1], rec[1], ridx[1], lane, sm, entries);
+        return;
+    }
+    const uint32_t R = rw * rh;
+    if (counting(p)) entries += (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_add(nb[0] + nb[1]), 63);
+    uint32_t bw[2], org[2], rect[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const uint32_t bx0 = lo[h] & 0xFFFFu, by0 = lo[h] >> 16;
+        bw[h] = valid[h] ? (hi[h] & 0xFFFFu) - bx0 + 1 : 0u;
+        org[h] = V.bin_base + by0 * V.bins_x + bx0;                  // first bin, global numbering
+        rect[h] = (by0 - ry0) * rw + (bx0 - rx0);                    // first bin, rectangle numbering
+    }
+    __builtin_amdgcn_wave_barrier();   // earlier reads of `sm` are done
+    for (uint32_t r = lane; r < R; r += 64) sm.slots[r] = 0;
+    uint32_t total = 0, st[2] = { 0, 0 };
+    if (wide) {
+        // the pairs in record order: record k holds pairs
+        // [start[k], start[k] + nb[k])
+        const uint32_t i0 = wave_scan_add(nb[0]), i1 = wave_scan_add(nb[1]);
+        const uint32_t total0 = (uint32_t)__builtin_amdgcn_readlane((int)i0, 63);
+        total = total0 + (uint32_t)__builtin_amdgcn_readlane((int)i1, 63);
+        st[0] = i0 - nb[0];
+        st[1] = total0 + i1 - nb[1];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            sm.start[h * 64 + lane] = st[h];
+            sm.org[h * 64 + lane] = org[h];
+            sm.rect[h * 64 + lane] = rect[h] | (bw[h] << 16);
+            sm.ridx[h * 64 + lane] = ridx[h];
+            sm.spill[h * 64 + lane] = 0;
+        }
+    }
+    // Record owning pair j0 + lane of the round of 64 pairs from j0: each
+    // record starting in the round marks its first pair, a max-scan carries
+    // the latest mark to the pairs after it, and pairs before the round's
+    // first mark belong to the record current at the previous round's end.
+    auto owner = [&](uint32_t j0, uint32_t& carry) -> uint32_t {
+        __builtin_amdgcn_wave_barrier();
+        sm.marks[lane] = 0;
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+            if (nb[h] && st[h] - j0 < 64u) sm.marks[st[h] - j0] = h * 64 + lane + 1;
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t m = wave_scan(sm.marks[lane], 0u, [](uint32_t a, uint32_t b) { return max(a, b); });
+        const uint32_t own = m ? m : carry;
+        carry = (uint32_t)__builtin_amdgcn_readlane((int)own, 63);
+        return own - 1;
+    };
+    __builtin_amdgcn_wave_barrier();
+    // pass 1: pairs per rectangle bin
+    {
+        uint32_t r[2] = { rect[0], rect[1] }, dx[2] = { 0, 0 };
+        for (uint32_t t = 0; t < steps; t++) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                if (t < nb[h]) {
+                    atomicAdd(&sm.slots[r[h]], 1u);
+                    r[h]++;
+                    if (++dx[h] == bw[h]) { dx[h] = 0; r[h] += rw - bw[h]; }
+                }
+            }
+        }
+    }
+    if (wide) {
+        uint32_t carry = 0;
+        for (uint32_t j0 = 0; j0 < total; j0 += 64) {
+            const uint32_t k = owner(j0, carry), j = j0 + lane;
+            if (j < total) {
+                const uint32_t q = j - sm.start[k], rc = sm.rect[k], w = rc >> 16;
+                const uint32_t dy = q / w, dx = q - dy * w;
+                atomicAdd(&sm.slots[(rc & 0xFFFFu) + dy * rw + dx], 1u);
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // one device atomic per bin with pairs: the bin's first slot (four per
+    // lane issued before the first result is waited for)
+    for (uint32_t g0 = 0; g0 < R; g0 += 256) {
+        uint32_t first_slot[4];
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++) {
+            const uint32_t r = g0 + i * 64 + lane;
+            first_slot[i] = 0;
+            if (g0 + i * 64 < R) {
+                const uint32_t n = r < R ? sm.slots[r] : 0u;
+                if (n) {
+                    const uint32_t dy = r / rw, dx = r - dy * rw;
+                    first_slot[i] = atomicAdd(&p.bin_counts[V.bin_base + (ry0 + dy) * V.bins_x + rx0 + dx], n);
+                }
+            }
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++)
+            if (g0 + i * 64 + lane < R) sm.slots[g0 + i * 64 + lane] = first_slot[i];
+    }
+    __builtin_amdgcn_wave_barrier();
+    // pass 2: each pair takes the next slot of its bin
+    bool spill[2] = { false, false };
+    {
+        uint32_t r[2] = { rect[0], rect[1] }, bin[2] = { org[0], org[1] }, dx[2] = { 0, 0 };
+        for (uint32_t t = 0; t < steps; t++) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                if (t < nb[h]) {
+                    const uint32_t slot = atomicAdd(&sm.slots[r[h]], 1u);
+                    if (slot < p.bin_capacity) p.bin_lists[(size_t)bin[h] * p.bin_capacity + slot] = ridx[h];
+                    else spill[h] = true;
+                    r[h]++;
+                    bin[h]++;
+                    if (++dx[h] == bw[h]) {
+                        dx[h] = 0;
+                        r[h] += rw - bw[h];
+                        bin[h] += V.bins_x - bw[h];
+                    }
+                }
+            }
+        }
+    }
+    if (wide) {
+        uint32_t carry = 0;
+        for (uint32_t j0 = 0; j0 < total; j0 += 64) {
+            const uint32_t k = owner(j0, carry), j = j0 + lane;
+            if (j < total) {
+                const uint32_t q = j - sm.start[k], rc = sm.rect[k], w = rc >> 16;
+                const uint32_t dy = q / w, dx = q - dy * w;
+                const uint32_t slot = atomicAdd(&sm.slots[(rc & 0xFFFFu) + dy * rw + dx], 1u);
+                const uint32_t bin = sm.org[k] + dy * V.bins_x + dx;
+                if (slot < p.bin_capacity) p.bin_lists[(size_t)bin * p.bin_capacity + slot] = sm.ridx[k];
+                else sm.spill[k] = 1;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        spill[0] = sm.spill[lane] != 0;
+        spill[1] = sm.spill[64 + lane] != 0;
+    }
+    // A record with a pair past its bin's capacity goes to the overflow list
+    // once and is flagged, so the bins that did take it skip it.
+    const bool s0 = valid[0] && spill[0], s1 = valid[1] && spill[1];
+    const unsigned long long m0 = __ballot(s0), m1 = __ballot(s1);
+    if (m0 | m1) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&p.scratch->num_spill, (uint32_t)(__popcll(m0) + __popcll(m1)));
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
+        if (s0) {
+            p.spill[base + mbcnt(m0)] = ridx[0];
+            p.records[ridx[0]].flags = rec[0].flags | kRecSpilled;
+        }
+        if (s1) {
+            p.spill[base + (uint32_t)__popcll(m0) + mbcnt(m1)] = ridx[1];
+            p.records[ridx[1]].flags = rec[1].flags | kRecSpilled;
+        }
+    }
+}
+
 // Binning of the meshlet records, then clipping of the queued triangles
 // (one wave each; the fan triangles go straight into the bins from the
 // wave's registers).
@@ -1647,30 +1897,42 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
     __shared__ BinScratch s_bins[4];
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
-    const uint32_t nvis = p.scratch->num_visible;
     const uint32_t waves = gridDim.x * 4;
     const uint32_t first = blockIdx.x * 4 + wid;
     unsigned long long entries = 0;
-    // one visible meshlet's records at a time (up to two rounds of 64)
-    for (uint32_t i = first; i < nvis; i += waves) {
-        const uint32_t cnt = p.meshlet_counts[i];
-        for (uint32_t half = 0; half * 64 < cnt; half++) {
-            const uint32_t base = i * 128 + half * 64;
-            const bool valid = half * 64 + lane < cnt;
-            TriRecord rec;
-            if (valid) rec = p.records[base + lane];
-            bin_records(p, valid, rec, base + lane, lane, s_bins[wid], entries);
-        }
+    // The first meshlet's record count loads beside the visible count
+    // instead of after it: every meshlet has a count slot, and a wave past
+    // the visible count ignores what it read.
+    uint32_t i = first, cnt = 0;
+    if (i < p.total_meshlets) cnt = p.meshlet_counts[i];
+    const uint32_t nvis = p.scratch->num_visible;
+    const uint32_t nclip = min(p.scratch->num_clip, p.clip_capacity);
+    const uint32_t q0 = waves - 1 - first;
+    // one visible meshlet's records at a time, the next one's count in flight
+    while (i < nvis) {
+        const uint32_t next = i + waves;
+        TriRecord r0 = {}, r1 = {};
+        if (lane < cnt) r0 = p.records[(size_t)i * 128 + lane];
+        if (lane + 64 < cnt) r1 = p.records[(size_t)i * 128 + 64 + lane];
+        const uint32_t ncnt = next < nvis ? p.meshlet_counts[next] : 0u;
+        const bool valid[2] = { lane < cnt, lane + 64 < cnt };
+        const TriRecord rec[2] = { r0, r1 };
+        const uint32_t ridx[2] = { i * 128 + lane, i * 128 + 64 + lane };
+        bin_meshlet(p, valid, rec, ridx, lane, s_bins[wid], entries);
+        i = next;
+        cnt = ncnt;
     }
     // Queued triangles go to the last waves of the grid, which have no
     // meshlet when the grid holds more waves than visible meshlets: clipping
     // then runs beside the binning instead of after it.
-    const uint32_t nclip = min(p.scratch->num_clip, p.clip_capacity);
-    for (uint32_t q = waves - 1 - first; q < nclip; q += waves) {
+    for (uint32_t q = q0; q < nclip; q += waves) {
         TriRecord rec;
         uint32_t ridx = 0;
-        const bool valid = clip_one<Alpha, true>(p, clip_src(p.clip_queue[q]), lane, rec, ridx);
-        if (__ballot(valid)) bin_records(p, valid, rec, ridx, lane, s_bins[wid], entries);
+        const bool valid = clip_one<Alpha, true>(p, clip_src(p.clip_queue[q]), lane, rec, ridx, q * 8);
+        const bool v2[2] = { valid, false };
+        const TriRecord r2[2] = { rec, rec };
+        const uint32_t x2[2] = { ridx, 0 };
+        bin_meshlet(p, v2, r2, x2, lane, s_bins[wid], entries);
     }
     if (counting(p) && lane == 0 && entries) atomicAdd(&p.counters->bin_entries, entries);
 }
