@@ -998,9 +998,12 @@ static_assert(sizeof(BinTriStep) == 80, "BinTriStep is five 16-byte LDS reads");
 template <int BS>
 using BinTri = typename std::conditional<(BS <= 32), BinTriMask, BinTriStep>::type;
 
-// (record, tile) items of one round of a bin raster chunk: four per thread
+// (record, tile) items of one round of a bin raster chunk: four per thread,
+// two for blocks of up to 128 threads (whose residency the LDS limits: one
+// more block per CU outweighs the extra rounds; a 256-thread block with two
+// per thread measured 3 us slower on the 1080p frame)
 __host__ __device__ constexpr uint32_t bin_items(int BS, int kThreads) {
-    return BS <= 32 ? 4u * (uint32_t)kThreads : 1u;
+    return BS <= 32 ? (kThreads <= 128 ? 2u : 4u) * (uint32_t)kThreads : 1u;
 }
 
 __device__ __forceinline__ void load_tri(const TriRecord& rec, int hw, int hh, TriLds& t, uint32_t& bmin,
@@ -2165,13 +2168,10 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
                 }
                 __syncthreads();
                 const uint32_t nr = min(total - r0, kItems);
-                const uint32_t per = (nr + kGroups - 1) / kGroups;
-                uint32_t j = min(g * per, nr);
-                const uint32_t j1 = min(j + per, nr);
-                if (countFrags && j < j1) tiles += j1 - j;
                 uint32_t kcur = 0xFFFFFFFFu;
-                for (; j < j1; j++) {
-                    const uint32_t it = items[j];
+                // one tile of item `it`: the record's state is read when the
+                // record changes
+                auto draw_item = [&](uint32_t it) {
                     const uint32_t k = it >> 6, b = it & 63u;
                     if (k != kcur) {
                         kcur = k;
@@ -2184,6 +2184,29 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
                           (uint32_t)T.w1 + (uint32_t)__mul24(T.a20, mx) + (uint32_t)__mul24(T.b20, my),
                           (uint32_t)T.w2 + (uint32_t)__mul24(T.a01, mx) + (uint32_t)__mul24(T.b01, my),
                           T.li0 + (tx << 4) + ty * (4 * BS) + px + (py << 2));
+                };
+                if constexpr (kThreads >= 256) {
+                    // whole groups of four items per 16-lane group, one LDS
+                    // read each (not an item read before every tile);
+                    // measured slower with 128-thread blocks, whose fewer
+                    // groups lose more to the coarser split
+                    const uint32_t nq = (nr + 3) >> 2;
+                    uint32_t j = min(4 * ((g * nq) / kGroups), nr);
+                    const uint32_t j1 = min(4 * (((g + 1) * nq) / kGroups), nr);
+                    if (countFrags && j < j1) tiles += j1 - j;
+                    for (; j < j1; j += 4) {
+                        const uint4 q4 = *reinterpret_cast<const uint4*>(&items[j]);
+                        const uint32_t quad[4] = { q4.x, q4.y, q4.z, q4.w };
+#pragma unroll
+                        for (uint32_t u = 0; u < 4; u++)
+                            if (j + u < j1) draw_item(quad[u]);
+                    }
+                } else {
+                    const uint32_t per = (nr + kGroups - 1) / kGroups;
+                    uint32_t j = min(g * per, nr);
+                    const uint32_t j1 = min(j + per, nr);
+                    if (countFrags && j < j1) tiles += j1 - j;
+                    for (; j < j1; j++) draw_item(items[j]);
                 }
                 __syncthreads();
             }
@@ -2359,7 +2382,7 @@ template <int BS, int kThreads, bool Alpha>
 __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
     __shared__ unsigned long long keys[BS * BS];
     __shared__ BinTri<BS> tl[kThreads];
-    __shared__ uint32_t items[bin_items(BS, kThreads)];
+    __shared__ __attribute__((aligned(16))) uint32_t items[bin_items(BS, kThreads)];
     __shared__ uint32_t s_wave_total[kThreads / kWave];
     __shared__ uint32_t talpha[Alpha ? kThreads : 1];
     __shared__ uint32_t s_bin;
