@@ -29,6 +29,16 @@ __device__ __forceinline__ uint32_t lerp16(uint32_t a, uint32_t b, uint32_t t) {
     return r;
 }
 
+// One channel of SampleLevel's bilinear filter. The reference lerps bytes
+// held in 16-bit lanes, a + mulhrs(b - a, f << 7) (lerp16 above); with
+// |b - a| <= 255 and f < 256 that is a + floor((b - a) * f / 256 + 1/2)
+// (mulhrs rounds half up), and every term of it is exact in fp32: one
+// subtract, one FMA, a floor and an add per channel instead of extracting,
+// multiplying and repacking 16-bit halves.
+__device__ __forceinline__ float lerp_byte(float a, float b, float fs) {
+    return a + floorf(__builtin_fmaf(b - a, fs, 0.5f));
+}
+
 // Texture2D<RGBA8u>::SampleLevel with Repeat wrap (Texture.h:412-459,
 // 506-556); `linear` is the tile's filter choice.
 __device__ inline uint32_t sample_level(const DevTexture& tex, float u, float v, int32_t mipIn, bool linear,
@@ -48,16 +58,18 @@ __device__ inline uint32_t sample_level(const DevTexture& tex, float u, float v,
     bool inY = ((ty + 1) << m) < (int32_t)tex.height;
     int32_t i00 = offset + (int32_t)texel_offset(tx, ty, stride);
     int32_t i01 = offset + (int32_t)texel_offset(tx, ty + (inY ? 1 : 0), stride);
-    uint32_t d00 = D[i00], d10 = D[i00 + 8], d01 = D[i01], d11 = D[i01 + 8];
-    uint32_t fx = (uint32_t)(xf & 255) << 7, fy = (uint32_t)(yf & 255) << 7;
-    fx = (fx << 16) | fx;
-    fy = (fy << 16) | fy;
-    if (!inX) fx = 0;
-    uint32_t rb1 = lerp16(d00 & 0x00FF00FFu, d10 & 0x00FF00FFu, fx);
-    uint32_t ga1 = lerp16((d00 >> 8) & 0x00FF00FFu, (d10 >> 8) & 0x00FF00FFu, fx);
-    uint32_t rb2 = lerp16(d01 & 0x00FF00FFu, d11 & 0x00FF00FFu, fx);
-    uint32_t ga2 = lerp16((d01 >> 8) & 0x00FF00FFu, (d11 >> 8) & 0x00FF00FFu, fx);
-    return lerp16(rb1, rb2, fy) | (lerp16(ga1, ga2, fy) << 8);
+    const uint32_t d00 = D[i00], d10 = D[i00 + 8], d01 = D[i01], d11 = D[i01 + 8];
+    const float fxs = inX ? (float)(xf & 255) * (1.0f / 256) : 0.0f;
+    const float fys = (float)(yf & 255) * (1.0f / 256);
+    uint32_t out = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < 4; c++) {
+        const uint32_t sh = 8 * c;
+        const float r1 = lerp_byte((float)((d00 >> sh) & 255u), (float)((d10 >> sh) & 255u), fxs);
+        const float r2 = lerp_byte((float)((d01 >> sh) & 255u), (float)((d11 >> sh) & 255u), fxs);
+        out = __builtin_amdgcn_cvt_pk_u8_f32(lerp_byte(r1, r2, fys), c, out);
+    }
+    return out;
 }
 
 // SkyboxTex->SampleOctLevel<EnvSampler>(dir, 1) (Shading.cpp:677,
