@@ -348,12 +348,7 @@ __global__ void __launch_bounds__(256) k_cull(DrawParams p) {
         views[k] = v;
         mine += ok ? 1u : 0u;
     }
-    uint32_t incl = mine;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t o = __shfl_up(incl, d);
-        if ((int)lane >= d) incl += o;
-    }
+    const uint32_t incl = wave_scan_add(mine);
     if (lane == 63) s_wave[wid] = incl;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1209,13 +1204,8 @@ __global__ void __launch_bounds__(256) k_raster_direct(DrawParams p) {
             tview[wid][lane] = v;
             if constexpr (Alpha) talpha[wid][lane] = (rec.flags & kRecAlpha) ? r : 0xFFFFFFFFu;
         }
-        uint32_t incl = ntiles;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            uint32_t o = __shfl_up(incl, d);
-            if ((int)lane >= d) incl += o;
-        }
-        const uint32_t total = __shfl(incl, 63);
+        const uint32_t incl = wave_scan_add(ntiles);
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
         t.start = incl - ntiles;
         if (lane < ck.count) tl[wid][lane] = t;
         else tl[wid][lane].start = 0xFFFFFFFFu;
@@ -1330,13 +1320,8 @@ __global__ void __launch_bounds__(256) k_raster_overdraw(DrawParams p) {
             ntiles = tw * th;
             tview[wid][lane] = v;
         }
-        uint32_t incl = ntiles;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            uint32_t o = __shfl_up(incl, d);
-            if ((int)lane >= d) incl += o;
-        }
-        const uint32_t total = __shfl(incl, 63);
+        const uint32_t incl = wave_scan_add(ntiles);
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
         t.start = incl - ntiles;
         if (lane < ck.count) tl[wid][lane] = t;
         else tl[wid][lane].start = 0xFFFFFFFFu;
@@ -1466,13 +1451,8 @@ __global__ void __launch_bounds__(256) k_gbuffer(DrawParams p) {
             tview[wid][lane] = v;
             talpha[wid][lane] = (rec.flags & kRecAlpha) ? r : 0xFFFFFFFFu;
         }
-        uint32_t incl = ntiles;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            uint32_t o = __shfl_up(incl, d);
-            if ((int)lane >= d) incl += o;
-        }
-        const uint32_t total = __shfl(incl, 63);
+        const uint32_t incl = wave_scan_add(ntiles);
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
         t.start = incl - ntiles;
         if (lane < ck.count) tl[wid][lane] = t;
         else tl[wid][lane].start = 0xFFFFFFFFu;
@@ -1605,13 +1585,8 @@ __device__ void bin_records(const DrawParams& p, bool valid, const TriRecord& re
         org = V.bin_base + by0 * V.bins_x + bx0;
         pitch = V.bins_x;
     }
-    uint32_t incl = nb;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t o = __shfl_up(incl, d);
-        if ((int)lane >= d) incl += o;
-    }
-    const uint32_t total = __shfl(incl, 63);
+    const uint32_t incl = wave_scan_add(nb);
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
     entries += total;
     __builtin_amdgcn_wave_barrier();   // the previous call's reads of `sm` are done
     // Starts must not decrease with the lane for the search below: an
@@ -2097,12 +2072,7 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
                 t.r2 = (int32_t)(((uint32_t)f.b01 << 2) - (uint32_t)__mul24(f.a01, (int32_t)(4 * (t.tw - 1))));
             }
         }
-        uint32_t incl = ntiles;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            uint32_t o = __shfl_up(incl, d);
-            if ((int)lane >= d) incl += o;
-        }
+        const uint32_t incl = wave_scan_add(ntiles);
         if (lane == 63) s_wave_total[wid] = incl;
         __syncthreads();
         uint32_t prev = 0, total = 0;
@@ -2363,8 +2333,7 @@ __global__ void __launch_bounds__(256) k_bin_order(DrawParams p) {
     }
     // the largest bin, for launch-shape choices: one device atomic per block,
     // and none when the block's largest is not above the value already there
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) offered = max(offered, (uint32_t)__shfl_xor((int)offered, d));
+    offered = wave_max(offered);
     if (lane_id() == 0 && offered) atomicMax(&s_max, offered);
     __syncthreads();
     if (tid == 0 && s_max > p.scratch->max_bin_count) atomicMax(&p.scratch->max_bin_count, s_max);
