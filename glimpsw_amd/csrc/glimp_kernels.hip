@@ -993,12 +993,10 @@ static_assert(sizeof(BinTriStep) == 80, "BinTriStep is five 16-byte LDS reads");
 template <int BS>
 using BinTri = typename std::conditional<(BS <= 32), BinTriMask, BinTriStep>::type;
 
-// (record, tile) items of one round of a bin raster chunk: four per thread,
-// two for blocks of up to 128 threads (whose residency the LDS limits: one
-// more block per CU outweighs the extra rounds; a 256-thread block with two
-// per thread measured 3 us slower on the 1080p frame)
+// LDS words of the bin raster's per-record tile masks (one u64 per thread,
+// bins of up to 32 pixels)
 __host__ __device__ constexpr uint32_t bin_items(int BS, int kThreads) {
-    return BS <= 32 ? (kThreads <= 128 ? 2u : 4u) * (uint32_t)kThreads : 1u;
+    return BS <= 32 ? 2u * (uint32_t)kThreads : 1u;
 }
 
 __device__ __forceinline__ void load_tri(const TriRecord& rec, int hw, int hh, TriLds& t, uint32_t& bmin,
@@ -1986,7 +1984,6 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
     uint32_t covered = 0, passed = 0, tiles = 0;
     const uint32_t g = tid >> 4, px = tid & 3, py = (tid >> 2) & 3;
     constexpr bool kMasked = BS <= 32;
-    constexpr uint32_t kItems = bin_items(BS, kThreads);
     constexpr uint32_t kTilesX = BS / 4;                                   // tiles per bin row
     constexpr uint32_t kTileShift = kTilesX == 2 ? 1u : (kTilesX == 4 ? 2u : 3u);
     for (uint32_t base = 0; base < n; base += kThreads) {
@@ -2125,61 +2122,50 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
         };
 
         if constexpr (kMasked) {
-            // The chunk's candidate tiles as a list of items (record << 6 |
-            // tile bit), written by each record's thread at its start, at
-            // most kItems per round; group g takes items [j, j1) of the round
-            // and reads a record's state only when the record changes.
-            for (uint32_t r0 = 0; r0 < total; r0 += kItems) {
-                if (ntiles && t.start < r0 + kItems && t.start + ntiles > r0) {
-                    unsigned long long m = tmask;
-                    for (uint32_t pos = t.start; m && pos < r0 + kItems; pos++, m &= m - 1ull) {
-                        if (pos >= r0) items[pos - r0] = (tid << 6) | (uint32_t)__builtin_ctzll(m);
-                    }
-                }
-                __syncthreads();
-                const uint32_t nr = min(total - r0, kItems);
-                uint32_t kcur = 0xFFFFFFFFu;
-                // one tile of item `it`: the record's state is read when the
-                // record changes
-                auto draw_item = [&](uint32_t it) {
-                    const uint32_t k = it >> 6, b = it & 63u;
-                    if (k != kcur) {
-                        kcur = k;
-                        T = tl[k];
-                        if constexpr (Alpha) ai = talpha[k];
-                    }
+            // A record's candidate tiles are the set bits of its mask, the
+            // first at position `start` of the chunk's tile list. Group g
+            // takes positions [j, j1): it finds the record holding j (the
+            // last one starting at or before it), skips that record's first
+            // j - start tiles and walks set bits, moving on to the next
+            // record with tiles when a mask runs out. (Laying the tiles out
+            // as an item list first cost a serial loop per record, as long
+            // as the block's largest triangle, and a barrier.)
+            unsigned long long* tmasks = reinterpret_cast<unsigned long long*>(items);
+            tmasks[tid] = tmask;
+            __syncthreads();
+            const uint32_t per = (total + kGroups - 1) / kGroups;
+            uint32_t j = min(g * per, total);
+            const uint32_t j1 = min(j + per, total);
+            if (countFrags && j < j1) tiles += j1 - j;
+            if (j < j1) {
+                uint32_t k = 0;
+#pragma unroll
+                for (uint32_t step = kThreads / 2; step > 0; step >>= 1)
+                    if (tl[k + step].start <= j) k += step;
+                T = tl[k];
+                if constexpr (Alpha) ai = talpha[k];
+                unsigned long long m = tmasks[k];
+                for (uint32_t skip = j - T.start; skip; skip--) m &= m - 1ull;
+                for (;;) {
+                    const uint32_t b = (uint32_t)__builtin_ctzll(m);
+                    m &= m - 1ull;
                     const uint32_t tx = b & (kTilesX - 1), ty = b >> kTileShift;
                     const int32_t mx = (int32_t)(px + 4 * tx), my = (int32_t)(py + 4 * ty);
                     shade((uint32_t)T.w0 + (uint32_t)__mul24(T.a12, mx) + (uint32_t)__mul24(T.b12, my),
                           (uint32_t)T.w1 + (uint32_t)__mul24(T.a20, mx) + (uint32_t)__mul24(T.b20, my),
                           (uint32_t)T.w2 + (uint32_t)__mul24(T.a01, mx) + (uint32_t)__mul24(T.b01, my),
                           T.li0 + (tx << 4) + ty * (4 * BS) + px + (py << 2));
-                };
-                if constexpr (kThreads >= 256) {
-                    // whole groups of four items per 16-lane group, one LDS
-                    // read each (not an item read before every tile);
-                    // measured slower with 128-thread blocks, whose fewer
-                    // groups lose more to the coarser split
-                    const uint32_t nq = (nr + 3) >> 2;
-                    uint32_t j = min(4 * ((g * nq) / kGroups), nr);
-                    const uint32_t j1 = min(4 * (((g + 1) * nq) / kGroups), nr);
-                    if (countFrags && j < j1) tiles += j1 - j;
-                    for (; j < j1; j += 4) {
-                        const uint4 q4 = *reinterpret_cast<const uint4*>(&items[j]);
-                        const uint32_t quad[4] = { q4.x, q4.y, q4.z, q4.w };
-#pragma unroll
-                        for (uint32_t u = 0; u < 4; u++)
-                            if (j + u < j1) draw_item(quad[u]);
+                    if (++j >= j1) break;
+                    if (!m) {          // tiles remain in the range: a later record has them
+                        do {
+                            m = tmasks[++k];
+                        } while (!m);
+                        T = tl[k];
+                        if constexpr (Alpha) ai = talpha[k];
                     }
-                } else {
-                    const uint32_t per = (nr + kGroups - 1) / kGroups;
-                    uint32_t j = min(g * per, nr);
-                    const uint32_t j1 = min(j + per, nr);
-                    if (countFrags && j < j1) tiles += j1 - j;
-                    for (; j < j1; j++) draw_item(items[j]);
                 }
-                __syncthreads();
             }
+            __syncthreads();
         } else {
             __syncthreads();
             // group g walks tiles [j, j1) of the block's tile list
