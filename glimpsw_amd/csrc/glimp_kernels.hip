@@ -969,8 +969,8 @@ struct TriLds {
 // the bin, the edge steps, the depth plane, the key's low word, the LDS key
 // index of the box's first tile and the record's first tile in the block's
 // tile list. In bins of up to 32x32 pixels (at most 8x8 tiles) only the
-// box's tiles that the triangle can cover are walked, from a list of
-// (record, tile) items; 64-pixel bins walk every tile of the box, with the
+// box's tiles that the triangle can cover are walked, the set bits of a
+// per-record tile mask; 64-pixel bins walk every tile of the box, with the
 // step from the last tile of a tile row to the first of the next
 // (4B - 4A (tw - 1)).
 struct BinTriMask {
