@@ -498,7 +498,7 @@ static_assert(sizeof(ClipRef) == sizeof(TriRecord), "a clip reference takes one 
 // `Alpha`: the launch has alpha-tested materials (p.alpha is set).
 template <bool Alpha>
 __device__ __forceinline__ void setup_meshlet(const DrawParams& p, const ApproxTables& AT, uint32_t i, uint32_t lane,
-                                              float (&lds)[4][64], const MeshletFetch& f) {
+                                              float4 (&lds)[64], const MeshletFetch& f) {
     const uint32_t g = f.g, v = f.v;           // global meshlet number, view
     const ViewParams& V = view_of(p, v);
     const uint32_t w = g - V.meshlet_base;     // meshlet index within the view
@@ -518,10 +518,11 @@ __device__ __forceinline__ void setup_meshlet(const DrawParams& p, const ApproxT
             alpha = Alpha && (dm.fs_id == 1 || p.deferred) && dm.texture >= 0;
             alpha_tex = (uint32_t)dm.texture | ((uint32_t)dm.alpha_cutoff << 24);
         }
-        // ShadeMeshlet (Shading.cpp:295-298): lane = vertex
+        // ShadeMeshlet (Shading.cpp:295-298): lane = vertex, its clip-space
+        // position one 16-byte LDS entry (a triangle gathers three)
         __builtin_amdgcn_wave_barrier();   // the previous meshlet's reads are done
-#pragma unroll
-        for (int c = 0; c < 4; c++) lds[c][lane] = xform(V.clip, c, f.px, f.py, f.pz);
+        lds[lane] = make_float4(xform(V.clip, 0, f.px, f.py, f.pz), xform(V.clip, 1, f.px, f.py, f.pz),
+                                xform(V.clip, 2, f.px, f.py, f.pz), xform(V.clip, 3, f.px, f.py, f.pz));
         __builtin_amdgcn_wave_barrier();
     }
     const uint32_t seq = V.seq_base + w;
@@ -543,8 +544,10 @@ __device__ __forceinline__ void setup_meshlet(const DrawParams& p, const ApproxT
             for (int k = 0; k < 3; k++) {
                 uint32_t ix = (f.idx[round] >> (8 * k)) & 255u;
                 uint32_t vi = ix < 32 ? ix : 32 + (ix & 31);   // gather_preload64 (SIMD.h:219-227)
-#pragma unroll
-                for (int c = 0; c < 4; c++) vv[k][c] = lds[c][vi];
+                {
+                    const float4 q = lds[vi];
+                    vv[k][0] = q.x; vv[k][1] = q.y; vv[k][2] = q.z; vv[k][3] = q.w;
+                }
                 float x = vv[k][0], y = vv[k][1], z = vv[k][2], ww = vv[k][3];
                 uint32_t oc = 0;
                 oc |= (x < -ww) ? 1u : 0u;
@@ -649,7 +652,7 @@ __device__ __forceinline__ void setup_meshlet(const DrawParams& p, const ApproxT
 
 template <bool Alpha>
 __global__ void __launch_bounds__(256) k_mesh_setup(DrawParams p) {
-    __shared__ float lds_pos[4][4][64];
+    __shared__ float4 lds_pos[4][64];
     // the VRCP14 / VRSQRT14 runs in LDS: triangle setup's reciprocals wait
     // on an LDS read instead of a cached global one
     __shared__ uint2 s_pwl[128];
