@@ -89,11 +89,14 @@ __device__ __forceinline__ float rcp14_special(const ApproxTables& T, float xf) 
 // 7), then x's sign. Everything else takes the general path.
 __device__ __forceinline__ float rcp14(const ApproxTables& T, float xf) {
     const uint32_t x = fbits(xf);
-    const uint32_t ex = x & 0x7F800000u;
-    if (__builtin_expect(ex - 0x00800000u >= (252u << 23), 0)) return rcp14_special(T, xf);
+    // d = x's exponent bits - (253 << 23), mod 2^32: exponents 1-252 (the
+    // fast path) give d in [0x82000000, 0xFF800000], and the result's bits
+    // are (mantissa << 7) - d
+    const uint32_t d = (x & 0x7F800000u) + 0x81800000u;
+    if (__builtin_expect(d <= 0x81FFFFFFu, 0)) return rcp14_special(T, xf);
     uint32_t mant = rcp14_mant(T, (x >> 7) & 0xFFFFu);
     mant = (x & 0x7FFFFFu) ? mant : 0x10000u;
-    const uint32_t r = (0x7E800000u - ex) + (mant << 7);
+    const uint32_t r = (mant << 7) - d;
     return bitsf((r & 0x7FFFFFFFu) | (x & 0x80000000u));
 }
 
