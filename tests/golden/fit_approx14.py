@@ -11,7 +11,7 @@ approx14_pwl.txt: 64 rcp lines then 64 rsqrt lines (parity 0 runs, then
 parity 1), "A B" each. The device evaluates the approximations from these
 128 pairs instead of the 256 KiB tables.
 
-    python -m glimpsw_amd.data.fit_approx14
+    python tests/golden/fit_approx14.py
 """
 from __future__ import annotations
 
@@ -20,12 +20,13 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+DATA = os.path.join(HERE, "..", "..", "glimpsw_amd", "data")
 RUN = 1024
 K_RCP, K_RSQRT = 12, 16
 
 
 def load_tables():
-    t = np.fromfile(os.path.join(HERE, "approx14.bin"), dtype=np.uint16).astype(np.int64)
+    t = np.fromfile(os.path.join(DATA, "approx14.bin"), dtype=np.uint16).astype(np.int64)
     assert t.size == 131072
     return t[:65536], t[65536:].reshape(2, 32768)
 
@@ -64,7 +65,7 @@ def main():
     for A, B in pr + ps:
         assert 0 <= A < 2 ** 32 and 0 <= B < 2 ** 31
         lines.append(f"{A} {B}")
-    with open(os.path.join(HERE, "approx14_pwl.txt"), "w") as f:
+    with open(os.path.join(DATA, "approx14_pwl.txt"), "w") as f:
         f.write("\n".join(lines) + "\n")
     print(f"wrote {len(lines)} pairs")
 

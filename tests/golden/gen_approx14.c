@@ -22,8 +22,9 @@
  * File layout (little endian u16): RCP[65536] then RSQRT[2][32768].
  *
  * Build and run on an AVX-512 host:
- *   gcc -O2 -mavx512f gen_approx14.c -o gen_approx14 && ./gen_approx14 approx14.bin
- *   ./gen_approx14 --verify approx14.bin     (exhaustive check, ~20 s)
+ *   gcc -O2 -mavx512f tests/golden/gen_approx14.c -o gen_approx14
+ *   ./gen_approx14 glimpsw_amd/data/approx14.bin
+ *   ./gen_approx14 --verify glimpsw_amd/data/approx14.bin   (exhaustive, ~20 s)
  */
 #include <immintrin.h>
 #include <stdint.h>

@@ -5,7 +5,7 @@ exponents, the exponent edges of the fast path (1, 252, 253, 254 and the
 denormal and overflow ranges), zeros, infinities, NaNs and random bit
 patterns; the division on random and edge operand pairs. The oracle's
 tables are the host CPU's own instructions over all 2^32 inputs
-(glimpsw_amd/data/gen_approx14.c), so this pins the device arithmetic
+(tests/golden/gen_approx14.c), so this pins the device arithmetic
 directly, not only through whole frames."""
 import ctypes as C
 import os

@@ -174,7 +174,7 @@ def test_exhaustive_approx_tables():
     if "GenuineIntel" not in cpu:
         pytest.skip("tables describe Intel's VRCP14PS/VRSQRT14PS")
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    src = os.path.join(root, "glimpsw_amd", "data", "gen_approx14.c")
+    src = os.path.join(root, "tests", "golden", "gen_approx14.c")
     exe = os.path.join(root, "oracle", "_build", "gen_approx14")
     os.makedirs(os.path.dirname(exe), exist_ok=True)
     subprocess.run(["gcc", "-O2", "-mavx512f", src, "-o", exe], check=True)
