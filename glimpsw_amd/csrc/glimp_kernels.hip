@@ -996,10 +996,10 @@ static_assert(sizeof(BinTriStep) == 80, "BinTriStep is five 16-byte LDS reads");
 template <int BS>
 using BinTri = typename std::conditional<(BS <= 32), BinTriMask, BinTriStep>::type;
 
-// LDS words of the bin raster's per-record tile masks (one u64 per thread,
-// bins of up to 32 pixels)
-__host__ __device__ constexpr uint32_t bin_items(int BS, int kThreads) {
-    return BS <= 32 ? 2u * (uint32_t)kThreads : 1u;
+// Entries of the bin raster's per-record tile masks in LDS (one per thread
+// for bins of up to 32 pixels; the 64-pixel walk has none)
+__host__ __device__ constexpr uint32_t bin_masks(int BS, int kThreads) {
+    return BS <= 32 ? (uint32_t)kThreads : 1u;
 }
 
 __device__ __forceinline__ void load_tri(const TriRecord& rec, int hw, int hh, TriLds& t, uint32_t& bmin,
@@ -1918,11 +1918,14 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
 
 // One block per bin of any view. The bin's keys live in LDS in the same 4x4
 // tile order as the vis-buffer. The block loads up to kThreads records at a
-// time, clips their render boxes to the bin and lays the clipped boxes out
-// as one list of 4x4 tiles (block-wide exclusive scan). The list is cut into
-// equal ranges, one per 16-lane group; a group walks its range tile by tile
-// (lane = pixel), stepping to the next triangle as the range crosses it, and
-// keeps the nearest fragment with an LDS 64-bit max. The finished bin is
+// time and clips their render boxes to the bin; for bins of up to 32 pixels
+// each record's thread marks the 4x4 tiles of its box the triangle can reach
+// in a 64-bit mask, and a block-wide exclusive scan of the masks' popcounts
+// numbers the chunk's candidate tiles. The numbering is cut into equal
+// ranges, one per 16-lane group; a group walks its range tile by tile (lane =
+// pixel), moving to the next record as the range crosses it, and keeps the
+// nearest fragment with an LDS 64-bit max (64-pixel bins walk every tile of
+// each box instead). The finished bin is
 // written to HBM once, one contiguous 4-pixel-high strip of tiles at a
 // time. After a clear the bin starts from the clear key, otherwise from the
 // vis-buffer's current contents.
@@ -1933,7 +1936,7 @@ __global__ void __launch_bounds__(256) k_bin_scatter(DrawParams p) {
 // the records they share were already in the MALL.)
 template <int BS, int kThreads, bool Alpha>
 __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long (&keys)[BS * BS],
-                           BinTri<BS> (&tl)[kThreads], uint32_t (&items)[bin_items(BS, kThreads)],
+                           BinTri<BS> (&tl)[kThreads], unsigned long long (&tmasks)[bin_masks(BS, kThreads)],
                            uint32_t (&s_wave_total)[kThreads / kWave],
                            uint32_t (&talpha)[Alpha ? kThreads : 1]) {
     constexpr int kWaves = kThreads / kWave;
@@ -2133,7 +2136,6 @@ __device__ void raster_bin(const DrawParams& p, uint32_t bin, unsigned long long
             // record with tiles when a mask runs out. (Laying the tiles out
             // as an item list first cost a serial loop per record, as long
             // as the block's largest triangle, and a barrier.)
-            unsigned long long* tmasks = reinterpret_cast<unsigned long long*>(items);
             tmasks[tid] = tmask;
             __syncthreads();
             const uint32_t per = (total + kGroups - 1) / kGroups;
@@ -2340,7 +2342,7 @@ template <int BS, int kThreads, bool Alpha>
 __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
     __shared__ unsigned long long keys[BS * BS];
     __shared__ BinTri<BS> tl[kThreads];
-    __shared__ __attribute__((aligned(16))) uint32_t items[bin_items(BS, kThreads)];
+    __shared__ unsigned long long tmasks[bin_masks(BS, kThreads)];
     __shared__ uint32_t s_wave_total[kThreads / kWave];
     __shared__ uint32_t talpha[Alpha ? kThreads : 1];
     __shared__ uint32_t s_bin;
@@ -2358,7 +2360,7 @@ __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
         s_bin = bin;
     }
     __syncthreads();
-    raster_bin<BS, kThreads, Alpha>(p, s_bin, keys, tl, items, s_wave_total, talpha);
+    raster_bin<BS, kThreads, Alpha>(p, s_bin, keys, tl, tmasks, s_wave_total, talpha);
 }
 
 // Clears every scratch counter: at the end of a direct draw (the binned
