@@ -2347,17 +2347,25 @@ __global__ void __launch_bounds__(kThreads) k_bin_raster(DrawParams p) {
     __shared__ uint32_t talpha[Alpha ? kThreads : 1];
     __shared__ uint32_t s_bin;
     if (threadIdx.x == 0) {
-        uint32_t item = blockIdx.x;
-        uint32_t bin = item;        // (every position is in a class)
+        // every class size in one load, then the class holding position
+        // blockIdx.x without branches (a search that stops at the first
+        // class large enough loads the sizes one after another)
+        uint32_t n[kBinBuckets];
+#pragma unroll
+        for (uint32_t c = 0; c < kBinBuckets; c++) n[c] = p.scratch->bucket[c];
+        const uint32_t item = blockIdx.x;
+        uint32_t cls = 0, start = 0, acc = 0;
+        bool found = false;
+#pragma unroll
         for (int c = (int)kBinBuckets - 1; c >= 0; c--) {
-            const uint32_t n = p.scratch->bucket[c];
-            if (item < n) {
-                bin = p.bin_order[(size_t)c * p.total_bins + item];
-                break;
-            }
-            item -= n;
+            const bool here = !found && item - acc < n[c];
+            cls = here ? (uint32_t)c : cls;
+            start = here ? acc : start;
+            found = found || here;
+            acc += n[c];
         }
-        s_bin = bin;
+        // (every position is in a class)
+        s_bin = found ? p.bin_order[(size_t)cls * p.total_bins + (item - start)] : item;
     }
     __syncthreads();
     raster_bin<BS, kThreads, Alpha>(p, s_bin, keys, tl, tmasks, s_wave_total, talpha);
